@@ -1,0 +1,367 @@
+// api.hip -- the C ABI (include/rt.h): contexts, device memory helpers, BVH
+// objects and ray queries.  Host code only; kernels live in trace.hip and
+// render.hip.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+
+#include "rt_internal.h"
+#include "rt_trace.h"
+
+struct rt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::recursive_mutex lock;
+};
+
+struct rt_bvh {
+    rt_ctx *ctx = nullptr;
+    rt::DevBvh dev;
+    rt::HostBvh host;  // kept for export and info; tris carry transparency
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+};
+
+namespace rt {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string &msg) { g_error = msg; }
+
+}  // namespace rt
+
+using rt::set_error;
+
+#define RT_HIP(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            set_error(std::string(#call) + " failed: " + hipGetErrorString(e_));        \
+            return e_ == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;                 \
+        }                                                                               \
+    } while (0)
+
+extern "C" {
+
+const char *rt_last_error(void) { return rt::g_error.c_str(); }
+
+uint32_t rt_version(void) { return (1u << 16) | 0u; }
+
+rt_status rt_ctx_create(int device, rt_ctx **out)
+{
+    if (!out) { set_error("rt_ctx_create: out is null"); return RT_ERR_INVALID; }
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        set_error("rt_ctx_create: no HIP device visible");
+        return RT_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= count) {
+        set_error("rt_ctx_create: device index out of range");
+        return RT_ERR_INVALID;
+    }
+    RT_HIP(hipSetDevice(device));
+    rt_ctx *c = new (std::nothrow) rt_ctx();
+    if (!c) { set_error("rt_ctx_create: host allocation failed"); return RT_ERR_OOM; }
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        set_error(std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
+        return RT_ERR_HIP;
+    }
+    *out = c;
+    return RT_OK;
+}
+
+rt_status rt_ctx_destroy(rt_ctx *ctx)
+{
+    if (!ctx) return RT_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return RT_OK;
+}
+
+rt_status rt_ctx_synchronize(rt_ctx *ctx)
+{
+    if (!ctx) { set_error("rt_ctx_synchronize: null context"); return RT_ERR_INVALID; }
+    RT_HIP(hipSetDevice(ctx->device));
+    RT_HIP(hipStreamSynchronize(ctx->stream));
+    return RT_OK;
+}
+
+void *rt_ctx_stream(rt_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+rt_status rt_malloc(rt_ctx *ctx, size_t bytes, void **out)
+{
+    if (!ctx || !out) { set_error("rt_malloc: null argument"); return RT_ERR_INVALID; }
+    RT_HIP(hipSetDevice(ctx->device));
+    RT_HIP(hipMalloc(out, bytes ? bytes : 1));
+    return RT_OK;
+}
+
+rt_status rt_free(rt_ctx *ctx, void *ptr)
+{
+    if (!ctx) { set_error("rt_free: null context"); return RT_ERR_INVALID; }
+    if (!ptr) return RT_OK;
+    RT_HIP(hipSetDevice(ctx->device));
+    RT_HIP(hipFree(ptr));
+    return RT_OK;
+}
+
+rt_status rt_memcpy_h2d(rt_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (!ctx || (bytes && (!dst || !src))) { set_error("rt_memcpy_h2d: null argument"); return RT_ERR_INVALID; }
+    RT_HIP(hipSetDevice(ctx->device));
+    RT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    RT_HIP(hipStreamSynchronize(ctx->stream));
+    return RT_OK;
+}
+
+rt_status rt_memcpy_d2h(rt_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (!ctx || (bytes && (!dst || !src))) { set_error("rt_memcpy_d2h: null argument"); return RT_ERR_INVALID; }
+    RT_HIP(hipSetDevice(ctx->device));
+    RT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    RT_HIP(hipStreamSynchronize(ctx->stream));
+    return RT_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* BVH                                                                  */
+/* ------------------------------------------------------------------ */
+static rt_status upload_bvh(rt_bvh *b)
+{
+    const size_t nb = b->host.nodes.size() * sizeof(rt::Node2);
+    const size_t tb = b->host.tris.size() * sizeof(rt::Tri3);
+    RT_HIP(hipMalloc(&b->dev.nodes, nb));
+    RT_HIP(hipMalloc(&b->dev.tris, tb));
+    RT_HIP(hipMemcpyAsync(b->dev.nodes, b->host.nodes.data(), nb, hipMemcpyHostToDevice, b->ctx->stream));
+    RT_HIP(hipMemcpyAsync(b->dev.tris, b->host.tris.data(), tb, hipMemcpyHostToDevice, b->ctx->stream));
+    RT_HIP(hipStreamSynchronize(b->ctx->stream));
+    b->dev.num_nodes = (uint32_t)b->host.nodes.size();
+    b->dev.num_tris = b->host.num_tris;
+    b->dev.max_depth = b->host.max_depth;
+    return RT_OK;
+}
+
+rt_status rt_bvh_build(rt_ctx *ctx, const float *xyz, uint32_t nv, const uint32_t *idx, uint32_t nt,
+                       const rt_bvh_opts *opts, rt_bvh **out)
+{
+    if (!out) { set_error("rt_bvh_build: out is null"); return RT_ERR_INVALID; }
+    *out = nullptr;
+    rt_bvh_opts o{4, 32, 1.2f};
+    if (opts) o = *opts;
+    rt_bvh *b = new (std::nothrow) rt_bvh();
+    if (!b) { set_error("rt_bvh_build: host allocation failed"); return RT_ERR_OOM; }
+    b->ctx = ctx;
+    std::string err;
+    if (!rt::build_bvh2(xyz, nv, idx, nt, o, b->host, err)) {
+        delete b;
+        set_error(err);
+        return RT_ERR_INVALID;
+    }
+    // The traversal stack lives in LDS: refuse trees too deep for it.
+    if (rt::trace_lds_bytes(b->host.max_depth) > 64 * 1024) {
+        delete b;
+        set_error("rt_bvh_build: tree depth exceeds the LDS traversal stack");
+        return RT_ERR_UNSUPPORTED;
+    }
+    if (!ctx) {  // host-only tree: info and export work, traces are refused
+        *out = b;
+        return RT_OK;
+    }
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    hipError_t he = hipSetDevice(ctx->device);
+    rt_status s = he == hipSuccess ? upload_bvh(b) : RT_ERR_HIP;
+    if (he != hipSuccess) set_error(std::string("hipSetDevice: ") + hipGetErrorString(he));
+    if (s != RT_OK) {
+        if (b->dev.nodes) (void)hipFree(b->dev.nodes);
+        if (b->dev.tris) (void)hipFree(b->dev.tris);
+        delete b;
+        return s;
+    }
+    (void)hipEventCreate(&b->ev0);
+    (void)hipEventCreate(&b->ev1);
+    *out = b;
+    return RT_OK;
+}
+
+rt_status rt_bvh_destroy(rt_bvh *b)
+{
+    if (!b) return RT_OK;
+    if (!b->ctx) {
+        delete b;
+        return RT_OK;
+    }
+    std::lock_guard<std::recursive_mutex> g(b->ctx->lock);
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    if (b->dev.nodes) (void)hipFree(b->dev.nodes);
+    if (b->dev.tris) (void)hipFree(b->dev.tris);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    delete b;
+    return RT_OK;
+}
+
+rt_status rt_bvh_get_info(const rt_bvh *b, rt_bvh_info *info)
+{
+    if (!b || !info) { set_error("rt_bvh_get_info: null argument"); return RT_ERR_INVALID; }
+    info->num_tris = b->host.num_tris;
+    info->num_nodes = (uint32_t)b->host.nodes.size();
+    info->num_leaves = b->host.num_leaves;
+    info->max_depth = b->host.max_depth;
+    info->node_bytes = b->host.nodes.size() * sizeof(rt::Node2);
+    info->tri_bytes = (uint64_t)b->host.num_tris * sizeof(rt::Tri3);
+    info->build_ms = b->host.build_ms;
+    info->sah_cost = b->host.sah_cost;
+    return RT_OK;
+}
+
+rt_status rt_bvh_set_transparent(rt_bvh *b, const uint8_t *flags)
+{
+    if (!b || (!flags && b->host.num_tris)) { set_error("rt_bvh_set_transparent: null argument"); return RT_ERR_INVALID; }
+    if (!b->ctx) {
+        for (auto &t : b->host.tris)
+            if (t.id >= 0) t.flags = flags[t.id] ? 1u : 0u;
+        return RT_OK;
+    }
+    std::lock_guard<std::recursive_mutex> g(b->ctx->lock);
+    RT_HIP(hipSetDevice(b->ctx->device));
+    for (auto &t : b->host.tris)
+        if (t.id >= 0) t.flags = flags[t.id] ? 1u : 0u;
+    const size_t tb = b->host.tris.size() * sizeof(rt::Tri3);
+    RT_HIP(hipMemcpyAsync(b->dev.tris, b->host.tris.data(), tb, hipMemcpyHostToDevice, b->ctx->stream));
+    RT_HIP(hipStreamSynchronize(b->ctx->stream));
+    return RT_OK;
+}
+
+rt_status rt_bvh_export(const rt_bvh *b, float *nodes, float *tris)
+{
+    if (!b || !nodes || !tris) { set_error("rt_bvh_export: null argument"); return RT_ERR_INVALID; }
+    std::memcpy(nodes, b->host.nodes.data(), b->host.nodes.size() * sizeof(rt::Node2));
+    // num_tris records plus the sentinel
+    std::memcpy(tris, b->host.tris.data(), b->host.tris.size() * sizeof(rt::Tri3));
+    return RT_OK;
+}
+
+rt_status rt_bvh_last_kernel_ms(const rt_bvh *b, float *ms)
+{
+    if (!b || !ms) { set_error("rt_bvh_last_kernel_ms: null argument"); return RT_ERR_INVALID; }
+    if (!b->timed) { set_error("rt_bvh_last_kernel_ms: no traced call yet"); return RT_ERR_INVALID; }
+    RT_HIP(hipEventSynchronize(b->ev1));
+    RT_HIP(hipEventElapsedTime(ms, b->ev0, b->ev1));
+    return RT_OK;
+}
+
+static bool rays_valid(const rt_rays *r)
+{
+    return r && r->ox && r->oy && r->oz && r->dx && r->dy && r->dz;
+}
+
+// Shared body of rt_trace_closest / rt_trace_any.
+static rt_status trace_common(rt_bvh *b, const rt_rays *rays, uint32_t n, int32_t *prim, float *t,
+                              uint8_t *occ, bool any, uint32_t flags, void *stream_)
+{
+    if (!b) { set_error("rt_trace: null bvh"); return RT_ERR_INVALID; }
+    if (!b->ctx) { set_error("rt_trace: host-only BVH (built without a context)"); return RT_ERR_INVALID; }
+    if (n == 0) return RT_OK;
+    if (!rays_valid(rays) || (any ? !occ : !prim)) { set_error("rt_trace: null ray or output array"); return RT_ERR_INVALID; }
+    std::lock_guard<std::recursive_mutex> g(b->ctx->lock);
+    RT_HIP(hipSetDevice(b->ctx->device));
+    hipStream_t stream = stream_ ? (hipStream_t)stream_ : b->ctx->stream;
+
+    if (!(flags & RT_FLAG_HOST_PTRS)) {
+        rt::RaysDev rd{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmax};
+        RT_HIP(hipEventRecord(b->ev0, stream));
+        RT_HIP(rt::launch_trace(b->dev, rd, n, prim, t, occ, any, stream));
+        RT_HIP(hipEventRecord(b->ev1, stream));
+        b->timed = true;
+        return RT_OK;
+    }
+
+    // Host pointers: stage through device memory and block (the reference's
+    // BVH::intersectRaysGpu contract, BVH.cu:562-584).
+    const size_t fb = (size_t)n * sizeof(float);
+    const int ncomp = rays->tmax ? 7 : 6;
+    char *buf = nullptr;
+    const size_t out_bytes = any ? (size_t)n : (size_t)n * (sizeof(int32_t) + (t ? sizeof(float) : 0));
+    RT_HIP(hipMalloc(&buf, ncomp * fb + out_bytes));
+    const float *src[7] = {rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmax};
+    float *dst[7];
+    for (int k = 0; k < ncomp; ++k) {
+        dst[k] = (float *)(buf + k * fb);
+        hipError_t e = hipMemcpyAsync(dst[k], src[k], fb, hipMemcpyHostToDevice, stream);
+        if (e != hipSuccess) { (void)hipFree(buf); set_error(std::string("hipMemcpyAsync: ") + hipGetErrorString(e)); return RT_ERR_HIP; }
+    }
+    char *out = buf + ncomp * fb;
+    rt::RaysDev rd{dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], ncomp == 7 ? dst[6] : nullptr};
+    int32_t *dprim = any ? nullptr : (int32_t *)out;
+    float *dt = (!any && t) ? (float *)(out + (size_t)n * sizeof(int32_t)) : nullptr;
+    uint8_t *docc = any ? (uint8_t *)out : nullptr;
+    (void)hipEventRecord(b->ev0, stream);
+    hipError_t e = rt::launch_trace(b->dev, rd, n, dprim, dt, docc, any, stream);
+    (void)hipEventRecord(b->ev1, stream);
+    b->timed = true;
+    if (e == hipSuccess) {
+        if (any) e = hipMemcpyAsync(occ, docc, n, hipMemcpyDeviceToHost, stream);
+        else {
+            e = hipMemcpyAsync(prim, dprim, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess && t) e = hipMemcpyAsync(t, dt, fb, hipMemcpyDeviceToHost, stream);
+        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    (void)hipFree(buf);
+    if (e != hipSuccess) { set_error(std::string("rt_trace (host pointers): ") + hipGetErrorString(e)); return RT_ERR_HIP; }
+    return RT_OK;
+}
+
+rt_status rt_trace_stats(rt_bvh *b, const rt_rays *rays, uint32_t n, int any, uint64_t *node_visits,
+                         uint64_t *tri_tests)
+{
+    if (!b || !b->ctx || !rays_valid(rays) || !node_visits || !tri_tests) {
+        set_error("rt_trace_stats: null argument or host-only BVH");
+        return RT_ERR_INVALID;
+    }
+    std::lock_guard<std::recursive_mutex> g(b->ctx->lock);
+    RT_HIP(hipSetDevice(b->ctx->device));
+    hipStream_t stream = b->ctx->stream;
+    char *buf = nullptr;
+    const size_t outb = any ? (size_t)n : (size_t)n * 8;
+    RT_HIP(hipMalloc(&buf, 16 + outb));
+    hipError_t e = hipMemsetAsync(buf, 0, 16, stream);
+    rt::RaysDev rd{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmax};
+    char *out = buf + 16;
+    if (e == hipSuccess)
+        e = rt::launch_trace(b->dev, rd, n, any ? nullptr : (int32_t *)out, any ? nullptr : (float *)(out + 4 * (size_t)n),
+                             any ? (uint8_t *)out : nullptr, any != 0, stream, (unsigned long long *)buf);
+    unsigned long long c[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(c, buf, 16, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    (void)hipFree(buf);
+    if (e != hipSuccess) { set_error(std::string("rt_trace_stats: ") + hipGetErrorString(e)); return RT_ERR_HIP; }
+    *node_visits = c[0];
+    *tri_tests = c[1];
+    return RT_OK;
+}
+
+rt_status rt_trace_closest(rt_bvh *b, const rt_rays *rays, uint32_t n, int32_t *prim, float *t,
+                           uint32_t flags, void *stream)
+{
+    return trace_common(b, rays, n, prim, t, nullptr, false, flags, stream);
+}
+
+rt_status rt_trace_any(rt_bvh *b, const rt_rays *rays, uint32_t n, uint8_t *occluded, uint32_t flags,
+                       void *stream)
+{
+    return trace_common(b, rays, n, nullptr, nullptr, occluded, true, flags, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,164 @@
+// rt_device.h -- device-side geometry shared by the HIP kernels: the
+// reference's watertight ray/triangle test in exact FP32 operation order, the
+// fused slab test of the two-wide BVH and the two-wide traversal itself.
+//
+// Every translation unit that includes this file is compiled with
+// -ffp-contract=off: the only fused multiply-adds are the explicit fmaf()
+// calls of the slab test, whose rounding the box padding absorbs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cstdint>
+
+namespace rt {
+
+// errorBound(n) = (n * eps/2) / (1 - n * eps/2) in double, rounded to float
+// (cuda_libraries/math.cu:117-120, types.h:23).
+__host__ __device__ constexpr float error_bound(int n)
+{
+    return (float)((n * ((double)FLT_EPSILON * 0.5)) / (1 - n * ((double)FLT_EPSILON * 0.5)));
+}
+
+constexpr float kTfarScale = 1.00000012f;  // 1 + 2^-23: slab rounding slack
+constexpr float kHitScale = 1.0000005f;    // cull against the current hit with slack
+
+// Per-ray constants of the watertight test (shapes/Triangle.cu:452-495):
+// the permutation by the largest |d| component and the shear.  They depend
+// on the ray only, so they are hoisted out of the triangle loop; the values
+// are the same FP32 numbers the reference computes per call.
+struct TriRay {
+    float ox, oy, oz;  // origin, permuted (x <- kx, y <- ky, z <- kz)
+    float Sx, Sy, Sz;
+    float tmax;        // ray.maximumOffset
+    int kx, ky, kz;
+};
+
+__device__ __forceinline__ float sel3(int k, float a, float b, float c)
+{
+    return k == 0 ? a : (k == 1 ? b : c);
+}
+
+__device__ __forceinline__ TriRay make_tri_ray(float ox, float oy, float oz, float dx, float dy,
+                                               float dz, float tmax)
+{
+    TriRay r;
+    float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+    int kz = (ax > ay) ? 0 : 1;
+    if (az > (kz == 0 ? ax : ay)) kz = 2;
+    int kx = kz + 1; if (kx == 3) kx = 0;
+    int ky = kx + 1; if (ky == 3) ky = 0;
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    float pdx = sel3(kx, dx, dy, dz), pdy = sel3(ky, dx, dy, dz), pdz = sel3(kz, dx, dy, dz);
+    r.Sx = -pdx / pdz;
+    r.Sy = -pdy / pdz;
+    r.Sz = 1.f / pdz;
+    r.ox = sel3(kx, ox, oy, oz);
+    r.oy = sel3(ky, ox, oy, oz);
+    r.oz = sel3(kz, ox, oy, oz);
+    r.tmax = tmax;
+    return r;
+}
+
+__device__ __forceinline__ float fmax3(float a, float b, float c) { return fmaxf(a, fmaxf(b, c)); }
+
+// Triangle::intersects (shapes/Triangle.cu:433-643), FP32 with the FP64
+// edge-function fallback.  hitOffset is lowered before the deltaT rejection
+// exactly as in the reference (:569-593).  Returns true on acceptance.
+__device__ __forceinline__ bool tri_intersects(const TriRay &r, const float4 a, const float4 b,
+                                               const float4 c, float &hitOffset)
+{
+    // translate (Triangle.cu:452-454), then permute (:482-484)
+    float p0x = sel3(r.kx, a.x, a.y, a.z) - r.ox;
+    float p0y = sel3(r.ky, a.x, a.y, a.z) - r.oy;
+    float p0z = sel3(r.kz, a.x, a.y, a.z) - r.oz;
+    float p1x = sel3(r.kx, b.x, b.y, b.z) - r.ox;
+    float p1y = sel3(r.ky, b.x, b.y, b.z) - r.oy;
+    float p1z = sel3(r.kz, b.x, b.y, b.z) - r.oz;
+    float p2x = sel3(r.kx, c.x, c.y, c.z) - r.ox;
+    float p2y = sel3(r.ky, c.x, c.y, c.z) - r.oy;
+    float p2z = sel3(r.kz, c.x, c.y, c.z) - r.oz;
+    // shear (:490-495)
+    p0x += r.Sx * p0z; p0y += r.Sy * p0z;
+    p1x += r.Sx * p1z; p1y += r.Sy * p1z;
+    p2x += r.Sx * p2z; p2y += r.Sy * p2z;
+    // edge functions (:500-522)
+    float e0 = p1x * p2y - p1y * p2x;
+    float e1 = p2x * p0y - p2y * p0x;
+    float e2 = p0x * p1y - p0y * p1x;
+    if (e0 == 0.0f || e1 == 0.0f || e2 == 0.0f) {
+        double p2txp1ty = (double)p2x * (double)p1y;
+        double p2typ1tx = (double)p2y * (double)p1x;
+        e0 = (float)(p2typ1tx - p2txp1ty);
+        double p0txp2ty = (double)p0x * (double)p2y;
+        double p0typ2tx = (double)p0y * (double)p2x;
+        e1 = (float)(p0typ2tx - p0txp2ty);
+        double p1txp0ty = (double)p1x * (double)p0y;
+        double p1typ0tx = (double)p1y * (double)p0x;
+        e2 = (float)(p1typ0tx - p1txp0ty);
+    }
+    if ((e0 < 0 || e1 < 0 || e2 < 0) && (e0 > 0 || e1 > 0 || e2 > 0)) return false;
+    float det = e0 + e1 + e2;
+    if (det == 0) return false;
+    p0z *= r.Sz; p1z *= r.Sz; p2z *= r.Sz;
+    float scaled = e0 * p0z + e1 * p1z + e2 * p2z;
+    if (det < 0 && (scaled >= 0 || scaled < r.tmax * det)) return false;
+    else if (det > 0 && (scaled <= 0 || scaled > r.tmax * det)) return false;
+    float invDet = 1 / det;
+    float t = scaled * invDet;
+    if (t > hitOffset) return false;
+    hitOffset = t;
+    // conservative deltaT rejection (:575-593)
+    float maxZt = fmax3(fabsf(p0z), fabsf(p1z), fabsf(p2z));
+    float deltaZ = error_bound(3) * maxZt;
+    float maxXt = fmax3(fabsf(p0x), fabsf(p1x), fabsf(p2x));
+    float maxYt = fmax3(fabsf(p0y), fabsf(p1y), fabsf(p2y));
+    float deltaX = error_bound(5) * (maxXt + maxZt);
+    float deltaY = error_bound(5) * (maxYt + maxZt);
+    float deltaE = 2 * (error_bound(2) * maxXt * maxYt + deltaY * maxXt + deltaX * maxYt);
+    float maxE = fmax3(fabsf(e0), fabsf(e1), fabsf(e2));
+    float deltaT = 3 * (error_bound(3) * maxE * maxZt + deltaE * maxZt + deltaZ * maxE) * fabsf(invDet);
+    return !(hitOffset <= deltaT);
+}
+
+// Slab-test constants.  A zero direction component is replaced by +-1e-30
+// so no 0 * inf NaN can arise; the padded boxes keep the test conservative.
+struct BoxRay {
+    float ix, iy, iz;     // 1 / d
+    float oix, oiy, oiz;  // o / d
+};
+
+__device__ __forceinline__ BoxRay make_box_ray(float ox, float oy, float oz, float dx, float dy, float dz)
+{
+    BoxRay b;
+    float ddx = fabsf(dx) > 1e-30f ? dx : copysignf(1e-30f, dx);
+    float ddy = fabsf(dy) > 1e-30f ? dy : copysignf(1e-30f, dy);
+    float ddz = fabsf(dz) > 1e-30f ? dz : copysignf(1e-30f, dz);
+    b.ix = 1.0f / ddx; b.iy = 1.0f / ddy; b.iz = 1.0f / ddz;
+    b.oix = ox * b.ix; b.oiy = oy * b.iy; b.oiz = oz * b.iz;
+    return b;
+}
+
+// Returns tnear; the box is entered iff tnear <= tfar (written to *hit).
+__device__ __forceinline__ float slab(const BoxRay &r, float lx, float hx, float ly, float hy,
+                                      float lz, float hz, float tlim, bool *hit)
+{
+    float tx0 = fmaf(lx, r.ix, -r.oix), tx1 = fmaf(hx, r.ix, -r.oix);
+    float ty0 = fmaf(ly, r.iy, -r.oiy), ty1 = fmaf(hy, r.iy, -r.oiy);
+    float tz0 = fmaf(lz, r.iz, -r.oiz), tz1 = fmaf(hz, r.iz, -r.oiz);
+    float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+    float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+    tf = fminf(tf * kTfarScale, tlim);
+    *hit = tn <= tf;
+    return tn;
+}
+
+// Leaf reference decoding: child < 0 is ~(first << 3 | count - 1).
+__device__ __forceinline__ void decode_leaf(int32_t c, uint32_t &first, uint32_t &count)
+{
+    uint32_t enc = ~(uint32_t)c;
+    first = enc >> 3;
+    count = (enc & 7u) + 1u;
+}
+
+}  // namespace rt

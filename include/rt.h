@@ -1,0 +1,163 @@
+/*
+ * rt.h -- C ABI of the MI355X-native path-tracing hot path.
+ *
+ * Plain C types only (pointers, sizes, enums, POD structs).  Every entry
+ * point returns an rt_status; on failure rt_last_error() returns a
+ * thread-local message.  No exception crosses this boundary.
+ *
+ * Pointer convention: ray, hit and image buffers passed to rt_trace_* and
+ * rt_render* are DEVICE pointers (HBM) unless RT_FLAG_HOST_PTRS is set, in
+ * which case they are host pointers and the call stages them through
+ * device memory and blocks until the results are on the host (the
+ * reference's BVH::intersectRays contract).
+ *
+ * Streams: a stream argument is a hipStream_t passed as void*; NULL means
+ * the context's own stream.  Calls on one rt_ctx from several host threads
+ * are serialised by the context's lock (the reference reaches
+ * BVH::intersectRays concurrently from per-light threads,
+ * integration/Scene.cu:557-574).
+ */
+#ifndef RT_AMD_H
+#define RT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    RT_OK = 0,
+    RT_ERR_INVALID = 1,     /* bad argument, size or state */
+    RT_ERR_HIP = 2,         /* a HIP runtime call failed */
+    RT_ERR_OOM = 3,         /* device allocation failed */
+    RT_ERR_UNSUPPORTED = 4, /* feature not built for this scene */
+    RT_ERR_NO_DEVICE = 5    /* no HIP device visible */
+} rt_status;
+
+enum { RT_FLAG_HOST_PTRS = 1u };
+
+/* Thread-local message of the last failing call on this thread. */
+const char *rt_last_error(void);
+/* ABI version (major << 16 | minor). */
+uint32_t rt_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Context                                                              */
+/* ------------------------------------------------------------------ */
+typedef struct rt_ctx rt_ctx;
+
+rt_status rt_ctx_create(int device, rt_ctx **out);
+rt_status rt_ctx_destroy(rt_ctx *ctx);
+rt_status rt_ctx_synchronize(rt_ctx *ctx);
+/* The context's stream (hipStream_t as void*). */
+void *rt_ctx_stream(rt_ctx *ctx);
+
+/* Device memory helpers for C callers that do not own an allocator. */
+rt_status rt_malloc(rt_ctx *ctx, size_t bytes, void **out);
+rt_status rt_free(rt_ctx *ctx, void *ptr);
+rt_status rt_memcpy_h2d(rt_ctx *ctx, void *dst, const void *src, size_t bytes);
+rt_status rt_memcpy_d2h(rt_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* ------------------------------------------------------------------ */
+/* BVH over a triangle soup                                             */
+/* ------------------------------------------------------------------ */
+/* Replaces the reference's BVH(meshes, initTriangles, meshData, cpuTris)
+ * constructor (cuda_libraries/BVH.cu:314-341).  The reference builds a
+ * Morton-pairing tree with one triangle per leaf; this build is a binned-SAH
+ * two-wide tree with up to max_leaf triangles per leaf whose nodes hold
+ * both child boxes (64-byte records, one cache line per visit).  Triangle
+ * ids are the index of the triangle in idx (the reference's concatenated
+ * triangle list, BVH.cu:288-291).  xyz and idx are HOST arrays.  With
+ * ctx == NULL the tree is built on the host only (rt_bvh_get_info and
+ * rt_bvh_export work; traces return RT_ERR_INVALID). */
+typedef struct {
+    uint32_t max_leaf;   /* 1..8, default 4 */
+    uint32_t bins;       /* SAH bins per axis, 4..64, default 32 */
+    float    trav_cost;  /* SAH traversal cost relative to a triangle test, default 1.2 */
+} rt_bvh_opts;
+
+typedef struct {
+    uint32_t num_tris;
+    uint32_t num_nodes;   /* internal nodes (64 B each) */
+    uint32_t num_leaves;
+    uint32_t max_depth;   /* internal levels on the longest root-leaf path */
+    uint64_t node_bytes;
+    uint64_t tri_bytes;   /* 48 B per triangle */
+    double   build_ms;
+    double   sah_cost;
+} rt_bvh_info;
+
+typedef struct rt_bvh rt_bvh;
+
+rt_status rt_bvh_build(rt_ctx *ctx, const float *xyz, uint32_t nv, const uint32_t *idx,
+                       uint32_t nt, const rt_bvh_opts *opts, rt_bvh **out);
+rt_status rt_bvh_destroy(rt_bvh *bvh);
+rt_status rt_bvh_get_info(const rt_bvh *bvh, rt_bvh_info *info);
+/* Per-triangle transparency (host array of nt bytes, indexed by triangle
+ * id): transparent triangles are skipped by rt_trace_any, as the reference
+ * skips them for shadow rays (shapes/Triangle.cu:439-440). */
+rt_status rt_bvh_set_transparent(rt_bvh *bvh, const uint8_t *flags);
+/* Copy the device tree to host arrays of num_nodes*16 and num_tris*12
+ * floats (layout documented in DESIGN.md and restated by the oracle). */
+rt_status rt_bvh_export(const rt_bvh *bvh, float *nodes, float *tris);
+
+/* ------------------------------------------------------------------ */
+/* Ray queries                                                          */
+/* ------------------------------------------------------------------ */
+/* Structure-of-arrays rays: origin, direction, tMax (FP32).  tmax may be
+ * NULL (+infinity). */
+typedef struct {
+    const float *ox, *oy, *oz;
+    const float *dx, *dy, *dz;
+    const float *tmax;
+} rt_rays;
+
+/* Closest hit.  Replaces BVH::intersectRays (cuda_libraries/BVH.cu:541-584)
+ * and its per-ray BVH::intersectRay (BVH.cu:355-448) with the
+ * Triangle::intersects arithmetic of shapes/Triangle.cu:433-643 done
+ * bit-for-bit in FP32.  prim[i] = triangle id or -1; t[i] = hit distance
+ * (+inf on a miss); t may be NULL. */
+rt_status rt_trace_closest(rt_bvh *bvh, const rt_rays *rays, uint32_t n, int32_t *prim,
+                           float *t, uint32_t flags, void *stream);
+
+/* Any hit (shadow rays).  Replaces the shadow use of the same call,
+ * Scene::intersects(..., checkShadow=true) (integration/Scene.cu:108-150)
+ * followed by the visibility test of Scene::isVisible (Scene.cu:262-273):
+ * occluded[i] = 1 iff a non-transparent triangle is accepted at
+ * t <= tmax[i] (tmax = distance to the light). */
+rt_status rt_trace_any(rt_bvh *bvh, const rt_rays *rays, uint32_t n, uint8_t *occluded,
+                       uint32_t flags, void *stream);
+
+/* Counting run of the same traversal over device rays: total node visits
+ * (64 B each) and triangle tests (48 B each).  Prices the algorithmic bytes
+ * of a launch; it is a separate, slower kernel variant, never the timed one. */
+rt_status rt_trace_stats(rt_bvh *bvh, const rt_rays *rays, uint32_t n, int any,
+                         uint64_t *node_visits, uint64_t *tri_tests);
+
+/* Time the dominant kernel of the last rt_trace_* call on this BVH:
+ * elapsed milliseconds between HIP events recorded around that kernel
+ * on the stream it was launched on. */
+rt_status rt_bvh_last_kernel_ms(const rt_bvh *bvh, float *ms);
+
+/* ------------------------------------------------------------------ */
+/* Synthetic workloads (host generators, deterministic)                 */
+/* ------------------------------------------------------------------ */
+/* Random triangle soup in the unit cube (centroids uniform, edge lengths
+ * log-normal(ln mu, sigma), random orientation), PCG32-seeded.  Writes
+ * 9*nt floats (v0 v1 v2 per triangle). */
+rt_status rt_gen_soup(uint32_t nt, uint64_t seed, float edge_mu, float edge_sigma, float *tri9);
+/* Rays: the first n_coherent are pinhole primary rays of a w x h camera
+ * (row-major, pixel centres) at eye looking along +z with vertical field
+ * of view vfov_deg; the rest are incoherent (origins uniform in the unit
+ * cube, cosine-hemisphere directions about uniform random normals).
+ * Output SoA arrays of n floats each; tmax = +inf. */
+rt_status rt_gen_rays(uint32_t w, uint32_t h, const float eye[3], float vfov_deg,
+                      uint32_t n_incoherent, uint64_t seed, float *ox, float *oy, float *oz,
+                      float *dx, float *dy, float *dz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

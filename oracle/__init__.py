@@ -1,0 +1,189 @@
+"""ctypes loader of liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+The oracle is the CPU restatement of the reference's hot path (see
+oracle.h).  It is imported by tests/, by __graft_entry__.smoke() and by
+bench.py's cpu_baseline leg, and only as the checker or the timed CPU
+baseline; the shipped library never loads it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+_P = C.c_void_p
+_U32 = C.c_uint32
+_F = C.c_float
+
+
+class OrcRay(C.Structure):
+    _fields_ = [("o", _F * 3), ("d", _F * 3), ("inv", _F * 3), ("tmax", _F)]
+
+
+class OrcTriInfo(C.Structure):
+    _fields_ = [("reached", C.c_int), ("t", _F), ("masked", C.c_int), ("fp64", C.c_int)]
+
+
+class OrcSphere(C.Structure):
+    _fields_ = [("o2w", _F * 16), ("w2o", _F * 16), ("radius", _F), ("center", _F * 3)]
+
+
+_SIGS = {
+    "orc_error_bound": (_F, [C.c_int]),
+    "orc_next_float_up": (_F, [_F]),
+    "orc_next_float_down": (_F, [_F]),
+    "orc_ray_make": (None, [C.POINTER(OrcRay), _P, _P, _F]),
+    "orc_tri_intersects": (C.c_int, [C.POINTER(OrcRay), _P, _P, _P, C.POINTER(_F), C.POINTER(OrcTriInfo)]),
+    "orc_box_intersects": (C.c_int, [_P, _P, C.POINTER(OrcRay), C.POINTER(_F)]),
+    "orc_refbvh_build": (_P, [_P, _P, _U32]),
+    "orc_refbvh_free": (None, [_P]),
+    "orc_refbvh_root": (_U32, [_P]),
+    "orc_refbvh_sorted_ids": (None, [_P, _P]),
+    "orc_refbvh_morton": (_U32, [_P, _U32]),
+    "orc_refbvh_intersect": (C.c_int32, [_P, C.POINTER(OrcRay), _P, C.c_int, C.POINTER(_F), _P, _P]),
+    "orc_refbvh_trace": (None, [_P] + [_P] * 7 + [_U32, _P, _P, C.c_int]),
+    "orc_brute_trace": (None, [_P, _P, _U32, _P, C.c_int] + [_P] * 7 + [_U32, _P, _P, _P, C.c_int]),
+    "orc_bvh2_trace": (None, [_P, _P] + [_P] * 7 + [_U32, _P, _P, _P, _P, C.c_int]),
+    "orc_sphere_init": (None, [C.POINTER(OrcSphere), _P, _F]),
+    "orc_sphere_intersects": (C.c_int, [C.POINTER(OrcSphere), C.POINTER(OrcRay), C.POINTER(_F)]),
+    "orc_sphere_intersect": (C.c_int, [C.POINTER(OrcSphere), C.POINTER(OrcRay), C.POINTER(_F), _P, _P, _P]),
+    "orc_mat4_inverse": (None, [_P, _P]),
+}
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            try:
+                f = getattr(L, name)
+            except AttributeError:
+                continue
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "oracle arrays must be contiguous"
+    return a.ctypes.data
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def make_ray(o, d, tmax=np.inf) -> OrcRay:
+    r = OrcRay()
+    o = f32(o)
+    d = f32(d)
+    lib().orc_ray_make(C.byref(r), _p(o), _p(d), float(tmax))
+    return r
+
+
+def tri_intersects(o, d, p0, p1, p2, tmax=np.inf, hit_offset=np.inf):
+    """One Triangle::intersects call; returns (accepted, new_hitOffset, info)."""
+    r = make_ray(o, d, tmax)
+    h = _F(hit_offset)
+    info = OrcTriInfo()
+    p0, p1, p2 = f32(p0), f32(p1), f32(p2)  # keep the arrays alive across the call
+    ok = lib().orc_tri_intersects(C.byref(r), _p(p0), _p(p1), _p(p2), C.byref(h), C.byref(info))
+    return bool(ok), h.value, info
+
+
+class RefBvh:
+    """The reference's Morton-pairing BVH restated."""
+
+    def __init__(self, xyz, idx):
+        self.xyz = f32(xyz).reshape(-1, 3)
+        self.idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1, 3)
+        self.h = lib().orc_refbvh_build(_p(self.xyz), _p(self.idx), self.idx.shape[0])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_refbvh_free(self.h)
+            self.h = None
+
+    def sorted_ids(self):
+        out = np.empty(self.idx.shape[0], np.uint32)
+        lib().orc_refbvh_sorted_ids(self.h, _p(out))
+        return out
+
+    def morton(self, i):
+        return lib().orc_refbvh_morton(self.h, i)
+
+    def trace(self, rays, tmax=None, nthreads=8):
+        rays = [f32(a) for a in rays]
+        n = rays[0].shape[0]
+        prim = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        lib().orc_refbvh_trace(self.h, *[_p(a) for a in rays], _p(tmax), n, _p(prim), _p(t), nthreads)
+        return prim, t
+
+
+def brute_trace(xyz, idx, rays, tmax=None, transparent=None, check_shadow=False, nthreads=8):
+    """Brute-force closest hit + ambiguity flags (bit0 tie, bit1 deltaT mask)."""
+    xyz = f32(xyz).reshape(-1, 3)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1, 3)
+    rays = [f32(a) for a in rays]
+    n = rays[0].shape[0]
+    prim = np.empty(n, np.int32)
+    t = np.empty(n, np.float32)
+    flags = np.empty(n, np.uint8)
+    tr = None if transparent is None else np.ascontiguousarray(transparent, dtype=np.uint8)
+    lib().orc_brute_trace(_p(xyz), _p(idx), idx.shape[0], _p(tr), int(check_shadow), *[_p(a) for a in rays],
+                          _p(tmax), n, _p(prim), _p(t), _p(flags), nthreads)
+    return prim, t, flags
+
+
+def bvh2_trace(nodes, tris, rays, tmax=None, nthreads=8):
+    """Counting traversal of the library's exported BVH2 layout."""
+    rays = [f32(a) for a in rays]
+    n = rays[0].shape[0]
+    prim = np.empty(n, np.int32)
+    t = np.empty(n, np.float32)
+    nn = C.c_uint64()
+    nt = C.c_uint64()
+    nodes, tris = f32(nodes), f32(tris)
+    lib().orc_bvh2_trace(_p(nodes), _p(tris), *[_p(a) for a in rays], _p(tmax), n, _p(prim), _p(t),
+                         C.byref(nn), C.byref(nt), nthreads)
+    return prim, t, nn.value, nt.value
+
+
+class Sphere:
+    def __init__(self, center, radius):
+        self.s = OrcSphere()
+        c = f32(center)
+        lib().orc_sphere_init(C.byref(self.s), _p(c), float(radius))
+
+    def intersects(self, o, d, tmax=np.inf, hit_offset=np.inf):
+        r = make_ray(o, d, tmax)
+        h = _F(hit_offset)
+        ok = lib().orc_sphere_intersects(C.byref(self.s), C.byref(r), C.byref(h))
+        return bool(ok), h.value
+
+    def intersect(self, o, d, tmax=np.inf, hit_offset=np.inf):
+        r = make_ray(o, d, tmax)
+        h = _F(hit_offset)
+        p = np.zeros(3, np.float32)
+        n = np.zeros(3, np.float32)
+        e = np.zeros(3, np.float32)
+        ok = lib().orc_sphere_intersect(C.byref(self.s), C.byref(r), C.byref(h), _p(p), _p(n), _p(e))
+        return bool(ok), h.value, p, n, e

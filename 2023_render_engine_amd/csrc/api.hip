@@ -5,6 +5,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <new>
 #include <string>
@@ -12,11 +13,31 @@
 #include "rt_internal.h"
 #include "rt_trace.h"
 
+// A context is reference counted: every BVH and scene built on it holds a
+// reference, so rt_ctx_destroy before them only marks it closed and the last
+// holder frees it.
 struct rt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::recursive_mutex lock;
+    std::atomic<int> refs{1};
 };
+
+namespace rt {
+
+void ctx_retain(rt_ctx *c) { c->refs.fetch_add(1); }
+
+void ctx_release(rt_ctx *c)
+{
+    if (c->refs.fetch_sub(1) == 1) {
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+    }
+}
+
+}  // namespace rt
 
 struct rt_bvh {
     rt_ctx *ctx = nullptr;
@@ -81,10 +102,7 @@ rt_status rt_ctx_create(int device, rt_ctx **out)
 rt_status rt_ctx_destroy(rt_ctx *ctx)
 {
     if (!ctx) return RT_OK;
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    rt::ctx_release(ctx);
     return RT_OK;
 }
 
@@ -189,6 +207,7 @@ rt_status rt_bvh_build(rt_ctx *ctx, const float *xyz, uint32_t nv, const uint32_
     }
     (void)hipEventCreate(&b->ev0);
     (void)hipEventCreate(&b->ev1);
+    rt::ctx_retain(ctx);
     *out = b;
     return RT_OK;
 }
@@ -200,14 +219,18 @@ rt_status rt_bvh_destroy(rt_bvh *b)
         delete b;
         return RT_OK;
     }
-    std::lock_guard<std::recursive_mutex> g(b->ctx->lock);
-    (void)hipSetDevice(b->ctx->device);
-    (void)hipStreamSynchronize(b->ctx->stream);
-    if (b->dev.nodes) (void)hipFree(b->dev.nodes);
-    if (b->dev.tris) (void)hipFree(b->dev.tris);
-    if (b->ev0) (void)hipEventDestroy(b->ev0);
-    if (b->ev1) (void)hipEventDestroy(b->ev1);
-    delete b;
+    rt_ctx *ctx = b->ctx;
+    {
+        std::lock_guard<std::recursive_mutex> g(ctx->lock);
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        if (b->dev.nodes) (void)hipFree(b->dev.nodes);
+        if (b->dev.tris) (void)hipFree(b->dev.tris);
+        if (b->ev0) (void)hipEventDestroy(b->ev0);
+        if (b->ev1) (void)hipEventDestroy(b->ev1);
+        delete b;
+    }
+    rt::ctx_release(ctx);
     return RT_OK;
 }
 

@@ -142,6 +142,98 @@ rt_status rt_trace_stats(rt_bvh *bvh, const rt_rays *rays, uint32_t n, int any,
 rt_status rt_bvh_last_kernel_ms(const rt_bvh *bvh, float *ms);
 
 /* ------------------------------------------------------------------ */
+/* Scene and path-traced rendering                                      */
+/* ------------------------------------------------------------------ */
+/* Material types: the reference's material classes on the hot path. */
+typedef enum {
+    RT_MAT_LAMBERTIAN = 0,       /* materials/LambertianMaterial.cu */
+    RT_MAT_DIELECTRIC = 1,       /* materials/DielectricMaterial.cu, GlassMaterial.cu (ior 1.5) */
+    RT_MAT_TROWBRIDGE = 2,       /* materials/TrowbridgeMaterial.cu (GGX, conductor Fresnel) */
+    RT_MAT_TROWBRIDGE_BASIC = 3, /* materials/TrowbridgeBasicMaterial.cu (GGX, F = 1) */
+    RT_MAT_LIGHT = 4             /* materials/LightMaterial.cu (emissive) */
+} rt_material_type;
+
+typedef struct {
+    uint32_t type;      /* rt_material_type */
+    float kd[3];        /* albedo / colour (kd, ks = kt, light colour) */
+    float ior;          /* dielectric index of refraction */
+    float roughness;    /* Trowbridge: alpha = sqrt(roughness) (TrowbridgeBSDF.cu:56-59) */
+    float eta[3], k[3]; /* Trowbridge conductor Fresnel */
+    float intensity;    /* light: emitted radiance = kd * intensity */
+} rt_material;
+
+/* PointLight(translate(pos), color, intensity) (light/PointLight.cu:15-66). */
+typedef struct {
+    float pos[3];
+    float color[3];
+    float intensity;
+} rt_point_light;
+
+/* The .obejota camera (cuda_libraries/FileImporting.cu:404-436): lookAt
+ * from pos to lookat with up, cameraToScreen = translate(cts), vertical
+ * frame size frame_y, resolution width x height. */
+typedef struct {
+    float pos[3], lookat[3], up[3], cts[3];
+    float frame_y;
+    uint32_t width, height;
+} rt_camera;
+
+typedef struct {
+    uint32_t spp;         /* samples per pixel in this call */
+    uint32_t spp_offset;  /* index of the first sample (progressive rendering) */
+    uint32_t max_depth;   /* MAX_RAY_RECURSION_DEPTH (types.h:41) */
+    uint32_t rr_depth;    /* Russian roulette when depth > rr_depth (pathTracing.cu:91), 3 */
+    uint32_t seed;        /* PCG32 seed */
+    uint32_t path_capacity; /* paths in flight per wave, 0 = default (2^21) */
+} rt_render_opts;
+
+typedef struct rt_scene rt_scene;
+
+/* Replaces Scene::build(shapes, lights, triangles, meshes)
+ * (integration/Scene.cu:56-95): shapes are added one by one, then
+ * rt_scene_commit builds the BVH and uploads everything. */
+rt_status rt_scene_create(rt_ctx *ctx, rt_scene **out);
+rt_status rt_scene_destroy(rt_scene *scene);
+rt_status rt_scene_add_material(rt_scene *scene, const rt_material *m, uint32_t *id);
+/* A triangle mesh (TriangleMesh + Triangle list, shapes/Triangle.cu:14-100);
+ * its triangles get consecutive ids after the ones already added; first_id
+ * (may be NULL) receives the first. */
+rt_status rt_scene_add_mesh(rt_scene *scene, const float *xyz, uint32_t nv, const uint32_t *idx,
+                            uint32_t nt, uint32_t material, uint32_t *first_id);
+/* Sphere(center, radius) (shapes/Sphere.cu:233-245).  is_light = 1 also
+ * samples it as a sphere area light (next-event estimation with MIS); its
+ * material must be RT_MAT_LIGHT. */
+rt_status rt_scene_add_sphere(rt_scene *scene, const float center[3], float radius, uint32_t material,
+                              int is_light);
+rt_status rt_scene_add_point_light(rt_scene *scene, const rt_point_light *light);
+rt_status rt_scene_set_camera(rt_scene *scene, const rt_camera *cam);
+rt_status rt_scene_commit(rt_scene *scene);
+/* The scene's triangle BVH (valid after commit; owned by the scene). */
+rt_bvh *rt_scene_bvh(rt_scene *scene);
+
+/* Path-trace spp samples per pixel and ADD them into accum (width*height*3
+ * doubles, per-pixel radiance sums) and nvalid (width*height counts of
+ * finite samples).  Replaces PathTracerIntegrator::render
+ * (integration/PathTracerIntegrator.cu:140-212) with computeSamples
+ * (pathTracing.cu:284-414); the film convention is Film::addSample
+ * (camera/Film.cu:111-118): the reference's pixel sum is
+ * accum * (samples / nvalid) (pathTracing.cu:474-486). */
+rt_status rt_render(rt_scene *scene, const rt_render_opts *opts, double *accum, uint32_t *nvalid,
+                    uint32_t flags, void *stream);
+
+/* Camera ray of pixel (px, py) with film jitter (u0, u1), as the renderer's
+ * generate stage computes it (host copy for checking). */
+rt_status rt_scene_camera_ray(const rt_scene *scene, uint32_t px, uint32_t py, float u0, float u1,
+                              float o[3], float d[3]);
+
+/* Statistics of the last rt_render call. */
+typedef struct {
+    uint64_t paths, extension_rays, shadow_rays;
+    double ms;             /* wall time of the call (device-synchronised) */
+} rt_render_stats;
+rt_status rt_render_get_stats(const rt_scene *scene, rt_render_stats *stats);
+
+/* ------------------------------------------------------------------ */
 /* Synthetic workloads (host generators, deterministic)                 */
 /* ------------------------------------------------------------------ */
 /* Random triangle soup in the unit cube (centroids uniform, edge lengths

@@ -382,3 +382,41 @@ int orc_sphere_intersect(const orc_sphere *s, const orc_ray *r, float *hitOffset
     n[2] = mi[2] * on[0] + mi[6] * on[1] + mi[10] * on[2];
     return 1;
 }
+
+/* Barycentrics of the watertight test, b_i = e_i * (1/det)
+ * (shapes/Triangle.cu:161-290, the same steps as orc_tri_intersects). */
+void orc_tri_barycentrics(const orc_ray *r, const float *wp0, const float *wp1, const float *wp2, float b[3])
+{
+    float l0[3], l1[3], l2[3];
+    for (int k = 0; k < 3; ++k) {
+        l0[k] = wp0[k] - r->o[k];
+        l1[k] = wp1[k] - r->o[k];
+        l2[k] = wp2[k] - r->o[k];
+    }
+    float ad[3] = {fabsf(r->d[0]), fabsf(r->d[1]), fabsf(r->d[2])};
+    int kz = (ad[0] > ad[1]) ? 0 : 1;
+    if (ad[2] > ad[kz]) kz = 2;
+    int kx = kz + 1; if (kx == 3) kx = 0;
+    int ky = kx + 1; if (ky == 3) ky = 0;
+    float dx = r->d[kx], dy = r->d[ky], dz = r->d[kz];
+    float p0x = l0[kx], p0y = l0[ky], p0z = l0[kz];
+    float p1x = l1[kx], p1y = l1[ky], p1z = l1[kz];
+    float p2x = l2[kx], p2y = l2[ky], p2z = l2[kz];
+    float Sx = -dx / dz, Sy = -dy / dz;
+    p0x += Sx * p0z; p0y += Sy * p0z;
+    p1x += Sx * p1z; p1y += Sy * p1z;
+    p2x += Sx * p2z; p2y += Sy * p2z;
+    float e0 = p1x * p2y - p1y * p2x;
+    float e1 = p2x * p0y - p2y * p0x;
+    float e2 = p0x * p1y - p0y * p1x;
+    if (e0 == 0.0f || e1 == 0.0f || e2 == 0.0f) {
+        e0 = (float)((double)p2y * (double)p1x - (double)p2x * (double)p1y);
+        e1 = (float)((double)p0y * (double)p2x - (double)p0x * (double)p2y);
+        e2 = (float)((double)p1y * (double)p0x - (double)p1x * (double)p0y);
+    }
+    float det = e0 + e1 + e2;
+    float invDet = 1 / det;
+    b[0] = e0 * invDet;
+    b[1] = e1 * invDet;
+    b[2] = e2 * invDet;
+}

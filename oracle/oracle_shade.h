@@ -1,0 +1,84 @@
+/*
+ * oracle_shade.h -- oracle (test infrastructure only): scene description and
+ * estimator entry points of shade.c.  The POD layouts match the library's
+ * rt_material / rt_point_light / rt_camera (include/rt.h) field for field so
+ * tests can pass the same arrays to both, but nothing here includes or
+ * links the library.
+ */
+#ifndef RT_ORACLE_SHADE_H
+#define RT_ORACLE_SHADE_H
+
+#include <stdint.h>
+
+#include "oracle.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ORC_MAT_LAMBERTIAN = 0,       /* materials/LambertianMaterial.cu */
+    ORC_MAT_DIELECTRIC = 1,       /* materials/DielectricMaterial.cu (GlassMaterial: ior 1.5) */
+    ORC_MAT_TROWBRIDGE = 2,       /* materials/TrowbridgeMaterial.cu (conductor Fresnel) */
+    ORC_MAT_TROWBRIDGE_BASIC = 3, /* materials/TrowbridgeBasicMaterial.cu (F = 1) */
+    ORC_MAT_LIGHT = 4             /* materials/LightMaterial.cu (emissive, NullBSDF) */
+};
+
+typedef struct {
+    uint32_t type;
+    float kd[3];       /* albedo / colour */
+    float ior;
+    float roughness;   /* alpha = sqrt(roughness) (TrowbridgeBSDF.cu:56-59) */
+    float eta[3], k[3];
+    float intensity;   /* light: emitted radiance = kd * intensity */
+} orc_material;
+
+typedef struct {
+    float pos[3];
+    float color[3];
+    float intensity;
+} orc_point_light;
+
+typedef struct {
+    float pos[3], lookat[3], up[3], cts[3];
+    float frame_y;
+    uint32_t width, height;
+} orc_camera;
+
+typedef struct orc_scene orc_scene;
+
+/* Barycentrics b0..b2 = e_i / det of the watertight test (Triangle.cu:563-566). */
+void orc_tri_barycentrics(const orc_ray *r, const float *p0, const float *p1, const float *p2, float b[3]);
+
+/* Camera matrices exactly as the reference composes them. */
+void orc_camera_setup(const orc_camera *c, float raster_to_camera[16], float camera_to_world[16]);
+void orc_camera_ray(const float *r2c, const float *c2w, uint32_t w, uint32_t h, uint32_t px, uint32_t py,
+                    float u0, float u1, float o[3], float d[3]);
+
+float orc_fresnel_dielectric(float cosO, float nO, float nD);
+uint64_t orc_path_initstate(uint32_t seed, uint32_t sample);
+
+/* spheres: 4 floats (cx, cy, cz, r) each; sph_light: 1 = also sampled as a
+ * sphere area light.  Triangles are xyz[3*nv] indexed by idx[3*nt]. */
+orc_scene *orc_scene_create(const float *xyz, const uint32_t *idx, uint32_t nt, const uint32_t *tri_mat,
+                            const float *spheres, const uint32_t *sph_mat, const uint8_t *sph_light,
+                            uint32_t nsph, const orc_material *mats, uint32_t nmat,
+                            const orc_point_light *lights, uint32_t nlight, const orc_camera *cam);
+void orc_scene_free(orc_scene *s);
+void orc_scene_camera(const orc_scene *s, float r2c[16], float c2w[16]);
+
+int orc_path(const orc_scene *s, uint32_t px, uint32_t py, uint32_t sample, uint32_t seed, uint32_t max_depth,
+             uint32_t rr_depth, uint64_t stream_base, float out[3]);
+
+/* Accumulates into accum (W*H*3 doubles) and nvalid (W*H) over the window
+ * [x0,x1) x [y0,y1). */
+void orc_render(const orc_scene *s, uint32_t spp, uint32_t seed, uint32_t max_depth, uint32_t rr_depth,
+                uint32_t spp_offset, uint64_t stream_base, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1,
+                double *accum, uint32_t *nvalid, int nthreads);
+
+float orc_compute_rmse(const double *sum_a, double n_a, const double *sum_b, double n_b, uint32_t npix);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,1 +1,838 @@
-/* shading oracle: filled in below */
+/*
+ * shade.c -- oracle (test infrastructure only): camera, BSDFs, lights and
+ * the path-tracing estimator of the reference, restated in C with FP32
+ * shading and FP64 pixel sums.  The GPU wavefront renderer implements the
+ * same estimator with the same random-number draw order, so with equal
+ * seeds the two agree path for path except where transcendental functions
+ * (sinf/cosf/sqrtf-based warps) round differently on the two sides.
+ *
+ * Draw order per path (the reference's own sampler is unseeded and shared
+ * between threads, sampling/Sampler.cu:139-145, so its values cannot be
+ * reproduced; this order is this project's definition):
+ *   camera: 2 (film jitter)
+ *   per bounce at a scattering material: 2 per sphere area light (NEE)
+ *   BSDF: Lambertian 2, Trowbridge 2 (0 if effectively smooth),
+ *         dielectric 1, light 0
+ *   Russian roulette: 1, only when it applies.
+ */
+#include "oracle.h"
+#include "oracle_shade.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define PIf 3.14159265358979323846f
+#define INV_PIf (1.0f / PIf)
+
+/* ------------------------------------------------------------------ */
+/* PCG32 (O'Neill 2014, pcg32 XSH-RR with pcg32_srandom seeding)        */
+/* ------------------------------------------------------------------ */
+typedef struct { uint64_t state, inc; } pcg32;
+
+static uint32_t pcg_next(pcg32 *g)
+{
+    uint64_t old = g->state;
+    g->state = old * 6364136223846793005ULL + g->inc;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((32u - rot) & 31u));
+}
+
+static pcg32 pcg_seed(uint64_t initstate, uint64_t initseq)
+{
+    pcg32 g = {0u, (initseq << 1u) | 1u};
+    pcg_next(&g);
+    g.state += initstate;
+    pcg_next(&g);
+    return g;
+}
+
+static float pcg_float(pcg32 *g) { return (float)(pcg_next(g) >> 8) * (1.0f / 16777216.0f); }
+
+uint64_t orc_path_initstate(uint32_t seed, uint32_t sample)
+{
+    return ((uint64_t)seed << 32) ^ (uint64_t)sample ^ 0x853c49e6748fea9bULL;
+}
+
+/* ------------------------------------------------------------------ */
+/* small vector helpers (FP32, no contraction)                          */
+/* ------------------------------------------------------------------ */
+typedef struct { float x, y, z; } v3;
+static inline v3 mk(float x, float y, float z) { v3 r = {x, y, z}; return r; }
+static inline v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 mul(v3 a, float s) { return mk(s * a.x, s * a.y, s * a.z); }
+static inline v3 neg(v3 a) { return mk(-a.x, -a.y, -a.z); }
+static inline float dot(v3 a, v3 b) { return (a.x * b.x) + (a.y * b.y) + (a.z * b.z); }
+static inline v3 cross(v3 a, v3 b)
+{
+    return mk((a.y * b.z) - (a.z * b.y), (a.z * b.x) - (a.x * b.z), (a.x * b.y) - (a.y * b.x));
+}
+static inline float len(v3 a) { return sqrtf(dot(a, a)); }
+/* normalize(): v / length() with operator/ multiplying by 1/len (Vector.cu:454-457, 163-169) */
+static inline v3 nrm(v3 a) { float inv = (float)1 / len(a); return mk(a.x * inv, a.y * inv, a.z * inv); }
+static inline v3 vabs(v3 a) { return mk(fabsf(a.x), fabsf(a.y), fabsf(a.z)); }
+
+/* ------------------------------------------------------------------ */
+/* Camera (camera/ProjectiveCamera.cu, FileImporting.cu:404-436,        */
+/* Transform.cu:475-504,611-640)                                        */
+/* ------------------------------------------------------------------ */
+static void mat_mul(const float *a, const float *b, float *out)
+{
+    float r[16];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            float s = 0;
+            for (int k = 0; k < 4; ++k) s += a[i * 4 + k] * b[k * 4 + j];
+            r[i * 4 + j] = s;
+        }
+    memcpy(out, r, sizeof r);
+}
+
+void orc_camera_setup(const orc_camera *c, float raster_to_camera[16], float camera_to_world[16])
+{
+    /* lookAt (Transform.cu:611-640): the matrix built is cameraToWorld, the
+     * Transform returned holds its inverse (the mInv argument is dropped by
+     * Transform(m, mInv), Transform.cu:34-37); the camera is constructed with
+     * inverse(lookAt) (FileImporting.cu:431), i.e. a double inversion. */
+    v3 pos = mk(c->pos[0], c->pos[1], c->pos[2]);
+    v3 look = mk(c->lookat[0], c->lookat[1], c->lookat[2]);
+    v3 up = mk(c->up[0], c->up[1], c->up[2]);
+    v3 dir = nrm(sub(look, pos));
+    v3 right = nrm(cross(nrm(up), dir));
+    v3 nup = cross(dir, right);
+    float c2w[16] = {right.x, nup.x, dir.x, pos.x,
+                     right.y, nup.y, dir.y, pos.y,
+                     right.z, nup.z, dir.z, pos.z,
+                     0, 0, 0, 1};
+    float la[16];
+    orc_mat4_inverse(c2w, la);
+    orc_mat4_inverse(la, camera_to_world);
+
+    /* screen window: half height = frameY / 2, half width scaled by the
+     * aspect ratio (FileImporting.cu:425-433) */
+    float vfs = c->frame_y;
+    vfs /= 2;
+    float hfs = (vfs / (float)c->height) * (float)c->width;
+    float pminx = -hfs, pminy = -vfs, pmaxx = hfs, pmaxy = vfs;
+    /* screenToRaster = scale(res) * scale(1/w, 1/(pMin.y - pMax.y)) *
+     * translate(-pMin.x, -pMax.y) (ProjectiveCamera.cu:37-53) */
+    float T[16] = {1, 0, 0, -pminx, 0, 1, 0, -pmaxy, 0, 0, 1, 0, 0, 0, 0, 1};
+    float S1[16] = {1 / (pmaxx - pminx), 0, 0, 0, 0, 1 / (pminy - pmaxy), 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    float S2[16] = {(float)c->width, 0, 0, 0, 0, (float)c->height, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    float tmp[16], s2r[16], r2s[16];
+    mat_mul(S1, T, tmp);       /* scale_to_0_1(move_to_origin) */
+    mat_mul(S2, tmp, s2r);     /* scale_to_resolution(...) */
+    orc_mat4_inverse(s2r, r2s);
+    /* rasterToCamera = inverse(cameraToScreen) * rasterToScreen, with
+     * cameraToScreen = translate(cts) */
+    float cts[16] = {1, 0, 0, c->cts[0], 0, 1, 0, c->cts[1], 0, 0, 1, c->cts[2], 0, 0, 0, 1};
+    float icts[16];
+    orc_mat4_inverse(cts, icts);
+    mat_mul(icts, r2s, raster_to_camera);
+}
+
+static v3 xform_point(const float *m, v3 p)
+{
+    float xp = m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3];
+    float yp = m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7];
+    float zp = m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11];
+    float wp = m[12] * p.x + m[13] * p.y + m[14] * p.z + m[15];
+    float inv = (float)1 / wp;
+    return mk(inv * xp, inv * yp, inv * zp);
+}
+
+static v3 xform_vec(const float *m, v3 v)
+{
+    return mk(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[4] * v.x + m[5] * v.y + m[6] * v.z,
+              m[8] * v.x + m[9] * v.y + m[10] * v.z);
+}
+
+/* ProjectiveCamera::generateRay (ProjectiveCamera.cu:71-104) with the
+ * camera sample of Sampler::getCameraSample (Sampler.cu:166-181) and the
+ * ray transform with origin error offset (Transform.cu:216-237). */
+void orc_camera_ray(const float *r2c, const float *c2w, uint32_t w, uint32_t h, uint32_t px, uint32_t py,
+                    float u0, float u1, float o[3], float d[3])
+{
+    float fx = (float)px + u0, fy = (float)py + u1;
+    fx = fminf(fmaxf(fx, 0.0f), (float)w);
+    fy = fminf(fmaxf(fy, 0.0f), (float)h);
+    v3 pc = xform_point(r2c, mk(fx, fy, 0));
+    v3 dir = nrm(pc);
+    /* origin (0,0,0) through cameraToWorld with error bounds */
+    float eb3 = orc_error_bound(3);
+    v3 oerr = mul(mk(fabsf(c2w[3]), fabsf(c2w[7]), fabsf(c2w[11])), eb3);
+    /* abs(m*0) terms are zero; the sums reduce to |m[i][3]| */
+    v3 org = xform_point(c2w, mk(0, 0, 0));
+    v3 wd = xform_vec(c2w, dir);
+    float ls = dot(wd, wd);
+    if (ls > 0) {
+        float dt = dot(vabs(wd), oerr) / ls;
+        org = add(org, mul(wd, dt));
+    }
+    o[0] = org.x; o[1] = org.y; o[2] = org.z;
+    d[0] = wd.x; d[1] = wd.y; d[2] = wd.z;
+}
+
+/* ------------------------------------------------------------------ */
+/* Sampling warps (sampling/Sampler.cu:13-96)                           */
+/* ------------------------------------------------------------------ */
+static void concentric_disk(float ux, float uy, float *dx, float *dy)
+{
+    float ox = 2.0f * ux - 1.0f, oy = 2.0f * uy - 1.0f;
+    if (ox == 0 && oy == 0) { *dx = 0; *dy = 0; return; }
+    float theta, r;
+    if (fabsf(ox) > fabsf(oy)) {
+        r = ox;
+        theta = (PIf / 4) * (oy / ox);
+    } else {
+        r = oy;
+        theta = (PIf / 2) - (PIf / 4) * (ox / oy);
+    }
+    *dx = r * cosf(theta);
+    *dy = r * sinf(theta);
+}
+
+static v3 cosine_hemisphere(float ux, float uy)
+{
+    float x, y;
+    concentric_disk(ux, uy, &x, &y);
+    float z = sqrtf(fmaxf(0.0f, 1 - x * x - y * y));
+    return mk(x, y, z);
+}
+
+static void uniform_disk(float ux, float uy, float *x, float *y)
+{
+    float r = sqrtf(fabsf(ux));
+    float th = 2 * PIf * uy;
+    *x = r * cosf(th);
+    *y = r * sinf(th);
+}
+
+/* ------------------------------------------------------------------ */
+/* Shading frame: orthonormal basis about the shading normal (Duff et    */
+/* al., JCGT 2017).  The reference builds its frame from the normal and  */
+/* dpdv (SurfaceInteraction.cu:54-107); every BSDF here is isotropic, so  */
+/* the frame's rotation about n does not change any distribution.       */
+/* ------------------------------------------------------------------ */
+static void onb(v3 n, v3 *t, v3 *b)
+{
+    float s = copysignf(1.0f, n.z);
+    float a = -1.0f / (s + n.z);
+    float c = n.x * n.y * a;
+    *t = mk(1.0f + s * n.x * n.x * a, s * c, -s * n.x);
+    *b = mk(c, s + n.y * n.y * a, -n.y);
+}
+
+/* ------------------------------------------------------------------ */
+/* BSDFs                                                                */
+/* ------------------------------------------------------------------ */
+static float cos2th(v3 w) { return w.z * w.z; }
+static float sin2th(v3 w) { return fmaxf(0.0f, 1 - cos2th(w)); }
+static float sinth(v3 w) { return sqrtf(sin2th(w)); }
+static float clampf(float x, float a, float b) { return fmaxf(a, fminf(b, x)); }
+static float cosphi(v3 w) { float s = sinth(w); return (s == 0) ? 1 : clampf(w.x / s, -1, 1); }
+static float sinphi(v3 w) { float s = sinth(w); return (s == 0) ? 0 : clampf(w.y / s, -1, 1); }
+static float tan2th(v3 w) { return sin2th(w) / cos2th(w); }
+static float abscos(v3 w) { return fabsf(w.z / len(w)); }
+static float abscos2(v3 a, v3 b) { return fabsf(dot(a, b)) / (len(a) * len(b)); }
+static float cosine2(v3 a, v3 b) { return dot(a, b) / (len(a) * len(b)); }
+
+/* TrowbridgeBSDF::distribution (materials/TrowbridgeBSDF.cu:71-84) */
+static float ggx_D(float ax, float ay, v3 h)
+{
+    float t2 = tan2th(h);
+    if (isinf(t2)) return 0;
+    float c4 = cos2th(h) * cos2th(h);
+    float cp = cosphi(h) / ax, sp = sinphi(h) / ay;
+    float e = t2 * (cp * cp + sp * sp);
+    float q = 1 + e;
+    return 1 / (PIf * ax * ay * c4 * (q * q));
+}
+
+/* lambda (TrowbridgeBSDF.cu:101-113); isInf is an == +inf test (math.cu:83-88) */
+static float ggx_lambda(float ax, float ay, v3 w)
+{
+    float t2 = tan2th(w);
+    if (t2 == INFINITY) return 0;
+    float a = cosphi(w) * ax, b = sinphi(w) * ay;
+    float a2 = a * a + b * b;
+    return (sqrtf(1 + a2 * t2) - 1) / 2;
+}
+
+/* distribution(wo, h): visible normals (TrowbridgeBSDF.cu:95-99) */
+static float ggx_Dv(float ax, float ay, v3 wo, v3 h)
+{
+    float g1 = 1 / (1 + ggx_lambda(ax, ay, wo));
+    return g1 / abscos(wo) * ggx_D(ax, ay, h) * abscos2(wo, h);
+}
+
+/* sampleHalf (TrowbridgeBSDF.cu:120-151), including the reference's
+ * uniformSampleDisk(2u - 1) mapping */
+static v3 ggx_sample_half(float ax, float ay, v3 wo, float u0, float u1)
+{
+    v3 half = nrm(mk(ax * wo.x, ay * wo.y, wo.z));
+    if (half.z < 0) half = neg(half);
+    v3 T1 = (half.z < 0.99999f) ? nrm(cross(mk(0, 0, 1), half)) : mk(1, 0, 0);
+    v3 T2 = cross(half, T1);
+    float sx = 2 * u0 - 1, sy = 2 * u1 - 1;
+    float px, py;
+    uniform_disk(sx, sy, &px, &py);
+    float hh = sqrtf(1 - px * px);
+    float tl = (1 + half.z) / 2;
+    py = (1 - tl) * hh + tl * py; /* lerp(x, a, b) = (1 - x) a + x b (math.cu:11-16) */
+    float pz = sqrtf(fmaxf(0.0f, 1 - (px * px + py * py)));
+    v3 nh = add(add(mul(T1, px), mul(T2, py)), mul(half, pz));
+    return nrm(mk(ax * nh.x, ay * nh.y, fmaxf(1e-6f, nh.z)));
+}
+
+/* fresnelConductor (geometricMath.cu:242-271) on complex eta = n + ik */
+typedef struct { float re, im; } cpx;
+static cpx cx(float r, float i) { cpx c = {r, i}; return c; }
+static cpx cadd(cpx a, cpx b) { return cx(a.re + b.re, a.im + b.im); }
+static cpx csub(cpx a, cpx b) { return cx(a.re - b.re, a.im - b.im); }
+static cpx cmul(cpx a, cpx b) { return cx(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+static cpx cdiv(cpx a, cpx b)
+{
+    float s = 1 / (b.re * b.re + b.im * b.im);
+    return cx(s * (a.re * b.re + a.im * b.im), s * (a.im * b.re - a.re * b.im));
+}
+static float cnorm(cpx a) { return a.re * a.re + a.im * a.im; }
+static cpx c_sqrt(cpx z)
+{
+    float n = sqrtf(sqrtf(cnorm(z)));
+    if (n == 0) return cx(0, 0);
+    float t = atan2f(z.im, z.re) / 2;
+    return cx(n * cosf(t), n * sinf(t));
+}
+
+static float fresnel_conductor(float cosi, float eta, float k)
+{
+    cosi = clampf(cosi, 0, 1);
+    cpx e = cx(eta, k);
+    float s2i = 1 - cosi * cosi;
+    cpx s2t = cdiv(cx(s2i, 0), cmul(e, e));
+    cpx ct = c_sqrt(csub(cx(1, 0), s2t));
+    cpx ec = cmul(e, cx(cosi, 0));
+    cpx rpl = cdiv(csub(ec, ct), cadd(ec, ct));
+    cpx ect = cmul(e, ct);
+    cpx rpp = cdiv(csub(cx(cosi, 0), ect), cadd(cx(cosi, 0), ect));
+    return (cnorm(rpl) + cnorm(rpp)) / 2;
+}
+
+/* fresnelDielectric, literally (geometricMath.cu:202-240): both
+ * polarisations use eta_t in numerator and denominator, the result is
+ * (r_par^2 + r_perp^2) / 2 * |cos|, TIR is tested on sin(theta_i). */
+float orc_fresnel_dielectric(float cosO, float nO, float nD)
+{
+    cosO = clampf(cosO, -1, 1);
+    if (!(cosO > 0)) cosO = fabsf(cosO);
+    float sinO = sqrtf(fmaxf(0.0f, 1 - cosO * cosO));
+    float sinD = nO / nD * sinO;
+    if (sinO >= 1) return 1;
+    float cosD = sqrtf(fmaxf(0.0f, 1 - sinD * sinD));
+    float r = ((nD * cosO) - (nD * cosD)) / ((nD * cosO) + (nD * cosD));
+    return (r * r + r * r) / 2 * fabsf(cosO);
+}
+
+static int is_scattering(const orc_material *m)
+{
+    return m->type == ORC_MAT_LAMBERTIAN || m->type == ORC_MAT_TROWBRIDGE || m->type == ORC_MAT_TROWBRIDGE_BASIC;
+}
+
+static float ggx_alpha(const orc_material *m) { return sqrtf(m->roughness); } /* roughnessToAlpha */
+
+/* lightReflected for the scattering materials (f * |cos wi|), shading space */
+static v3 bsdf_f(const orc_material *m, v3 wo, v3 wi)
+{
+    if (m->type == ORC_MAT_LAMBERTIAN) {
+        /* LambertianBRDF: kd/pi * absCosine(wi) (LambertianBSDF.cu:22-38) */
+        float c = abscos(wi);
+        return mk(m->kd[0] * INV_PIf * c, m->kd[1] * INV_PIf * c, m->kd[2] * INV_PIf * c);
+    }
+    /* TrowbridgeBSDF::lightReflected (TrowbridgeBSDF.cu:156-200) */
+    float a = ggx_alpha(m);
+    v3 o = nrm(wo), i = nrm(wi);
+    if (!(o.z * i.z > 0.0f)) return mk(0, 0, 0);
+    if (a < 1e-3f) return mk(0, 0, 0);
+    float co = abscos(o), ci = abscos(i);
+    if (ci == 0 || co == 0) return mk(0, 0, 0);
+    v3 h = add(i, o);
+    if (dot(h, h) == 0) return mk(0, 0, 0);
+    h = nrm(h);
+    float D = ggx_D(a, a, h);
+    float G = 1 / (1 + ggx_lambda(a, a, o) + ggx_lambda(a, a, i));
+    float F[3] = {1, 1, 1};
+    if (m->type == ORC_MAT_TROWBRIDGE) {
+        float c = abscos2(o, h);
+        for (int k = 0; k < 3; ++k) F[k] = fresnel_conductor(c, m->eta[k], m->k[k]);
+    }
+    float den = 4 * ci * co;
+    v3 r;
+    r.x = D * F[0] * G / den * m->kd[0] * ci;
+    r.y = D * F[1] * G / den * m->kd[1] * ci;
+    r.z = D * F[2] * G / den * m->kd[2] * ci;
+    return r;
+}
+
+static float bsdf_pdf(const orc_material *m, v3 wo, v3 wi)
+{
+    if (m->type == ORC_MAT_LAMBERTIAN) {
+        /* cosineHemispherePdf if same hemisphere (LambertianBSDF.cu:40-50) */
+        if (wo.z * wi.z > 0.0f) return (wi.z / len(wi)) * INV_PIf;
+        return 0;
+    }
+    if (m->type == ORC_MAT_TROWBRIDGE || m->type == ORC_MAT_TROWBRIDGE_BASIC) {
+        /* TrowbridgeBSDF::pdf (TrowbridgeBSDF.cu:202-214) */
+        float a = ggx_alpha(m);
+        if (!(wo.z * wi.z > 0.0f)) return 0;
+        if (a < 1e-3f) return 0;
+        v3 h = add(wo, wi);
+        if (dot(h, h) == 0) return 0;
+        h = nrm(h);
+        if (h.z < 0.f) h = neg(h); /* faceForward(h, (0,0,1)) */
+        return ggx_Dv(a, a, wo, h) / (4 * abscos2(wo, h));
+    }
+    return 0;
+}
+
+/* Material::sampleDirection for every type; returns the weight and sets wi.
+ * Consumes the draws listed at the top of this file. */
+static v3 bsdf_sample(const orc_material *m, v3 wo, pcg32 *g, v3 *wi, float *pdf_out, int *delta)
+{
+    *pdf_out = 0;
+    *delta = 0;
+    if (m->type == ORC_MAT_LAMBERTIAN) {
+        float u0 = pcg_float(g), u1 = pcg_float(g);
+        *wi = cosine_hemisphere(u0, u1);
+        *pdf_out = (wi->z / len(*wi)) * INV_PIf;
+        return mk(m->kd[0], m->kd[1], m->kd[2]); /* weight = kd (LambertianBSDF.cu:52-74) */
+    }
+    if (m->type == ORC_MAT_DIELECTRIC) {
+        *delta = 1;
+        float u = pcg_float(g);
+        if (u < 0.7f) {
+            /* SpecularRefractionBSDF::sampleDirection (SpecularRefractionBSDF.cu:31-63) */
+            float nO = 1, nD = m->ior;
+            if (wo.z / len(wo) < 0) { float t = nO; nO = nD; nD = t; }
+            float eta = nO / nD;
+            v3 n = (dot(mk(0, 0, 1), wo) < 0.f) ? mk(0, 0, -1) : mk(0, 0, 1);
+            float cosO = cosine2(n, wo);
+            float s2o = fmaxf(0.f, 1.f - cosO * cosO);
+            float s2t = eta * eta * s2o;
+            if (s2t >= 1) { *wi = mk(0, 0, 0); return mk(0, 0, 0); }
+            float ct = sqrtf(1 - s2t);
+            *wi = add(mul(neg(wo), eta), mul(n, eta * cosO - ct));
+            float tr = 1 - orc_fresnel_dielectric(cosO, nO, nD);
+            return mk(m->kd[0] * tr, m->kd[1] * tr, m->kd[2] * tr);
+        }
+        /* SpecularReflectionBSDF::sampleDirection (SpecularReflectionBSDF.cu:30-42) */
+        *wi = mk(-wo.x, -wo.y, wo.z);
+        float fr = orc_fresnel_dielectric(wi->z / len(*wi), 1, m->ior);
+        return mk(m->kd[0] * fr, m->kd[1] * fr, m->kd[2] * fr);
+    }
+    if (m->type == ORC_MAT_TROWBRIDGE || m->type == ORC_MAT_TROWBRIDGE_BASIC) {
+        /* TrowbridgeBSDF::sampleDirection (TrowbridgeBSDF.cu:216-246) */
+        float a = ggx_alpha(m);
+        v3 o = nrm(wo);
+        if (a < 1e-3f) {
+            *delta = 1;
+            *wi = mk(-o.x, -o.y, o.z);
+            if (m->type == ORC_MAT_TROWBRIDGE_BASIC) return mk(1, 1, 1);
+            float c = abscos(*wi);
+            return mk(fresnel_conductor(c, m->eta[0], m->k[0]) / c, fresnel_conductor(c, m->eta[1], m->k[1]) / c,
+                      fresnel_conductor(c, m->eta[2], m->k[2]) / c);
+        }
+        float u0 = pcg_float(g), u1 = pcg_float(g);
+        v3 h = ggx_sample_half(a, a, o, u0, u1);
+        *wi = add(neg(o), mul(h, 2 * dot(o, h))); /* reflect (geometricMath.cu:196-199) */
+        float pdf = ggx_Dv(a, a, o, h) / (4 * abscos2(o, h));
+        *pdf_out = pdf;
+        v3 f = bsdf_f(m, o, *wi);
+        return mk(f.x / pdf, f.y / pdf, f.z / pdf);
+    }
+    /* light: NullBSDF (NullBSDF.cu) */
+    *wi = mk(0, 0, 0);
+    return mk(0, 0, 0);
+}
+
+/* ------------------------------------------------------------------ */
+/* Scene                                                                */
+/* ------------------------------------------------------------------ */
+struct orc_scene {
+    uint32_t nt, nsph, nmat, nlight;
+    float *xyz;        /* 9 floats per triangle */
+    uint32_t *tri_mat;
+    orc_sphere *sph;
+    uint32_t *sph_mat;
+    uint8_t *sph_light;
+    uint8_t *tri_transparent;
+    orc_material *mat;
+    orc_point_light *pl;
+    orc_camera cam;
+    float r2c[16], c2w[16];
+    orc_refbvh *bvh;
+    uint32_t *idx_seq;
+};
+
+orc_scene *orc_scene_create(const float *xyz, const uint32_t *idx, uint32_t nt, const uint32_t *tri_mat,
+                            const float *spheres, const uint32_t *sph_mat, const uint8_t *sph_light,
+                            uint32_t nsph, const orc_material *mats, uint32_t nmat,
+                            const orc_point_light *lights, uint32_t nlight, const orc_camera *cam)
+{
+    orc_scene *s = calloc(1, sizeof *s);
+    s->nt = nt; s->nsph = nsph; s->nmat = nmat; s->nlight = nlight;
+    s->xyz = malloc(sizeof(float) * 9 * (nt ? nt : 1));
+    s->idx_seq = malloc(sizeof(uint32_t) * 3 * (nt ? nt : 1));
+    for (uint32_t i = 0; i < nt; ++i)
+        for (int v = 0; v < 3; ++v) {
+            for (int k = 0; k < 3; ++k) s->xyz[9 * i + 3 * v + k] = xyz[3 * idx[3 * i + v] + k];
+            s->idx_seq[3 * i + v] = 3 * i + v;
+        }
+    s->tri_mat = malloc(sizeof(uint32_t) * (nt ? nt : 1));
+    memcpy(s->tri_mat, tri_mat, sizeof(uint32_t) * nt);
+    s->mat = malloc(sizeof(orc_material) * (nmat ? nmat : 1));
+    memcpy(s->mat, mats, sizeof(orc_material) * nmat);
+    s->tri_transparent = malloc(nt ? nt : 1);
+    for (uint32_t i = 0; i < nt; ++i) s->tri_transparent[i] = mats[tri_mat[i]].type == ORC_MAT_DIELECTRIC;
+    s->sph = malloc(sizeof(orc_sphere) * (nsph ? nsph : 1));
+    s->sph_mat = malloc(sizeof(uint32_t) * (nsph ? nsph : 1));
+    s->sph_light = malloc(nsph ? nsph : 1);
+    for (uint32_t j = 0; j < nsph; ++j) {
+        orc_sphere_init(&s->sph[j], spheres + 4 * j, spheres[4 * j + 3]);
+        s->sph_mat[j] = sph_mat[j];
+        s->sph_light[j] = sph_light ? sph_light[j] : 0;
+    }
+    s->pl = malloc(sizeof(orc_point_light) * (nlight ? nlight : 1));
+    memcpy(s->pl, lights, sizeof(orc_point_light) * nlight);
+    s->cam = *cam;
+    orc_camera_setup(cam, s->r2c, s->c2w);
+    s->bvh = orc_refbvh_build(s->xyz, s->idx_seq, nt);
+    return s;
+}
+
+void orc_scene_free(orc_scene *s)
+{
+    if (!s) return;
+    orc_refbvh_free(s->bvh);
+    free(s->xyz); free(s->idx_seq); free(s->tri_mat); free(s->mat); free(s->tri_transparent);
+    free(s->sph); free(s->sph_mat); free(s->sph_light); free(s->pl); free(s);
+}
+
+void orc_scene_camera(const orc_scene *s, float r2c[16], float c2w[16])
+{
+    memcpy(r2c, s->r2c, sizeof s->r2c);
+    memcpy(c2w, s->c2w, sizeof s->c2w);
+}
+
+typedef struct {
+    int kind;       /* 0 miss, 1 triangle, 2 sphere */
+    int32_t id;
+    float t;
+    v3 p, n, perr;  /* world hit point, shading normal, error */
+} hit_rec;
+
+/* Scene::intersectTriangles + intersectShapes (integration/Scene.cu:165-236):
+ * the BVH's triangle, re-tested with Triangle::intersect from hitOffset =
+ * +inf, then every sphere in insertion order with a strict < test. */
+static hit_rec scene_intersect(const orc_scene *s, v3 o, v3 d)
+{
+    hit_rec h;
+    memset(&h, 0, sizeof h);
+    h.t = INFINITY;
+    orc_ray r;
+    float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
+    orc_ray_make(&r, of, df, INFINITY);
+    if (s->nt) {
+        float tt;
+        int32_t id = orc_refbvh_intersect(s->bvh, &r, NULL, 0, &tt, NULL, NULL);
+        if (id >= 0) {
+            const float *t = s->xyz + 9 * id;
+            float hh = INFINITY;
+            orc_tri_info info;
+            if (orc_tri_intersects(&r, t, t + 3, t + 6, &hh, &info)) {
+                h.kind = 1; h.id = id; h.t = hh;
+            }
+        }
+    }
+    for (uint32_t j = 0; j < s->nsph; ++j) {
+        float p[3], n[3], e[3];
+        float hh = h.t;
+        if (orc_sphere_intersect(&s->sph[j], &r, &hh, p, n, e)) {
+            h.kind = 2; h.id = (int32_t)j; h.t = hh;
+            h.p = mk(p[0], p[1], p[2]); h.n = mk(n[0], n[1], n[2]); h.perr = mk(e[0], e[1], e[2]);
+        }
+    }
+    if (h.kind == 1) {
+        /* surface data of Triangle::intersect (Triangle.cu:161-431) */
+        const float *t = s->xyz + 9 * h.id;
+        float bc[3];
+        orc_tri_barycentrics(&r, t, t + 3, t + 6, bc);
+        v3 p0 = mk(t[0], t[1], t[2]), p1 = mk(t[3], t[4], t[5]), p2 = mk(t[6], t[7], t[8]);
+        h.p = add(add(mul(p0, bc[0]), mul(p1, bc[1])), mul(p2, bc[2]));
+        float xs = fabsf(bc[0] * p0.x) + fabsf(bc[1] * p1.x) + fabsf(bc[2] * p2.x);
+        float ys = fabsf(bc[0] * p0.y) + fabsf(bc[1] * p1.y) + fabsf(bc[2] * p2.y);
+        float zs = fabsf(bc[0] * p0.z) + fabsf(bc[1] * p1.z) + fabsf(bc[2] * p2.z);
+        h.perr = mul(mk(xs, ys, zs), orc_error_bound(7));
+        v3 n = nrm(cross(sub(p0, p2), sub(p1, p2)));
+        if (dot(n, d) > 0) n = neg(n);
+        h.n = n;
+    }
+    return h;
+}
+
+/* Scene::intersects with checkShadow (Scene.cu:108-150) + isVisible
+ * (:262-273): occluded iff an opaque triangle or sphere is hit at t <= dist.
+ * skip_sphere excludes an area light's own sphere from its shadow test. */
+static int occluded(const orc_scene *s, v3 o, v3 d, float dist, int skip_sphere)
+{
+    orc_ray r;
+    float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
+    orc_ray_make(&r, of, df, INFINITY);
+    float t = INFINITY;
+    int hit = 0;
+    if (s->nt) {
+        float tt;
+        int32_t id = orc_refbvh_intersect(s->bvh, &r, s->tri_transparent, 1, &tt, NULL, NULL);
+        if (id >= 0) { hit = 1; t = tt; }
+    }
+    for (uint32_t j = 0; j < s->nsph; ++j) {
+        if ((int)j == skip_sphere) continue;
+        if (s->mat[s->sph_mat[j]].type == ORC_MAT_DIELECTRIC) continue; /* Sphere.cu:362-363 */
+        if (orc_sphere_intersects(&s->sph[j], &r, &t)) hit = 1;
+    }
+    return hit && !(t > dist);
+}
+
+/* SurfaceInteraction::spawnRay (SurfaceInteraction.cu:163-185) */
+static v3 spawn_origin(v3 p, v3 perr, v3 n, v3 w)
+{
+    float dd = dot(vabs(n), perr);
+    v3 off = mul(n, dd);
+    if (dot(w, n) < 0) off = neg(off);
+    v3 po = add(p, off);
+    float *pp = &po.x;
+    const float *oo = &off.x;
+    for (int i = 0; i < 3; ++i) {
+        if (oo[i] > 0) pp[i] = orc_next_float_up(pp[i]);
+        else if (oo[i] < 0) pp[i] = orc_next_float_down(pp[i]);
+    }
+    return po;
+}
+
+static inline float lum(v3 c) /* Spectrum::brightness, Spectrum.cu:68-75 */
+{
+    return (float)(0.212671f * (double)c.x + 0.715160f * (double)c.y + 0.072169f * (double)c.z);
+}
+
+/* Solid-angle cone sampling of a sphere light seen from p (the reference
+ * has no area lights; this is the definition used by the renderer). */
+static int sphere_light_sample(const orc_sphere *sp, v3 p, float u0, float u1, v3 *wi, float *dist, float *pdf)
+{
+    v3 c = mk(sp->center[0], sp->center[1], sp->center[2]);
+    v3 pc = sub(c, p);
+    float d2 = dot(pc, pc);
+    float r = sp->radius;
+    if (d2 <= r * r) return 0;
+    float dc = sqrtf(d2);
+    v3 wc = mul(pc, 1.0f / dc);
+    float s2 = r * r / d2;
+    float cmax = sqrtf(fmaxf(0.0f, 1 - s2));
+    float ct = 1 - u0 * (1 - cmax);
+    float st = sqrtf(fmaxf(0.0f, 1 - ct * ct));
+    float ph = 2 * PIf * u1;
+    v3 t, b;
+    onb(wc, &t, &b);
+    *wi = nrm(add(add(mul(t, st * cosf(ph)), mul(b, st * sinf(ph))), mul(wc, ct)));
+    /* distance to the near intersection with the sphere along wi */
+    float bb = dot(pc, *wi);
+    float disc = bb * bb - (d2 - r * r);
+    *dist = bb - sqrtf(fmaxf(0.0f, disc));
+    *pdf = 1.0f / (2 * PIf * (1 - cmax));
+    return 1;
+}
+
+static float sphere_light_pdf(const orc_sphere *sp, v3 p)
+{
+    v3 c = mk(sp->center[0], sp->center[1], sp->center[2]);
+    v3 pc = sub(c, p);
+    float d2 = dot(pc, pc);
+    float r = sp->radius;
+    if (d2 <= r * r) return 0;
+    float cmax = sqrtf(fmaxf(0.0f, 1 - r * r / d2));
+    return 1.0f / (2 * PIf * (1 - cmax));
+}
+
+/* One path of the estimator (integration/pathTracing.cu:33-123,173-205,
+ * Scene.cu:280-335,468-525).  Returns L in out[3] and 1 if finite. */
+int orc_path(const orc_scene *s, uint32_t px, uint32_t py, uint32_t sample, uint32_t seed, uint32_t max_depth,
+             uint32_t rr_depth, uint64_t stream_base, float out[3])
+{
+    const uint32_t W = s->cam.width;
+    pcg32 g = pcg_seed(orc_path_initstate(seed, sample), stream_base + (uint64_t)py * W + px);
+    float u0 = pcg_float(&g), u1 = pcg_float(&g);
+    float of[3], df[3];
+    orc_camera_ray(s->r2c, s->c2w, s->cam.width, s->cam.height, px, py, u0, u1, of, df);
+    v3 o = mk(of[0], of[1], of[2]), d = mk(df[0], df[1], df[2]);
+    v3 beta = mk(1, 1, 1), L = mk(0, 0, 0);
+    float prev_pdf = 0;   /* BSDF pdf of the direction that produced this ray */
+    int prev_delta = 1;   /* camera rays count as delta: no MIS on a light hit */
+    v3 prev_p = o;
+    for (uint32_t depth = 0; depth < max_depth; ++depth) {
+        hit_rec h = scene_intersect(s, o, d);
+        if (h.kind == 0) break;
+        const orc_material *m = &s->mat[h.kind == 1 ? s->tri_mat[h.id] : s->sph_mat[h.id]];
+        v3 n = h.n, t, b;
+        onb(n, &t, &b);
+        v3 wo_w = neg(d);
+        v3 wo = mk(dot(wo_w, t), dot(wo_w, b), dot(wo_w, n));
+        if (is_scattering(m)) {
+            v3 Ld = mk(0, 0, 0);
+            for (uint32_t l = 0; l < s->nlight; ++l) {
+                /* PointLight::sampleLight (PointLight.cu:56-66) */
+                v3 lp = mk(s->pl[l].pos[0], s->pl[l].pos[1], s->pl[l].pos[2]);
+                v3 tol = sub(lp, h.p);
+                v3 Lw = nrm(tol);
+                float d2 = dot(tol, tol);
+                float I = s->pl[l].intensity;
+                v3 Li = mk(s->pl[l].color[0] * I * (1 / d2), s->pl[l].color[1] * I * (1 / d2),
+                           s->pl[l].color[2] * I * (1 / d2));
+                if (!(dot(Lw, n) > 0)) continue; /* isOrientedTo (SurfaceInteraction.cu:283) */
+                v3 wi = mk(dot(Lw, t), dot(Lw, b), dot(Lw, n));
+                v3 f = bsdf_f(m, wo, wi);
+                float pf = bsdf_pdf(m, wo, wi);
+                if ((f.x == 0 && f.y == 0 && f.z == 0) || (Li.x == 0 && Li.y == 0 && Li.z == 0)) continue;
+                v3 so = spawn_origin(h.p, h.perr, n, Lw);
+                if (occluded(s, so, Lw, len(tol), -1)) continue;
+                float w = 1.0f / (1.0f + pf * pf); /* powerHeuristic(1, 1, 1, pf), Scene.cu:331 */
+                Ld = add(Ld, mk(f.x * Li.x * w, f.y * Li.y * w, f.z * Li.z * w));
+            }
+            for (uint32_t j = 0; j < s->nsph; ++j) {
+                if (!s->sph_light[j]) continue;
+                float a0 = pcg_float(&g), a1 = pcg_float(&g);
+                v3 Lw;
+                float dist, pl;
+                if (!sphere_light_sample(&s->sph[j], h.p, a0, a1, &Lw, &dist, &pl)) continue;
+                if (!(dot(Lw, n) > 0)) continue;
+                v3 wi = mk(dot(Lw, t), dot(Lw, b), dot(Lw, n));
+                v3 f = bsdf_f(m, wo, wi);
+                if (f.x == 0 && f.y == 0 && f.z == 0) continue;
+                float pf = bsdf_pdf(m, wo, wi);
+                v3 so = spawn_origin(h.p, h.perr, n, Lw);
+                if (occluded(s, so, Lw, dist * 0.9999f, (int)j)) continue;
+                const orc_material *lm = &s->mat[s->sph_mat[j]];
+                float w = pl * pl / (pl * pl + pf * pf);
+                float k = w / pl;
+                Ld = add(Ld, mk(f.x * lm->kd[0] * lm->intensity * k, f.y * lm->kd[1] * lm->intensity * k,
+                                f.z * lm->kd[2] * lm->intensity * k));
+            }
+            L = add(L, mk(beta.x * Ld.x, beta.y * Ld.y, beta.z * Ld.z));
+        }
+        if (m->type == ORC_MAT_LIGHT) {
+            /* getEmittedRadiance (LightMaterial.cu:20-23); MIS against the
+             * light's own NEE when it is a sampled sphere light */
+            float w = 1;
+            if (h.kind == 2 && s->sph_light[h.id] && !prev_delta) {
+                float pl = sphere_light_pdf(&s->sph[h.id], prev_p);
+                w = prev_pdf * prev_pdf / (prev_pdf * prev_pdf + pl * pl);
+            }
+            float I = m->intensity;
+            L = add(L, mk(beta.x * m->kd[0] * I * w, beta.y * m->kd[1] * I * w, beta.z * m->kd[2] * I * w));
+        }
+        v3 wi;
+        float pdf;
+        int delta;
+        v3 wgt = bsdf_sample(m, wo, &g, &wi, &pdf, &delta);
+        beta = mk(beta.x * wgt.x, beta.y * wgt.y, beta.z * wgt.z);
+        if (wgt.x == 0 && wgt.y == 0 && wgt.z == 0) break;
+        if (is_scattering(m) && depth > rr_depth) {
+            float q = fmaxf(0.05f, 1 - lum(beta));
+            if (pcg_float(&g) < q) break;
+            float sc = 1 - q;
+            beta = mk(beta.x / sc, beta.y / sc, beta.z / sc);
+        }
+        v3 ww = add(add(mul(t, wi.x), mul(b, wi.y)), mul(n, wi.z));
+        o = spawn_origin(h.p, h.perr, n, ww);
+        d = ww;
+        prev_pdf = pdf;
+        prev_delta = delta;
+        prev_p = h.p;
+    }
+    out[0] = L.x; out[1] = L.y; out[2] = L.z;
+    return isfinite(L.x) && isfinite(L.y) && isfinite(L.z);
+}
+
+typedef struct {
+    const orc_scene *s;
+    uint32_t spp, seed, max_depth, rr_depth, spp_offset;
+    uint32_t x0, y0, x1, y1;
+    double *accum;
+    uint32_t *nvalid;
+    uint64_t stream_base;
+} render_ctx;
+
+static void *render_rows(void *p)
+{
+    render_ctx *c = p;
+    const uint32_t W = c->s->cam.width;
+    for (uint32_t y = c->y0; y < c->y1; ++y)
+        for (uint32_t x = c->x0; x < c->x1; ++x) {
+            double acc[3] = {0, 0, 0};
+            uint32_t nv = 0;
+            for (uint32_t k = 0; k < c->spp; ++k) {
+                float L[3];
+                if (orc_path(c->s, x, y, c->spp_offset + k, c->seed, c->max_depth, c->rr_depth, c->stream_base, L)) {
+                    acc[0] += L[0]; acc[1] += L[1]; acc[2] += L[2];
+                    ++nv;
+                }
+            }
+            size_t i = (size_t)y * W + x;
+            c->accum[3 * i] += acc[0];
+            c->accum[3 * i + 1] += acc[1];
+            c->accum[3 * i + 2] += acc[2];
+            c->nvalid[i] += nv;
+        }
+    return NULL;
+}
+
+void orc_render(const orc_scene *s, uint32_t spp, uint32_t seed, uint32_t max_depth, uint32_t rr_depth,
+                uint32_t spp_offset, uint64_t stream_base, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1,
+                double *accum, uint32_t *nvalid, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+    uint32_t rows = y1 > y0 ? y1 - y0 : 0;
+    if ((uint32_t)nthreads > rows) nthreads = rows ? (int)rows : 1;
+    pthread_t *th = malloc(sizeof(pthread_t) * nthreads);
+    render_ctx *cs = malloc(sizeof(render_ctx) * nthreads);
+    /* interleave row blocks so threads get similar work */
+    uint32_t chunk = (rows + nthreads - 1) / nthreads;
+    for (int i = 0; i < nthreads; ++i) {
+        render_ctx c = {s, spp, seed, max_depth, rr_depth, spp_offset, x0, y0 + i * chunk, x1, y0 + (i + 1) * chunk,
+                        accum, nvalid, stream_base};
+        if (c.y0 > y1) c.y0 = y1;
+        if (c.y1 > y1) c.y1 = y1;
+        cs[i] = c;
+        pthread_create(&th[i], NULL, render_rows, &cs[i]);
+    }
+    for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
+    free(th); free(cs);
+}
+
+/* computeRMSE (camera/Film.cu:611-632): per pixel the squared norm of the
+ * difference of mean radiances, as a float (Spectrum.cu:204-207),
+ * accumulated in float, divided by the pixel count, square root. */
+float orc_compute_rmse(const double *sum_a, double n_a, const double *sum_b, double n_b, uint32_t npix)
+{
+    float rmse = 0;
+    for (uint32_t i = 0; i < npix; ++i) {
+        double dr = sum_a[3 * i] / n_a - sum_b[3 * i] / n_b;
+        double dg = sum_a[3 * i + 1] / n_a - sum_b[3 * i + 1] / n_b;
+        double db = sum_a[3 * i + 2] / n_a - sum_b[3 * i + 2] / n_b;
+        rmse += (float)(dr * dr + dg * dg + db * db);
+    }
+    rmse /= (float)npix;
+    return sqrtf(rmse);
+}

@@ -68,7 +68,7 @@ def test_tmax_clips_closest_hit(rt, orc, gpu_ctx):
 
 
 def test_any_hit_with_transparency(rt, orc, gpu_ctx):
-    xyz, idx = rt.gen_soup(8192, seed=9)
+    xyz, idx = rt.gen_soup(8192, seed=9, edge_mu=0.03)  # larger triangles: many occluders
     rays = rt.gen_rays(32, 32, n_incoherent=6000, seed=10)
     rng = np.random.default_rng(2)
     transparent = (rng.uniform(size=idx.shape[0]) < 0.3).astype(np.uint8)

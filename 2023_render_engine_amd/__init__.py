@@ -43,6 +43,37 @@ class Rays(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("ox", "oy", "oz", "dx", "dy", "dz", "tmax")]
 
 
+MAT_LAMBERTIAN, MAT_DIELECTRIC, MAT_TROWBRIDGE, MAT_TROWBRIDGE_BASIC, MAT_LIGHT = range(5)
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("kd", C.c_float * 3), ("ior", C.c_float), ("roughness", C.c_float),
+                ("eta", C.c_float * 3), ("k", C.c_float * 3), ("intensity", C.c_float)]
+
+
+class PointLight(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("color", C.c_float * 3), ("intensity", C.c_float)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("lookat", C.c_float * 3), ("up", C.c_float * 3), ("cts", C.c_float * 3),
+                ("frame_y", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class RenderOpts(C.Structure):
+    _fields_ = [("spp", C.c_uint32), ("spp_offset", C.c_uint32), ("max_depth", C.c_uint32),
+                ("rr_depth", C.c_uint32), ("seed", C.c_uint32), ("path_capacity", C.c_uint32)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("ms", C.c_double)]
+
+
+def f3(v):
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
 _P = C.c_void_p
 _U32 = C.c_uint32
 _SIGS = {
@@ -65,6 +96,18 @@ _SIGS = {
     "rt_trace_closest": (C.c_int, [_P, C.POINTER(Rays), _U32, _P, _P, _U32, _P]),
     "rt_trace_any": (C.c_int, [_P, C.POINTER(Rays), _U32, _P, _U32, _P]),
     "rt_trace_stats": (C.c_int, [_P, C.POINTER(Rays), _U32, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "rt_scene_create": (C.c_int, [_P, C.POINTER(_P)]),
+    "rt_scene_destroy": (C.c_int, [_P]),
+    "rt_scene_add_material": (C.c_int, [_P, C.POINTER(Material), C.POINTER(_U32)]),
+    "rt_scene_add_mesh": (C.c_int, [_P, _P, _U32, _P, _U32, _U32, C.POINTER(_U32)]),
+    "rt_scene_add_sphere": (C.c_int, [_P, _P, C.c_float, _U32, C.c_int]),
+    "rt_scene_add_point_light": (C.c_int, [_P, C.POINTER(PointLight)]),
+    "rt_scene_set_camera": (C.c_int, [_P, C.POINTER(Camera)]),
+    "rt_scene_commit": (C.c_int, [_P]),
+    "rt_scene_bvh": (_P, [_P]),
+    "rt_scene_camera_ray": (C.c_int, [_P, _U32, _U32, C.c_float, C.c_float, _P, _P]),
+    "rt_render": (C.c_int, [_P, C.POINTER(RenderOpts), _P, _P, _U32, _P]),
+    "rt_render_get_stats": (C.c_int, [_P, C.POINTER(RenderStats)]),
     "rt_gen_soup": (C.c_int, [_U32, C.c_uint64, C.c_float, C.c_float, _P]),
     "rt_gen_rays": (C.c_int, [_U32, _U32, _P, C.c_float, _U32, C.c_uint64, _P, _P, _P, _P, _P, _P]),
 }
@@ -251,3 +294,103 @@ def gen_rays(w: int, h: int, eye=(0.5, 0.5, -1.0), vfov=60.0, n_incoherent: int 
     e = np.asarray(eye, np.float32)
     check(lib().rt_gen_rays(w, h, _ptr(e), vfov, n_incoherent, seed, *[_ptr(a) for a in arrs]))
     return arrs
+
+
+class Scene:
+    """rt_scene built from a scenes.SceneDesc; film buffers are torch
+    tensors on the context's device (FP64 sums + per-pixel valid counts)."""
+
+    def __init__(self, ctx: Context, desc):
+        import numpy as np
+        self.ctx = ctx
+        self.desc = desc
+        L = lib()
+        h = _P()
+        check(L.rt_scene_create(ctx.h, C.byref(h)))
+        self.h = h
+        for m in desc.material_structs():
+            mid = C.c_uint32()
+            check(L.rt_scene_add_material(h, C.byref(m), C.byref(mid)))
+        xyz = np.ascontiguousarray(desc.xyz, np.float32)
+        tm = np.asarray(desc.tri_mat, np.uint32)
+        idx = np.ascontiguousarray(desc.idx, np.uint32)
+        # one rt_scene_add_mesh per run of equal material ids (mesh order = id order)
+        nt = idx.shape[0]
+        start = 0
+        while start < nt:
+            m = tm[start]
+            end = start + int(np.argmax(tm[start:] != m)) if np.any(tm[start:] != m) else nt
+            sub = np.ascontiguousarray(idx[start:end])
+            check(L.rt_scene_add_mesh(h, _ptr(xyz), xyz.shape[0], _ptr(sub), sub.shape[0], int(m), None))
+            start = end
+        for c, r, m, is_light in desc.spheres:
+            cc = np.asarray(c, np.float32)
+            check(L.rt_scene_add_sphere(h, _ptr(cc), float(r), int(m), int(is_light)))
+        for p, col, inten in desc.lights:
+            pl = PointLight(f3(p), f3(col), float(inten))
+            check(L.rt_scene_add_point_light(h, C.byref(pl)))
+        cam = desc.camera_struct()
+        check(L.rt_scene_set_camera(h, C.byref(cam)))
+        check(L.rt_scene_commit(h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rt_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def new_film(self):
+        import torch
+        n = self.desc.width * self.desc.height
+        dev = torch.device("cuda", self.ctx.device)
+        return (torch.zeros(n * 3, dtype=torch.float64, device=dev), torch.zeros(n, dtype=torch.int32, device=dev))
+
+    def render(self, film, spp=None, spp_offset=0, max_depth=None, rr_depth=None, seed=None, path_capacity=0,
+               stream=None):
+        import torch
+        d = self.desc
+        o = RenderOpts(spp if spp is not None else d.spp, spp_offset, max_depth or d.max_depth,
+                       d.rr_depth if rr_depth is None else rr_depth, d.seed if seed is None else seed, path_capacity)
+        accum, nvalid = film
+        s = stream if stream is not None else torch.cuda.current_stream(accum.device).cuda_stream
+        check(lib().rt_render(self.h, C.byref(o), _ptr(accum), _ptr(nvalid), 0, s))
+        return film
+
+    def stats(self) -> RenderStats:
+        st = RenderStats()
+        check(lib().rt_render_get_stats(self.h, C.byref(st)))
+        return st
+
+    def camera_ray(self, px, py, u0, u1):
+        import numpy as np
+        o = np.zeros(3, np.float32)
+        d = np.zeros(3, np.float32)
+        check(lib().rt_scene_camera_ray(self.h, px, py, u0, u1, _ptr(o), _ptr(d)))
+        return o, d
+
+
+def film_image(film, spp_total):
+    """Mean radiance per pixel (H*W*3 float64 numpy) from FP64 sums and valid
+    counts: the reference's pixel sum is sum * (spp / nvalid)
+    (integration/pathTracing.cu:474-486), divided by spp for the mean."""
+    import numpy as np
+    accum, nvalid = film
+    a = accum.double().cpu().numpy().reshape(-1, 3)
+    n = nvalid.cpu().numpy().astype(np.float64)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        img = np.where(n[:, None] > 0, a / n[:, None], 0.0)
+    return img
+
+
+def compute_rmse(img_a, img_b):
+    """computeRMSE (camera/Film.cu:611-632): sqrt of the mean over pixels of
+    the per-pixel squared norm of the difference of mean radiances (the RGB
+    channel SUM of squares), accumulated here in float64."""
+    import numpy as np
+    d = np.asarray(img_a, np.float64).reshape(-1, 3) - np.asarray(img_b, np.float64).reshape(-1, 3)
+    return float(np.sqrt(np.mean(np.sum(d * d, axis=1))))

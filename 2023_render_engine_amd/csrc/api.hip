@@ -49,6 +49,15 @@ struct rt_bvh {
 
 namespace rt {
 
+int ctx_device(rt_ctx *c) { return c->device; }
+void *ctx_stream_ptr(rt_ctx *c) { return (void *)c->stream; }
+std::recursive_mutex &ctx_lock(rt_ctx *c) { return c->lock; }
+DevBvh bvh_device(const rt_bvh *b) { return b->dev; }
+
+}  // namespace rt
+
+namespace rt {
+
 static thread_local std::string g_error;
 
 void set_error(const std::string &msg) { g_error = msg; }

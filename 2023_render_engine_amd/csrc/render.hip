@@ -1,0 +1,594 @@
+// render.hip -- wavefront path tracer for gfx950.
+//
+// A wave of up to P paths lives in structure-of-arrays state in HBM.  Per
+// bounce:
+//   extend   closest hit (BVH2 + analytic spheres) for every active path,
+//            then material classification into per-class queues with
+//            wavefront ballots (one atomic per 64 lanes per queue);
+//   shade<M> one kernel per material class (Lambertian, dielectric, GGX,
+//            light, miss), so a wavefront runs a single BSDF: resolves the
+//            previous bounce's next-event estimate, emits this bounce's
+//            shadow rays, samples the BSDF, applies Russian roulette and
+//            appends survivors to the next active queue;
+//   connect  any hit for every shadow ray; occluded rays clear their light
+//            bit in the path's NEE mask;
+// and a final pass adds finished paths to the FP64 film.
+//
+// The estimator is the reference's (integration/pathTracing.cu:33-123,
+// 173-205, 284-384; integration/Scene.cu:165-335, 468-585): emission at
+// every hit, next-event estimation at scattering materials over every point
+// light with weight 1 / (1 + pdf_bsdf^2) (Scene.cu:331, pdf_light = 1),
+// Russian roulette at scattering materials beyond rr_depth with
+// q = max(0.05, 1 - luminance(beta)) (pathTracing.cu:91-102).  Next-event
+// contributions are summed per path in light order, so the result does not
+// depend on the order in which shadow rays complete.
+//
+// Every queue kernel is a grid-stride loop over a count read from device
+// memory, so a wave is a fixed launch sequence with no host round trip.
+#include "rt_device.h"
+#include "rt_render.h"
+#include "rt_shade.h"
+
+namespace rt {
+
+constexpr int kBlock = 128;
+constexpr int kGrid = 2048;  // 8 workgroups per CU on 256 CUs
+
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred)
+{
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0) return 0;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)mask) - 1);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, (int)leader);
+    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+__device__ __forceinline__ Pcg32 load_rng(const PathsDev &p, uint32_t slot, const WaveParams &wp)
+{
+    Pcg32 g;
+    g.state = p.rng[slot];
+    g.inc = ((wp.stream_base + p.pix[slot]) << 1u) | 1u;
+    return g;
+}
+
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_reset(uint32_t *counters, uint32_t cur, uint32_t init_active)
+{
+    const uint32_t t = threadIdx.x;
+    if (t < kNumCounters) {
+        if (t == kCActive0 + cur) {
+            if (init_active != 0xffffffffu) counters[t] = init_active;
+        } else if (t == kCActive0 + (cur ^ 1u) || (t >= kCQueue && t <= kCTerm)) {
+            counters[t] = 0;
+        }
+    }
+}
+
+// generate: camera rays for slots [0, count)
+__global__ __launch_bounds__(kBlock) void k_generate(SceneDev sc, PathsDev p, QueuesDev q, WaveParams wp)
+{
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < wp.count; i += gridDim.x * kBlock) {
+        const uint64_t g = wp.first_path + i;
+        const uint32_t sample = wp.spp_offset + (uint32_t)(g / wp.npix);
+        const uint32_t pix = (uint32_t)(g % wp.npix);
+        const uint32_t px = pix % sc.cam.width, py = pix / sc.cam.width;
+        const uint64_t initstate = ((uint64_t)wp.seed << 32) ^ (uint64_t)sample ^ 0x853c49e6748fea9bULL;
+        Pcg32 r = pcg32_seed(initstate, wp.stream_base + pix);
+        const float u0 = pcg32_float(r), u1 = pcg32_float(r);
+        V3 o, d;
+        camera_ray(sc.cam, px, py, u0, u1, o, d);
+        p.ox[i] = o.x; p.oy[i] = o.y; p.oz[i] = o.z;
+        p.dx[i] = d.x; p.dy[i] = d.y; p.dz[i] = d.z;
+        p.bx[i] = 1; p.by[i] = 1; p.bz[i] = 1;
+        p.lx[i] = 0; p.ly[i] = 0; p.lz[i] = 0;
+        p.rng[i] = r.state;
+        p.pix[i] = pix;
+        p.meta[i] = 1u << 16;  // depth 0, camera ray counts as delta for MIS
+        p.ppdf[i] = 0;
+        p.ppx[i] = o.x; p.ppy[i] = o.y; p.ppz[i] = o.z;
+        p.nee_mask[i] = 0;
+        q.active[0][i] = i;
+    }
+}
+
+// ---------------------------------------------------------------------
+// extend: closest hit + classification
+// ---------------------------------------------------------------------
+__device__ __forceinline__ uint32_t classify(const SceneDev &sc, int32_t hid)
+{
+    if (hid == -1) return kQMiss;
+    const uint32_t mi = hid >= 0 ? sc.tri_mat[hid] : sc.sph[-hid - 2].material;
+    const uint32_t t = sc.mat[mi].type;
+    if (t == RT_MAT_LAMBERTIAN) return kQLambert;
+    if (t == RT_MAT_DIELECTRIC) return kQDielectric;
+    if (t == RT_MAT_LIGHT) return kQLight;
+    return kQGgx;
+}
+
+__global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc, PathsDev p, QueuesDev q, uint32_t cur, int cap)
+{
+    extern __shared__ int32_t s_stack[];
+    int32_t *stack = s_stack + threadIdx.x;
+    const uint32_t n = q.counters[kCActive0 + cur];
+    const float4 *nodes = (const float4 *)sc.bvh.nodes;
+    const float4 *tris = (const float4 *)sc.bvh.tris;
+    // every lane of a wavefront runs the same number of iterations so the
+    // ballots below see the whole wavefront
+    const uint32_t stride = gridDim.x * kBlock;
+    const uint32_t base0 = blockIdx.x * kBlock;
+    for (uint32_t i0 = base0; i0 < n; i0 += stride) {
+        const uint32_t i = i0 + threadIdx.x;
+        const bool live = i < n;
+        uint32_t slot = 0;
+        int32_t hid = -1;
+        float th = INFINITY;
+        if (live) {
+            slot = q.active[cur][i];
+            const float ox = p.ox[slot], oy = p.oy[slot], oz = p.oz[slot];
+            const float dx = p.dx[slot], dy = p.dy[slot], dz = p.dz[slot];
+            const TriRay tr = make_tri_ray(ox, oy, oz, dx, dy, dz, INFINITY);
+            const BoxRay br = make_box_ray(ox, oy, oz, dx, dy, dz);
+            float h = INFINITY;
+            int sp = 0;
+            int32_t curn = 0;
+            for (;;) {
+                if (curn >= 0) {
+                    const float4 *np = nodes + 4 * (size_t)curn;
+                    const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+                    const float tlim = h * kHitScale;
+                    bool h0, h1;
+                    const float t0 = slab(br, a.x, a.y, a.z, a.w, c.x, c.y, tlim, &h0);
+                    const float t1 = slab(br, b.x, b.y, b.z, b.w, c.z, c.w, tlim, &h1);
+                    const int32_t c0 = __float_as_int(d.x), c1 = __float_as_int(d.y);
+                    if (h0 && h1) {
+                        const bool sw = t1 < t0;
+                        if (sp < cap) { stack[sp * kBlock] = sw ? c0 : c1; ++sp; }
+                        curn = sw ? c1 : c0;
+                    } else if (h0) {
+                        curn = c0;
+                    } else if (h1) {
+                        curn = c1;
+                    } else if (sp > 0) {
+                        --sp;
+                        curn = stack[sp * kBlock];
+                    } else {
+                        break;
+                    }
+                } else {
+                    uint32_t first, count;
+                    decode_leaf(curn, first, count);
+                    for (uint32_t k = 0; k < count; ++k) {
+                        const float4 *tp = tris + 3 * (size_t)(first + k);
+                        if (tri_intersects(tr, tp[0], tp[1], tp[2], h)) {
+                            hid = __float_as_int(tp[0].w);
+                            th = h;
+                        }
+                    }
+                    if (sp > 0) {
+                        --sp;
+                        curn = stack[sp * kBlock];
+                    } else {
+                        break;
+                    }
+                }
+            }
+            // analytic shapes after the BVH triangle, strict < (Scene.cu:221-227, Sphere.cu:321)
+            for (uint32_t j = 0; j < sc.nsph; ++j) {
+                V3 so, sd;
+                const float ts = sphere_core(sc.sph[j], mk(ox, oy, oz), mk(dx, dy, dz), INFINITY, so, sd);
+                if (ts >= 0 && ts < th) {
+                    th = ts;
+                    hid = -(int32_t)j - 2;
+                }
+            }
+            p.hid[slot] = hid;
+            p.ht[slot] = th;
+        }
+        const uint32_t cls = live ? classify(sc, hid) : 0xffu;
+#pragma unroll
+        for (uint32_t k = 0; k < kNumQ; ++k) {
+            const uint32_t pos = wave_append(&q.counters[kCQueue + k], cls == k);
+            if (cls == k) q.q[k][pos] = slot;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// shading
+// ---------------------------------------------------------------------
+struct Surface {
+    V3 p, n, perr;
+    uint32_t mat;
+};
+
+__device__ __forceinline__ void triangle_surface(const SceneDev &sc, int32_t id, V3 o, V3 d, Surface &s)
+{
+    const float *t = sc.tri_xyz + 9 * (size_t)id;
+    const float4 a = make_float4(t[0], t[1], t[2], 0), b = make_float4(t[3], t[4], t[5], 0),
+                 c = make_float4(t[6], t[7], t[8], 0);
+    // barycentrics of the watertight test (Triangle.cu:161-290)
+    const TriRay tr = make_tri_ray(o.x, o.y, o.z, d.x, d.y, d.z, INFINITY);
+    float p0x = sel3(tr.kx, a.x, a.y, a.z) - tr.ox, p0y = sel3(tr.ky, a.x, a.y, a.z) - tr.oy,
+          p0z = sel3(tr.kz, a.x, a.y, a.z) - tr.oz;
+    float p1x = sel3(tr.kx, b.x, b.y, b.z) - tr.ox, p1y = sel3(tr.ky, b.x, b.y, b.z) - tr.oy,
+          p1z = sel3(tr.kz, b.x, b.y, b.z) - tr.oz;
+    float p2x = sel3(tr.kx, c.x, c.y, c.z) - tr.ox, p2y = sel3(tr.ky, c.x, c.y, c.z) - tr.oy,
+          p2z = sel3(tr.kz, c.x, c.y, c.z) - tr.oz;
+    p0x += tr.Sx * p0z; p0y += tr.Sy * p0z;
+    p1x += tr.Sx * p1z; p1y += tr.Sy * p1z;
+    p2x += tr.Sx * p2z; p2y += tr.Sy * p2z;
+    float e0 = p1x * p2y - p1y * p2x, e1 = p2x * p0y - p2y * p0x, e2 = p0x * p1y - p0y * p1x;
+    if (e0 == 0.0f || e1 == 0.0f || e2 == 0.0f) {
+        e0 = (float)((double)p2y * (double)p1x - (double)p2x * (double)p1y);
+        e1 = (float)((double)p0y * (double)p2x - (double)p0x * (double)p2y);
+        e2 = (float)((double)p1y * (double)p0x - (double)p1x * (double)p0y);
+    }
+    const float det = e0 + e1 + e2;
+    const float invDet = 1 / det;
+    const float b0 = e0 * invDet, b1 = e1 * invDet, b2 = e2 * invDet;
+    const V3 P0 = mk(t[0], t[1], t[2]), P1 = mk(t[3], t[4], t[5]), P2 = mk(t[6], t[7], t[8]);
+    s.p = add(add(mul(P0, b0), mul(P1, b1)), mul(P2, b2));
+    const float xs = fabsf(b0 * P0.x) + fabsf(b1 * P1.x) + fabsf(b2 * P2.x);
+    const float ys = fabsf(b0 * P0.y) + fabsf(b1 * P1.y) + fabsf(b2 * P2.y);
+    const float zs = fabsf(b0 * P0.z) + fabsf(b1 * P1.z) + fabsf(b2 * P2.z);
+    s.perr = mul(mk(xs, ys, zs), error_bound(7));
+    V3 n = nrm(cross(sub(P0, P2), sub(P1, P2)));
+    if (dot(n, d) > 0) n = neg(n);  // normal faces the ray (Triangle.cu:368-373)
+    s.n = n;
+    s.mat = sc.tri_mat[id];
+}
+
+__device__ __forceinline__ void surface_of(const SceneDev &sc, const PathsDev &p, uint32_t slot, Surface &s, V3 &d)
+{
+    const V3 o = mk(p.ox[slot], p.oy[slot], p.oz[slot]);
+    d = mk(p.dx[slot], p.dy[slot], p.dz[slot]);
+    const int32_t hid = p.hid[slot];
+    if (hid >= 0) {
+        triangle_surface(sc, hid, o, d, s);
+    } else {
+        const SphereDev &sp = sc.sph[-hid - 2];
+        V3 so, sd;
+        const float ts = sphere_core(sp, o, d, INFINITY, so, sd);
+        sphere_surface(sp, so, sd, ts, s.p, s.n, s.perr);
+        s.mat = sp.material;
+    }
+}
+
+// Adds the previous bounce's unoccluded next-event contributions, in light
+// order, times the throughput they were computed with.
+__device__ __forceinline__ void resolve_nee(const SceneDev &sc, const PathsDev &p, uint32_t slot)
+{
+    uint32_t mask = p.nee_mask[slot];
+    if (mask == 0) return;
+    V3 Ld = mk(0, 0, 0);
+    const float *c = p.nee_c + (size_t)slot * sc.nlights * 3;
+    while (mask) {
+        const uint32_t l = __ffs(mask) - 1;
+        mask &= mask - 1;
+        Ld = add(Ld, mk(c[3 * l], c[3 * l + 1], c[3 * l + 2]));
+    }
+    const V3 nb = mk(p.nbx[slot], p.nby[slot], p.nbz[slot]);
+    p.lx[slot] = p.lx[slot] + nb.x * Ld.x;
+    p.ly[slot] = p.ly[slot] + nb.y * Ld.y;
+    p.lz[slot] = p.lz[slot] + nb.z * Ld.z;
+    p.nee_mask[slot] = 0;
+}
+
+template <uint32_t CLS>
+__global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathsDev p, ShadowDev sh, QueuesDev q,
+                                                  WaveParams wp, uint32_t cur)
+{
+    constexpr bool kScatter = CLS == kQLambert || CLS == kQGgx;
+    const uint32_t n = q.counters[kCQueue + CLS];
+    const uint32_t *queue = q.q[CLS];
+    const uint32_t nxt = cur ^ 1u;
+    const uint32_t stride = gridDim.x * kBlock;
+    // Every lane of a wavefront runs the same iterations (and the same light
+    // loops, whose trip counts are kernel arguments) so the ballots in
+    // wave_append see whole wavefronts.
+    for (uint32_t i0 = blockIdx.x * kBlock; i0 < n; i0 += stride) {
+        const uint32_t i = i0 + threadIdx.x;
+        const bool live = i < n;
+        const uint32_t slot = live ? queue[i] : 0;
+        bool alive = false;
+        Surface s{};
+        V3 d = mk(0, 0, 1), t = mk(1, 0, 0), b = mk(0, 1, 0), wo = mk(0, 0, 1);
+        rt_material m{};
+        Pcg32 g{0, 1};
+        uint32_t meta = 0, depth = 0;
+        V3 beta = mk(0, 0, 0), L = mk(0, 0, 0);
+        if (live) {
+            resolve_nee(sc, p, slot);
+            if (CLS != kQMiss) {
+                surface_of(sc, p, slot, s, d);
+                m = sc.mat[s.mat];
+                g = load_rng(p, slot, wp);
+                meta = p.meta[slot];
+                depth = meta & 0xffffu;
+                beta = mk(p.bx[slot], p.by[slot], p.bz[slot]);
+                L = mk(p.lx[slot], p.ly[slot], p.lz[slot]);
+                onb(s.n, t, b);
+                const V3 wow = neg(d);
+                wo = mk(dot(wow, t), dot(wow, b), dot(wow, s.n));
+            }
+        }
+        if (kScatter) {
+            // next-event estimation over every point light (Scene.cu:468-525)
+            uint32_t emit = 0;
+            float *c = p.nee_c + (size_t)slot * sc.nlights * 3;
+            for (uint32_t l = 0; l < sc.npl; ++l) {
+                bool e = false;
+                V3 Lw = mk(0, 0, 1);
+                float dist = 0;
+                if (live) {
+                    const rt_point_light &pl = sc.pl[l];
+                    const V3 lp = mk(pl.pos[0], pl.pos[1], pl.pos[2]);
+                    const V3 tol = sub(lp, s.p);
+                    Lw = nrm(tol);
+                    const float d2 = dot(tol, tol);
+                    const float I = pl.intensity;
+                    const V3 Li = mk(pl.color[0] * I * (1 / d2), pl.color[1] * I * (1 / d2), pl.color[2] * I * (1 / d2));
+                    if (dot(Lw, s.n) > 0) {  // isOrientedTo (SurfaceInteraction.cu:283)
+                        const V3 wi = mk(dot(Lw, t), dot(Lw, b), dot(Lw, s.n));
+                        const V3 f = bsdf_f(m, wo, wi);
+                        const float pf = bsdf_pdf(m, wo, wi);
+                        if (!black(f) && !black(Li)) {
+                            const float w = 1.0f / (1.0f + pf * pf);  // powerHeuristic(1, 1, 1, pf)
+                            c[3 * l] = f.x * Li.x * w;
+                            c[3 * l + 1] = f.y * Li.y * w;
+                            c[3 * l + 2] = f.z * Li.z * w;
+                            e = true;
+                            dist = len(tol);
+                        }
+                    }
+                }
+                const uint32_t pos = wave_append(&q.counters[kCShadow], e);
+                if (e) {
+                    emit |= 1u << l;
+                    const V3 o2 = spawn_origin(s.p, s.perr, s.n, Lw);
+                    sh.ox[pos] = o2.x; sh.oy[pos] = o2.y; sh.oz[pos] = o2.z;
+                    sh.dx[pos] = Lw.x; sh.dy[pos] = Lw.y; sh.dz[pos] = Lw.z;
+                    sh.tmax[pos] = dist;
+                    sh.slot[pos] = (slot << 5) | l;
+                    sh.skip[pos] = -1;
+                }
+            }
+            const uint32_t nal = sc.nlights - sc.npl;
+            for (uint32_t a = 0; a < nal; ++a) {
+                const uint32_t j = sc.area_light[a];
+                const uint32_t l = sc.npl + a;
+                bool e = false;
+                V3 Lw = mk(0, 0, 1);
+                float dist = 0;
+                if (live) {
+                    const float a0 = pcg32_float(g), a1 = pcg32_float(g);
+                    float plp;
+                    if (sphere_light_sample(sc.sph[j], s.p, a0, a1, Lw, dist, plp) && dot(Lw, s.n) > 0) {
+                        const V3 wi = mk(dot(Lw, t), dot(Lw, b), dot(Lw, s.n));
+                        const V3 f = bsdf_f(m, wo, wi);
+                        if (!black(f)) {
+                            const float pf = bsdf_pdf(m, wo, wi);
+                            const rt_material &lm = sc.mat[sc.sph[j].material];
+                            const float w = plp * plp / (plp * plp + pf * pf);
+                            const float k = w / plp;
+                            c[3 * l] = f.x * lm.kd[0] * lm.intensity * k;
+                            c[3 * l + 1] = f.y * lm.kd[1] * lm.intensity * k;
+                            c[3 * l + 2] = f.z * lm.kd[2] * lm.intensity * k;
+                            e = true;
+                        }
+                    }
+                }
+                const uint32_t pos = wave_append(&q.counters[kCShadow], e);
+                if (e) {
+                    emit |= 1u << l;
+                    const V3 o2 = spawn_origin(s.p, s.perr, s.n, Lw);
+                    sh.ox[pos] = o2.x; sh.oy[pos] = o2.y; sh.oz[pos] = o2.z;
+                    sh.dx[pos] = Lw.x; sh.dy[pos] = Lw.y; sh.dz[pos] = Lw.z;
+                    sh.tmax[pos] = dist * 0.9999f;
+                    sh.slot[pos] = (slot << 5) | l;
+                    sh.skip[pos] = (int32_t)j;
+                }
+            }
+            if (live) {
+                p.nbx[slot] = beta.x; p.nby[slot] = beta.y; p.nbz[slot] = beta.z;
+                p.nee_mask[slot] = emit;
+            }
+        }
+        if (live && CLS != kQMiss) {
+            if (CLS == kQLight) {
+                // getEmittedRadiance (LightMaterial.cu:20-23), MIS-weighted
+                // against the light's own next-event sampling
+                float w = 1;
+                const int32_t hid = p.hid[slot];
+                if (hid <= -2 && sc.sph[-hid - 2].is_light && !((meta >> 16) & 1u)) {
+                    const float plp = sphere_light_pdf(sc.sph[-hid - 2], mk(p.ppx[slot], p.ppy[slot], p.ppz[slot]));
+                    const float pp = p.ppdf[slot];
+                    w = pp * pp / (pp * pp + plp * plp);
+                }
+                const float I = m.intensity;
+                L = add(L, mk(beta.x * m.kd[0] * I * w, beta.y * m.kd[1] * I * w, beta.z * m.kd[2] * I * w));
+            }
+            V3 wi;
+            float pdf;
+            bool delta;
+            constexpr uint32_t MT = CLS == kQLambert ? RT_MAT_LAMBERTIAN
+                                    : CLS == kQDielectric ? RT_MAT_DIELECTRIC
+                                    : CLS == kQGgx ? RT_MAT_TROWBRIDGE : RT_MAT_LIGHT;
+            const V3 wgt = bsdf_sample<MT>(m, wo, g, wi, pdf, delta);
+            beta = hmul(beta, wgt);
+            alive = !black(wgt);
+            if (alive && kScatter && depth > wp.rr_depth) {
+                const float qq = fmaxf(0.05f, 1 - luminance(beta));
+                if (pcg32_float(g) < qq) {
+                    alive = false;
+                } else {
+                    const float sc1 = 1 - qq;
+                    beta = mk(beta.x / sc1, beta.y / sc1, beta.z / sc1);
+                }
+            }
+            if (alive && depth + 1 >= wp.max_depth) alive = false;
+            if (alive) {
+                const V3 ww = add(add(mul(t, wi.x), mul(b, wi.y)), mul(s.n, wi.z));
+                const V3 no = spawn_origin(s.p, s.perr, s.n, ww);
+                p.ox[slot] = no.x; p.oy[slot] = no.y; p.oz[slot] = no.z;
+                p.dx[slot] = ww.x; p.dy[slot] = ww.y; p.dz[slot] = ww.z;
+                p.ppdf[slot] = pdf;
+                p.ppx[slot] = s.p.x; p.ppy[slot] = s.p.y; p.ppz[slot] = s.p.z;
+                p.meta[slot] = (depth + 1) | ((delta ? 1u : 0u) << 16);
+            }
+            p.bx[slot] = beta.x; p.by[slot] = beta.y; p.bz[slot] = beta.z;
+            p.lx[slot] = L.x; p.ly[slot] = L.y; p.lz[slot] = L.z;
+            p.rng[slot] = g.state;
+        }
+        const uint32_t ppos = wave_append(&q.counters[kCActive0 + nxt], live && alive);
+        if (live && alive) q.active[nxt][ppos] = slot;
+        const uint32_t tpos = wave_append(&q.counters[kCTerm], live && !alive);
+        if (live && !alive) q.term[tpos] = slot;
+    }
+}
+
+// ---------------------------------------------------------------------
+// connect: any hit for shadow rays (Scene.cu:108-150, 238-276)
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc, PathsDev p, ShadowDev sh, QueuesDev q, int cap)
+{
+    extern __shared__ int32_t s_stack[];
+    int32_t *stack = s_stack + threadIdx.x;
+    const uint32_t n = q.counters[kCShadow];
+    const float4 *nodes = (const float4 *)sc.bvh.nodes;
+    const float4 *tris = (const float4 *)sc.bvh.tris;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const float ox = sh.ox[i], oy = sh.oy[i], oz = sh.oz[i];
+        const float dx = sh.dx[i], dy = sh.dy[i], dz = sh.dz[i];
+        const float tmax = sh.tmax[i];
+        bool occ = false;
+        // analytic shapes first: cheap, and any opaque hit decides
+        const int32_t skip = sh.skip[i];
+        for (uint32_t j = 0; j < sc.nsph && !occ; ++j) {
+            if ((int32_t)j == skip || sc.sph[j].transparent) continue;
+            V3 so, sd;
+            const float ts = sphere_core(sc.sph[j], mk(ox, oy, oz), mk(dx, dy, dz), INFINITY, so, sd);
+            if (ts >= 0 && !(ts > tmax)) occ = true;
+        }
+        if (!occ) {
+            const TriRay tr = make_tri_ray(ox, oy, oz, dx, dy, dz, INFINITY);
+            const BoxRay br = make_box_ray(ox, oy, oz, dx, dy, dz);
+            const float tlim = tmax * kHitScale;
+            int sp = 0;
+            int32_t curn = 0;
+            for (;;) {
+                if (curn >= 0) {
+                    const float4 *np = nodes + 4 * (size_t)curn;
+                    const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+                    bool h0, h1;
+                    const float t0 = slab(br, a.x, a.y, a.z, a.w, c.x, c.y, tlim, &h0);
+                    const float t1 = slab(br, b.x, b.y, b.z, b.w, c.z, c.w, tlim, &h1);
+                    const int32_t c0 = __float_as_int(d.x), c1 = __float_as_int(d.y);
+                    if (h0 && h1) {
+                        const bool sw = t1 < t0;
+                        if (sp < cap) { stack[sp * kBlock] = sw ? c0 : c1; ++sp; }
+                        curn = sw ? c1 : c0;
+                    } else if (h0) {
+                        curn = c0;
+                    } else if (h1) {
+                        curn = c1;
+                    } else if (sp > 0) {
+                        --sp;
+                        curn = stack[sp * kBlock];
+                    } else {
+                        break;
+                    }
+                } else {
+                    uint32_t first, count;
+                    decode_leaf(curn, first, count);
+                    for (uint32_t k = 0; k < count && !occ; ++k) {
+                        const float4 *tp = tris + 3 * (size_t)(first + k);
+                        const float4 vb = tp[1];
+                        if (__float_as_uint(vb.w) & 1u) continue;  // transparent (Triangle.cu:439-440)
+                        float hh = INFINITY;
+                        if (tri_intersects(tr, tp[0], vb, tp[2], hh) && !(hh > tmax)) occ = true;
+                    }
+                    if (occ) break;
+                    if (sp > 0) {
+                        --sp;
+                        curn = stack[sp * kBlock];
+                    } else {
+                        break;
+                    }
+                }
+            }
+        }
+        if (occ) {
+            const uint32_t sl = sh.slot[i];
+            atomicAnd(&p.nee_mask[sl >> 5], ~(1u << (sl & 31u)));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// finalize: FP64 film accumulation with the reference's NaN/Inf sentinel
+// bookkeeping (pathTracing.cu:474-486): only finite samples are summed and
+// counted; the film consumer rescales by samples / nvalid.
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_finalize(SceneDev sc, PathsDev p, QueuesDev q, FilmDev film,
+                                                     uint32_t which, uint32_t cur)
+{
+    const uint32_t n = which ? q.counters[kCActive0 + cur] : q.counters[kCTerm];
+    const uint32_t *list = which ? q.active[cur] : q.term;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const uint32_t slot = list[i];
+        resolve_nee(sc, p, slot);
+        const float lx = p.lx[slot], ly = p.ly[slot], lz = p.lz[slot];
+        if (isfinite(lx) && isfinite(ly) && isfinite(lz)) {
+            const uint32_t px = p.pix[slot];
+            unsafeAtomicAdd(&film.accum[3 * (size_t)px], (double)lx);
+            unsafeAtomicAdd(&film.accum[3 * (size_t)px + 1], (double)ly);
+            unsafeAtomicAdd(&film.accum[3 * (size_t)px + 2], (double)lz);
+            atomicAdd(&film.nvalid[px], 1u);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_count(QueuesDev q, uint32_t cur)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        q.stats[0] += q.counters[kCActive0 + cur];
+        q.stats[1] += q.counters[kCShadow];
+    }
+}
+
+size_t extend_lds_bytes(uint32_t max_depth) { return (size_t)(max_depth + 1) * kBlock * sizeof(int32_t); }
+
+hipError_t render_wave(const SceneDev &sc, const PathsDev &paths, const ShadowDev &shadow, const QueuesDev &q,
+                       const FilmDev &film, const WaveParams &wp, uint32_t capacity, hipStream_t stream)
+{
+    const int cap = (int)sc.bvh.max_depth + 1;
+    const size_t lds = extend_lds_bytes(sc.bvh.max_depth);
+    const dim3 blk(kBlock);
+    const uint32_t gen_grid = (wp.count + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_reset, dim3(1), blk, 0, stream, q.counters, 0u, wp.count);
+    hipLaunchKernelGGL(k_generate, dim3(gen_grid ? gen_grid : 1), blk, 0, stream, sc, paths, q, wp);
+    uint32_t cur = 0;
+    for (uint32_t bounce = 0; bounce < wp.max_depth; ++bounce) {
+        if (bounce > 0) hipLaunchKernelGGL(k_reset, dim3(1), blk, 0, stream, q.counters, cur, 0xffffffffu);
+        hipLaunchKernelGGL(k_extend, dim3(kGrid), blk, lds, stream, sc, paths, q, cur, cap);
+        hipLaunchKernelGGL((k_shade<kQLambert>), dim3(kGrid), blk, 0, stream, sc, paths, shadow, q, wp, cur);
+        hipLaunchKernelGGL((k_shade<kQDielectric>), dim3(kGrid), blk, 0, stream, sc, paths, shadow, q, wp, cur);
+        hipLaunchKernelGGL((k_shade<kQGgx>), dim3(kGrid), blk, 0, stream, sc, paths, shadow, q, wp, cur);
+        hipLaunchKernelGGL((k_shade<kQLight>), dim3(kGrid), blk, 0, stream, sc, paths, shadow, q, wp, cur);
+        hipLaunchKernelGGL((k_shade<kQMiss>), dim3(kGrid), blk, 0, stream, sc, paths, shadow, q, wp, cur);
+        hipLaunchKernelGGL(k_count, dim3(1), blk, 0, stream, q, cur);
+        hipLaunchKernelGGL(k_connect, dim3(kGrid), blk, lds, stream, sc, paths, shadow, q, cap);
+        hipLaunchKernelGGL(k_finalize, dim3(kGrid), blk, 0, stream, sc, paths, q, film, 0u, cur);
+        cur ^= 1u;
+    }
+    // paths still alive after max_depth bounces cannot exist (shade stops
+    // them at max_depth), but the list is flushed for safety
+    hipLaunchKernelGGL(k_finalize, dim3(kGrid), blk, 0, stream, sc, paths, q, film, 1u, cur);
+    return hipGetLastError();
+}
+
+}  // namespace rt

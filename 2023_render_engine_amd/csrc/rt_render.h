@@ -1,0 +1,86 @@
+// rt_render.h -- device buffers and launch interface of the wavefront path
+// tracer (render.hip), used by the host scene code (scene.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "rt_scene_types.h"
+#include "rt_trace.h"
+
+namespace rt {
+
+// Queue classes filled by the extend stage (material-sorted shading).
+enum : uint32_t { kQLambert = 0, kQDielectric = 1, kQGgx = 2, kQLight = 3, kQMiss = 4, kNumQ = 5 };
+
+// Device counters (one u32 each).
+enum : uint32_t {
+    kCActive0 = 0, kCActive1 = 1, kCQueue = 2, /* 2..6 */ kCShadow = 7, kCTerm = 8, kNumCounters = 16
+};
+
+// Scene as the kernels see it (passed by value).
+struct SceneDev {
+    DevBvh bvh;
+    const float *tri_xyz;      // 9 floats per triangle id
+    const uint32_t *tri_mat;   // material per triangle id
+    const rt_material *mat;
+    const rt_point_light *pl;
+    SphereDev sph[kMaxSpheres];
+    CameraDev cam;
+    uint32_t nsph, npl, nlights;  // nlights = npl + area lights (NEE slots per path)
+    uint32_t area_light[kMaxSpheres];  // sphere index of area light a
+    uint32_t nmat;
+};
+
+// Structure-of-arrays path state (capacity P).
+struct PathsDev {
+    float *ox, *oy, *oz, *dx, *dy, *dz;
+    float *bx, *by, *bz;        // throughput
+    float *lx, *ly, *lz;        // radiance
+    uint64_t *rng;              // PCG32 state (the increment is derived from the pixel)
+    uint32_t *pix;
+    uint32_t *meta;             // depth | prev_delta << 16
+    float *ppdf, *ppx, *ppy, *ppz;  // previous BSDF pdf and vertex (MIS of sphere lights)
+    int32_t *hid;               // >= 0 triangle, <= -2 sphere -(j+2), -1 miss
+    float *ht;
+    float *nbx, *nby, *nbz;     // throughput at the last next-event estimation
+    uint32_t *nee_mask;         // unoccluded light slots of the last NEE
+    float *nee_c;               // P * nlights * 3: per-light f * Li * w
+};
+
+struct ShadowDev {
+    float *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
+    uint32_t *slot;             // path slot << 5 | light slot
+    int32_t *skip;              // sphere to ignore (the sampled light), -1 none
+};
+
+struct QueuesDev {
+    uint32_t *active[2];
+    uint32_t *q[kNumQ];
+    uint32_t *term;
+    uint32_t *counters;
+    unsigned long long *stats;  // [0] extension rays, [1] shadow rays
+};
+
+struct FilmDev {
+    double *accum;              // W*H*3
+    uint32_t *nvalid;           // W*H
+};
+
+struct WaveParams {
+    uint64_t first_path;        // global path index of slot 0
+    uint32_t count;             // paths in this wave
+    uint32_t npix;              // W*H
+    uint32_t spp_offset, seed, max_depth, rr_depth;
+    uint64_t stream_base;       // PCG stream offset (scene id for batches)
+};
+
+// Launches one wave (generate, max_depth x {extend, shade, connect},
+// finalize) on `stream`.  Fails only on launch errors.
+hipError_t render_wave(const SceneDev &scene, const PathsDev &paths, const ShadowDev &shadow,
+                       const QueuesDev &queues, const FilmDev &film, const WaveParams &wp, uint32_t capacity,
+                       hipStream_t stream);
+
+size_t extend_lds_bytes(uint32_t max_depth);
+
+}  // namespace rt

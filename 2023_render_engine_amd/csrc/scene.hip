@@ -1,0 +1,509 @@
+// scene.hip -- host side of the scene API (rt_scene_*) and rt_render:
+// gathers meshes, spheres, materials and lights, composes the camera and
+// sphere transforms exactly as the reference does, builds the BVH, owns the
+// wavefront buffers and launches waves.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "rt_internal.h"
+#include "rt_render.h"
+
+struct rt_ctx;
+namespace rt {
+void ctx_retain(rt_ctx *c);
+void ctx_release(rt_ctx *c);
+int ctx_device(rt_ctx *c);
+void *ctx_stream_ptr(rt_ctx *c);
+std::recursive_mutex &ctx_lock(rt_ctx *c);
+DevBvh bvh_device(const rt_bvh *b);
+}  // namespace rt
+
+using rt::set_error;
+
+#define RT_HIP(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            set_error(std::string(#call) + " failed: " + hipGetErrorString(e_));        \
+            return e_ == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;                 \
+        }                                                                               \
+    } while (0)
+
+namespace {
+
+// ---- reference matrix algebra (transformations/Transform.cu,
+//      cuda_libraries/Matrix4x4.cu) in FP32, row-major ----
+void mat_mul(const float *a, const float *b, float *out)
+{
+    float r[16];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            float s = 0;
+            for (int k = 0; k < 4; ++k) s += a[i * 4 + k] * b[k * 4 + j];
+            r[i * 4 + j] = s;
+        }
+    std::memcpy(out, r, sizeof r);
+}
+
+// Cofactor inverse with the reference's term order (Matrix4x4.cu:303-440).
+void mat_inverse(const float *m, float *out)
+{
+    float v[16];
+    v[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] + m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+    v[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] - m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+    v[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] + m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+    v[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] - m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+    v[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] - m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+    v[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] + m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+    v[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] - m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+    v[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] + m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+    v[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] + m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+    v[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] - m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+    v[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] + m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+    v[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] - m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+    v[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] - m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+    v[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] + m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+    v[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] - m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+    v[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] + m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+    float det = m[0] * v[0] + m[1] * v[4] + m[2] * v[8] + m[3] * v[12];
+    if (det == 0) {
+        std::memset(out, 0, 16 * sizeof(float));
+        return;
+    }
+    det = (float)(1.0 / det);
+    for (int i = 0; i < 16; ++i) out[i] = v[i] * det;
+}
+
+struct F3 {
+    float x, y, z;
+};
+F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+F3 fsub(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+F3 fcross(F3 a, F3 b)
+{
+    return f3((a.y * b.z) - (a.z * b.y), (a.z * b.x) - (a.x * b.z), (a.x * b.y) - (a.y * b.x));
+}
+F3 fnrm(F3 a)
+{
+    float l = std::sqrt((a.x * a.x) + (a.y * a.y) + (a.z * a.z));
+    float inv = (float)1 / l;
+    return f3(a.x * inv, a.y * inv, a.z * inv);
+}
+
+// ProjectiveCamera(inverse(lookAt(p, look, up)), translate(cts), window)
+// (FileImporting.cu:404-436, ProjectiveCamera.cu:26-57, Transform.cu:611-640).
+void compose_camera(const rt_camera &c, rt::CameraDev &out)
+{
+    F3 pos = f3(c.pos[0], c.pos[1], c.pos[2]);
+    F3 look = f3(c.lookat[0], c.lookat[1], c.lookat[2]);
+    F3 up = f3(c.up[0], c.up[1], c.up[2]);
+    F3 dir = fnrm(fsub(look, pos));
+    F3 right = fnrm(fcross(fnrm(up), dir));
+    F3 nup = fcross(dir, right);
+    float c2w[16] = {right.x, nup.x, dir.x, pos.x, right.y, nup.y, dir.y, pos.y,
+                     right.z, nup.z, dir.z, pos.z, 0, 0, 0, 1};
+    float la[16];
+    mat_inverse(c2w, la);          // the Transform lookAt returns (its mInv is dropped)
+    mat_inverse(la, out.c2w);      // the camera's cameraToWorld = inverse(lookAt)
+    float vfs = c.frame_y;
+    vfs /= 2;
+    float hfs = (vfs / (float)c.height) * (float)c.width;
+    float pminx = -hfs, pminy = -vfs, pmaxx = hfs, pmaxy = vfs;
+    float T[16] = {1, 0, 0, -pminx, 0, 1, 0, -pmaxy, 0, 0, 1, 0, 0, 0, 0, 1};
+    float S1[16] = {1 / (pmaxx - pminx), 0, 0, 0, 0, 1 / (pminy - pmaxy), 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    float S2[16] = {(float)c.width, 0, 0, 0, 0, (float)c.height, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    float tmp[16], s2r[16], r2s[16];
+    mat_mul(S1, T, tmp);
+    mat_mul(S2, tmp, s2r);
+    mat_inverse(s2r, r2s);
+    float cts[16] = {1, 0, 0, c.cts[0], 0, 1, 0, c.cts[1], 0, 0, 1, c.cts[2], 0, 0, 0, 1};
+    float icts[16];
+    mat_inverse(cts, icts);
+    mat_mul(icts, r2s, out.r2c);
+    out.width = c.width;
+    out.height = c.height;
+}
+
+// Sphere(center, radius): translate(c) * rotateZ(90) (Sphere.cu:233-245,
+// Transform.cu:475-488, 536-547; radians() at math.cu:66-68).
+void compose_sphere(const float c[3], float r, rt::SphereDev &s)
+{
+    const float PIf = 3.14159265358979323846f;
+    float ang = (float)((90.0f * PIf) / 180.0);
+    float st = std::sin(ang), ct = std::cos(ang);
+    float T[16] = {1, 0, 0, c[0], 0, 1, 0, c[1], 0, 0, 1, c[2], 0, 0, 0, 1};
+    float R[16] = {ct, -st, 0, 0, st, ct, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    mat_mul(T, R, s.o2w);
+    mat_inverse(s.o2w, s.w2o);
+    s.center[0] = c[0]; s.center[1] = c[1]; s.center[2] = c[2];
+    s.radius = r;
+}
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace
+
+struct rt_scene {
+    rt_ctx *ctx = nullptr;
+    // host description
+    std::vector<float> xyz;         // 9 floats per triangle, id order
+    std::vector<uint32_t> tri_mat;
+    std::vector<rt_material> mats;
+    std::vector<rt::SphereDev> spheres;
+    std::vector<rt_point_light> lights;
+    rt_camera cam{};
+    bool has_cam = false;
+    bool committed = false;
+    // device state
+    rt_bvh *bvh = nullptr;
+    float *d_tri_xyz = nullptr;
+    uint32_t *d_tri_mat = nullptr;
+    rt_material *d_mat = nullptr;
+    rt_point_light *d_pl = nullptr;
+    rt::SceneDev dev{};
+    // wavefront buffers (grown on demand)
+    uint32_t capacity = 0, cap_lights = 0;
+    void *arena = nullptr;
+    rt::PathsDev paths{};
+    rt::ShadowDev shadow{};
+    rt::QueuesDev queues{};
+    rt_render_stats stats{};
+    uint64_t stream_base = 0;
+};
+
+namespace rt {
+
+static void free_arena(rt_scene *s)
+{
+    if (s->arena) (void)hipFree(s->arena);
+    s->arena = nullptr;
+    s->capacity = 0;
+}
+
+// One allocation for all wavefront buffers, carved in 256-byte aligned
+// pieces: paths (SoA), shadow queue (capacity * lights entries), queues.
+static rt_status ensure_buffers(rt_scene *s, uint32_t P)
+{
+    const uint32_t NL = std::max<uint32_t>(1, s->dev.nlights);
+    if (s->capacity >= P && s->cap_lights >= NL) return RT_OK;
+    free_arena(s);
+    const size_t SQ = (size_t)P * NL;
+    size_t total = 0;
+    auto take = [&](size_t bytes) {
+        size_t off = total;
+        total += (bytes + 255) & ~(size_t)255;
+        return off;
+    };
+    struct Piece { void **dst; size_t off; };
+    std::vector<Piece> pieces;
+    auto add = [&](void **dst, size_t bytes) { pieces.push_back({dst, take(bytes)}); };
+    PathsDev &p = s->paths;
+    float **pf[] = {&p.ox, &p.oy, &p.oz, &p.dx, &p.dy, &p.dz, &p.bx, &p.by, &p.bz, &p.lx, &p.ly, &p.lz,
+                    &p.ppdf, &p.ppx, &p.ppy, &p.ppz, &p.ht, &p.nbx, &p.nby, &p.nbz};
+    for (float **f : pf) add((void **)f, (size_t)P * 4);
+    add((void **)&p.rng, (size_t)P * 8);
+    add((void **)&p.pix, (size_t)P * 4);
+    add((void **)&p.meta, (size_t)P * 4);
+    add((void **)&p.hid, (size_t)P * 4);
+    add((void **)&p.nee_mask, (size_t)P * 4);
+    add((void **)&p.nee_c, (size_t)P * NL * 12);
+    ShadowDev &sh = s->shadow;
+    float **sf[] = {&sh.ox, &sh.oy, &sh.oz, &sh.dx, &sh.dy, &sh.dz, &sh.tmax};
+    for (float **f : sf) add((void **)f, SQ * 4);
+    add((void **)&sh.slot, SQ * 4);
+    add((void **)&sh.skip, SQ * 4);
+    QueuesDev &q = s->queues;
+    add((void **)&q.active[0], (size_t)P * 4);
+    add((void **)&q.active[1], (size_t)P * 4);
+    for (uint32_t k = 0; k < kNumQ; ++k) add((void **)&q.q[k], (size_t)P * 4);
+    add((void **)&q.term, (size_t)P * 4);
+    add((void **)&q.counters, kNumCounters * 4);
+    add((void **)&q.stats, 2 * 8);
+    char *base = nullptr;
+    hipError_t e = hipMalloc(&base, total);
+    if (e != hipSuccess) {
+        set_error(std::string("rt_render: wavefront buffers (") + std::to_string(total >> 20) +
+                  " MiB): " + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;
+    }
+    for (auto &pc : pieces) *pc.dst = base + pc.off;
+    s->arena = base;
+    s->capacity = P;
+    s->cap_lights = NL;
+    return RT_OK;
+}
+
+}  // namespace rt
+
+extern "C" {
+
+rt_status rt_scene_create(rt_ctx *ctx, rt_scene **out)
+{
+    if (!ctx || !out) { set_error("rt_scene_create: null argument"); return RT_ERR_INVALID; }
+    rt_scene *s = new (std::nothrow) rt_scene();
+    if (!s) { set_error("rt_scene_create: host allocation failed"); return RT_ERR_OOM; }
+    s->ctx = ctx;
+    rt::ctx_retain(ctx);
+    *out = s;
+    return RT_OK;
+}
+
+static void free_device(rt_scene *s)
+{
+    if (s->bvh) rt_bvh_destroy(s->bvh);
+    s->bvh = nullptr;
+    if (s->d_tri_xyz) (void)hipFree(s->d_tri_xyz);
+    if (s->d_tri_mat) (void)hipFree(s->d_tri_mat);
+    if (s->d_mat) (void)hipFree(s->d_mat);
+    if (s->d_pl) (void)hipFree(s->d_pl);
+    s->d_tri_xyz = nullptr; s->d_tri_mat = nullptr; s->d_mat = nullptr; s->d_pl = nullptr;
+    rt::free_arena(s);
+    s->committed = false;
+}
+
+rt_status rt_scene_destroy(rt_scene *s)
+{
+    if (!s) return RT_OK;
+    rt_ctx *ctx = s->ctx;
+    {
+        std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(ctx));
+        (void)hipSetDevice(rt::ctx_device(ctx));
+        (void)hipStreamSynchronize((hipStream_t)rt::ctx_stream_ptr(ctx));
+        free_device(s);
+        delete s;
+    }
+    rt::ctx_release(ctx);
+    return RT_OK;
+}
+
+rt_status rt_scene_add_material(rt_scene *s, const rt_material *m, uint32_t *id)
+{
+    if (!s || !m) { set_error("rt_scene_add_material: null argument"); return RT_ERR_INVALID; }
+    if (m->type > RT_MAT_LIGHT) { set_error("rt_scene_add_material: unknown material type"); return RT_ERR_INVALID; }
+    if (m->type == RT_MAT_DIELECTRIC && !(m->ior > 0)) { set_error("rt_scene_add_material: ior must be > 0"); return RT_ERR_INVALID; }
+    if ((m->type == RT_MAT_TROWBRIDGE || m->type == RT_MAT_TROWBRIDGE_BASIC) && !(m->roughness >= 0)) {
+        set_error("rt_scene_add_material: roughness must be >= 0");
+        return RT_ERR_INVALID;
+    }
+    s->mats.push_back(*m);
+    if (id) *id = (uint32_t)s->mats.size() - 1;
+    s->committed = false;
+    return RT_OK;
+}
+
+rt_status rt_scene_add_mesh(rt_scene *s, const float *xyz, uint32_t nv, const uint32_t *idx, uint32_t nt,
+                            uint32_t material, uint32_t *first_id)
+{
+    if (!s || (nt && (!xyz || !idx))) { set_error("rt_scene_add_mesh: null argument"); return RT_ERR_INVALID; }
+    if (material >= s->mats.size()) { set_error("rt_scene_add_mesh: unknown material id"); return RT_ERR_INVALID; }
+    const size_t base = s->tri_mat.size();
+    if (base + nt >= (1u << 27)) { set_error("rt_scene_add_mesh: more than 2^27 triangles"); return RT_ERR_INVALID; }
+    for (uint32_t i = 0; i < 3 * nt; ++i)
+        if (idx[i] >= nv) { set_error("rt_scene_add_mesh: vertex index out of range"); return RT_ERR_INVALID; }
+    s->xyz.resize((base + nt) * 9);
+    for (uint32_t i = 0; i < nt; ++i)
+        for (int v = 0; v < 3; ++v)
+            for (int k = 0; k < 3; ++k) s->xyz[(base + i) * 9 + v * 3 + k] = xyz[3 * (size_t)idx[3 * i + v] + k];
+    s->tri_mat.resize(base + nt, material);
+    if (first_id) *first_id = (uint32_t)base;
+    s->committed = false;
+    return RT_OK;
+}
+
+rt_status rt_scene_add_sphere(rt_scene *s, const float center[3], float radius, uint32_t material, int is_light)
+{
+    if (!s || !center) { set_error("rt_scene_add_sphere: null argument"); return RT_ERR_INVALID; }
+    if (material >= s->mats.size()) { set_error("rt_scene_add_sphere: unknown material id"); return RT_ERR_INVALID; }
+    if (!(radius > 0)) { set_error("rt_scene_add_sphere: radius must be > 0"); return RT_ERR_INVALID; }
+    if (s->spheres.size() >= (size_t)rt::kMaxSpheres) { set_error("rt_scene_add_sphere: too many spheres"); return RT_ERR_UNSUPPORTED; }
+    if (is_light && s->mats[material].type != RT_MAT_LIGHT) {
+        set_error("rt_scene_add_sphere: a sphere light needs a RT_MAT_LIGHT material");
+        return RT_ERR_INVALID;
+    }
+    rt::SphereDev d{};
+    compose_sphere(center, radius, d);
+    d.material = material;
+    d.is_light = is_light ? 1u : 0u;
+    d.transparent = s->mats[material].type == RT_MAT_DIELECTRIC ? 1u : 0u;
+    s->spheres.push_back(d);
+    s->committed = false;
+    return RT_OK;
+}
+
+rt_status rt_scene_add_point_light(rt_scene *s, const rt_point_light *l)
+{
+    if (!s || !l) { set_error("rt_scene_add_point_light: null argument"); return RT_ERR_INVALID; }
+    if (s->lights.size() >= (size_t)rt::kMaxPointLights) { set_error("rt_scene_add_point_light: too many lights"); return RT_ERR_UNSUPPORTED; }
+    s->lights.push_back(*l);
+    s->committed = false;
+    return RT_OK;
+}
+
+rt_status rt_scene_set_camera(rt_scene *s, const rt_camera *c)
+{
+    if (!s || !c) { set_error("rt_scene_set_camera: null argument"); return RT_ERR_INVALID; }
+    if (c->width == 0 || c->height == 0 || !(c->frame_y > 0)) { set_error("rt_scene_set_camera: bad resolution or frame"); return RT_ERR_INVALID; }
+    s->cam = *c;
+    s->has_cam = true;
+    compose_camera(*c, s->dev.cam);
+    s->committed = false;
+    return RT_OK;
+}
+
+rt_status rt_scene_commit(rt_scene *s)
+{
+    if (!s) { set_error("rt_scene_commit: null scene"); return RT_ERR_INVALID; }
+    if (!s->has_cam) { set_error("rt_scene_commit: no camera"); return RT_ERR_INVALID; }
+    std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(s->ctx));
+    RT_HIP(hipSetDevice(rt::ctx_device(s->ctx)));
+    hipStream_t stream = (hipStream_t)rt::ctx_stream_ptr(s->ctx);
+    free_device(s);
+    const uint32_t nt = (uint32_t)s->tri_mat.size();
+    std::vector<uint32_t> idx(3 * (size_t)nt);
+    for (uint32_t i = 0; i < 3 * nt; ++i) idx[i] = i;
+    rt_bvh_opts o{4, 32, 1.2f};
+    rt_status st = rt_bvh_build(s->ctx, s->xyz.data(), 3 * nt, idx.data(), nt, &o, &s->bvh);
+    if (st != RT_OK) return st;
+    std::vector<uint8_t> transparent(nt);
+    for (uint32_t i = 0; i < nt; ++i) transparent[i] = s->mats[s->tri_mat[i]].type == RT_MAT_DIELECTRIC;
+    if (nt) {
+        st = rt_bvh_set_transparent(s->bvh, transparent.data());
+        if (st != RT_OK) return st;
+    }
+    const size_t nmat = std::max<size_t>(1, s->mats.size()), npl = std::max<size_t>(1, s->lights.size());
+    RT_HIP(hipMalloc(&s->d_tri_xyz, std::max<size_t>(1, s->xyz.size()) * 4));
+    RT_HIP(hipMalloc(&s->d_tri_mat, std::max<size_t>(1, nt) * 4));
+    RT_HIP(hipMalloc(&s->d_mat, nmat * sizeof(rt_material)));
+    RT_HIP(hipMalloc(&s->d_pl, npl * sizeof(rt_point_light)));
+    if (!s->xyz.empty()) RT_HIP(hipMemcpyAsync(s->d_tri_xyz, s->xyz.data(), s->xyz.size() * 4, hipMemcpyHostToDevice, stream));
+    if (nt) RT_HIP(hipMemcpyAsync(s->d_tri_mat, s->tri_mat.data(), nt * 4, hipMemcpyHostToDevice, stream));
+    if (!s->mats.empty()) RT_HIP(hipMemcpyAsync(s->d_mat, s->mats.data(), s->mats.size() * sizeof(rt_material), hipMemcpyHostToDevice, stream));
+    if (!s->lights.empty()) RT_HIP(hipMemcpyAsync(s->d_pl, s->lights.data(), s->lights.size() * sizeof(rt_point_light), hipMemcpyHostToDevice, stream));
+    RT_HIP(hipStreamSynchronize(stream));
+
+    rt::SceneDev &d = s->dev;
+    rt_bvh_info info;
+    rt_bvh_get_info(s->bvh, &info);
+    d.bvh = rt::bvh_device(s->bvh);
+    d.tri_xyz = s->d_tri_xyz;
+    d.tri_mat = s->d_tri_mat;
+    d.mat = s->d_mat;
+    d.pl = s->d_pl;
+    d.nsph = (uint32_t)s->spheres.size();
+    for (uint32_t j = 0; j < d.nsph; ++j) d.sph[j] = s->spheres[j];
+    d.npl = (uint32_t)s->lights.size();
+    uint32_t na = 0;
+    for (uint32_t j = 0; j < d.nsph; ++j)
+        if (s->spheres[j].is_light) d.area_light[na++] = j;
+    d.nlights = d.npl + na;
+    d.nmat = (uint32_t)s->mats.size();
+    if (d.nlights > 32) { set_error("rt_scene_commit: more than 32 light slots"); return RT_ERR_UNSUPPORTED; }
+    if (rt::extend_lds_bytes(info.max_depth) > 64 * 1024) { set_error("rt_scene_commit: BVH too deep"); return RT_ERR_UNSUPPORTED; }
+    s->committed = true;
+    return RT_OK;
+}
+
+rt_bvh *rt_scene_bvh(rt_scene *s) { return s ? s->bvh : nullptr; }
+
+rt_status rt_scene_camera_ray(const rt_scene *s, uint32_t px, uint32_t py, float u0, float u1, float o[3], float d[3])
+{
+    if (!s || !s->has_cam || !o || !d) { set_error("rt_scene_camera_ray: null argument or no camera"); return RT_ERR_INVALID; }
+    // same arithmetic as camera_ray() in rt_shade.h, on the host
+    const rt::CameraDev &c = s->dev.cam;
+    float fx = (float)px + u0, fy = (float)py + u1;
+    fx = std::fmin(std::fmax(fx, 0.0f), (float)c.width);
+    fy = std::fmin(std::fmax(fy, 0.0f), (float)c.height);
+    const float *m = c.r2c;
+    float xp = m[0] * fx + m[1] * fy + m[2] * 0.0f + m[3];
+    float yp = m[4] * fx + m[5] * fy + m[6] * 0.0f + m[7];
+    float zp = m[8] * fx + m[9] * fy + m[10] * 0.0f + m[11];
+    float wp = m[12] * fx + m[13] * fy + m[14] * 0.0f + m[15];
+    float inv = 1.0f / wp;
+    F3 pc = f3(inv * xp, inv * yp, inv * zp);
+    F3 dir = fnrm(pc);
+    const float *w = c.c2w;
+    const double me = (double)FLT_EPSILON * 0.5;
+    const float eb3 = (float)((3 * me) / (1 - 3 * me));
+    F3 oerr = f3(eb3 * std::fabs(w[3]), eb3 * std::fabs(w[7]), eb3 * std::fabs(w[11]));
+    float ox = w[0] * 0.0f + w[1] * 0.0f + w[2] * 0.0f + w[3];
+    float oy = w[4] * 0.0f + w[5] * 0.0f + w[6] * 0.0f + w[7];
+    float oz = w[8] * 0.0f + w[9] * 0.0f + w[10] * 0.0f + w[11];
+    float ow = w[12] * 0.0f + w[13] * 0.0f + w[14] * 0.0f + w[15];
+    float oinv = 1.0f / ow;
+    F3 org = f3(oinv * ox, oinv * oy, oinv * oz);
+    F3 wd = f3(w[0] * dir.x + w[1] * dir.y + w[2] * dir.z, w[4] * dir.x + w[5] * dir.y + w[6] * dir.z,
+               w[8] * dir.x + w[9] * dir.y + w[10] * dir.z);
+    float ls = (wd.x * wd.x) + (wd.y * wd.y) + (wd.z * wd.z);
+    if (ls > 0) {
+        float dt = ((std::fabs(wd.x) * oerr.x) + (std::fabs(wd.y) * oerr.y) + (std::fabs(wd.z) * oerr.z)) / ls;
+        org = f3(org.x + dt * wd.x, org.y + dt * wd.y, org.z + dt * wd.z);
+    }
+    o[0] = org.x; o[1] = org.y; o[2] = org.z;
+    d[0] = wd.x; d[1] = wd.y; d[2] = wd.z;
+    return RT_OK;
+}
+
+rt_status rt_render(rt_scene *s, const rt_render_opts *opts, double *accum, uint32_t *nvalid, uint32_t flags,
+                    void *stream_)
+{
+    if (!s || !opts || !accum || !nvalid) { set_error("rt_render: null argument"); return RT_ERR_INVALID; }
+    if (!s->committed) { set_error("rt_render: scene not committed"); return RT_ERR_INVALID; }
+    if (flags & RT_FLAG_HOST_PTRS) { set_error("rt_render: film buffers must be device pointers"); return RT_ERR_UNSUPPORTED; }
+    if (opts->max_depth == 0 || opts->max_depth > 0xffffu) { set_error("rt_render: max_depth must be 1..65535"); return RT_ERR_INVALID; }
+    std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(s->ctx));
+    RT_HIP(hipSetDevice(rt::ctx_device(s->ctx)));
+    hipStream_t stream = stream_ ? (hipStream_t)stream_ : (hipStream_t)rt::ctx_stream_ptr(s->ctx);
+    const uint32_t npix = s->cam.width * s->cam.height;
+    const uint64_t total = (uint64_t)npix * opts->spp;
+    uint32_t P = opts->path_capacity ? opts->path_capacity : (1u << 21);
+    if (P > (1u << 26)) { set_error("rt_render: path_capacity above 2^26"); return RT_ERR_INVALID; }
+    if ((uint64_t)P > total) P = (uint32_t)std::max<uint64_t>(1, total);
+    rt_status st = rt::ensure_buffers(s, P);
+    if (st != RT_OK) return st;
+    auto t0 = std::chrono::steady_clock::now();
+    RT_HIP(hipMemsetAsync(s->queues.counters, 0, rt::kNumCounters * 4, stream));
+    RT_HIP(hipMemsetAsync(s->queues.stats, 0, 16, stream));
+    rt::FilmDev film{accum, nvalid};
+    for (uint64_t first = 0; first < total; first += P) {
+        rt::WaveParams wp{};
+        wp.first_path = first;
+        wp.count = (uint32_t)std::min<uint64_t>(P, total - first);
+        wp.npix = npix;
+        wp.spp_offset = opts->spp_offset;
+        wp.seed = opts->seed;
+        wp.max_depth = opts->max_depth;
+        wp.rr_depth = opts->rr_depth;
+        wp.stream_base = s->stream_base;
+        RT_HIP(rt::render_wave(s->dev, s->paths, s->shadow, s->queues, film, wp, P, stream));
+    }
+    unsigned long long counts[2];
+    RT_HIP(hipMemcpyAsync(counts, s->queues.stats, sizeof counts, hipMemcpyDeviceToHost, stream));
+    RT_HIP(hipStreamSynchronize(stream));
+    s->stats.paths = total;
+    s->stats.extension_rays = counts[0];
+    s->stats.shadow_rays = counts[1];
+    s->stats.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return RT_OK;
+}
+
+rt_status rt_render_get_stats(const rt_scene *s, rt_render_stats *st)
+{
+    if (!s || !st) { set_error("rt_render_get_stats: null argument"); return RT_ERR_INVALID; }
+    *st = s->stats;
+    return RT_OK;
+}
+
+}  // extern "C"

@@ -53,7 +53,105 @@ _SIGS = {
     "orc_sphere_intersects": (C.c_int, [C.POINTER(OrcSphere), C.POINTER(OrcRay), C.POINTER(_F)]),
     "orc_sphere_intersect": (C.c_int, [C.POINTER(OrcSphere), C.POINTER(OrcRay), C.POINTER(_F), _P, _P, _P]),
     "orc_mat4_inverse": (None, [_P, _P]),
+    "orc_tri_barycentrics": (None, [C.POINTER(OrcRay), _P, _P, _P, _P]),
+    "orc_camera_ray": (None, [_P, _P, _U32, _U32, _U32, _U32, _F, _F, _P, _P]),
+    "orc_fresnel_dielectric": (_F, [_F, _F, _F]),
+    "orc_path_initstate": (C.c_uint64, [_U32, _U32]),
+    "orc_scene_create": (_P, [_P, _P, _U32, _P, _P, _P, _P, _U32, _P, _U32, _P, _U32, _P]),
+    "orc_scene_free": (None, [_P]),
+    "orc_scene_camera": (None, [_P, _P, _P]),
+    "orc_path": (C.c_int, [_P, _U32, _U32, _U32, _U32, _U32, _U32, C.c_uint64, _P]),
+    "orc_render": (None, [_P, _U32, _U32, _U32, _U32, _U32, C.c_uint64, _U32, _U32, _U32, _U32, _P, _P, C.c_int]),
+    "orc_compute_rmse": (_F, [_P, C.c_double, _P, C.c_double, _U32]),
 }
+
+
+class OrcMaterial(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("kd", _F * 3), ("ior", _F), ("roughness", _F), ("eta", _F * 3),
+                ("k", _F * 3), ("intensity", _F)]
+
+
+class OrcPointLight(C.Structure):
+    _fields_ = [("pos", _F * 3), ("color", _F * 3), ("intensity", _F)]
+
+
+class OrcCamera(C.Structure):
+    _fields_ = [("pos", _F * 3), ("lookat", _F * 3), ("up", _F * 3), ("cts", _F * 3), ("frame_y", _F),
+                ("width", _U32), ("height", _U32)]
+
+
+def _f3(v):
+    return (_F * 3)(*[float(x) for x in v])
+
+
+class OrcScene:
+    """The oracle's copy of a scene description (same arrays the library
+    gets); renders with the restated estimator on the host."""
+
+    def __init__(self, desc):
+        self.desc = desc
+        self.xyz = f32(desc.xyz).reshape(-1, 3)
+        self.idx = np.ascontiguousarray(desc.idx, np.uint32)
+        self.tri_mat = np.ascontiguousarray(desc.tri_mat, np.uint32)
+        sph = np.array([[*c, r] for c, r, m, l in desc.spheres], np.float32).reshape(-1, 4)
+        self.sph = np.ascontiguousarray(sph)
+        self.sph_mat = np.array([m for c, r, m, l in desc.spheres], np.uint32)
+        self.sph_light = np.array([l for c, r, m, l in desc.spheres], np.uint8)
+        mats = (OrcMaterial * max(1, len(desc.materials)))()
+        for i, m in enumerate(desc.materials):
+            mats[i] = OrcMaterial(m["type"], _f3(m.get("kd", (0, 0, 0))), m.get("ior", 1.5), m.get("roughness", 0.0),
+                                  _f3(m.get("eta", (1, 1, 1))), _f3(m.get("k", (0, 0, 0))), m.get("intensity", 0.0))
+        self.mats = mats
+        lights = (OrcPointLight * max(1, len(desc.lights)))()
+        for i, (p, c, inten) in enumerate(desc.lights):
+            lights[i] = OrcPointLight(_f3(p), _f3(c), inten)
+        self.lights = lights
+        c = desc.camera
+        self.cam = OrcCamera(_f3(c["pos"]), _f3(c["lookat"]), _f3(c["up"]), _f3(c["cts"]), c["frame_y"],
+                             c["width"], c["height"])
+        self.h = lib().orc_scene_create(_p(self.xyz), _p(self.idx), self.idx.shape[0], _p(self.tri_mat),
+                                        _p(self.sph), _p(self.sph_mat), _p(self.sph_light), self.sph.shape[0],
+                                        C.cast(self.mats, _P), len(desc.materials), C.cast(self.lights, _P),
+                                        len(desc.lights), C.byref(self.cam))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_free(self.h)
+            self.h = None
+
+    def camera(self):
+        r2c = np.zeros(16, np.float32)
+        c2w = np.zeros(16, np.float32)
+        lib().orc_scene_camera(self.h, _p(r2c), _p(c2w))
+        return r2c, c2w
+
+    def camera_ray(self, px, py, u0, u1):
+        r2c, c2w = self.camera()
+        o = np.zeros(3, np.float32)
+        d = np.zeros(3, np.float32)
+        lib().orc_camera_ray(_p(r2c), _p(c2w), self.cam.width, self.cam.height, px, py, u0, u1, _p(o), _p(d))
+        return o, d
+
+    def path(self, px, py, sample, seed=None, max_depth=None, rr_depth=None, stream_base=0):
+        d = self.desc
+        out = np.zeros(3, np.float32)
+        ok = lib().orc_path(self.h, px, py, sample, d.seed if seed is None else seed, max_depth or d.max_depth,
+                            d.rr_depth if rr_depth is None else rr_depth, stream_base, _p(out))
+        return bool(ok), out
+
+    def render(self, spp=None, spp_offset=0, window=None, seed=None, nthreads=8, stream_base=0,
+               max_depth=None):
+        """Returns (accum (H*W*3 float64), nvalid (H*W uint32)) for the window
+        (x0, y0, x1, y1) (default: whole image)."""
+        d = self.desc
+        W, H = d.width, d.height
+        x0, y0, x1, y1 = window or (0, 0, W, H)
+        accum = np.zeros(W * H * 3, np.float64)
+        nvalid = np.zeros(W * H, np.uint32)
+        lib().orc_render(self.h, spp if spp is not None else d.spp, d.seed if seed is None else seed,
+                         max_depth or d.max_depth, d.rr_depth, spp_offset, stream_base, x0, y0, x1, y1,
+                         _p(accum), _p(nvalid), nthreads)
+        return accum, nvalid
 
 _lib = None
 

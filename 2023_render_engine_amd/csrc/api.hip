@@ -45,6 +45,12 @@ struct rt_bvh {
     rt::HostBvh host;  // kept for export and info; tris carry transparency
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
+    // ray-fetch counters of the persistent kernels: a ring, so launches that
+    // are in flight together on different streams do not share one
+    uint32_t *ray_counters = nullptr;
+    uint32_t seq = 0;
+    uint32_t *next_counter() { return ray_counters + (seq++ % kRayCounterRing); }
+    static constexpr uint32_t kRayCounterRing = 256;
 };
 
 namespace rt {
@@ -169,6 +175,7 @@ static rt_status upload_bvh(rt_bvh *b)
     const size_t tb = b->host.tris.size() * sizeof(rt::Tri3);
     RT_HIP(hipMalloc(&b->dev.nodes, nb));
     RT_HIP(hipMalloc(&b->dev.tris, tb));
+    RT_HIP(hipMalloc(&b->ray_counters, rt_bvh::kRayCounterRing * sizeof(uint32_t)));
     RT_HIP(hipMemcpyAsync(b->dev.nodes, b->host.nodes.data(), nb, hipMemcpyHostToDevice, b->ctx->stream));
     RT_HIP(hipMemcpyAsync(b->dev.tris, b->host.tris.data(), tb, hipMemcpyHostToDevice, b->ctx->stream));
     RT_HIP(hipStreamSynchronize(b->ctx->stream));
@@ -235,6 +242,7 @@ rt_status rt_bvh_destroy(rt_bvh *b)
         (void)hipStreamSynchronize(ctx->stream);
         if (b->dev.nodes) (void)hipFree(b->dev.nodes);
         if (b->dev.tris) (void)hipFree(b->dev.tris);
+        if (b->ray_counters) (void)hipFree(b->ray_counters);
         if (b->ev0) (void)hipEventDestroy(b->ev0);
         if (b->ev1) (void)hipEventDestroy(b->ev1);
         delete b;
@@ -312,8 +320,10 @@ static rt_status trace_common(rt_bvh *b, const rt_rays *rays, uint32_t n, int32_
 
     if (!(flags & RT_FLAG_HOST_PTRS)) {
         rt::RaysDev rd{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmax};
+        uint32_t *rc = b->next_counter();
+        RT_HIP(hipMemsetAsync(rc, 0, sizeof(uint32_t), stream));
         RT_HIP(hipEventRecord(b->ev0, stream));
-        RT_HIP(rt::launch_trace(b->dev, rd, n, prim, t, occ, any, stream));
+        RT_HIP(rt::launch_trace(b->dev, rd, n, prim, t, occ, any, stream, nullptr, rc));
         RT_HIP(hipEventRecord(b->ev1, stream));
         b->timed = true;
         return RT_OK;
@@ -338,8 +348,10 @@ static rt_status trace_common(rt_bvh *b, const rt_rays *rays, uint32_t n, int32_
     int32_t *dprim = any ? nullptr : (int32_t *)out;
     float *dt = (!any && t) ? (float *)(out + (size_t)n * sizeof(int32_t)) : nullptr;
     uint8_t *docc = any ? (uint8_t *)out : nullptr;
+    uint32_t *rc = b->next_counter();
+    hipError_t e = hipMemsetAsync(rc, 0, sizeof(uint32_t), stream);
     (void)hipEventRecord(b->ev0, stream);
-    hipError_t e = rt::launch_trace(b->dev, rd, n, dprim, dt, docc, any, stream);
+    if (e == hipSuccess) e = rt::launch_trace(b->dev, rd, n, dprim, dt, docc, any, stream, nullptr, rc);
     (void)hipEventRecord(b->ev1, stream);
     b->timed = true;
     if (e == hipSuccess) {
@@ -367,13 +379,14 @@ rt_status rt_trace_stats(rt_bvh *b, const rt_rays *rays, uint32_t n, int any, ui
     hipStream_t stream = b->ctx->stream;
     char *buf = nullptr;
     const size_t outb = any ? (size_t)n : (size_t)n * 8;
-    RT_HIP(hipMalloc(&buf, 16 + outb));
-    hipError_t e = hipMemsetAsync(buf, 0, 16, stream);
+    RT_HIP(hipMalloc(&buf, 32 + outb));
+    hipError_t e = hipMemsetAsync(buf, 0, 32, stream);
     rt::RaysDev rd{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmax};
-    char *out = buf + 16;
+    char *out = buf + 32;
     if (e == hipSuccess)
         e = rt::launch_trace(b->dev, rd, n, any ? nullptr : (int32_t *)out, any ? nullptr : (float *)(out + 4 * (size_t)n),
-                             any ? (uint8_t *)out : nullptr, any != 0, stream, (unsigned long long *)buf);
+                             any ? (uint8_t *)out : nullptr, any != 0, stream, (unsigned long long *)buf,
+                             (uint32_t *)(buf + 16));
     unsigned long long c[2] = {0, 0};
     if (e == hipSuccess) e = hipMemcpyAsync(c, buf, 16, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);

@@ -529,26 +529,45 @@ __global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc, PathsDev p, Sha
 }
 
 // ---------------------------------------------------------------------
-// finalize: FP64 film accumulation with the reference's NaN/Inf sentinel
-// bookkeeping (pathTracing.cu:474-486): only finite samples are summed and
-// counted; the film consumer rescales by samples / nvalid.
+// finalize: terminated paths take their last next-event contributions; the
+// radiance stays in the path slot until the wave's film pass.
 // ---------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_finalize(SceneDev sc, PathsDev p, QueuesDev q, FilmDev film,
-                                                     uint32_t which, uint32_t cur)
+__global__ __launch_bounds__(kBlock) void k_finalize(SceneDev sc, PathsDev p, QueuesDev q, uint32_t which,
+                                                     uint32_t cur)
 {
     const uint32_t n = which ? q.counters[kCActive0 + cur] : q.counters[kCTerm];
     const uint32_t *list = which ? q.active[cur] : q.term;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const uint32_t slot = list[i];
-        resolve_nee(sc, p, slot);
-        const float lx = p.lx[slot], ly = p.ly[slot], lz = p.lz[slot];
-        if (isfinite(lx) && isfinite(ly) && isfinite(lz)) {
-            const uint32_t px = p.pix[slot];
-            unsafeAtomicAdd(&film.accum[3 * (size_t)px], (double)lx);
-            unsafeAtomicAdd(&film.accum[3 * (size_t)px + 1], (double)ly);
-            unsafeAtomicAdd(&film.accum[3 * (size_t)px + 2], (double)lz);
-            atomicAdd(&film.nvalid[px], 1u);
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        resolve_nee(sc, p, list[i]);
+}
+
+// film: FP64 accumulation with the reference's NaN/Inf sentinel bookkeeping
+// (pathTracing.cu:474-486): only finite samples are summed and counted (the
+// consumer rescales by samples / nvalid).  One thread per pixel sums that
+// pixel's samples of the wave in sample order, so the result is
+// bit-reproducible and needs no atomics.
+__global__ __launch_bounds__(kBlock) void k_film(PathsDev p, FilmDev film, WaveParams wp)
+{
+    const uint32_t first_pix = (uint32_t)(wp.first_path % wp.npix);
+    const uint32_t npx = wp.count < wp.npix ? wp.count : wp.npix;
+    for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < npx; t += gridDim.x * kBlock) {
+        double sx = 0, sy = 0, sz = 0;
+        uint32_t nv = 0;
+        for (uint32_t slot = t; slot < wp.count; slot += wp.npix) {
+            const float lx = p.lx[slot], ly = p.ly[slot], lz = p.lz[slot];
+            if (isfinite(lx) && isfinite(ly) && isfinite(lz)) {
+                sx += (double)lx;
+                sy += (double)ly;
+                sz += (double)lz;
+                ++nv;
+            }
         }
+        uint32_t px = first_pix + t;
+        if (px >= wp.npix) px -= wp.npix;
+        film.accum[3 * (size_t)px] += sx;
+        film.accum[3 * (size_t)px + 1] += sy;
+        film.accum[3 * (size_t)px + 2] += sz;
+        film.nvalid[px] += nv;
     }
 }
 
@@ -582,12 +601,13 @@ hipError_t render_wave(const SceneDev &sc, const PathsDev &paths, const ShadowDe
         hipLaunchKernelGGL((k_shade<kQMiss>), dim3(kGrid), blk, 0, stream, sc, paths, shadow, q, wp, cur);
         hipLaunchKernelGGL(k_count, dim3(1), blk, 0, stream, q, cur);
         hipLaunchKernelGGL(k_connect, dim3(kGrid), blk, lds, stream, sc, paths, shadow, q, cap);
-        hipLaunchKernelGGL(k_finalize, dim3(kGrid), blk, 0, stream, sc, paths, q, film, 0u, cur);
+        hipLaunchKernelGGL(k_finalize, dim3(kGrid), blk, 0, stream, sc, paths, q, 0u, cur);
         cur ^= 1u;
     }
     // paths still alive after max_depth bounces cannot exist (shade stops
     // them at max_depth), but the list is flushed for safety
-    hipLaunchKernelGGL(k_finalize, dim3(kGrid), blk, 0, stream, sc, paths, q, film, 1u, cur);
+    hipLaunchKernelGGL(k_finalize, dim3(kGrid), blk, 0, stream, sc, paths, q, 1u, cur);
+    hipLaunchKernelGGL(k_film, dim3(kGrid), blk, 0, stream, paths, film, wp);
     return hipGetLastError();
 }
 

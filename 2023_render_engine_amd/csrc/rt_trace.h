@@ -18,10 +18,12 @@ struct DevBvh {
 };
 
 // counters != nullptr selects the counting variant (2 x u64: node visits,
-// triangle tests, accumulated).
+// triangle tests, accumulated).  ray_counter: one device u32, zero at launch,
+// that the persistent kernel fetches rays from; two launches in flight at
+// once need different counters.
 hipError_t launch_trace(const DevBvh &bvh, const RaysDev &rays, uint32_t n, int32_t *prim, float *t,
-                        uint8_t *occluded, bool any, hipStream_t stream,
-                        unsigned long long *counters = nullptr);
+                        uint8_t *occluded, bool any, hipStream_t stream, unsigned long long *counters,
+                        uint32_t *ray_counter);
 size_t trace_lds_bytes(uint32_t max_depth);
 
 }  // namespace rt

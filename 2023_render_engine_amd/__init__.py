@@ -74,6 +74,19 @@ def f3(v):
     return (C.c_float * 3)(*[float(x) for x in v])
 
 
+class CacheOpts(C.Structure):
+    _fields_ = [("levels", C.c_uint32), ("log2_table", C.c_uint32), ("base_res", C.c_uint32),
+                ("per_level_scale", C.c_float), ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
+                ("eps", C.c_float), ("batch", C.c_uint32), ("seed", C.c_uint32), ("aabb_lo", C.c_float * 3),
+                ("aabb_hi", C.c_float * 3)]
+
+
+class CacheInfo(C.Structure):
+    _fields_ = [("levels", C.c_uint32), ("table_size", C.c_uint32), ("res", C.c_uint32 * 16),
+                ("dense", C.c_uint32 * 16), ("mlp_params", C.c_uint64), ("grid_params", C.c_uint64),
+                ("steps", C.c_uint32)]
+
+
 _P = C.c_void_p
 _U32 = C.c_uint32
 _SIGS = {
@@ -108,6 +121,14 @@ _SIGS = {
     "rt_scene_camera_ray": (C.c_int, [_P, _U32, _U32, C.c_float, C.c_float, _P, _P]),
     "rt_render": (C.c_int, [_P, C.POINTER(RenderOpts), _P, _P, _U32, _P]),
     "rt_render_get_stats": (C.c_int, [_P, C.POINTER(RenderStats)]),
+    "rt_cache_create": (C.c_int, [_P, C.POINTER(CacheOpts), C.POINTER(_P)]),
+    "rt_cache_destroy": (C.c_int, [_P]),
+    "rt_cache_get_info": (C.c_int, [_P, C.POINTER(CacheInfo)]),
+    "rt_cache_query": (C.c_int, [_P, _P, _U32, _P, _P]),
+    "rt_cache_train_step": (C.c_int, [_P, _P, _P, _U32, C.POINTER(C.c_float), _P]),
+    "rt_cache_gradients": (C.c_int, [_P, _P, _P, _U32, _P, _P, C.POINTER(C.c_float)]),
+    "rt_cache_get_params": (C.c_int, [_P, _P, _P]),
+    "rt_cache_set_params": (C.c_int, [_P, _P, _P]),
     "rt_gen_soup": (C.c_int, [_U32, C.c_uint64, C.c_float, C.c_float, _P]),
     "rt_gen_rays": (C.c_int, [_U32, _U32, _P, C.c_float, _U32, C.c_uint64, _P, _P, _P, _P, _P, _P]),
 }
@@ -394,3 +415,73 @@ def compute_rmse(img_a, img_b):
     import numpy as np
     d = np.asarray(img_a, np.float64).reshape(-1, 3) - np.asarray(img_b, np.float64).reshape(-1, 3)
     return float(np.sqrt(np.mean(np.sum(d * d, axis=1))))
+
+
+class RadianceCache:
+    """rt_cache: hash-grid + MLP radiance cache trained on the GPU."""
+
+    def __init__(self, ctx: Context, aabb_lo, aabb_hi, levels=16, log2_table=19, base_res=16, per_level_scale=1.38,
+                 lr=1e-2, beta1=0.9, beta2=0.99, eps=1e-8, batch=65536, seed=1):
+        self.ctx = ctx
+        o = CacheOpts(levels, log2_table, base_res, per_level_scale, lr, beta1, beta2, eps, batch, seed,
+                      f3(aabb_lo), f3(aabb_hi))
+        h = _P()
+        check(lib().rt_cache_create(ctx.h, C.byref(o), C.byref(h)))
+        self.h = h
+        self.opts = o
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rt_cache_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> CacheInfo:
+        i = CacheInfo()
+        check(lib().rt_cache_get_info(self.h, C.byref(i)))
+        return i
+
+    def query(self, inputs, rgb=None, stream=None):
+        import torch
+        n = int(inputs.shape[0])
+        rgb = torch.empty((n, 3), dtype=torch.float32, device=inputs.device) if rgb is None else rgb
+        s = stream if stream is not None else torch.cuda.current_stream(inputs.device).cuda_stream
+        check(lib().rt_cache_query(self.h, _ptr(inputs), n, _ptr(rgb), s))
+        return rgb
+
+    def train_step(self, inputs, targets, want_loss=True, stream=None):
+        import torch
+        loss = C.c_float(0.0)
+        s = stream if stream is not None else torch.cuda.current_stream(inputs.device).cuda_stream
+        check(lib().rt_cache_train_step(self.h, _ptr(inputs), _ptr(targets), int(inputs.shape[0]),
+                                        C.byref(loss) if want_loss else None, s))
+        return loss.value if want_loss else None
+
+    def gradients(self, inputs, targets):
+        import numpy as np
+        inf = self.info()
+        mg = np.zeros(inf.mlp_params, np.float32)
+        gg = np.zeros(inf.grid_params, np.float32)
+        loss = C.c_float(0.0)
+        check(lib().rt_cache_gradients(self.h, _ptr(inputs), _ptr(targets), int(inputs.shape[0]), _ptr(mg),
+                                       _ptr(gg), C.byref(loss)))
+        return loss.value, mg, gg
+
+    def get_params(self):
+        import numpy as np
+        inf = self.info()
+        mg = np.zeros(inf.mlp_params, np.float32)
+        gg = np.zeros(inf.grid_params, np.float32)
+        check(lib().rt_cache_get_params(self.h, _ptr(mg), _ptr(gg)))
+        return mg, gg
+
+    def set_params(self, mlp, grid):
+        import numpy as np
+        mlp = np.ascontiguousarray(mlp, np.float32)
+        grid = np.ascontiguousarray(grid, np.float32)
+        check(lib().rt_cache_set_params(self.h, _ptr(mlp), _ptr(grid)))

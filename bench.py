@@ -126,8 +126,8 @@ def run_c1(args):
 
     # coherent / incoherent halves separately (same kernel, one launch each)
     half = 4096 * 2048
-    split = {}
-    for name, sl in (("coherent", slice(0, half)), ("incoherent", slice(half, n))):
+    split = {"coherent": None, "incoherent": None}
+    for name, sl in (() if args.no_split else (("coherent", slice(0, half)), ("incoherent", slice(half, n)))):
         sub = [r[sl] for r in rays]
         tm = tmax[sl]
         bvh.trace_closest(*sub, tmax=tm, prim=prim[sl], t=t[sl], stream=s)
@@ -188,13 +188,88 @@ def run_c1(args):
     return out
 
 
+def run_render(args, which):
+    """Path-tracing workloads: c0 = Cornell box 256^2 16 spp depth 8;
+    c2 = procedural 3.4M-triangle scene 1920x1080 64 spp depth 16.  One step
+    = one full frame (all samples of all pixels); value = samples/s."""
+    import numpy as np
+    import torch
+
+    rt = importlib.import_module("2023_render_engine_amd")
+    scenes = importlib.import_module("2023_render_engine_amd.scenes")
+    if which == "c0":
+        desc = scenes.cornell()
+        label = "Cornell box (10 triangles + 2 spheres, 1 point light) 256x256, 16 spp, depth 8, seed 1"
+    else:
+        log("generating the procedural scene")
+        desc = scenes.procedural()
+        label = ("procedural: 1024^2 heightfield + 64 icospheres (5 subdivisions) = "
+                 f"{desc.idx.shape[0]} triangles, 8 point lights + 1 sphere light, 1920x1080, 64 spp, depth 16")
+    ctx = rt.Context(0)
+    t0 = time.time()
+    sc = rt.Scene(ctx, desc)
+    log(f"scene committed in {time.time() - t0:.1f} s")
+    film = sc.new_film()
+    for _ in range(args.warmup):
+        sc.render(film)
+    torch.cuda.synchronize()
+    ext = shadow = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sc.render(film)
+        st = sc.stats()
+        ext += st.extension_rays
+        shadow += st.shadow_rays
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    samples = desc.width * desc.height * desc.spp * args.steps
+    out = {
+        "metric": METRIC,
+        "value": round(samples / wall / 1e6, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (FP64 film)", "data": "synthetic",
+        "config": {"workload": label, "triangles": int(desc.idx.shape[0])},
+        "mrays_total": round((ext + shadow) / wall / 1e6, 1),
+        "mrays_extension": round(ext / wall / 1e6, 1),
+        "mrays_shadow": round(shadow / wall / 1e6, 1),
+        "rays_per_sample": round((ext + shadow) / samples, 2),
+    }
+    if not args.no_cpu_baseline:
+        import oracle
+        os_ = oracle.OrcScene(desc)
+        W, H = desc.width, desc.height
+        # bounded sample: a centred window, growing until the budget is used
+        spent, done = 0.0, 0
+        w = 32
+        while spent < args.cpu_budget and w <= min(W, H):
+            x0, y0 = (W - w) // 2, (H - w) // 2
+            t1 = time.time()
+            os_.render(window=(x0, y0, x0 + w, y0 + w), nthreads=cpu_threads())
+            spent += time.time() - t1
+            done += w * w * desc.spp
+            w *= 2
+        out["cpu_baseline"] = {
+            "value": round(done / spent / 1e6, 4), "unit": "Msamples/s", "cores": cpu_threads(), "kind": "port",
+            "sample": f"{done} samples (centred square windows of the same frame, all {desc.spp} spp) with the "
+                      f"oracle's restatement of the reference estimator, {spent:.1f} s",
+        }
+    sc.close()
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c1", choices=["c1"])
+    ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c2"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-split", action="store_true",
+                    help="c1: skip the separate coherent/incoherent launches (profiling runs)")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -205,7 +280,10 @@ def main():
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
-    out = run_c1(args)
+    if args.workload == "c1":
+        out = run_c1(args)
+    else:
+        out = run_render(args, args.workload)
     print(json.dumps(out), flush=True)
 
 

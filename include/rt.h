@@ -234,6 +234,57 @@ typedef struct {
 rt_status rt_render_get_stats(const rt_scene *scene, rt_render_stats *stats);
 
 /* ------------------------------------------------------------------ */
+/* Neural radiance cache                                                */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference (whose only learned component is the
+ * per-shape polynomial regression of sampling/ShapeLightDistribution.cu:
+ * 20-345, run in its MACHINE_LEARNING mode).  A multiresolution hash-grid
+ * encoding of position plus normal, view direction and roughness feeds a
+ * 4x64 ReLU MLP predicting outgoing RGB radiance; BF16 matrix-core matmuls
+ * with FP32 accumulation, FP32 master weights and Adam. */
+typedef struct {
+    uint32_t levels;          /* hash-grid levels, 1..16 */
+    uint32_t log2_table;      /* entries per level = 2^log2_table */
+    uint32_t base_res;        /* coarsest level resolution */
+    float per_level_scale;    /* resolution growth per level */
+    float lr, beta1, beta2, eps;  /* Adam */
+    uint32_t batch;           /* largest training batch (and query chunk) */
+    uint32_t seed;            /* parameter initialisation */
+    float aabb_lo[3], aabb_hi[3];  /* positions are normalised to this box */
+} rt_cache_opts;
+
+typedef struct {
+    uint32_t levels, table_size;
+    uint32_t res[16], dense[16];
+    uint64_t mlp_params, grid_params;
+    uint32_t steps;
+} rt_cache_info;
+
+typedef struct rt_cache rt_cache;
+
+rt_status rt_cache_create(rt_ctx *ctx, const rt_cache_opts *opts, rt_cache **out);
+rt_status rt_cache_destroy(rt_cache *cache);
+rt_status rt_cache_get_info(const rt_cache *cache, rt_cache_info *info);
+/* Queries: n records of 12 floats (position 3, normal 3, view direction 3,
+ * roughness 1, 2 unused), device pointers; rgb: n x 3 floats (>= 0). */
+rt_status rt_cache_query(rt_cache *cache, const float *in, uint32_t n, float *rgb, void *stream);
+/* One Adam step on a batch (n <= opts.batch) of records and RGB targets
+ * (device).  loss (host, may be NULL) receives the batch's mean relative L2
+ * loss and makes the call blocking. */
+rt_status rt_cache_train_step(rt_cache *cache, const float *in, const float *target, uint32_t n, float *loss,
+                              void *stream);
+/* Gradients of the batch's mean loss without an update (host outputs of
+ * mlp_params and grid_params floats); for checking against a CPU model. */
+rt_status rt_cache_gradients(rt_cache *cache, const float *in, const float *target, uint32_t n,
+                             float *mlp_grad, float *grid_grad, float *loss);
+/* FP32 parameters (host arrays of mlp_params / grid_params floats).  MLP
+ * layout: 5 matrices W_l[in][out] row-major at offsets 0, 4096, 8192,
+ * 12288, 16384; the last is 64 x 16 with columns 3..15 unused.  Grid:
+ * [levels][table][2]. */
+rt_status rt_cache_get_params(const rt_cache *cache, float *mlp, float *grid);
+rt_status rt_cache_set_params(rt_cache *cache, const float *mlp, const float *grid);
+
+/* ------------------------------------------------------------------ */
 /* Synthetic workloads (host generators, deterministic)                 */
 /* ------------------------------------------------------------------ */
 /* Random triangle soup in the unit cube (centroids uniform, edge lengths

@@ -63,7 +63,48 @@ _SIGS = {
     "orc_path": (C.c_int, [_P, _U32, _U32, _U32, _U32, _U32, _U32, C.c_uint64, _P]),
     "orc_render": (None, [_P, _U32, _U32, _U32, _U32, _U32, C.c_uint64, _U32, _U32, _U32, _U32, _P, _P, C.c_int]),
     "orc_compute_rmse": (_F, [_P, C.c_double, _P, C.c_double, _U32]),
+    "orc_cache_loss_grad": (C.c_double, [_P, _P, _P, _P, _P, _U32, C.c_int, _P, _P, _P, _P]),
 }
+
+
+class OrcCacheCfg(C.Structure):
+    _fields_ = [("levels", _U32), ("table_size", _U32), ("res", _U32 * 16), ("dense", _U32 * 16),
+                ("lo", _F * 3), ("inv_ext", _F * 3)]
+
+
+CACHE_PARAMS = 17408
+
+
+def cache_cfg(levels, log2_table, base_res, scale, lo, hi):
+    """Level table exactly as rt_cache_create derives it."""
+    import math
+    c = OrcCacheCfg()
+    c.levels = levels
+    c.table_size = 1 << log2_table
+    for l in range(levels):
+        r = math.floor(base_res * math.pow(scale, l))
+        c.res[l] = r
+        c.dense[l] = 1 if (r + 1.0) ** 3 <= c.table_size else 0
+    for k in range(3):
+        c.lo[k] = lo[k]
+        c.inv_ext[k] = np.float32(1.0) / (np.float32(hi[k]) - np.float32(lo[k]))
+    return c
+
+
+def cache_loss_grad(cfg, mlp, grid, inputs, targets, emulate_bf16=True, want_grad=True, den_fixed=None):
+    """(loss, mlp_grad float64[17408], grid_grad float64[L*T*2], y float64[n,3])."""
+    mlp = f32(mlp)
+    grid = f32(grid)
+    inputs = f32(inputs).reshape(-1, 12)
+    targets = f32(targets).reshape(-1, 3)
+    n = inputs.shape[0]
+    mg = np.zeros(CACHE_PARAMS) if want_grad else None
+    gg = np.zeros(cfg.levels * cfg.table_size * 2) if want_grad else None
+    y = np.zeros((n, 3))
+    den = None if den_fixed is None else np.ascontiguousarray(den_fixed, np.float64)
+    loss = lib().orc_cache_loss_grad(C.byref(cfg), _p(mlp), _p(grid), _p(inputs), _p(targets), n, int(emulate_bf16),
+                                     _p(mg), _p(gg), _p(y), _p(den))
+    return loss, mg, gg, y
 
 
 class OrcMaterial(C.Structure):

@@ -121,6 +121,9 @@ _SIGS = {
     "rt_scene_camera_ray": (C.c_int, [_P, _U32, _U32, C.c_float, C.c_float, _P, _P]),
     "rt_render": (C.c_int, [_P, C.POINTER(RenderOpts), _P, _P, _U32, _P]),
     "rt_render_get_stats": (C.c_int, [_P, C.POINTER(RenderStats)]),
+    "rt_render_cached": (C.c_int, [_P, C.POINTER(RenderOpts), _P, _U32, _P, _P, _U32, _P]),
+    "rt_cache_train_scene": (C.c_int, [_P, _P, C.POINTER(RenderOpts), _U32, _U32, C.POINTER(C.c_float),
+                                       C.POINTER(_U32), _P]),
     "rt_cache_create": (C.c_int, [_P, C.POINTER(CacheOpts), C.POINTER(_P)]),
     "rt_cache_destroy": (C.c_int, [_P]),
     "rt_cache_get_info": (C.c_int, [_P, C.POINTER(CacheInfo)]),
@@ -381,6 +384,36 @@ class Scene:
         s = stream if stream is not None else torch.cuda.current_stream(accum.device).cuda_stream
         check(lib().rt_render(self.h, C.byref(o), _ptr(accum), _ptr(nvalid), 0, s))
         return film
+
+    def _opts(self, spp, spp_offset, max_depth, rr_depth, seed, path_capacity):
+        d = self.desc
+        return RenderOpts(spp if spp is not None else d.spp, spp_offset, max_depth or d.max_depth,
+                          d.rr_depth if rr_depth is None else rr_depth, d.seed if seed is None else seed,
+                          path_capacity)
+
+    def render_cached(self, film, cache, cache_depth=2, spp=None, spp_offset=0, max_depth=None, rr_depth=None,
+                      seed=None, path_capacity=0, stream=None):
+        """rt_render_cached: paths end at their first cacheable vertex at
+        depth >= cache_depth and take the radiance cache's answer."""
+        import torch
+        o = self._opts(spp, spp_offset, max_depth, rr_depth, seed, path_capacity)
+        accum, nvalid = film
+        s = stream if stream is not None else torch.cuda.current_stream(accum.device).cuda_stream
+        check(lib().rt_render_cached(self.h, C.byref(o), cache.h, cache_depth, _ptr(accum), _ptr(nvalid), 0, s))
+        return film
+
+    def train_cache(self, cache, frame, n_paths=65536, cache_depth=2, max_depth=None, rr_depth=None, seed=None,
+                    stream=None):
+        """rt_cache_train_scene: one online training step of `cache` on
+        n_paths training paths of frame `frame`; returns (loss, samples)."""
+        import torch
+        o = self._opts(1, frame, max_depth, rr_depth, seed, 0)
+        loss = C.c_float(0.0)
+        n = C.c_uint32(0)
+        s = stream if stream is not None else torch.cuda.current_stream(torch.device("cuda", self.ctx.device)).cuda_stream
+        check(lib().rt_cache_train_scene(cache.h, self.h, C.byref(o), cache_depth, n_paths, C.byref(loss),
+                                         C.byref(n), s))
+        return loss.value, n.value
 
     def stats(self) -> RenderStats:
         st = RenderStats()

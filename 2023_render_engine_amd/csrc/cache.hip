@@ -84,8 +84,9 @@ __device__ __forceinline__ void normalise_pos(const CacheDev &c, const float *q,
 // X[n][64] (BF16): 2 features per level, normal, view direction, roughness,
 // a constant 1 (the bias-free MLP's offset input) and zero padding.
 __global__ __launch_bounds__(256) void k_encode(CacheDev c, const float *__restrict__ in, uint32_t n,
-                                                uint16_t *__restrict__ X)
+                                                const uint32_t *__restrict__ n_dev, uint16_t *__restrict__ X)
 {
+    if (n_dev) n = min(n, *n_dev);
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         const float *q = in + (size_t)i * kCacheInStride;
         float px, py, pz;
@@ -123,8 +124,9 @@ __global__ __launch_bounds__(256) void k_encode(CacheDev c, const float *__restr
 // ---------------------------------------------------------------------
 template <int NOUT, int MODE>
 __global__ __launch_bounds__(256) void k_fwd(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wt,
-                                             uint32_t n, void *__restrict__ Y)
+                                             uint32_t n, const uint32_t *__restrict__ n_dev, void *__restrict__ Y)
 {
+    if (n_dev) n = min(n, *n_dev);
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x / 64;
     const uint32_t li = lane & 15, lq = lane >> 4;
@@ -360,8 +362,10 @@ __global__ __launch_bounds__(256) void k_fill(float *__restrict__ p, uint32_t co
 }
 
 // inference output: rgb = max(Y[:, 0..2], 0)
-__global__ __launch_bounds__(256) void k_out_rgb(const float *__restrict__ Y, uint32_t n, float *__restrict__ rgb)
+__global__ __launch_bounds__(256) void k_out_rgb(const float *__restrict__ Y, uint32_t n,
+                                                 const uint32_t *__restrict__ n_dev, float *__restrict__ rgb)
 {
+    if (n_dev) n = min(n, *n_dev);
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
 #pragma unroll
         for (int c = 0; c < 3; ++c) rgb[(size_t)i * 3 + c] = fmaxf(Y[(size_t)i * 16 + c], 0.0f);
@@ -378,24 +382,27 @@ static dim3 grid_for(uint32_t work, uint32_t per_block, uint32_t cap = 4096)
     return dim3(b);
 }
 
-hipError_t cache_forward(const CacheDev &c, const CacheWork &w, const float *in, uint32_t n, hipStream_t s)
+hipError_t cache_forward(const CacheDev &c, const CacheWork &w, const float *in, uint32_t n, hipStream_t s,
+                         const uint32_t *n_dev)
 {
     if (n == 0) return hipSuccess;
     const dim3 blk(256);
-    hipLaunchKernelGGL(k_encode, grid_for(n, 256), blk, 0, s, c, in, n, w.act[0]);
+    hipLaunchKernelGGL(k_encode, grid_for(n, 256), blk, 0, s, c, in, n, n_dev, w.act[0]);
     const dim3 g = grid_for((n + 15) / 16, 4);
     for (uint32_t l = 0; l + 1 < kCacheLayers; ++l)
-        hipLaunchKernelGGL((k_fwd<64, 0>), g, blk, 0, s, w.act[l], c.w_t + kCacheLayerOffset(l), n, (void *)w.act[l + 1]);
+        hipLaunchKernelGGL((k_fwd<64, 0>), g, blk, 0, s, w.act[l], c.w_t + kCacheLayerOffset(l), n, n_dev,
+                           (void *)w.act[l + 1]);
     hipLaunchKernelGGL((k_fwd<16, 1>), g, blk, 0, s, w.act[kCacheLayers - 1],
-                       c.w_t + kCacheLayerOffset(kCacheLayers - 1), n, (void *)w.y);
+                       c.w_t + kCacheLayerOffset(kCacheLayers - 1), n, n_dev, (void *)w.y);
     return hipGetLastError();
 }
 
-hipError_t cache_infer(const CacheDev &c, const CacheWork &w, const float *in, uint32_t n, float *rgb, hipStream_t s)
+hipError_t cache_infer(const CacheDev &c, const CacheWork &w, const float *in, uint32_t n, float *rgb, hipStream_t s,
+                       const uint32_t *n_dev)
 {
-    hipError_t e = cache_forward(c, w, in, n, s);
+    hipError_t e = cache_forward(c, w, in, n, s, n_dev);
     if (e != hipSuccess || n == 0) return e;
-    hipLaunchKernelGGL(k_out_rgb, grid_for(n, 256), dim3(256), 0, s, w.y, n, rgb);
+    hipLaunchKernelGGL(k_out_rgb, grid_for(n, 256), dim3(256), 0, s, w.y, n, n_dev, rgb);
     return hipGetLastError();
 }
 
@@ -404,7 +411,7 @@ hipError_t cache_backward(const CacheDev &c, const CacheWork &w, const float *in
 {
     if (n == 0) return hipSuccess;
     const dim3 blk(256);
-    hipError_t e = cache_forward(c, w, in, n, s);
+    hipError_t e = cache_forward(c, w, in, n, s, nullptr);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_loss, grid_for(n, 256), blk, 0, s, w.y, target, n, 1.0f / (3.0f * (float)n), w.dy16, loss);
     const dim3 g = grid_for((n + 15) / 16, 4);

@@ -42,6 +42,13 @@ struct rt_cache {
     uint32_t step = 0;
 };
 
+// accessors for the renderer (scene.hip)
+namespace rt {
+const CacheDev &cache_device(const rt_cache *c) { return c->dev; }
+rt_ctx *cache_ctx(const rt_cache *c) { return c->ctx; }
+uint32_t cache_batch(const rt_cache *c) { return c->work.rows; }
+}  // namespace rt
+
 namespace {
 
 uint16_t host_f2bf(float f)

@@ -28,6 +28,7 @@
 #include "rt_device.h"
 #include "rt_render.h"
 #include "rt_shade.h"
+#include "rt_cache.h"
 
 namespace rt {
 
@@ -59,9 +60,12 @@ __global__ __launch_bounds__(kBlock) void k_reset(uint32_t *counters, uint32_t c
 {
     const uint32_t t = threadIdx.x;
     if (t < kNumCounters) {
+        const bool wave_start = init_active != 0xffffffffu;
         if (t == kCActive0 + cur) {
-            if (init_active != 0xffffffffu) counters[t] = init_active;
+            if (wave_start) counters[t] = init_active;
         } else if (t == kCActive0 + (cur ^ 1u) || (t >= kCQueue && t <= kCTerm)) {
+            counters[t] = 0;
+        } else if (wave_start && (t == kCCacheQ || t == kCTrain)) {
             counters[t] = 0;
         }
     }
@@ -72,8 +76,15 @@ __global__ __launch_bounds__(kBlock) void k_generate(SceneDev sc, PathsDev p, Qu
 {
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < wp.count; i += gridDim.x * kBlock) {
         const uint64_t g = wp.first_path + i;
-        const uint32_t sample = wp.spp_offset + (uint32_t)(g / wp.npix);
-        const uint32_t pix = (uint32_t)(g % wp.npix);
+        uint32_t sample = wp.spp_offset + (uint32_t)(g / wp.npix);
+        uint32_t pix = (uint32_t)(g % wp.npix);
+        if (wp.train) {
+            // training paths: pseudo-random pixels, sample indices disjoint
+            // from the rendering samples (top bit set)
+            Pcg32 h = pcg32_seed(g, 0x7a11u);
+            pix = pcg32_next(h) % wp.npix;
+            sample = 0x80000000u | (uint32_t)g;
+        }
         const uint32_t px = pix % sc.cam.width, py = pix / sc.cam.width;
         const uint64_t initstate = ((uint64_t)wp.seed << 32) ^ (uint64_t)sample ^ 0x853c49e6748fea9bULL;
         Pcg32 r = pcg32_seed(initstate, wp.stream_base + pix);
@@ -90,6 +101,7 @@ __global__ __launch_bounds__(kBlock) void k_generate(SceneDev sc, PathsDev p, Qu
         p.ppdf[i] = 0;
         p.ppx[i] = o.x; p.ppy[i] = o.y; p.ppz[i] = o.z;
         p.nee_mask[i] = 0;
+        p.tvalid[i] = 0;
         q.active[0][i] = i;
     }
 }
@@ -97,10 +109,20 @@ __global__ __launch_bounds__(kBlock) void k_generate(SceneDev sc, PathsDev p, Qu
 // ---------------------------------------------------------------------
 // extend: closest hit + classification
 // ---------------------------------------------------------------------
-__device__ __forceinline__ uint32_t classify(const SceneDev &sc, int32_t hid)
+// A vertex the radiance cache may stand for: diffuse, or glossy with GGX
+// alpha >= 0.2 (roughness = alpha^2).  Specular and sharper glossy vertices
+// keep tracing, so the cache only ever answers smooth view dependence.
+__device__ __forceinline__ bool cacheable(const rt_material &m)
+{
+    return m.type == RT_MAT_LAMBERTIAN ||
+           ((m.type == RT_MAT_TROWBRIDGE || m.type == RT_MAT_TROWBRIDGE_BASIC) && m.roughness >= 0.04f);
+}
+
+__device__ __forceinline__ uint32_t classify(const SceneDev &sc, int32_t hid, bool cache_depth_reached)
 {
     if (hid == -1) return kQMiss;
     const uint32_t mi = hid >= 0 ? sc.tri_mat[hid] : sc.sph[-hid - 2].material;
+    if (cache_depth_reached && cacheable(sc.mat[mi])) return kQCache;
     const uint32_t t = sc.mat[mi].type;
     if (t == RT_MAT_LAMBERTIAN) return kQLambert;
     if (t == RT_MAT_DIELECTRIC) return kQDielectric;
@@ -108,7 +130,8 @@ __device__ __forceinline__ uint32_t classify(const SceneDev &sc, int32_t hid)
     return kQGgx;
 }
 
-__global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc, PathsDev p, QueuesDev q, uint32_t cur, int cap)
+__global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc, PathsDev p, QueuesDev q, uint32_t cur, int cap,
+                                                   uint32_t cache_depth)
 {
     extern __shared__ int32_t s_stack[];
     int32_t *stack = s_stack + threadIdx.x;
@@ -187,7 +210,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc, PathsDev p, Queu
             p.hid[slot] = hid;
             p.ht[slot] = th;
         }
-        const uint32_t cls = live ? classify(sc, hid) : 0xffu;
+        const bool deep = live && cache_depth != kNoCache && (p.meta[slot] & 0xffffu) >= cache_depth;
+        const uint32_t cls = live ? classify(sc, hid, deep) : 0xffu;
 #pragma unroll
         for (uint32_t k = 0; k < kNumQ; ++k) {
             const uint32_t pos = wave_append(&q.counters[kCQueue + k], cls == k);
@@ -277,6 +301,18 @@ __device__ __forceinline__ void resolve_nee(const SceneDev &sc, const PathsDev &
     p.nee_mask[slot] = 0;
 }
 
+// Radiance-cache query record (12 floats): position, the normal facing the
+// incoming ray, the direction back along it, a roughness feature (diffuse 1,
+// GGX alpha), padding.
+__device__ __forceinline__ void write_record(float *r, const Surface &s, V3 d, const rt_material &m)
+{
+    r[0] = s.p.x; r[1] = s.p.y; r[2] = s.p.z;
+    r[3] = s.n.x; r[4] = s.n.y; r[5] = s.n.z;
+    r[6] = -d.x; r[7] = -d.y; r[8] = -d.z;
+    r[9] = m.type == RT_MAT_LAMBERTIAN ? 1.0f : sqrtf(m.roughness);
+    r[10] = 0; r[11] = 0;
+}
+
 template <uint32_t CLS>
 __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathsDev p, ShadowDev sh, QueuesDev q,
                                                   WaveParams wp, uint32_t cur)
@@ -313,6 +349,15 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathsDev p, Shado
                 onb(s.n, t, b);
                 const V3 wow = neg(d);
                 wo = mk(dot(wow, t), dot(wow, b), dot(wow, s.n));
+                // training paths remember their first cacheable vertex at or
+                // beyond the cache depth: the radiance gathered from here on,
+                // divided by the throughput on arrival, is the cache's target
+                if (kScatter && wp.train && depth >= wp.cache_depth && p.tvalid[slot] == 0 && cacheable(m)) {
+                    write_record(p.tin + (size_t)slot * 12, s, d, m);
+                    p.tLb[3 * (size_t)slot] = L.x; p.tLb[3 * (size_t)slot + 1] = L.y; p.tLb[3 * (size_t)slot + 2] = L.z;
+                    p.tB[3 * (size_t)slot] = beta.x; p.tB[3 * (size_t)slot + 1] = beta.y; p.tB[3 * (size_t)slot + 2] = beta.z;
+                    p.tvalid[slot] = 1;
+                }
             }
         }
         if (kScatter) {
@@ -450,6 +495,34 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathsDev p, Shado
     }
 }
 
+// Cache vertices: the path ends here.  Its previous next-event estimate is
+// resolved and its query record is appended to the wave's query list; the
+// cache's answer times the throughput on arrival is added after the bounce
+// loop (k_cache_apply).
+__global__ __launch_bounds__(kBlock) void k_cache_vertex(SceneDev sc, PathsDev p, QueuesDev q)
+{
+    const uint32_t n = q.counters[kCQueue + kQCache];
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t i0 = blockIdx.x * kBlock; i0 < n; i0 += stride) {
+        const uint32_t i = i0 + threadIdx.x;
+        const bool live = i < n;
+        const uint32_t slot = live ? q.q[kQCache][i] : 0;
+        Surface s{};
+        V3 d = mk(0, 0, 1);
+        if (live) {
+            resolve_nee(sc, p, slot);
+            surface_of(sc, p, slot, s, d);
+        }
+        const uint32_t k = wave_append(&q.counters[kCCacheQ], live);
+        if (live) {
+            write_record(p.cin + (size_t)k * 12, s, d, sc.mat[s.mat]);
+            p.cslot[k] = slot;
+        }
+        const uint32_t tpos = wave_append(&q.counters[kCTerm], live);
+        if (live) q.term[tpos] = slot;
+    }
+}
+
 // ---------------------------------------------------------------------
 // connect: any hit for shadow rays (Scene.cu:108-150, 238-276)
 // ---------------------------------------------------------------------
@@ -541,6 +614,51 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SceneDev sc, PathsDev p, Qu
         resolve_nee(sc, p, list[i]);
 }
 
+// L += beta * cache(record) for every cache query of the wave
+__global__ __launch_bounds__(kBlock) void k_cache_apply(PathsDev p, QueuesDev q)
+{
+    const uint32_t n = q.counters[kCCacheQ];
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+        const uint32_t slot = p.cslot[k];
+        const float *c = p.crgb + 3 * (size_t)k;
+        p.lx[slot] = p.lx[slot] + p.bx[slot] * c[0];
+        p.ly[slot] = p.ly[slot] + p.by[slot] * c[1];
+        p.lz[slot] = p.lz[slot] + p.bz[slot] * c[2];
+    }
+}
+
+// Training targets of a finished training wave: for every path that passed a
+// cacheable vertex at the cache depth, (L_end - L_arrival) / beta_arrival per
+// channel (0 where beta is 0), compacted with its record into cin / ttgt.
+// Non-finite targets are dropped.
+__global__ __launch_bounds__(kBlock) void k_train_targets(PathsDev p, QueuesDev q, uint32_t count)
+{
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t i0 = blockIdx.x * kBlock; i0 < count; i0 += stride) {
+        const uint32_t slot = i0 + threadIdx.x;
+        bool ok = false;
+        float tg[3] = {0, 0, 0};
+        if (slot < count && p.tvalid[slot]) {
+            const float Le[3] = {p.lx[slot], p.ly[slot], p.lz[slot]};
+            ok = true;
+            for (int c = 0; c < 3; ++c) {
+                const float b = p.tB[3 * (size_t)slot + c];
+                tg[c] = b > 0 ? (Le[c] - p.tLb[3 * (size_t)slot + c]) / b : 0.0f;
+                ok = ok && isfinite(tg[c]);
+            }
+        }
+        const uint32_t k = wave_append(&q.counters[kCTrain], ok);
+        if (ok) {
+            const float4 *src = (const float4 *)(p.tin + (size_t)slot * 12);
+            float4 *dst = (float4 *)(p.cin + (size_t)k * 12);
+            dst[0] = src[0]; dst[1] = src[1]; dst[2] = src[2];
+            p.ttgt[3 * (size_t)k] = fmaxf(tg[0], 0.0f);
+            p.ttgt[3 * (size_t)k + 1] = fmaxf(tg[1], 0.0f);
+            p.ttgt[3 * (size_t)k + 2] = fmaxf(tg[2], 0.0f);
+        }
+    }
+}
+
 // film: FP64 accumulation with the reference's NaN/Inf sentinel bookkeeping
 // (pathTracing.cu:474-486): only finite samples are summed and counted (the
 // consumer rescales by samples / nvalid).  One thread per pixel sums that
@@ -582,23 +700,31 @@ __global__ __launch_bounds__(kBlock) void k_count(QueuesDev q, uint32_t cur)
 size_t extend_lds_bytes(uint32_t max_depth) { return (size_t)(max_depth + 1) * kBlock * sizeof(int32_t); }
 
 hipError_t render_wave(const SceneDev &sc, const PathsDev &paths, const ShadowDev &shadow, const QueuesDev &q,
-                       const FilmDev &film, const WaveParams &wp, uint32_t capacity, hipStream_t stream)
+                       const FilmDev &film, const WaveParams &wp, uint32_t capacity, hipStream_t stream,
+                       const CacheDev *cache, const CacheWork *cwork)
 {
     const int cap = (int)sc.bvh.max_depth + 1;
     const size_t lds = extend_lds_bytes(sc.bvh.max_depth);
     const dim3 blk(kBlock);
     const uint32_t gen_grid = (wp.count + kBlock - 1) / kBlock;
+    // cache vertices end rendering paths only; training paths trace to the end
+    const bool lookups = !wp.train && wp.cache_depth != kNoCache && cache && cwork;
+    if (!wp.train && wp.cache_depth != kNoCache && !lookups) return hipErrorInvalidValue;
+    if (lookups && cwork->rows < capacity) return hipErrorInvalidValue;
+    const uint32_t ext_depth = lookups ? wp.cache_depth : kNoCache;
     hipLaunchKernelGGL(k_reset, dim3(1), blk, 0, stream, q.counters, 0u, wp.count);
     hipLaunchKernelGGL(k_generate, dim3(gen_grid ? gen_grid : 1), blk, 0, stream, sc, paths, q, wp);
     uint32_t cur = 0;
     for (uint32_t bounce = 0; bounce < wp.max_depth; ++bounce) {
         if (bounce > 0) hipLaunchKernelGGL(k_reset, dim3(1), blk, 0, stream, q.counters, cur, 0xffffffffu);
-        hipLaunchKernelGGL(k_extend, dim3(kGrid), blk, lds, stream, sc, paths, q, cur, cap);
+        hipLaunchKernelGGL(k_extend, dim3(kGrid), blk, lds, stream, sc, paths, q, cur, cap, ext_depth);
         hipLaunchKernelGGL((k_shade<kQLambert>), dim3(kGrid), blk, 0, stream, sc, paths, shadow, q, wp, cur);
         hipLaunchKernelGGL((k_shade<kQDielectric>), dim3(kGrid), blk, 0, stream, sc, paths, shadow, q, wp, cur);
         hipLaunchKernelGGL((k_shade<kQGgx>), dim3(kGrid), blk, 0, stream, sc, paths, shadow, q, wp, cur);
         hipLaunchKernelGGL((k_shade<kQLight>), dim3(kGrid), blk, 0, stream, sc, paths, shadow, q, wp, cur);
         hipLaunchKernelGGL((k_shade<kQMiss>), dim3(kGrid), blk, 0, stream, sc, paths, shadow, q, wp, cur);
+        if (lookups && bounce >= wp.cache_depth)
+            hipLaunchKernelGGL(k_cache_vertex, dim3(kGrid), blk, 0, stream, sc, paths, q);
         hipLaunchKernelGGL(k_count, dim3(1), blk, 0, stream, q, cur);
         hipLaunchKernelGGL(k_connect, dim3(kGrid), blk, lds, stream, sc, paths, shadow, q, cap);
         hipLaunchKernelGGL(k_finalize, dim3(kGrid), blk, 0, stream, sc, paths, q, 0u, cur);
@@ -607,6 +733,15 @@ hipError_t render_wave(const SceneDev &sc, const PathsDev &paths, const ShadowDe
     // paths still alive after max_depth bounces cannot exist (shade stops
     // them at max_depth), but the list is flushed for safety
     hipLaunchKernelGGL(k_finalize, dim3(kGrid), blk, 0, stream, sc, paths, q, 1u, cur);
+    if (wp.train) {
+        hipLaunchKernelGGL(k_train_targets, dim3(kGrid), blk, 0, stream, paths, q, wp.count);
+        return hipGetLastError();
+    }
+    if (lookups) {
+        hipError_t e = cache_infer(*cache, *cwork, paths.cin, wp.count, paths.crgb, stream, &q.counters[kCCacheQ]);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_cache_apply, dim3(kGrid), blk, 0, stream, paths, q);
+    }
     hipLaunchKernelGGL(k_film, dim3(kGrid), blk, 0, stream, paths, film, wp);
     return hipGetLastError();
 }

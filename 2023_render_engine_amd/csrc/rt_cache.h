@@ -45,9 +45,13 @@ struct CacheWork {
     float *dx32;                  // [rows][64] encoding gradient
 };
 
-hipError_t cache_forward(const CacheDev &c, const CacheWork &w, const float *in, uint32_t n, hipStream_t s);
+// n is the row count (and grid size); with n_dev the kernels read the
+// actual count from device memory (capped at n), so a launch sequence can be
+// fixed before the count is known.
+hipError_t cache_forward(const CacheDev &c, const CacheWork &w, const float *in, uint32_t n, hipStream_t s,
+                         const uint32_t *n_dev = nullptr);
 hipError_t cache_infer(const CacheDev &c, const CacheWork &w, const float *in, uint32_t n, float *rgb,
-                       hipStream_t s);
+                       hipStream_t s, const uint32_t *n_dev = nullptr);
 // Forward + loss + backward; accumulates gradients (c.grad, c.grid_grad) and
 // adds the summed loss to *loss (device, may be null).
 hipError_t cache_backward(const CacheDev &c, const CacheWork &w, const float *in, const float *target,

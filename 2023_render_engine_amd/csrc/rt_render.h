@@ -10,13 +10,17 @@
 
 namespace rt {
 
-// Queue classes filled by the extend stage (material-sorted shading).
-enum : uint32_t { kQLambert = 0, kQDielectric = 1, kQGgx = 2, kQLight = 3, kQMiss = 4, kNumQ = 5 };
+// Queue classes filled by the extend stage (material-sorted shading).  kQCache
+// takes every hit at the radiance cache's depth when a cache is attached.
+enum : uint32_t { kQLambert = 0, kQDielectric = 1, kQGgx = 2, kQLight = 3, kQMiss = 4, kQCache = 5, kNumQ = 6 };
 
 // Device counters (one u32 each).
 enum : uint32_t {
-    kCActive0 = 0, kCActive1 = 1, kCQueue = 2, /* 2..6 */ kCShadow = 7, kCTerm = 8, kNumCounters = 16
+    kCActive0 = 0, kCActive1 = 1, kCQueue = 2, /* 2..7 */ kCShadow = 8, kCTerm = 9, kCCacheQ = 10,
+    kCTrain = 11, kNumCounters = 16
 };
+
+constexpr uint32_t kNoCache = 0xffffffffu;
 
 // Scene as the kernels see it (passed by value).
 struct SceneDev {
@@ -46,6 +50,14 @@ struct PathsDev {
     float *nbx, *nby, *nbz;     // throughput at the last next-event estimation
     uint32_t *nee_mask;         // unoccluded light slots of the last NEE
     float *nee_c;               // P * nlights * 3: per-light f * Li * w
+    // radiance cache: query records (P x 12) and results (P x 3), in queue order
+    float *cin, *crgb;
+    uint32_t *cslot;            // path slot of cache query k
+    // training paths: the record at the cache depth, L and beta on arrival
+    float *tin;                 // P x 12
+    float *tLb, *tB;            // P x 3 each
+    uint32_t *tvalid;
+    float *ttgt;                // compacted targets (P x 3); records compacted into cin
 };
 
 struct ShadowDev {
@@ -73,13 +85,21 @@ struct WaveParams {
     uint32_t npix;              // W*H
     uint32_t spp_offset, seed, max_depth, rr_depth;
     uint64_t stream_base;       // PCG stream offset (scene id for batches)
+    uint32_t cache_depth;       // hits at this depth end in the cache (kNoCache: off)
+    uint32_t train;             // 1: training wave (random pixels, no film, records at cache_depth)
 };
 
+struct CacheDev;
+struct CacheWork;
+
 // Launches one wave (generate, max_depth x {extend, shade, connect},
-// finalize) on `stream`.  Fails only on launch errors.
+// finalize, cache lookups, film) on `stream`.  With a cache and a rendering
+// wave, hits at cache_depth are answered by the cache; with a training wave
+// the compacted records and targets are left in paths.cin / paths.ttgt with
+// their count in counters[kCTrain].  Fails only on launch errors.
 hipError_t render_wave(const SceneDev &scene, const PathsDev &paths, const ShadowDev &shadow,
                        const QueuesDev &queues, const FilmDev &film, const WaveParams &wp, uint32_t capacity,
-                       hipStream_t stream);
+                       hipStream_t stream, const CacheDev *cache = nullptr, const CacheWork *cwork = nullptr);
 
 size_t extend_lds_bytes(uint32_t max_depth);
 

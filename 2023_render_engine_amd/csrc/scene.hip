@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "rt_cache.h"
 #include "rt_internal.h"
 #include "rt_render.h"
 
@@ -25,6 +26,9 @@ int ctx_device(rt_ctx *c);
 void *ctx_stream_ptr(rt_ctx *c);
 std::recursive_mutex &ctx_lock(rt_ctx *c);
 DevBvh bvh_device(const rt_bvh *b);
+const CacheDev &cache_device(const rt_cache *c);
+rt_ctx *cache_ctx(const rt_cache *c);
+uint32_t cache_batch(const rt_cache *c);
 }  // namespace rt
 
 using rt::set_error;
@@ -182,6 +186,9 @@ struct rt_scene {
     rt::QueuesDev queues{};
     rt_render_stats stats{};
     uint64_t stream_base = 0;
+    // radiance-cache inference scratch (one row per path slot), on demand
+    void *cache_arena = nullptr;
+    rt::CacheWork cwork{};
 };
 
 namespace rt {
@@ -191,6 +198,31 @@ static void free_arena(rt_scene *s)
     if (s->arena) (void)hipFree(s->arena);
     s->arena = nullptr;
     s->capacity = 0;
+    if (s->cache_arena) (void)hipFree(s->cache_arena);
+    s->cache_arena = nullptr;
+    s->cwork = rt::CacheWork{};
+}
+
+// Inference activations for `rows` cache queries (forward only).
+static rt_status ensure_cache_work(rt_scene *s, uint32_t rows)
+{
+    if (s->cache_arena && s->cwork.rows >= rows) return RT_OK;
+    if (s->cache_arena) (void)hipFree(s->cache_arena);
+    s->cache_arena = nullptr;
+    s->cwork = CacheWork{};
+    const size_t act = ((size_t)rows * kCacheWidth * 2 + 255) & ~(size_t)255;
+    const size_t y = (size_t)rows * kCacheOut * 4;
+    char *base = nullptr;
+    hipError_t e = hipMalloc(&base, act * kCacheLayers + y);
+    if (e != hipSuccess) {
+        set_error(std::string("rt_render_cached: cache scratch: ") + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;
+    }
+    s->cache_arena = base;
+    s->cwork.rows = rows;
+    for (uint32_t l = 0; l < kCacheLayers; ++l) s->cwork.act[l] = (uint16_t *)(base + l * act);
+    s->cwork.y = (float *)(base + kCacheLayers * act);
+    return RT_OK;
 }
 
 // One allocation for all wavefront buffers, carved in 256-byte aligned
@@ -220,6 +252,14 @@ static rt_status ensure_buffers(rt_scene *s, uint32_t P)
     add((void **)&p.hid, (size_t)P * 4);
     add((void **)&p.nee_mask, (size_t)P * 4);
     add((void **)&p.nee_c, (size_t)P * NL * 12);
+    add((void **)&p.cin, (size_t)P * 48);
+    add((void **)&p.crgb, (size_t)P * 12);
+    add((void **)&p.cslot, (size_t)P * 4);
+    add((void **)&p.tin, (size_t)P * 48);
+    add((void **)&p.tLb, (size_t)P * 12);
+    add((void **)&p.tB, (size_t)P * 12);
+    add((void **)&p.tvalid, (size_t)P * 4);
+    add((void **)&p.ttgt, (size_t)P * 12);
     ShadowDev &sh = s->shadow;
     float **sf[] = {&sh.ox, &sh.oy, &sh.oz, &sh.dx, &sh.dy, &sh.dz, &sh.tmax};
     for (float **f : sf) add((void **)f, SQ * 4);
@@ -456,23 +496,31 @@ rt_status rt_scene_camera_ray(const rt_scene *s, uint32_t px, uint32_t py, float
     return RT_OK;
 }
 
-rt_status rt_render(rt_scene *s, const rt_render_opts *opts, double *accum, uint32_t *nvalid, uint32_t flags,
-                    void *stream_)
+// Shared body of rt_render and rt_render_cached.
+static rt_status render_frames(const char *fn, rt_scene *s, const rt_render_opts *opts, rt_cache *cache,
+                               uint32_t cache_depth, double *accum, uint32_t *nvalid, uint32_t flags, void *stream_)
 {
-    if (!s || !opts || !accum || !nvalid) { set_error("rt_render: null argument"); return RT_ERR_INVALID; }
-    if (!s->committed) { set_error("rt_render: scene not committed"); return RT_ERR_INVALID; }
-    if (flags & RT_FLAG_HOST_PTRS) { set_error("rt_render: film buffers must be device pointers"); return RT_ERR_UNSUPPORTED; }
-    if (opts->max_depth == 0 || opts->max_depth > 0xffffu) { set_error("rt_render: max_depth must be 1..65535"); return RT_ERR_INVALID; }
+    const std::string name(fn);
+    if (!s || !opts || !accum || !nvalid) { set_error(name + ": null argument"); return RT_ERR_INVALID; }
+    if (!s->committed) { set_error(name + ": scene not committed"); return RT_ERR_INVALID; }
+    if (flags & RT_FLAG_HOST_PTRS) { set_error(name + ": film buffers must be device pointers"); return RT_ERR_UNSUPPORTED; }
+    if (opts->max_depth == 0 || opts->max_depth > 0xffffu) { set_error(name + ": max_depth must be 1..65535"); return RT_ERR_INVALID; }
+    if (cache && rt::cache_ctx(cache) != s->ctx) { set_error(name + ": cache and scene belong to different contexts"); return RT_ERR_INVALID; }
+    if (cache && cache_depth >= 0xffffu) { set_error(name + ": cache_depth must be below 65535"); return RT_ERR_INVALID; }
     std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(s->ctx));
     RT_HIP(hipSetDevice(rt::ctx_device(s->ctx)));
     hipStream_t stream = stream_ ? (hipStream_t)stream_ : (hipStream_t)rt::ctx_stream_ptr(s->ctx);
     const uint32_t npix = s->cam.width * s->cam.height;
     const uint64_t total = (uint64_t)npix * opts->spp;
     uint32_t P = opts->path_capacity ? opts->path_capacity : (1u << 21);
-    if (P > (1u << 26)) { set_error("rt_render: path_capacity above 2^26"); return RT_ERR_INVALID; }
+    if (P > (1u << 26)) { set_error(name + ": path_capacity above 2^26"); return RT_ERR_INVALID; }
     if ((uint64_t)P > total) P = (uint32_t)std::max<uint64_t>(1, total);
     rt_status st = rt::ensure_buffers(s, P);
     if (st != RT_OK) return st;
+    if (cache) {
+        st = rt::ensure_cache_work(s, s->capacity);
+        if (st != RT_OK) return st;
+    }
     auto t0 = std::chrono::steady_clock::now();
     RT_HIP(hipMemsetAsync(s->queues.counters, 0, rt::kNumCounters * 4, stream));
     RT_HIP(hipMemsetAsync(s->queues.stats, 0, 16, stream));
@@ -487,7 +535,10 @@ rt_status rt_render(rt_scene *s, const rt_render_opts *opts, double *accum, uint
         wp.max_depth = opts->max_depth;
         wp.rr_depth = opts->rr_depth;
         wp.stream_base = s->stream_base;
-        RT_HIP(rt::render_wave(s->dev, s->paths, s->shadow, s->queues, film, wp, P, stream));
+        wp.cache_depth = cache ? cache_depth : rt::kNoCache;
+        wp.train = 0;
+        RT_HIP(rt::render_wave(s->dev, s->paths, s->shadow, s->queues, film, wp, s->capacity, stream,
+                               cache ? &rt::cache_device(cache) : nullptr, cache ? &s->cwork : nullptr));
     }
     unsigned long long counts[2];
     RT_HIP(hipMemcpyAsync(counts, s->queues.stats, sizeof counts, hipMemcpyDeviceToHost, stream));
@@ -496,6 +547,61 @@ rt_status rt_render(rt_scene *s, const rt_render_opts *opts, double *accum, uint
     s->stats.extension_rays = counts[0];
     s->stats.shadow_rays = counts[1];
     s->stats.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return RT_OK;
+}
+
+rt_status rt_render(rt_scene *s, const rt_render_opts *opts, double *accum, uint32_t *nvalid, uint32_t flags,
+                    void *stream)
+{
+    return render_frames("rt_render", s, opts, nullptr, 0, accum, nvalid, flags, stream);
+}
+
+rt_status rt_render_cached(rt_scene *s, const rt_render_opts *opts, rt_cache *cache, uint32_t cache_depth,
+                           double *accum, uint32_t *nvalid, uint32_t flags, void *stream)
+{
+    if (!cache) { set_error("rt_render_cached: null cache"); return RT_ERR_INVALID; }
+    return render_frames("rt_render_cached", s, opts, cache, cache_depth, accum, nvalid, flags, stream);
+}
+
+rt_status rt_cache_train_scene(rt_cache *c, rt_scene *s, const rt_render_opts *opts, uint32_t cache_depth,
+                               uint32_t n_paths, float *loss, uint32_t *n_samples, void *stream_)
+{
+    if (n_samples) *n_samples = 0;
+    if (!c || !s || !opts) { set_error("rt_cache_train_scene: null argument"); return RT_ERR_INVALID; }
+    if (!s->committed) { set_error("rt_cache_train_scene: scene not committed"); return RT_ERR_INVALID; }
+    if (rt::cache_ctx(c) != s->ctx) { set_error("rt_cache_train_scene: cache and scene belong to different contexts"); return RT_ERR_INVALID; }
+    if (opts->max_depth == 0 || opts->max_depth > 0xffffu) { set_error("rt_cache_train_scene: max_depth must be 1..65535"); return RT_ERR_INVALID; }
+    if (cache_depth >= 0xffffu) { set_error("rt_cache_train_scene: cache_depth must be below 65535"); return RT_ERR_INVALID; }
+    if (n_paths == 0 || n_paths > (1u << 26)) { set_error("rt_cache_train_scene: n_paths must be 1..2^26"); return RT_ERR_INVALID; }
+    std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(s->ctx));
+    RT_HIP(hipSetDevice(rt::ctx_device(s->ctx)));
+    hipStream_t stream = stream_ ? (hipStream_t)stream_ : (hipStream_t)rt::ctx_stream_ptr(s->ctx);
+    rt_status st = rt::ensure_buffers(s, n_paths);
+    if (st != RT_OK) return st;
+    RT_HIP(hipMemsetAsync(s->queues.counters, 0, rt::kNumCounters * 4, stream));
+    RT_HIP(hipMemsetAsync(s->queues.stats, 0, 16, stream));
+    rt::WaveParams wp{};
+    wp.first_path = (uint64_t)opts->spp_offset * n_paths;
+    wp.count = n_paths;
+    wp.npix = s->cam.width * s->cam.height;
+    wp.spp_offset = 0;
+    wp.seed = opts->seed;
+    wp.max_depth = opts->max_depth;
+    wp.rr_depth = opts->rr_depth;
+    wp.stream_base = s->stream_base;
+    wp.cache_depth = cache_depth;
+    wp.train = 1;
+    rt::FilmDev film{nullptr, nullptr};
+    RT_HIP(rt::render_wave(s->dev, s->paths, s->shadow, s->queues, film, wp, s->capacity, stream));
+    uint32_t k = 0;
+    RT_HIP(hipMemcpyAsync(&k, s->queues.counters + rt::kCTrain, 4, hipMemcpyDeviceToHost, stream));
+    RT_HIP(hipStreamSynchronize(stream));
+    const uint32_t n = std::min(k, rt::cache_batch(c));
+    if (n == 0) return RT_OK;
+    st = rt_cache_train_step(c, s->paths.cin, s->paths.ttgt, n, loss, stream);
+    if (st != RT_OK) return st;
+    if (n_samples) *n_samples = n;
+    if (!loss) RT_HIP(hipStreamSynchronize(stream));
     return RT_OK;
 }
 

@@ -284,6 +284,35 @@ rt_status rt_cache_gradients(rt_cache *cache, const float *in, const float *targ
 rt_status rt_cache_get_params(const rt_cache *cache, float *mlp, float *grid);
 rt_status rt_cache_set_params(rt_cache *cache, const float *mlp, const float *grid);
 
+/* The radiance cache inside the path tracer.  No counterpart in the
+ * reference (its learned mode is a per-shape polynomial fit,
+ * integration/Denoiser.cu, sampling/ShapeLightDistribution.cu:84-272); these
+ * extend PathTracerIntegrator::render (integration/PathTracerIntegrator.cu:
+ * 146-210) as the plain rt_render does.
+ *
+ * A cache vertex is the first hit, at path depth >= cache_depth (camera hit =
+ * depth 0), on a diffuse surface or a GGX surface with roughness >= 0.04
+ * (alpha >= 0.2).  Specular and sharper glossy hits keep tracing.
+ *
+ * rt_render_cached: as rt_render, but every path ends at its cache vertex
+ * (after the emission and next-event estimate of the vertices before it) and
+ * takes beta * cache(vertex).  The cache and the scene share one context.
+ *
+ * rt_cache_train_scene: one online training step.  Traces n_paths training
+ * paths (pseudo-random pixels; sample indices disjoint from rendering ones;
+ * opts->spp_offset selects the frame's set, opts->spp is ignored) to the end
+ * with the plain estimator, takes for each path that has a cache vertex the
+ * radiance it gathered from that vertex on divided by its throughput on
+ * arrival, and runs one Adam step on up to the cache's batch size of these
+ * (record, radiance) pairs.  *loss (mean relative L2 loss before the step) and
+ * *n_samples may be NULL; n_samples = 0 means no step was taken.  Blocks until
+ * the step is done. */
+rt_status rt_render_cached(rt_scene *scene, const rt_render_opts *opts, rt_cache *cache, uint32_t cache_depth,
+                           double *accum, uint32_t *nvalid, uint32_t flags, void *stream);
+rt_status rt_cache_train_scene(rt_cache *cache, rt_scene *scene, const rt_render_opts *opts,
+                               uint32_t cache_depth, uint32_t n_paths, float *loss, uint32_t *n_samples,
+                               void *stream);
+
 /* ------------------------------------------------------------------ */
 /* Synthetic workloads (host generators, deterministic)                 */
 /* ------------------------------------------------------------------ */

@@ -62,6 +62,8 @@ _SIGS = {
     "orc_scene_camera": (None, [_P, _P, _P]),
     "orc_path": (C.c_int, [_P, _U32, _U32, _U32, _U32, _U32, _U32, C.c_uint64, _P]),
     "orc_render": (None, [_P, _U32, _U32, _U32, _U32, _U32, C.c_uint64, _U32, _U32, _U32, _U32, _P, _P, C.c_int]),
+    "orc_render_const_cache": (None, [_P, _U32, _U32, _U32, _U32, _U32, C.c_uint64, _U32, _U32, _U32, _U32, _P, _P,
+                                      C.c_int, _U32, _P]),
     "orc_compute_rmse": (_F, [_P, C.c_double, _P, C.c_double, _U32]),
     "orc_cache_loss_grad": (C.c_double, [_P, _P, _P, _P, _P, _U32, C.c_int, _P, _P, _P, _P]),
 }
@@ -181,17 +183,24 @@ class OrcScene:
         return bool(ok), out
 
     def render(self, spp=None, spp_offset=0, window=None, seed=None, nthreads=8, stream_base=0,
-               max_depth=None):
+               max_depth=None, cache_depth=None, cache_rgb=None):
         """Returns (accum (H*W*3 float64), nvalid (H*W uint32)) for the window
-        (x0, y0, x1, y1) (default: whole image)."""
+        (x0, y0, x1, y1) (default: whole image).  With cache_rgb, paths end
+        at their first cacheable vertex at depth >= cache_depth and take
+        beta * cache_rgb (orc_render_const_cache)."""
         d = self.desc
         W, H = d.width, d.height
         x0, y0, x1, y1 = window or (0, 0, W, H)
         accum = np.zeros(W * H * 3, np.float64)
         nvalid = np.zeros(W * H, np.uint32)
-        lib().orc_render(self.h, spp if spp is not None else d.spp, d.seed if seed is None else seed,
-                         max_depth or d.max_depth, d.rr_depth, spp_offset, stream_base, x0, y0, x1, y1,
-                         _p(accum), _p(nvalid), nthreads)
+        args = (self.h, spp if spp is not None else d.spp, d.seed if seed is None else seed,
+                max_depth or d.max_depth, d.rr_depth, spp_offset, stream_base, x0, y0, x1, y1,
+                _p(accum), _p(nvalid), nthreads)
+        if cache_rgb is None:
+            lib().orc_render(*args)
+        else:
+            c = np.ascontiguousarray(cache_rgb, np.float32)
+            lib().orc_render_const_cache(*args, cache_depth, _p(c))
         return accum, nvalid
 
 _lib = None

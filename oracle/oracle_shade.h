@@ -76,6 +76,15 @@ void orc_render(const orc_scene *s, uint32_t spp, uint32_t seed, uint32_t max_de
                 uint32_t spp_offset, uint64_t stream_base, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1,
                 double *accum, uint32_t *nvalid, int nthreads);
 
+/* orc_render with paths that end at their first cacheable vertex (diffuse, or
+ * GGX roughness >= 0.04) at depth >= cache_depth and add beta * cache_rgb
+ * there: the library's cache termination with a cache that answers one
+ * constant. */
+void orc_render_const_cache(const orc_scene *s, uint32_t spp, uint32_t seed, uint32_t max_depth,
+                            uint32_t rr_depth, uint32_t spp_offset, uint64_t stream_base, uint32_t x0, uint32_t y0,
+                            uint32_t x1, uint32_t y1, double *accum, uint32_t *nvalid, int nthreads,
+                            uint32_t cache_depth, const float cache_rgb[3]);
+
 float orc_compute_rmse(const double *sum_a, double n_a, const double *sum_b, double n_b, uint32_t npix);
 
 #ifdef __cplusplus

@@ -666,10 +666,23 @@ static float sphere_light_pdf(const orc_sphere *sp, v3 p)
     return 1.0f / (2 * PIf * (1 - cmax));
 }
 
+/* Vertices a radiance cache may stand for: diffuse, or GGX with roughness
+ * >= 0.04 (alpha >= 0.2).  The reference has no cache; this is the library's
+ * rule, restated for the termination tests. */
+static int cacheable(const orc_material *m)
+{
+    return m->type == ORC_MAT_LAMBERTIAN ||
+           ((m->type == ORC_MAT_TROWBRIDGE || m->type == ORC_MAT_TROWBRIDGE_BASIC) && m->roughness >= 0.04f);
+}
+
 /* One path of the estimator (integration/pathTracing.cu:33-123,173-205,
- * Scene.cu:280-335,468-525).  Returns L in out[3] and 1 if finite. */
-int orc_path(const orc_scene *s, uint32_t px, uint32_t py, uint32_t sample, uint32_t seed, uint32_t max_depth,
-             uint32_t rr_depth, uint64_t stream_base, float out[3])
+ * Scene.cu:280-335,468-525).  Returns L in out[3] and 1 if finite.  With
+ * cache_rgb non-NULL the path ends at its first cacheable vertex at depth >=
+ * cache_depth and adds beta * cache_rgb there (a cache that answers one
+ * constant). */
+static int path_impl(const orc_scene *s, uint32_t px, uint32_t py, uint32_t sample, uint32_t seed,
+                     uint32_t max_depth, uint32_t rr_depth, uint64_t stream_base, uint32_t cache_depth,
+                     const float *cache_rgb, float out[3])
 {
     const uint32_t W = s->cam.width;
     pcg32 g = pcg_seed(orc_path_initstate(seed, sample), stream_base + (uint64_t)py * W + px);
@@ -685,6 +698,10 @@ int orc_path(const orc_scene *s, uint32_t px, uint32_t py, uint32_t sample, uint
         hit_rec h = scene_intersect(s, o, d);
         if (h.kind == 0) break;
         const orc_material *m = &s->mat[h.kind == 1 ? s->tri_mat[h.id] : s->sph_mat[h.id]];
+        if (cache_rgb && depth >= cache_depth && cacheable(m)) {
+            L = add(L, mk(beta.x * cache_rgb[0], beta.y * cache_rgb[1], beta.z * cache_rgb[2]));
+            break;
+        }
         v3 n = h.n, t, b;
         onb(n, &t, &b);
         v3 wo_w = neg(d);
@@ -765,6 +782,12 @@ int orc_path(const orc_scene *s, uint32_t px, uint32_t py, uint32_t sample, uint
     return isfinite(L.x) && isfinite(L.y) && isfinite(L.z);
 }
 
+int orc_path(const orc_scene *s, uint32_t px, uint32_t py, uint32_t sample, uint32_t seed, uint32_t max_depth,
+             uint32_t rr_depth, uint64_t stream_base, float out[3])
+{
+    return path_impl(s, px, py, sample, seed, max_depth, rr_depth, stream_base, 0, NULL, out);
+}
+
 typedef struct {
     const orc_scene *s;
     uint32_t spp, seed, max_depth, rr_depth, spp_offset;
@@ -772,6 +795,8 @@ typedef struct {
     double *accum;
     uint32_t *nvalid;
     uint64_t stream_base;
+    uint32_t cache_depth;
+    const float *cache_rgb;
 } render_ctx;
 
 static void *render_rows(void *p)
@@ -784,7 +809,8 @@ static void *render_rows(void *p)
             uint32_t nv = 0;
             for (uint32_t k = 0; k < c->spp; ++k) {
                 float L[3];
-                if (orc_path(c->s, x, y, c->spp_offset + k, c->seed, c->max_depth, c->rr_depth, c->stream_base, L)) {
+                if (path_impl(c->s, x, y, c->spp_offset + k, c->seed, c->max_depth, c->rr_depth, c->stream_base,
+                              c->cache_depth, c->cache_rgb, L)) {
                     acc[0] += L[0]; acc[1] += L[1]; acc[2] += L[2];
                     ++nv;
                 }
@@ -798,9 +824,10 @@ static void *render_rows(void *p)
     return NULL;
 }
 
-void orc_render(const orc_scene *s, uint32_t spp, uint32_t seed, uint32_t max_depth, uint32_t rr_depth,
-                uint32_t spp_offset, uint64_t stream_base, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1,
-                double *accum, uint32_t *nvalid, int nthreads)
+static void render_impl(const orc_scene *s, uint32_t spp, uint32_t seed, uint32_t max_depth, uint32_t rr_depth,
+                        uint32_t spp_offset, uint64_t stream_base, uint32_t x0, uint32_t y0, uint32_t x1,
+                        uint32_t y1, double *accum, uint32_t *nvalid, int nthreads, uint32_t cache_depth,
+                        const float *cache_rgb)
 {
     if (nthreads < 1) nthreads = 1;
     uint32_t rows = y1 > y0 ? y1 - y0 : 0;
@@ -811,7 +838,7 @@ void orc_render(const orc_scene *s, uint32_t spp, uint32_t seed, uint32_t max_de
     uint32_t chunk = (rows + nthreads - 1) / nthreads;
     for (int i = 0; i < nthreads; ++i) {
         render_ctx c = {s, spp, seed, max_depth, rr_depth, spp_offset, x0, y0 + i * chunk, x1, y0 + (i + 1) * chunk,
-                        accum, nvalid, stream_base};
+                        accum, nvalid, stream_base, cache_depth, cache_rgb};
         if (c.y0 > y1) c.y0 = y1;
         if (c.y1 > y1) c.y1 = y1;
         cs[i] = c;
@@ -819,6 +846,23 @@ void orc_render(const orc_scene *s, uint32_t spp, uint32_t seed, uint32_t max_de
     }
     for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
     free(th); free(cs);
+}
+
+void orc_render(const orc_scene *s, uint32_t spp, uint32_t seed, uint32_t max_depth, uint32_t rr_depth,
+                uint32_t spp_offset, uint64_t stream_base, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1,
+                double *accum, uint32_t *nvalid, int nthreads)
+{
+    render_impl(s, spp, seed, max_depth, rr_depth, spp_offset, stream_base, x0, y0, x1, y1, accum, nvalid,
+                nthreads, 0, NULL);
+}
+
+void orc_render_const_cache(const orc_scene *s, uint32_t spp, uint32_t seed, uint32_t max_depth,
+                            uint32_t rr_depth, uint32_t spp_offset, uint64_t stream_base, uint32_t x0, uint32_t y0,
+                            uint32_t x1, uint32_t y1, double *accum, uint32_t *nvalid, int nthreads,
+                            uint32_t cache_depth, const float cache_rgb[3])
+{
+    render_impl(s, spp, seed, max_depth, rr_depth, spp_offset, stream_base, x0, y0, x1, y1, accum, nvalid,
+                nthreads, cache_depth, cache_rgb);
 }
 
 /* computeRMSE (camera/Film.cu:611-632): per pixel the squared norm of the

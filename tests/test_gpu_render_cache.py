@@ -1,0 +1,143 @@
+"""The radiance cache inside the GPU path tracer (rt_render_cached,
+rt_cache_train_scene through the C ABI).
+
+1. Routing parity: with a cache whose network answers one constant, the
+   GPU film matches the oracle's orc_render_const_cache (pinned by closed
+   forms on the CPU) to the bar of the plain render parity tests.
+2. A cache depth beyond the maximum depth is the plain renderer, bit for bit.
+3. Online training on the scene's own radiance lowers the loss, and the
+   trained cache beats both the untrained cache and plain path tracing at
+   equal samples per pixel against a high-spp reference."""
+import dataclasses
+import importlib
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+scenes = importlib.import_module("2023_render_engine_amd.scenes")
+C = np.array([0.5, 0.25, 0.125], np.float32)
+
+
+def constant_cache(rt, ctx, aabb_lo, aabb_hi, c):
+    """A cache whose MLP maps the encoding's constant-1 input through hidden
+    unit 0 to the output c (all other weights 0): exact in BF16 for c in
+    {0.5, 0.25, 0.125}."""
+    cache = rt.RadianceCache(ctx, aabb_lo, aabb_hi, levels=16, log2_table=12, batch=1024, seed=2)
+    mlp, grid = cache.get_params()
+    mlp[:] = 0
+    one = 2 * 16 + 7  # after 2 features per level: normal 3, view 3, roughness 1, then the constant
+    mlp[one * 64 + 0] = 1.0
+    for l in (1, 2, 3):
+        mlp[l * 4096 + 0] = 1.0
+    mlp[16384 + 0:16384 + 3] = c
+    cache.set_params(mlp, grid)
+    return cache
+
+
+def aabb(desc):
+    lo = np.minimum(desc.xyz.min(0), -1) if len(desc.xyz) else np.array([-1, -1, -1])
+    hi = np.maximum(desc.xyz.max(0), 1) if len(desc.xyz) else np.array([1, 1, 1])
+    return tuple(float(x) for x in lo), tuple(float(x) for x in hi)
+
+
+def films(sc, film):
+    torch.cuda.synchronize()
+    return film[0].cpu().numpy().copy(), film[1].cpu().numpy().astype(np.uint32).copy()
+
+
+def mean_img(acc, nv):
+    return acc.reshape(-1, 3) / np.maximum(nv.astype(np.float64), 1)[:, None]
+
+
+def test_constant_cache_query_is_exact(rt, gpu_ctx):
+    cache = constant_cache(rt, gpu_ctx, (-1, -1, -1), (1, 1, 1), C)
+    rec = torch.rand((4096, 12), device="cuda") * 2 - 1
+    y = cache.query(rec).cpu().numpy()
+    assert np.all(y == C[None, :])
+    cache.close()
+
+
+@pytest.mark.parametrize("which,cache_depth", [("cornell", 1), ("cornell", 2), ("procedural", 2)])
+def test_constant_cache_render_matches_oracle(rt, orc, gpu_ctx, which, cache_depth):
+    if which == "cornell":
+        desc = scenes.cornell(width=64, height=64, spp=16)
+    else:
+        desc = scenes.procedural(grid=64, n_spheres=8, subdiv=2, width=96, height=64, spp=16, max_depth=8)
+    print(f"[render cache parity] {which} depth {cache_depth}", flush=True)
+    lo, hi = aabb(desc)
+    cache = constant_cache(rt, gpu_ctx, lo, hi, C)
+    sc = rt.Scene(gpu_ctx, desc)
+    film = sc.new_film()
+    sc.render_cached(film, cache, cache_depth=cache_depth)
+    ga, gn = films(sc, film)
+    os_ = orc.OrcScene(desc)
+    oa, on = os_.render(nthreads=16, cache_depth=cache_depth, cache_rgb=C)
+    ob, onb = os_.render(nthreads=16, cache_depth=cache_depth, cache_rgb=C, seed=desc.seed + 1000)
+    gi, oi, oi2 = mean_img(ga, gn), mean_img(oa, on), mean_img(ob, onb)
+    r_go, r_noise = rt.compute_rmse(gi, oi), rt.compute_rmse(oi, oi2)
+    close = np.all(np.abs(gi - oi) <= 1e-4 * np.maximum(np.abs(oi), 1e-3), axis=1).mean()
+    print(f"[render cache parity] {which} depth {cache_depth}: rmse(gpu, oracle) {r_go:.3e}, noise {r_noise:.3e}, "
+          f"pixels within 1e-4: {close:.4f}", flush=True)
+    assert np.isfinite(gi).all()
+    assert r_go < 0.05 * r_noise
+    assert close > 0.9
+    sc.close()
+    cache.close()
+
+
+def test_cache_depth_beyond_max_depth_is_plain_render(rt, gpu_ctx):
+    desc = scenes.cornell(width=64, height=64, spp=8)
+    cache = constant_cache(rt, gpu_ctx, (-1, -1, -1), (1, 1, 1), C)
+    sc = rt.Scene(gpu_ctx, desc)
+    f1 = sc.new_film()
+    sc.render(f1)
+    f2 = sc.new_film()
+    sc.render_cached(f2, cache, cache_depth=desc.max_depth)
+    a1, n1 = films(sc, f1)
+    a2, n2 = films(sc, f2)
+    assert np.array_equal(a1, a2) and np.array_equal(n1, n2)
+    sc.close()
+    cache.close()
+
+
+def test_online_training_beats_plain_and_untrained(rt, gpu_ctx):
+    """Cornell box at 64x64: 300 training steps of 16384 paths each on the
+    scene's radiance (cache after 2 bounces).  At 4 spp the trained cache's
+    image is closer to a 1024-spp plain reference than the plain 4-spp
+    image and than the image with the untrained cache."""
+    desc = dataclasses.replace(scenes.cornell(width=64, height=64, spp=4), max_depth=16)
+    lo, hi = (-0.51, -0.51, -0.51), (0.51, 0.51, 0.51)
+    sc = rt.Scene(gpu_ctx, desc)
+    ref = sc.new_film()
+    sc.render(ref, spp=1024, spp_offset=1 << 20)
+    ri = mean_img(*films(sc, ref))
+    cache = rt.RadianceCache(gpu_ctx, lo, hi, levels=16, log2_table=16, batch=16384, seed=1)
+
+    def cached_img():
+        f = sc.new_film()
+        sc.render_cached(f, cache, cache_depth=2)
+        return mean_img(*films(sc, f))
+
+    untrained = rt.compute_rmse(cached_img(), ri)
+    losses, counts = [], []
+    for frame in range(300):
+        l, n = sc.train_cache(cache, frame, n_paths=16384, cache_depth=2)
+        losses.append(l)
+        counts.append(n)
+    trained = rt.compute_rmse(cached_img(), ri)
+    fp = sc.new_film()
+    sc.render(fp)
+    plain = rt.compute_rmse(mean_img(*films(sc, fp)), ri)
+    print(f"[cache scene train] samples/step {np.mean(counts):.0f}, loss {np.mean(losses[:10]):.4f} -> "
+          f"{np.mean(losses[-10:]):.4f}; rmse vs 1024 spp: plain 4 spp {plain:.4f}, untrained cache {untrained:.4f}, "
+          f"trained cache {trained:.4f}", flush=True)
+    assert min(counts) > 1000
+    assert np.isfinite(losses).all()
+    assert np.mean(losses[-10:]) < 0.7 * np.mean(losses[:10])
+    assert trained < 0.5 * untrained
+    assert trained < plain
+    sc.close()
+    cache.close()

@@ -184,6 +184,7 @@ class Context:
         check(lib().rt_ctx_create(device, C.byref(h)))
         self.h = h
         self.device = device
+        self._ext = None
 
     def close(self):
         if self.h:
@@ -202,6 +203,36 @@ class Context:
     @property
     def stream(self) -> int:
         return lib().rt_ctx_stream(self.h)
+
+    def launch(self, fn, stream=None):
+        """Calls fn(stream_handle) in order with torch's current stream.
+
+        A created torch stream is passed through.  torch's default stream has
+        handle 0, which the C ABI reads as "the context's own stream" (a
+        non-blocking stream that does not synchronise with the default one),
+        so the call then runs on the context's stream fenced by events on
+        both sides: it sees all earlier torch work and later torch work sees
+        its results."""
+        if stream is not None:
+            return fn(stream)
+        import torch
+        dev = torch.device("cuda", self.device)
+        cur = torch.cuda.current_stream(dev)
+        if cur.cuda_stream:
+            return fn(cur.cuda_stream)
+        if self._ext is None:
+            self._ext = torch.cuda.ExternalStream(self.stream, device=dev)
+        self._ext.wait_stream(cur)
+        try:
+            return fn(self._ext.cuda_stream)
+        finally:
+            cur.wait_stream(self._ext)
+
+    def fence_torch(self):
+        """Waits for torch's current stream (before calls that take no stream
+        and run on the context's own)."""
+        import torch
+        torch.cuda.current_stream(torch.device("cuda", self.device)).synchronize()
 
 
 class Bvh:
@@ -266,8 +297,8 @@ class Bvh:
         dev = ox.device
         prim = torch.empty(n, dtype=torch.int32, device=dev) if prim is None else prim
         t = torch.empty(n, dtype=torch.float32, device=dev) if t is None else t
-        s = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
-        check(lib().rt_trace_closest(self.h, C.byref(r), n, _ptr(prim), _ptr(t), 0, s))
+        self.ctx.launch(lambda s: check(lib().rt_trace_closest(self.h, C.byref(r), n, _ptr(prim), _ptr(t), 0, s)),
+                        stream)
         return prim, t
 
     def trace_any(self, ox, oy, oz, dx, dy, dz, tmax, occ=None, host=False, stream=None):
@@ -280,8 +311,7 @@ class Bvh:
             return occ
         import torch
         occ = torch.empty(n, dtype=torch.uint8, device=ox.device) if occ is None else occ
-        s = stream if stream is not None else torch.cuda.current_stream(ox.device).cuda_stream
-        check(lib().rt_trace_any(self.h, C.byref(r), n, _ptr(occ), 0, s))
+        self.ctx.launch(lambda s: check(lib().rt_trace_any(self.h, C.byref(r), n, _ptr(occ), 0, s)), stream)
         return occ
 
     def trace_stats(self, ox, oy, oz, dx, dy, dz, tmax=None, any_hit=False):
@@ -290,6 +320,7 @@ class Bvh:
         r = self._rays((ox, oy, oz, dx, dy, dz), tmax)
         a = C.c_uint64()
         b = C.c_uint64()
+        self.ctx.fence_torch()
         check(lib().rt_trace_stats(self.h, C.byref(r), n, int(any_hit), C.byref(a), C.byref(b)))
         return a.value, b.value
 
@@ -381,8 +412,8 @@ class Scene:
         o = RenderOpts(spp if spp is not None else d.spp, spp_offset, max_depth or d.max_depth,
                        d.rr_depth if rr_depth is None else rr_depth, d.seed if seed is None else seed, path_capacity)
         accum, nvalid = film
-        s = stream if stream is not None else torch.cuda.current_stream(accum.device).cuda_stream
-        check(lib().rt_render(self.h, C.byref(o), _ptr(accum), _ptr(nvalid), 0, s))
+        self.ctx.launch(lambda s: check(lib().rt_render(self.h, C.byref(o), _ptr(accum), _ptr(nvalid), 0, s)),
+                        stream)
         return film
 
     def _opts(self, spp, spp_offset, max_depth, rr_depth, seed, path_capacity):
@@ -398,8 +429,8 @@ class Scene:
         import torch
         o = self._opts(spp, spp_offset, max_depth, rr_depth, seed, path_capacity)
         accum, nvalid = film
-        s = stream if stream is not None else torch.cuda.current_stream(accum.device).cuda_stream
-        check(lib().rt_render_cached(self.h, C.byref(o), cache.h, cache_depth, _ptr(accum), _ptr(nvalid), 0, s))
+        self.ctx.launch(lambda s: check(lib().rt_render_cached(self.h, C.byref(o), cache.h, cache_depth, _ptr(accum),
+                                                               _ptr(nvalid), 0, s)), stream)
         return film
 
     def train_cache(self, cache, frame, n_paths=65536, cache_depth=2, max_depth=None, rr_depth=None, seed=None,
@@ -410,9 +441,8 @@ class Scene:
         o = self._opts(1, frame, max_depth, rr_depth, seed, 0)
         loss = C.c_float(0.0)
         n = C.c_uint32(0)
-        s = stream if stream is not None else torch.cuda.current_stream(torch.device("cuda", self.ctx.device)).cuda_stream
-        check(lib().rt_cache_train_scene(cache.h, self.h, C.byref(o), cache_depth, n_paths, C.byref(loss),
-                                         C.byref(n), s))
+        self.ctx.launch(lambda s: check(lib().rt_cache_train_scene(cache.h, self.h, C.byref(o), cache_depth, n_paths,
+                                                                   C.byref(loss), C.byref(n), s)), stream)
         return loss.value, n.value
 
     def stats(self) -> RenderStats:
@@ -483,16 +513,15 @@ class RadianceCache:
         import torch
         n = int(inputs.shape[0])
         rgb = torch.empty((n, 3), dtype=torch.float32, device=inputs.device) if rgb is None else rgb
-        s = stream if stream is not None else torch.cuda.current_stream(inputs.device).cuda_stream
-        check(lib().rt_cache_query(self.h, _ptr(inputs), n, _ptr(rgb), s))
+        self.ctx.launch(lambda s: check(lib().rt_cache_query(self.h, _ptr(inputs), n, _ptr(rgb), s)), stream)
         return rgb
 
     def train_step(self, inputs, targets, want_loss=True, stream=None):
         import torch
         loss = C.c_float(0.0)
-        s = stream if stream is not None else torch.cuda.current_stream(inputs.device).cuda_stream
-        check(lib().rt_cache_train_step(self.h, _ptr(inputs), _ptr(targets), int(inputs.shape[0]),
-                                        C.byref(loss) if want_loss else None, s))
+        self.ctx.launch(lambda s: check(lib().rt_cache_train_step(self.h, _ptr(inputs), _ptr(targets),
+                                                                  int(inputs.shape[0]),
+                                                                  C.byref(loss) if want_loss else None, s)), stream)
         return loss.value if want_loss else None
 
     def gradients(self, inputs, targets):
@@ -501,6 +530,7 @@ class RadianceCache:
         mg = np.zeros(inf.mlp_params, np.float32)
         gg = np.zeros(inf.grid_params, np.float32)
         loss = C.c_float(0.0)
+        self.ctx.fence_torch()
         check(lib().rt_cache_gradients(self.h, _ptr(inputs), _ptr(targets), int(inputs.shape[0]), _ptr(mg),
                                        _ptr(gg), C.byref(loss)))
         return loss.value, mg, gg

@@ -261,12 +261,150 @@ def run_render(args, which):
     return out
 
 
+def _rmse_dev(acc, nv, ref_img):
+    """computeRMSE (camera/Film.cu:611-632) on the device, FP64: mean over
+    pixels of the RGB channel sum of squared differences of mean radiance."""
+    import torch
+    img = acc.view(-1, 3) / nv.double().clamp(min=1).unsqueeze(1)
+    return float(torch.sqrt(((img - ref_img) ** 2).sum(1).mean()))
+
+
+def run_c4(args):
+    """Time to RMSE <= target against a high-spp FP64 reference, plain path
+    tracing vs the online-trained radiance cache, on the procedural scene.
+
+    Frames are 4 spp each.  Plain: frames accumulate until the image's RMSE
+    against the reference reaches the target.  Cache: every frame first
+    takes one training step on 65,536 training paths (rt_cache_train_scene,
+    cache after 2 bounces), then renders its 4 spp with paths ending in the
+    cache (rt_render_cached); the first `warm` frames only train.  The timed
+    region holds every render and training call (device-synchronised per
+    frame); the RMSE evaluation between frames is not timed.  The reference
+    is the plain renderer with FP64 accumulation at ref_spp (its own
+    parity against the CPU restatement of the reference estimator is tested
+    separately), with sample indices disjoint from both runs."""
+    import numpy as np
+    import torch
+
+    rt = importlib.import_module("2023_render_engine_amd")
+    scenes = importlib.import_module("2023_render_engine_amd.scenes")
+    W, H = args.c4_width, args.c4_height
+    log(f"generating the procedural scene at {W}x{H}")
+    desc = scenes.procedural(width=W, height=H, spp=4)
+    label = (f"radiance cache on the procedural scene ({desc.idx.shape[0]} triangles, 8 point lights + 1 sphere "
+             f"light, depth 16, RR after 3) at {W}x{H}: 4 spp frames, 65536 training paths/frame, cache after "
+             f"2 bounces; reference {args.c4_ref_spp} spp FP64")
+    ctx = rt.Context(0)
+    sc = rt.Scene(ctx, desc)
+    # reference (untimed), in 256-spp chunks at sample offsets far from the runs'
+    ref = sc.new_film()
+    t0 = time.time()
+    for k in range(0, args.c4_ref_spp, 256):
+        sc.render(ref, spp=256, spp_offset=(1 << 24) + k)
+    torch.cuda.synchronize()
+    log(f"reference {args.c4_ref_spp} spp in {time.time() - t0:.1f} s")
+    ref_img = ref[0].view(-1, 3) / ref[1].double().clamp(min=1).unsqueeze(1)
+    target = args.c4_target
+    budget = args.c4_budget
+
+    def run(use_cache):
+        film = sc.new_film()
+        cache = None
+        if use_cache:
+            lo = np.minimum(desc.xyz.min(0), [0.38, 0.38, 0.98]) - 1e-3
+            hi = np.maximum(desc.xyz.max(0), [0.62, 0.62, 1.22]) + 1e-3
+            cache = rt.RadianceCache(ctx, tuple(lo), tuple(hi), levels=16, log2_table=19, batch=65536, seed=1)
+        curve, spent, frame, reached = [], 0.0, 0, None
+        losses = []
+        while spent < budget:
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            if use_cache:
+                l, _ = sc.train_cache(cache, frame, n_paths=65536, cache_depth=2)
+                losses.append(l)
+                if frame >= args.c4_warm:
+                    sc.render_cached(film, cache, cache_depth=2, spp=4, spp_offset=4 * frame)
+            else:
+                sc.render(film, spp=4, spp_offset=4 * frame)
+            torch.cuda.synchronize()
+            spent += time.perf_counter() - t1
+            frame += 1
+            if use_cache and frame <= args.c4_warm:
+                continue
+            r = _rmse_dev(film[0], film[1], ref_img)
+            curve.append((round(spent, 3), round(r, 5)))
+            if r <= target and reached is None:
+                reached = spent
+                break
+        if cache is not None:
+            cache.close()
+        res = {"time_s": None if reached is None else round(reached, 3), "frames": frame,
+               "final_rmse": curve[-1][1] if curve else None, "timed_s": round(spent, 3),
+               "curve": curve[:: max(1, len(curve) // 12)] + curve[-1:]}
+        if losses:
+            res["loss_first_last"] = [round(float(np.mean(losses[:5])), 4), round(float(np.mean(losses[-5:])), 4)]
+        return res
+
+    log("plain path tracing")
+    plain = run(False)
+    log(f"plain: {plain}")
+    log("radiance cache")
+    cached = run(True)
+    log(f"cache: {cached}")
+    tp, tc = plain["time_s"], cached["time_s"]
+    ratio = round(tp / tc, 3) if (tp and tc) else None
+    out = {
+        "metric": METRIC,
+        "value": ratio,
+        "unit": "x (time-to-RMSE, plain / cache)",
+        "n_gpus": 1, "steps": cached["frames"], "warmup": args.c4_warm,
+        "ms_per_step": round(cached["timed_s"] * 1e3 / max(1, cached["frames"]), 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 shading, bf16 MFMA cache, FP64 film", "data": "synthetic",
+        "config": {"workload": label, "triangles": int(desc.idx.shape[0]), "target_rmse": target},
+        "plain": plain, "cache": cached,
+    }
+    sc.close()
+    ctx.close()
+    return out
+
+
+def run_copy(args):
+    """HBM calibration: a plain device-to-device copy of 1 GiB (torch), so the
+    FETCH_SIZE / WRITE_SIZE counters of a kernel of known traffic can scale
+    the traversal kernel's counters.  Not a benchmark of the renderer."""
+    import torch
+    n = 1 << 28  # float32 elements = 1 GiB
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(args.warmup):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gbs = 2 * 4 * n * args.steps / wall / 1e9
+    return {"metric": "HBM copy calibration", "value": round(gbs, 1), "unit": "GB/s (read + write)",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(wall * 1e3 / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "device-to-device copy of 2^28 float32 (1 GiB read, 1 GiB written)"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c2"])
+    ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c2", "c4", "copy"])
+    ap.add_argument("--c4-width", type=int, default=480)
+    ap.add_argument("--c4-height", type=int, default=270)
+    ap.add_argument("--c4-ref-spp", type=int, default=16384)
+    ap.add_argument("--c4-target", type=float, default=1e-2)
+    ap.add_argument("--c4-budget", type=float, default=60.0, help="c4: timed seconds per method at most")
+    ap.add_argument("--c4-warm", type=int, default=20, help="c4: training-only frames before the cache renders")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-split", action="store_true",
                     help="c1: skip the separate coherent/incoherent launches (profiling runs)")
@@ -282,6 +420,10 @@ def main():
         sys.exit(1)
     if args.workload == "c1":
         out = run_c1(args)
+    elif args.workload == "c4":
+        out = run_c4(args)
+    elif args.workload == "copy":
+        out = run_copy(args)
     else:
         out = run_render(args, args.workload)
     print(json.dumps(out), flush=True)

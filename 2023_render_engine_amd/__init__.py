@@ -458,6 +458,50 @@ class Scene:
         return o, d
 
 
+def reference_sums(film, spp_total):
+    """Per-pixel sums as the reference's Film holds them (H*W*3 float64):
+    the valid samples' sum plus the valid mean for every invalid sample,
+    i.e. sum * spp / nvalid (integration/pathTracing.cu:474-486)."""
+    import numpy as np
+    accum, nvalid = film
+    a = accum.double().cpu().numpy().reshape(-1, 3) if hasattr(accum, "cpu") else np.asarray(accum).reshape(-1, 3)
+    n = (nvalid.cpu().numpy() if hasattr(nvalid, "cpu") else np.asarray(nvalid)).astype(np.float64)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return np.where(n[:, None] > 0, a * (spp_total / n[:, None]), 0.0)
+
+
+def write_hdrbmp(path, sums, width, height, samples):
+    """Film::storeContributionsToFile (camera/Film.cu:366-394): writes
+    path + ".hdrbmp" as "P6\\n<w> <h>F\\n<samples>\\n" then "r g b " per pixel
+    with 6 significant digits."""
+    import numpy as np
+    s = np.asarray(sums, np.float64).reshape(-1, 3)
+    if s.shape[0] != width * height:
+        raise ValueError("sums do not match the resolution")
+    with open(path + ".hdrbmp", "w") as f:
+        f.write(f"P6\n{width} {height}F\n{int(samples)}\n")
+        f.write("".join(f"{r:.6g} {g:.6g} {b:.6g} " for r, g, b in s))
+
+
+def read_hdrbmp(path, width=None, height=None):
+    """Film::loadContributionsFromFile (camera/Film.cu:396-449): returns
+    (sums (H*W, 3) float64, samples, width, height).  Raises ValueError on a
+    bad header, a resolution other than the one asked for, or truncated
+    pixel data (the reference prints an error for a mismatch)."""
+    import numpy as np
+    with open(path + ".hdrbmp") as f:
+        tok = f.read().split()
+    if len(tok) < 5 or tok[0] != "P6" or not tok[2].endswith(("F", "f", "d")):
+        raise ValueError(f"{path}.hdrbmp: bad header")
+    w, h, n = int(tok[1]), int(tok[2][:-1]), int(tok[3])
+    if (width is not None and w != width) or (height is not None and h != height):
+        raise ValueError(f"{path}.hdrbmp: resolution {w}x{h} does not match the film")
+    vals = tok[4:]
+    if len(vals) < 3 * w * h:
+        raise ValueError(f"{path}.hdrbmp: truncated pixel data")
+    return np.array(vals[:3 * w * h], np.float64).reshape(-1, 3), n, w, h
+
+
 def film_image(film, spp_total):
     """Mean radiance per pixel (H*W*3 float64 numpy) from FP64 sums and valid
     counts: the reference's pixel sum is sum * (spp / nvalid)

@@ -1,0 +1,495 @@
+// rt.hpp -- C++17 host API over the C ABI of rt.h, with the reference
+// renderer's nouns: Scene, Shape (TriangleMesh, Sphere), Material
+// (Lambertian, Dielectric/Glass, Trowbridge, TrowbridgeBasic, Light),
+// PointLight, ProjectiveCamera, Film, PathTracerIntegrator, BVH.
+//
+// Header-only; link with librt_amd.so.  Every failing C call throws
+// rt_amd::Error carrying the status and rt_last_error().  Device buffers are
+// owned by the objects that allocate them (RAII).
+//
+// Reference counterparts:
+//   Scene::build                      integration/Scene.cu:56-95
+//   PathTracerIntegrator::render      integration/PathTracerIntegrator.cu:146-210
+//   Film::addSample / computeRMSE     camera/Film.cu:111-118, 611-632
+//   Film::{store,load}ContributionsToFile  camera/Film.cu:366-449 (.hdrbmp text)
+//   BVH::intersectRays                cuda_libraries/BVH.cu:541-560
+//   materials and lights              materials/*.cu, light/PointLight.cu
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <fstream>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "rt.h"
+
+namespace rt_amd {
+
+class Error : public std::runtime_error {
+public:
+    Error(rt_status s, const std::string &what) : std::runtime_error(what), status(s) {}
+    rt_status status;
+};
+
+inline void check(rt_status s)
+{
+    if (s != RT_OK) {
+        const char *m = rt_last_error();
+        throw Error(s, m ? m : "rt error");
+    }
+}
+
+struct Spectrum {
+    float r = 0, g = 0, b = 0;
+    Spectrum() = default;
+    Spectrum(float v) : r(v), g(v), b(v) {}
+    Spectrum(float r_, float g_, float b_) : r(r_), g(g_), b(b_) {}
+};
+
+struct Point3f {
+    float x = 0, y = 0, z = 0;
+};
+
+// ---------------------------------------------------------------------
+// context and device memory
+// ---------------------------------------------------------------------
+class Context {
+public:
+    explicit Context(int device = 0) { check(rt_ctx_create(device, &h_)); }
+    ~Context() { if (h_) rt_ctx_destroy(h_); }
+    Context(const Context &) = delete;
+    Context &operator=(const Context &) = delete;
+    rt_ctx *get() const { return h_; }
+    void synchronize() const { check(rt_ctx_synchronize(h_)); }
+
+private:
+    rt_ctx *h_ = nullptr;
+};
+
+class DeviceBuffer {
+public:
+    DeviceBuffer(Context &ctx, size_t bytes) : ctx_(&ctx), bytes_(bytes) { check(rt_malloc(ctx.get(), bytes, &p_)); }
+    ~DeviceBuffer() { if (p_) rt_free(ctx_->get(), p_); }
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    void *get() const { return p_; }
+    size_t size() const { return bytes_; }
+    void upload(const void *src, size_t n) { check(rt_memcpy_h2d(ctx_->get(), p_, src, n)); }
+    void download(void *dst, size_t n) const { check(rt_memcpy_d2h(ctx_->get(), dst, p_, n)); }
+
+private:
+    Context *ctx_;
+    size_t bytes_;
+    void *p_ = nullptr;
+};
+
+// ---------------------------------------------------------------------
+// materials (materials/*.cu)
+// ---------------------------------------------------------------------
+class Material {
+public:
+    virtual ~Material() = default;
+    virtual rt_material desc() const = 0;
+};
+
+// LambertianMaterial(kd) (materials/LambertianMaterial.cu:18)
+class LambertianMaterial : public Material {
+public:
+    explicit LambertianMaterial(const Spectrum &kd) : kd_(kd) {}
+    rt_material desc() const override
+    {
+        rt_material m{};
+        m.type = RT_MAT_LAMBERTIAN;
+        m.kd[0] = kd_.r; m.kd[1] = kd_.g; m.kd[2] = kd_.b;
+        return m;
+    }
+
+private:
+    Spectrum kd_;
+};
+
+// DielectricMaterial(color, ior) (materials/DielectricMaterial.cu:21)
+class DielectricMaterial : public Material {
+public:
+    DielectricMaterial(const Spectrum &color, float ior) : c_(color), ior_(ior) {}
+    rt_material desc() const override
+    {
+        rt_material m{};
+        m.type = RT_MAT_DIELECTRIC;
+        m.kd[0] = c_.r; m.kd[1] = c_.g; m.kd[2] = c_.b;
+        m.ior = ior_;
+        return m;
+    }
+
+private:
+    Spectrum c_;
+    float ior_;
+};
+
+// GlassMaterial(color) = DielectricMaterial(color, 1.5) (materials/GlassMaterial.cu:12-13)
+class GlassMaterial : public DielectricMaterial {
+public:
+    explicit GlassMaterial(const Spectrum &color) : DielectricMaterial(color, 1.5f) {}
+};
+
+// TrowbridgeMaterial(kd, roughness, eta, k): GGX with conductor Fresnel
+// (materials/TrowbridgeMaterial.cu:15); alpha = sqrt(roughness)
+class TrowbridgeMaterial : public Material {
+public:
+    TrowbridgeMaterial(const Spectrum &kd, float roughness, const Spectrum &eta, const Spectrum &k)
+        : kd_(kd), rough_(roughness), eta_(eta), k_(k) {}
+    rt_material desc() const override
+    {
+        rt_material m{};
+        m.type = RT_MAT_TROWBRIDGE;
+        m.kd[0] = kd_.r; m.kd[1] = kd_.g; m.kd[2] = kd_.b;
+        m.roughness = rough_;
+        m.eta[0] = eta_.r; m.eta[1] = eta_.g; m.eta[2] = eta_.b;
+        m.k[0] = k_.r; m.k[1] = k_.g; m.k[2] = k_.b;
+        return m;
+    }
+
+private:
+    Spectrum kd_;
+    float rough_;
+    Spectrum eta_, k_;
+};
+
+// TrowbridgeBasicMaterial(kd, roughness): GGX with F = 1
+// (materials/TrowbridgeBasicMaterial.cu:15)
+class TrowbridgeBasicMaterial : public Material {
+public:
+    TrowbridgeBasicMaterial(const Spectrum &kd, float roughness) : kd_(kd), rough_(roughness) {}
+    rt_material desc() const override
+    {
+        rt_material m{};
+        m.type = RT_MAT_TROWBRIDGE_BASIC;
+        m.kd[0] = kd_.r; m.kd[1] = kd_.g; m.kd[2] = kd_.b;
+        m.roughness = rough_;
+        return m;
+    }
+
+private:
+    Spectrum kd_;
+    float rough_;
+};
+
+// LightMaterial(color, intensity): emissive, NullBSDF (materials/LightMaterial.cu:19)
+class LightMaterial : public Material {
+public:
+    LightMaterial(const Spectrum &color, float intensity) : c_(color), i_(intensity) {}
+    rt_material desc() const override
+    {
+        rt_material m{};
+        m.type = RT_MAT_LIGHT;
+        m.kd[0] = c_.r; m.kd[1] = c_.g; m.kd[2] = c_.b;
+        m.intensity = i_;
+        return m;
+    }
+
+private:
+    Spectrum c_;
+    float i_;
+};
+
+// ---------------------------------------------------------------------
+// shapes and lights
+// ---------------------------------------------------------------------
+class Shape {
+public:
+    explicit Shape(std::shared_ptr<Material> m) : material(std::move(m)) {}
+    virtual ~Shape() = default;
+    std::shared_ptr<Material> material;
+};
+
+// A triangle mesh: xyz (3 floats per vertex) and 3 indices per triangle.
+// Triangle ids are assigned in mesh order, then face order (BVH.cu:288-291).
+class TriangleMesh : public Shape {
+public:
+    TriangleMesh(std::vector<float> xyz, std::vector<uint32_t> idx, std::shared_ptr<Material> m)
+        : Shape(std::move(m)), vertices(std::move(xyz)), indices(std::move(idx)) {}
+    std::vector<float> vertices;
+    std::vector<uint32_t> indices;
+};
+
+// Sphere(center, radius) (shapes/Sphere.cu:233-245).  area_light: also
+// sampled as a sphere light (needs a LightMaterial).
+class Sphere : public Shape {
+public:
+    Sphere(Point3f c, float r, std::shared_ptr<Material> m, bool area_light = false)
+        : Shape(std::move(m)), center(c), radius(r), is_light(area_light) {}
+    Point3f center;
+    float radius;
+    bool is_light;
+};
+
+// PointLight(position, color, intensity) (light/PointLight.cu:21)
+class PointLight {
+public:
+    PointLight(Point3f p, const Spectrum &color, float intensity) : pos(p), c(color), I(intensity) {}
+    Point3f pos;
+    Spectrum c;
+    float I;
+};
+
+// ---------------------------------------------------------------------
+// film (camera/Film.cu)
+// ---------------------------------------------------------------------
+// Per-pixel radiance sums in FP64 and one sample count, as the reference's
+// Film keeps them (the count is tracked for the whole film, Film.cu:112-118).
+class Film {
+public:
+    Film(uint32_t w, uint32_t h) : w_(w), h_(h), sums_((size_t)w * h * 3, 0.0) {}
+    uint32_t width() const { return w_; }
+    uint32_t height() const { return h_; }
+    uint64_t samples() const { return samples_; }
+    std::vector<double> &sums() { return sums_; }
+    const std::vector<double> &sums() const { return sums_; }
+
+    // Film::addSample: adds a pixel's radiance sum over nSamples samples
+    void addSample(uint32_t x, uint32_t y, const double rgb[3], int nSamples)
+    {
+        const size_t i = (size_t)y * w_ + x;
+        sums_[3 * i] += rgb[0]; sums_[3 * i + 1] += rgb[1]; sums_[3 * i + 2] += rgb[2];
+        if (x == 0 && y == 0) samples_ += (uint64_t)nSamples;
+    }
+
+    // Adds a device render (FP64 valid-sample sums + valid counts for spp
+    // samples per pixel) with the reference's pixel estimate: the sum of the
+    // valid samples plus the mean of the valid ones for every invalid sample
+    // (integration/pathTracing.cu:474-486).
+    void addRender(const std::vector<double> &valid_sums, const std::vector<uint32_t> &nvalid, uint32_t spp)
+    {
+        for (size_t i = 0; i < (size_t)w_ * h_; ++i) {
+            const double s = nvalid[i] ? (double)spp / (double)nvalid[i] : 0.0;
+            for (int c = 0; c < 3; ++c) sums_[3 * i + c] += valid_sums[3 * i + c] * s;
+        }
+        samples_ += spp;
+    }
+
+    // storeContributionsToFile (Film.cu:366-394): "P6\n<w> <h>F\n<samples>\n"
+    // then "r g b " per pixel, default stream precision.
+    void storeContributionsToFile(const std::string &name) const
+    {
+        std::ofstream f(name + ".hdrbmp", std::ios::binary);
+        if (!f) throw Error(RT_ERR_INVALID, "cannot write " + name + ".hdrbmp");
+        f << "P6\n" << w_ << " " << h_ << "F\n" << samples_ << "\n";
+        for (size_t i = 0; i < (size_t)w_ * h_; ++i)
+            f << sums_[3 * i] << " " << sums_[3 * i + 1] << " " << sums_[3 * i + 2] << " ";
+        if (!f) throw Error(RT_ERR_INVALID, "short write to " + name + ".hdrbmp");
+    }
+
+    // loadContributionsFromFile (Film.cu:396-449): ADDS the file's sums and
+    // sample count.  Returns false if the file does not exist; throws on a
+    // resolution mismatch or a truncated file.
+    bool loadContributionsFromFile(const std::string &name)
+    {
+        std::ifstream f(name + ".hdrbmp", std::ios::binary);
+        if (!f.is_open()) return false;
+        std::string header;
+        long long w = 0, h = 0, n = 0;
+        char type = 0;
+        f >> header >> w >> h >> type >> n;
+        if (!f || header != "P6") throw Error(RT_ERR_INVALID, name + ".hdrbmp: bad header");
+        if (w != (long long)w_ || h != (long long)h_)
+            throw Error(RT_ERR_INVALID, name + ".hdrbmp: resolution does not match the film");
+        std::vector<double> add((size_t)w_ * h_ * 3);
+        for (double &v : add)
+            if (!(f >> v)) throw Error(RT_ERR_INVALID, name + ".hdrbmp: truncated pixel data");
+        for (size_t i = 0; i < add.size(); ++i) sums_[i] += add[i];
+        samples_ += (uint64_t)n;
+        return true;
+    }
+
+private:
+    uint32_t w_, h_;
+    std::vector<double> sums_;
+    uint64_t samples_ = 0;
+};
+
+// computeRMSE (camera/Film.cu:611-632): sqrt of the mean over pixels of the
+// squared norm (RGB channel sum) of the difference of mean radiances; -1 on a
+// resolution mismatch.  Accumulated in float like the reference.
+inline float computeRMSE(const Film &a, const Film &b)
+{
+    if (a.width() != b.width() || a.height() != b.height()) return -1;
+    const size_t n = (size_t)a.width() * a.height();
+    float acc = 0;
+    for (size_t i = 0; i < n; ++i) {
+        double d2 = 0;
+        for (int c = 0; c < 3; ++c) {
+            const double d = a.sums()[3 * i + c] / (double)a.samples() - b.sums()[3 * i + c] / (double)b.samples();
+            d2 += d * d;
+        }
+        acc += (float)d2;
+    }
+    return std::sqrt(acc / (float)n);
+}
+
+// ProjectiveCamera as loaded from a .obejota file (FileImporting.cu:404-436):
+// position, look-at point, up, camera-to-screen translation cts, screen
+// window height frameY, resolution; owns its film.
+class ProjectiveCamera {
+public:
+    ProjectiveCamera(Point3f pos, Point3f lookat, Point3f up, Point3f cts, float frame_y, uint32_t w, uint32_t h)
+        : film(w, h)
+    {
+        c_.pos[0] = pos.x; c_.pos[1] = pos.y; c_.pos[2] = pos.z;
+        c_.lookat[0] = lookat.x; c_.lookat[1] = lookat.y; c_.lookat[2] = lookat.z;
+        c_.up[0] = up.x; c_.up[1] = up.y; c_.up[2] = up.z;
+        c_.cts[0] = cts.x; c_.cts[1] = cts.y; c_.cts[2] = cts.z;
+        c_.frame_y = frame_y;
+        c_.width = w;
+        c_.height = h;
+    }
+    const rt_camera &desc() const { return c_; }
+    Film film;
+
+private:
+    rt_camera c_{};
+};
+
+// ---------------------------------------------------------------------
+// scene (integration/Scene.cu)
+// ---------------------------------------------------------------------
+class Scene {
+public:
+    explicit Scene(Context &ctx) : ctx_(&ctx) { check(rt_scene_create(ctx.get(), &h_)); }
+    ~Scene() { if (h_) rt_scene_destroy(h_); }
+    Scene(const Scene &) = delete;
+    Scene &operator=(const Scene &) = delete;
+
+    // Scene::build(shapes, lights, ...): materials are registered once per
+    // distinct object; meshes first in the given order (triangle ids follow
+    // it), spheres in the given order (intersection order, Scene.cu:221-227).
+    void build(const std::vector<std::shared_ptr<Shape>> &shapes, const std::vector<PointLight> &lights,
+               const ProjectiveCamera &camera)
+    {
+        std::vector<std::pair<const Material *, uint32_t>> ids;
+        auto mat_id = [&](const std::shared_ptr<Material> &m) {
+            for (auto &p : ids)
+                if (p.first == m.get()) return p.second;
+            rt_material d = m->desc();
+            uint32_t id = 0;
+            check(rt_scene_add_material(h_, &d, &id));
+            ids.emplace_back(m.get(), id);
+            return id;
+        };
+        for (const auto &s : shapes)
+            if (auto *t = dynamic_cast<const TriangleMesh *>(s.get())) {
+                const uint32_t m = mat_id(s->material);
+                check(rt_scene_add_mesh(h_, t->vertices.data(), (uint32_t)(t->vertices.size() / 3), t->indices.data(),
+                                        (uint32_t)(t->indices.size() / 3), m, nullptr));
+            }
+        for (const auto &s : shapes)
+            if (auto *sp = dynamic_cast<const Sphere *>(s.get())) {
+                const uint32_t m = mat_id(s->material);
+                const float c[3] = {sp->center.x, sp->center.y, sp->center.z};
+                check(rt_scene_add_sphere(h_, c, sp->radius, m, sp->is_light ? 1 : 0));
+            }
+        for (const auto &l : lights) {
+            rt_point_light pl{};
+            pl.pos[0] = l.pos.x; pl.pos[1] = l.pos.y; pl.pos[2] = l.pos.z;
+            pl.color[0] = l.c.r; pl.color[1] = l.c.g; pl.color[2] = l.c.b;
+            pl.intensity = l.I;
+            check(rt_scene_add_point_light(h_, &pl));
+        }
+        check(rt_scene_set_camera(h_, &camera.desc()));
+        check(rt_scene_commit(h_));
+    }
+
+    rt_scene *get() const { return h_; }
+    Context &context() const { return *ctx_; }
+
+private:
+    Context *ctx_;
+    rt_scene *h_ = nullptr;
+};
+
+// Render settings of the reference's compile-time macros (types.h:41-42,73-74).
+struct RenderSettings {
+    uint32_t spp = 16, max_depth = 8, rr_depth = 3, seed = 1, spp_offset = 0, path_capacity = 0;
+};
+
+// PathTracerIntegrator(camera).render(scene): renders every pixel at
+// settings.spp samples on the GPU and adds the result to camera->film.
+class PathTracerIntegrator {
+public:
+    explicit PathTracerIntegrator(ProjectiveCamera *camera) : cam_(camera) {}
+
+    void render(Scene &scene, const RenderSettings &s = RenderSettings())
+    {
+        Context &ctx = scene.context();
+        const size_t npix = (size_t)cam_->film.width() * cam_->film.height();
+        DeviceBuffer accum(ctx, npix * 3 * sizeof(double)), nvalid(ctx, npix * sizeof(uint32_t));
+        std::vector<double> zeros(npix * 3, 0.0);
+        std::vector<uint32_t> zc(npix, 0);
+        accum.upload(zeros.data(), zeros.size() * sizeof(double));
+        nvalid.upload(zc.data(), zc.size() * sizeof(uint32_t));
+        rt_render_opts o{};
+        o.spp = s.spp;
+        o.spp_offset = s.spp_offset;
+        o.max_depth = s.max_depth;
+        o.rr_depth = s.rr_depth;
+        o.seed = s.seed;
+        o.path_capacity = s.path_capacity;
+        check(rt_render(scene.get(), &o, (double *)accum.get(), (uint32_t *)nvalid.get(), 0, nullptr));
+        ctx.synchronize();
+        std::vector<double> sums(npix * 3);
+        std::vector<uint32_t> nv(npix);
+        accum.download(sums.data(), sums.size() * sizeof(double));
+        nvalid.download(nv.data(), nv.size() * sizeof(uint32_t));
+        cam_->film.addRender(sums, nv, s.spp);
+    }
+
+private:
+    ProjectiveCamera *cam_;
+};
+
+// ---------------------------------------------------------------------
+// BVH (cuda_libraries/BVH.cu): a tree over a triangle soup and the batch
+// closest-hit call on host arrays (BVH::intersectRays)
+// ---------------------------------------------------------------------
+class BVH {
+public:
+    // ctx may be null: host-only tree (info and export only)
+    BVH(Context *ctx, const std::vector<float> &xyz, const std::vector<uint32_t> &idx, uint32_t max_leaf = 4)
+    {
+        rt_bvh_opts o{max_leaf, 32, 1.2f};
+        check(rt_bvh_build(ctx ? ctx->get() : nullptr, xyz.data(), (uint32_t)(xyz.size() / 3), idx.data(),
+                           (uint32_t)(idx.size() / 3), &o, &h_));
+    }
+    ~BVH() { if (h_) rt_bvh_destroy(h_); }
+    BVH(const BVH &) = delete;
+    BVH &operator=(const BVH &) = delete;
+
+    rt_bvh_info info() const
+    {
+        rt_bvh_info i{};
+        check(rt_bvh_get_info(h_, &i));
+        return i;
+    }
+
+    // SoA host rays (o, d, tMax) -> hit ids (-1 = miss) and distances.
+    void intersectRays(const std::vector<float> &ox, const std::vector<float> &oy, const std::vector<float> &oz,
+                       const std::vector<float> &dx, const std::vector<float> &dy, const std::vector<float> &dz,
+                       const std::vector<float> &tmax, std::vector<int32_t> &hitIds, std::vector<float> &t) const
+    {
+        const size_t n = ox.size();
+        hitIds.resize(n);
+        t.resize(n);
+        rt_rays r{ox.data(), oy.data(), oz.data(), dx.data(), dy.data(), dz.data(), tmax.data()};
+        check(rt_trace_closest(h_, &r, (uint32_t)n, hitIds.data(), t.data(), RT_FLAG_HOST_PTRS, nullptr));
+    }
+
+    rt_bvh *get() const { return h_; }
+
+private:
+    rt_bvh *h_ = nullptr;
+};
+
+}  // namespace rt_amd

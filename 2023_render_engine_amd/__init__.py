@@ -74,6 +74,15 @@ def f3(v):
     return (C.c_float * 3)(*[float(x) for x in v])
 
 
+class BatchOpts(C.Structure):
+    _fields_ = [("spp", C.c_uint32), ("spp_offset", C.c_uint32), ("max_depth", C.c_uint32),
+                ("rr_depth", C.c_uint32), ("path_capacity", C.c_uint32), ("grid", C.c_uint32)]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("path_capacity", C.c_uint32), ("grid", C.c_uint32), ("ms", C.c_float)]
+
+
 class CacheOpts(C.Structure):
     _fields_ = [("levels", C.c_uint32), ("log2_table", C.c_uint32), ("base_res", C.c_uint32),
                 ("per_level_scale", C.c_float), ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
@@ -122,6 +131,12 @@ _SIGS = {
     "rt_render": (C.c_int, [_P, C.POINTER(RenderOpts), _P, _P, _U32, _P]),
     "rt_render_get_stats": (C.c_int, [_P, C.POINTER(RenderStats)]),
     "rt_render_cached": (C.c_int, [_P, C.POINTER(RenderOpts), _P, _U32, _P, _P, _U32, _P]),
+    "rt_batch_create": (C.c_int, [_P, C.POINTER(_P), _U32, C.POINTER(_P)]),
+    "rt_batch_destroy": (C.c_int, [_P]),
+    "rt_batch_render": (C.c_int, [_P, C.POINTER(BatchOpts), _P, C.POINTER(_P), C.POINTER(_P), _U32, _P]),
+    "rt_batch_autotune": (C.c_int, [_P, C.POINTER(BatchOpts), C.POINTER(TuneEntry), C.POINTER(TuneEntry), _U32,
+                                    C.POINTER(_U32)]),
+    "rt_batch_get_stats": (C.c_int, [_P, C.POINTER(RenderStats)]),
     "rt_cache_train_scene": (C.c_int, [_P, _P, C.POINTER(RenderOpts), _U32, _U32, C.POINTER(C.c_float),
                                        C.POINTER(_U32), _P]),
     "rt_cache_create": (C.c_int, [_P, C.POINTER(CacheOpts), C.POINTER(_P)]),
@@ -456,6 +471,61 @@ class Scene:
         d = np.zeros(3, np.float32)
         check(lib().rt_scene_camera_ray(self.h, px, py, u0, u1, _ptr(o), _ptr(d)))
         return o, d
+
+
+RT_FLAG_GRAPH = 2
+
+
+class Batch:
+    """rt_batch: several committed Scenes rendered in one launch sequence
+    with shared ray queues.  Films are the scenes' own (Scene.new_film)."""
+
+    def __init__(self, ctx: Context, scenes):
+        self.ctx = ctx
+        self.scenes = list(scenes)
+        arr = (_P * len(self.scenes))(*[sc.h for sc in self.scenes])
+        h = _P()
+        check(lib().rt_batch_create(ctx.h, arr, len(self.scenes), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, films, spp, spp_offset=0, max_depth=12, rr_depth=3, seeds=None, path_capacity=0, grid=0,
+               graph=False, stream=None):
+        import numpy as np
+        o = BatchOpts(spp, spp_offset, max_depth, rr_depth, path_capacity, grid)
+        n = len(self.scenes)
+        acc = (_P * n)(*[_ptr(f[0]) for f in films])
+        nv = (_P * n)(*[_ptr(f[1]) for f in films])
+        sd = None if seeds is None else np.ascontiguousarray(seeds, np.uint32)
+        flags = RT_FLAG_GRAPH if graph else 0
+        self.ctx.launch(lambda s: check(lib().rt_batch_render(self.h, C.byref(o), None if sd is None else _ptr(sd),
+                                                              acc, nv, flags, s)), stream)
+        return films
+
+    def autotune(self, spp=1, max_depth=12, rr_depth=3):
+        """Returns (best TuneEntry, [TuneEntry ...]) and makes best the default."""
+        o = BatchOpts(spp, 0, max_depth, rr_depth, 0, 0)
+        best = TuneEntry()
+        ents = (TuneEntry * 64)()
+        cnt = C.c_uint32(0)
+        self.ctx.fence_torch()
+        check(lib().rt_batch_autotune(self.h, C.byref(o), C.byref(best), ents, 64, C.byref(cnt)))
+        return best, [ents[i] for i in range(cnt.value)]
+
+    def stats(self) -> RenderStats:
+        st = RenderStats()
+        check(lib().rt_batch_get_stats(self.h, C.byref(st)))
+        return st
 
 
 def reference_sums(film, spp_total):

@@ -1,0 +1,390 @@
+// batch.hip -- many scenes in one launch sequence (rt_batch_*): a device
+// table of the scenes, one wave stream over the concatenation of their
+// sample index spaces, graph capture of whole frames and the autotuner of
+// wave size and grid.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "rt_internal.h"
+#include "rt_render.h"
+#include "rt_waves.h"
+
+struct rt_ctx;
+namespace rt {
+void ctx_retain(rt_ctx *c);
+void ctx_release(rt_ctx *c);
+int ctx_device(rt_ctx *c);
+void *ctx_stream_ptr(rt_ctx *c);
+std::recursive_mutex &ctx_lock(rt_ctx *c);
+bool scene_committed(const rt_scene *s);
+rt_ctx *scene_ctx(const rt_scene *s);
+const SceneDev &scene_device(const rt_scene *s);
+uint64_t scene_stream_base(const rt_scene *s);
+}  // namespace rt
+
+using rt::set_error;
+
+#define RT_HIP(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            set_error(std::string(#call) + " failed: " + hipGetErrorString(e_));        \
+            return e_ == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;                 \
+        }                                                                               \
+    } while (0)
+
+namespace {
+
+// What a captured frame depends on besides the device-side sample offset
+// and scene table.
+struct GraphKey {
+    uint32_t spp = 0, max_depth = 0, rr_depth = 0, capacity = 0, grid = 0;
+    std::vector<rt::FilmDev> films;
+    bool operator==(const GraphKey &o) const
+    {
+        if (spp != o.spp || max_depth != o.max_depth || rr_depth != o.rr_depth || capacity != o.capacity ||
+            grid != o.grid || films.size() != o.films.size())
+            return false;
+        for (size_t i = 0; i < films.size(); ++i)
+            if (films[i].accum != o.films[i].accum || films[i].nvalid != o.films[i].nvalid) return false;
+        return true;
+    }
+};
+
+}  // namespace
+
+struct rt_batch {
+    rt_ctx *ctx = nullptr;
+    std::vector<rt_scene *> scenes;
+    std::vector<rt::SceneDev> host_table;   // as uploaded (seeds included)
+    std::vector<uint32_t> npix;
+    uint32_t max_depth = 0, max_lights = 1;
+    uint64_t paths_per_sample = 0;           // sum of pixels
+    rt::SceneDev *d_table = nullptr;
+    uint64_t *d_bases = nullptr;             // per call: n + 1 path-index bases
+    uint32_t *d_spp_offset = nullptr;
+    std::vector<uint64_t> bases;             // host copy for the current spp
+    uint32_t bases_spp = 0;
+    rt::WaveBuffers wb;
+    uint32_t tuned_capacity = 0, tuned_grid = 0;
+    // graph of the last captured frame
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    GraphKey key;
+    rt_render_stats stats{};
+};
+
+namespace {
+
+void drop_graph(rt_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->key = GraphKey{};
+}
+
+// Uploads the scene table when the seeds change (seeds live in the table).
+rt_status upload_table(rt_batch *b, const uint32_t *seeds, hipStream_t s)
+{
+    bool changed = false;
+    for (size_t i = 0; i < b->scenes.size(); ++i) {
+        const uint32_t sd = seeds ? seeds[i] : 1u;
+        if (b->host_table[i].seed != sd) {
+            b->host_table[i].seed = sd;
+            changed = true;
+        }
+    }
+    if (changed || !b->d_table) {
+        if (!b->d_table) RT_HIP(hipMalloc(&b->d_table, b->host_table.size() * sizeof(rt::SceneDev)));
+        RT_HIP(hipMemcpyAsync(b->d_table, b->host_table.data(), b->host_table.size() * sizeof(rt::SceneDev),
+                              hipMemcpyHostToDevice, s));
+        RT_HIP(hipStreamSynchronize(s));
+    }
+    return RT_OK;
+}
+
+rt_status upload_bases(rt_batch *b, uint32_t spp, hipStream_t s)
+{
+    if (b->bases_spp == spp && !b->bases.empty()) return RT_OK;
+    const size_t n = b->scenes.size();
+    b->bases.assign(n + 1, 0);
+    for (size_t i = 0; i < n; ++i) b->bases[i + 1] = b->bases[i] + (uint64_t)b->npix[i] * spp;
+    RT_HIP(hipMemcpyAsync(b->d_bases, b->bases.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    RT_HIP(hipStreamSynchronize(s));
+    b->bases_spp = spp;
+    return RT_OK;
+}
+
+// Launches every wave of one frame (kernel launches only: capturable).
+hipError_t launch_frame(rt_batch *b, const rt_batch_opts &o, uint32_t P, uint32_t grid,
+                        const std::vector<rt::FilmDev> &films, hipStream_t s)
+{
+    rt::BatchLaunch bl{};
+    bl.dev = rt::BatchDev{b->d_table, b->d_bases, (uint32_t)b->scenes.size()};
+    bl.bases = b->bases.data();
+    bl.npix = b->npix.data();
+    bl.films = films.data();
+    bl.max_depth = b->max_depth;
+    const uint64_t total = b->bases.back();
+    for (uint64_t first = 0; first < total; first += P) {
+        rt::WaveParams wp{};
+        wp.first_path = first;
+        wp.count = (uint32_t)std::min<uint64_t>(P, total - first);
+        wp.npix = 0;
+        wp.spp_offset = o.spp_offset;
+        wp.seed = 0;
+        wp.max_depth = o.max_depth;
+        wp.rr_depth = o.rr_depth;
+        wp.stream_base = 0;
+        wp.cache_depth = rt::kNoCache;
+        wp.train = 0;
+        wp.grid = grid;
+        wp.spp_offset_dev = b->d_spp_offset;
+        hipError_t e = rt::render_wave(b->host_table[0], b->wb.paths, b->wb.shadow, b->wb.queues, rt::FilmDev{},
+                                       wp, P, s, nullptr, nullptr, &bl);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+rt_status check_opts(const char *fn, const rt_batch *b, const rt_batch_opts *o)
+{
+    const std::string name(fn);
+    if (!b || !o) { set_error(name + ": null argument"); return RT_ERR_INVALID; }
+    if (o->spp == 0) { set_error(name + ": spp must be >= 1"); return RT_ERR_INVALID; }
+    if (o->max_depth == 0 || o->max_depth > 0xffffu) { set_error(name + ": max_depth must be 1..65535"); return RT_ERR_INVALID; }
+    if (o->path_capacity > (1u << 26)) { set_error(name + ": path_capacity above 2^26"); return RT_ERR_INVALID; }
+    if (o->grid > 65536) { set_error(name + ": grid above 65536"); return RT_ERR_INVALID; }
+    return RT_OK;
+}
+
+uint32_t wave_size(const rt_batch *b, const rt_batch_opts &o, uint64_t total)
+{
+    uint32_t P = o.path_capacity ? o.path_capacity : (b->tuned_capacity ? b->tuned_capacity : (1u << 21));
+    if ((uint64_t)P > total) P = (uint32_t)std::max<uint64_t>(1, total);
+    return P;
+}
+
+}  // namespace
+
+extern "C" {
+
+rt_status rt_batch_create(rt_ctx *ctx, rt_scene *const *scenes, uint32_t n, rt_batch **out)
+{
+    if (!ctx || !scenes || !out || n == 0) { set_error("rt_batch_create: null argument or no scenes"); return RT_ERR_INVALID; }
+    if (n > 4096) { set_error("rt_batch_create: more than 4096 scenes"); return RT_ERR_INVALID; }
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!scenes[i]) { set_error("rt_batch_create: null scene"); return RT_ERR_INVALID; }
+        if (!rt::scene_committed(scenes[i])) { set_error("rt_batch_create: scene " + std::to_string(i) + " not committed"); return RT_ERR_INVALID; }
+        if (rt::scene_ctx(scenes[i]) != ctx) { set_error("rt_batch_create: scene " + std::to_string(i) + " belongs to another context"); return RT_ERR_INVALID; }
+    }
+    std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(ctx));
+    RT_HIP(hipSetDevice(rt::ctx_device(ctx)));
+    rt_batch *b = new (std::nothrow) rt_batch();
+    if (!b) { set_error("rt_batch_create: host allocation failed"); return RT_ERR_OOM; }
+    b->ctx = ctx;
+    b->scenes.assign(scenes, scenes + n);
+    for (uint32_t i = 0; i < n; ++i) {
+        rt::SceneDev d = rt::scene_device(scenes[i]);
+        d.seed = 0xffffffffu;  // forces the first upload
+        d.stream_base = rt::scene_stream_base(scenes[i]);
+        b->host_table.push_back(d);
+        b->npix.push_back(d.cam.width * d.cam.height);
+        b->max_depth = std::max(b->max_depth, d.bvh.max_depth);
+        b->max_lights = std::max(b->max_lights, d.nlights);
+        b->paths_per_sample += (uint64_t)d.cam.width * d.cam.height;
+    }
+    if (rt::extend_lds_bytes(b->max_depth) > 64 * 1024) {
+        delete b;
+        set_error("rt_batch_create: BVH too deep");
+        return RT_ERR_UNSUPPORTED;
+    }
+    hipError_t e = hipMalloc(&b->d_bases, (n + 1) * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc(&b->d_spp_offset, sizeof(uint32_t));
+    if (e != hipSuccess) {
+        if (b->d_bases) (void)hipFree(b->d_bases);
+        delete b;
+        set_error(std::string("rt_batch_create: ") + hipGetErrorString(e));
+        return RT_ERR_OOM;
+    }
+    rt::ctx_retain(ctx);
+    *out = b;
+    return RT_OK;
+}
+
+rt_status rt_batch_destroy(rt_batch *b)
+{
+    if (!b) return RT_OK;
+    rt_ctx *ctx = b->ctx;
+    {
+        std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(ctx));
+        (void)hipSetDevice(rt::ctx_device(ctx));
+        (void)hipStreamSynchronize((hipStream_t)rt::ctx_stream_ptr(ctx));
+        drop_graph(b);
+        rt::free_wave_buffers(b->wb);
+        if (b->d_table) (void)hipFree(b->d_table);
+        if (b->d_bases) (void)hipFree(b->d_bases);
+        if (b->d_spp_offset) (void)hipFree(b->d_spp_offset);
+        delete b;
+    }
+    rt::ctx_release(ctx);
+    return RT_OK;
+}
+
+rt_status rt_batch_render(rt_batch *b, const rt_batch_opts *opts, const uint32_t *seeds, double *const *accum,
+                          uint32_t *const *nvalid, uint32_t flags, void *stream_)
+{
+    rt_status st = check_opts("rt_batch_render", b, opts);
+    if (st != RT_OK) return st;
+    if (!accum || !nvalid) { set_error("rt_batch_render: null film arrays"); return RT_ERR_INVALID; }
+    if (flags & RT_FLAG_HOST_PTRS) { set_error("rt_batch_render: films must be device pointers"); return RT_ERR_UNSUPPORTED; }
+    const size_t n = b->scenes.size();
+    std::vector<rt::FilmDev> films(n);
+    for (size_t i = 0; i < n; ++i) {
+        if (!accum[i] || !nvalid[i]) { set_error("rt_batch_render: null film of scene " + std::to_string(i)); return RT_ERR_INVALID; }
+        films[i] = rt::FilmDev{accum[i], nvalid[i]};
+    }
+    std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(b->ctx));
+    RT_HIP(hipSetDevice(rt::ctx_device(b->ctx)));
+    hipStream_t s = stream_ ? (hipStream_t)stream_ : (hipStream_t)rt::ctx_stream_ptr(b->ctx);
+    const uint64_t total = b->paths_per_sample * opts->spp;
+    const uint32_t P = wave_size(b, *opts, total);
+    const uint32_t grid = opts->grid ? opts->grid : (b->tuned_grid ? b->tuned_grid : 2048);
+    st = rt::ensure_wave_buffers(b->wb, P, b->max_lights, "rt_batch_render");
+    if (st != RT_OK) return st;
+    st = upload_table(b, seeds, s);
+    if (st != RT_OK) return st;
+    st = upload_bases(b, opts->spp, s);
+    if (st != RT_OK) return st;
+    auto t0 = std::chrono::steady_clock::now();
+    RT_HIP(hipMemsetAsync(b->wb.queues.stats, 0, 16, s));
+    RT_HIP(rt::set_u32(b->d_spp_offset, opts->spp_offset, s));
+    if (flags & RT_FLAG_GRAPH) {
+        GraphKey k;
+        k.spp = opts->spp; k.max_depth = opts->max_depth; k.rr_depth = opts->rr_depth;
+        k.capacity = P; k.grid = grid; k.films = films;
+        if (!b->exec || !(k == b->key)) {
+            drop_graph(b);
+            RT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            hipError_t e = launch_frame(b, *opts, P, grid, films, s);
+            hipGraph_t gr = nullptr;
+            hipError_t e2 = hipStreamEndCapture(s, &gr);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (gr) (void)hipGraphDestroy(gr);
+                set_error(std::string("rt_batch_render: graph capture failed: ") +
+                          hipGetErrorString(e != hipSuccess ? e : e2));
+                return RT_ERR_HIP;
+            }
+            b->graph = gr;
+            RT_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+            b->key = k;
+        }
+        RT_HIP(hipGraphLaunch(b->exec, s));
+    } else {
+        RT_HIP(launch_frame(b, *opts, P, grid, films, s));
+    }
+    unsigned long long counts[2];
+    RT_HIP(hipMemcpyAsync(counts, b->wb.queues.stats, sizeof counts, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    b->stats.paths = total;
+    b->stats.extension_rays = counts[0];
+    b->stats.shadow_rays = counts[1];
+    b->stats.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return RT_OK;
+}
+
+rt_status rt_batch_autotune(rt_batch *b, const rt_batch_opts *opts, rt_tune_entry *best, rt_tune_entry *entries,
+                            uint32_t max_entries, uint32_t *n_entries)
+{
+    rt_status st = check_opts("rt_batch_autotune", b, opts);
+    if (st != RT_OK) return st;
+    if (n_entries) *n_entries = 0;
+    std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(b->ctx));
+    RT_HIP(hipSetDevice(rt::ctx_device(b->ctx)));
+    hipStream_t s = (hipStream_t)rt::ctx_stream_ptr(b->ctx);
+    const size_t n = b->scenes.size();
+    // scratch films
+    std::vector<rt::FilmDev> films(n);
+    size_t bytes = 0;
+    for (size_t i = 0; i < n; ++i) bytes += (size_t)b->npix[i] * (24 + 4) + 512;
+    char *scratch = nullptr;
+    RT_HIP(hipMalloc(&scratch, bytes));
+    size_t off = 0;
+    for (size_t i = 0; i < n; ++i) {
+        films[i].accum = (double *)(scratch + off);
+        off += ((size_t)b->npix[i] * 24 + 255) & ~(size_t)255;
+        films[i].nvalid = (uint32_t *)(scratch + off);
+        off += ((size_t)b->npix[i] * 4 + 255) & ~(size_t)255;
+    }
+    rt_batch_opts o = *opts;
+    o.spp = 1;
+    o.spp_offset = 0x40000000u;  // far from rendering samples
+    const uint64_t total = b->paths_per_sample;
+    std::vector<uint32_t> caps;
+    for (uint32_t P = 1u << 19; P <= (1u << 22); P <<= 1) {
+        const uint32_t c = (uint32_t)std::min<uint64_t>(P, total);
+        if (caps.empty() || caps.back() != c) caps.push_back(c);
+    }
+    const uint32_t grids[3] = {1024, 2048, 4096};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    rt_tune_entry top{0, 0, 1e30f};
+    uint32_t count = 0;
+    rt_status result = RT_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) result = RT_ERR_HIP;
+    if (result == RT_OK) result = upload_table(b, nullptr, s);
+    if (result == RT_OK) result = upload_bases(b, 1, s);
+    for (uint32_t P : caps) {
+        if (result != RT_OK) break;
+        result = rt::ensure_wave_buffers(b->wb, P, b->max_lights, "rt_batch_autotune");
+        for (uint32_t gsz : grids) {
+            if (result != RT_OK) break;
+            float ms_best = 1e30f;
+            for (int rep = 0; rep < 2 && result == RT_OK; ++rep) {
+                if (rt::set_u32(b->d_spp_offset, o.spp_offset + (uint32_t)rep, s) != hipSuccess ||
+                    hipEventRecord(e0, s) != hipSuccess || launch_frame(b, o, P, gsz, films, s) != hipSuccess ||
+                    hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                    set_error("rt_batch_autotune: trial launch failed");
+                    result = RT_ERR_HIP;
+                    break;
+                }
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                ms_best = std::min(ms_best, ms);
+            }
+            if (result != RT_OK) break;
+            if (entries && count < max_entries) entries[count] = rt_tune_entry{P, gsz, ms_best};
+            ++count;
+            if (ms_best < top.ms) top = rt_tune_entry{P, gsz, ms_best};
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(scratch);
+    b->bases_spp = 0;  // bases were set up for spp = 1
+    if (result != RT_OK) return result;
+    b->tuned_capacity = top.path_capacity;
+    b->tuned_grid = top.grid;
+    drop_graph(b);
+    if (best) *best = top;
+    if (n_entries) *n_entries = std::min(count, max_entries);
+    return RT_OK;
+}
+
+rt_status rt_batch_get_stats(const rt_batch *b, rt_render_stats *st)
+{
+    if (!b || !st) { set_error("rt_batch_get_stats: null argument"); return RT_ERR_INVALID; }
+    *st = b->stats;
+    return RT_OK;
+}
+
+}  // extern "C"

@@ -47,11 +47,34 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred)
     return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
-__device__ __forceinline__ Pcg32 load_rng(const PathsDev &p, uint32_t slot, const WaveParams &wp)
+__device__ __forceinline__ uint32_t wave_max(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
+    return v;
+}
+
+// The scene of path `slot`: the kernel's scene, or the batch table entry.
+template <bool B>
+__device__ __forceinline__ const SceneDev &scene_of(const SceneDev &sc0, const BatchDev &bt, const PathsDev &p,
+                                                    uint32_t slot)
+{
+    if constexpr (B) return bt.scenes[p.sid[slot]];
+    else return sc0;
+}
+
+template <bool B>
+__device__ __forceinline__ uint64_t stream_base_of(const SceneDev &sc, const WaveParams &wp)
+{
+    if constexpr (B) return sc.stream_base;
+    else return wp.stream_base;
+}
+
+__device__ __forceinline__ Pcg32 load_rng(const PathsDev &p, uint32_t slot, uint64_t stream_base)
 {
     Pcg32 g;
     g.state = p.rng[slot];
-    g.inc = ((wp.stream_base + p.pix[slot]) << 1u) | 1u;
+    g.inc = ((stream_base + p.pix[slot]) << 1u) | 1u;
     return g;
 }
 
@@ -71,23 +94,44 @@ __global__ __launch_bounds__(kBlock) void k_reset(uint32_t *counters, uint32_t c
     }
 }
 
-// generate: camera rays for slots [0, count)
-__global__ __launch_bounds__(kBlock) void k_generate(SceneDev sc, PathsDev p, QueuesDev q, WaveParams wp)
+// generate: camera rays for slots [0, count).  Global path index g is
+// sample * npix + pixel within its scene (a batch concatenates its scenes'
+// index ranges).
+template <bool B>
+__global__ __launch_bounds__(kBlock) void k_generate(SceneDev sc0, BatchDev bt, PathsDev p, QueuesDev q,
+                                                     WaveParams wp)
 {
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < wp.count; i += gridDim.x * kBlock) {
-        const uint64_t g = wp.first_path + i;
-        uint32_t sample = wp.spp_offset + (uint32_t)(g / wp.npix);
-        uint32_t pix = (uint32_t)(g % wp.npix);
-        if (wp.train) {
+        uint64_t g = wp.first_path + i;
+        uint32_t sid = 0;
+        if constexpr (B) {
+            uint32_t lo = 0, hi = bt.n;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (bt.bases[mid] <= g) lo = mid;
+                else hi = mid;
+            }
+            sid = lo;
+            g -= bt.bases[sid];
+            p.sid[i] = sid;
+        }
+        const SceneDev &sc = B ? bt.scenes[sid] : sc0;
+        const uint32_t npix = B ? sc.cam.width * sc.cam.height : wp.npix;
+        const uint32_t seed = B ? sc.seed : wp.seed;
+        const uint64_t stream_base = stream_base_of<B>(sc, wp);
+        const uint32_t first_sample = wp.spp_offset_dev ? *wp.spp_offset_dev : wp.spp_offset;
+        uint32_t sample = first_sample + (uint32_t)(g / npix);
+        uint32_t pix = (uint32_t)(g % npix);
+        if (!B && wp.train) {
             // training paths: pseudo-random pixels, sample indices disjoint
             // from the rendering samples (top bit set)
             Pcg32 h = pcg32_seed(g, 0x7a11u);
-            pix = pcg32_next(h) % wp.npix;
+            pix = pcg32_next(h) % npix;
             sample = 0x80000000u | (uint32_t)g;
         }
         const uint32_t px = pix % sc.cam.width, py = pix / sc.cam.width;
-        const uint64_t initstate = ((uint64_t)wp.seed << 32) ^ (uint64_t)sample ^ 0x853c49e6748fea9bULL;
-        Pcg32 r = pcg32_seed(initstate, wp.stream_base + pix);
+        const uint64_t initstate = ((uint64_t)seed << 32) ^ (uint64_t)sample ^ 0x853c49e6748fea9bULL;
+        Pcg32 r = pcg32_seed(initstate, stream_base + pix);
         const float u0 = pcg32_float(r), u1 = pcg32_float(r);
         V3 o, d;
         camera_ray(sc.cam, px, py, u0, u1, o, d);
@@ -130,14 +174,13 @@ __device__ __forceinline__ uint32_t classify(const SceneDev &sc, int32_t hid, bo
     return kQGgx;
 }
 
-__global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc, PathsDev p, QueuesDev q, uint32_t cur, int cap,
-                                                   uint32_t cache_depth)
+template <bool B>
+__global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc0, BatchDev bt, PathsDev p, QueuesDev q, uint32_t cur,
+                                                   int cap, uint32_t cache_depth)
 {
     extern __shared__ int32_t s_stack[];
     int32_t *stack = s_stack + threadIdx.x;
     const uint32_t n = q.counters[kCActive0 + cur];
-    const float4 *nodes = (const float4 *)sc.bvh.nodes;
-    const float4 *tris = (const float4 *)sc.bvh.tris;
     // every lane of a wavefront runs the same number of iterations so the
     // ballots below see the whole wavefront
     const uint32_t stride = gridDim.x * kBlock;
@@ -148,8 +191,12 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc, PathsDev p, Queu
         uint32_t slot = 0;
         int32_t hid = -1;
         float th = INFINITY;
+        uint32_t cls = 0xffu;
         if (live) {
             slot = q.active[cur][i];
+            const SceneDev &sc = scene_of<B>(sc0, bt, p, slot);
+            const float4 *nodes = (const float4 *)sc.bvh.nodes;
+            const float4 *tris = (const float4 *)sc.bvh.tris;
             const float ox = p.ox[slot], oy = p.oy[slot], oz = p.oz[slot];
             const float dx = p.dx[slot], dy = p.dy[slot], dz = p.dz[slot];
             const TriRay tr = make_tri_ray(ox, oy, oz, dx, dy, dz, INFINITY);
@@ -209,9 +256,9 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc, PathsDev p, Queu
             }
             p.hid[slot] = hid;
             p.ht[slot] = th;
+            const bool deep = cache_depth != kNoCache && (p.meta[slot] & 0xffffu) >= cache_depth;
+            cls = classify(sc, hid, deep);
         }
-        const bool deep = live && cache_depth != kNoCache && (p.meta[slot] & 0xffffu) >= cache_depth;
-        const uint32_t cls = live ? classify(sc, hid, deep) : 0xffu;
 #pragma unroll
         for (uint32_t k = 0; k < kNumQ; ++k) {
             const uint32_t pos = wave_append(&q.counters[kCQueue + k], cls == k);
@@ -283,12 +330,12 @@ __device__ __forceinline__ void surface_of(const SceneDev &sc, const PathsDev &p
 
 // Adds the previous bounce's unoccluded next-event contributions, in light
 // order, times the throughput they were computed with.
-__device__ __forceinline__ void resolve_nee(const SceneDev &sc, const PathsDev &p, uint32_t slot)
+__device__ __forceinline__ void resolve_nee(const PathsDev &p, uint32_t slot)
 {
     uint32_t mask = p.nee_mask[slot];
     if (mask == 0) return;
     V3 Ld = mk(0, 0, 0);
-    const float *c = p.nee_c + (size_t)slot * sc.nlights * 3;
+    const float *c = p.nee_c + (size_t)slot * p.nee_stride * 3;
     while (mask) {
         const uint32_t l = __ffs(mask) - 1;
         mask &= mask - 1;
@@ -313,8 +360,8 @@ __device__ __forceinline__ void write_record(float *r, const Surface &s, V3 d, c
     r[10] = 0; r[11] = 0;
 }
 
-template <uint32_t CLS>
-__global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathsDev p, ShadowDev sh, QueuesDev q,
+template <uint32_t CLS, bool B>
+__global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, PathsDev p, ShadowDev sh, QueuesDev q,
                                                   WaveParams wp, uint32_t cur)
 {
     constexpr bool kScatter = CLS == kQLambert || CLS == kQGgx;
@@ -323,12 +370,13 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathsDev p, Shado
     const uint32_t nxt = cur ^ 1u;
     const uint32_t stride = gridDim.x * kBlock;
     // Every lane of a wavefront runs the same iterations (and the same light
-    // loops, whose trip counts are kernel arguments) so the ballots in
-    // wave_append see whole wavefronts.
+    // loops: kernel arguments, or in a batch the wavefront's maximum) so the
+    // ballots in wave_append see whole wavefronts.
     for (uint32_t i0 = blockIdx.x * kBlock; i0 < n; i0 += stride) {
         const uint32_t i = i0 + threadIdx.x;
         const bool live = i < n;
         const uint32_t slot = live ? queue[i] : 0;
+        const SceneDev &sc = scene_of<B>(sc0, bt, p, slot);
         bool alive = false;
         Surface s{};
         V3 d = mk(0, 0, 1), t = mk(1, 0, 0), b = mk(0, 1, 0), wo = mk(0, 0, 1);
@@ -337,11 +385,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathsDev p, Shado
         uint32_t meta = 0, depth = 0;
         V3 beta = mk(0, 0, 0), L = mk(0, 0, 0);
         if (live) {
-            resolve_nee(sc, p, slot);
+            resolve_nee(p, slot);
             if (CLS != kQMiss) {
                 surface_of(sc, p, slot, s, d);
                 m = sc.mat[s.mat];
-                g = load_rng(p, slot, wp);
+                g = load_rng(p, slot, stream_base_of<B>(sc, wp));
                 meta = p.meta[slot];
                 depth = meta & 0xffffu;
                 beta = mk(p.bx[slot], p.by[slot], p.bz[slot]);
@@ -352,7 +400,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathsDev p, Shado
                 // training paths remember their first cacheable vertex at or
                 // beyond the cache depth: the radiance gathered from here on,
                 // divided by the throughput on arrival, is the cache's target
-                if (kScatter && wp.train && depth >= wp.cache_depth && p.tvalid[slot] == 0 && cacheable(m)) {
+                if (!B && kScatter && wp.train && depth >= wp.cache_depth && p.tvalid[slot] == 0 && cacheable(m)) {
                     write_record(p.tin + (size_t)slot * 12, s, d, m);
                     p.tLb[3 * (size_t)slot] = L.x; p.tLb[3 * (size_t)slot + 1] = L.y; p.tLb[3 * (size_t)slot + 2] = L.z;
                     p.tB[3 * (size_t)slot] = beta.x; p.tB[3 * (size_t)slot + 1] = beta.y; p.tB[3 * (size_t)slot + 2] = beta.z;
@@ -363,12 +411,15 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathsDev p, Shado
         if (kScatter) {
             // next-event estimation over every point light (Scene.cu:468-525)
             uint32_t emit = 0;
-            float *c = p.nee_c + (size_t)slot * sc.nlights * 3;
-            for (uint32_t l = 0; l < sc.npl; ++l) {
+            float *c = p.nee_c + (size_t)slot * p.nee_stride * 3;
+            const uint32_t npl = live ? sc.npl : 0u, nal = live ? sc.nlights - sc.npl : 0u;
+            const uint32_t npl_w = B ? wave_max(npl) : sc0.npl;
+            const uint32_t nal_w = B ? wave_max(nal) : sc0.nlights - sc0.npl;
+            for (uint32_t l = 0; l < npl_w; ++l) {
                 bool e = false;
                 V3 Lw = mk(0, 0, 1);
                 float dist = 0;
-                if (live) {
+                if (live && l < npl) {
                     const rt_point_light &pl = sc.pl[l];
                     const V3 lp = mk(pl.pos[0], pl.pos[1], pl.pos[2]);
                     const V3 tol = sub(lp, s.p);
@@ -401,14 +452,14 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, PathsDev p, Shado
                     sh.skip[pos] = -1;
                 }
             }
-            const uint32_t nal = sc.nlights - sc.npl;
-            for (uint32_t a = 0; a < nal; ++a) {
-                const uint32_t j = sc.area_light[a];
-                const uint32_t l = sc.npl + a;
+            for (uint32_t a = 0; a < nal_w; ++a) {
+                uint32_t j = 0, l = 0;
                 bool e = false;
                 V3 Lw = mk(0, 0, 1);
                 float dist = 0;
-                if (live) {
+                if (live && a < nal) {
+                    j = sc.area_light[a];
+                    l = sc.npl + a;
                     const float a0 = pcg32_float(g), a1 = pcg32_float(g);
                     float plp;
                     if (sphere_light_sample(sc.sph[j], s.p, a0, a1, Lw, dist, plp) && dot(Lw, s.n) > 0) {
@@ -510,7 +561,7 @@ __global__ __launch_bounds__(kBlock) void k_cache_vertex(SceneDev sc, PathsDev p
         Surface s{};
         V3 d = mk(0, 0, 1);
         if (live) {
-            resolve_nee(sc, p, slot);
+            resolve_nee(p, slot);
             surface_of(sc, p, slot, s, d);
         }
         const uint32_t k = wave_append(&q.counters[kCCacheQ], live);
@@ -526,14 +577,18 @@ __global__ __launch_bounds__(kBlock) void k_cache_vertex(SceneDev sc, PathsDev p
 // ---------------------------------------------------------------------
 // connect: any hit for shadow rays (Scene.cu:108-150, 238-276)
 // ---------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc, PathsDev p, ShadowDev sh, QueuesDev q, int cap)
+template <bool B>
+__global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc0, BatchDev bt, PathsDev p, ShadowDev sh, QueuesDev q,
+                                                    int cap)
 {
     extern __shared__ int32_t s_stack[];
     int32_t *stack = s_stack + threadIdx.x;
     const uint32_t n = q.counters[kCShadow];
-    const float4 *nodes = (const float4 *)sc.bvh.nodes;
-    const float4 *tris = (const float4 *)sc.bvh.tris;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const uint32_t sl = sh.slot[i];
+        const SceneDev &sc = scene_of<B>(sc0, bt, p, sl >> 5);
+        const float4 *nodes = (const float4 *)sc.bvh.nodes;
+        const float4 *tris = (const float4 *)sc.bvh.tris;
         const float ox = sh.ox[i], oy = sh.oy[i], oz = sh.oz[i];
         const float dx = sh.dx[i], dy = sh.dy[i], dz = sh.dz[i];
         const float tmax = sh.tmax[i];
@@ -594,10 +649,7 @@ __global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc, PathsDev p, Sha
                 }
             }
         }
-        if (occ) {
-            const uint32_t sl = sh.slot[i];
-            atomicAnd(&p.nee_mask[sl >> 5], ~(1u << (sl & 31u)));
-        }
+        if (occ) atomicAnd(&p.nee_mask[sl >> 5], ~(1u << (sl & 31u)));
     }
 }
 
@@ -605,13 +657,12 @@ __global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc, PathsDev p, Sha
 // finalize: terminated paths take their last next-event contributions; the
 // radiance stays in the path slot until the wave's film pass.
 // ---------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_finalize(SceneDev sc, PathsDev p, QueuesDev q, uint32_t which,
-                                                     uint32_t cur)
+__global__ __launch_bounds__(kBlock) void k_finalize(PathsDev p, QueuesDev q, uint32_t which, uint32_t cur)
 {
     const uint32_t n = which ? q.counters[kCActive0 + cur] : q.counters[kCTerm];
     const uint32_t *list = which ? q.active[cur] : q.term;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
-        resolve_nee(sc, p, list[i]);
+        resolve_nee(p, list[i]);
 }
 
 // L += beta * cache(record) for every cache query of the wave
@@ -663,15 +714,17 @@ __global__ __launch_bounds__(kBlock) void k_train_targets(PathsDev p, QueuesDev 
 // (pathTracing.cu:474-486): only finite samples are summed and counted (the
 // consumer rescales by samples / nvalid).  One thread per pixel sums that
 // pixel's samples of the wave in sample order, so the result is
-// bit-reproducible and needs no atomics.
-__global__ __launch_bounds__(kBlock) void k_film(PathsDev p, FilmDev film, WaveParams wp)
+// bit-reproducible and needs no atomics.  Slots [lo, hi) of the wave belong
+// to one scene of npix pixels; slot lo is that scene's pixel pix0.
+__global__ __launch_bounds__(kBlock) void k_film(PathsDev p, FilmDev film, uint32_t lo, uint32_t hi, uint32_t npix,
+                                                 uint32_t pix0)
 {
-    const uint32_t first_pix = (uint32_t)(wp.first_path % wp.npix);
-    const uint32_t npx = wp.count < wp.npix ? wp.count : wp.npix;
+    const uint32_t span = hi - lo;
+    const uint32_t npx = span < npix ? span : npix;
     for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < npx; t += gridDim.x * kBlock) {
         double sx = 0, sy = 0, sz = 0;
         uint32_t nv = 0;
-        for (uint32_t slot = t; slot < wp.count; slot += wp.npix) {
+        for (uint32_t slot = lo + t; slot < hi; slot += npix) {
             const float lx = p.lx[slot], ly = p.ly[slot], lz = p.lz[slot];
             if (isfinite(lx) && isfinite(ly) && isfinite(lz)) {
                 sx += (double)lx;
@@ -680,8 +733,8 @@ __global__ __launch_bounds__(kBlock) void k_film(PathsDev p, FilmDev film, WaveP
                 ++nv;
             }
         }
-        uint32_t px = first_pix + t;
-        if (px >= wp.npix) px -= wp.npix;
+        uint32_t px = pix0 + t;
+        if (px >= npix) px -= npix;
         film.accum[3 * (size_t)px] += sx;
         film.accum[3 * (size_t)px + 1] += sy;
         film.accum[3 * (size_t)px + 2] += sz;
@@ -697,52 +750,93 @@ __global__ __launch_bounds__(kBlock) void k_count(QueuesDev q, uint32_t cur)
     }
 }
 
+__global__ void k_set_u32(uint32_t *p, uint32_t v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+hipError_t set_u32(uint32_t *p, uint32_t v, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, s, p, v);
+    return hipGetLastError();
+}
+
 size_t extend_lds_bytes(uint32_t max_depth) { return (size_t)(max_depth + 1) * kBlock * sizeof(int32_t); }
 
 hipError_t render_wave(const SceneDev &sc, const PathsDev &paths, const ShadowDev &shadow, const QueuesDev &q,
                        const FilmDev &film, const WaveParams &wp, uint32_t capacity, hipStream_t stream,
-                       const CacheDev *cache, const CacheWork *cwork)
+                       const CacheDev *cache, const CacheWork *cwork, const BatchLaunch *batch)
 {
-    const int cap = (int)sc.bvh.max_depth + 1;
-    const size_t lds = extend_lds_bytes(sc.bvh.max_depth);
+    const bool B = batch != nullptr;
+    const uint32_t max_depth = B ? batch->max_depth : sc.bvh.max_depth;
+    const int cap = (int)max_depth + 1;
+    const size_t lds = extend_lds_bytes(max_depth);
     const dim3 blk(kBlock);
+    const dim3 grid(wp.grid ? wp.grid : kGrid);
     const uint32_t gen_grid = (wp.count + kBlock - 1) / kBlock;
+    const BatchDev bt = B ? batch->dev : BatchDev{nullptr, nullptr, 0};
     // cache vertices end rendering paths only; training paths trace to the end
-    const bool lookups = !wp.train && wp.cache_depth != kNoCache && cache && cwork;
+    const bool lookups = !B && !wp.train && wp.cache_depth != kNoCache && cache && cwork;
     if (!wp.train && wp.cache_depth != kNoCache && !lookups) return hipErrorInvalidValue;
+    if (B && wp.train) return hipErrorInvalidValue;
     if (lookups && cwork->rows < capacity) return hipErrorInvalidValue;
     const uint32_t ext_depth = lookups ? wp.cache_depth : kNoCache;
     hipLaunchKernelGGL(k_reset, dim3(1), blk, 0, stream, q.counters, 0u, wp.count);
-    hipLaunchKernelGGL(k_generate, dim3(gen_grid ? gen_grid : 1), blk, 0, stream, sc, paths, q, wp);
+    if (B) hipLaunchKernelGGL(k_generate<true>, dim3(gen_grid ? gen_grid : 1), blk, 0, stream, sc, bt, paths, q, wp);
+    else hipLaunchKernelGGL(k_generate<false>, dim3(gen_grid ? gen_grid : 1), blk, 0, stream, sc, bt, paths, q, wp);
     uint32_t cur = 0;
     for (uint32_t bounce = 0; bounce < wp.max_depth; ++bounce) {
         if (bounce > 0) hipLaunchKernelGGL(k_reset, dim3(1), blk, 0, stream, q.counters, cur, 0xffffffffu);
-        hipLaunchKernelGGL(k_extend, dim3(kGrid), blk, lds, stream, sc, paths, q, cur, cap, ext_depth);
-        hipLaunchKernelGGL((k_shade<kQLambert>), dim3(kGrid), blk, 0, stream, sc, paths, shadow, q, wp, cur);
-        hipLaunchKernelGGL((k_shade<kQDielectric>), dim3(kGrid), blk, 0, stream, sc, paths, shadow, q, wp, cur);
-        hipLaunchKernelGGL((k_shade<kQGgx>), dim3(kGrid), blk, 0, stream, sc, paths, shadow, q, wp, cur);
-        hipLaunchKernelGGL((k_shade<kQLight>), dim3(kGrid), blk, 0, stream, sc, paths, shadow, q, wp, cur);
-        hipLaunchKernelGGL((k_shade<kQMiss>), dim3(kGrid), blk, 0, stream, sc, paths, shadow, q, wp, cur);
+        if (B) {
+            hipLaunchKernelGGL(k_extend<true>, grid, blk, lds, stream, sc, bt, paths, q, cur, cap, ext_depth);
+            hipLaunchKernelGGL((k_shade<kQLambert, true>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
+            hipLaunchKernelGGL((k_shade<kQDielectric, true>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
+            hipLaunchKernelGGL((k_shade<kQGgx, true>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
+            hipLaunchKernelGGL((k_shade<kQLight, true>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
+            hipLaunchKernelGGL((k_shade<kQMiss, true>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
+        } else {
+            hipLaunchKernelGGL(k_extend<false>, grid, blk, lds, stream, sc, bt, paths, q, cur, cap, ext_depth);
+            hipLaunchKernelGGL((k_shade<kQLambert, false>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
+            hipLaunchKernelGGL((k_shade<kQDielectric, false>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
+            hipLaunchKernelGGL((k_shade<kQGgx, false>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
+            hipLaunchKernelGGL((k_shade<kQLight, false>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
+            hipLaunchKernelGGL((k_shade<kQMiss, false>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
+        }
         if (lookups && bounce >= wp.cache_depth)
-            hipLaunchKernelGGL(k_cache_vertex, dim3(kGrid), blk, 0, stream, sc, paths, q);
+            hipLaunchKernelGGL(k_cache_vertex, grid, blk, 0, stream, sc, paths, q);
         hipLaunchKernelGGL(k_count, dim3(1), blk, 0, stream, q, cur);
-        hipLaunchKernelGGL(k_connect, dim3(kGrid), blk, lds, stream, sc, paths, shadow, q, cap);
-        hipLaunchKernelGGL(k_finalize, dim3(kGrid), blk, 0, stream, sc, paths, q, 0u, cur);
+        if (B) hipLaunchKernelGGL(k_connect<true>, grid, blk, lds, stream, sc, bt, paths, shadow, q, cap);
+        else hipLaunchKernelGGL(k_connect<false>, grid, blk, lds, stream, sc, bt, paths, shadow, q, cap);
+        hipLaunchKernelGGL(k_finalize, grid, blk, 0, stream, paths, q, 0u, cur);
         cur ^= 1u;
     }
     // paths still alive after max_depth bounces cannot exist (shade stops
     // them at max_depth), but the list is flushed for safety
-    hipLaunchKernelGGL(k_finalize, dim3(kGrid), blk, 0, stream, sc, paths, q, 1u, cur);
+    hipLaunchKernelGGL(k_finalize, grid, blk, 0, stream, paths, q, 1u, cur);
     if (wp.train) {
-        hipLaunchKernelGGL(k_train_targets, dim3(kGrid), blk, 0, stream, paths, q, wp.count);
+        hipLaunchKernelGGL(k_train_targets, grid, blk, 0, stream, paths, q, wp.count);
         return hipGetLastError();
     }
     if (lookups) {
         hipError_t e = cache_infer(*cache, *cwork, paths.cin, wp.count, paths.crgb, stream, &q.counters[kCCacheQ]);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_cache_apply, dim3(kGrid), blk, 0, stream, paths, q);
+        hipLaunchKernelGGL(k_cache_apply, grid, blk, 0, stream, paths, q);
     }
-    hipLaunchKernelGGL(k_film, dim3(kGrid), blk, 0, stream, paths, film, wp);
+    if (!B) {
+        hipLaunchKernelGGL(k_film, grid, blk, 0, stream, paths, film, 0u, wp.count, wp.npix,
+                           (uint32_t)(wp.first_path % wp.npix));
+    } else {
+        // one film segment per scene that has paths in this wave
+        const uint64_t w0 = wp.first_path, w1 = wp.first_path + wp.count;
+        for (uint32_t s = 0; s < batch->dev.n; ++s) {
+            const uint64_t a = batch->bases[s] > w0 ? batch->bases[s] : w0;
+            const uint64_t b = batch->bases[s + 1] < w1 ? batch->bases[s + 1] : w1;
+            if (a >= b) continue;
+            const uint32_t np = batch->npix[s];
+            hipLaunchKernelGGL(k_film, grid, blk, 0, stream, paths, batch->films[s], (uint32_t)(a - w0),
+                               (uint32_t)(b - w0), np, (uint32_t)((a - batch->bases[s]) % np));
+        }
+    }
     return hipGetLastError();
 }
 

@@ -34,6 +34,18 @@ struct SceneDev {
     uint32_t nsph, npl, nlights;  // nlights = npl + area lights (NEE slots per path)
     uint32_t area_light[kMaxSpheres];  // sphere index of area light a
     uint32_t nmat;
+    // batched rendering: the scene's own PCG seed and stream offset (a
+    // single-scene wave takes both from WaveParams)
+    uint32_t seed;
+    uint64_t stream_base;
+};
+
+// Scenes of a batch as the kernels see them: every path carries its scene
+// index; scene s owns global path indices [bases[s], bases[s + 1]).
+struct BatchDev {
+    const SceneDev *scenes;
+    const uint64_t *bases;  // n + 1 entries
+    uint32_t n;
 };
 
 // Structure-of-arrays path state (capacity P).
@@ -43,13 +55,15 @@ struct PathsDev {
     float *lx, *ly, *lz;        // radiance
     uint64_t *rng;              // PCG32 state (the increment is derived from the pixel)
     uint32_t *pix;
+    uint32_t *sid;              // scene of the path (batched waves)
     uint32_t *meta;             // depth | prev_delta << 16
     float *ppdf, *ppx, *ppy, *ppz;  // previous BSDF pdf and vertex (MIS of sphere lights)
     int32_t *hid;               // >= 0 triangle, <= -2 sphere -(j+2), -1 miss
     float *ht;
     float *nbx, *nby, *nbz;     // throughput at the last next-event estimation
     uint32_t *nee_mask;         // unoccluded light slots of the last NEE
-    float *nee_c;               // P * nlights * 3: per-light f * Li * w
+    float *nee_c;               // P * nee_stride * 3: per-light f * Li * w
+    uint32_t nee_stride;        // light slots per path (max over a batch's scenes)
     // radiance cache: query records (P x 12) and results (P x 3), in queue order
     float *cin, *crgb;
     uint32_t *cslot;            // path slot of cache query k
@@ -87,6 +101,18 @@ struct WaveParams {
     uint64_t stream_base;       // PCG stream offset (scene id for batches)
     uint32_t cache_depth;       // hits at this depth end in the cache (kNoCache: off)
     uint32_t train;             // 1: training wave (random pixels, no film, records at cache_depth)
+    uint32_t grid;              // workgroups of the queue kernels (0: default)
+    const uint32_t *spp_offset_dev;  // if set, the first sample index is read here (graph replays)
+};
+
+// Host view of a batch for one wave launch: the device tables plus, per
+// scene, the path-index base (n + 1 entries), pixel count and film.
+struct BatchLaunch {
+    BatchDev dev;
+    const uint64_t *bases;
+    const uint32_t *npix;
+    const FilmDev *films;
+    uint32_t max_depth;         // deepest BVH of the batch (LDS stack size)
 };
 
 struct CacheDev;
@@ -96,11 +122,18 @@ struct CacheWork;
 // finalize, cache lookups, film) on `stream`.  With a cache and a rendering
 // wave, hits at cache_depth are answered by the cache; with a training wave
 // the compacted records and targets are left in paths.cin / paths.ttgt with
-// their count in counters[kCTrain].  Fails only on launch errors.
+// their count in counters[kCTrain].  With `batch`, the wave's paths belong to
+// the batch's scenes (no cache); `scene` and `film` are then unused.  Kernel
+// launches only (no memset, copy or host synchronisation), so the sequence
+// can be captured into a graph.  Fails only on launch errors.
 hipError_t render_wave(const SceneDev &scene, const PathsDev &paths, const ShadowDev &shadow,
                        const QueuesDev &queues, const FilmDev &film, const WaveParams &wp, uint32_t capacity,
-                       hipStream_t stream, const CacheDev *cache = nullptr, const CacheWork *cwork = nullptr);
+                       hipStream_t stream, const CacheDev *cache = nullptr, const CacheWork *cwork = nullptr,
+                       const BatchLaunch *batch = nullptr);
 
 size_t extend_lds_bytes(uint32_t max_depth);
+
+// *p = v from a one-thread kernel (for values a captured graph reads).
+hipError_t set_u32(uint32_t *p, uint32_t v, hipStream_t s);
 
 }  // namespace rt

@@ -17,6 +17,7 @@
 #include "rt_cache.h"
 #include "rt_internal.h"
 #include "rt_render.h"
+#include "rt_waves.h"
 
 struct rt_ctx;
 namespace rt {
@@ -179,11 +180,7 @@ struct rt_scene {
     rt_point_light *d_pl = nullptr;
     rt::SceneDev dev{};
     // wavefront buffers (grown on demand)
-    uint32_t capacity = 0, cap_lights = 0;
-    void *arena = nullptr;
-    rt::PathsDev paths{};
-    rt::ShadowDev shadow{};
-    rt::QueuesDev queues{};
+    rt::WaveBuffers wb;
     rt_render_stats stats{};
     uint64_t stream_base = 0;
     // radiance-cache inference scratch (one row per path slot), on demand
@@ -193,11 +190,15 @@ struct rt_scene {
 
 namespace rt {
 
+void free_wave_buffers(WaveBuffers &w)
+{
+    if (w.arena) (void)hipFree(w.arena);
+    w = WaveBuffers{};
+}
+
 static void free_arena(rt_scene *s)
 {
-    if (s->arena) (void)hipFree(s->arena);
-    s->arena = nullptr;
-    s->capacity = 0;
+    free_wave_buffers(s->wb);
     if (s->cache_arena) (void)hipFree(s->cache_arena);
     s->cache_arena = nullptr;
     s->cwork = rt::CacheWork{};
@@ -227,11 +228,11 @@ static rt_status ensure_cache_work(rt_scene *s, uint32_t rows)
 
 // One allocation for all wavefront buffers, carved in 256-byte aligned
 // pieces: paths (SoA), shadow queue (capacity * lights entries), queues.
-static rt_status ensure_buffers(rt_scene *s, uint32_t P)
+rt_status ensure_wave_buffers(WaveBuffers &w, uint32_t P, uint32_t NL, const char *who)
 {
-    const uint32_t NL = std::max<uint32_t>(1, s->dev.nlights);
-    if (s->capacity >= P && s->cap_lights >= NL) return RT_OK;
-    free_arena(s);
+    NL = std::max<uint32_t>(1, NL);
+    if (w.capacity >= P && w.cap_lights >= NL) return RT_OK;
+    free_wave_buffers(w);
     const size_t SQ = (size_t)P * NL;
     size_t total = 0;
     auto take = [&](size_t bytes) {
@@ -242,12 +243,13 @@ static rt_status ensure_buffers(rt_scene *s, uint32_t P)
     struct Piece { void **dst; size_t off; };
     std::vector<Piece> pieces;
     auto add = [&](void **dst, size_t bytes) { pieces.push_back({dst, take(bytes)}); };
-    PathsDev &p = s->paths;
+    PathsDev &p = w.paths;
     float **pf[] = {&p.ox, &p.oy, &p.oz, &p.dx, &p.dy, &p.dz, &p.bx, &p.by, &p.bz, &p.lx, &p.ly, &p.lz,
                     &p.ppdf, &p.ppx, &p.ppy, &p.ppz, &p.ht, &p.nbx, &p.nby, &p.nbz};
     for (float **f : pf) add((void **)f, (size_t)P * 4);
     add((void **)&p.rng, (size_t)P * 8);
     add((void **)&p.pix, (size_t)P * 4);
+    add((void **)&p.sid, (size_t)P * 4);
     add((void **)&p.meta, (size_t)P * 4);
     add((void **)&p.hid, (size_t)P * 4);
     add((void **)&p.nee_mask, (size_t)P * 4);
@@ -260,12 +262,12 @@ static rt_status ensure_buffers(rt_scene *s, uint32_t P)
     add((void **)&p.tB, (size_t)P * 12);
     add((void **)&p.tvalid, (size_t)P * 4);
     add((void **)&p.ttgt, (size_t)P * 12);
-    ShadowDev &sh = s->shadow;
+    ShadowDev &sh = w.shadow;
     float **sf[] = {&sh.ox, &sh.oy, &sh.oz, &sh.dx, &sh.dy, &sh.dz, &sh.tmax};
     for (float **f : sf) add((void **)f, SQ * 4);
     add((void **)&sh.slot, SQ * 4);
     add((void **)&sh.skip, SQ * 4);
-    QueuesDev &q = s->queues;
+    QueuesDev &q = w.queues;
     add((void **)&q.active[0], (size_t)P * 4);
     add((void **)&q.active[1], (size_t)P * 4);
     for (uint32_t k = 0; k < kNumQ; ++k) add((void **)&q.q[k], (size_t)P * 4);
@@ -275,15 +277,23 @@ static rt_status ensure_buffers(rt_scene *s, uint32_t P)
     char *base = nullptr;
     hipError_t e = hipMalloc(&base, total);
     if (e != hipSuccess) {
-        set_error(std::string("rt_render: wavefront buffers (") + std::to_string(total >> 20) +
+        set_error(std::string(who) + ": wavefront buffers (" + std::to_string(total >> 20) +
                   " MiB): " + hipGetErrorString(e));
         return e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;
     }
     for (auto &pc : pieces) *pc.dst = base + pc.off;
-    s->arena = base;
-    s->capacity = P;
-    s->cap_lights = NL;
+    p.nee_stride = NL;
+    w.arena = base;
+    w.capacity = P;
+    w.cap_lights = NL;
     return RT_OK;
+}
+
+static rt_status ensure_buffers(rt_scene *s, uint32_t P, const char *who)
+{
+    if (s->wb.capacity >= P && s->wb.cap_lights >= std::max<uint32_t>(1, s->dev.nlights)) return RT_OK;
+    free_arena(s);  // the cache scratch is sized to the wave capacity too
+    return ensure_wave_buffers(s->wb, P, s->dev.nlights, who);
 }
 
 }  // namespace rt
@@ -458,6 +468,18 @@ rt_status rt_scene_commit(rt_scene *s)
 
 rt_bvh *rt_scene_bvh(rt_scene *s) { return s ? s->bvh : nullptr; }
 
+}  // extern "C"
+
+// accessors for batches (batch.hip)
+namespace rt {
+bool scene_committed(const rt_scene *s) { return s->committed; }
+rt_ctx *scene_ctx(const rt_scene *s) { return s->ctx; }
+const SceneDev &scene_device(const rt_scene *s) { return s->dev; }
+uint64_t scene_stream_base(const rt_scene *s) { return s->stream_base; }
+}  // namespace rt
+
+extern "C" {
+
 rt_status rt_scene_camera_ray(const rt_scene *s, uint32_t px, uint32_t py, float u0, float u1, float o[3], float d[3])
 {
     if (!s || !s->has_cam || !o || !d) { set_error("rt_scene_camera_ray: null argument or no camera"); return RT_ERR_INVALID; }
@@ -515,15 +537,15 @@ static rt_status render_frames(const char *fn, rt_scene *s, const rt_render_opts
     uint32_t P = opts->path_capacity ? opts->path_capacity : (1u << 21);
     if (P > (1u << 26)) { set_error(name + ": path_capacity above 2^26"); return RT_ERR_INVALID; }
     if ((uint64_t)P > total) P = (uint32_t)std::max<uint64_t>(1, total);
-    rt_status st = rt::ensure_buffers(s, P);
+    rt_status st = rt::ensure_buffers(s, P, fn);
     if (st != RT_OK) return st;
     if (cache) {
-        st = rt::ensure_cache_work(s, s->capacity);
+        st = rt::ensure_cache_work(s, s->wb.capacity);
         if (st != RT_OK) return st;
     }
     auto t0 = std::chrono::steady_clock::now();
-    RT_HIP(hipMemsetAsync(s->queues.counters, 0, rt::kNumCounters * 4, stream));
-    RT_HIP(hipMemsetAsync(s->queues.stats, 0, 16, stream));
+    RT_HIP(hipMemsetAsync(s->wb.queues.counters, 0, rt::kNumCounters * 4, stream));
+    RT_HIP(hipMemsetAsync(s->wb.queues.stats, 0, 16, stream));
     rt::FilmDev film{accum, nvalid};
     for (uint64_t first = 0; first < total; first += P) {
         rt::WaveParams wp{};
@@ -537,11 +559,11 @@ static rt_status render_frames(const char *fn, rt_scene *s, const rt_render_opts
         wp.stream_base = s->stream_base;
         wp.cache_depth = cache ? cache_depth : rt::kNoCache;
         wp.train = 0;
-        RT_HIP(rt::render_wave(s->dev, s->paths, s->shadow, s->queues, film, wp, s->capacity, stream,
+        RT_HIP(rt::render_wave(s->dev, s->wb.paths, s->wb.shadow, s->wb.queues, film, wp, s->wb.capacity, stream,
                                cache ? &rt::cache_device(cache) : nullptr, cache ? &s->cwork : nullptr));
     }
     unsigned long long counts[2];
-    RT_HIP(hipMemcpyAsync(counts, s->queues.stats, sizeof counts, hipMemcpyDeviceToHost, stream));
+    RT_HIP(hipMemcpyAsync(counts, s->wb.queues.stats, sizeof counts, hipMemcpyDeviceToHost, stream));
     RT_HIP(hipStreamSynchronize(stream));
     s->stats.paths = total;
     s->stats.extension_rays = counts[0];
@@ -576,10 +598,10 @@ rt_status rt_cache_train_scene(rt_cache *c, rt_scene *s, const rt_render_opts *o
     std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(s->ctx));
     RT_HIP(hipSetDevice(rt::ctx_device(s->ctx)));
     hipStream_t stream = stream_ ? (hipStream_t)stream_ : (hipStream_t)rt::ctx_stream_ptr(s->ctx);
-    rt_status st = rt::ensure_buffers(s, n_paths);
+    rt_status st = rt::ensure_buffers(s, n_paths, "rt_cache_train_scene");
     if (st != RT_OK) return st;
-    RT_HIP(hipMemsetAsync(s->queues.counters, 0, rt::kNumCounters * 4, stream));
-    RT_HIP(hipMemsetAsync(s->queues.stats, 0, 16, stream));
+    RT_HIP(hipMemsetAsync(s->wb.queues.counters, 0, rt::kNumCounters * 4, stream));
+    RT_HIP(hipMemsetAsync(s->wb.queues.stats, 0, 16, stream));
     rt::WaveParams wp{};
     wp.first_path = (uint64_t)opts->spp_offset * n_paths;
     wp.count = n_paths;
@@ -592,13 +614,13 @@ rt_status rt_cache_train_scene(rt_cache *c, rt_scene *s, const rt_render_opts *o
     wp.cache_depth = cache_depth;
     wp.train = 1;
     rt::FilmDev film{nullptr, nullptr};
-    RT_HIP(rt::render_wave(s->dev, s->paths, s->shadow, s->queues, film, wp, s->capacity, stream));
+    RT_HIP(rt::render_wave(s->dev, s->wb.paths, s->wb.shadow, s->wb.queues, film, wp, s->wb.capacity, stream));
     uint32_t k = 0;
-    RT_HIP(hipMemcpyAsync(&k, s->queues.counters + rt::kCTrain, 4, hipMemcpyDeviceToHost, stream));
+    RT_HIP(hipMemcpyAsync(&k, s->wb.queues.counters + rt::kCTrain, 4, hipMemcpyDeviceToHost, stream));
     RT_HIP(hipStreamSynchronize(stream));
     const uint32_t n = std::min(k, rt::cache_batch(c));
     if (n == 0) return RT_OK;
-    st = rt_cache_train_step(c, s->paths.cin, s->paths.ttgt, n, loss, stream);
+    st = rt_cache_train_step(c, s->wb.paths.cin, s->wb.paths.ttgt, n, loss, stream);
     if (st != RT_OK) return st;
     if (n_samples) *n_samples = n;
     if (!loss) RT_HIP(hipStreamSynchronize(stream));

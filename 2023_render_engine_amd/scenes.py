@@ -13,6 +13,7 @@ procedural():   heightfield + icospheres + 8 point lights + a sphere light
 from __future__ import annotations
 
 import ctypes as C
+import math
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -206,3 +207,31 @@ def procedural(grid=1024, n_spheres=64, subdiv=5, width=1920, height=1080, spp=6
     cam = dict(pos=(0.5, -0.55, 0.75), lookat=(0.5, 0.45, 0.0), up=(0, 0, 1), cts=(0, 0, -1.0), frame_y=0.9,
                width=width, height=height)
     return SceneDesc(xyz, idx, tri_mat, mats, spheres, lights, cam, spp, max_depth, 3, seed)
+
+
+def procedural_sized(target_tris: int, size: int, seed: int, spp=32, max_depth=12) -> SceneDesc:
+    """A scene from the procedural generator with about target_tris
+    triangles at size x size pixels: 8 icospheres take ~40% of the budget
+    (subdivision level rounded to the nearest power of 4), the heightfield
+    the rest."""
+    import math
+    sph_budget = 0.4 * target_tris
+    subdiv = int(min(5, max(0, round(math.log(max(sph_budget / (8 * 20), 1.0), 4)))))
+    rest = max(2.0, target_tris - 8 * 20 * 4 ** subdiv)
+    grid = int(max(8, round(math.sqrt(rest / 2.0)) + 1))
+    return procedural(grid=grid, n_spheres=8, subdiv=subdiv, width=size, height=size, spp=spp,
+                      max_depth=max_depth, seed=seed)
+
+
+def batch_suite(n=32, seed=2024, spp=32, max_depth=12):
+    """The batched-rendering workload: n procedural scenes with triangle
+    counts log-uniform in [1e4, 5e5] and resolutions uniform over
+    {256^2, 512^2, 1024^2}; scene i has generator seed 1000 + i (also its
+    PCG seed)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        tris = int(round(math.exp(rng.uniform(math.log(1e4), math.log(5e5)))))
+        size = int(rng.choice([256, 512, 1024]))
+        out.append(procedural_sized(tris, size, 1000 + i, spp=spp, max_depth=max_depth))
+    return out

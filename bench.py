@@ -369,6 +369,109 @@ def run_c4(args):
     return out
 
 
+def run_c3(args):
+    """Batched rendering: 32 procedural scenes (triangles log-uniform in
+    [1e4, 5e5], resolutions uniform over {256^2, 512^2, 1024^2}, 32 spp,
+    depth 12, scene i seeded 1000 + i) in one rt_batch launch sequence with
+    shared queues, frames replayed from a captured hipGraph, wave size and
+    grid from the autotuner.  One step = one frame of every scene at 32 spp.
+    Also timed: the same frame as 32 separate rt_render calls.  Per-scene
+    RMSE is taken against a plain FP64 render of the same scene at
+    --c3-ref-spp with disjoint samples (itself noisy: its own RMSE to the
+    converged image is about sqrt(32 / ref_spp) of the frame's)."""
+    import numpy as np
+    import torch
+
+    rt = importlib.import_module("2023_render_engine_amd")
+    scenes = importlib.import_module("2023_render_engine_amd.scenes")
+    log("generating 32 procedural scenes")
+    descs = scenes.batch_suite(spp=32, max_depth=12)
+    ntris = sum(int(d.idx.shape[0]) for d in descs)
+    npix = sum(d.width * d.height for d in descs)
+    ctx = rt.Context(0)
+    t0 = time.time()
+    scs = [rt.Scene(ctx, d) for d in descs]
+    log(f"32 scenes committed in {time.time() - t0:.1f} s ({ntris} triangles, {npix} pixels)")
+    batch = rt.Batch(ctx, scs)
+    seeds = [1000 + i for i in range(len(descs))]
+    best, entries = batch.autotune(spp=1, max_depth=12)
+    log("autotune: " + ", ".join(f"P={e.path_capacity}/grid={e.grid}: {e.ms:.1f} ms" for e in entries))
+    films = [s.new_film() for s in scs]
+    for w in range(args.warmup):
+        batch.render(films, spp=32, spp_offset=32 * w, max_depth=12, seeds=seeds, graph=True)
+    torch.cuda.synchronize()
+    for f in films:
+        f[0].zero_()
+        f[1].zero_()
+    ext = shadow = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        batch.render(films, spp=32, spp_offset=32 * k, max_depth=12, seeds=seeds, graph=True)
+        st = batch.stats()
+        ext += st.extension_rays
+        shadow += st.shadow_rays
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    samples = npix * 32 * args.steps
+    # the same frame rendered scene by scene
+    seq = [s.new_film() for s in scs]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for s, f, sd in zip(scs, seq, seeds):
+        s.render(f, spp=32, spp_offset=0, max_depth=12, seed=sd)
+    torch.cuda.synchronize()
+    seq_s = time.perf_counter() - t1
+    rmse = []
+    if args.c3_ref_spp > 0:
+        log(f"references at {args.c3_ref_spp} spp")
+        for s, f, sd in zip(scs, films, seeds):
+            ref = s.new_film()
+            s.render(ref, spp=args.c3_ref_spp, spp_offset=1 << 24, max_depth=12, seed=sd)
+            ri = ref[0].view(-1, 3) / ref[1].double().clamp(min=1).unsqueeze(1)
+            rmse.append(_rmse_dev(f[0], f[1], ri))
+    out = {
+        "metric": METRIC,
+        "value": round(samples / wall / 1e6, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (FP64 film)", "data": "synthetic",
+        "config": {"workload": ("batched: 32 procedural scenes (triangles log-uniform 1e4-5e5, resolutions "
+                                "{256,512,1024}^2), 32 spp, depth 12, one rt_batch launch sequence, graph-replayed "
+                                "frames, autotuned wave size and grid"),
+                   "triangles": ntris, "pixels": npix,
+                   "autotuned": {"path_capacity": best.path_capacity, "grid": best.grid}},
+        "mrays_total": round((ext + shadow) / wall / 1e6, 1),
+        "mrays_extension": round(ext / wall / 1e6, 1),
+        "sequential_msamples_s": round(npix * 32 / seq_s / 1e6, 3),
+        "batch_speedup_vs_sequential": round(seq_s / (wall / args.steps), 3),
+    }
+    if rmse:
+        out["rmse_vs_ref"] = {"ref_spp": args.c3_ref_spp, "frames": args.steps, "mean": round(float(np.mean(rmse)), 5),
+                              "max": round(float(np.max(rmse)), 5)}
+    if not args.no_cpu_baseline:
+        import oracle
+        spent, done = 0.0, 0
+        for d in descs[:2]:
+            os_ = oracle.OrcScene(d)
+            w = 32
+            x0, y0 = (d.width - w) // 2, (d.height - w) // 2
+            t2 = time.time()
+            os_.render(spp=32, window=(x0, y0, x0 + w, y0 + w), nthreads=cpu_threads())
+            spent += time.time() - t2
+            done += w * w * 32
+        out["cpu_baseline"] = {
+            "value": round(done / spent / 1e6, 4), "unit": "Msamples/s", "cores": cpu_threads(), "kind": "port",
+            "sample": f"{done} samples: centred 32x32 windows of 2 of the 32 scenes at 32 spp with the oracle's "
+                      f"restatement of the reference estimator, {spent:.1f} s"}
+    batch.close()
+    for s in scs:
+        s.close()
+    ctx.close()
+    return out
+
+
 def run_copy(args):
     """HBM calibration: a plain device-to-device copy of 1 GiB (torch), so the
     FETCH_SIZE / WRITE_SIZE counters of a kernel of known traffic can scale
@@ -398,7 +501,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c2", "c4", "copy"])
+    ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c2", "c3", "c4", "copy"])
+    ap.add_argument("--c3-ref-spp", type=int, default=128)
     ap.add_argument("--c4-width", type=int, default=480)
     ap.add_argument("--c4-height", type=int, default=270)
     ap.add_argument("--c4-ref-spp", type=int, default=16384)
@@ -420,6 +524,8 @@ def main():
         sys.exit(1)
     if args.workload == "c1":
         out = run_c1(args)
+    elif args.workload == "c3":
+        out = run_c3(args)
     elif args.workload == "c4":
         out = run_c4(args)
     elif args.workload == "copy":

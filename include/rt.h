@@ -36,7 +36,7 @@ typedef enum {
     RT_ERR_NO_DEVICE = 5    /* no HIP device visible */
 } rt_status;
 
-enum { RT_FLAG_HOST_PTRS = 1u };
+enum { RT_FLAG_HOST_PTRS = 1u, RT_FLAG_GRAPH = 2u };
 
 /* Thread-local message of the last failing call on this thread. */
 const char *rt_last_error(void);
@@ -100,7 +100,10 @@ rt_status rt_bvh_get_info(const rt_bvh *bvh, rt_bvh_info *info);
  * skips them for shadow rays (shapes/Triangle.cu:439-440). */
 rt_status rt_bvh_set_transparent(rt_bvh *bvh, const uint8_t *flags);
 /* Copy the device tree to host arrays of num_nodes*16 and num_tris*12
- * floats (layout documented in DESIGN.md and restated by the oracle). */
+ * floats: 64-byte nodes (child 0 box x/y, child 1 box x/y, both z ranges,
+ * two child links; a link < 0 is a leaf ~(first << 3 | count - 1)) and
+ * 48-byte triangles (v0 xyz + id, v1 xyz + flags, v2 xyz + 0), as the
+ * oracle's counting traversal reads them. */
 rt_status rt_bvh_export(const rt_bvh *bvh, float *nodes, float *tris);
 
 /* ------------------------------------------------------------------ */
@@ -232,6 +235,53 @@ typedef struct {
     double ms;             /* wall time of the call (device-synchronised) */
 } rt_render_stats;
 rt_status rt_render_get_stats(const rt_scene *scene, rt_render_stats *stats);
+
+/* ------------------------------------------------------------------ */
+/* Batches: many scenes in one launch sequence                          */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference, which renders one scene per process
+ * (PathTracerIntegrator::render, integration/PathTracerIntegrator.cu:146-210;
+ * several GPUs = several processes, boxMulti.cu:174-190).  A batch renders
+ * n committed scenes of any sizes together: their samples form one index
+ * space, waves of paths from all scenes share the ray and material queues,
+ * and each path carries its scene (own BVH, materials, lights, camera).
+ * Per scene, the image equals rt_render's with the same seed, sample offset
+ * and options (up to FP64 summation order across wave boundaries).  Scenes
+ * must stay alive and committed while the batch exists. */
+typedef struct rt_batch rt_batch;
+
+typedef struct {
+    uint32_t spp, spp_offset, max_depth, rr_depth;
+    uint32_t path_capacity;  /* paths per wave, 0 = the autotuned or default value */
+    uint32_t grid;           /* workgroups of the queue kernels, 0 = autotuned or default */
+} rt_batch_opts;
+
+rt_status rt_batch_create(rt_ctx *ctx, rt_scene *const *scenes, uint32_t n, rt_batch **out);
+rt_status rt_batch_destroy(rt_batch *batch);
+
+/* Renders opts->spp samples per pixel of every scene, accumulating into the
+ * device films accum[i] (W_i*H_i*3 FP64 sums) and nvalid[i] (W_i*H_i valid
+ * counts).  seeds: n PCG seeds (NULL: 1 for every scene).  RT_FLAG_GRAPH:
+ * the frame's launch sequence is captured into a hipGraph on the first call
+ * and replayed by later calls with the same spp, depths, wave size, grid and
+ * films; the sample offset and seeds are read from device memory at replay,
+ * so each call renders its own samples.  A change of any captured option
+ * recaptures.  Blocks until the frame is done. */
+rt_status rt_batch_render(rt_batch *batch, const rt_batch_opts *opts, const uint32_t *seeds, double *const *accum,
+                          uint32_t *const *nvalid, uint32_t flags, void *stream);
+
+/* Autotuner: renders one sample per pixel of every scene (into scratch
+ * films) for each candidate wave size {2^19 .. 2^22, capped at the batch's
+ * paths per sample} x grid {1024, 2048, 4096}, twice each, and keeps the
+ * fastest pair as the batch's default.  entries (may be NULL) receives up to
+ * max_entries (path_capacity, grid, ms) triples in the order tried. */
+typedef struct {
+    uint32_t path_capacity, grid;
+    float ms;  /* best of the two trials of one 1-spp frame */
+} rt_tune_entry;
+rt_status rt_batch_autotune(rt_batch *batch, const rt_batch_opts *opts, rt_tune_entry *best, rt_tune_entry *entries,
+                            uint32_t max_entries, uint32_t *n_entries);
+rt_status rt_batch_get_stats(const rt_batch *batch, rt_render_stats *stats);
 
 /* ------------------------------------------------------------------ */
 /* Neural radiance cache                                                */

@@ -1,0 +1,103 @@
+"""Batched rendering (rt_batch_* through the C ABI): several scenes of
+different sizes in one launch sequence with shared queues.
+
+Bar: per scene, the batch's film equals that scene's own rt_render film
+with the same seed and sample offset -- valid counts exactly, FP64 sums to
+1e-12 relative (waves split a pixel's samples at different points, which
+only reorders the FP64 additions).  Graph replays equal eager frames bit
+for bit, replay after replay, with other GPU work in between."""
+import importlib
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+scenes = importlib.import_module("2023_render_engine_amd.scenes")
+
+
+def descs():
+    return [scenes.cornell(width=48, height=40, spp=8, max_depth=8),
+            scenes.procedural(grid=48, n_spheres=6, subdiv=2, width=64, height=48, spp=8, max_depth=8, seed=5),
+            scenes.procedural(grid=32, n_spheres=4, subdiv=1, width=40, height=40, spp=8, max_depth=8, seed=9)]
+
+
+def host(film):
+    torch.cuda.synchronize()
+    return film[0].cpu().numpy().copy(), film[1].cpu().numpy().copy()
+
+
+def assert_same(a, b, exact=False):
+    (sa, na), (sb, nb) = a, b
+    assert np.array_equal(na, nb)
+    if exact:
+        assert np.array_equal(sa, sb)
+    else:
+        assert np.allclose(sa, sb, rtol=1e-12, atol=1e-300)
+
+
+@pytest.fixture(scope="module")
+def batch3(rt, gpu_ctx):
+    scs = [rt.Scene(gpu_ctx, d) for d in descs()]
+    b = rt.Batch(gpu_ctx, scs)
+    yield scs, b
+    b.close()
+    for s in scs:
+        s.close()
+
+
+@pytest.mark.parametrize("cap", [0, 5000])
+def test_batch_equals_standalone_renders(rt, batch3, cap):
+    """cap 5000 forces waves that cross scene boundaries."""
+    scs, b = batch3
+    print(f"[batch] 3 scenes, path_capacity {cap}", flush=True)
+    seeds = [3, 1, 7]
+    films = [s.new_film() for s in scs]
+    b.render(films, spp=8, spp_offset=16, max_depth=8, rr_depth=3, seeds=seeds, path_capacity=cap)
+    st = b.stats()
+    assert st.paths == sum(d.width * d.height for d in descs()) * 8
+    for s, f, sd in zip(scs, films, seeds):
+        ref = s.new_film()
+        s.render(ref, spp=8, spp_offset=16, seed=sd, max_depth=8, rr_depth=3)
+        assert_same(host(f), host(ref))
+
+
+def test_graph_replays_equal_eager_frames(rt, batch3):
+    scs, b = batch3
+    print("[batch] graph replays", flush=True)
+    fg = [s.new_film() for s in scs]
+    fe = [s.new_film() for s in scs]
+    x = torch.randn(1 << 20, device="cuda")
+    for frame in range(4):
+        b.render(fg, spp=4, spp_offset=4 * frame, max_depth=8, path_capacity=6000, graph=True)
+        # unrelated eager GPU work between replays
+        x = torch.sin(x) * 1.0001
+        _ = float(x.sum())
+        b.render(fe, spp=4, spp_offset=4 * frame, max_depth=8, path_capacity=6000, graph=False)
+        for g, e in zip(fg, fe):
+            assert_same(host(g), host(e), exact=True)
+    # the replays rendered new samples each frame: 16 spp in total
+    for s, g in zip(scs, fg):
+        ref = s.new_film()
+        s.render(ref, spp=16, spp_offset=0, max_depth=8, seed=1)
+        _, n = host(g)
+        assert n.max() <= 16 and n.sum() >= 0.999 * 16 * n.size
+        assert_same(host(g), host(ref))
+
+
+def test_autotune_picks_the_fastest_and_keeps_results(rt, batch3):
+    scs, b = batch3
+    print("[batch] autotune", flush=True)
+    best, entries = b.autotune(spp=1, max_depth=8)
+    assert len(entries) >= 3
+    assert best.ms == min(e.ms for e in entries)
+    assert (best.path_capacity, best.grid) in [(e.path_capacity, e.grid) for e in entries]
+    print("[batch] autotune: " + ", ".join(f"P={e.path_capacity} grid={e.grid}: {e.ms:.3f} ms" for e in entries),
+          flush=True)
+    films = [s.new_film() for s in scs]
+    b.render(films, spp=4, spp_offset=0, max_depth=8)
+    for s, f in zip(scs, films):
+        ref = s.new_film()
+        s.render(ref, spp=4, spp_offset=0, seed=1, max_depth=8)
+        assert_same(host(f), host(ref))

@@ -449,16 +449,17 @@ class Scene:
         return film
 
     def train_cache(self, cache, frame, n_paths=65536, cache_depth=2, max_depth=None, rr_depth=None, seed=None,
-                    stream=None):
+                    want_loss=True, stream=None):
         """rt_cache_train_scene: one online training step of `cache` on
-        n_paths training paths of frame `frame`; returns (loss, samples)."""
-        import torch
+        n_paths training paths of frame `frame`; returns (loss, samples), or
+        (None, None) without waiting when want_loss is False."""
         o = self._opts(1, frame, max_depth, rr_depth, seed, 0)
         loss = C.c_float(0.0)
         n = C.c_uint32(0)
+        lp, np_ = (C.byref(loss), C.byref(n)) if want_loss else (None, None)
         self.ctx.launch(lambda s: check(lib().rt_cache_train_scene(cache.h, self.h, C.byref(o), cache_depth, n_paths,
-                                                                   C.byref(loss), C.byref(n), s)), stream)
-        return loss.value, n.value
+                                                                   lp, np_, s)), stream)
+        return (loss.value, n.value) if want_loss else (None, None)
 
     def stats(self) -> RenderStats:
         st = RenderStats()

@@ -171,8 +171,10 @@ __global__ __launch_bounds__(256) void k_fwd(const uint16_t *__restrict__ X, con
 // ---------------------------------------------------------------------
 template <int NIN, bool MASK, bool OUTF32>
 __global__ __launch_bounds__(256) void k_bwd_data(const uint16_t *__restrict__ dY, const uint16_t *__restrict__ W,
-                                                  const uint16_t *__restrict__ A, uint32_t n, void *__restrict__ dX)
+                                                  const uint16_t *__restrict__ A, uint32_t n,
+                                                  const uint32_t *__restrict__ n_dev, void *__restrict__ dX)
 {
+    if (n_dev) n = min(n, *n_dev);
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x / 64;
     const uint32_t li = lane & 15, lq = lane >> 4;
@@ -216,8 +218,9 @@ __global__ __launch_bounds__(256) void k_bwd_data(const uint16_t *__restrict__ d
 // ---------------------------------------------------------------------
 template <int NOUT>
 __global__ __launch_bounds__(256) void k_wgrad(const uint16_t *__restrict__ X, const uint16_t *__restrict__ dY,
-                                               uint32_t n, float *__restrict__ dW)
+                                               uint32_t n, const uint32_t *__restrict__ n_dev, float *__restrict__ dW)
 {
+    if (n_dev) n = min(n, *n_dev);
     __shared__ uint16_t sX[64 * kCacheWidth];
     __shared__ uint16_t sD[64 * NOUT];
     const uint32_t lane = __lane_id();
@@ -271,12 +274,17 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t *__restrict__ X, c
 // ---------------------------------------------------------------------
 // relative L2 loss (Mueller et al. 2021): l = (y - t)^2 / (sg(y)^2 + 0.01),
 // mean over rows and the 3 channels; writes dY (BF16, [n][16]) and adds the
-// loss sum to *loss.
+// loss sum to *loss.  With n_dev the row count (and the mean's scale) comes
+// from device memory.
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ Y, const float *__restrict__ target,
-                                              uint32_t n, float scale, uint16_t *__restrict__ dY,
-                                              float *__restrict__ loss)
+                                              uint32_t n, const uint32_t *__restrict__ n_dev, float scale,
+                                              uint16_t *__restrict__ dY, float *__restrict__ loss)
 {
+    if (n_dev) {
+        n = min(n, *n_dev);
+        scale = n ? 1.0f / (3.0f * (float)n) : 0.0f;
+    }
     float lsum = 0;
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         alignas(16) uint16_t g[16];
@@ -301,8 +309,10 @@ __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ Y, const
 
 // hash-grid gradient: scatter dX[:, 0..2L) through the trilinear weights
 __global__ __launch_bounds__(256) void k_grid_bwd(CacheDev c, const float *__restrict__ in, uint32_t n,
-                                                  const float *__restrict__ dX, float *__restrict__ dgrid)
+                                                  const uint32_t *__restrict__ n_dev, const float *__restrict__ dX,
+                                                  float *__restrict__ dgrid)
 {
+    if (n_dev) n = min(n, *n_dev);
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         const float *q = in + (size_t)i * kCacheInStride;
         float px, py, pz;
@@ -407,33 +417,35 @@ hipError_t cache_infer(const CacheDev &c, const CacheWork &w, const float *in, u
 }
 
 hipError_t cache_backward(const CacheDev &c, const CacheWork &w, const float *in, const float *target, uint32_t n,
-                          float *loss, hipStream_t s)
+                          float *loss, hipStream_t s, const uint32_t *n_dev)
 {
     if (n == 0) return hipSuccess;
     const dim3 blk(256);
-    hipError_t e = cache_forward(c, w, in, n, s, nullptr);
+    hipError_t e = cache_forward(c, w, in, n, s, n_dev);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_loss, grid_for(n, 256), blk, 0, s, w.y, target, n, 1.0f / (3.0f * (float)n), w.dy16, loss);
+    hipLaunchKernelGGL(k_loss, grid_for(n, 256), blk, 0, s, w.y, target, n, n_dev, 1.0f / (3.0f * (float)n), w.dy16,
+                       loss);
     const dim3 g = grid_for((n + 15) / 16, 4);
     const dim3 gw = grid_for((n + 63) / 64, 1, 512);
     const uint32_t L = kCacheLayers;
     // output layer
-    hipLaunchKernelGGL((k_wgrad<16>), gw, blk, 0, s, w.act[L - 1], w.dy16, n, c.grad + kCacheLayerOffset(L - 1));
+    hipLaunchKernelGGL((k_wgrad<16>), gw, blk, 0, s, w.act[L - 1], w.dy16, n, n_dev, c.grad + kCacheLayerOffset(L - 1));
     hipLaunchKernelGGL((k_bwd_data<16, true, false>), g, blk, 0, s, w.dy16, c.w + kCacheLayerOffset(L - 1),
-                       w.act[L - 1], n, (void *)w.dact[0]);
+                       w.act[L - 1], n, n_dev, (void *)w.dact[0]);
     // hidden layers l = L-2 .. 1: input act[l], output grad in dact
     uint32_t cur = 0;
     for (int l = (int)L - 2; l >= 1; --l) {
-        hipLaunchKernelGGL((k_wgrad<64>), gw, blk, 0, s, w.act[l], w.dact[cur], n, c.grad + kCacheLayerOffset(l));
+        hipLaunchKernelGGL((k_wgrad<64>), gw, blk, 0, s, w.act[l], w.dact[cur], n, n_dev,
+                           c.grad + kCacheLayerOffset(l));
         hipLaunchKernelGGL((k_bwd_data<64, true, false>), g, blk, 0, s, w.dact[cur], c.w + kCacheLayerOffset(l),
-                           w.act[l], n, (void *)w.dact[cur ^ 1]);
+                           w.act[l], n, n_dev, (void *)w.dact[cur ^ 1]);
         cur ^= 1;
     }
     // first layer: weight gradient and the encoding gradient (FP32)
-    hipLaunchKernelGGL((k_wgrad<64>), gw, blk, 0, s, w.act[0], w.dact[cur], n, c.grad + kCacheLayerOffset(0));
+    hipLaunchKernelGGL((k_wgrad<64>), gw, blk, 0, s, w.act[0], w.dact[cur], n, n_dev, c.grad + kCacheLayerOffset(0));
     hipLaunchKernelGGL((k_bwd_data<64, false, true>), g, blk, 0, s, w.dact[cur], c.w + kCacheLayerOffset(0),
-                       w.act[0], n, (void *)w.dx32);
-    hipLaunchKernelGGL(k_grid_bwd, grid_for(n, 256), blk, 0, s, c, in, n, w.dx32, c.grid_grad);
+                       w.act[0], n, n_dev, (void *)w.dx32);
+    hipLaunchKernelGGL(k_grid_bwd, grid_for(n, 256), blk, 0, s, c, in, n, n_dev, w.dx32, c.grid_grad);
     return hipGetLastError();
 }
 

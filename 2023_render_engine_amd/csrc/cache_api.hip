@@ -244,6 +244,28 @@ static rt_status train_batch(rt_cache *c, const float *in, const float *target, 
     return RT_OK;
 }
 
+}  // extern "C"
+
+namespace rt {
+// One optimiser step on min(n, batch, *n_dev) rows whose count is known only
+// on the device; the loss sum is left in *loss_dev.  No host synchronisation.
+rt_status cache_train_rows_dev(rt_cache *c, const float *in, const float *target, uint32_t n, const uint32_t *n_dev,
+                               hipStream_t s, float **loss_dev)
+{
+    n = std::min(n, c->work.rows);
+    if (loss_dev) *loss_dev = c->d_loss;
+    RT_HIP(rt::cache_fill(c->d_loss, 1, 0.0f, s));
+    if (n == 0) return RT_OK;
+    RT_HIP(rt::cache_backward(c->dev, c->work, in, target, n, c->d_loss, s, n_dev));
+    ++c->step;
+    rt::CacheOptsDev od{c->opts.lr, c->opts.beta1, c->opts.beta2, c->opts.eps};
+    RT_HIP(rt::cache_adam(c->dev, od, c->step, s));
+    return RT_OK;
+}
+}  // namespace rt
+
+extern "C" {
+
 rt_status rt_cache_train_step(rt_cache *c, const float *in, const float *target, uint32_t n, float *loss,
                               void *stream_)
 {

@@ -53,9 +53,10 @@ hipError_t cache_forward(const CacheDev &c, const CacheWork &w, const float *in,
 hipError_t cache_infer(const CacheDev &c, const CacheWork &w, const float *in, uint32_t n, float *rgb,
                        hipStream_t s, const uint32_t *n_dev = nullptr);
 // Forward + loss + backward; accumulates gradients (c.grad, c.grid_grad) and
-// adds the summed loss to *loss (device, may be null).
+// adds the summed loss to *loss (device, may be null).  With n_dev the row
+// count is min(n, *n_dev), read on the device.
 hipError_t cache_backward(const CacheDev &c, const CacheWork &w, const float *in, const float *target,
-                          uint32_t n, float *loss, hipStream_t s);
+                          uint32_t n, float *loss, hipStream_t s, const uint32_t *n_dev = nullptr);
 // One Adam step over every parameter; clears the gradients.
 hipError_t cache_adam(const CacheDev &c, const CacheOptsDev &o, uint32_t step, hipStream_t s);
 hipError_t cache_fill(float *p, uint32_t count, float value, hipStream_t s);

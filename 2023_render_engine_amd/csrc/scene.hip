@@ -30,6 +30,8 @@ DevBvh bvh_device(const rt_bvh *b);
 const CacheDev &cache_device(const rt_cache *c);
 rt_ctx *cache_ctx(const rt_cache *c);
 uint32_t cache_batch(const rt_cache *c);
+rt_status cache_train_rows_dev(rt_cache *c, const float *in, const float *target, uint32_t n, const uint32_t *n_dev,
+                               hipStream_t s, float **loss_dev);
 }  // namespace rt
 
 using rt::set_error;
@@ -615,15 +617,22 @@ rt_status rt_cache_train_scene(rt_cache *c, rt_scene *s, const rt_render_opts *o
     wp.train = 1;
     rt::FilmDev film{nullptr, nullptr};
     RT_HIP(rt::render_wave(s->dev, s->wb.paths, s->wb.shadow, s->wb.queues, film, wp, s->wb.capacity, stream));
-    uint32_t k = 0;
-    RT_HIP(hipMemcpyAsync(&k, s->wb.queues.counters + rt::kCTrain, 4, hipMemcpyDeviceToHost, stream));
-    RT_HIP(hipStreamSynchronize(stream));
-    const uint32_t n = std::min(k, rt::cache_batch(c));
-    if (n == 0) return RT_OK;
-    st = rt_cache_train_step(c, s->wb.paths.cin, s->wb.paths.ttgt, n, loss, stream);
+    // the sample count stays on the device: no host round trip unless the
+    // caller asks for the loss or the count
+    const uint32_t *k_dev = s->wb.queues.counters + rt::kCTrain;
+    float *loss_dev = nullptr;
+    st = rt::cache_train_rows_dev(c, s->wb.paths.cin, s->wb.paths.ttgt, n_paths, k_dev, stream, &loss_dev);
     if (st != RT_OK) return st;
-    if (n_samples) *n_samples = n;
-    if (!loss) RT_HIP(hipStreamSynchronize(stream));
+    if (loss || n_samples) {
+        uint32_t k = 0;
+        float l = 0;
+        RT_HIP(hipMemcpyAsync(&k, k_dev, 4, hipMemcpyDeviceToHost, stream));
+        RT_HIP(hipMemcpyAsync(&l, loss_dev, 4, hipMemcpyDeviceToHost, stream));
+        RT_HIP(hipStreamSynchronize(stream));
+        const uint32_t n = std::min(std::min(k, n_paths), rt::cache_batch(c));
+        if (n_samples) *n_samples = n;
+        if (loss) *loss = n ? l / (3.0f * (float)n) : 0.0f;
+    }
     return RT_OK;
 }
 

@@ -354,9 +354,10 @@ rt_status rt_cache_set_params(rt_cache *cache, const float *mlp, const float *gr
  * with the plain estimator, takes for each path that has a cache vertex the
  * radiance it gathered from that vertex on divided by its throughput on
  * arrival, and runs one Adam step on up to the cache's batch size of these
- * (record, radiance) pairs.  *loss (mean relative L2 loss before the step) and
- * *n_samples may be NULL; n_samples = 0 means no step was taken.  Blocks until
- * the step is done. */
+ * (record, radiance) pairs.  The pair count stays on the device, so the call
+ * only enqueues work; it blocks only when loss (mean relative L2 loss before
+ * the step) or n_samples (pairs used; 0 = the step saw no pairs) is
+ * requested.  Both may be NULL. */
 rt_status rt_render_cached(rt_scene *scene, const rt_render_opts *opts, rt_cache *cache, uint32_t cache_depth,
                            double *accum, uint32_t *nvalid, uint32_t flags, void *stream);
 rt_status rt_cache_train_scene(rt_cache *cache, rt_scene *scene, const rt_render_opts *opts,

@@ -28,14 +28,15 @@ class RtError(RuntimeError):
 
 
 class BvhOpts(C.Structure):
-    _fields_ = [("max_leaf", C.c_uint32), ("bins", C.c_uint32), ("trav_cost", C.c_float)]
+    _fields_ = [("max_leaf", C.c_uint32), ("bins", C.c_uint32), ("trav_cost", C.c_float), ("width", C.c_uint32)]
 
 
 class BvhInfo(C.Structure):
     _fields_ = [
         ("num_tris", C.c_uint32), ("num_nodes", C.c_uint32), ("num_leaves", C.c_uint32),
         ("max_depth", C.c_uint32), ("node_bytes", C.c_uint64), ("tri_bytes", C.c_uint64),
-        ("build_ms", C.c_double), ("sah_cost", C.c_double),
+        ("build_ms", C.c_double), ("sah_cost", C.c_double), ("width", C.c_uint32), ("num_nodes4", C.c_uint32),
+        ("depth4", C.c_uint32), ("stack_cap", C.c_uint32),
     ]
 
 
@@ -62,7 +63,8 @@ class Camera(C.Structure):
 
 class RenderOpts(C.Structure):
     _fields_ = [("spp", C.c_uint32), ("spp_offset", C.c_uint32), ("max_depth", C.c_uint32),
-                ("rr_depth", C.c_uint32), ("seed", C.c_uint32), ("path_capacity", C.c_uint32)]
+                ("rr_depth", C.c_uint32), ("seed", C.c_uint32), ("path_capacity", C.c_uint32),
+                ("grid", C.c_uint32)]
 
 
 class RenderStats(C.Structure):
@@ -114,6 +116,7 @@ _SIGS = {
     "rt_bvh_get_info": (C.c_int, [_P, C.POINTER(BvhInfo)]),
     "rt_bvh_set_transparent": (C.c_int, [_P, _P]),
     "rt_bvh_export": (C.c_int, [_P, _P, _P]),
+    "rt_bvh_export4": (C.c_int, [_P, _P]),
     "rt_bvh_last_kernel_ms": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "rt_trace_closest": (C.c_int, [_P, C.POINTER(Rays), _U32, _P, _P, _U32, _P]),
     "rt_trace_any": (C.c_int, [_P, C.POINTER(Rays), _U32, _P, _U32, _P]),
@@ -254,12 +257,12 @@ class Bvh:
     """rt_bvh over a triangle soup given as host float32 vertices (nv,3) and
     uint32 indices (nt,3)."""
 
-    def __init__(self, ctx: Context, xyz, idx, max_leaf=4, bins=32, trav_cost=1.2):
+    def __init__(self, ctx: Context, xyz, idx, max_leaf=4, bins=32, trav_cost=1.2, width=4):
         import numpy as np
         xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
         idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1, 3)
         self.ctx = ctx
-        opts = BvhOpts(max_leaf, bins, trav_cost)
+        opts = BvhOpts(max_leaf, bins, trav_cost, width)
         h = _P()
         check(lib().rt_bvh_build(ctx.h, _ptr(xyz), xyz.shape[0], _ptr(idx), idx.shape[0],
                                  C.byref(opts), C.byref(h)))
@@ -288,6 +291,13 @@ class Bvh:
         tris = np.zeros((inf.num_tris + 1, 12), np.float32)
         check(lib().rt_bvh_export(self.h, _ptr(nodes), _ptr(tris)))
         return nodes, tris
+
+    def export4(self):
+        import numpy as np
+        inf = self.info()
+        nodes4 = np.zeros((inf.num_nodes4, 16), np.float32)
+        check(lib().rt_bvh_export4(self.h, _ptr(nodes4)))
+        return nodes4
 
     def set_transparent(self, flags):
         import numpy as np
@@ -421,21 +431,18 @@ class Scene:
         return (torch.zeros(n * 3, dtype=torch.float64, device=dev), torch.zeros(n, dtype=torch.int32, device=dev))
 
     def render(self, film, spp=None, spp_offset=0, max_depth=None, rr_depth=None, seed=None, path_capacity=0,
-               stream=None):
-        import torch
-        d = self.desc
-        o = RenderOpts(spp if spp is not None else d.spp, spp_offset, max_depth or d.max_depth,
-                       d.rr_depth if rr_depth is None else rr_depth, d.seed if seed is None else seed, path_capacity)
+               grid=0, stream=None):
+        o = self._opts(spp, spp_offset, max_depth, rr_depth, seed, path_capacity, grid)
         accum, nvalid = film
         self.ctx.launch(lambda s: check(lib().rt_render(self.h, C.byref(o), _ptr(accum), _ptr(nvalid), 0, s)),
                         stream)
         return film
 
-    def _opts(self, spp, spp_offset, max_depth, rr_depth, seed, path_capacity):
+    def _opts(self, spp, spp_offset, max_depth, rr_depth, seed, path_capacity, grid=0):
         d = self.desc
         return RenderOpts(spp if spp is not None else d.spp, spp_offset, max_depth or d.max_depth,
                           d.rr_depth if rr_depth is None else rr_depth, d.seed if seed is None else seed,
-                          path_capacity)
+                          path_capacity, grid)
 
     def render_cached(self, film, cache, cache_depth=2, spp=None, spp_offset=0, max_depth=None, rr_depth=None,
                       seed=None, path_capacity=0, stream=None):

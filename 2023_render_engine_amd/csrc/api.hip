@@ -51,7 +51,14 @@ struct rt_bvh {
     uint32_t seq = 0;
     uint32_t *next_counter() { return ray_counters + (seq++ % kRayCounterRing); }
     static constexpr uint32_t kRayCounterRing = 256;
+    uint32_t width = 4;
 };
+
+// Traversal stack entries a lane needs for this tree's layout.
+static uint32_t stack_cap(const rt_bvh *b)
+{
+    return b->width == 4 ? b->host.max_stack4 + 1 : b->host.max_depth + 1;
+}
 
 namespace rt {
 
@@ -171,17 +178,21 @@ rt_status rt_memcpy_d2h(rt_ctx *ctx, void *dst, const void *src, size_t bytes)
 /* ------------------------------------------------------------------ */
 static rt_status upload_bvh(rt_bvh *b)
 {
-    const size_t nb = b->host.nodes.size() * sizeof(rt::Node2);
+    const bool wide = b->width == 4;
+    const void *src = wide ? (const void *)b->host.nodes4.data() : (const void *)b->host.nodes.data();
+    const size_t nn = wide ? b->host.nodes4.size() : b->host.nodes.size();
+    const size_t nb = nn * 64;
     const size_t tb = b->host.tris.size() * sizeof(rt::Tri3);
     RT_HIP(hipMalloc(&b->dev.nodes, nb));
     RT_HIP(hipMalloc(&b->dev.tris, tb));
     RT_HIP(hipMalloc(&b->ray_counters, rt_bvh::kRayCounterRing * sizeof(uint32_t)));
-    RT_HIP(hipMemcpyAsync(b->dev.nodes, b->host.nodes.data(), nb, hipMemcpyHostToDevice, b->ctx->stream));
+    RT_HIP(hipMemcpyAsync(b->dev.nodes, src, nb, hipMemcpyHostToDevice, b->ctx->stream));
     RT_HIP(hipMemcpyAsync(b->dev.tris, b->host.tris.data(), tb, hipMemcpyHostToDevice, b->ctx->stream));
     RT_HIP(hipStreamSynchronize(b->ctx->stream));
-    b->dev.num_nodes = (uint32_t)b->host.nodes.size();
+    b->dev.num_nodes = (uint32_t)nn;
     b->dev.num_tris = b->host.num_tris;
-    b->dev.max_depth = b->host.max_depth;
+    b->dev.stack_cap = stack_cap(b);
+    b->dev.wide = wide ? 1u : 0u;
     return RT_OK;
 }
 
@@ -190,19 +201,23 @@ rt_status rt_bvh_build(rt_ctx *ctx, const float *xyz, uint32_t nv, const uint32_
 {
     if (!out) { set_error("rt_bvh_build: out is null"); return RT_ERR_INVALID; }
     *out = nullptr;
-    rt_bvh_opts o{4, 32, 1.2f};
+    rt_bvh_opts o{4, 32, 1.2f, 4};
     if (opts) o = *opts;
+    if (o.width == 0) o.width = 4;
+    if (o.width != 2 && o.width != 4) { set_error("rt_bvh_build: width must be 2 or 4"); return RT_ERR_INVALID; }
     rt_bvh *b = new (std::nothrow) rt_bvh();
     if (!b) { set_error("rt_bvh_build: host allocation failed"); return RT_ERR_OOM; }
     b->ctx = ctx;
+    b->width = o.width;
     std::string err;
     if (!rt::build_bvh2(xyz, nv, idx, nt, o, b->host, err)) {
         delete b;
         set_error(err);
         return RT_ERR_INVALID;
     }
+    rt::build_bvh4(b->host);
     // The traversal stack lives in LDS: refuse trees too deep for it.
-    if (rt::trace_lds_bytes(b->host.max_depth) > 64 * 1024) {
+    if (rt::trace_lds_bytes(stack_cap(b)) > 64 * 1024) {
         delete b;
         set_error("rt_bvh_build: tree depth exceeds the LDS traversal stack");
         return RT_ERR_UNSUPPORTED;
@@ -262,6 +277,10 @@ rt_status rt_bvh_get_info(const rt_bvh *b, rt_bvh_info *info)
     info->tri_bytes = (uint64_t)b->host.num_tris * sizeof(rt::Tri3);
     info->build_ms = b->host.build_ms;
     info->sah_cost = b->host.sah_cost;
+    info->width = b->width;
+    info->num_nodes4 = (uint32_t)b->host.nodes4.size();
+    info->depth4 = b->host.depth4;
+    info->stack_cap = stack_cap(b);
     return RT_OK;
 }
 
@@ -289,6 +308,13 @@ rt_status rt_bvh_export(const rt_bvh *b, float *nodes, float *tris)
     std::memcpy(nodes, b->host.nodes.data(), b->host.nodes.size() * sizeof(rt::Node2));
     // num_tris records plus the sentinel
     std::memcpy(tris, b->host.tris.data(), b->host.tris.size() * sizeof(rt::Tri3));
+    return RT_OK;
+}
+
+rt_status rt_bvh_export4(const rt_bvh *b, float *nodes4)
+{
+    if (!b || !nodes4) { set_error("rt_bvh_export4: null argument"); return RT_ERR_INVALID; }
+    std::memcpy(nodes4, b->host.nodes4.data(), b->host.nodes4.size() * sizeof(rt::Node4));
     return RT_OK;
 }
 

@@ -65,7 +65,7 @@ struct rt_batch {
     std::vector<rt_scene *> scenes;
     std::vector<rt::SceneDev> host_table;   // as uploaded (seeds included)
     std::vector<uint32_t> npix;
-    uint32_t max_depth = 0, max_lights = 1;
+    uint32_t stack_cap = 0, max_lights = 1;
     uint64_t paths_per_sample = 0;           // sum of pixels
     rt::SceneDev *d_table = nullptr;
     uint64_t *d_bases = nullptr;             // per call: n + 1 path-index bases
@@ -133,7 +133,7 @@ hipError_t launch_frame(rt_batch *b, const rt_batch_opts &o, uint32_t P, uint32_
     bl.bases = b->bases.data();
     bl.npix = b->npix.data();
     bl.films = films.data();
-    bl.max_depth = b->max_depth;
+    bl.stack_cap = b->stack_cap;
     const uint64_t total = b->bases.back();
     for (uint64_t first = 0; first < total; first += P) {
         rt::WaveParams wp{};
@@ -199,11 +199,16 @@ rt_status rt_batch_create(rt_ctx *ctx, rt_scene *const *scenes, uint32_t n, rt_b
         d.stream_base = rt::scene_stream_base(scenes[i]);
         b->host_table.push_back(d);
         b->npix.push_back(d.cam.width * d.cam.height);
-        b->max_depth = std::max(b->max_depth, d.bvh.max_depth);
+        b->stack_cap = std::max(b->stack_cap, d.bvh.stack_cap);
+        if (d.bvh.wide != b->host_table[0].bvh.wide) {
+            delete b;
+            set_error("rt_batch_create: scenes mix two-wide and four-wide trees");
+            return RT_ERR_INVALID;
+        }
         b->max_lights = std::max(b->max_lights, d.nlights);
         b->paths_per_sample += (uint64_t)d.cam.width * d.cam.height;
     }
-    if (rt::extend_lds_bytes(b->max_depth) > 64 * 1024) {
+    if (rt::extend_lds_bytes(b->stack_cap) > 64 * 1024) {
         delete b;
         set_error("rt_batch_create: BVH too deep");
         return RT_ERR_UNSUPPORTED;
@@ -330,11 +335,11 @@ rt_status rt_batch_autotune(rt_batch *b, const rt_batch_opts *opts, rt_tune_entr
     o.spp_offset = 0x40000000u;  // far from rendering samples
     const uint64_t total = b->paths_per_sample;
     std::vector<uint32_t> caps;
-    for (uint32_t P = 1u << 19; P <= (1u << 22); P <<= 1) {
+    for (uint32_t P = 1u << 19; P <= (1u << 23); P <<= 1) {
         const uint32_t c = (uint32_t)std::min<uint64_t>(P, total);
         if (caps.empty() || caps.back() != c) caps.push_back(c);
     }
-    const uint32_t grids[3] = {1024, 2048, 4096};
+    const uint32_t grids[4] = {1024, 2048, 4096, 8192};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     rt_tune_entry top{0, 0, 1e30f};
     uint32_t count = 0;

@@ -339,4 +339,126 @@ bool build_bvh2(const float *xyz, uint32_t nv, const uint32_t *idx, uint32_t nt,
     return true;
 }
 
+
+namespace {
+
+struct Kid {
+    float lo[3], hi[3];
+    int32_t ref;
+};
+
+double kid_area(const Kid &k)
+{
+    double d[3];
+    for (int a = 0; a < 3; ++a) d[a] = std::max(0.0, (double)k.hi[a] - (double)k.lo[a]);
+    return 2.0 * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]);
+}
+
+void node2_kids(const Node2 &n, Kid &k0, Kid &k1)
+{
+    k0.lo[0] = n.f[0]; k0.hi[0] = n.f[1]; k0.lo[1] = n.f[2]; k0.hi[1] = n.f[3]; k0.lo[2] = n.f[8]; k0.hi[2] = n.f[9];
+    k1.lo[0] = n.f[4]; k1.hi[0] = n.f[5]; k1.lo[1] = n.f[6]; k1.hi[1] = n.f[7]; k1.lo[2] = n.f[10]; k1.hi[2] = n.f[11];
+    k0.ref = n.child[0];
+    k1.ref = n.child[1];
+}
+
+// Smallest power-of-two scale s = 2^e (e >= -126) with 255 * s >= extent.
+int scale_exponent(double extent)
+{
+    if (!(extent > 0.0)) return -126;
+    int e = (int)std::ceil(std::log2(extent / 255.0));
+    if (e < -126) e = -126;
+    while (255.0 * std::ldexp(1.0, e) < extent) ++e;
+    while (e > -126 && 255.0 * std::ldexp(1.0, e - 1) >= extent) --e;
+    return e;
+}
+
+struct Collapser {
+    const HostBvh &in;
+    std::vector<Node4> &out;
+    uint32_t depth_max = 0;
+
+    // Emits the wide node for two-wide node n2; returns its index and the
+    // pending-entry bound of its subtree.
+    int32_t emit(uint32_t n2, uint32_t depth, uint32_t &need)
+    {
+        Kid kids[4];
+        Node2 root = in.nodes[n2];
+        node2_kids(root, kids[0], kids[1]);
+        int nk = 2;
+        while (nk < 4) {
+            int best = -1;
+            double best_area = -1.0;
+            for (int i = 0; i < nk; ++i)
+                if (kids[i].ref >= 0 && kid_area(kids[i]) > best_area) {
+                    best_area = kid_area(kids[i]);
+                    best = i;
+                }
+            if (best < 0) break;
+            Kid a, b;
+            node2_kids(in.nodes[kids[best].ref], a, b);
+            kids[best] = a;
+            kids[nk++] = b;
+        }
+        const uint32_t my = (uint32_t)out.size();
+        out.emplace_back();
+        depth_max = std::max(depth_max, depth);
+        int32_t refs[4] = {0, 0, 0, 0};
+        uint32_t sub = 0;
+        for (int i = 0; i < nk; ++i) {
+            if (kids[i].ref >= 0) {
+                uint32_t nd = 0;
+                refs[i] = emit((uint32_t)kids[i].ref, depth + 1, nd);
+                sub = std::max(sub, nd);
+            } else {
+                refs[i] = kids[i].ref;
+            }
+        }
+        need = (uint32_t)(nk - 1) + sub;
+        Node4 &n = out[my];
+        std::memset(&n, 0, sizeof n);
+        uint32_t meta = 0;
+        for (int a = 0; a < 3; ++a) {
+            float p = kids[0].lo[a];
+            for (int i = 1; i < nk; ++i) p = std::min(p, kids[i].lo[a]);
+            double ext = 0.0;
+            for (int i = 0; i < nk; ++i) ext = std::max(ext, (double)kids[i].hi[a] - (double)p);
+            const int e = scale_exponent(ext);
+            const double sc = std::ldexp(1.0, e);
+            n.origin[a] = p;
+            meta |= (uint32_t)(e + 127) << (8 * a);
+            uint32_t qlo = 0, qhi = 0;
+            for (int i = 0; i < nk; ++i) {
+                double l = std::floor(((double)kids[i].lo[a] - (double)p) / sc);
+                double h = std::ceil(((double)kids[i].hi[a] - (double)p) / sc);
+                l = std::min(std::max(l, 0.0), 255.0);
+                h = std::min(std::max(h, 0.0), 255.0);
+                // outward: the dequantized bounds enclose the box exactly
+                while (l > 0.0 && (double)p + l * sc > (double)kids[i].lo[a]) l -= 1.0;
+                while (h < 255.0 && (double)p + h * sc < (double)kids[i].hi[a]) h += 1.0;
+                qlo |= (uint32_t)l << (8 * i);
+                qhi |= (uint32_t)h << (8 * i);
+            }
+            n.q[2 * a] = qlo;
+            n.q[2 * a + 1] = qhi;
+        }
+        meta |= (uint32_t)((1u << nk) - 1u) << 24;
+        n.meta = meta;
+        for (int i = 0; i < 4; ++i) n.child[i] = refs[i];
+        return (int32_t)my;
+    }
+};
+
+}  // namespace
+
+void build_bvh4(HostBvh &h)
+{
+    h.nodes4.clear();
+    h.nodes4.reserve(h.nodes.size() / 2 + 1);
+    Collapser c{h, h.nodes4};
+    uint32_t need = 0;
+    c.emit(0, 1, need);
+    h.max_stack4 = need;
+    h.depth4 = c.depth_max;
+}
 }  // namespace rt

@@ -88,6 +88,8 @@ __global__ __launch_bounds__(kBlock) void k_reset(uint32_t *counters, uint32_t c
             if (wave_start) counters[t] = init_active;
         } else if (t == kCActive0 + (cur ^ 1u) || (t >= kCQueue && t <= kCTerm)) {
             counters[t] = 0;
+        } else if (t == kCFetchExt || t == kCFetchCon) {
+            counters[t] = 0;
         } else if (wave_start && (t == kCCacheQ || t == kCTrain)) {
             counters[t] = 0;
         }
@@ -174,88 +176,84 @@ __device__ __forceinline__ uint32_t classify(const SceneDev &sc, int32_t hid, bo
     return kQGgx;
 }
 
-template <bool B>
+// Persistent closest hit against the BVH: lanes fetch active paths from a
+// global counter and walk them one node visit per step (rt_device.h), so a
+// few long rays do not hold the rest of the wavefront idle.  The analytic
+// shapes and the material classification follow in k_classify, which keeps
+// this loop's register footprint at the traversal's.
+template <bool B, bool WIDE>
 __global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc0, BatchDev bt, PathsDev p, QueuesDev q, uint32_t cur,
-                                                   int cap, uint32_t cache_depth)
+                                                   int cap)
 {
     extern __shared__ int32_t s_stack[];
     int32_t *stack = s_stack + threadIdx.x;
     const uint32_t n = q.counters[kCActive0 + cur];
+    uint32_t *fetch = &q.counters[kCFetchExt];
+    TravState s;
+    bool has = false, exhausted = false;
+    uint32_t slot = 0, dummy0 = 0, dummy1 = 0;
+    const float4 *nodes = (const float4 *)sc0.bvh.nodes;
+    const float4 *tris = (const float4 *)sc0.bvh.tris;
+    for (;;) {
+        const uint32_t r = wave_refill(fetch, n, has, exhausted);
+        if (r != 0xffffffffu) {
+            slot = q.active[cur][r];
+            if constexpr (B) {
+                const SceneDev &sc = bt.scenes[p.sid[slot]];
+                nodes = (const float4 *)sc.bvh.nodes;
+                tris = (const float4 *)sc.bvh.tris;
+            }
+            trav_begin(s, p.ox[slot], p.oy[slot], p.oz[slot], p.dx[slot], p.dy[slot], p.dz[slot], INFINITY, false);
+            has = true;
+        }
+        if (__ballot(has) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (has && trav_step_any<WIDE, false, false>(s, nodes, tris, stack, cap, dummy0, dummy1)) {
+            p.hid[slot] = s.hit;
+            p.ht[slot] = s.thit;
+            has = false;
+        }
+    }
+}
+
+// Analytic shapes after the BVH triangle, strict < (Scene.cu:221-227,
+// Sphere.cu:321), then material classification into the per-class queues
+// with wavefront ballots (one atomic per 64 lanes per queue).
+template <bool B>
+__global__ __launch_bounds__(kBlock) void k_classify(SceneDev sc0, BatchDev bt, PathsDev p, QueuesDev q, uint32_t cur,
+                                                     uint32_t cache_depth)
+{
+    const uint32_t n = q.counters[kCActive0 + cur];
     // every lane of a wavefront runs the same number of iterations so the
     // ballots below see the whole wavefront
     const uint32_t stride = gridDim.x * kBlock;
-    const uint32_t base0 = blockIdx.x * kBlock;
-    for (uint32_t i0 = base0; i0 < n; i0 += stride) {
+    for (uint32_t i0 = blockIdx.x * kBlock; i0 < n; i0 += stride) {
         const uint32_t i = i0 + threadIdx.x;
-        const bool live = i < n;
-        uint32_t slot = 0;
-        int32_t hid = -1;
-        float th = INFINITY;
-        uint32_t cls = 0xffu;
-        if (live) {
+        uint32_t slot = 0, cls = 0xffu;
+        if (i < n) {
             slot = q.active[cur][i];
             const SceneDev &sc = scene_of<B>(sc0, bt, p, slot);
-            const float4 *nodes = (const float4 *)sc.bvh.nodes;
-            const float4 *tris = (const float4 *)sc.bvh.tris;
-            const float ox = p.ox[slot], oy = p.oy[slot], oz = p.oz[slot];
-            const float dx = p.dx[slot], dy = p.dy[slot], dz = p.dz[slot];
-            const TriRay tr = make_tri_ray(ox, oy, oz, dx, dy, dz, INFINITY);
-            const BoxRay br = make_box_ray(ox, oy, oz, dx, dy, dz);
-            float h = INFINITY;
-            int sp = 0;
-            int32_t curn = 0;
-            for (;;) {
-                if (curn >= 0) {
-                    const float4 *np = nodes + 4 * (size_t)curn;
-                    const float4 a = np[0], b = np[1], c = np[2], d = np[3];
-                    const float tlim = h * kHitScale;
-                    bool h0, h1;
-                    const float t0 = slab(br, a.x, a.y, a.z, a.w, c.x, c.y, tlim, &h0);
-                    const float t1 = slab(br, b.x, b.y, b.z, b.w, c.z, c.w, tlim, &h1);
-                    const int32_t c0 = __float_as_int(d.x), c1 = __float_as_int(d.y);
-                    if (h0 && h1) {
-                        const bool sw = t1 < t0;
-                        if (sp < cap) { stack[sp * kBlock] = sw ? c0 : c1; ++sp; }
-                        curn = sw ? c1 : c0;
-                    } else if (h0) {
-                        curn = c0;
-                    } else if (h1) {
-                        curn = c1;
-                    } else if (sp > 0) {
-                        --sp;
-                        curn = stack[sp * kBlock];
-                    } else {
-                        break;
-                    }
-                } else {
-                    uint32_t first, count;
-                    decode_leaf(curn, first, count);
-                    for (uint32_t k = 0; k < count; ++k) {
-                        const float4 *tp = tris + 3 * (size_t)(first + k);
-                        if (tri_intersects(tr, tp[0], tp[1], tp[2], h)) {
-                            hid = __float_as_int(tp[0].w);
-                            th = h;
-                        }
-                    }
-                    if (sp > 0) {
-                        --sp;
-                        curn = stack[sp * kBlock];
-                    } else {
-                        break;
+            int32_t hid = p.hid[slot];
+            float th = p.ht[slot];
+            if (sc.nsph) {
+                const V3 o = mk(p.ox[slot], p.oy[slot], p.oz[slot]), d = mk(p.dx[slot], p.dy[slot], p.dz[slot]);
+                bool changed = false;
+                for (uint32_t j = 0; j < sc.nsph; ++j) {
+                    V3 so, sd;
+                    const float ts = sphere_core(sc.sph[j], o, d, INFINITY, so, sd);
+                    if (ts >= 0 && ts < th) {
+                        th = ts;
+                        hid = -(int32_t)j - 2;
+                        changed = true;
                     }
                 }
-            }
-            // analytic shapes after the BVH triangle, strict < (Scene.cu:221-227, Sphere.cu:321)
-            for (uint32_t j = 0; j < sc.nsph; ++j) {
-                V3 so, sd;
-                const float ts = sphere_core(sc.sph[j], mk(ox, oy, oz), mk(dx, dy, dz), INFINITY, so, sd);
-                if (ts >= 0 && ts < th) {
-                    th = ts;
-                    hid = -(int32_t)j - 2;
+                if (changed) {
+                    p.hid[slot] = hid;
+                    p.ht[slot] = th;
                 }
             }
-            p.hid[slot] = hid;
-            p.ht[slot] = th;
             const bool deep = cache_depth != kNoCache && (p.meta[slot] & 0xffffu) >= cache_depth;
             cls = classify(sc, hid, deep);
         }
@@ -348,6 +346,19 @@ __device__ __forceinline__ void resolve_nee(const PathsDev &p, uint32_t slot)
     p.nee_mask[slot] = 0;
 }
 
+// Shadow-ray occlusion by the analytic shapes (Scene.cu:108-150): opaque
+// spheres other than `skip` (the sampled light) hit at t <= tmax.
+__device__ __forceinline__ bool spheres_occlude(const SceneDev &sc, V3 o, V3 d, float tmax, int32_t skip)
+{
+    for (uint32_t j = 0; j < sc.nsph; ++j) {
+        if ((int32_t)j == skip || sc.sph[j].transparent) continue;
+        V3 so, sd;
+        const float ts = sphere_core(sc.sph[j], o, d, INFINITY, so, sd);
+        if (ts >= 0 && !(ts > tmax)) return true;
+    }
+    return false;
+}
+
 // Radiance-cache query record (12 floats): position, the normal facing the
 // incoming ray, the direction back along it, a roughness feature (diffuse 1,
 // GGX alpha), padding.
@@ -417,7 +428,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
             const uint32_t nal_w = B ? wave_max(nal) : sc0.nlights - sc0.npl;
             for (uint32_t l = 0; l < npl_w; ++l) {
                 bool e = false;
-                V3 Lw = mk(0, 0, 1);
+                V3 Lw = mk(0, 0, 1), o2 = mk(0, 0, 0);
                 float dist = 0;
                 if (live && l < npl) {
                     const rt_point_light &pl = sc.pl[l];
@@ -436,26 +447,25 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
                             c[3 * l] = f.x * Li.x * w;
                             c[3 * l + 1] = f.y * Li.y * w;
                             c[3 * l + 2] = f.z * Li.z * w;
-                            e = true;
                             dist = len(tol);
+                            o2 = spawn_origin(s.p, s.perr, s.n, Lw);
+                            e = !spheres_occlude(sc, o2, Lw, dist, -1);
                         }
                     }
                 }
                 const uint32_t pos = wave_append(&q.counters[kCShadow], e);
                 if (e) {
                     emit |= 1u << l;
-                    const V3 o2 = spawn_origin(s.p, s.perr, s.n, Lw);
                     sh.ox[pos] = o2.x; sh.oy[pos] = o2.y; sh.oz[pos] = o2.z;
                     sh.dx[pos] = Lw.x; sh.dy[pos] = Lw.y; sh.dz[pos] = Lw.z;
                     sh.tmax[pos] = dist;
                     sh.slot[pos] = (slot << 5) | l;
-                    sh.skip[pos] = -1;
                 }
             }
             for (uint32_t a = 0; a < nal_w; ++a) {
                 uint32_t j = 0, l = 0;
                 bool e = false;
-                V3 Lw = mk(0, 0, 1);
+                V3 Lw = mk(0, 0, 1), o2 = mk(0, 0, 0);
                 float dist = 0;
                 if (live && a < nal) {
                     j = sc.area_light[a];
@@ -473,19 +483,18 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
                             c[3 * l] = f.x * lm.kd[0] * lm.intensity * k;
                             c[3 * l + 1] = f.y * lm.kd[1] * lm.intensity * k;
                             c[3 * l + 2] = f.z * lm.kd[2] * lm.intensity * k;
-                            e = true;
+                            o2 = spawn_origin(s.p, s.perr, s.n, Lw);
+                            e = !spheres_occlude(sc, o2, Lw, dist * 0.9999f, (int32_t)j);
                         }
                     }
                 }
                 const uint32_t pos = wave_append(&q.counters[kCShadow], e);
                 if (e) {
                     emit |= 1u << l;
-                    const V3 o2 = spawn_origin(s.p, s.perr, s.n, Lw);
                     sh.ox[pos] = o2.x; sh.oy[pos] = o2.y; sh.oz[pos] = o2.z;
                     sh.dx[pos] = Lw.x; sh.dy[pos] = Lw.y; sh.dz[pos] = Lw.z;
                     sh.tmax[pos] = dist * 0.9999f;
                     sh.slot[pos] = (slot << 5) | l;
-                    sh.skip[pos] = (int32_t)j;
                 }
             }
             if (live) {
@@ -577,79 +586,42 @@ __global__ __launch_bounds__(kBlock) void k_cache_vertex(SceneDev sc, PathsDev p
 // ---------------------------------------------------------------------
 // connect: any hit for shadow rays (Scene.cu:108-150, 238-276)
 // ---------------------------------------------------------------------
-template <bool B>
+// Persistent any hit of the shadow queue against the BVH (the analytic
+// shapes were tested when the shadow ray was made, in k_shade); an occluded
+// ray clears its light bit in the path's NEE mask.
+template <bool B, bool WIDE>
 __global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc0, BatchDev bt, PathsDev p, ShadowDev sh, QueuesDev q,
                                                     int cap)
 {
     extern __shared__ int32_t s_stack[];
     int32_t *stack = s_stack + threadIdx.x;
     const uint32_t n = q.counters[kCShadow];
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const uint32_t sl = sh.slot[i];
-        const SceneDev &sc = scene_of<B>(sc0, bt, p, sl >> 5);
-        const float4 *nodes = (const float4 *)sc.bvh.nodes;
-        const float4 *tris = (const float4 *)sc.bvh.tris;
-        const float ox = sh.ox[i], oy = sh.oy[i], oz = sh.oz[i];
-        const float dx = sh.dx[i], dy = sh.dy[i], dz = sh.dz[i];
-        const float tmax = sh.tmax[i];
-        bool occ = false;
-        // analytic shapes first: cheap, and any opaque hit decides
-        const int32_t skip = sh.skip[i];
-        for (uint32_t j = 0; j < sc.nsph && !occ; ++j) {
-            if ((int32_t)j == skip || sc.sph[j].transparent) continue;
-            V3 so, sd;
-            const float ts = sphere_core(sc.sph[j], mk(ox, oy, oz), mk(dx, dy, dz), INFINITY, so, sd);
-            if (ts >= 0 && !(ts > tmax)) occ = true;
-        }
-        if (!occ) {
-            const TriRay tr = make_tri_ray(ox, oy, oz, dx, dy, dz, INFINITY);
-            const BoxRay br = make_box_ray(ox, oy, oz, dx, dy, dz);
-            const float tlim = tmax * kHitScale;
-            int sp = 0;
-            int32_t curn = 0;
-            for (;;) {
-                if (curn >= 0) {
-                    const float4 *np = nodes + 4 * (size_t)curn;
-                    const float4 a = np[0], b = np[1], c = np[2], d = np[3];
-                    bool h0, h1;
-                    const float t0 = slab(br, a.x, a.y, a.z, a.w, c.x, c.y, tlim, &h0);
-                    const float t1 = slab(br, b.x, b.y, b.z, b.w, c.z, c.w, tlim, &h1);
-                    const int32_t c0 = __float_as_int(d.x), c1 = __float_as_int(d.y);
-                    if (h0 && h1) {
-                        const bool sw = t1 < t0;
-                        if (sp < cap) { stack[sp * kBlock] = sw ? c0 : c1; ++sp; }
-                        curn = sw ? c1 : c0;
-                    } else if (h0) {
-                        curn = c0;
-                    } else if (h1) {
-                        curn = c1;
-                    } else if (sp > 0) {
-                        --sp;
-                        curn = stack[sp * kBlock];
-                    } else {
-                        break;
-                    }
-                } else {
-                    uint32_t first, count;
-                    decode_leaf(curn, first, count);
-                    for (uint32_t k = 0; k < count && !occ; ++k) {
-                        const float4 *tp = tris + 3 * (size_t)(first + k);
-                        const float4 vb = tp[1];
-                        if (__float_as_uint(vb.w) & 1u) continue;  // transparent (Triangle.cu:439-440)
-                        float hh = INFINITY;
-                        if (tri_intersects(tr, tp[0], vb, tp[2], hh) && !(hh > tmax)) occ = true;
-                    }
-                    if (occ) break;
-                    if (sp > 0) {
-                        --sp;
-                        curn = stack[sp * kBlock];
-                    } else {
-                        break;
-                    }
-                }
+    uint32_t *fetch = &q.counters[kCFetchCon];
+    TravState s;
+    bool has = false, exhausted = false;
+    uint32_t sl = 0, dummy0 = 0, dummy1 = 0;
+    const float4 *nodes = (const float4 *)sc0.bvh.nodes;
+    const float4 *tris = (const float4 *)sc0.bvh.tris;
+    for (;;) {
+        const uint32_t r = wave_refill(fetch, n, has, exhausted);
+        if (r != 0xffffffffu) {
+            sl = sh.slot[r];
+            if constexpr (B) {
+                const SceneDev &sc = bt.scenes[p.sid[sl >> 5]];
+                nodes = (const float4 *)sc.bvh.nodes;
+                tris = (const float4 *)sc.bvh.tris;
             }
+            trav_begin(s, sh.ox[r], sh.oy[r], sh.oz[r], sh.dx[r], sh.dy[r], sh.dz[r], sh.tmax[r], true);
+            has = true;
         }
-        if (occ) atomicAnd(&p.nee_mask[sl >> 5], ~(1u << (sl & 31u)));
+        if (__ballot(has) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (has && trav_step_any<WIDE, true, false>(s, nodes, tris, stack, cap, dummy0, dummy1)) {
+            if (s.occ) atomicAnd(&p.nee_mask[sl >> 5], ~(1u << (sl & 31u)));
+            has = false;
+        }
     }
 }
 
@@ -761,18 +733,36 @@ hipError_t set_u32(uint32_t *p, uint32_t v, hipStream_t s)
     return hipGetLastError();
 }
 
-size_t extend_lds_bytes(uint32_t max_depth) { return (size_t)(max_depth + 1) * kBlock * sizeof(int32_t); }
+size_t extend_lds_bytes(uint32_t stack_cap) { return (size_t)stack_cap * kBlock * sizeof(int32_t); }
+
+// Workgroups of a persistent traversal launch: as many as fit on the chip
+// (LDS-limited, at most 16 per CU).
+static dim3 persistent_grid(size_t lds)
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess || cus <= 0) cus = 256;
+    }
+    size_t per_cu = lds ? (160u * 1024u) / lds : 16;
+    per_cu = per_cu > 16 ? 16 : (per_cu < 1 ? 1 : per_cu);
+    return dim3((unsigned)(cus * per_cu));
+}
 
 hipError_t render_wave(const SceneDev &sc, const PathsDev &paths, const ShadowDev &shadow, const QueuesDev &q,
                        const FilmDev &film, const WaveParams &wp, uint32_t capacity, hipStream_t stream,
                        const CacheDev *cache, const CacheWork *cwork, const BatchLaunch *batch)
 {
     const bool B = batch != nullptr;
-    const uint32_t max_depth = B ? batch->max_depth : sc.bvh.max_depth;
-    const int cap = (int)max_depth + 1;
-    const size_t lds = extend_lds_bytes(max_depth);
+    const uint32_t stack_cap = B ? batch->stack_cap : sc.bvh.stack_cap;
+    const bool wide = sc.bvh.wide != 0;  // a batch's scenes share one layout
+    const int cap = (int)stack_cap;
+    const size_t lds = extend_lds_bytes(stack_cap);
     const dim3 blk(kBlock);
     const dim3 grid(wp.grid ? wp.grid : kGrid);
+    const dim3 pgrid = persistent_grid(lds);
     const uint32_t gen_grid = (wp.count + kBlock - 1) / kBlock;
     const BatchDev bt = B ? batch->dev : BatchDev{nullptr, nullptr, 0};
     // cache vertices end rendering paths only; training paths trace to the end
@@ -788,14 +778,18 @@ hipError_t render_wave(const SceneDev &sc, const PathsDev &paths, const ShadowDe
     for (uint32_t bounce = 0; bounce < wp.max_depth; ++bounce) {
         if (bounce > 0) hipLaunchKernelGGL(k_reset, dim3(1), blk, 0, stream, q.counters, cur, 0xffffffffu);
         if (B) {
-            hipLaunchKernelGGL(k_extend<true>, grid, blk, lds, stream, sc, bt, paths, q, cur, cap, ext_depth);
+            if (wide) hipLaunchKernelGGL((k_extend<true, true>), pgrid, blk, lds, stream, sc, bt, paths, q, cur, cap);
+            else hipLaunchKernelGGL((k_extend<true, false>), pgrid, blk, lds, stream, sc, bt, paths, q, cur, cap);
+            hipLaunchKernelGGL(k_classify<true>, grid, blk, 0, stream, sc, bt, paths, q, cur, ext_depth);
             hipLaunchKernelGGL((k_shade<kQLambert, true>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQDielectric, true>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQGgx, true>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQLight, true>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQMiss, true>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
         } else {
-            hipLaunchKernelGGL(k_extend<false>, grid, blk, lds, stream, sc, bt, paths, q, cur, cap, ext_depth);
+            if (wide) hipLaunchKernelGGL((k_extend<false, true>), pgrid, blk, lds, stream, sc, bt, paths, q, cur, cap);
+            else hipLaunchKernelGGL((k_extend<false, false>), pgrid, blk, lds, stream, sc, bt, paths, q, cur, cap);
+            hipLaunchKernelGGL(k_classify<false>, grid, blk, 0, stream, sc, bt, paths, q, cur, ext_depth);
             hipLaunchKernelGGL((k_shade<kQLambert, false>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQDielectric, false>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQGgx, false>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
@@ -805,8 +799,10 @@ hipError_t render_wave(const SceneDev &sc, const PathsDev &paths, const ShadowDe
         if (lookups && bounce >= wp.cache_depth)
             hipLaunchKernelGGL(k_cache_vertex, grid, blk, 0, stream, sc, paths, q);
         hipLaunchKernelGGL(k_count, dim3(1), blk, 0, stream, q, cur);
-        if (B) hipLaunchKernelGGL(k_connect<true>, grid, blk, lds, stream, sc, bt, paths, shadow, q, cap);
-        else hipLaunchKernelGGL(k_connect<false>, grid, blk, lds, stream, sc, bt, paths, shadow, q, cap);
+        if (B && wide) hipLaunchKernelGGL((k_connect<true, true>), pgrid, blk, lds, stream, sc, bt, paths, shadow, q, cap);
+        else if (B) hipLaunchKernelGGL((k_connect<true, false>), pgrid, blk, lds, stream, sc, bt, paths, shadow, q, cap);
+        else if (wide) hipLaunchKernelGGL((k_connect<false, true>), pgrid, blk, lds, stream, sc, bt, paths, shadow, q, cap);
+        else hipLaunchKernelGGL((k_connect<false, false>), pgrid, blk, lds, stream, sc, bt, paths, shadow, q, cap);
         hipLaunchKernelGGL(k_finalize, grid, blk, 0, stream, paths, q, 0u, cur);
         cur ^= 1u;
     }

@@ -161,4 +161,218 @@ __device__ __forceinline__ void decode_leaf(int32_t c, uint32_t &first, uint32_t
     count = (enc & 7u) + 1u;
 }
 
+// ---------------------------------------------------------------------
+// Persistent traversal state machine, shared by the ray-query kernels and
+// the renderer's extend / connect kernels.  Each lane walks its ray one node
+// visit (or one leaf) per step; the stack lives in LDS laid out
+// [depth][lane] with a stride of kTravBlock entries, so a wavefront's pushes
+// hit distinct banks.  Its depth is the tree's internal depth + 1, which
+// bounds the pending entries, so it cannot overflow.
+// ---------------------------------------------------------------------
+constexpr int kTravBlock = 128;
+constexpr uint32_t kRefillLanes = 16;  // refill a wavefront when at least this many lanes are idle
+
+struct TravState {
+    TriRay tr;
+    BoxRay br;
+    float tmax, h, thit;
+    int32_t hit, cur;
+    int sp;
+    bool occ;
+};
+
+// Closest hit: the triangle test sees ray.maximumOffset = tmax and a running
+// hitOffset starting at +inf (BVH.cu:362).  Any hit: shadow rays have
+// maximumOffset = +inf and each triangle is judged on its own
+// (order-independent), occluding iff accepted at t <= tmax.
+__device__ __forceinline__ void trav_begin(TravState &s, float ox, float oy, float oz, float dx, float dy, float dz,
+                                           float tmax, bool any)
+{
+    s.tmax = tmax;
+    s.tr = make_tri_ray(ox, oy, oz, dx, dy, dz, any ? INFINITY : tmax);
+    s.br = make_box_ray(ox, oy, oz, dx, dy, dz);
+    s.h = INFINITY;
+    s.thit = INFINITY;
+    s.hit = -1;
+    s.cur = 0;
+    s.sp = 0;
+    s.occ = false;
+}
+
+// One traversal step.  Returns true when the ray is finished.
+template <bool ANY, bool STATS>
+__device__ __forceinline__ bool trav_step(TravState &s, const float4 *__restrict__ nodes,
+                                          const float4 *__restrict__ tris, int32_t *stack, int cap,
+                                          uint32_t &n_node, uint32_t &n_tri)
+{
+    if (s.cur >= 0) {
+        if (STATS) ++n_node;
+        const float4 *np = nodes + 4 * (size_t)s.cur;
+        const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+        const float tlim = ANY ? s.tmax * kHitScale : fminf(s.h, s.tmax) * kHitScale;
+        bool h0, h1;
+        const float t0 = slab(s.br, a.x, a.y, a.z, a.w, c.x, c.y, tlim, &h0);
+        const float t1 = slab(s.br, b.x, b.y, b.z, b.w, c.z, c.w, tlim, &h1);
+        const int32_t c0 = __float_as_int(d.x), c1 = __float_as_int(d.y);
+        if (h0 && h1) {
+            const bool swap = t1 < t0;
+            if (s.sp < cap) {
+                stack[s.sp * kTravBlock] = swap ? c0 : c1;
+                ++s.sp;
+            }
+            s.cur = swap ? c1 : c0;
+            return false;
+        }
+        if (h0) { s.cur = c0; return false; }
+        if (h1) { s.cur = c1; return false; }
+    } else {
+        uint32_t first, count;
+        decode_leaf(s.cur, first, count);
+        for (uint32_t k = 0; k < count; ++k) {
+            const float4 *tp = tris + 3 * (size_t)(first + k);
+            const float4 va = tp[0], vb = tp[1], vc = tp[2];
+            if (STATS) ++n_tri;
+            if (ANY) {
+                if (__float_as_uint(vb.w) & 1u) continue;  // transparent (Triangle.cu:439-440)
+                float hh = INFINITY;
+                if (tri_intersects(s.tr, va, vb, vc, hh) && !(hh > s.tmax)) {
+                    s.occ = true;
+                    return true;
+                }
+            } else if (tri_intersects(s.tr, va, vb, vc, s.h)) {
+                s.hit = __float_as_int(va.w);
+                s.thit = s.h;
+            }
+        }
+    }
+    if (s.sp > 0) {
+        --s.sp;
+        s.cur = stack[s.sp * kTravBlock];
+        return false;
+    }
+    return true;
+}
+
+// One step over the four-wide quantized tree (Node4, rt_internal.h): the
+// four child boxes are dequantized with one FMA per bound against the
+// node's origin and power-of-two scale, hit children are sorted by entry
+// distance (a 5-comparator network on the distance bits; misses sort last),
+// the nearest is followed and the others are pushed farthest first.
+__device__ __forceinline__ void cswap_key(uint32_t &ka, int32_t &ra, uint32_t &kb, int32_t &rb)
+{
+    const bool sw = kb < ka;
+    const uint32_t k0 = sw ? kb : ka, k1 = sw ? ka : kb;
+    const int32_t r0 = sw ? rb : ra, r1 = sw ? ra : rb;
+    ka = k0; kb = k1; ra = r0; rb = r1;
+}
+
+template <bool ANY, bool STATS>
+__device__ __forceinline__ bool trav_step4(TravState &s, const float4 *__restrict__ nodes,
+                                           const float4 *__restrict__ tris, int32_t *stack, int cap,
+                                           uint32_t &n_node, uint32_t &n_tri)
+{
+    if (s.cur >= 0) {
+        if (STATS) ++n_node;
+        const float4 *np = nodes + 4 * (size_t)s.cur;
+        const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+        const float tlim = ANY ? s.tmax * kHitScale : fminf(s.h, s.tmax) * kHitScale;
+        const uint32_t meta = __float_as_uint(a.w);
+        const float Ax = __uint_as_float((meta & 0xffu) << 23) * s.br.ix;
+        const float Ay = __uint_as_float(((meta >> 8) & 0xffu) << 23) * s.br.iy;
+        const float Az = __uint_as_float(((meta >> 16) & 0xffu) << 23) * s.br.iz;
+        const float Bx = fmaf(a.x, s.br.ix, -s.br.oix);
+        const float By = fmaf(a.y, s.br.iy, -s.br.oiy);
+        const float Bz = fmaf(a.z, s.br.iz, -s.br.oiz);
+        const uint32_t qlx = __float_as_uint(c.x), qhx = __float_as_uint(c.y);
+        const uint32_t qly = __float_as_uint(c.z), qhy = __float_as_uint(c.w);
+        const uint32_t qlz = __float_as_uint(d.x), qhz = __float_as_uint(d.y);
+        uint32_t key[4];
+        int32_t ref[4] = {__float_as_int(b.x), __float_as_int(b.y), __float_as_int(b.z), __float_as_int(b.w)};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int sh = 8 * i;
+            const float tx0 = fmaf((float)((qlx >> sh) & 0xffu), Ax, Bx);
+            const float tx1 = fmaf((float)((qhx >> sh) & 0xffu), Ax, Bx);
+            const float ty0 = fmaf((float)((qly >> sh) & 0xffu), Ay, By);
+            const float ty1 = fmaf((float)((qhy >> sh) & 0xffu), Ay, By);
+            const float tz0 = fmaf((float)((qlz >> sh) & 0xffu), Az, Bz);
+            const float tz1 = fmaf((float)((qhz >> sh) & 0xffu), Az, Bz);
+            const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+            float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+            tf = fminf(tf * kTfarScale, tlim);
+            const bool hit = ((meta >> (24 + i)) & 1u) && tn <= tf;
+            key[i] = hit ? __float_as_uint(tn) : 0xffffffffu;
+        }
+        cswap_key(key[0], ref[0], key[1], ref[1]);
+        cswap_key(key[2], ref[2], key[3], ref[3]);
+        cswap_key(key[0], ref[0], key[2], ref[2]);
+        cswap_key(key[1], ref[1], key[3], ref[3]);
+        cswap_key(key[1], ref[1], key[2], ref[2]);
+        if (key[0] != 0xffffffffu) {
+#pragma unroll
+            for (int i = 3; i >= 1; --i)
+                if (key[i] != 0xffffffffu && s.sp < cap) {
+                    stack[s.sp * kTravBlock] = ref[i];
+                    ++s.sp;
+                }
+            s.cur = ref[0];
+            return false;
+        }
+    } else {
+        uint32_t first, count;
+        decode_leaf(s.cur, first, count);
+        for (uint32_t k = 0; k < count; ++k) {
+            const float4 *tp = tris + 3 * (size_t)(first + k);
+            const float4 va = tp[0], vb = tp[1], vc = tp[2];
+            if (STATS) ++n_tri;
+            if (ANY) {
+                if (__float_as_uint(vb.w) & 1u) continue;  // transparent (Triangle.cu:439-440)
+                float hh = INFINITY;
+                if (tri_intersects(s.tr, va, vb, vc, hh) && !(hh > s.tmax)) {
+                    s.occ = true;
+                    return true;
+                }
+            } else if (tri_intersects(s.tr, va, vb, vc, s.h)) {
+                s.hit = __float_as_int(va.w);
+                s.thit = s.h;
+            }
+        }
+    }
+    if (s.sp > 0) {
+        --s.sp;
+        s.cur = stack[s.sp * kTravBlock];
+        return false;
+    }
+    return true;
+}
+
+// Dispatch on the tree layout.
+template <bool WIDE, bool ANY, bool STATS>
+__device__ __forceinline__ bool trav_step_any(TravState &s, const float4 *__restrict__ nodes,
+                                              const float4 *__restrict__ tris, int32_t *stack, int cap,
+                                              uint32_t &n_node, uint32_t &n_tri)
+{
+    if constexpr (WIDE) return trav_step4<ANY, STATS>(s, nodes, tris, stack, cap, n_node, n_tri);
+    else return trav_step<ANY, STATS>(s, nodes, tris, stack, cap, n_node, n_tri);
+}
+
+// Wavefront-uniform refill of idle lanes from a global work counter: one
+// atomic per refill.  Returns the lane's new work index, or UINT32_MAX.
+// `exhausted` (wavefront-uniform) is set once the counter has passed n.
+__device__ __forceinline__ uint32_t wave_refill(uint32_t *counter, uint32_t n, bool has, bool &exhausted)
+{
+    const uint64_t idle = __ballot(!has);
+    const uint32_t nidle = (uint32_t)__popcll(idle);
+    if (exhausted || !(nidle >= kRefillLanes || nidle == 64u)) return 0xffffffffu;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)idle) - 1);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, nidle);
+    base = __shfl(base, (int)leader);
+    if (base + nidle >= n) exhausted = true;
+    if (has) return 0xffffffffu;
+    const uint32_t r = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+    return r < n ? r : 0xffffffffu;
+}
+
 }  // namespace rt

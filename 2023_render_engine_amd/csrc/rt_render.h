@@ -17,7 +17,7 @@ enum : uint32_t { kQLambert = 0, kQDielectric = 1, kQGgx = 2, kQLight = 3, kQMis
 // Device counters (one u32 each).
 enum : uint32_t {
     kCActive0 = 0, kCActive1 = 1, kCQueue = 2, /* 2..7 */ kCShadow = 8, kCTerm = 9, kCCacheQ = 10,
-    kCTrain = 11, kNumCounters = 16
+    kCTrain = 11, kCFetchExt = 12, kCFetchCon = 13, kNumCounters = 16
 };
 
 constexpr uint32_t kNoCache = 0xffffffffu;
@@ -77,7 +77,6 @@ struct PathsDev {
 struct ShadowDev {
     float *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
     uint32_t *slot;             // path slot << 5 | light slot
-    int32_t *skip;              // sphere to ignore (the sampled light), -1 none
 };
 
 struct QueuesDev {
@@ -112,7 +111,7 @@ struct BatchLaunch {
     const uint64_t *bases;
     const uint32_t *npix;
     const FilmDev *films;
-    uint32_t max_depth;         // deepest BVH of the batch (LDS stack size)
+    uint32_t stack_cap;         // largest traversal stack of the batch's trees (LDS)
 };
 
 struct CacheDev;
@@ -131,7 +130,7 @@ hipError_t render_wave(const SceneDev &scene, const PathsDev &paths, const Shado
                        hipStream_t stream, const CacheDev *cache = nullptr, const CacheWork *cwork = nullptr,
                        const BatchLaunch *batch = nullptr);
 
-size_t extend_lds_bytes(uint32_t max_depth);
+size_t extend_lds_bytes(uint32_t stack_cap);
 
 // *p = v from a one-thread kernel (for values a captured graph reads).
 hipError_t set_u32(uint32_t *p, uint32_t v, hipStream_t s);

@@ -12,9 +12,11 @@ struct RaysDev {
 
 // Device copy of a built BVH (see rt_internal.h for the record layouts).
 struct DevBvh {
-    void *nodes = nullptr;  // Node2[num_nodes]
+    void *nodes = nullptr;  // Node4[num_nodes] when wide, else Node2[num_nodes]
     void *tris = nullptr;   // Tri3[num_tris + 1]
-    uint32_t num_nodes = 0, num_tris = 0, max_depth = 0;
+    uint32_t num_nodes = 0, num_tris = 0;
+    uint32_t stack_cap = 0; // traversal stack entries per lane
+    uint32_t wide = 0;      // 1: four-wide quantized nodes
 };
 
 // counters != nullptr selects the counting variant (2 x u64: node visits,
@@ -24,6 +26,6 @@ struct DevBvh {
 hipError_t launch_trace(const DevBvh &bvh, const RaysDev &rays, uint32_t n, int32_t *prim, float *t,
                         uint8_t *occluded, bool any, hipStream_t stream, unsigned long long *counters,
                         uint32_t *ray_counter);
-size_t trace_lds_bytes(uint32_t max_depth);
+size_t trace_lds_bytes(uint32_t stack_cap);
 
 }  // namespace rt

@@ -268,7 +268,6 @@ rt_status ensure_wave_buffers(WaveBuffers &w, uint32_t P, uint32_t NL, const cha
     float **sf[] = {&sh.ox, &sh.oy, &sh.oz, &sh.dx, &sh.dy, &sh.dz, &sh.tmax};
     for (float **f : sf) add((void **)f, SQ * 4);
     add((void **)&sh.slot, SQ * 4);
-    add((void **)&sh.skip, SQ * 4);
     QueuesDev &q = w.queues;
     add((void **)&q.active[0], (size_t)P * 4);
     add((void **)&q.active[1], (size_t)P * 4);
@@ -426,7 +425,7 @@ rt_status rt_scene_commit(rt_scene *s)
     const uint32_t nt = (uint32_t)s->tri_mat.size();
     std::vector<uint32_t> idx(3 * (size_t)nt);
     for (uint32_t i = 0; i < 3 * nt; ++i) idx[i] = i;
-    rt_bvh_opts o{4, 32, 1.2f};
+    rt_bvh_opts o{4, 32, 1.2f, 4};
     rt_status st = rt_bvh_build(s->ctx, s->xyz.data(), 3 * nt, idx.data(), nt, &o, &s->bvh);
     if (st != RT_OK) return st;
     std::vector<uint8_t> transparent(nt);
@@ -463,7 +462,7 @@ rt_status rt_scene_commit(rt_scene *s)
     d.nlights = d.npl + na;
     d.nmat = (uint32_t)s->mats.size();
     if (d.nlights > 32) { set_error("rt_scene_commit: more than 32 light slots"); return RT_ERR_UNSUPPORTED; }
-    if (rt::extend_lds_bytes(info.max_depth) > 64 * 1024) { set_error("rt_scene_commit: BVH too deep"); return RT_ERR_UNSUPPORTED; }
+    if (rt::extend_lds_bytes(d.bvh.stack_cap) > 64 * 1024) { set_error("rt_scene_commit: BVH too deep"); return RT_ERR_UNSUPPORTED; }
     s->committed = true;
     return RT_OK;
 }
@@ -561,6 +560,7 @@ static rt_status render_frames(const char *fn, rt_scene *s, const rt_render_opts
         wp.stream_base = s->stream_base;
         wp.cache_depth = cache ? cache_depth : rt::kNoCache;
         wp.train = 0;
+        wp.grid = opts->grid;
         RT_HIP(rt::render_wave(s->dev, s->wb.paths, s->wb.shadow, s->wb.queues, film, wp, s->wb.capacity, stream,
                                cache ? &rt::cache_device(cache) : nullptr, cache ? &s->cwork : nullptr));
     }

@@ -21,96 +21,17 @@
 
 namespace rt {
 
-constexpr int kTraceBlock = 128;
-constexpr uint32_t kRefillLanes = 16;  // refill when at least this many lanes are idle
-
-// Per-lane traversal state.
-struct TravState {
-    TriRay tr;
-    BoxRay br;
-    float tmax, h, thit;
-    int32_t hit, cur;
-    int sp;
-    bool occ;
-};
+constexpr int kTraceBlock = kTravBlock;
 
 __device__ __forceinline__ void trav_init(TravState &s, const RaysDev &rays, uint32_t i, bool any)
 {
-    const float ox = rays.ox[i], oy = rays.oy[i], oz = rays.oz[i];
-    const float dx = rays.dx[i], dy = rays.dy[i], dz = rays.dz[i];
-    s.tmax = rays.tmax ? rays.tmax[i] : INFINITY;
-    // Closest hit: the triangle test sees ray.maximumOffset = tmax and a
-    // running hitOffset starting at +inf (BVH.cu:362).  Any hit: shadow rays
-    // have maximumOffset = +inf and each triangle is judged on its own
-    // (order-independent), occluding iff accepted at t <= tmax.
-    s.tr = make_tri_ray(ox, oy, oz, dx, dy, dz, any ? INFINITY : s.tmax);
-    s.br = make_box_ray(ox, oy, oz, dx, dy, dz);
-    s.h = INFINITY;
-    s.thit = INFINITY;
-    s.hit = -1;
-    s.cur = 0;
-    s.sp = 0;
-    s.occ = false;
-}
-
-// One traversal step (one node visit or one leaf).  Returns true when the
-// ray is finished.
-template <bool ANY, bool STATS>
-__device__ __forceinline__ bool trav_step(TravState &s, const float4 *__restrict__ nodes,
-                                          const float4 *__restrict__ tris, int32_t *stack, int cap,
-                                          uint32_t &n_node, uint32_t &n_tri)
-{
-    if (s.cur >= 0) {
-        if (STATS) ++n_node;
-        const float4 *np = nodes + 4 * (size_t)s.cur;
-        const float4 a = np[0], b = np[1], c = np[2], d = np[3];
-        const float tlim = ANY ? s.tmax * kHitScale : fminf(s.h, s.tmax) * kHitScale;
-        bool h0, h1;
-        const float t0 = slab(s.br, a.x, a.y, a.z, a.w, c.x, c.y, tlim, &h0);
-        const float t1 = slab(s.br, b.x, b.y, b.z, b.w, c.z, c.w, tlim, &h1);
-        const int32_t c0 = __float_as_int(d.x), c1 = __float_as_int(d.y);
-        if (h0 && h1) {
-            const bool swap = t1 < t0;
-            if (s.sp < cap) {
-                stack[s.sp * kTraceBlock] = swap ? c0 : c1;
-                ++s.sp;
-            }
-            s.cur = swap ? c1 : c0;
-            return false;
-        }
-        if (h0) { s.cur = c0; return false; }
-        if (h1) { s.cur = c1; return false; }
-    } else {
-        uint32_t first, count;
-        decode_leaf(s.cur, first, count);
-        for (uint32_t k = 0; k < count; ++k) {
-            const float4 *tp = tris + 3 * (size_t)(first + k);
-            const float4 va = tp[0], vb = tp[1], vc = tp[2];
-            if (STATS) ++n_tri;
-            if (ANY) {
-                if (__float_as_uint(vb.w) & 1u) continue;  // transparent (Triangle.cu:439-440)
-                float hh = INFINITY;
-                if (tri_intersects(s.tr, va, vb, vc, hh) && !(hh > s.tmax)) {
-                    s.occ = true;
-                    return true;
-                }
-            } else if (tri_intersects(s.tr, va, vb, vc, s.h)) {
-                s.hit = __float_as_int(va.w);
-                s.thit = s.h;
-            }
-        }
-    }
-    if (s.sp > 0) {
-        --s.sp;
-        s.cur = stack[s.sp * kTraceBlock];
-        return false;
-    }
-    return true;
+    trav_begin(s, rays.ox[i], rays.oy[i], rays.oz[i], rays.dx[i], rays.dy[i], rays.dz[i],
+               rays.tmax ? rays.tmax[i] : INFINITY, any);
 }
 
 // STATS = true counts node visits and triangle tests (for pricing the
 // algorithmic bytes of a launch); it is never the timed kernel.
-template <bool ANY, bool STATS>
+template <bool WIDE, bool ANY, bool STATS>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(const float4 *__restrict__ nodes,
                                                        const float4 *__restrict__ tris,
                                                        RaysDev rays, uint32_t n,
@@ -122,36 +43,23 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(const float4 *__restrict_
 {
     extern __shared__ int32_t s_stack[];
     int32_t *stack = s_stack + threadIdx.x;
-    const uint32_t lane = __lane_id();
     TravState s;
     bool has = false;
     bool exhausted = false;  // wavefront-uniform: the ray counter passed n
     uint32_t ray = 0;
     uint32_t n_node = 0, n_tri = 0;
     for (;;) {
-        // refill idle lanes (wavefront-uniform decision)
-        const uint64_t idle = __ballot(!has);
-        const uint32_t nidle = (uint32_t)__popcll(idle);
-        if (!exhausted && (nidle >= kRefillLanes || nidle == 64u)) {
-            const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)idle) - 1);
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(ray_counter, nidle);
-            base = __shfl(base, (int)leader);
-            if (base + nidle >= n) exhausted = true;
-            if (!has) {
-                const uint32_t r = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-                if (r < n) {
-                    ray = r;
-                    trav_init(s, rays, r, ANY);
-                    has = true;
-                }
-            }
+        const uint32_t r = wave_refill(ray_counter, n, has, exhausted);
+        if (r != 0xffffffffu) {
+            ray = r;
+            trav_init(s, rays, r, ANY);
+            has = true;
         }
         if (__ballot(has) == 0) {
             if (exhausted) break;
             continue;
         }
-        if (has && trav_step<ANY, STATS>(s, nodes, tris, stack, cap, n_node, n_tri)) {
+        if (has && trav_step_any<WIDE, ANY, STATS>(s, nodes, tris, stack, cap, n_node, n_tri)) {
             if (ANY) {
                 occluded[ray] = s.occ ? 1 : 0;
             } else {
@@ -180,7 +88,7 @@ hipError_t launch_trace(const DevBvh &bvh, const RaysDev &rays, uint32_t n, int3
         if (e == hipSuccess) e = hipDeviceGetAttribute(&g_cus, hipDeviceAttributeMultiprocessorCount, dev);
         if (e != hipSuccess || g_cus <= 0) g_cus = 256;
     }
-    const int cap = (int)bvh.max_depth + 1;
+    const int cap = (int)bvh.stack_cap;
     const size_t lds = (size_t)cap * kTraceBlock * sizeof(int32_t);
     // persistent launch: as many workgroups as fit (LDS-limited, at most 16
     // per CU), never more than the rays need
@@ -192,23 +100,24 @@ hipError_t launch_trace(const DevBvh &bvh, const RaysDev &rays, uint32_t n, int3
     if (blocks > need) blocks = need;
     const dim3 grid((unsigned)blocks);
     const float4 *nd = (const float4 *)bvh.nodes, *tr = (const float4 *)bvh.tris;
-    if (counters) {
-        if (any)
-            hipLaunchKernelGGL((k_trace<true, true>), grid, dim3(kTraceBlock), lds, stream, nd, tr, rays, n, prim, t, occluded, cap, ray_counter, counters);
-        else
-            hipLaunchKernelGGL((k_trace<false, true>), grid, dim3(kTraceBlock), lds, stream, nd, tr, rays, n, prim, t, occluded, cap, ray_counter, counters);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, grid, dim3(kTraceBlock), lds, stream, nd, tr, rays, n, prim, t, occluded, cap,
+                           ray_counter, counters);
+    };
+    const bool wide = bvh.wide != 0, stats = counters != nullptr;
+    if (wide) {
+        if (stats) any ? go(k_trace<true, true, true>) : go(k_trace<true, false, true>);
+        else any ? go(k_trace<true, true, false>) : go(k_trace<true, false, false>);
     } else {
-        if (any)
-            hipLaunchKernelGGL((k_trace<true, false>), grid, dim3(kTraceBlock), lds, stream, nd, tr, rays, n, prim, t, occluded, cap, ray_counter, counters);
-        else
-            hipLaunchKernelGGL((k_trace<false, false>), grid, dim3(kTraceBlock), lds, stream, nd, tr, rays, n, prim, t, occluded, cap, ray_counter, counters);
+        if (stats) any ? go(k_trace<false, true, true>) : go(k_trace<false, false, true>);
+        else any ? go(k_trace<false, true, false>) : go(k_trace<false, false, false>);
     }
     return hipGetLastError();
 }
 
-size_t trace_lds_bytes(uint32_t max_depth)
+size_t trace_lds_bytes(uint32_t stack_cap)
 {
-    return (size_t)(max_depth + 1) * kTraceBlock * sizeof(int32_t);
+    return (size_t)stack_cap * kTraceBlock * sizeof(int32_t);
 }
 
 }  // namespace rt

@@ -92,7 +92,7 @@ def run_c1(args):
     tmax_np = np.full(n, np.inf, np.float32)
     ctx = rt.Context(0)
     t0 = time.time()
-    bvh = rt.Bvh(ctx, xyz, idx, max_leaf=4, bins=32)
+    bvh = rt.Bvh(ctx, xyz, idx, max_leaf=4, bins=32, width=args.width)
     inf = bvh.info()
     log(f"BVH: {inf.num_nodes} nodes, {inf.num_leaves} leaves, depth {inf.max_depth}, "
         f"build {inf.build_ms:.0f} ms (wall {time.time() - t0:.1f} s)")
@@ -140,7 +140,7 @@ def run_c1(args):
 
     achieved = alg_bytes / (k_avg / 1e3) / 1e9
     traffic = None
-    pmc_path = os.path.join(ROOT, "profiles", "c1_trace_closest_hbm_traffic.json")
+    pmc_path = os.path.join(ROOT, "profiles", f"c1_trace_closest_bvh{args.width}_hbm_traffic.json")
     if os.path.exists(pmc_path):
         try:
             traffic = json.load(open(pmc_path)).get("hbm_bytes_per_launch")
@@ -160,10 +160,12 @@ def run_c1(args):
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": ("BVH closest hit: 1M-triangle soup in the unit cube (seed 7), binned-SAH BVH2 leaves<=4, "
+            "workload": ("BVH closest hit: 1M-triangle soup in the unit cube (seed 7), binned-SAH BVH leaves<=4, "
                          "16M rays = 8M coherent 4096x2048 pinhole + 8M incoherent"),
-            "rays": n, "triangles": int(idx.shape[0]), "bvh_nodes": int(inf.num_nodes),
-            "bvh_depth": int(inf.max_depth),
+            "rays": n, "triangles": int(idx.shape[0]),
+            "bvh_layout": ("four-wide, 8-bit quantized child boxes" if args.width == 4 else "two-wide, FP32 boxes"),
+            "bvh_nodes": int(inf.num_nodes4 if args.width == 4 else inf.num_nodes),
+            "bvh_depth": int(inf.depth4 if args.width == 4 else inf.max_depth),
         },
         "roofline": {
             "bound": "hbm",
@@ -378,7 +380,8 @@ def run_c3(args):
     depth 12, scene i seeded 1000 + i) in one rt_batch launch sequence with
     shared queues, frames replayed from a captured hipGraph, wave size and
     grid from the autotuner.  One step = one frame of every scene at 32 spp.
-    Also timed: the same frame as 32 separate rt_render calls.  Per-scene
+    Also timed: the same frame as 32 separate rt_render calls with the same
+    wave size and grid.  Per-scene
     RMSE is taken against a plain FP64 render of the same scene at
     --c3-ref-spp with disjoint samples (itself noisy: its own RMSE to the
     converged image is about sqrt(32 / ref_spp) of the frame's)."""
@@ -421,7 +424,7 @@ def run_c3(args):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     for s, f, sd in zip(scs, seq, seeds):
-        s.render(f, spp=32, spp_offset=0, max_depth=12, seed=sd)
+        s.render(f, spp=32, spp_offset=0, max_depth=12, seed=sd, path_capacity=best.path_capacity, grid=best.grid)
     torch.cuda.synchronize()
     seq_s = time.perf_counter() - t1
     rmse = []
@@ -506,6 +509,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c2", "c3", "c4", "copy"])
     ap.add_argument("--c3-ref-spp", type=int, default=128)
+    ap.add_argument("--width", type=int, default=4, choices=[2, 4],
+                    help="c1: tree layout (4 = four-wide quantized nodes, 2 = two-wide FP32 nodes)")
     ap.add_argument("--c4-width", type=int, default=480)
     ap.add_argument("--c4-height", type=int, default=270)
     ap.add_argument("--c4-ref-spp", type=int, default=16384)

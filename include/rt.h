@@ -76,6 +76,8 @@ typedef struct {
     uint32_t max_leaf;   /* 1..8, default 4 */
     uint32_t bins;       /* SAH bins per axis, 4..64, default 32 */
     float    trav_cost;  /* SAH traversal cost relative to a triangle test, default 1.2 */
+    uint32_t width;      /* traversal layout: 4 = four-wide nodes with 8-bit quantized child
+                            boxes (default, also for 0), 2 = two-wide nodes with FP32 boxes */
 } rt_bvh_opts;
 
 typedef struct {
@@ -87,6 +89,10 @@ typedef struct {
     uint64_t tri_bytes;   /* 48 B per triangle */
     double   build_ms;
     double   sah_cost;
+    uint32_t width;       /* layout the traces use */
+    uint32_t num_nodes4;  /* four-wide nodes (64 B each) */
+    uint32_t depth4;      /* levels of the four-wide tree */
+    uint32_t stack_cap;   /* traversal stack entries per lane for the layout in use */
 } rt_bvh_info;
 
 typedef struct rt_bvh rt_bvh;
@@ -105,6 +111,11 @@ rt_status rt_bvh_set_transparent(rt_bvh *bvh, const uint8_t *flags);
  * 48-byte triangles (v0 xyz + id, v1 xyz + flags, v2 xyz + 0), as the
  * oracle's counting traversal reads them. */
 rt_status rt_bvh_export(const rt_bvh *bvh, float *nodes, float *tris);
+/* Copy the four-wide tree (num_nodes4 * 16 floats): per node the origin,
+ * a word of scale exponents and slot-valid bits, four child links and the
+ * 8-bit quantized child bounds (lo_x, hi_x, lo_y, hi_y, lo_z, hi_z words,
+ * byte i = child i), followed by two zero words. */
+rt_status rt_bvh_export4(const rt_bvh *bvh, float *nodes4);
 
 /* ------------------------------------------------------------------ */
 /* Ray queries                                                          */
@@ -188,6 +199,7 @@ typedef struct {
     uint32_t rr_depth;    /* Russian roulette when depth > rr_depth (pathTracing.cu:91), 3 */
     uint32_t seed;        /* PCG32 seed */
     uint32_t path_capacity; /* paths in flight per wave, 0 = default (2^21) */
+    uint32_t grid;          /* workgroups of the queue kernels, 0 = default (2048) */
 } rt_render_opts;
 
 typedef struct rt_scene rt_scene;
@@ -271,9 +283,9 @@ rt_status rt_batch_render(rt_batch *batch, const rt_batch_opts *opts, const uint
                           uint32_t *const *nvalid, uint32_t flags, void *stream);
 
 /* Autotuner: renders one sample per pixel of every scene (into scratch
- * films) for each candidate wave size {2^19 .. 2^22, capped at the batch's
- * paths per sample} x grid {1024, 2048, 4096}, twice each, and keeps the
- * fastest pair as the batch's default.  entries (may be NULL) receives up to
+ * films) for each candidate wave size {2^19 .. 2^23, capped at the batch's
+ * paths per sample} x grid {1024, 2048, 4096, 8192}, twice each, and keeps
+ * the fastest pair as the batch's default.  entries (may be NULL) receives up to
  * max_entries (path_capacity, grid, ms) triples in the order tried. */
 typedef struct {
     uint32_t path_capacity, grid;

@@ -412,7 +412,7 @@ private:
 
 // Render settings of the reference's compile-time macros (types.h:41-42,73-74).
 struct RenderSettings {
-    uint32_t spp = 16, max_depth = 8, rr_depth = 3, seed = 1, spp_offset = 0, path_capacity = 0;
+    uint32_t spp = 16, max_depth = 8, rr_depth = 3, seed = 1, spp_offset = 0, path_capacity = 0, grid = 0;
 };
 
 // PathTracerIntegrator(camera).render(scene): renders every pixel at
@@ -437,6 +437,7 @@ public:
         o.rr_depth = s.rr_depth;
         o.seed = s.seed;
         o.path_capacity = s.path_capacity;
+        o.grid = s.grid;
         check(rt_render(scene.get(), &o, (double *)accum.get(), (uint32_t *)nvalid.get(), 0, nullptr));
         ctx.synchronize();
         std::vector<double> sums(npix * 3);
@@ -459,7 +460,7 @@ public:
     // ctx may be null: host-only tree (info and export only)
     BVH(Context *ctx, const std::vector<float> &xyz, const std::vector<uint32_t> &idx, uint32_t max_leaf = 4)
     {
-        rt_bvh_opts o{max_leaf, 32, 1.2f};
+        rt_bvh_opts o{max_leaf, 32, 1.2f, 4};
         check(rt_bvh_build(ctx ? ctx->get() : nullptr, xyz.data(), (uint32_t)(xyz.size() / 3), idx.data(),
                            (uint32_t)(idx.size() / 3), &o, &h_));
     }

@@ -61,6 +61,8 @@ _SIGS = {
     "orc_scene_free": (None, [_P]),
     "orc_scene_camera": (None, [_P, _P, _P]),
     "orc_path": (C.c_int, [_P, _U32, _U32, _U32, _U32, _U32, _U32, C.c_uint64, _P]),
+    "orc_bvh4_trace": (None, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _U32, _P, _P, C.POINTER(C.c_uint64),
+                              C.POINTER(C.c_uint64), C.POINTER(_U32), C.c_int]),
     "orc_render": (None, [_P, _U32, _U32, _U32, _U32, _U32, C.c_uint64, _U32, _U32, _U32, _U32, _P, _P, C.c_int]),
     "orc_render_const_cache": (None, [_P, _U32, _U32, _U32, _U32, _U32, C.c_uint64, _U32, _U32, _U32, _U32, _P, _P,
                                       C.c_int, _U32, _P]),
@@ -313,6 +315,22 @@ def bvh2_trace(nodes, tris, rays, tmax=None, nthreads=8):
     lib().orc_bvh2_trace(_p(nodes), _p(tris), *[_p(a) for a in rays], _p(tmax), n, _p(prim), _p(t),
                          C.byref(nn), C.byref(nt), nthreads)
     return prim, t, nn.value, nt.value
+
+
+def bvh4_trace(nodes4, tris, rays, tmax=None, nthreads=8):
+    """Counting traversal of the library's exported four-wide layout:
+    (prim, t, node visits, triangle tests, deepest stack)."""
+    rays = [f32(a) for a in rays]
+    n = rays[0].shape[0]
+    prim = np.empty(n, np.int32)
+    t = np.empty(n, np.float32)
+    nn = C.c_uint64()
+    nt = C.c_uint64()
+    msp = C.c_uint32()
+    nodes4, tris = f32(nodes4), f32(tris)
+    lib().orc_bvh4_trace(_p(nodes4), _p(tris), *[_p(a) for a in rays], _p(tmax), n, _p(prim), _p(t),
+                         C.byref(nn), C.byref(nt), C.byref(msp), nthreads)
+    return prim, t, nn.value, nt.value, msp.value
 
 
 class Sphere:

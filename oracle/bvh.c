@@ -455,3 +455,141 @@ void orc_bvh2_trace(const float *nodes, const float *tris, const float *ox, cons
     if (n_tri) *n_tri = b;
     free(cnt);
 }
+
+/* ------------------------------------------------------------------ */
+/* Counting traversal of the library's four-wide quantized layout        */
+/* ------------------------------------------------------------------ */
+/* Node (16 words): origin.xyz (float), meta = ex | ey<<8 | ez<<16 |
+ * valid<<24, child[4] (int32), then lo_x hi_x lo_y hi_y lo_z hi_z words
+ * (byte i = child i) and two zero words.  Child i's box along x is
+ * [origin.x + lo_x[i]*2^(ex-127), origin.x + hi_x[i]*2^(ex-127)].  The
+ * operations below are the kernel's (trav_step4), one for one: the slab
+ * distances fma(q, s*inv, fma(origin, inv, -o*inv)), entry distances sorted
+ * as bit patterns with misses as 0xffffffff by the same 5-comparator
+ * network, the nearest child followed and the others pushed farthest
+ * first.  max_sp reports the deepest stack the rays needed, which the
+ * library's stack bound must cover. */
+static inline uint32_t as_u32(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float u32_as_f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+static inline void cswap_key(uint32_t *ka, int32_t *ra, uint32_t *kb, int32_t *rb)
+{
+    int sw = *kb < *ka;
+    uint32_t k0 = sw ? *kb : *ka, k1 = sw ? *ka : *kb;
+    int32_t r0 = sw ? *rb : *ra, r1 = sw ? *ra : *rb;
+    *ka = k0; *kb = k1; *ra = r0; *rb = r1;
+}
+
+typedef struct {
+    const float *nodes, *tris;
+    const float *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
+    int32_t *prim; float *t;
+    uint64_t *cnt; /* 3 per thread: nodes, tris, max sp */
+} b4_ctx;
+
+static void b4_range(void *p, uint32_t b0, uint32_t e0, int tid)
+{
+    b4_ctx *c = p;
+    uint64_t nn = 0, ntr = 0, msp = 0;
+    for (uint32_t i = b0; i < e0; ++i) {
+        orc_ray r;
+        float o[3] = {c->ox[i], c->oy[i], c->oz[i]}, d[3] = {c->dx[i], c->dy[i], c->dz[i]};
+        orc_ray_make(&r, o, d, c->tmax ? c->tmax[i] : INFINITY);
+        float inv[3], oinv[3];
+        kernel_inv_dir(d, inv, oinv, o);
+        float h = INFINITY, thit = INFINITY;
+        int32_t hit = -1;
+        int32_t stack[1024];
+        int sp = 0, done = 0;
+        int32_t cur = 0;
+        while (!done) {
+            if (cur >= 0) {
+                const float *n = c->nodes + 16 * (size_t)cur;
+                ++nn;
+                float tlim = fminf(h, r.tmax) * RT_HIT_SCALE;
+                uint32_t meta = as_u32(n[3]);
+                float A[3], B[3];
+                for (int a = 0; a < 3; ++a) {
+                    float sc = u32_as_f(((meta >> (8 * a)) & 0xffu) << 23);
+                    A[a] = sc * inv[a];
+                    B[a] = fmaf(n[a], inv[a], -oinv[a]);
+                }
+                uint32_t q[6];
+                for (int k = 0; k < 6; ++k) q[k] = as_u32(n[8 + k]);
+                uint32_t key[4];
+                int32_t ref[4];
+                for (int k = 0; k < 4; ++k) ref[k] = as_i32(n[4 + k]);
+                for (int k = 0; k < 4; ++k) {
+                    int sh = 8 * k;
+                    float tx0 = fmaf((float)((q[0] >> sh) & 0xffu), A[0], B[0]);
+                    float tx1 = fmaf((float)((q[1] >> sh) & 0xffu), A[0], B[0]);
+                    float ty0 = fmaf((float)((q[2] >> sh) & 0xffu), A[1], B[1]);
+                    float ty1 = fmaf((float)((q[3] >> sh) & 0xffu), A[1], B[1]);
+                    float tz0 = fmaf((float)((q[4] >> sh) & 0xffu), A[2], B[2]);
+                    float tz1 = fmaf((float)((q[5] >> sh) & 0xffu), A[2], B[2]);
+                    float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+                    float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+                    tf = fminf(tf * RT_TFAR_SCALE, tlim);
+                    int hit4 = ((meta >> (24 + k)) & 1u) && tn <= tf;
+                    key[k] = hit4 ? as_u32(tn) : 0xffffffffu;
+                }
+                cswap_key(&key[0], &ref[0], &key[1], &ref[1]);
+                cswap_key(&key[2], &ref[2], &key[3], &ref[3]);
+                cswap_key(&key[0], &ref[0], &key[2], &ref[2]);
+                cswap_key(&key[1], &ref[1], &key[3], &ref[3]);
+                cswap_key(&key[1], &ref[1], &key[2], &ref[2]);
+                if (key[0] != 0xffffffffu) {
+                    for (int k = 3; k >= 1; --k)
+                        if (key[k] != 0xffffffffu) {
+                            if (sp >= 1024) abort(); /* a 1024-deep stack means a broken tree */
+                            stack[sp++] = ref[k];
+                        }
+                    if ((uint64_t)sp > msp) msp = (uint64_t)sp;
+                    cur = ref[0];
+                    continue;
+                }
+                if (sp) cur = stack[--sp];
+                else done = 1;
+            } else {
+                uint32_t enc = ~(uint32_t)cur;
+                uint32_t first = enc >> 3, cnt = (enc & 7) + 1;
+                for (uint32_t k = 0; k < cnt; ++k) {
+                    const float *tr = c->tris + 12 * (size_t)(first + k);
+                    ++ntr;
+                    if (orc_tri_intersects(&r, tr, tr + 4, tr + 8, &h, NULL)) {
+                        hit = as_i32(tr[3]);
+                        thit = h;
+                    }
+                }
+                if (sp) cur = stack[--sp];
+                else done = 1;
+            }
+        }
+        c->prim[i] = hit;
+        if (c->t) c->t[i] = thit;
+    }
+    c->cnt[3 * tid] = nn;
+    c->cnt[3 * tid + 1] = ntr;
+    c->cnt[3 * tid + 2] = msp;
+}
+
+void orc_bvh4_trace(const float *nodes4, const float *tris, const float *ox, const float *oy,
+                    const float *oz, const float *dx, const float *dy, const float *dz,
+                    const float *tmax, uint32_t n, int32_t *prim, float *t, uint64_t *n_node,
+                    uint64_t *n_tri, uint32_t *max_sp, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+    uint64_t *cnt = calloc(3 * (size_t)nthreads, sizeof(uint64_t));
+    b4_ctx c = {nodes4, tris, ox, oy, oz, dx, dy, dz, tmax, prim, t, cnt};
+    parallel_for(n, nthreads, b4_range, &c);
+    uint64_t a = 0, b = 0, m = 0;
+    for (int i = 0; i < nthreads; ++i) {
+        a += cnt[3 * i];
+        b += cnt[3 * i + 1];
+        if (cnt[3 * i + 2] > m) m = cnt[3 * i + 2];
+    }
+    if (n_node) *n_node = a;
+    if (n_tri) *n_tri = b;
+    if (max_sp) *max_sp = (uint32_t)m;
+    free(cnt);
+}

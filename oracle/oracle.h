@@ -110,6 +110,15 @@ void orc_bvh2_trace(const float *nodes, const float *tris, const float *ox,
                     uint32_t n, int32_t *prim, float *t, uint64_t *n_node,
                     uint64_t *n_tri, int nthreads);
 
+/* Traversal of the library's four-wide quantized layout (include/rt.h:
+ * rt_bvh_export4), operation for operation as the kernel walks it: counts
+ * node visits and triangle tests and reports the deepest stack used. */
+void orc_bvh4_trace(const float *nodes4, const float *tris, const float *ox,
+                    const float *oy, const float *oz, const float *dx,
+                    const float *dy, const float *dz, const float *tmax,
+                    uint32_t n, int32_t *prim, float *t, uint64_t *n_node,
+                    uint64_t *n_tri, uint32_t *max_sp, int nthreads);
+
 /* ------------------------------------------------------------------ */
 /* Sphere (shapes/Sphere.cu)                                            */
 /* ------------------------------------------------------------------ */

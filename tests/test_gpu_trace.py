@@ -31,15 +31,16 @@ def cat(a, b):
     return [np.concatenate([x, y]) for x, y in zip(a, b)]
 
 
+@pytest.mark.parametrize("width", [4, 2])
 @pytest.mark.parametrize("nt", [0, 1, 3, 4, 5, 100, 4096, 32768])
-def test_closest_hit_matches_brute_force(rt, orc, gpu_ctx, nt):
-    print(f"[closest nt={nt}]", flush=True)
+def test_closest_hit_matches_brute_force(rt, orc, gpu_ctx, nt, width):
+    print(f"[closest nt={nt} width={width}]", flush=True)
     if nt:
         xyz, idx = rt.gen_soup(nt, seed=100 + nt)
     else:
         xyz, idx = np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32)
     rays = cat(rt.gen_rays(64, 48, n_incoherent=4096, seed=nt), edge_rays(512, nt))
-    bvh = rt.Bvh(gpu_ctx, xyz, idx)
+    bvh = rt.Bvh(gpu_ctx, xyz, idx, width=width)
     prim, t = bvh.trace_closest(*to_dev(rays))
     torch.cuda.synchronize()
     prim, t = prim.cpu().numpy(), t.cpu().numpy()
@@ -96,14 +97,19 @@ def test_host_pointer_path_equals_device_path(rt, gpu_ctx):
     assert np.array_equal(ht, dt.cpu().numpy())
 
 
-def test_kernel_counts_match_oracle_traversal(rt, orc, gpu_ctx):
+@pytest.mark.parametrize("width", [2, 4])
+def test_kernel_counts_match_oracle_traversal(rt, orc, gpu_ctx, width):
     """The counting kernel variant visits exactly the nodes and triangles the
-    oracle's restatement of the kernel traversal visits on the same tree."""
+    oracle's restatement of the kernel traversal visits on the same tree
+    (two-wide FP32 nodes, or four-wide quantized nodes)."""
     xyz, idx = rt.gen_soup(50000, seed=12)
     rays = rt.gen_rays(64, 64, n_incoherent=8192, seed=13)
-    bvh = rt.Bvh(gpu_ctx, xyz, idx)
+    bvh = rt.Bvh(gpu_ctx, xyz, idx, width=width)
     nodes, tris = bvh.export()
-    p2, t2, nn, ntr = orc.bvh2_trace(nodes, tris, rays)
+    if width == 2:
+        p2, t2, nn, ntr = orc.bvh2_trace(nodes, tris, rays)
+    else:
+        p2, t2, nn, ntr, _ = orc.bvh4_trace(bvh.export4(), tris, rays)
     gn, gt = bvh.trace_stats(*to_dev(rays))
     assert (gn, gt) == (nn, ntr)
     prim, t = bvh.trace_closest(*to_dev(rays))
@@ -129,7 +135,7 @@ def test_full_size_c1_against_reference_bvh(rt, orc, gpu_ctx):
     torch.cuda.synchronize()
     prim, t = prim.cpu().numpy(), t.cpu().numpy()
     nodes, tris = bvh.export()
-    p2, t2, _, _ = orc.bvh2_trace(nodes, tris, rays, nthreads=16)
+    p2, t2, _, _, _ = orc.bvh4_trace(bvh.export4(), tris, rays, nthreads=16)
     assert np.array_equal(prim, p2) and np.array_equal(t, t2)
     rp, rtt = orc.RefBvh(xyz, idx).trace(rays, nthreads=16)
     diff = np.nonzero(prim != rp)[0]

@@ -42,21 +42,20 @@ using rt::set_error;
 
 namespace {
 
-// What a captured frame depends on besides the device-side sample offset
-// and scene table.
+// What a captured iteration block depends on (the sample offset, the
+// chunk's first path and count and the scene table are read from device
+// memory at replay; the wave buffers are fixed while the key holds).
 struct GraphKey {
-    uint32_t spp = 0, max_depth = 0, rr_depth = 0, capacity = 0, grid = 0;
-    std::vector<rt::FilmDev> films;
+    uint32_t max_depth = 0, rr_depth = 0, capacity = 0, grid = 0;
+    void *arena = nullptr, *res = nullptr;
     bool operator==(const GraphKey &o) const
     {
-        if (spp != o.spp || max_depth != o.max_depth || rr_depth != o.rr_depth || capacity != o.capacity ||
-            grid != o.grid || films.size() != o.films.size())
-            return false;
-        for (size_t i = 0; i < films.size(); ++i)
-            if (films[i].accum != o.films[i].accum || films[i].nvalid != o.films[i].nvalid) return false;
-        return true;
+        return max_depth == o.max_depth && rr_depth == o.rr_depth && capacity == o.capacity && grid == o.grid &&
+               arena == o.arena && res == o.res;
     }
 };
+
+constexpr uint32_t kBlockIters = 4;  // iterations per captured block (even)
 
 }  // namespace
 
@@ -70,6 +69,7 @@ struct rt_batch {
     rt::SceneDev *d_table = nullptr;
     uint64_t *d_bases = nullptr;             // per call: n + 1 path-index bases
     uint32_t *d_spp_offset = nullptr;
+    uint64_t *d_chunk = nullptr;             // {first path, count} of the running chunk
     std::vector<uint64_t> bases;             // host copy for the current spp
     uint32_t bases_spp = 0;
     rt::WaveBuffers wb;
@@ -124,9 +124,7 @@ rt_status upload_bases(rt_batch *b, uint32_t spp, hipStream_t s)
     return RT_OK;
 }
 
-// Launches every wave of one frame (kernel launches only: capturable).
-hipError_t launch_frame(rt_batch *b, const rt_batch_opts &o, uint32_t P, uint32_t grid,
-                        const std::vector<rt::FilmDev> &films, hipStream_t s)
+rt::BatchLaunch batch_launch(rt_batch *b, const std::vector<rt::FilmDev> &films)
 {
     rt::BatchLaunch bl{};
     bl.dev = rt::BatchDev{b->d_table, b->d_bases, (uint32_t)b->scenes.size()};
@@ -134,23 +132,83 @@ hipError_t launch_frame(rt_batch *b, const rt_batch_opts &o, uint32_t P, uint32_
     bl.npix = b->npix.data();
     bl.films = films.data();
     bl.stack_cap = b->stack_cap;
+    return bl;
+}
+
+rt::WaveParams batch_params(rt_batch *b, const rt_batch_opts &o, uint32_t grid)
+{
+    rt::WaveParams wp{};
+    wp.spp_offset = o.spp_offset;
+    wp.max_depth = o.max_depth;
+    wp.rr_depth = o.rr_depth;
+    wp.cache_depth = rt::kNoCache;
+    wp.grid = grid;
+    wp.spp_offset_dev = b->d_spp_offset;
+    wp.chunk_dev = b->d_chunk;
+    return wp;
+}
+
+// One frame: every chunk of the batch's path index space through the
+// streaming loop.  With `graph`, the iteration block is replayed from the
+// captured graph (captured on first use for the current key).
+hipError_t run_frame(rt_batch *b, const rt_batch_opts &o, uint32_t P, uint32_t grid,
+                     const std::vector<rt::FilmDev> &films, bool graph, hipStream_t s)
+{
+    const rt::BatchLaunch bl = batch_launch(b, films);
+    rt::ChunkLaunch cl{};
+    cl.scene = &b->host_table[0];
+    cl.paths = b->wb.paths;
+    cl.shadow = b->wb.shadow;
+    cl.queues = b->wb.queues;
+    cl.capacity = P;
+    cl.res = b->wb.res;
+    cl.batch = &bl;
+    rt::WaveParams wp = batch_params(b, o, grid);
+    if (graph) {
+        GraphKey k;
+        k.max_depth = o.max_depth; k.rr_depth = o.rr_depth; k.capacity = P; k.grid = grid;
+        k.arena = b->wb.arena; k.res = b->wb.res;
+        if (!b->exec || !(k == b->key)) {
+            drop_graph(b);
+            hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) return e;
+            e = rt::chunk_iterate(cl, wp, kBlockIters, s);
+            hipGraph_t gr = nullptr;
+            hipError_t e2 = hipStreamEndCapture(s, &gr);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return e != hipSuccess ? e : e2;
+            }
+            b->graph = gr;
+            e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) return e;
+            b->key = k;
+        }
+    }
     const uint64_t total = b->bases.back();
-    for (uint64_t first = 0; first < total; first += P) {
-        rt::WaveParams wp{};
+    const uint64_t R = std::min<uint64_t>(total, b->wb.res_cap);
+    uint32_t *hc = b->wb.host_counters;
+    for (uint64_t first = 0; first < total; first += R) {
         wp.first_path = first;
-        wp.count = (uint32_t)std::min<uint64_t>(P, total - first);
-        wp.npix = 0;
-        wp.spp_offset = o.spp_offset;
-        wp.seed = 0;
-        wp.max_depth = o.max_depth;
-        wp.rr_depth = o.rr_depth;
-        wp.stream_base = 0;
-        wp.cache_depth = rt::kNoCache;
-        wp.train = 0;
-        wp.grid = grid;
-        wp.spp_offset_dev = b->d_spp_offset;
-        hipError_t e = rt::render_wave(b->host_table[0], b->wb.paths, b->wb.shadow, b->wb.queues, rt::FilmDev{},
-                                       wp, P, s, nullptr, nullptr, &bl);
+        wp.count = (uint32_t)std::min<uint64_t>(R, total - first);
+        hipError_t e = rt::set_u64(b->d_chunk, first, s);
+        if (e == hipSuccess) e = rt::set_u64(b->d_chunk + 1, wp.count, s);
+        if (e == hipSuccess) e = rt::chunk_begin(cl, wp, s);
+        if (e != hipSuccess) return e;
+        const uint64_t rounds = ((uint64_t)wp.count + P - 1) / P + 1;
+        const uint64_t max_iters = rounds * o.max_depth + kBlockIters;
+        bool done = false;
+        for (uint64_t it = 0; it < max_iters && !done; it += kBlockIters) {
+            e = graph ? hipGraphLaunch(b->exec, s) : rt::chunk_iterate(cl, wp, kBlockIters, s);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(hc, b->wb.queues.counters, rt::kNumCounters * sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return e;
+            done = hc[rt::kCNext] >= wp.count && hc[rt::kCActive0] == 0;
+        }
+        if (!done) return hipErrorUnknown;  // cannot happen: the bound above holds
+        e = rt::chunk_film(cl, films[0], wp, s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -215,8 +273,10 @@ rt_status rt_batch_create(rt_ctx *ctx, rt_scene *const *scenes, uint32_t n, rt_b
     }
     hipError_t e = hipMalloc(&b->d_bases, (n + 1) * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMalloc(&b->d_spp_offset, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&b->d_chunk, 2 * sizeof(uint64_t));
     if (e != hipSuccess) {
         if (b->d_bases) (void)hipFree(b->d_bases);
+        if (b->d_spp_offset) (void)hipFree(b->d_spp_offset);
         delete b;
         set_error(std::string("rt_batch_create: ") + hipGetErrorString(e));
         return RT_ERR_OOM;
@@ -239,6 +299,7 @@ rt_status rt_batch_destroy(rt_batch *b)
         if (b->d_table) (void)hipFree(b->d_table);
         if (b->d_bases) (void)hipFree(b->d_bases);
         if (b->d_spp_offset) (void)hipFree(b->d_spp_offset);
+        if (b->d_chunk) (void)hipFree(b->d_chunk);
         delete b;
     }
     rt::ctx_release(ctx);
@@ -265,6 +326,7 @@ rt_status rt_batch_render(rt_batch *b, const rt_batch_opts *opts, const uint32_t
     const uint32_t P = wave_size(b, *opts, total);
     const uint32_t grid = opts->grid ? opts->grid : (b->tuned_grid ? b->tuned_grid : 2048);
     st = rt::ensure_wave_buffers(b->wb, P, b->max_lights, "rt_batch_render");
+    if (st == RT_OK) st = rt::ensure_result_buffer(b->wb, std::min<uint64_t>(total, rt::kChunkPaths), "rt_batch_render");
     if (st != RT_OK) return st;
     st = upload_table(b, seeds, s);
     if (st != RT_OK) return st;
@@ -273,29 +335,12 @@ rt_status rt_batch_render(rt_batch *b, const rt_batch_opts *opts, const uint32_t
     auto t0 = std::chrono::steady_clock::now();
     RT_HIP(hipMemsetAsync(b->wb.queues.stats, 0, 16, s));
     RT_HIP(rt::set_u32(b->d_spp_offset, opts->spp_offset, s));
-    if (flags & RT_FLAG_GRAPH) {
-        GraphKey k;
-        k.spp = opts->spp; k.max_depth = opts->max_depth; k.rr_depth = opts->rr_depth;
-        k.capacity = P; k.grid = grid; k.films = films;
-        if (!b->exec || !(k == b->key)) {
-            drop_graph(b);
-            RT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            hipError_t e = launch_frame(b, *opts, P, grid, films, s);
-            hipGraph_t gr = nullptr;
-            hipError_t e2 = hipStreamEndCapture(s, &gr);
-            if (e != hipSuccess || e2 != hipSuccess) {
-                if (gr) (void)hipGraphDestroy(gr);
-                set_error(std::string("rt_batch_render: graph capture failed: ") +
-                          hipGetErrorString(e != hipSuccess ? e : e2));
-                return RT_ERR_HIP;
-            }
-            b->graph = gr;
-            RT_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-            b->key = k;
+    {
+        hipError_t e = run_frame(b, *opts, P, grid, films, (flags & RT_FLAG_GRAPH) != 0, s);
+        if (e != hipSuccess) {
+            set_error(std::string("rt_batch_render: ") + hipGetErrorString(e));
+            return RT_ERR_HIP;
         }
-        RT_HIP(hipGraphLaunch(b->exec, s));
-    } else {
-        RT_HIP(launch_frame(b, *opts, P, grid, films, s));
     }
     unsigned long long counts[2];
     RT_HIP(hipMemcpyAsync(counts, b->wb.queues.stats, sizeof counts, hipMemcpyDeviceToHost, s));
@@ -350,12 +395,14 @@ rt_status rt_batch_autotune(rt_batch *b, const rt_batch_opts *opts, rt_tune_entr
     for (uint32_t P : caps) {
         if (result != RT_OK) break;
         result = rt::ensure_wave_buffers(b->wb, P, b->max_lights, "rt_batch_autotune");
+        if (result == RT_OK)
+            result = rt::ensure_result_buffer(b->wb, std::min<uint64_t>(total, rt::kChunkPaths), "rt_batch_autotune");
         for (uint32_t gsz : grids) {
             if (result != RT_OK) break;
             float ms_best = 1e30f;
             for (int rep = 0; rep < 2 && result == RT_OK; ++rep) {
                 if (rt::set_u32(b->d_spp_offset, o.spp_offset + (uint32_t)rep, s) != hipSuccess ||
-                    hipEventRecord(e0, s) != hipSuccess || launch_frame(b, o, P, gsz, films, s) != hipSuccess ||
+                    hipEventRecord(e0, s) != hipSuccess || run_frame(b, o, P, gsz, films, false, s) != hipSuccess ||
                     hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
                     set_error("rt_batch_autotune: trial launch failed");
                     result = RT_ERR_HIP;

@@ -79,76 +79,100 @@ __device__ __forceinline__ Pcg32 load_rng(const PathsDev &p, uint32_t slot, uint
 }
 
 // ---------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_reset(uint32_t *counters, uint32_t cur, uint32_t init_active)
+// Counter resets.  Per iteration: the next live list, the material queues,
+// the shadow queue, the terminated list, the fetch counters of the
+// persistent kernels and the cache queries; at a chunk's start: all.
+__global__ __launch_bounds__(kBlock) void k_reset(uint32_t *counters, uint32_t cur, uint32_t chunk_start)
 {
     const uint32_t t = threadIdx.x;
-    if (t < kNumCounters) {
-        const bool wave_start = init_active != 0xffffffffu;
-        if (t == kCActive0 + cur) {
-            if (wave_start) counters[t] = init_active;
-        } else if (t == kCActive0 + (cur ^ 1u) || (t >= kCQueue && t <= kCTerm)) {
-            counters[t] = 0;
-        } else if (t == kCFetchExt || t == kCFetchCon) {
-            counters[t] = 0;
-        } else if (wave_start && (t == kCCacheQ || t == kCTrain)) {
-            counters[t] = 0;
-        }
+    if (t >= kNumCounters) return;
+    if (chunk_start) {
+        counters[t] = 0;
+    } else if (t == kCActive0 + (cur ^ 1u) || (t >= kCQueue && t <= kCTerm) || t == kCFetchExt ||
+               t == kCFetchCon || t == kCCacheQ) {
+        counters[t] = 0;
     }
 }
 
-// generate: camera rays for slots [0, count).  Global path index g is
-// sample * npix + pixel within its scene (a batch concatenates its scenes'
-// index ranges).
+// A new path in `slot`: global path index g (chunk index k).  Within its
+// scene, g is sample * npix + pixel (a batch concatenates its scenes' index
+// ranges).  Camera ray and PCG32 stream as the single-scene renderer.
 template <bool B>
-__global__ __launch_bounds__(kBlock) void k_generate(SceneDev sc0, BatchDev bt, PathsDev p, QueuesDev q,
-                                                     WaveParams wp)
+__device__ __forceinline__ void init_path(const SceneDev &sc0, const BatchDev &bt, const PathsDev &p,
+                                          const WaveParams &wp, uint32_t slot, uint64_t g, uint32_t k)
 {
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < wp.count; i += gridDim.x * kBlock) {
-        uint64_t g = wp.first_path + i;
-        uint32_t sid = 0;
-        if constexpr (B) {
-            uint32_t lo = 0, hi = bt.n;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (bt.bases[mid] <= g) lo = mid;
-                else hi = mid;
-            }
-            sid = lo;
-            g -= bt.bases[sid];
-            p.sid[i] = sid;
+    uint32_t sid = 0;
+    if constexpr (B) {
+        uint32_t lo = 0, hi = bt.n;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (bt.bases[mid] <= g) lo = mid;
+            else hi = mid;
         }
-        const SceneDev &sc = B ? bt.scenes[sid] : sc0;
-        const uint32_t npix = B ? sc.cam.width * sc.cam.height : wp.npix;
-        const uint32_t seed = B ? sc.seed : wp.seed;
-        const uint64_t stream_base = stream_base_of<B>(sc, wp);
-        const uint32_t first_sample = wp.spp_offset_dev ? *wp.spp_offset_dev : wp.spp_offset;
-        uint32_t sample = first_sample + (uint32_t)(g / npix);
-        uint32_t pix = (uint32_t)(g % npix);
-        if (!B && wp.train) {
-            // training paths: pseudo-random pixels, sample indices disjoint
-            // from the rendering samples (top bit set)
-            Pcg32 h = pcg32_seed(g, 0x7a11u);
-            pix = pcg32_next(h) % npix;
-            sample = 0x80000000u | (uint32_t)g;
-        }
-        const uint32_t px = pix % sc.cam.width, py = pix / sc.cam.width;
-        const uint64_t initstate = ((uint64_t)seed << 32) ^ (uint64_t)sample ^ 0x853c49e6748fea9bULL;
-        Pcg32 r = pcg32_seed(initstate, stream_base + pix);
-        const float u0 = pcg32_float(r), u1 = pcg32_float(r);
-        V3 o, d;
-        camera_ray(sc.cam, px, py, u0, u1, o, d);
-        p.ox[i] = o.x; p.oy[i] = o.y; p.oz[i] = o.z;
-        p.dx[i] = d.x; p.dy[i] = d.y; p.dz[i] = d.z;
-        p.bx[i] = 1; p.by[i] = 1; p.bz[i] = 1;
-        p.lx[i] = 0; p.ly[i] = 0; p.lz[i] = 0;
-        p.rng[i] = r.state;
-        p.pix[i] = pix;
-        p.meta[i] = 1u << 16;  // depth 0, camera ray counts as delta for MIS
-        p.ppdf[i] = 0;
-        p.ppx[i] = o.x; p.ppy[i] = o.y; p.ppz[i] = o.z;
-        p.nee_mask[i] = 0;
-        p.tvalid[i] = 0;
-        q.active[0][i] = i;
+        sid = lo;
+        g -= bt.bases[sid];
+        p.sid[slot] = sid;
+    }
+    const SceneDev &sc = B ? bt.scenes[sid] : sc0;
+    const uint32_t npix = B ? sc.cam.width * sc.cam.height : wp.npix;
+    const uint32_t seed = B ? sc.seed : wp.seed;
+    const uint64_t stream_base = stream_base_of<B>(sc, wp);
+    const uint32_t first_sample = wp.spp_offset_dev ? *wp.spp_offset_dev : wp.spp_offset;
+    uint32_t sample = first_sample + (uint32_t)(g / npix);
+    uint32_t pix = (uint32_t)(g % npix);
+    if (!B && wp.train) {
+        // training paths: pseudo-random pixels, sample indices disjoint
+        // from the rendering samples (top bit set)
+        Pcg32 h = pcg32_seed(g, 0x7a11u);
+        pix = pcg32_next(h) % npix;
+        sample = 0x80000000u | (uint32_t)g;
+    }
+    const uint32_t px = pix % sc.cam.width, py = pix / sc.cam.width;
+    const uint64_t initstate = ((uint64_t)seed << 32) ^ (uint64_t)sample ^ 0x853c49e6748fea9bULL;
+    Pcg32 r = pcg32_seed(initstate, stream_base + pix);
+    const float u0 = pcg32_float(r), u1 = pcg32_float(r);
+    V3 o, d;
+    camera_ray(sc.cam, px, py, u0, u1, o, d);
+    p.ox[slot] = o.x; p.oy[slot] = o.y; p.oz[slot] = o.z;
+    p.dx[slot] = d.x; p.dy[slot] = d.y; p.dz[slot] = d.z;
+    p.bx[slot] = 1; p.by[slot] = 1; p.bz[slot] = 1;
+    p.lx[slot] = 0; p.ly[slot] = 0; p.lz[slot] = 0;
+    p.rng[slot] = r.state;
+    p.pix[slot] = pix;
+    p.gidx[slot] = k;
+    p.meta[slot] = 1u << 16;  // depth 0, camera ray counts as delta for MIS
+    p.ppdf[slot] = 0;
+    p.ppx[slot] = o.x; p.ppy[slot] = o.y; p.ppz[slot] = o.z;
+    p.nee_mask[slot] = 0;
+    p.tvalid[slot] = 0;
+}
+
+// Path regeneration: every free slot (all slots at a chunk's start, else
+// the slots whose paths ended this iteration) claims the chunk's next path
+// index (one atomic per wavefront) and joins the next live list.  Claims
+// stop once the chunk is exhausted, which bounds the claim counter.
+template <bool B>
+__global__ __launch_bounds__(kBlock) void k_regen(SceneDev sc0, BatchDev bt, PathsDev p, QueuesDev q, WaveParams wp,
+                                                  uint32_t nxt, uint32_t init, uint32_t capacity)
+{
+    const uint64_t first = wp.chunk_dev ? wp.chunk_dev[0] : wp.first_path;
+    const uint32_t count = wp.chunk_dev ? (uint32_t)wp.chunk_dev[1] : wp.count;
+    const uint32_t nfree = init ? capacity : q.counters[kCTerm];
+    const uint32_t stride = gridDim.x * kBlock;
+    const uint32_t lane = __lane_id();
+    for (uint32_t i0 = blockIdx.x * kBlock; i0 < nfree; i0 += stride) {
+        uint32_t claimed = 0;
+        if (lane == 0) claimed = atomicAdd(&q.counters[kCNext], 0u);
+        claimed = __shfl(claimed, 0);
+        if (claimed >= count) break;  // wavefront-uniform
+        const uint32_t i = i0 + threadIdx.x;
+        const bool live = i < nfree;
+        const uint32_t slot = live ? (init ? i : q.term[i]) : 0;
+        const uint32_t k = wave_append(&q.counters[kCNext], live);
+        const bool gen = live && k < count;
+        if (gen) init_path<B>(sc0, bt, p, wp, slot, first + k, k);
+        const uint32_t pos = wave_append(&q.counters[kCActive0 + nxt], gen);
+        if (gen) q.active[nxt][pos] = slot;
     }
 }
 
@@ -576,7 +600,10 @@ __global__ __launch_bounds__(kBlock) void k_cache_vertex(SceneDev sc, PathsDev p
         const uint32_t k = wave_append(&q.counters[kCCacheQ], live);
         if (live) {
             write_record(p.cin + (size_t)k * 12, s, d, sc.mat[s.mat]);
-            p.cslot[k] = slot;
+            p.cidx[k] = p.gidx[slot];
+            p.cbeta[3 * (size_t)k] = p.bx[slot];
+            p.cbeta[3 * (size_t)k + 1] = p.by[slot];
+            p.cbeta[3 * (size_t)k + 2] = p.bz[slot];
         }
         const uint32_t tpos = wave_append(&q.counters[kCTerm], live);
         if (live) q.term[tpos] = slot;
@@ -626,42 +653,32 @@ __global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc0, BatchDev bt, P
 }
 
 // ---------------------------------------------------------------------
-// finalize: terminated paths take their last next-event contributions; the
-// radiance stays in the path slot until the wave's film pass.
+// finalize: paths that ended this iteration take their last next-event
+// contributions.  Rendering: the radiance goes to res[path index].
+// Training: a path that passed a cacheable vertex at the cache depth yields
+// the target (L_end - L_arrival) / beta_arrival per channel (0 where beta is
+// 0), compacted with its record into cin / ttgt; non-finite targets are
+// dropped.
 // ---------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_finalize(PathsDev p, QueuesDev q, uint32_t which, uint32_t cur)
+__global__ __launch_bounds__(kBlock) void k_finalize(PathsDev p, QueuesDev q, float *res, uint32_t train)
 {
-    const uint32_t n = which ? q.counters[kCActive0 + cur] : q.counters[kCTerm];
-    const uint32_t *list = which ? q.active[cur] : q.term;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
-        resolve_nee(p, list[i]);
-}
-
-// L += beta * cache(record) for every cache query of the wave
-__global__ __launch_bounds__(kBlock) void k_cache_apply(PathsDev p, QueuesDev q)
-{
-    const uint32_t n = q.counters[kCCacheQ];
-    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
-        const uint32_t slot = p.cslot[k];
-        const float *c = p.crgb + 3 * (size_t)k;
-        p.lx[slot] = p.lx[slot] + p.bx[slot] * c[0];
-        p.ly[slot] = p.ly[slot] + p.by[slot] * c[1];
-        p.lz[slot] = p.lz[slot] + p.bz[slot] * c[2];
-    }
-}
-
-// Training targets of a finished training wave: for every path that passed a
-// cacheable vertex at the cache depth, (L_end - L_arrival) / beta_arrival per
-// channel (0 where beta is 0), compacted with its record into cin / ttgt.
-// Non-finite targets are dropped.
-__global__ __launch_bounds__(kBlock) void k_train_targets(PathsDev p, QueuesDev q, uint32_t count)
-{
+    const uint32_t n = q.counters[kCTerm];
     const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t i0 = blockIdx.x * kBlock; i0 < count; i0 += stride) {
-        const uint32_t slot = i0 + threadIdx.x;
+    for (uint32_t i0 = blockIdx.x * kBlock; i0 < n; i0 += stride) {
+        const uint32_t i = i0 + threadIdx.x;
+        const bool live = i < n;
+        const uint32_t slot = live ? q.term[i] : 0;
+        if (live) resolve_nee(p, slot);
+        if (!train) {
+            if (live) {
+                float *r = res + 3 * (size_t)p.gidx[slot];
+                r[0] = p.lx[slot]; r[1] = p.ly[slot]; r[2] = p.lz[slot];
+            }
+            continue;  // wavefront-uniform (train is a kernel argument)
+        }
         bool ok = false;
         float tg[3] = {0, 0, 0};
-        if (slot < count && p.tvalid[slot]) {
+        if (live && p.tvalid[slot]) {
             const float Le[3] = {p.lx[slot], p.ly[slot], p.lz[slot]};
             ok = true;
             for (int c = 0; c < 3; ++c) {
@@ -682,22 +699,36 @@ __global__ __launch_bounds__(kBlock) void k_train_targets(PathsDev p, QueuesDev 
     }
 }
 
+// res[path] += beta * cache(record) for this iteration's cache queries
+__global__ __launch_bounds__(kBlock) void k_cache_apply(PathsDev p, QueuesDev q, float *res)
+{
+    const uint32_t n = q.counters[kCCacheQ];
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+        float *r = res + 3 * (size_t)p.cidx[k];
+        const float *c = p.crgb + 3 * (size_t)k;
+        const float *b = p.cbeta + 3 * (size_t)k;
+        r[0] = r[0] + b[0] * c[0];
+        r[1] = r[1] + b[1] * c[1];
+        r[2] = r[2] + b[2] * c[2];
+    }
+}
+
 // film: FP64 accumulation with the reference's NaN/Inf sentinel bookkeeping
 // (pathTracing.cu:474-486): only finite samples are summed and counted (the
 // consumer rescales by samples / nvalid).  One thread per pixel sums that
-// pixel's samples of the wave in sample order, so the result is
-// bit-reproducible and needs no atomics.  Slots [lo, hi) of the wave belong
-// to one scene of npix pixels; slot lo is that scene's pixel pix0.
-__global__ __launch_bounds__(kBlock) void k_film(PathsDev p, FilmDev film, uint32_t lo, uint32_t hi, uint32_t npix,
-                                                 uint32_t pix0)
+// pixel's samples in sample order, so the result is bit-reproducible and
+// needs no atomics.  Chunk paths [lo, hi) belong to one scene of npix
+// pixels; path lo is that scene's pixel pix0.
+__global__ __launch_bounds__(kBlock) void k_film(const float *res, FilmDev film, uint32_t lo, uint32_t hi,
+                                                 uint32_t npix, uint32_t pix0)
 {
     const uint32_t span = hi - lo;
     const uint32_t npx = span < npix ? span : npix;
     for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < npx; t += gridDim.x * kBlock) {
         double sx = 0, sy = 0, sz = 0;
         uint32_t nv = 0;
-        for (uint32_t slot = lo + t; slot < hi; slot += npix) {
-            const float lx = p.lx[slot], ly = p.ly[slot], lz = p.lz[slot];
+        for (uint32_t k = lo + t; k < hi; k += npix) {
+            const float lx = res[3 * (size_t)k], ly = res[3 * (size_t)k + 1], lz = res[3 * (size_t)k + 2];
             if (isfinite(lx) && isfinite(ly) && isfinite(lz)) {
                 sx += (double)lx;
                 sy += (double)ly;
@@ -733,6 +764,17 @@ hipError_t set_u32(uint32_t *p, uint32_t v, hipStream_t s)
     return hipGetLastError();
 }
 
+__global__ void k_set_u64(uint64_t *p, uint64_t v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+hipError_t set_u64(uint64_t *p, uint64_t v, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, s, p, v);
+    return hipGetLastError();
+}
+
 size_t extend_lds_bytes(uint32_t stack_cap) { return (size_t)stack_cap * kBlock * sizeof(int32_t); }
 
 // Workgroups of a persistent traversal launch: as many as fit on the chip
@@ -751,89 +793,137 @@ static dim3 persistent_grid(size_t lds)
     return dim3((unsigned)(cus * per_cu));
 }
 
-hipError_t render_wave(const SceneDev &sc, const PathsDev &paths, const ShadowDev &shadow, const QueuesDev &q,
-                       const FilmDev &film, const WaveParams &wp, uint32_t capacity, hipStream_t stream,
-                       const CacheDev *cache, const CacheWork *cwork, const BatchLaunch *batch)
+hipError_t chunk_begin(const ChunkLaunch &c, const WaveParams &wp, hipStream_t s)
 {
-    const bool B = batch != nullptr;
-    const uint32_t stack_cap = B ? batch->stack_cap : sc.bvh.stack_cap;
+    const dim3 blk(kBlock);
+    const dim3 grid(wp.grid ? wp.grid : kGrid);
+    const SceneDev &sc = c.batch ? *c.scene : *c.scene;
+    const BatchDev bt = c.batch ? c.batch->dev : BatchDev{nullptr, nullptr, 0};
+    hipLaunchKernelGGL(k_reset, dim3(1), blk, 0, s, c.queues.counters, 0u, 1u);
+    if (c.batch) hipLaunchKernelGGL(k_regen<true>, grid, blk, 0, s, sc, bt, c.paths, c.queues, wp, 0u, 1u, c.capacity);
+    else hipLaunchKernelGGL(k_regen<false>, grid, blk, 0, s, sc, bt, c.paths, c.queues, wp, 0u, 1u, c.capacity);
+    return hipGetLastError();
+}
+
+hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t iterations, hipStream_t s)
+{
+    const bool B = c.batch != nullptr;
+    const SceneDev &sc = *c.scene;
+    const BatchDev bt = B ? c.batch->dev : BatchDev{nullptr, nullptr, 0};
+    const uint32_t stack_cap = B ? c.batch->stack_cap : sc.bvh.stack_cap;
     const bool wide = sc.bvh.wide != 0;  // a batch's scenes share one layout
     const int cap = (int)stack_cap;
     const size_t lds = extend_lds_bytes(stack_cap);
     const dim3 blk(kBlock);
     const dim3 grid(wp.grid ? wp.grid : kGrid);
     const dim3 pgrid = persistent_grid(lds);
-    const uint32_t gen_grid = (wp.count + kBlock - 1) / kBlock;
-    const BatchDev bt = B ? batch->dev : BatchDev{nullptr, nullptr, 0};
+    const PathsDev &paths = c.paths;
+    const ShadowDev &shadow = c.shadow;
+    const QueuesDev &q = c.queues;
     // cache vertices end rendering paths only; training paths trace to the end
-    const bool lookups = !B && !wp.train && wp.cache_depth != kNoCache && cache && cwork;
+    const bool lookups = !B && !wp.train && wp.cache_depth != kNoCache && c.cache && c.cwork;
     if (!wp.train && wp.cache_depth != kNoCache && !lookups) return hipErrorInvalidValue;
     if (B && wp.train) return hipErrorInvalidValue;
-    if (lookups && cwork->rows < capacity) return hipErrorInvalidValue;
+    if (lookups && c.cwork->rows < c.capacity) return hipErrorInvalidValue;
+    if (iterations & 1u) return hipErrorInvalidValue;
     const uint32_t ext_depth = lookups ? wp.cache_depth : kNoCache;
-    hipLaunchKernelGGL(k_reset, dim3(1), blk, 0, stream, q.counters, 0u, wp.count);
-    if (B) hipLaunchKernelGGL(k_generate<true>, dim3(gen_grid ? gen_grid : 1), blk, 0, stream, sc, bt, paths, q, wp);
-    else hipLaunchKernelGGL(k_generate<false>, dim3(gen_grid ? gen_grid : 1), blk, 0, stream, sc, bt, paths, q, wp);
     uint32_t cur = 0;
-    for (uint32_t bounce = 0; bounce < wp.max_depth; ++bounce) {
-        if (bounce > 0) hipLaunchKernelGGL(k_reset, dim3(1), blk, 0, stream, q.counters, cur, 0xffffffffu);
+    for (uint32_t it = 0; it < iterations; ++it) {
+        const uint32_t nxt = cur ^ 1u;
+        hipLaunchKernelGGL(k_reset, dim3(1), blk, 0, s, q.counters, cur, 0u);
         if (B) {
-            if (wide) hipLaunchKernelGGL((k_extend<true, true>), pgrid, blk, lds, stream, sc, bt, paths, q, cur, cap);
-            else hipLaunchKernelGGL((k_extend<true, false>), pgrid, blk, lds, stream, sc, bt, paths, q, cur, cap);
-            hipLaunchKernelGGL(k_classify<true>, grid, blk, 0, stream, sc, bt, paths, q, cur, ext_depth);
-            hipLaunchKernelGGL((k_shade<kQLambert, true>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
-            hipLaunchKernelGGL((k_shade<kQDielectric, true>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
-            hipLaunchKernelGGL((k_shade<kQGgx, true>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
-            hipLaunchKernelGGL((k_shade<kQLight, true>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
-            hipLaunchKernelGGL((k_shade<kQMiss, true>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
+            if (wide) hipLaunchKernelGGL((k_extend<true, true>), pgrid, blk, lds, s, sc, bt, paths, q, cur, cap);
+            else hipLaunchKernelGGL((k_extend<true, false>), pgrid, blk, lds, s, sc, bt, paths, q, cur, cap);
+            hipLaunchKernelGGL(k_classify<true>, grid, blk, 0, s, sc, bt, paths, q, cur, ext_depth);
+            hipLaunchKernelGGL((k_shade<kQLambert, true>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
+            hipLaunchKernelGGL((k_shade<kQDielectric, true>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
+            hipLaunchKernelGGL((k_shade<kQGgx, true>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
+            hipLaunchKernelGGL((k_shade<kQLight, true>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
+            hipLaunchKernelGGL((k_shade<kQMiss, true>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
         } else {
-            if (wide) hipLaunchKernelGGL((k_extend<false, true>), pgrid, blk, lds, stream, sc, bt, paths, q, cur, cap);
-            else hipLaunchKernelGGL((k_extend<false, false>), pgrid, blk, lds, stream, sc, bt, paths, q, cur, cap);
-            hipLaunchKernelGGL(k_classify<false>, grid, blk, 0, stream, sc, bt, paths, q, cur, ext_depth);
-            hipLaunchKernelGGL((k_shade<kQLambert, false>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
-            hipLaunchKernelGGL((k_shade<kQDielectric, false>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
-            hipLaunchKernelGGL((k_shade<kQGgx, false>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
-            hipLaunchKernelGGL((k_shade<kQLight, false>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
-            hipLaunchKernelGGL((k_shade<kQMiss, false>), grid, blk, 0, stream, sc, bt, paths, shadow, q, wp, cur);
+            if (wide) hipLaunchKernelGGL((k_extend<false, true>), pgrid, blk, lds, s, sc, bt, paths, q, cur, cap);
+            else hipLaunchKernelGGL((k_extend<false, false>), pgrid, blk, lds, s, sc, bt, paths, q, cur, cap);
+            hipLaunchKernelGGL(k_classify<false>, grid, blk, 0, s, sc, bt, paths, q, cur, ext_depth);
+            hipLaunchKernelGGL((k_shade<kQLambert, false>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
+            hipLaunchKernelGGL((k_shade<kQDielectric, false>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
+            hipLaunchKernelGGL((k_shade<kQGgx, false>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
+            hipLaunchKernelGGL((k_shade<kQLight, false>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
+            hipLaunchKernelGGL((k_shade<kQMiss, false>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
         }
-        if (lookups && bounce >= wp.cache_depth)
-            hipLaunchKernelGGL(k_cache_vertex, grid, blk, 0, stream, sc, paths, q);
-        hipLaunchKernelGGL(k_count, dim3(1), blk, 0, stream, q, cur);
-        if (B && wide) hipLaunchKernelGGL((k_connect<true, true>), pgrid, blk, lds, stream, sc, bt, paths, shadow, q, cap);
-        else if (B) hipLaunchKernelGGL((k_connect<true, false>), pgrid, blk, lds, stream, sc, bt, paths, shadow, q, cap);
-        else if (wide) hipLaunchKernelGGL((k_connect<false, true>), pgrid, blk, lds, stream, sc, bt, paths, shadow, q, cap);
-        else hipLaunchKernelGGL((k_connect<false, false>), pgrid, blk, lds, stream, sc, bt, paths, shadow, q, cap);
-        hipLaunchKernelGGL(k_finalize, grid, blk, 0, stream, paths, q, 0u, cur);
-        cur ^= 1u;
+        if (lookups) hipLaunchKernelGGL(k_cache_vertex, grid, blk, 0, s, sc, paths, q);
+        hipLaunchKernelGGL(k_count, dim3(1), blk, 0, s, q, cur);
+        if (B && wide) hipLaunchKernelGGL((k_connect<true, true>), pgrid, blk, lds, s, sc, bt, paths, shadow, q, cap);
+        else if (B) hipLaunchKernelGGL((k_connect<true, false>), pgrid, blk, lds, s, sc, bt, paths, shadow, q, cap);
+        else if (wide) hipLaunchKernelGGL((k_connect<false, true>), pgrid, blk, lds, s, sc, bt, paths, shadow, q, cap);
+        else hipLaunchKernelGGL((k_connect<false, false>), pgrid, blk, lds, s, sc, bt, paths, shadow, q, cap);
+        hipLaunchKernelGGL(k_finalize, grid, blk, 0, s, paths, q, c.res, wp.train);
+        if (lookups) {
+            hipError_t e = cache_infer(*c.cache, *c.cwork, paths.cin, c.capacity, paths.crgb, s,
+                                       &q.counters[kCCacheQ]);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(k_cache_apply, grid, blk, 0, s, paths, q, c.res);
+        }
+        if (B) hipLaunchKernelGGL(k_regen<true>, grid, blk, 0, s, sc, bt, paths, q, wp, nxt, 0u, c.capacity);
+        else hipLaunchKernelGGL(k_regen<false>, grid, blk, 0, s, sc, bt, paths, q, wp, nxt, 0u, c.capacity);
+        cur = nxt;
     }
-    // paths still alive after max_depth bounces cannot exist (shade stops
-    // them at max_depth), but the list is flushed for safety
-    hipLaunchKernelGGL(k_finalize, grid, blk, 0, stream, paths, q, 1u, cur);
-    if (wp.train) {
-        hipLaunchKernelGGL(k_train_targets, grid, blk, 0, stream, paths, q, wp.count);
-        return hipGetLastError();
-    }
-    if (lookups) {
-        hipError_t e = cache_infer(*cache, *cwork, paths.cin, wp.count, paths.crgb, stream, &q.counters[kCCacheQ]);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_cache_apply, grid, blk, 0, stream, paths, q);
-    }
-    if (!B) {
-        hipLaunchKernelGGL(k_film, grid, blk, 0, stream, paths, film, 0u, wp.count, wp.npix,
+    return hipGetLastError();
+}
+
+hipError_t chunk_film(const ChunkLaunch &c, const FilmDev &film, const WaveParams &wp, hipStream_t s)
+{
+    if (wp.train) return hipSuccess;
+    const dim3 blk(kBlock);
+    const dim3 grid(wp.grid ? wp.grid : kGrid);
+    if (!c.batch) {
+        hipLaunchKernelGGL(k_film, grid, blk, 0, s, c.res, film, 0u, wp.count, wp.npix,
                            (uint32_t)(wp.first_path % wp.npix));
     } else {
-        // one film segment per scene that has paths in this wave
+        // one film segment per scene that has paths in this chunk
+        const BatchLaunch &b = *c.batch;
         const uint64_t w0 = wp.first_path, w1 = wp.first_path + wp.count;
-        for (uint32_t s = 0; s < batch->dev.n; ++s) {
-            const uint64_t a = batch->bases[s] > w0 ? batch->bases[s] : w0;
-            const uint64_t b = batch->bases[s + 1] < w1 ? batch->bases[s + 1] : w1;
-            if (a >= b) continue;
-            const uint32_t np = batch->npix[s];
-            hipLaunchKernelGGL(k_film, grid, blk, 0, stream, paths, batch->films[s], (uint32_t)(a - w0),
-                               (uint32_t)(b - w0), np, (uint32_t)((a - batch->bases[s]) % np));
+        for (uint32_t sidx = 0; sidx < b.dev.n; ++sidx) {
+            const uint64_t lo = b.bases[sidx] > w0 ? b.bases[sidx] : w0;
+            const uint64_t hi = b.bases[sidx + 1] < w1 ? b.bases[sidx + 1] : w1;
+            if (lo >= hi) continue;
+            const uint32_t np = b.npix[sidx];
+            hipLaunchKernelGGL(k_film, grid, blk, 0, s, c.res, b.films[sidx], (uint32_t)(lo - w0), (uint32_t)(hi - w0),
+                               np, (uint32_t)((lo - b.bases[sidx]) % np));
         }
     }
     return hipGetLastError();
+}
+
+hipError_t render_chunk(const ChunkLaunch &c, const FilmDev &film, const WaveParams &wp, uint32_t *host_counters,
+                        hipStream_t s)
+{
+    constexpr uint32_t kBlockIters = 4;
+    hipError_t e = chunk_begin(c, wp, s);
+    if (e != hipSuccess) return e;
+    if (!host_counters) {
+        // no read-back: every path is generated at the start (count <=
+        // capacity) and needs at most max_depth iterations
+        if (wp.count > c.capacity) return hipErrorInvalidValue;
+        e = chunk_iterate(c, wp, (wp.max_depth + 1u) & ~1u, s);
+        if (e != hipSuccess) return e;
+        return chunk_film(c, film, wp, s);
+    }
+    // While unclaimed paths remain every slot is busy, and a slot holds a
+    // path for at most max_depth iterations; the drain takes max_depth more.
+    const uint64_t rounds = ((uint64_t)wp.count + c.capacity - 1) / c.capacity + 1;
+    const uint64_t max_iters = rounds * wp.max_depth + kBlockIters;
+    bool done = false;
+    for (uint64_t it = 0; it < max_iters && !done; it += kBlockIters) {
+        e = chunk_iterate(c, wp, kBlockIters, s);
+        if (e != hipSuccess) return e;
+        e = hipMemcpyAsync(host_counters, c.queues.counters, kNumCounters * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+        done = host_counters[kCNext] >= wp.count && host_counters[kCActive0] == 0;
+    }
+    if (!done) return hipErrorUnknown;  // cannot happen: the bound above holds
+    return chunk_film(c, film, wp, s);
 }
 
 }  // namespace rt

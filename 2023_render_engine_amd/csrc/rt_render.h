@@ -17,7 +17,7 @@ enum : uint32_t { kQLambert = 0, kQDielectric = 1, kQGgx = 2, kQLight = 3, kQMis
 // Device counters (one u32 each).
 enum : uint32_t {
     kCActive0 = 0, kCActive1 = 1, kCQueue = 2, /* 2..7 */ kCShadow = 8, kCTerm = 9, kCCacheQ = 10,
-    kCTrain = 11, kCFetchExt = 12, kCFetchCon = 13, kNumCounters = 16
+    kCTrain = 11, kCFetchExt = 12, kCFetchCon = 13, kCNext = 14, kNumCounters = 16
 };
 
 constexpr uint32_t kNoCache = 0xffffffffu;
@@ -55,6 +55,7 @@ struct PathsDev {
     float *lx, *ly, *lz;        // radiance
     uint64_t *rng;              // PCG32 state (the increment is derived from the pixel)
     uint32_t *pix;
+    uint32_t *gidx;             // path index within the chunk (result slot)
     uint32_t *sid;              // scene of the path (batched waves)
     uint32_t *meta;             // depth | prev_delta << 16
     float *ppdf, *ppx, *ppy, *ppz;  // previous BSDF pdf and vertex (MIS of sphere lights)
@@ -66,7 +67,8 @@ struct PathsDev {
     uint32_t nee_stride;        // light slots per path (max over a batch's scenes)
     // radiance cache: query records (P x 12) and results (P x 3), in queue order
     float *cin, *crgb;
-    uint32_t *cslot;            // path slot of cache query k
+    uint32_t *cidx;             // result slot (chunk path index) of cache query k
+    float *cbeta;               // P x 3: throughput on arrival at the cache vertex
     // training paths: the record at the cache depth, L and beta on arrival
     float *tin;                 // P x 12
     float *tLb, *tB;            // P x 3 each
@@ -93,8 +95,8 @@ struct FilmDev {
 };
 
 struct WaveParams {
-    uint64_t first_path;        // global path index of slot 0
-    uint32_t count;             // paths in this wave
+    uint64_t first_path;        // global path index of the chunk's first path
+    uint32_t count;             // paths in the chunk
     uint32_t npix;              // W*H
     uint32_t spp_offset, seed, max_depth, rr_depth;
     uint64_t stream_base;       // PCG stream offset (scene id for batches)
@@ -102,6 +104,7 @@ struct WaveParams {
     uint32_t train;             // 1: training wave (random pixels, no film, records at cache_depth)
     uint32_t grid;              // workgroups of the queue kernels (0: default)
     const uint32_t *spp_offset_dev;  // if set, the first sample index is read here (graph replays)
+    const uint64_t *chunk_dev;  // if set, {first_path, count} are read here (graph replays)
 };
 
 // Host view of a batch for one wave launch: the device tables plus, per
@@ -117,22 +120,45 @@ struct BatchLaunch {
 struct CacheDev;
 struct CacheWork;
 
-// Launches one wave (generate, max_depth x {extend, shade, connect},
-// finalize, cache lookups, film) on `stream`.  With a cache and a rendering
-// wave, hits at cache_depth are answered by the cache; with a training wave
-// the compacted records and targets are left in paths.cin / paths.ttgt with
-// their count in counters[kCTrain].  With `batch`, the wave's paths belong to
-// the batch's scenes (no cache); `scene` and `film` are then unused.  Kernel
-// launches only (no memset, copy or host synchronisation), so the sequence
-// can be captured into a graph.  Fails only on launch errors.
-hipError_t render_wave(const SceneDev &scene, const PathsDev &paths, const ShadowDev &shadow,
-                       const QueuesDev &queues, const FilmDev &film, const WaveParams &wp, uint32_t capacity,
-                       hipStream_t stream, const CacheDev *cache = nullptr, const CacheWork *cwork = nullptr,
-                       const BatchLaunch *batch = nullptr);
+// Streaming renderer.  A chunk of wp.count paths (global indices
+// wp.first_path ...) runs through `capacity` path slots: every iteration
+// extends, shades and connects the live paths, writes the radiance of the
+// paths that ended to res[path index] and refills their slots with the
+// chunk's next paths, so the slots stay full until the chunk runs dry.
+// The film pass then sums res per pixel in sample order (bit-reproducible).
+//
+// Everything a stage launches is kernels only (no memset, copy or host
+// synchronisation), so stages can be captured into graphs; the chunk's
+// first path and count are read from device memory (chunk_dev) so one
+// captured iteration block serves every chunk.
+struct ChunkLaunch {
+    const SceneDev *scene;      // single scene (batch == nullptr)
+    PathsDev paths;
+    ShadowDev shadow;
+    QueuesDev queues;
+    uint32_t capacity;
+    float *res;                 // 3 floats per chunk path
+    const CacheDev *cache;      // rendering with cache lookups (single scene)
+    const CacheWork *cwork;
+    const BatchLaunch *batch;
+};
+// chunk start: clears every counter and fills all slots
+hipError_t chunk_begin(const ChunkLaunch &c, const WaveParams &wp, hipStream_t s);
+// `iterations` bounce iterations (even, so the live list ends where it began)
+hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t iterations, hipStream_t s);
+// film pass of a finished chunk (FP64 sums; skipped for training)
+hipError_t chunk_film(const ChunkLaunch &c, const FilmDev &film, const WaveParams &wp, hipStream_t s);
+
+// Host driver: begin, iterate in blocks of 4 until the chunk's paths are
+// all generated and none is live (one small read-back per block), film.
+// host_counters: pinned host memory for kNumCounters words.
+hipError_t render_chunk(const ChunkLaunch &c, const FilmDev &film, const WaveParams &wp, uint32_t *host_counters,
+                        hipStream_t s);
 
 size_t extend_lds_bytes(uint32_t stack_cap);
 
 // *p = v from a one-thread kernel (for values a captured graph reads).
 hipError_t set_u32(uint32_t *p, uint32_t v, hipStream_t s);
+hipError_t set_u64(uint64_t *p, uint64_t v, hipStream_t s);
 
 }  // namespace rt

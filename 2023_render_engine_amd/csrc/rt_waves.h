@@ -10,17 +10,26 @@
 namespace rt {
 
 struct WaveBuffers {
-    uint32_t capacity = 0;    // paths per wave
+    uint32_t capacity = 0;    // path slots
     uint32_t cap_lights = 0;  // light slots per path
     void *arena = nullptr;
     PathsDev paths{};
     ShadowDev shadow{};
     QueuesDev queues{};
+    float *res = nullptr;     // per-path radiance of a chunk (3 floats each)
+    uint64_t res_cap = 0;     // paths res holds
+    uint32_t *host_counters = nullptr;  // pinned, kNumCounters words
 };
 
-// Grows `w` to at least P paths with NL light slots each (one allocation,
-// 256-byte aligned pieces).  `who` names the caller in the error text.
+// Grows `w` to at least P path slots with NL light slots each (one
+// allocation, 256-byte aligned pieces).  `who` names the caller in the
+// error text.
 rt_status ensure_wave_buffers(WaveBuffers &w, uint32_t P, uint32_t NL, const char *who);
+// Grows the chunk result buffer to R paths.
+rt_status ensure_result_buffer(WaveBuffers &w, uint64_t R, const char *who);
 void free_wave_buffers(WaveBuffers &w);
+
+// Paths per chunk: the result buffer holds 12 bytes per path (768 MiB).
+constexpr uint64_t kChunkPaths = 1ull << 26;
 
 }  // namespace rt

@@ -195,7 +195,33 @@ namespace rt {
 void free_wave_buffers(WaveBuffers &w)
 {
     if (w.arena) (void)hipFree(w.arena);
+    if (w.res) (void)hipFree(w.res);
+    if (w.host_counters) (void)hipHostFree(w.host_counters);
     w = WaveBuffers{};
+}
+
+rt_status ensure_result_buffer(WaveBuffers &w, uint64_t R, const char *who)
+{
+    if (!w.host_counters) {
+        hipError_t e = hipHostMalloc((void **)&w.host_counters, kNumCounters * sizeof(uint32_t), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            w.host_counters = nullptr;
+            set_error(std::string(who) + ": pinned counters: " + hipGetErrorString(e));
+            return RT_ERR_HIP;
+        }
+    }
+    if (w.res_cap >= R && w.res) return RT_OK;
+    if (w.res) (void)hipFree(w.res);
+    w.res = nullptr;
+    w.res_cap = 0;
+    hipError_t e = hipMalloc(&w.res, std::max<uint64_t>(1, R) * 12);
+    if (e != hipSuccess) {
+        w.res = nullptr;
+        set_error(std::string(who) + ": result buffer: " + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;
+    }
+    w.res_cap = R;
+    return RT_OK;
 }
 
 static void free_arena(rt_scene *s)
@@ -234,7 +260,9 @@ rt_status ensure_wave_buffers(WaveBuffers &w, uint32_t P, uint32_t NL, const cha
 {
     NL = std::max<uint32_t>(1, NL);
     if (w.capacity >= P && w.cap_lights >= NL) return RT_OK;
-    free_wave_buffers(w);
+    if (w.arena) (void)hipFree(w.arena);
+    w.arena = nullptr;
+    w.capacity = 0;
     const size_t SQ = (size_t)P * NL;
     size_t total = 0;
     auto take = [&](size_t bytes) {
@@ -251,6 +279,7 @@ rt_status ensure_wave_buffers(WaveBuffers &w, uint32_t P, uint32_t NL, const cha
     for (float **f : pf) add((void **)f, (size_t)P * 4);
     add((void **)&p.rng, (size_t)P * 8);
     add((void **)&p.pix, (size_t)P * 4);
+    add((void **)&p.gidx, (size_t)P * 4);
     add((void **)&p.sid, (size_t)P * 4);
     add((void **)&p.meta, (size_t)P * 4);
     add((void **)&p.hid, (size_t)P * 4);
@@ -258,7 +287,8 @@ rt_status ensure_wave_buffers(WaveBuffers &w, uint32_t P, uint32_t NL, const cha
     add((void **)&p.nee_c, (size_t)P * NL * 12);
     add((void **)&p.cin, (size_t)P * 48);
     add((void **)&p.crgb, (size_t)P * 12);
-    add((void **)&p.cslot, (size_t)P * 4);
+    add((void **)&p.cidx, (size_t)P * 4);
+    add((void **)&p.cbeta, (size_t)P * 12);
     add((void **)&p.tin, (size_t)P * 48);
     add((void **)&p.tLb, (size_t)P * 12);
     add((void **)&p.tB, (size_t)P * 12);
@@ -538,20 +568,31 @@ static rt_status render_frames(const char *fn, rt_scene *s, const rt_render_opts
     uint32_t P = opts->path_capacity ? opts->path_capacity : (1u << 21);
     if (P > (1u << 26)) { set_error(name + ": path_capacity above 2^26"); return RT_ERR_INVALID; }
     if ((uint64_t)P > total) P = (uint32_t)std::max<uint64_t>(1, total);
+    const uint64_t R = std::min<uint64_t>(total, rt::kChunkPaths);
     rt_status st = rt::ensure_buffers(s, P, fn);
+    if (st == RT_OK) st = rt::ensure_result_buffer(s->wb, R, fn);
     if (st != RT_OK) return st;
     if (cache) {
         st = rt::ensure_cache_work(s, s->wb.capacity);
         if (st != RT_OK) return st;
     }
     auto t0 = std::chrono::steady_clock::now();
-    RT_HIP(hipMemsetAsync(s->wb.queues.counters, 0, rt::kNumCounters * 4, stream));
     RT_HIP(hipMemsetAsync(s->wb.queues.stats, 0, 16, stream));
     rt::FilmDev film{accum, nvalid};
-    for (uint64_t first = 0; first < total; first += P) {
+    rt::ChunkLaunch cl{};
+    cl.scene = &s->dev;
+    cl.paths = s->wb.paths;
+    cl.shadow = s->wb.shadow;
+    cl.queues = s->wb.queues;
+    cl.capacity = std::min<uint32_t>(P, s->wb.capacity);
+    cl.res = s->wb.res;
+    cl.cache = cache ? &rt::cache_device(cache) : nullptr;
+    cl.cwork = cache ? &s->cwork : nullptr;
+    cl.batch = nullptr;
+    for (uint64_t first = 0; first < total; first += R) {
         rt::WaveParams wp{};
         wp.first_path = first;
-        wp.count = (uint32_t)std::min<uint64_t>(P, total - first);
+        wp.count = (uint32_t)std::min<uint64_t>(R, total - first);
         wp.npix = npix;
         wp.spp_offset = opts->spp_offset;
         wp.seed = opts->seed;
@@ -561,8 +602,7 @@ static rt_status render_frames(const char *fn, rt_scene *s, const rt_render_opts
         wp.cache_depth = cache ? cache_depth : rt::kNoCache;
         wp.train = 0;
         wp.grid = opts->grid;
-        RT_HIP(rt::render_wave(s->dev, s->wb.paths, s->wb.shadow, s->wb.queues, film, wp, s->wb.capacity, stream,
-                               cache ? &rt::cache_device(cache) : nullptr, cache ? &s->cwork : nullptr));
+        RT_HIP(rt::render_chunk(cl, film, wp, s->wb.host_counters, stream));
     }
     unsigned long long counts[2];
     RT_HIP(hipMemcpyAsync(counts, s->wb.queues.stats, sizeof counts, hipMemcpyDeviceToHost, stream));
@@ -602,7 +642,6 @@ rt_status rt_cache_train_scene(rt_cache *c, rt_scene *s, const rt_render_opts *o
     hipStream_t stream = stream_ ? (hipStream_t)stream_ : (hipStream_t)rt::ctx_stream_ptr(s->ctx);
     rt_status st = rt::ensure_buffers(s, n_paths, "rt_cache_train_scene");
     if (st != RT_OK) return st;
-    RT_HIP(hipMemsetAsync(s->wb.queues.counters, 0, rt::kNumCounters * 4, stream));
     RT_HIP(hipMemsetAsync(s->wb.queues.stats, 0, 16, stream));
     rt::WaveParams wp{};
     wp.first_path = (uint64_t)opts->spp_offset * n_paths;
@@ -616,7 +655,14 @@ rt_status rt_cache_train_scene(rt_cache *c, rt_scene *s, const rt_render_opts *o
     wp.cache_depth = cache_depth;
     wp.train = 1;
     rt::FilmDev film{nullptr, nullptr};
-    RT_HIP(rt::render_wave(s->dev, s->wb.paths, s->wb.shadow, s->wb.queues, film, wp, s->wb.capacity, stream));
+    rt::ChunkLaunch cl{};
+    cl.scene = &s->dev;
+    cl.paths = s->wb.paths;
+    cl.shadow = s->wb.shadow;
+    cl.queues = s->wb.queues;
+    cl.capacity = n_paths;  // one slot per training path: no read-back needed
+    // no host read-back: every training path starts at once and ends within max_depth iterations
+    RT_HIP(rt::render_chunk(cl, film, wp, nullptr, stream));
     // the sample count stays on the device: no host round trip unless the
     // caller asks for the loss or the count
     const uint32_t *k_dev = s->wb.queues.counters + rt::kCTrain;

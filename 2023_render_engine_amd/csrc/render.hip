@@ -247,7 +247,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc0, BatchDev bt, Pa
 // with wavefront ballots (one atomic per 64 lanes per queue).
 template <bool B>
 __global__ __launch_bounds__(kBlock) void k_classify(SceneDev sc0, BatchDev bt, PathsDev p, QueuesDev q, uint32_t cur,
-                                                     uint32_t cache_depth)
+                                                     uint32_t cache_depth, uint32_t need_record)
 {
     const uint32_t n = q.counters[kCActive0 + cur];
     // every lane of a wavefront runs the same number of iterations so the
@@ -278,7 +278,9 @@ __global__ __launch_bounds__(kBlock) void k_classify(SceneDev sc0, BatchDev bt, 
                     p.ht[slot] = th;
                 }
             }
-            const bool deep = cache_depth != kNoCache && (p.meta[slot] & 0xffffu) >= cache_depth;
+            // training paths end in the cache only after their training record
+            const bool deep = cache_depth != kNoCache && (p.meta[slot] & 0xffffu) >= cache_depth &&
+                              (!need_record || p.tvalid[slot] != 0);
             cls = classify(sc, hid, deep);
         }
 #pragma unroll
@@ -599,11 +601,8 @@ __global__ __launch_bounds__(kBlock) void k_cache_vertex(SceneDev sc, PathsDev p
         }
         const uint32_t k = wave_append(&q.counters[kCCacheQ], live);
         if (live) {
-            write_record(p.cin + (size_t)k * 12, s, d, sc.mat[s.mat]);
-            p.cidx[k] = p.gidx[slot];
-            p.cbeta[3 * (size_t)k] = p.bx[slot];
-            p.cbeta[3 * (size_t)k + 1] = p.by[slot];
-            p.cbeta[3 * (size_t)k + 2] = p.bz[slot];
+            write_record(p.cqin + (size_t)k * 12, s, d, sc.mat[s.mat]);
+            p.cslot[k] = slot;
         }
         const uint32_t tpos = wave_append(&q.counters[kCTerm], live);
         if (live) q.term[tpos] = slot;
@@ -699,17 +698,17 @@ __global__ __launch_bounds__(kBlock) void k_finalize(PathsDev p, QueuesDev q, fl
     }
 }
 
-// res[path] += beta * cache(record) for this iteration's cache queries
-__global__ __launch_bounds__(kBlock) void k_cache_apply(PathsDev p, QueuesDev q, float *res)
+// L += beta * cache(record) for this iteration's cache queries; the paths
+// ended at their cache vertex and are finalized right after
+__global__ __launch_bounds__(kBlock) void k_cache_apply(PathsDev p, QueuesDev q)
 {
     const uint32_t n = q.counters[kCCacheQ];
     for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
-        float *r = res + 3 * (size_t)p.cidx[k];
+        const uint32_t slot = p.cslot[k];
         const float *c = p.crgb + 3 * (size_t)k;
-        const float *b = p.cbeta + 3 * (size_t)k;
-        r[0] = r[0] + b[0] * c[0];
-        r[1] = r[1] + b[1] * c[1];
-        r[2] = r[2] + b[2] * c[2];
+        p.lx[slot] = p.lx[slot] + p.bx[slot] * c[0];
+        p.ly[slot] = p.ly[slot] + p.by[slot] * c[1];
+        p.lz[slot] = p.lz[slot] + p.bz[slot] * c[2];
     }
 }
 
@@ -820,13 +819,17 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
     const PathsDev &paths = c.paths;
     const ShadowDev &shadow = c.shadow;
     const QueuesDev &q = c.queues;
-    // cache vertices end rendering paths only; training paths trace to the end
-    const bool lookups = !B && !wp.train && wp.cache_depth != kNoCache && c.cache && c.cwork;
+    // Rendering paths end at their first cacheable vertex at the cache depth.
+    // Training paths record there, trace kTrainSuffix more bounces and end
+    // at the next cacheable vertex, whose cache answer completes their
+    // target (self-training on the cache's own prediction).
+    const bool lookups = !B && wp.cache_depth != kNoCache && c.cache && c.cwork;
     if (!wp.train && wp.cache_depth != kNoCache && !lookups) return hipErrorInvalidValue;
     if (B && wp.train) return hipErrorInvalidValue;
     if (lookups && c.cwork->rows < c.capacity) return hipErrorInvalidValue;
     if (iterations & 1u) return hipErrorInvalidValue;
-    const uint32_t ext_depth = lookups ? wp.cache_depth : kNoCache;
+    const uint32_t ext_depth = !lookups ? kNoCache : (wp.train ? wp.cache_depth + kTrainSuffix : wp.cache_depth);
+    const uint32_t need_record = wp.train ? 1u : 0u;
     uint32_t cur = 0;
     for (uint32_t it = 0; it < iterations; ++it) {
         const uint32_t nxt = cur ^ 1u;
@@ -834,7 +837,7 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
         if (B) {
             if (wide) hipLaunchKernelGGL((k_extend<true, true>), pgrid, blk, lds, s, sc, bt, paths, q, cur, cap);
             else hipLaunchKernelGGL((k_extend<true, false>), pgrid, blk, lds, s, sc, bt, paths, q, cur, cap);
-            hipLaunchKernelGGL(k_classify<true>, grid, blk, 0, s, sc, bt, paths, q, cur, ext_depth);
+            hipLaunchKernelGGL(k_classify<true>, grid, blk, 0, s, sc, bt, paths, q, cur, ext_depth, need_record);
             hipLaunchKernelGGL((k_shade<kQLambert, true>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQDielectric, true>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQGgx, true>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
@@ -843,7 +846,7 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
         } else {
             if (wide) hipLaunchKernelGGL((k_extend<false, true>), pgrid, blk, lds, s, sc, bt, paths, q, cur, cap);
             else hipLaunchKernelGGL((k_extend<false, false>), pgrid, blk, lds, s, sc, bt, paths, q, cur, cap);
-            hipLaunchKernelGGL(k_classify<false>, grid, blk, 0, s, sc, bt, paths, q, cur, ext_depth);
+            hipLaunchKernelGGL(k_classify<false>, grid, blk, 0, s, sc, bt, paths, q, cur, ext_depth, need_record);
             hipLaunchKernelGGL((k_shade<kQLambert, false>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQDielectric, false>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQGgx, false>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
@@ -856,13 +859,15 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
         else if (B) hipLaunchKernelGGL((k_connect<true, false>), pgrid, blk, lds, s, sc, bt, paths, shadow, q, cap);
         else if (wide) hipLaunchKernelGGL((k_connect<false, true>), pgrid, blk, lds, s, sc, bt, paths, shadow, q, cap);
         else hipLaunchKernelGGL((k_connect<false, false>), pgrid, blk, lds, s, sc, bt, paths, shadow, q, cap);
-        hipLaunchKernelGGL(k_finalize, grid, blk, 0, s, paths, q, c.res, wp.train);
         if (lookups) {
-            hipError_t e = cache_infer(*c.cache, *c.cwork, paths.cin, c.capacity, paths.crgb, s,
+            // cin rows are this iteration's queries (training records are
+            // compacted into cin only by k_finalize, after this)
+            hipError_t e = cache_infer(*c.cache, *c.cwork, paths.cqin, c.capacity, paths.crgb, s,
                                        &q.counters[kCCacheQ]);
             if (e != hipSuccess) return e;
-            hipLaunchKernelGGL(k_cache_apply, grid, blk, 0, s, paths, q, c.res);
+            hipLaunchKernelGGL(k_cache_apply, grid, blk, 0, s, paths, q);
         }
+        hipLaunchKernelGGL(k_finalize, grid, blk, 0, s, paths, q, c.res, wp.train);
         if (B) hipLaunchKernelGGL(k_regen<true>, grid, blk, 0, s, sc, bt, paths, q, wp, nxt, 0u, c.capacity);
         else hipLaunchKernelGGL(k_regen<false>, grid, blk, 0, s, sc, bt, paths, q, wp, nxt, 0u, c.capacity);
         cur = nxt;

@@ -21,6 +21,9 @@ enum : uint32_t {
 };
 
 constexpr uint32_t kNoCache = 0xffffffffu;
+// bounces a training path traces past its training record before it ends
+// in the cache
+constexpr uint32_t kTrainSuffix = 2;
 
 // Scene as the kernels see it (passed by value).
 struct SceneDev {
@@ -65,10 +68,11 @@ struct PathsDev {
     uint32_t *nee_mask;         // unoccluded light slots of the last NEE
     float *nee_c;               // P * nee_stride * 3: per-light f * Li * w
     uint32_t nee_stride;        // light slots per path (max over a batch's scenes)
-    // radiance cache: query records (P x 12) and results (P x 3), in queue order
-    float *cin, *crgb;
-    uint32_t *cidx;             // result slot (chunk path index) of cache query k
-    float *cbeta;               // P x 3: throughput on arrival at the cache vertex
+    // radiance cache: query records (P x 12) and answers (P x 3) of one
+    // iteration, in queue order, and the path slot of each query
+    float *cqin, *crgb;
+    uint32_t *cslot;
+    float *cin;                 // compacted training records (P x 12)
     // training paths: the record at the cache depth, L and beta on arrival
     float *tin;                 // P x 12
     float *tLb, *tB;            // P x 3 each

@@ -287,8 +287,8 @@ rt_status ensure_wave_buffers(WaveBuffers &w, uint32_t P, uint32_t NL, const cha
     add((void **)&p.nee_c, (size_t)P * NL * 12);
     add((void **)&p.cin, (size_t)P * 48);
     add((void **)&p.crgb, (size_t)P * 12);
-    add((void **)&p.cidx, (size_t)P * 4);
-    add((void **)&p.cbeta, (size_t)P * 12);
+    add((void **)&p.cslot, (size_t)P * 4);
+    add((void **)&p.cqin, (size_t)P * 48);
     add((void **)&p.tin, (size_t)P * 48);
     add((void **)&p.tLb, (size_t)P * 12);
     add((void **)&p.tB, (size_t)P * 12);
@@ -641,6 +641,7 @@ rt_status rt_cache_train_scene(rt_cache *c, rt_scene *s, const rt_render_opts *o
     RT_HIP(hipSetDevice(rt::ctx_device(s->ctx)));
     hipStream_t stream = stream_ ? (hipStream_t)stream_ : (hipStream_t)rt::ctx_stream_ptr(s->ctx);
     rt_status st = rt::ensure_buffers(s, n_paths, "rt_cache_train_scene");
+    if (st == RT_OK) st = rt::ensure_cache_work(s, s->wb.capacity);
     if (st != RT_OK) return st;
     RT_HIP(hipMemsetAsync(s->wb.queues.stats, 0, 16, stream));
     rt::WaveParams wp{};
@@ -661,6 +662,8 @@ rt_status rt_cache_train_scene(rt_cache *c, rt_scene *s, const rt_render_opts *o
     cl.shadow = s->wb.shadow;
     cl.queues = s->wb.queues;
     cl.capacity = n_paths;  // one slot per training path: no read-back needed
+    cl.cache = &rt::cache_device(c);  // training paths end in the cache after their suffix
+    cl.cwork = &s->cwork;
     // no host read-back: every training path starts at once and ends within max_depth iterations
     RT_HIP(rt::render_chunk(cl, film, wp, nullptr, stream));
     // the sample count stays on the device: no host round trip unless the

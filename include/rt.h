@@ -362,11 +362,13 @@ rt_status rt_cache_set_params(rt_cache *cache, const float *mlp, const float *gr
  *
  * rt_cache_train_scene: one online training step.  Traces n_paths training
  * paths (pseudo-random pixels; sample indices disjoint from rendering ones;
- * opts->spp_offset selects the frame's set, opts->spp is ignored) to the end
- * with the plain estimator, takes for each path that has a cache vertex the
- * radiance it gathered from that vertex on divided by its throughput on
- * arrival, and runs one Adam step on up to the cache's batch size of these
- * (record, radiance) pairs.  The pair count stays on the device, so the call
+ * opts->spp_offset selects the frame's set, opts->spp is ignored) with the
+ * plain estimator.  A path records its first cache vertex, traces two more
+ * bounces and ends at its next cache vertex, where the cache's own answer
+ * stands for the rest of the path (self-training, as in neural radiance
+ * caching).  The radiance gathered from the recorded vertex on, divided by
+ * the throughput on arrival there, is the target, and one Adam step runs on
+ * up to the cache's batch size of these (record, radiance) pairs.  The pair count stays on the device, so the call
  * only enqueues work; it blocks only when loss (mean relative L2 loss before
  * the step) or n_samples (pairs used; 0 = the step saw no pairs) is
  * requested.  Both may be NULL. */

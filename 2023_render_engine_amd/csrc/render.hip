@@ -147,10 +147,11 @@ __device__ __forceinline__ void init_path(const SceneDev &sc0, const BatchDev &b
     p.tvalid[slot] = 0;
 }
 
-// Path regeneration: every free slot (all slots at a chunk's start, else
-// the slots whose paths ended this iteration) claims the chunk's next path
-// index (one atomic per wavefront) and joins the next live list.  Claims
-// stop once the chunk is exhausted, which bounds the claim counter.
+// Path regeneration: the free slots (all slots at a chunk's start, else the
+// slots whose paths ended this iteration, compact in the terminated list)
+// take the chunk's next paths in order: free slot i gets chunk path
+// next + i and live-list position n_live + i, for i < n_new = min(free,
+// count - next).  No atomics: k_regen_commit advances both counters after.
 template <bool B>
 __global__ __launch_bounds__(kBlock) void k_regen(SceneDev sc0, BatchDev bt, PathsDev p, QueuesDev q, WaveParams wp,
                                                   uint32_t nxt, uint32_t init, uint32_t capacity)
@@ -158,22 +159,27 @@ __global__ __launch_bounds__(kBlock) void k_regen(SceneDev sc0, BatchDev bt, Pat
     const uint64_t first = wp.chunk_dev ? wp.chunk_dev[0] : wp.first_path;
     const uint32_t count = wp.chunk_dev ? (uint32_t)wp.chunk_dev[1] : wp.count;
     const uint32_t nfree = init ? capacity : q.counters[kCTerm];
-    const uint32_t stride = gridDim.x * kBlock;
-    const uint32_t lane = __lane_id();
-    for (uint32_t i0 = blockIdx.x * kBlock; i0 < nfree; i0 += stride) {
-        uint32_t claimed = 0;
-        if (lane == 0) claimed = atomicAdd(&q.counters[kCNext], 0u);
-        claimed = __shfl(claimed, 0);
-        if (claimed >= count) break;  // wavefront-uniform
-        const uint32_t i = i0 + threadIdx.x;
-        const bool live = i < nfree;
-        const uint32_t slot = live ? (init ? i : q.term[i]) : 0;
-        const uint32_t k = wave_append(&q.counters[kCNext], live);
-        const bool gen = live && k < count;
-        if (gen) init_path<B>(sc0, bt, p, wp, slot, first + k, k);
-        const uint32_t pos = wave_append(&q.counters[kCActive0 + nxt], gen);
-        if (gen) q.active[nxt][pos] = slot;
+    const uint32_t next = q.counters[kCNext];
+    const uint32_t n_live = q.counters[kCActive0 + nxt];
+    const uint32_t left = next < count ? count - next : 0u;
+    const uint32_t n_new = nfree < left ? nfree : left;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n_new; i += gridDim.x * kBlock) {
+        const uint32_t slot = init ? i : q.term[i];
+        init_path<B>(sc0, bt, p, wp, slot, first + next + i, next + i);
+        q.active[nxt][n_live + i] = slot;
     }
+}
+
+__global__ void k_regen_commit(QueuesDev q, WaveParams wp, uint32_t nxt, uint32_t init, uint32_t capacity)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t count = wp.chunk_dev ? (uint32_t)wp.chunk_dev[1] : wp.count;
+    const uint32_t nfree = init ? capacity : q.counters[kCTerm];
+    const uint32_t next = q.counters[kCNext];
+    const uint32_t left = next < count ? count - next : 0u;
+    const uint32_t n_new = nfree < left ? nfree : left;
+    q.counters[kCNext] = next + n_new;
+    q.counters[kCActive0 + nxt] += n_new;
 }
 
 // ---------------------------------------------------------------------
@@ -801,6 +807,7 @@ hipError_t chunk_begin(const ChunkLaunch &c, const WaveParams &wp, hipStream_t s
     hipLaunchKernelGGL(k_reset, dim3(1), blk, 0, s, c.queues.counters, 0u, 1u);
     if (c.batch) hipLaunchKernelGGL(k_regen<true>, grid, blk, 0, s, sc, bt, c.paths, c.queues, wp, 0u, 1u, c.capacity);
     else hipLaunchKernelGGL(k_regen<false>, grid, blk, 0, s, sc, bt, c.paths, c.queues, wp, 0u, 1u, c.capacity);
+    hipLaunchKernelGGL(k_regen_commit, dim3(1), dim3(64), 0, s, c.queues, wp, 0u, 1u, c.capacity);
     return hipGetLastError();
 }
 
@@ -870,6 +877,7 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
         hipLaunchKernelGGL(k_finalize, grid, blk, 0, s, paths, q, c.res, wp.train);
         if (B) hipLaunchKernelGGL(k_regen<true>, grid, blk, 0, s, sc, bt, paths, q, wp, nxt, 0u, c.capacity);
         else hipLaunchKernelGGL(k_regen<false>, grid, blk, 0, s, sc, bt, paths, q, wp, nxt, 0u, c.capacity);
+        hipLaunchKernelGGL(k_regen_commit, dim3(1), dim3(64), 0, s, q, wp, nxt, 0u, c.capacity);
         cur = nxt;
     }
     return hipGetLastError();

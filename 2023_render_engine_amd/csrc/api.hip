@@ -45,12 +45,13 @@ struct rt_bvh {
     rt::HostBvh host;  // kept for export and info; tris carry transparency
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
-    // ray-fetch counters of the persistent kernels: a ring, so launches that
-    // are in flight together on different streams do not share one
+    // ray-claim counters of the persistent kernels (rt::kClaimWords words per
+    // launch): a ring, so launches that are in flight together on different
+    // streams do not share one
     uint32_t *ray_counters = nullptr;
     uint32_t seq = 0;
-    uint32_t *next_counter() { return ray_counters + (seq++ % kRayCounterRing); }
-    static constexpr uint32_t kRayCounterRing = 256;
+    uint32_t *next_counter() { return ray_counters + (size_t)(seq++ % kRayCounterRing) * rt::kClaimWords; }
+    static constexpr uint32_t kRayCounterRing = 64;
     uint32_t width = 4;
 };
 
@@ -185,7 +186,7 @@ static rt_status upload_bvh(rt_bvh *b)
     const size_t tb = b->host.tris.size() * sizeof(rt::Tri3);
     RT_HIP(hipMalloc(&b->dev.nodes, nb));
     RT_HIP(hipMalloc(&b->dev.tris, tb));
-    RT_HIP(hipMalloc(&b->ray_counters, rt_bvh::kRayCounterRing * sizeof(uint32_t)));
+    RT_HIP(hipMalloc(&b->ray_counters, (size_t)rt_bvh::kRayCounterRing * rt::kClaimWords * sizeof(uint32_t)));
     RT_HIP(hipMemcpyAsync(b->dev.nodes, src, nb, hipMemcpyHostToDevice, b->ctx->stream));
     RT_HIP(hipMemcpyAsync(b->dev.tris, b->host.tris.data(), tb, hipMemcpyHostToDevice, b->ctx->stream));
     RT_HIP(hipStreamSynchronize(b->ctx->stream));
@@ -347,7 +348,7 @@ static rt_status trace_common(rt_bvh *b, const rt_rays *rays, uint32_t n, int32_
     if (!(flags & RT_FLAG_HOST_PTRS)) {
         rt::RaysDev rd{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmax};
         uint32_t *rc = b->next_counter();
-        RT_HIP(hipMemsetAsync(rc, 0, sizeof(uint32_t), stream));
+        RT_HIP(rt::clear_claims(rc, stream));
         RT_HIP(hipEventRecord(b->ev0, stream));
         RT_HIP(rt::launch_trace(b->dev, rd, n, prim, t, occ, any, stream, nullptr, rc));
         RT_HIP(hipEventRecord(b->ev1, stream));
@@ -375,7 +376,7 @@ static rt_status trace_common(rt_bvh *b, const rt_rays *rays, uint32_t n, int32_
     float *dt = (!any && t) ? (float *)(out + (size_t)n * sizeof(int32_t)) : nullptr;
     uint8_t *docc = any ? (uint8_t *)out : nullptr;
     uint32_t *rc = b->next_counter();
-    hipError_t e = hipMemsetAsync(rc, 0, sizeof(uint32_t), stream);
+    hipError_t e = rt::clear_claims(rc, stream);
     (void)hipEventRecord(b->ev0, stream);
     if (e == hipSuccess) e = rt::launch_trace(b->dev, rd, n, dprim, dt, docc, any, stream, nullptr, rc);
     (void)hipEventRecord(b->ev1, stream);
@@ -405,14 +406,15 @@ rt_status rt_trace_stats(rt_bvh *b, const rt_rays *rays, uint32_t n, int any, ui
     hipStream_t stream = b->ctx->stream;
     char *buf = nullptr;
     const size_t outb = any ? (size_t)n : (size_t)n * 8;
-    RT_HIP(hipMalloc(&buf, 32 + outb));
-    hipError_t e = hipMemsetAsync(buf, 0, 32, stream);
+    const size_t claims = 256, head = claims + rt::kClaimWords * sizeof(uint32_t);
+    RT_HIP(hipMalloc(&buf, head + outb));
+    hipError_t e = hipMemsetAsync(buf, 0, head, stream);
     rt::RaysDev rd{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmax};
-    char *out = buf + 32;
+    char *out = buf + head;
     if (e == hipSuccess)
         e = rt::launch_trace(b->dev, rd, n, any ? nullptr : (int32_t *)out, any ? nullptr : (float *)(out + 4 * (size_t)n),
                              any ? (uint8_t *)out : nullptr, any != 0, stream, (unsigned long long *)buf,
-                             (uint32_t *)(buf + 16));
+                             (uint32_t *)(buf + claims));
     unsigned long long c[2] = {0, 0};
     if (e == hipSuccess) e = hipMemcpyAsync(c, buf, 16, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);

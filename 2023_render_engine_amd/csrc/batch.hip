@@ -201,11 +201,11 @@ hipError_t run_frame(rt_batch *b, const rt_batch_opts &o, uint32_t P, uint32_t g
         for (uint64_t it = 0; it < max_iters && !done; it += kBlockIters) {
             e = graph ? hipGraphLaunch(b->exec, s) : rt::chunk_iterate(cl, wp, kBlockIters, s);
             if (e == hipSuccess)
-                e = hipMemcpyAsync(hc, b->wb.queues.counters, rt::kNumCounters * sizeof(uint32_t),
+                e = hipMemcpyAsync(hc, b->wb.queues.counters, rt::kCounterWords * sizeof(uint32_t),
                                    hipMemcpyDeviceToHost, s);
             if (e == hipSuccess) e = hipStreamSynchronize(s);
             if (e != hipSuccess) return e;
-            done = hc[rt::kCNext] >= wp.count && hc[rt::kCActive0] == 0;
+            done = hc[rt::kCNext * rt::kCtrStride] >= wp.count && hc[rt::kCLive * rt::kCtrStride] == 0;
         }
         if (!done) return hipErrorUnknown;  // cannot happen: the bound above holds
         e = rt::chunk_film(cl, films[0], wp, s);
@@ -229,7 +229,7 @@ uint32_t wave_size(const rt_batch *b, const rt_batch_opts &o, uint64_t total)
 {
     uint32_t P = o.path_capacity ? o.path_capacity : (b->tuned_capacity ? b->tuned_capacity : (1u << 21));
     if ((uint64_t)P > total) P = (uint32_t)std::max<uint64_t>(1, total);
-    return P;
+    return rt::stripe_capacity(P);
 }
 
 }  // namespace
@@ -381,7 +381,7 @@ rt_status rt_batch_autotune(rt_batch *b, const rt_batch_opts *opts, rt_tune_entr
     const uint64_t total = b->paths_per_sample;
     std::vector<uint32_t> caps;
     for (uint32_t P = 1u << 19; P <= (1u << 23); P <<= 1) {
-        const uint32_t c = (uint32_t)std::min<uint64_t>(P, total);
+        const uint32_t c = rt::stripe_capacity((uint32_t)std::min<uint64_t>(P, total));
         if (caps.empty() || caps.back() != c) caps.push_back(c);
     }
     const uint32_t grids[4] = {1024, 2048, 4096, 8192};

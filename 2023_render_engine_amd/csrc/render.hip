@@ -2,7 +2,7 @@
 //
 // A wave of up to P paths lives in structure-of-arrays state in HBM.  Per
 // bounce:
-//   extend   closest hit (BVH2 + analytic spheres) for every active path,
+//   extend   closest hit (BVH + analytic spheres) for every active path,
 //            then material classification into per-class queues with
 //            wavefront ballots (one atomic per 64 lanes per queue);
 //   shade<M> one kernel per material class (Lambertian, dielectric, GGX,
@@ -23,8 +23,11 @@
 // contributions are summed per path in light order, so the result does not
 // depend on the order in which shadow rays complete.
 //
-// Every queue kernel is a grid-stride loop over a count read from device
+// Every queue kernel is a grid-stride loop over counts read from device
 // memory, so a wave is a fixed launch sequence with no host round trip.
+// Path slots, queues and their counters are striped (rt_render.h): a
+// workgroup serves stripe blockIdx % kStripes, so the appends of the whole
+// chip spread over 64 counters instead of queueing on one address.
 #include "rt_device.h"
 #include "rt_render.h"
 #include "rt_shade.h"
@@ -54,6 +57,41 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v)
     return v;
 }
 
+// Exclusive prefix sum over a wavefront's lanes; `total` is the full sum.
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t &total)
+{
+    const uint32_t lane = __lane_id();
+    uint32_t incl = v;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t t = (uint32_t)__shfl_up((int)incl, off);
+        if (lane >= off) incl += t;
+    }
+    total = (uint32_t)__shfl((int)incl, 63);
+    return incl - v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off);
+    return v;
+}
+
+// The stripe a queue kernel's workgroup serves, and its share of the
+// stripe's items: [j0 + thread, ...) with step `step` (every stripe has at
+// least one workgroup when the grid has at least kStripes).
+struct StripeSpan {
+    uint32_t s, j0, step;
+};
+
+__device__ __forceinline__ StripeSpan stripe_span()
+{
+    const uint32_t s = blockIdx.x % kStripes;
+    const uint32_t nb = (gridDim.x - s + kStripes - 1) / kStripes;
+    return {s, (blockIdx.x / kStripes) * kBlock, nb * kBlock};
+}
+
 // The scene of path `slot`: the kernel's scene, or the batch table entry.
 template <bool B>
 __device__ __forceinline__ const SceneDev &scene_of(const SceneDev &sc0, const BatchDev &bt, const PathsDev &p,
@@ -80,17 +118,19 @@ __device__ __forceinline__ Pcg32 load_rng(const PathsDev &p, uint32_t slot, uint
 
 // ---------------------------------------------------------------------
 // Counter resets.  Per iteration: the next live list, the material queues,
-// the shadow queue, the terminated list, the fetch counters of the
+// the shadow list, the terminated list, the claim counters of the
 // persistent kernels and the cache queries; at a chunk's start: all.
-__global__ __launch_bounds__(kBlock) void k_reset(uint32_t *counters, uint32_t cur, uint32_t chunk_start)
+// Workgroup L < kNumLists clears list L's stripe counters (one lane per
+// stripe), the last workgroup the scalar counters.
+__global__ __launch_bounds__(kStripes) void k_reset(QueuesDev q, uint32_t cur, uint32_t chunk_start)
 {
-    const uint32_t t = threadIdx.x;
-    if (t >= kNumCounters) return;
-    if (chunk_start) {
-        counters[t] = 0;
-    } else if (t == kCActive0 + (cur ^ 1u) || (t >= kCQueue && t <= kCTerm) || t == kCFetchExt ||
-               t == kCFetchCon || t == kCCacheQ) {
-        counters[t] = 0;
+    const uint32_t L = blockIdx.x, t = threadIdx.x;
+    if (L < kNumLists) {
+        const bool clear = chunk_start || L == kLActive0 + (cur ^ 1u) || (L >= kLQueue && L <= kLTerm) ||
+                           L == kLClaimExt || L == kLClaimCon;
+        if (clear) *lcount(q, L, t) = 0;
+    } else if (t < kNumCounters && (chunk_start || t == kCCacheQ)) {
+        *scount(q, t) = 0;
     }
 }
 
@@ -147,39 +187,77 @@ __device__ __forceinline__ void init_path(const SceneDev &sc0, const BatchDev &b
     p.tvalid[slot] = 0;
 }
 
-// Path regeneration: the free slots (all slots at a chunk's start, else the
-// slots whose paths ended this iteration, compact in the terminated list)
-// take the chunk's next paths in order: free slot i gets chunk path
-// next + i and live-list position n_live + i, for i < n_new = min(free,
-// count - next).  No atomics: k_regen_commit advances both counters after.
+// Path regeneration, per stripe.  The free slots of stripe s (all of them at
+// a chunk's start, else the slots whose paths ended this iteration, compact
+// in the stripe's terminated list) take new paths and are appended to the
+// stripe's next live list after its surviving paths.  At a chunk's start
+// stripe s takes chunk paths s, s + 64, s + 128, ... (balanced stripes);
+// later, the stripes' free slots in stripe order take the chunk's next
+// paths in order.  No atomics: k_regen_commit advances the counters after.
+// Called by whole wavefronts (lane j reads stripe j's free count).
+struct RegenPlan {
+    uint32_t next;   // chunk paths handed out before this regeneration
+    uint32_t n_new;  // paths handed out now (all stripes)
+    uint32_t take;   // paths stripe s takes
+    uint32_t pre;    // chunk index of its first path - next (later iterations)
+};
+
+__device__ __forceinline__ RegenPlan regen_plan(const QueuesDev &q, const WaveParams &wp, uint32_t init,
+                                                uint32_t capacity, uint32_t s)
+{
+    RegenPlan r;
+    const uint32_t count = wp.chunk_dev ? (uint32_t)wp.chunk_dev[1] : wp.count;
+    r.next = *scount(q, kCNext);
+    const uint32_t left = r.next < count ? count - r.next : 0u;
+    if (init) {
+        r.n_new = capacity < left ? capacity : left;
+        r.take = r.n_new > s ? (r.n_new - s + kStripes - 1) / kStripes : 0u;
+        r.pre = 0;
+    } else {
+        const uint32_t nfree = *lcount(q, kLTerm, __lane_id());
+        uint32_t total = 0;
+        const uint32_t pre = wave_excl_scan(nfree, total);
+        r.n_new = total < left ? total : left;
+        r.pre = (uint32_t)__shfl((int)pre, (int)s);
+        const uint32_t f = (uint32_t)__shfl((int)nfree, (int)s);
+        r.take = left > r.pre ? (left - r.pre < f ? left - r.pre : f) : 0u;
+    }
+    return r;
+}
+
 template <bool B>
 __global__ __launch_bounds__(kBlock) void k_regen(SceneDev sc0, BatchDev bt, PathsDev p, QueuesDev q, WaveParams wp,
                                                   uint32_t nxt, uint32_t init, uint32_t capacity)
 {
     const uint64_t first = wp.chunk_dev ? wp.chunk_dev[0] : wp.first_path;
-    const uint32_t count = wp.chunk_dev ? (uint32_t)wp.chunk_dev[1] : wp.count;
-    const uint32_t nfree = init ? capacity : q.counters[kCTerm];
-    const uint32_t next = q.counters[kCNext];
-    const uint32_t n_live = q.counters[kCActive0 + nxt];
-    const uint32_t left = next < count ? count - next : 0u;
-    const uint32_t n_new = nfree < left ? nfree : left;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n_new; i += gridDim.x * kBlock) {
-        const uint32_t slot = init ? i : q.term[i];
-        init_path<B>(sc0, bt, p, wp, slot, first + next + i, next + i);
-        q.active[nxt][n_live + i] = slot;
+    const StripeSpan sp = stripe_span();
+    const uint32_t s = sp.s, cs = q.cs;
+    const RegenPlan r = regen_plan(q, wp, init, capacity, s);
+    const uint32_t n_live = *lcount(q, kLActive0 + nxt, s);
+    for (uint32_t j = sp.j0 + threadIdx.x; j < r.take; j += sp.step) {
+        const uint32_t slot = init ? s * cs + j : q.term[s * cs + j];
+        const uint32_t k = r.next + (init ? j * kStripes + s : r.pre + j);
+        init_path<B>(sc0, bt, p, wp, slot, first + k, k);
+        q.active[nxt][s * cs + n_live + j] = slot;
     }
 }
 
-__global__ void k_regen_commit(QueuesDev q, WaveParams wp, uint32_t nxt, uint32_t init, uint32_t capacity)
+// One wavefront, lane s = stripe s: the stripes' live counts grow by what
+// they took; the next unclaimed path and the total live count (for the
+// host's end-of-chunk test) are updated.
+__global__ __launch_bounds__(kStripes) void k_regen_commit(QueuesDev q, WaveParams wp, uint32_t nxt, uint32_t init,
+                                                           uint32_t capacity)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const uint32_t count = wp.chunk_dev ? (uint32_t)wp.chunk_dev[1] : wp.count;
-    const uint32_t nfree = init ? capacity : q.counters[kCTerm];
-    const uint32_t next = q.counters[kCNext];
-    const uint32_t left = next < count ? count - next : 0u;
-    const uint32_t n_new = nfree < left ? nfree : left;
-    q.counters[kCNext] = next + n_new;
-    q.counters[kCActive0 + nxt] += n_new;
+    const uint32_t s = threadIdx.x;
+    const RegenPlan r = regen_plan(q, wp, init, capacity, s);
+    uint32_t *live = lcount(q, kLActive0 + nxt, s);
+    const uint32_t n = *live + r.take;
+    const uint32_t total = wave_sum(n);
+    *live = n;
+    if (s == 0) {
+        *scount(q, kCNext) = r.next + r.n_new;
+        *scount(q, kCLive) = total;
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -206,28 +284,33 @@ __device__ __forceinline__ uint32_t classify(const SceneDev &sc, int32_t hid, bo
     return kQGgx;
 }
 
-// Persistent closest hit against the BVH: lanes fetch active paths from a
-// global counter and walk them one node visit per step (rt_device.h), so a
-// few long rays do not hold the rest of the wavefront idle.  The analytic
-// shapes and the material classification follow in k_classify, which keeps
-// this loop's register footprint at the traversal's.
+// Persistent closest hit against the BVH: lanes claim active paths (their
+// workgroup's stripe first, then any stripe with paths left) and walk them
+// one node visit per step (rt_device.h), so a few long rays do not hold the
+// rest of the wavefront idle.  The analytic shapes and the material
+// classification follow in k_classify, which keeps this loop's register
+// footprint at the traversal's.
 template <bool B, bool WIDE>
 __global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc0, BatchDev bt, PathsDev p, QueuesDev q, uint32_t cur,
                                                    int cap)
 {
     extern __shared__ int32_t s_stack[];
     int32_t *stack = s_stack + threadIdx.x;
-    const uint32_t n = q.counters[kCActive0 + cur];
-    uint32_t *fetch = &q.counters[kCFetchExt];
+    const uint32_t *active = q.active[cur];
+    const uint32_t cs = q.cs;
+    auto count = [&](uint32_t j) { return *lcount(q, kLActive0 + cur, j); };
+    StripeCursor c;
+    cursor_init(c, blockIdx.x % kStripes, count);
     TravState s;
-    bool has = false, exhausted = false;
+    bool has = false;
     uint32_t slot = 0, dummy0 = 0, dummy1 = 0;
     const float4 *nodes = (const float4 *)sc0.bvh.nodes;
     const float4 *tris = (const float4 *)sc0.bvh.tris;
     for (;;) {
-        const uint32_t r = wave_refill(fetch, n, has, exhausted);
+        uint32_t stripe = 0;
+        const uint32_t r = wave_claim(c, lcount(q, kLClaimExt, 0), count, has, stripe);
         if (r != 0xffffffffu) {
-            slot = q.active[cur][r];
+            slot = active[stripe * cs + r];
             if constexpr (B) {
                 const SceneDev &sc = bt.scenes[p.sid[slot]];
                 nodes = (const float4 *)sc.bvh.nodes;
@@ -237,7 +320,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc0, BatchDev bt, Pa
             has = true;
         }
         if (__ballot(has) == 0) {
-            if (exhausted) break;
+            if (cursor_finished(c)) break;
             continue;
         }
         if (has && trav_step_any<WIDE, false, false>(s, nodes, tris, stack, cap, dummy0, dummy1)) {
@@ -249,33 +332,35 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc0, BatchDev bt, Pa
 }
 
 // Analytic shapes after the BVH triangle, strict < (Scene.cu:221-227,
-// Sphere.cu:321), then material classification into the per-class queues
-// with wavefront ballots (one atomic per 64 lanes per queue).
+// Sphere.cu:321), then material classification into the stripe's
+// per-class queues with wavefront ballots (one atomic per 64 lanes per
+// queue, on the stripe's counter).
 template <bool B>
 __global__ __launch_bounds__(kBlock) void k_classify(SceneDev sc0, BatchDev bt, PathsDev p, QueuesDev q, uint32_t cur,
                                                      uint32_t cache_depth, uint32_t need_record)
 {
-    const uint32_t n = q.counters[kCActive0 + cur];
+    const StripeSpan sp = stripe_span();
+    const uint32_t cs = q.cs, base = sp.s * cs;
+    const uint32_t n = *lcount(q, kLActive0 + cur, sp.s);
     // every lane of a wavefront runs the same number of iterations so the
     // ballots below see the whole wavefront
-    const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t i0 = blockIdx.x * kBlock; i0 < n; i0 += stride) {
-        const uint32_t i = i0 + threadIdx.x;
+    for (uint32_t j0 = sp.j0; j0 < n; j0 += sp.step) {
+        const uint32_t j = j0 + threadIdx.x;
         uint32_t slot = 0, cls = 0xffu;
-        if (i < n) {
-            slot = q.active[cur][i];
+        if (j < n) {
+            slot = q.active[cur][base + j];
             const SceneDev &sc = scene_of<B>(sc0, bt, p, slot);
             int32_t hid = p.hid[slot];
             float th = p.ht[slot];
             if (sc.nsph) {
                 const V3 o = mk(p.ox[slot], p.oy[slot], p.oz[slot]), d = mk(p.dx[slot], p.dy[slot], p.dz[slot]);
                 bool changed = false;
-                for (uint32_t j = 0; j < sc.nsph; ++j) {
+                for (uint32_t k = 0; k < sc.nsph; ++k) {
                     V3 so, sd;
-                    const float ts = sphere_core(sc.sph[j], o, d, INFINITY, so, sd);
+                    const float ts = sphere_core(sc.sph[k], o, d, INFINITY, so, sd);
                     if (ts >= 0 && ts < th) {
                         th = ts;
-                        hid = -(int32_t)j - 2;
+                        hid = -(int32_t)k - 2;
                         changed = true;
                     }
                 }
@@ -291,8 +376,8 @@ __global__ __launch_bounds__(kBlock) void k_classify(SceneDev sc0, BatchDev bt, 
         }
 #pragma unroll
         for (uint32_t k = 0; k < kNumQ; ++k) {
-            const uint32_t pos = wave_append(&q.counters[kCQueue + k], cls == k);
-            if (cls == k) q.q[k][pos] = slot;
+            const uint32_t pos = wave_append(lcount(q, kLQueue + k, sp.s), cls == k);
+            if (cls == k) q.q[k][base + pos] = slot;
         }
     }
 }
@@ -408,15 +493,17 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
                                                   WaveParams wp, uint32_t cur)
 {
     constexpr bool kScatter = CLS == kQLambert || CLS == kQGgx;
-    const uint32_t n = q.counters[kCQueue + CLS];
-    const uint32_t *queue = q.q[CLS];
+    const StripeSpan sp = stripe_span();
+    const uint32_t cs = q.cs, base = sp.s * cs;
+    const uint32_t n = *lcount(q, kLQueue + CLS, sp.s);
+    const uint32_t *queue = q.q[CLS] + base;
+    const size_t sbase = (size_t)base * p.nee_stride;  // the stripe's shadow rays
     const uint32_t nxt = cur ^ 1u;
-    const uint32_t stride = gridDim.x * kBlock;
     // Every lane of a wavefront runs the same iterations (and the same light
     // loops: kernel arguments, or in a batch the wavefront's maximum) so the
     // ballots in wave_append see whole wavefronts.
-    for (uint32_t i0 = blockIdx.x * kBlock; i0 < n; i0 += stride) {
-        const uint32_t i = i0 + threadIdx.x;
+    for (uint32_t j0 = sp.j0; j0 < n; j0 += sp.step) {
+        const uint32_t i = j0 + threadIdx.x;
         const bool live = i < n;
         const uint32_t slot = live ? queue[i] : 0;
         const SceneDev &sc = scene_of<B>(sc0, bt, p, slot);
@@ -485,7 +572,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
                         }
                     }
                 }
-                const uint32_t pos = wave_append(&q.counters[kCShadow], e);
+                const size_t pos = sbase + wave_append(lcount(q, kLShadow, sp.s), e);
                 if (e) {
                     emit |= 1u << l;
                     sh.ox[pos] = o2.x; sh.oy[pos] = o2.y; sh.oz[pos] = o2.z;
@@ -520,7 +607,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
                         }
                     }
                 }
-                const uint32_t pos = wave_append(&q.counters[kCShadow], e);
+                const size_t pos = sbase + wave_append(lcount(q, kLShadow, sp.s), e);
                 if (e) {
                     emit |= 1u << l;
                     sh.ox[pos] = o2.x; sh.oy[pos] = o2.y; sh.oz[pos] = o2.z;
@@ -580,38 +667,45 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
             p.lx[slot] = L.x; p.ly[slot] = L.y; p.lz[slot] = L.z;
             p.rng[slot] = g.state;
         }
-        const uint32_t ppos = wave_append(&q.counters[kCActive0 + nxt], live && alive);
-        if (live && alive) q.active[nxt][ppos] = slot;
-        const uint32_t tpos = wave_append(&q.counters[kCTerm], live && !alive);
-        if (live && !alive) q.term[tpos] = slot;
+        const uint32_t ppos = wave_append(lcount(q, kLActive0 + nxt, sp.s), live && alive);
+        if (live && alive) q.active[nxt][base + ppos] = slot;
+        const uint32_t tpos = wave_append(lcount(q, kLTerm, sp.s), live && !alive);
+        if (live && !alive) q.term[base + tpos] = slot;
     }
 }
 
 // Cache vertices: the path ends here.  Its previous next-event estimate is
-// resolved and its query record is appended to the wave's query list; the
-// cache's answer times the throughput on arrival is added after the bounce
-// loop (k_cache_apply).
+// resolved and its query record is written to the iteration's query rows;
+// the cache's answer times the throughput on arrival is added after the
+// bounce loop (k_cache_apply).  The rows are dense in stripe order: stripe
+// s's queries start at the sum of the lower stripes' cache-queue counts,
+// so no counter is shared.
 __global__ __launch_bounds__(kBlock) void k_cache_vertex(SceneDev sc, PathsDev p, QueuesDev q)
 {
-    const uint32_t n = q.counters[kCQueue + kQCache];
-    const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t i0 = blockIdx.x * kBlock; i0 < n; i0 += stride) {
-        const uint32_t i = i0 + threadIdx.x;
+    const StripeSpan sp = stripe_span();
+    const uint32_t base = sp.s * q.cs;
+    const uint32_t n = *lcount(q, kLQueue + kQCache, sp.s);
+    uint32_t total = 0;
+    const uint32_t pre = wave_excl_scan(*lcount(q, kLQueue + kQCache, __lane_id()), total);
+    const uint32_t row0 = (uint32_t)__shfl((int)pre, (int)sp.s);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *scount(q, kCCacheQ) = total;
+    for (uint32_t j0 = sp.j0; j0 < n; j0 += sp.step) {
+        const uint32_t i = j0 + threadIdx.x;
         const bool live = i < n;
-        const uint32_t slot = live ? q.q[kQCache][i] : 0;
+        const uint32_t slot = live ? q.q[kQCache][base + i] : 0;
         Surface s{};
         V3 d = mk(0, 0, 1);
         if (live) {
             resolve_nee(p, slot);
             surface_of(sc, p, slot, s, d);
         }
-        const uint32_t k = wave_append(&q.counters[kCCacheQ], live);
+        const uint32_t k = row0 + i;
         if (live) {
             write_record(p.cqin + (size_t)k * 12, s, d, sc.mat[s.mat]);
             p.cslot[k] = slot;
         }
-        const uint32_t tpos = wave_append(&q.counters[kCTerm], live);
-        if (live) q.term[tpos] = slot;
+        const uint32_t tpos = wave_append(lcount(q, kLTerm, sp.s), live);
+        if (live) q.term[base + tpos] = slot;
     }
 }
 
@@ -627,27 +721,31 @@ __global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc0, BatchDev bt, P
 {
     extern __shared__ int32_t s_stack[];
     int32_t *stack = s_stack + threadIdx.x;
-    const uint32_t n = q.counters[kCShadow];
-    uint32_t *fetch = &q.counters[kCFetchCon];
+    const size_t sstride = (size_t)q.cs * p.nee_stride;  // shadow rays per stripe
+    auto count = [&](uint32_t j) { return *lcount(q, kLShadow, j); };
+    StripeCursor c;
+    cursor_init(c, blockIdx.x % kStripes, count);
     TravState s;
-    bool has = false, exhausted = false;
+    bool has = false;
     uint32_t sl = 0, dummy0 = 0, dummy1 = 0;
     const float4 *nodes = (const float4 *)sc0.bvh.nodes;
     const float4 *tris = (const float4 *)sc0.bvh.tris;
     for (;;) {
-        const uint32_t r = wave_refill(fetch, n, has, exhausted);
+        uint32_t stripe = 0;
+        const uint32_t r = wave_claim(c, lcount(q, kLClaimCon, 0), count, has, stripe);
         if (r != 0xffffffffu) {
-            sl = sh.slot[r];
+            const size_t e = stripe * sstride + r;
+            sl = sh.slot[e];
             if constexpr (B) {
                 const SceneDev &sc = bt.scenes[p.sid[sl >> 5]];
                 nodes = (const float4 *)sc.bvh.nodes;
                 tris = (const float4 *)sc.bvh.tris;
             }
-            trav_begin(s, sh.ox[r], sh.oy[r], sh.oz[r], sh.dx[r], sh.dy[r], sh.dz[r], sh.tmax[r], true);
+            trav_begin(s, sh.ox[e], sh.oy[e], sh.oz[e], sh.dx[e], sh.dy[e], sh.dz[e], sh.tmax[e], true);
             has = true;
         }
         if (__ballot(has) == 0) {
-            if (exhausted) break;
+            if (cursor_finished(c)) break;
             continue;
         }
         if (has && trav_step_any<WIDE, true, false>(s, nodes, tris, stack, cap, dummy0, dummy1)) {
@@ -667,12 +765,13 @@ __global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc0, BatchDev bt, P
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_finalize(PathsDev p, QueuesDev q, float *res, uint32_t train)
 {
-    const uint32_t n = q.counters[kCTerm];
-    const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t i0 = blockIdx.x * kBlock; i0 < n; i0 += stride) {
-        const uint32_t i = i0 + threadIdx.x;
+    const StripeSpan sp = stripe_span();
+    const uint32_t n = *lcount(q, kLTerm, sp.s);
+    const uint32_t *term = q.term + sp.s * q.cs;
+    for (uint32_t j0 = sp.j0; j0 < n; j0 += sp.step) {
+        const uint32_t i = j0 + threadIdx.x;
         const bool live = i < n;
-        const uint32_t slot = live ? q.term[i] : 0;
+        const uint32_t slot = live ? term[i] : 0;
         if (live) resolve_nee(p, slot);
         if (!train) {
             if (live) {
@@ -692,7 +791,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(PathsDev p, QueuesDev q, fl
                 ok = ok && isfinite(tg[c]);
             }
         }
-        const uint32_t k = wave_append(&q.counters[kCTrain], ok);
+        const uint32_t k = wave_append(scount(q, kCTrain), ok);
         if (ok) {
             const float4 *src = (const float4 *)(p.tin + (size_t)slot * 12);
             float4 *dst = (float4 *)(p.cin + (size_t)k * 12);
@@ -708,7 +807,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(PathsDev p, QueuesDev q, fl
 // ended at their cache vertex and are finalized right after
 __global__ __launch_bounds__(kBlock) void k_cache_apply(PathsDev p, QueuesDev q)
 {
-    const uint32_t n = q.counters[kCCacheQ];
+    const uint32_t n = *scount(q, kCCacheQ);
     for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
         const uint32_t slot = p.cslot[k];
         const float *c = p.crgb + 3 * (size_t)k;
@@ -750,11 +849,15 @@ __global__ __launch_bounds__(kBlock) void k_film(const float *res, FilmDev film,
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_count(QueuesDev q, uint32_t cur)
+// Ray statistics of an iteration (one wavefront, lane s = stripe s).
+__global__ __launch_bounds__(kStripes) void k_count(QueuesDev q, uint32_t cur)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        q.stats[0] += q.counters[kCActive0 + cur];
-        q.stats[1] += q.counters[kCShadow];
+    const uint32_t s = threadIdx.x;
+    const uint32_t ext = wave_sum(*lcount(q, kLActive0 + cur, s));
+    const uint32_t shd = wave_sum(*lcount(q, kLShadow, s));
+    if (s == 0) {
+        q.stats[0] += ext;
+        q.stats[1] += shd;
     }
 }
 
@@ -798,16 +901,35 @@ static dim3 persistent_grid(size_t lds)
     return dim3((unsigned)(cus * per_cu));
 }
 
+// Workgroups of the queue kernels: every stripe needs at least one.
+static dim3 queue_grid(const WaveParams &wp)
+{
+    const uint32_t g = wp.grid ? wp.grid : (uint32_t)kGrid;
+    return dim3(g < kStripes ? kStripes : g);
+}
+
+// The chunk's queues with the stripe size of its capacity.
+static hipError_t chunk_queues(const ChunkLaunch &c, QueuesDev &q)
+{
+    if (c.capacity == 0 || c.capacity % kStripes) return hipErrorInvalidValue;
+    q = c.queues;
+    q.cs = c.capacity / kStripes;
+    return hipSuccess;
+}
+
 hipError_t chunk_begin(const ChunkLaunch &c, const WaveParams &wp, hipStream_t s)
 {
+    QueuesDev q;
+    hipError_t e = chunk_queues(c, q);
+    if (e != hipSuccess) return e;
     const dim3 blk(kBlock);
-    const dim3 grid(wp.grid ? wp.grid : kGrid);
-    const SceneDev &sc = c.batch ? *c.scene : *c.scene;
+    const dim3 grid = queue_grid(wp);
+    const SceneDev &sc = *c.scene;
     const BatchDev bt = c.batch ? c.batch->dev : BatchDev{nullptr, nullptr, 0};
-    hipLaunchKernelGGL(k_reset, dim3(1), blk, 0, s, c.queues.counters, 0u, 1u);
-    if (c.batch) hipLaunchKernelGGL(k_regen<true>, grid, blk, 0, s, sc, bt, c.paths, c.queues, wp, 0u, 1u, c.capacity);
-    else hipLaunchKernelGGL(k_regen<false>, grid, blk, 0, s, sc, bt, c.paths, c.queues, wp, 0u, 1u, c.capacity);
-    hipLaunchKernelGGL(k_regen_commit, dim3(1), dim3(64), 0, s, c.queues, wp, 0u, 1u, c.capacity);
+    hipLaunchKernelGGL(k_reset, dim3(kNumLists + 1), dim3(kStripes), 0, s, q, 0u, 1u);
+    if (c.batch) hipLaunchKernelGGL(k_regen<true>, grid, blk, 0, s, sc, bt, c.paths, q, wp, 0u, 1u, c.capacity);
+    else hipLaunchKernelGGL(k_regen<false>, grid, blk, 0, s, sc, bt, c.paths, q, wp, 0u, 1u, c.capacity);
+    hipLaunchKernelGGL(k_regen_commit, dim3(1), dim3(kStripes), 0, s, q, wp, 0u, 1u, c.capacity);
     return hipGetLastError();
 }
 
@@ -821,11 +943,13 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
     const int cap = (int)stack_cap;
     const size_t lds = extend_lds_bytes(stack_cap);
     const dim3 blk(kBlock);
-    const dim3 grid(wp.grid ? wp.grid : kGrid);
+    const dim3 grid = queue_grid(wp);
     const dim3 pgrid = persistent_grid(lds);
     const PathsDev &paths = c.paths;
     const ShadowDev &shadow = c.shadow;
-    const QueuesDev &q = c.queues;
+    QueuesDev q;
+    hipError_t qe = chunk_queues(c, q);
+    if (qe != hipSuccess) return qe;
     // Rendering paths end at their first cacheable vertex at the cache depth.
     // Training paths record there, trace kTrainSuffix more bounces and end
     // at the next cacheable vertex, whose cache answer completes their
@@ -840,7 +964,7 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
     uint32_t cur = 0;
     for (uint32_t it = 0; it < iterations; ++it) {
         const uint32_t nxt = cur ^ 1u;
-        hipLaunchKernelGGL(k_reset, dim3(1), blk, 0, s, q.counters, cur, 0u);
+        hipLaunchKernelGGL(k_reset, dim3(kNumLists + 1), dim3(kStripes), 0, s, q, cur, 0u);
         if (B) {
             if (wide) hipLaunchKernelGGL((k_extend<true, true>), pgrid, blk, lds, s, sc, bt, paths, q, cur, cap);
             else hipLaunchKernelGGL((k_extend<true, false>), pgrid, blk, lds, s, sc, bt, paths, q, cur, cap);
@@ -861,7 +985,7 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
             hipLaunchKernelGGL((k_shade<kQMiss, false>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
         }
         if (lookups) hipLaunchKernelGGL(k_cache_vertex, grid, blk, 0, s, sc, paths, q);
-        hipLaunchKernelGGL(k_count, dim3(1), blk, 0, s, q, cur);
+        hipLaunchKernelGGL(k_count, dim3(1), dim3(kStripes), 0, s, q, cur);
         if (B && wide) hipLaunchKernelGGL((k_connect<true, true>), pgrid, blk, lds, s, sc, bt, paths, shadow, q, cap);
         else if (B) hipLaunchKernelGGL((k_connect<true, false>), pgrid, blk, lds, s, sc, bt, paths, shadow, q, cap);
         else if (wide) hipLaunchKernelGGL((k_connect<false, true>), pgrid, blk, lds, s, sc, bt, paths, shadow, q, cap);
@@ -870,14 +994,14 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
             // cin rows are this iteration's queries (training records are
             // compacted into cin only by k_finalize, after this)
             hipError_t e = cache_infer(*c.cache, *c.cwork, paths.cqin, c.capacity, paths.crgb, s,
-                                       &q.counters[kCCacheQ]);
+                                       scount(q, kCCacheQ));
             if (e != hipSuccess) return e;
             hipLaunchKernelGGL(k_cache_apply, grid, blk, 0, s, paths, q);
         }
         hipLaunchKernelGGL(k_finalize, grid, blk, 0, s, paths, q, c.res, wp.train);
         if (B) hipLaunchKernelGGL(k_regen<true>, grid, blk, 0, s, sc, bt, paths, q, wp, nxt, 0u, c.capacity);
         else hipLaunchKernelGGL(k_regen<false>, grid, blk, 0, s, sc, bt, paths, q, wp, nxt, 0u, c.capacity);
-        hipLaunchKernelGGL(k_regen_commit, dim3(1), dim3(64), 0, s, q, wp, nxt, 0u, c.capacity);
+        hipLaunchKernelGGL(k_regen_commit, dim3(1), dim3(kStripes), 0, s, q, wp, nxt, 0u, c.capacity);
         cur = nxt;
     }
     return hipGetLastError();
@@ -887,7 +1011,7 @@ hipError_t chunk_film(const ChunkLaunch &c, const FilmDev &film, const WaveParam
 {
     if (wp.train) return hipSuccess;
     const dim3 blk(kBlock);
-    const dim3 grid(wp.grid ? wp.grid : kGrid);
+    const dim3 grid = queue_grid(wp);
     if (!c.batch) {
         hipLaunchKernelGGL(k_film, grid, blk, 0, s, c.res, film, 0u, wp.count, wp.npix,
                            (uint32_t)(wp.first_path % wp.npix));
@@ -929,11 +1053,12 @@ hipError_t render_chunk(const ChunkLaunch &c, const FilmDev &film, const WavePar
     for (uint64_t it = 0; it < max_iters && !done; it += kBlockIters) {
         e = chunk_iterate(c, wp, kBlockIters, s);
         if (e != hipSuccess) return e;
-        e = hipMemcpyAsync(host_counters, c.queues.counters, kNumCounters * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+        e = hipMemcpyAsync(host_counters, c.queues.counters, kCounterWords * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           s);
         if (e != hipSuccess) return e;
         e = hipStreamSynchronize(s);
         if (e != hipSuccess) return e;
-        done = host_counters[kCNext] >= wp.count && host_counters[kCActive0] == 0;
+        done = host_counters[kCNext * kCtrStride] >= wp.count && host_counters[kCLive * kCtrStride] == 0;
     }
     if (!done) return hipErrorUnknown;  // cannot happen: the bound above holds
     return chunk_film(c, film, wp, s);

@@ -11,6 +11,8 @@
 #include <cfloat>
 #include <cstdint>
 
+#include "rt_trace.h"
+
 namespace rt {
 
 // errorBound(n) = (n * eps/2) / (1 - n * eps/2) in double, rounded to float
@@ -356,23 +358,129 @@ __device__ __forceinline__ bool trav_step_any(TravState &s, const float4 *__rest
     else return trav_step<ANY, STATS>(s, nodes, tris, stack, cap, n_node, n_tri);
 }
 
-// Wavefront-uniform refill of idle lanes from a global work counter: one
-// atomic per refill.  Returns the lane's new work index, or UINT32_MAX.
-// `exhausted` (wavefront-uniform) is set once the counter has passed n.
-__device__ __forceinline__ uint32_t wave_refill(uint32_t *counter, uint32_t n, bool has, bool &exhausted)
+// ---------------------------------------------------------------------
+// Work striping.  Device-scope atomics on one address serialise at about
+// 11 ns each on MI355X (0.09 G/s from the whole chip, against 15 G/s spread
+// over many addresses), so a counter that every wavefront touches once per
+// refill or append caps a kernel near 90 M operations/s.  Work lists and ray
+// ranges are therefore cut into kStripes stripes, each with its own counter
+// on its own 128-byte line.  A workgroup works on stripe blockIdx % kStripes
+// (workgroups are dispatched round-robin over the 8 XCDs, so a stripe stays
+// on one XCD and its L2), and a persistent wavefront that finds its stripe
+// drained claims work from the others.
+// ---------------------------------------------------------------------
+// kStripes and kCtrStride are in rt_trace.h (the host sizes the counters)
+static_assert(kStripes == 64, "stripe scans use one wavefront lane per stripe");
+
+// A persistent wavefront's position in the stripes (wavefront-uniform).
+// Claims take kClaimChunk items at once; the wavefront hands them to its
+// idle lanes from the local range [lb, le) of stripe ls, so partial refills
+// cost no atomic.
+constexpr uint32_t kClaimChunk = 128;
+
+struct StripeCursor {
+    uint32_t s;       // stripe being claimed from
+    uint32_t n;       // items in that stripe
+    uint32_t salt;    // per-wavefront pseudo-random state for picking a stripe to help
+    uint32_t ls, lb, le;  // claimed, not yet handed out: items [lb, le) of stripe ls
+    bool empty;       // all of stripe s is claimed: scan before the next claim
+    bool done;        // every stripe is claimed
+};
+
+// No lane will get more work: every stripe is claimed and the local range
+// is handed out.
+__device__ __forceinline__ bool cursor_finished(const StripeCursor &c) { return c.done && c.lb == c.le; }
+
+template <class Count>
+__device__ __forceinline__ void cursor_init(StripeCursor &c, uint32_t home, Count count)
+{
+    c.s = home;
+    c.n = count(home);
+    c.salt = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 0x9E3779B1u + 0x7F4A7C15u;
+    c.ls = home;
+    c.lb = 0;
+    c.le = 0;
+    c.empty = false;
+    c.done = false;
+}
+
+// One pass over every stripe's claim counter (lane j reads stripe j): move
+// to a stripe with unclaimed items, drawn at random among them so that the
+// wavefronts whose own stripe ran dry spread over the rest instead of all
+// queueing on one counter; or mark the wavefront done.  Counters only grow
+// and item counts are fixed during a launch, so a stale read can only
+// under-count claims: "done" is never concluded early.
+template <class Count>
+__device__ __forceinline__ bool cursor_scan(StripeCursor &c, uint32_t *fetch, Count count)
+{
+    const uint32_t lane = __lane_id();
+    const uint32_t claimed =
+        __hip_atomic_load(fetch + lane * kCtrStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t n = count(lane);
+    const bool open = claimed < n;
+    const uint64_t left = __ballot(open);
+    if (left == 0) {
+        c.done = true;
+        return false;
+    }
+    // pick = floor(u * open / 2^24) for a 24-bit draw u: always < open, and no
+    // integer division (the compiler lowers a 24-bit `%` through a float
+    // reciprocal that can overshoot the quotient by one for dividends above
+    // 2^23, which would leave no lane selected)
+    c.salt = c.salt * 1664525u + 1013904223u;
+    const uint32_t pick = ((c.salt >> 8) * (uint32_t)__popcll(left)) >> 24;
+    const uint32_t rank = (uint32_t)__popcll(left & ((1ull << lane) - 1ull));
+    const uint64_t sel = __ballot(open && rank == pick);
+    // `sel` has exactly one bit; the first open stripe stands in should it
+    // ever be empty, so the stripe index is always one of the 64
+    const uint32_t t = (uint32_t)(__ffsll((unsigned long long)(sel ? sel : left)) - 1);
+    c.s = t;
+    c.n = (uint32_t)__shfl((int)n, (int)t);
+    c.empty = false;
+    return true;
+}
+
+// Wavefront-uniform refill of idle lanes: when at least kRefillLanes lanes
+// are idle (or all are), they take consecutive items of the local range;
+// an empty local range is first refilled with one atomic claim of
+// kClaimChunk items from the cursor's stripe.  Returns the lane's item
+// offset within `stripe`, or UINT32_MAX.  The claim loop is
+// wavefront-uniform and ends: every turn either claims items or finds a
+// stripe drained, and claims are finite.
+template <class Count>
+__device__ __forceinline__ uint32_t wave_claim(StripeCursor &c, uint32_t *fetch, Count count, bool has,
+                                               uint32_t &stripe)
 {
     const uint64_t idle = __ballot(!has);
     const uint32_t nidle = (uint32_t)__popcll(idle);
-    if (exhausted || !(nidle >= kRefillLanes || nidle == 64u)) return 0xffffffffu;
+    if (!(nidle >= kRefillLanes || nidle == 64u)) return 0xffffffffu;
     const uint32_t lane = __lane_id();
-    const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)idle) - 1);
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, nidle);
-    base = __shfl(base, (int)leader);
-    if (base + nidle >= n) exhausted = true;
+    if (c.lb == c.le) {
+        if (c.done) return 0xffffffffu;
+        const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)idle) - 1);
+        for (;;) {
+            if (c.empty && !cursor_scan(c, fetch, count)) return 0xffffffffu;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(fetch + c.s * kCtrStride, kClaimChunk);
+            base = (uint32_t)__shfl((int)base, (int)leader);
+            if (base >= c.n) {
+                c.empty = true;
+                continue;
+            }
+            c.ls = c.s;
+            c.lb = base;
+            c.le = c.n - base > kClaimChunk ? base + kClaimChunk : c.n;
+            if (c.le == c.n) c.empty = true;
+            break;
+        }
+    }
+    const uint32_t take = c.le - c.lb < nidle ? c.le - c.lb : nidle;
+    const uint32_t b0 = c.lb;
+    c.lb += take;
+    stripe = c.ls;
     if (has) return 0xffffffffu;
-    const uint32_t r = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-    return r < n ? r : 0xffffffffu;
+    const uint32_t r = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+    return r < take ? b0 + r : 0xffffffffu;
 }
 
 }  // namespace rt

@@ -14,12 +14,25 @@ namespace rt {
 // takes every hit at the radiance cache's depth when a cache is attached.
 enum : uint32_t { kQLambert = 0, kQDielectric = 1, kQGgx = 2, kQLight = 3, kQMiss = 4, kQCache = 5, kNumQ = 6 };
 
-// Device counters (one u32 each).
+// Striped work lists.  Every list is cut into kStripes stripes of cs path
+// slots (rt_device.h): stripe s of list L keeps its item count at
+// sctr[(L * kStripes + s) * kCtrStride] and its items at [s * cs, (s + 1) * cs)
+// of the list's array (the shadow list: cs * nee_stride entries per stripe).
+// Path slots are striped the same way, and a queue kernel's workgroup works
+// on stripe blockIdx % kStripes only, so what a stripe's paths produce stays
+// in that stripe and can never overflow it.  kLClaimExt / kLClaimCon are the
+// claim counters of the persistent extend / connect kernels.
 enum : uint32_t {
-    kCActive0 = 0, kCActive1 = 1, kCQueue = 2, /* 2..7 */ kCShadow = 8, kCTerm = 9, kCCacheQ = 10,
-    kCTrain = 11, kCFetchExt = 12, kCFetchCon = 13, kCNext = 14, kNumCounters = 16
+    kLActive0 = 0, kLActive1 = 1, kLQueue = 2, /* 2..7 */ kLShadow = 8, kLTerm = 9, kLClaimExt = 10,
+    kLClaimCon = 11, kNumLists = 12
 };
-
+// Scalar device counters, kCtrStride words apart: the chunk's next unclaimed
+// path, the live paths after regeneration (summed over stripes, for the
+// host's end-of-chunk test), the cache queries of an iteration and the
+// training records of a chunk.
+enum : uint32_t { kCNext = 0, kCLive = 1, kCCacheQ = 2, kCTrain = 3, kNumCounters = 4 };
+constexpr uint32_t kCounterWords = kNumCounters * kCtrStride;
+constexpr uint32_t kStripedWords = kNumLists * kStripes * kCtrStride;
 constexpr uint32_t kNoCache = 0xffffffffu;
 // bounces a training path traces past its training record before it ends
 // in the cache
@@ -89,9 +102,26 @@ struct QueuesDev {
     uint32_t *active[2];
     uint32_t *q[kNumQ];
     uint32_t *term;
-    uint32_t *counters;
+    uint32_t *sctr;             // kStripedWords: striped list counters
+    uint32_t *counters;         // kCounterWords: scalar counters
     unsigned long long *stats;  // [0] extension rays, [1] shadow rays
+    uint32_t cs;                // path slots per stripe (capacity / kStripes)
 };
+
+__host__ __device__ inline uint32_t *lcount(const QueuesDev &q, uint32_t list, uint32_t stripe)
+{
+    return q.sctr + ((size_t)list * kStripes + stripe) * kCtrStride;
+}
+__host__ __device__ inline uint32_t *scount(const QueuesDev &q, uint32_t id)
+{
+    return q.counters + (size_t)id * kCtrStride;
+}
+// Path slots of a wave of `requested` paths: a whole number per stripe.
+inline uint32_t stripe_capacity(uint32_t requested)
+{
+    const uint32_t r = requested ? requested : 1u;
+    return (r + kStripes - 1) / kStripes * kStripes;
+}
 
 struct FilmDev {
     double *accum;              // W*H*3
@@ -155,7 +185,8 @@ hipError_t chunk_film(const ChunkLaunch &c, const FilmDev &film, const WaveParam
 
 // Host driver: begin, iterate in blocks of 4 until the chunk's paths are
 // all generated and none is live (one small read-back per block), film.
-// host_counters: pinned host memory for kNumCounters words.
+// host_counters: pinned host memory for kCounterWords words.  c.capacity
+// must be a multiple of kStripes.
 hipError_t render_chunk(const ChunkLaunch &c, const FilmDev &film, const WaveParams &wp, uint32_t *host_counters,
                         hipStream_t s);
 

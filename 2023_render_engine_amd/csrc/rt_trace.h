@@ -6,6 +6,13 @@
 
 namespace rt {
 
+// Work striping of the persistent kernels and the wavefront queues: work is
+// cut into kStripes stripes whose counters sit kCtrStride u32 words (one
+// 128-byte line) apart, so device atomics spread over 64 addresses.
+constexpr uint32_t kStripes = 64;
+constexpr uint32_t kCtrStride = 32;
+constexpr uint32_t kClaimWords = kStripes * kCtrStride;  // one striped counter set
+
 struct RaysDev {
     const float *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
 };
@@ -20,12 +27,16 @@ struct DevBvh {
 };
 
 // counters != nullptr selects the counting variant (2 x u64: node visits,
-// triangle tests, accumulated).  ray_counter: one device u32, zero at launch,
-// that the persistent kernel fetches rays from; two launches in flight at
-// once need different counters.
+// triangle tests, accumulated).  ray_counter: kClaimWords device words
+// (kStripes claim counters, kCtrStride words apart), cleared by clear_claims
+// before the launch, that the persistent kernel claims rays from; two
+// launches in flight at once need different counters.
 hipError_t launch_trace(const DevBvh &bvh, const RaysDev &rays, uint32_t n, int32_t *prim, float *t,
                         uint8_t *occluded, bool any, hipStream_t stream, unsigned long long *counters,
                         uint32_t *ray_counter);
 size_t trace_lds_bytes(uint32_t stack_cap);
+// zeroes the kStripes claim counters of ray_counter (a kernel, so it can be
+// captured into a graph)
+hipError_t clear_claims(uint32_t *ray_counter, hipStream_t stream);
 
 }  // namespace rt

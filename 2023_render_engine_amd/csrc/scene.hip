@@ -203,7 +203,7 @@ void free_wave_buffers(WaveBuffers &w)
 rt_status ensure_result_buffer(WaveBuffers &w, uint64_t R, const char *who)
 {
     if (!w.host_counters) {
-        hipError_t e = hipHostMalloc((void **)&w.host_counters, kNumCounters * sizeof(uint32_t), hipHostMallocDefault);
+        hipError_t e = hipHostMalloc((void **)&w.host_counters, kCounterWords * sizeof(uint32_t), hipHostMallocDefault);
         if (e != hipSuccess) {
             w.host_counters = nullptr;
             set_error(std::string(who) + ": pinned counters: " + hipGetErrorString(e));
@@ -303,7 +303,8 @@ rt_status ensure_wave_buffers(WaveBuffers &w, uint32_t P, uint32_t NL, const cha
     add((void **)&q.active[1], (size_t)P * 4);
     for (uint32_t k = 0; k < kNumQ; ++k) add((void **)&q.q[k], (size_t)P * 4);
     add((void **)&q.term, (size_t)P * 4);
-    add((void **)&q.counters, kNumCounters * 4);
+    add((void **)&q.sctr, kStripedWords * 4);
+    add((void **)&q.counters, kCounterWords * 4);
     add((void **)&q.stats, 2 * 8);
     char *base = nullptr;
     hipError_t e = hipMalloc(&base, total);
@@ -568,6 +569,7 @@ static rt_status render_frames(const char *fn, rt_scene *s, const rt_render_opts
     uint32_t P = opts->path_capacity ? opts->path_capacity : (1u << 21);
     if (P > (1u << 26)) { set_error(name + ": path_capacity above 2^26"); return RT_ERR_INVALID; }
     if ((uint64_t)P > total) P = (uint32_t)std::max<uint64_t>(1, total);
+    P = rt::stripe_capacity(P);
     const uint64_t R = std::min<uint64_t>(total, rt::kChunkPaths);
     rt_status st = rt::ensure_buffers(s, P, fn);
     if (st == RT_OK) st = rt::ensure_result_buffer(s->wb, R, fn);
@@ -640,7 +642,8 @@ rt_status rt_cache_train_scene(rt_cache *c, rt_scene *s, const rt_render_opts *o
     std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(s->ctx));
     RT_HIP(hipSetDevice(rt::ctx_device(s->ctx)));
     hipStream_t stream = stream_ ? (hipStream_t)stream_ : (hipStream_t)rt::ctx_stream_ptr(s->ctx);
-    rt_status st = rt::ensure_buffers(s, n_paths, "rt_cache_train_scene");
+    const uint32_t P = rt::stripe_capacity(n_paths);
+    rt_status st = rt::ensure_buffers(s, P, "rt_cache_train_scene");
     if (st == RT_OK) st = rt::ensure_cache_work(s, s->wb.capacity);
     if (st != RT_OK) return st;
     RT_HIP(hipMemsetAsync(s->wb.queues.stats, 0, 16, stream));
@@ -661,14 +664,14 @@ rt_status rt_cache_train_scene(rt_cache *c, rt_scene *s, const rt_render_opts *o
     cl.paths = s->wb.paths;
     cl.shadow = s->wb.shadow;
     cl.queues = s->wb.queues;
-    cl.capacity = n_paths;  // one slot per training path: no read-back needed
+    cl.capacity = P;  // a slot for every training path: no read-back needed
     cl.cache = &rt::cache_device(c);  // training paths end in the cache after their suffix
     cl.cwork = &s->cwork;
     // no host read-back: every training path starts at once and ends within max_depth iterations
     RT_HIP(rt::render_chunk(cl, film, wp, nullptr, stream));
     // the sample count stays on the device: no host round trip unless the
     // caller asks for the loss or the count
-    const uint32_t *k_dev = s->wb.queues.counters + rt::kCTrain;
+    const uint32_t *k_dev = rt::scount(s->wb.queues, rt::kCTrain);
     float *loss_dev = nullptr;
     st = rt::cache_train_rows_dev(c, s->wb.paths.cin, s->wb.paths.ttgt, n_paths, k_dev, stream, &loss_dev);
     if (st != RT_OK) return st;

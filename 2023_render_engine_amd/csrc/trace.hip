@@ -9,9 +9,11 @@
 //
 // Scheduling: persistent wavefronts with dynamic ray fetch.  A launch is
 // sized to fill the chip; each 64-lane wavefront keeps its lanes busy by
-// refilling lanes whose ray finished from a global ray counter (one atomic
-// per refill per wavefront), so a few long rays no longer hold 63 idle lanes
-// hostage (incoherent rays differ in visit count by an order of magnitude).
+// refilling lanes whose ray finished (one atomic per refill per wavefront),
+// so a few long rays no longer hold 63 idle lanes hostage (incoherent rays
+// differ in visit count by an order of magnitude).  The rays are cut into
+// kStripes contiguous ranges with one claim counter each (rt_device.h), so
+// the refills of the whole chip do not queue on one address.
 //
 // Replaces BVH::intersectRay (cuda_libraries/BVH.cu:355-448), which visits
 // both children in fixed order with no hit-distance culling and a 64-entry
@@ -38,25 +40,40 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(const float4 *__restrict_
                                                        int32_t *__restrict__ prim,
                                                        float *__restrict__ tout,
                                                        uint8_t *__restrict__ occluded, int cap,
-                                                       uint32_t *__restrict__ ray_counter,
+                                                       uint32_t lc, uint32_t *__restrict__ ray_counter,
                                                        unsigned long long *__restrict__ counters)
 {
     extern __shared__ int32_t s_stack[];
     int32_t *stack = s_stack + threadIdx.x;
+    // the rays are dealt to the stripes in chunks of 2^lc consecutive rays
+    // (chunk c to stripe c % kStripes), so every stripe holds a sample of
+    // the whole batch and the stripes take about equally long
+    const uint32_t nfull = n >> lc, rem = n & ((1u << lc) - 1u);
+    const uint32_t last = nfull & (kStripes - 1u);
+    auto count = [&](uint32_t j) {
+        uint32_t k = ((nfull >> 6) + (j < last ? 1u : 0u)) << lc;
+        if (rem && j == last) k += rem;
+        return k;
+    };
+    auto ray_of = [&](uint32_t j, uint32_t off) {
+        return ((((off >> lc) << 6) + j) << lc) + (off & ((1u << lc) - 1u));
+    };
+    StripeCursor cur;
+    cursor_init(cur, blockIdx.x % kStripes, count);
     TravState s;
     bool has = false;
-    bool exhausted = false;  // wavefront-uniform: the ray counter passed n
     uint32_t ray = 0;
     uint32_t n_node = 0, n_tri = 0;
     for (;;) {
-        const uint32_t r = wave_refill(ray_counter, n, has, exhausted);
+        uint32_t stripe = 0;
+        const uint32_t r = wave_claim(cur, ray_counter, count, has, stripe);
         if (r != 0xffffffffu) {
-            ray = r;
-            trav_init(s, rays, r, ANY);
+            ray = ray_of(stripe, r);
+            trav_init(s, rays, ray, ANY);
             has = true;
         }
         if (__ballot(has) == 0) {
-            if (exhausted) break;
+            if (cursor_finished(cur)) break;
             continue;
         }
         if (has && trav_step_any<WIDE, ANY, STATS>(s, nodes, tris, stack, cap, n_node, n_tri)) {
@@ -89,6 +106,9 @@ hipError_t launch_trace(const DevBvh &bvh, const RaysDev &rays, uint32_t n, int3
         if (e != hipSuccess || g_cus <= 0) g_cus = 256;
     }
     const int cap = (int)bvh.stack_cap;
+    // stripe chunk 2^lc: a power of two in [64, 4096], about n / 2048 rays
+    uint32_t lc = 6;
+    while (lc < 12 && ((uint64_t)2048 << lc) <= n) ++lc;
     const size_t lds = (size_t)cap * kTraceBlock * sizeof(int32_t);
     // persistent launch: as many workgroups as fit (LDS-limited, at most 16
     // per CU), never more than the rays need
@@ -102,7 +122,7 @@ hipError_t launch_trace(const DevBvh &bvh, const RaysDev &rays, uint32_t n, int3
     const float4 *nd = (const float4 *)bvh.nodes, *tr = (const float4 *)bvh.tris;
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, grid, dim3(kTraceBlock), lds, stream, nd, tr, rays, n, prim, t, occluded, cap,
-                           ray_counter, counters);
+                           lc, ray_counter, counters);
     };
     const bool wide = bvh.wide != 0, stats = counters != nullptr;
     if (wide) {
@@ -112,6 +132,17 @@ hipError_t launch_trace(const DevBvh &bvh, const RaysDev &rays, uint32_t n, int3
         if (stats) any ? go(k_trace<false, true, true>) : go(k_trace<false, false, true>);
         else any ? go(k_trace<false, true, false>) : go(k_trace<false, false, false>);
     }
+    return hipGetLastError();
+}
+
+__global__ void k_clear_claims(uint32_t *ctr)
+{
+    if (blockIdx.x == 0 && threadIdx.x < kStripes) ctr[threadIdx.x * kCtrStride] = 0;
+}
+
+hipError_t clear_claims(uint32_t *ctr, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_clear_claims, dim3(1), dim3(kStripes), 0, stream, ctr);
     return hipGetLastError();
 }
 

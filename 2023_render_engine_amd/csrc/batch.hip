@@ -67,6 +67,8 @@ struct rt_batch {
     uint32_t stack_cap = 0, max_lights = 1;
     uint64_t paths_per_sample = 0;           // sum of pixels
     rt::SceneDev *d_table = nullptr;
+    void *d_nodes = nullptr, *d_tris = nullptr;  // the scenes' trees, pooled
+    uint32_t *d_root = nullptr;              // root node of scene s in d_nodes
     uint64_t *d_bases = nullptr;             // per call: n + 1 path-index bases
     uint32_t *d_spp_offset = nullptr;
     uint64_t *d_chunk = nullptr;             // {first path, count} of the running chunk
@@ -82,6 +84,71 @@ struct rt_batch {
 };
 
 namespace {
+
+// Copies one scene's tree into the batch pool: node references move by
+// nbase, leaf references' first triangle by tbase.  Two-wide nodes keep
+// their children in words 12-13; four-wide nodes in words 4-7, used slots
+// marked by bits 24-27 of word 3.
+__global__ void k_pool_nodes(const int32_t *__restrict__ src, int32_t *__restrict__ dst, uint32_t n, int32_t nbase,
+                             uint32_t tbase, uint32_t wide)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t *a = src + 16 * (size_t)i;
+    int32_t *d = dst + 16 * (size_t)i;
+    for (int k = 0; k < 16; ++k) d[k] = a[k];
+    auto rebase = [&](int32_t c) {
+        if (c >= 0) return c + nbase;
+        const uint32_t enc = ~(uint32_t)c;
+        return (int32_t)~((((enc >> rt::kLeafCountBits) + tbase) << rt::kLeafCountBits) |
+                          (enc & (rt::kMaxLeafCount - 1u)));
+    };
+    if (wide) {
+        const uint32_t valid = ((uint32_t)a[3] >> 24) & 0xFu;
+        for (int k = 0; k < 4; ++k)
+            if ((valid >> k) & 1u) d[4 + k] = rebase(a[4 + k]);
+    } else {
+        d[12] = rebase(a[12]);
+        d[13] = rebase(a[13]);
+    }
+}
+
+// Pools the scenes' trees (nodes and triangle records, each scene's block
+// after the previous one's) and records each scene's root.
+rt_status pool_trees(rt_batch *b, hipStream_t s)
+{
+    const size_t n = b->host_table.size();
+    std::vector<uint32_t> root(n), tbase(n);
+    uint64_t nn = 0, nt = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const rt::DevBvh &v = b->host_table[i].bvh;
+        root[i] = (uint32_t)nn;
+        tbase[i] = (uint32_t)nt;
+        nn += v.num_nodes;
+        nt += (uint64_t)v.num_tris + 1;  // records plus the sentinel
+    }
+    if (nn >= (1ull << 31) || nt >= (1ull << (31 - rt::kLeafCountBits))) {
+        set_error("rt_batch_create: the scenes' trees are too large to pool");
+        return RT_ERR_UNSUPPORTED;
+    }
+    RT_HIP(hipMalloc(&b->d_nodes, std::max<uint64_t>(1, nn) * 64));
+    RT_HIP(hipMalloc(&b->d_tris, std::max<uint64_t>(1, nt) * sizeof(rt::Tri3)));
+    RT_HIP(hipMalloc(&b->d_root, n * sizeof(uint32_t)));
+    for (size_t i = 0; i < n; ++i) {
+        const rt::DevBvh &v = b->host_table[i].bvh;
+        if (v.num_nodes) {
+            hipLaunchKernelGGL(k_pool_nodes, dim3((v.num_nodes + 255) / 256), dim3(256), 0, s, (const int32_t *)v.nodes,
+                               (int32_t *)b->d_nodes + 16 * (size_t)root[i], v.num_nodes, (int32_t)root[i], tbase[i],
+                               v.wide);
+            RT_HIP(hipGetLastError());
+        }
+        RT_HIP(hipMemcpyAsync((char *)b->d_tris + (size_t)tbase[i] * sizeof(rt::Tri3), v.tris,
+                              ((size_t)v.num_tris + 1) * sizeof(rt::Tri3), hipMemcpyDeviceToDevice, s));
+    }
+    RT_HIP(hipMemcpyAsync(b->d_root, root.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    RT_HIP(hipStreamSynchronize(s));
+    return RT_OK;
+}
 
 void drop_graph(rt_batch *b)
 {
@@ -127,7 +194,7 @@ rt_status upload_bases(rt_batch *b, uint32_t spp, hipStream_t s)
 rt::BatchLaunch batch_launch(rt_batch *b, const std::vector<rt::FilmDev> &films)
 {
     rt::BatchLaunch bl{};
-    bl.dev = rt::BatchDev{b->d_table, b->d_bases, (uint32_t)b->scenes.size()};
+    bl.dev = rt::BatchDev{b->d_table, b->d_bases, (uint32_t)b->scenes.size(), b->d_nodes, b->d_tris, b->d_root};
     bl.bases = b->bases.data();
     bl.npix = b->npix.data();
     bl.films = films.data();
@@ -271,15 +338,27 @@ rt_status rt_batch_create(rt_ctx *ctx, rt_scene *const *scenes, uint32_t n, rt_b
         set_error("rt_batch_create: BVH too deep");
         return RT_ERR_UNSUPPORTED;
     }
+    auto release = [&]() {
+        if (b->d_bases) (void)hipFree(b->d_bases);
+        if (b->d_spp_offset) (void)hipFree(b->d_spp_offset);
+        if (b->d_chunk) (void)hipFree(b->d_chunk);
+        if (b->d_nodes) (void)hipFree(b->d_nodes);
+        if (b->d_tris) (void)hipFree(b->d_tris);
+        if (b->d_root) (void)hipFree(b->d_root);
+        delete b;
+    };
     hipError_t e = hipMalloc(&b->d_bases, (n + 1) * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMalloc(&b->d_spp_offset, sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&b->d_chunk, 2 * sizeof(uint64_t));
     if (e != hipSuccess) {
-        if (b->d_bases) (void)hipFree(b->d_bases);
-        if (b->d_spp_offset) (void)hipFree(b->d_spp_offset);
-        delete b;
+        release();
         set_error(std::string("rt_batch_create: ") + hipGetErrorString(e));
         return RT_ERR_OOM;
+    }
+    const rt_status pst = pool_trees(b, (hipStream_t)rt::ctx_stream_ptr(ctx));
+    if (pst != RT_OK) {
+        release();
+        return pst;
     }
     rt::ctx_retain(ctx);
     *out = b;
@@ -300,6 +379,9 @@ rt_status rt_batch_destroy(rt_batch *b)
         if (b->d_bases) (void)hipFree(b->d_bases);
         if (b->d_spp_offset) (void)hipFree(b->d_spp_offset);
         if (b->d_chunk) (void)hipFree(b->d_chunk);
+        if (b->d_nodes) (void)hipFree(b->d_nodes);
+        if (b->d_tris) (void)hipFree(b->d_tris);
+        if (b->d_root) (void)hipFree(b->d_root);
         delete b;
     }
     rt::ctx_release(ctx);

@@ -304,19 +304,17 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc0, BatchDev bt, Pa
     TravState s;
     bool has = false;
     uint32_t slot = 0, dummy0 = 0, dummy1 = 0;
-    const float4 *nodes = (const float4 *)sc0.bvh.nodes;
-    const float4 *tris = (const float4 *)sc0.bvh.tris;
+    // a batch traverses its pooled trees from each path's scene root
+    const float4 *nodes = (const float4 *)(B ? bt.nodes : sc0.bvh.nodes);
+    const float4 *tris = (const float4 *)(B ? bt.tris : sc0.bvh.tris);
     for (;;) {
         uint32_t stripe = 0;
         const uint32_t r = wave_claim(c, lcount(q, kLClaimExt, 0), count, has, stripe);
         if (r != 0xffffffffu) {
             slot = active[stripe * cs + r];
-            if constexpr (B) {
-                const SceneDev &sc = bt.scenes[p.sid[slot]];
-                nodes = (const float4 *)sc.bvh.nodes;
-                tris = (const float4 *)sc.bvh.tris;
-            }
-            trav_begin(s, p.ox[slot], p.oy[slot], p.oz[slot], p.dx[slot], p.dy[slot], p.dz[slot], INFINITY, false);
+            const int32_t root = B ? (int32_t)bt.root[p.sid[slot]] : 0;
+            trav_begin(s, p.ox[slot], p.oy[slot], p.oz[slot], p.dx[slot], p.dy[slot], p.dz[slot], INFINITY, false,
+                       root);
             has = true;
         }
         if (__ballot(has) == 0) {
@@ -559,8 +557,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
                     const V3 Li = mk(pl.color[0] * I * (1 / d2), pl.color[1] * I * (1 / d2), pl.color[2] * I * (1 / d2));
                     if (dot(Lw, s.n) > 0) {  // isOrientedTo (SurfaceInteraction.cu:283)
                         const V3 wi = mk(dot(Lw, t), dot(Lw, b), dot(Lw, s.n));
-                        const V3 f = bsdf_f(m, wo, wi);
-                        const float pf = bsdf_pdf(m, wo, wi);
+                        const V3 f = bsdf_f_of<CLS == kQLambert>(m, wo, wi);
+                        const float pf = bsdf_pdf_of<CLS == kQLambert>(m, wo, wi);
                         if (!black(f) && !black(Li)) {
                             const float w = 1.0f / (1.0f + pf * pf);  // powerHeuristic(1, 1, 1, pf)
                             c[3 * l] = f.x * Li.x * w;
@@ -593,9 +591,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
                     float plp;
                     if (sphere_light_sample(sc.sph[j], s.p, a0, a1, Lw, dist, plp) && dot(Lw, s.n) > 0) {
                         const V3 wi = mk(dot(Lw, t), dot(Lw, b), dot(Lw, s.n));
-                        const V3 f = bsdf_f(m, wo, wi);
+                        const V3 f = bsdf_f_of<CLS == kQLambert>(m, wo, wi);
                         if (!black(f)) {
-                            const float pf = bsdf_pdf(m, wo, wi);
+                            const float pf = bsdf_pdf_of<CLS == kQLambert>(m, wo, wi);
                             const rt_material &lm = sc.mat[sc.sph[j].material];
                             const float w = plp * plp / (plp * plp + pf * pf);
                             const float k = w / plp;
@@ -728,20 +726,16 @@ __global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc0, BatchDev bt, P
     TravState s;
     bool has = false;
     uint32_t sl = 0, dummy0 = 0, dummy1 = 0;
-    const float4 *nodes = (const float4 *)sc0.bvh.nodes;
-    const float4 *tris = (const float4 *)sc0.bvh.tris;
+    const float4 *nodes = (const float4 *)(B ? bt.nodes : sc0.bvh.nodes);
+    const float4 *tris = (const float4 *)(B ? bt.tris : sc0.bvh.tris);
     for (;;) {
         uint32_t stripe = 0;
         const uint32_t r = wave_claim(c, lcount(q, kLClaimCon, 0), count, has, stripe);
         if (r != 0xffffffffu) {
             const size_t e = stripe * sstride + r;
             sl = sh.slot[e];
-            if constexpr (B) {
-                const SceneDev &sc = bt.scenes[p.sid[sl >> 5]];
-                nodes = (const float4 *)sc.bvh.nodes;
-                tris = (const float4 *)sc.bvh.tris;
-            }
-            trav_begin(s, sh.ox[e], sh.oy[e], sh.oz[e], sh.dx[e], sh.dy[e], sh.dz[e], sh.tmax[e], true);
+            const int32_t root = B ? (int32_t)bt.root[p.sid[sl >> 5]] : 0;
+            trav_begin(s, sh.ox[e], sh.oy[e], sh.oz[e], sh.dx[e], sh.dy[e], sh.dz[e], sh.tmax[e], true, root);
             has = true;
         }
         if (__ballot(has) == 0) {
@@ -925,7 +919,7 @@ hipError_t chunk_begin(const ChunkLaunch &c, const WaveParams &wp, hipStream_t s
     const dim3 blk(kBlock);
     const dim3 grid = queue_grid(wp);
     const SceneDev &sc = *c.scene;
-    const BatchDev bt = c.batch ? c.batch->dev : BatchDev{nullptr, nullptr, 0};
+    const BatchDev bt = c.batch ? c.batch->dev : BatchDev{nullptr, nullptr, 0, nullptr, nullptr, nullptr};
     hipLaunchKernelGGL(k_reset, dim3(kNumLists + 1), dim3(kStripes), 0, s, q, 0u, 1u);
     if (c.batch) hipLaunchKernelGGL(k_regen<true>, grid, blk, 0, s, sc, bt, c.paths, q, wp, 0u, 1u, c.capacity);
     else hipLaunchKernelGGL(k_regen<false>, grid, blk, 0, s, sc, bt, c.paths, q, wp, 0u, 1u, c.capacity);
@@ -937,7 +931,7 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
 {
     const bool B = c.batch != nullptr;
     const SceneDev &sc = *c.scene;
-    const BatchDev bt = B ? c.batch->dev : BatchDev{nullptr, nullptr, 0};
+    const BatchDev bt = B ? c.batch->dev : BatchDev{nullptr, nullptr, 0, nullptr, nullptr, nullptr};
     const uint32_t stack_cap = B ? c.batch->stack_cap : sc.bvh.stack_cap;
     const bool wide = sc.bvh.wide != 0;  // a batch's scenes share one layout
     const int cap = (int)stack_cap;

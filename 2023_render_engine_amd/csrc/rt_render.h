@@ -62,6 +62,12 @@ struct BatchDev {
     const SceneDev *scenes;
     const uint64_t *bases;  // n + 1 entries
     uint32_t n;
+    // the scenes' trees pooled into one node array and one triangle array
+    // (child and leaf references rebased), so traversal reads uniform base
+    // pointers; scene s starts at node root[s]
+    const void *nodes;
+    const void *tris;
+    const uint32_t *root;
 };
 
 // Structure-of-arrays path state (capacity P).

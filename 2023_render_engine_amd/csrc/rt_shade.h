@@ -248,50 +248,69 @@ __device__ __forceinline__ bool is_scattering(uint32_t type)
 }
 
 // lightReflected (f * |cos wi|) for the scattering materials, shading space
-__device__ __forceinline__ V3 bsdf_f(const rt_material &m, V3 wo, V3 wi)
+// BSDF value and pdf of one material family: LAMBERT (LambertianBSDF.cu)
+// or the Trowbridge-Reitz family (TrowbridgeBSDF.cu; conductor Fresnel for
+// RT_MAT_TROWBRIDGE, none for RT_MAT_TROWBRIDGE_BASIC).  A shading kernel
+// that serves one family instantiates only its own branch.
+template <bool LAMBERT>
+__device__ __forceinline__ V3 bsdf_f_of(const rt_material &m, V3 wo, V3 wi)
 {
-    if (m.type == RT_MAT_LAMBERTIAN) {
+    if constexpr (LAMBERT) {
         float c = abscos(wi);  // LambertianBSDF.cu:22-38
         return mk(m.kd[0] * kInvPi * c, m.kd[1] * kInvPi * c, m.kd[2] * kInvPi * c);
+    } else {
+        // TrowbridgeBSDF::lightReflected (TrowbridgeBSDF.cu:156-200)
+        float a = sqrtf(m.roughness);
+        V3 o = nrm(wo), i = nrm(wi);
+        if (!(o.z * i.z > 0.0f)) return mk(0, 0, 0);
+        if (a < 1e-3f) return mk(0, 0, 0);
+        float co = abscos(o), ci = abscos(i);
+        if (ci == 0 || co == 0) return mk(0, 0, 0);
+        V3 h = add(i, o);
+        if (dot(h, h) == 0) return mk(0, 0, 0);
+        h = nrm(h);
+        float D = ggx_D(a, a, h);
+        float G = 1 / (1 + ggx_lambda(a, a, o) + ggx_lambda(a, a, i));
+        float F0 = 1, F1 = 1, F2 = 1;
+        if (m.type == RT_MAT_TROWBRIDGE) {
+            float c = abscos2(o, h);
+            F0 = fresnel_conductor(c, m.eta[0], m.k[0]);
+            F1 = fresnel_conductor(c, m.eta[1], m.k[1]);
+            F2 = fresnel_conductor(c, m.eta[2], m.k[2]);
+        }
+        float den = 4 * ci * co;
+        return mk(D * F0 * G / den * m.kd[0] * ci, D * F1 * G / den * m.kd[1] * ci, D * F2 * G / den * m.kd[2] * ci);
     }
-    // TrowbridgeBSDF::lightReflected (TrowbridgeBSDF.cu:156-200)
-    float a = sqrtf(m.roughness);
-    V3 o = nrm(wo), i = nrm(wi);
-    if (!(o.z * i.z > 0.0f)) return mk(0, 0, 0);
-    if (a < 1e-3f) return mk(0, 0, 0);
-    float co = abscos(o), ci = abscos(i);
-    if (ci == 0 || co == 0) return mk(0, 0, 0);
-    V3 h = add(i, o);
-    if (dot(h, h) == 0) return mk(0, 0, 0);
-    h = nrm(h);
-    float D = ggx_D(a, a, h);
-    float G = 1 / (1 + ggx_lambda(a, a, o) + ggx_lambda(a, a, i));
-    float F0 = 1, F1 = 1, F2 = 1;
-    if (m.type == RT_MAT_TROWBRIDGE) {
-        float c = abscos2(o, h);
-        F0 = fresnel_conductor(c, m.eta[0], m.k[0]);
-        F1 = fresnel_conductor(c, m.eta[1], m.k[1]);
-        F2 = fresnel_conductor(c, m.eta[2], m.k[2]);
+}
+
+template <bool LAMBERT>
+__device__ __forceinline__ float bsdf_pdf_of(const rt_material &m, V3 wo, V3 wi)
+{
+    if constexpr (LAMBERT) {
+        if (wo.z * wi.z > 0.0f) return (wi.z / len(wi)) * kInvPi;  // LambertianBSDF.cu:40-50
+        return 0;
+    } else {
+        // TrowbridgeBSDF::pdf (TrowbridgeBSDF.cu:202-214)
+        float a = sqrtf(m.roughness);
+        if (!(wo.z * wi.z > 0.0f)) return 0;
+        if (a < 1e-3f) return 0;
+        V3 h = add(wo, wi);
+        if (dot(h, h) == 0) return 0;
+        h = nrm(h);
+        if (h.z < 0.f) h = neg(h);
+        return ggx_Dv(a, a, wo, h) / (4 * abscos2(wo, h));
     }
-    float den = 4 * ci * co;
-    return mk(D * F0 * G / den * m.kd[0] * ci, D * F1 * G / den * m.kd[1] * ci, D * F2 * G / den * m.kd[2] * ci);
+}
+
+// Any scattering material (Lambertian or Trowbridge).
+__device__ __forceinline__ V3 bsdf_f(const rt_material &m, V3 wo, V3 wi)
+{
+    return m.type == RT_MAT_LAMBERTIAN ? bsdf_f_of<true>(m, wo, wi) : bsdf_f_of<false>(m, wo, wi);
 }
 
 __device__ __forceinline__ float bsdf_pdf(const rt_material &m, V3 wo, V3 wi)
 {
-    if (m.type == RT_MAT_LAMBERTIAN) {
-        if (wo.z * wi.z > 0.0f) return (wi.z / len(wi)) * kInvPi;  // LambertianBSDF.cu:40-50
-        return 0;
-    }
-    // TrowbridgeBSDF::pdf (TrowbridgeBSDF.cu:202-214)
-    float a = sqrtf(m.roughness);
-    if (!(wo.z * wi.z > 0.0f)) return 0;
-    if (a < 1e-3f) return 0;
-    V3 h = add(wo, wi);
-    if (dot(h, h) == 0) return 0;
-    h = nrm(h);
-    if (h.z < 0.f) h = neg(h);
-    return ggx_Dv(a, a, wo, h) / (4 * abscos2(wo, h));
+    return m.type == RT_MAT_LAMBERTIAN ? bsdf_pdf_of<true>(m, wo, wi) : bsdf_pdf_of<false>(m, wo, wi);
 }
 
 // Material::sampleDirection.  Draws: Lambertian 2, dielectric 1, Trowbridge

@@ -188,7 +188,7 @@ struct TravState {
 // maximumOffset = +inf and each triangle is judged on its own
 // (order-independent), occluding iff accepted at t <= tmax.
 __device__ __forceinline__ void trav_begin(TravState &s, float ox, float oy, float oz, float dx, float dy, float dz,
-                                           float tmax, bool any)
+                                           float tmax, bool any, int32_t root = 0)
 {
     s.tmax = tmax;
     s.tr = make_tri_ray(ox, oy, oz, dx, dy, dz, any ? INFINITY : tmax);
@@ -196,7 +196,7 @@ __device__ __forceinline__ void trav_begin(TravState &s, float ox, float oy, flo
     s.h = INFINITY;
     s.thit = INFINITY;
     s.hit = -1;
-    s.cur = 0;
+    s.cur = root;
     s.sp = 0;
     s.occ = false;
 }
@@ -285,22 +285,29 @@ __device__ __forceinline__ bool trav_step4(TravState &s, const float4 *__restric
         const float Bx = fmaf(a.x, s.br.ix, -s.br.oix);
         const float By = fmaf(a.y, s.br.iy, -s.br.oiy);
         const float Bz = fmaf(a.z, s.br.iz, -s.br.oiz);
+        // Near and far planes per axis by the sign of 1/d: A has that sign
+        // and fma(q, A, B) is monotone in q, so this selects exactly the
+        // min and max of the two plane distances without computing both.
         const uint32_t qlx = __float_as_uint(c.x), qhx = __float_as_uint(c.y);
         const uint32_t qly = __float_as_uint(c.z), qhy = __float_as_uint(c.w);
         const uint32_t qlz = __float_as_uint(d.x), qhz = __float_as_uint(d.y);
+        const bool nx = s.br.ix < 0, ny = s.br.iy < 0, nz = s.br.iz < 0;
+        const uint32_t qnx = nx ? qhx : qlx, qfx = nx ? qlx : qhx;
+        const uint32_t qny = ny ? qhy : qly, qfy = ny ? qly : qhy;
+        const uint32_t qnz = nz ? qhz : qlz, qfz = nz ? qlz : qhz;
         uint32_t key[4];
         int32_t ref[4] = {__float_as_int(b.x), __float_as_int(b.y), __float_as_int(b.z), __float_as_int(b.w)};
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int sh = 8 * i;
-            const float tx0 = fmaf((float)((qlx >> sh) & 0xffu), Ax, Bx);
-            const float tx1 = fmaf((float)((qhx >> sh) & 0xffu), Ax, Bx);
-            const float ty0 = fmaf((float)((qly >> sh) & 0xffu), Ay, By);
-            const float ty1 = fmaf((float)((qhy >> sh) & 0xffu), Ay, By);
-            const float tz0 = fmaf((float)((qlz >> sh) & 0xffu), Az, Bz);
-            const float tz1 = fmaf((float)((qhz >> sh) & 0xffu), Az, Bz);
-            const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-            float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+            const float txn = fmaf((float)((qnx >> sh) & 0xffu), Ax, Bx);
+            const float txf = fmaf((float)((qfx >> sh) & 0xffu), Ax, Bx);
+            const float tyn = fmaf((float)((qny >> sh) & 0xffu), Ay, By);
+            const float tyf = fmaf((float)((qfy >> sh) & 0xffu), Ay, By);
+            const float tzn = fmaf((float)((qnz >> sh) & 0xffu), Az, Bz);
+            const float tzf = fmaf((float)((qfz >> sh) & 0xffu), Az, Bz);
+            const float tn = fmaxf(fmaxf(txn, tyn), fmaxf(tzn, 0.0f));
+            float tf = fminf(fminf(txf, tyf), tzf);
             tf = fminf(tf * kTfarScale, tlim);
             const bool hit = ((meta >> (24 + i)) & 1u) && tn <= tf;
             key[i] = hit ? __float_as_uint(tn) : 0xffffffffu;

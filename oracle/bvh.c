@@ -463,8 +463,11 @@ void orc_bvh2_trace(const float *nodes, const float *tris, const float *ox, cons
  * valid<<24, child[4] (int32), then lo_x hi_x lo_y hi_y lo_z hi_z words
  * (byte i = child i) and two zero words.  Child i's box along x is
  * [origin.x + lo_x[i]*2^(ex-127), origin.x + hi_x[i]*2^(ex-127)].  The
- * operations below are the kernel's (trav_step4), one for one: the slab
- * distances fma(q, s*inv, fma(origin, inv, -o*inv)), entry distances sorted
+ * operations below are the kernel's (trav_step4): the slab distances
+ * fma(q, s*inv, fma(origin, inv, -o*inv)) (the kernel evaluates only the
+ * near and far plane per axis, chosen by the sign of inv; since s*inv has
+ * that sign and fma is monotone in q, they are the min and max taken here,
+ * so the entry and exit distances are the same numbers), entry distances sorted
  * as bit patterns with misses as 0xffffffff by the same 5-comparator
  * network, the nearest child followed and the others pushed farthest
  * first.  max_sp reports the deepest stack the rays needed, which the

@@ -99,9 +99,14 @@ def test_host_pointer_path_equals_device_path(rt, gpu_ctx):
 
 @pytest.mark.parametrize("width", [2, 4])
 def test_kernel_counts_match_oracle_traversal(rt, orc, gpu_ctx, width):
-    """The counting kernel variant visits exactly the nodes and triangles the
-    oracle's restatement of the kernel traversal visits on the same tree
-    (two-wide FP32 nodes, or four-wide quantized nodes)."""
+    """The counting kernel variant against the oracle's restatement of the
+    kernel traversal on the same tree.  Two-wide: exactly the same node
+    visits and triangle tests, and the same hit on every ray (same order).
+    Four-wide: the kernel parks reached leaves until the wavefront's leaf
+    phase, so its running hit distance culls a little later -- it visits
+    every node and tests every triangle the immediate-leaf order does, plus
+    a few more (bounded here), and hits differ only on rays whose result
+    depends on the test order (ambiguous for brute force)."""
     xyz, idx = rt.gen_soup(50000, seed=12)
     rays = rt.gen_rays(64, 64, n_incoherent=8192, seed=13)
     bvh = rt.Bvh(gpu_ctx, xyz, idx, width=width)
@@ -111,11 +116,23 @@ def test_kernel_counts_match_oracle_traversal(rt, orc, gpu_ctx, width):
     else:
         p2, t2, nn, ntr, _ = orc.bvh4_trace(bvh.export4(), tris, rays)
     gn, gt = bvh.trace_stats(*to_dev(rays))
-    assert (gn, gt) == (nn, ntr)
     prim, t = bvh.trace_closest(*to_dev(rays))
     torch.cuda.synchronize()
-    assert np.array_equal(prim.cpu().numpy(), p2)  # same tree, same order: every ray
-    assert np.array_equal(t.cpu().numpy(), t2)
+    prim, t = prim.cpu().numpy(), t.cpu().numpy()
+    print(f"[counts width={width}] nodes {gn} vs {nn} ({gn / nn:.4f}), tris {gt} vs {ntr} ({gt / max(ntr, 1):.4f})",
+          flush=True)
+    if width == 2:
+        assert (gn, gt) == (nn, ntr)
+        assert np.array_equal(prim, p2)  # same tree, same order: every ray
+        assert np.array_equal(t, t2)
+    else:
+        assert nn <= gn <= 1.15 * nn
+        assert ntr <= gt <= 1.3 * ntr
+        diff = np.nonzero((prim != p2) | (t != t2))[0]
+        assert diff.size <= 1e-3 * prim.size
+        if diff.size:
+            _, _, fl = orc.brute_trace(xyz, idx, [a[diff] for a in rays])
+            assert np.all(fl != 0), diff[fl == 0][:10]
 
 
 def test_full_size_c1_against_reference_bvh(rt, orc, gpu_ctx):
@@ -136,7 +153,14 @@ def test_full_size_c1_against_reference_bvh(rt, orc, gpu_ctx):
     prim, t = prim.cpu().numpy(), t.cpu().numpy()
     nodes, tris = bvh.export()
     p2, t2, _, _, _ = orc.bvh4_trace(bvh.export4(), tris, rays, nthreads=16)
-    assert np.array_equal(prim, p2) and np.array_equal(t, t2)
+    # the kernel's leaf order differs from the restated immediate order: only
+    # order-dependent (ambiguous) rays may differ
+    dk = np.nonzero((prim != p2) | (t != t2))[0]
+    print(f"[c1 full size] kernel vs restated traversal: {dk.size} rays differ", flush=True)
+    assert dk.size <= 1e-4 * prim.size
+    if dk.size:
+        _, _, fk = orc.brute_trace(xyz, idx, [a[dk] for a in rays], nthreads=16)
+        assert np.all(fk != 0), dk[fk == 0][:10]
     rp, rtt = orc.RefBvh(xyz, idx).trace(rays, nthreads=16)
     diff = np.nonzero(prim != rp)[0]
     print(f"[c1 full size] hits {np.mean(prim >= 0):.3f}, differ from reference tree: {diff.size}", flush=True)

@@ -64,7 +64,7 @@ class Camera(C.Structure):
 class RenderOpts(C.Structure):
     _fields_ = [("spp", C.c_uint32), ("spp_offset", C.c_uint32), ("max_depth", C.c_uint32),
                 ("rr_depth", C.c_uint32), ("seed", C.c_uint32), ("path_capacity", C.c_uint32),
-                ("grid", C.c_uint32)]
+                ("grid", C.c_uint32), ("chunk_paths", C.c_uint32)]
 
 
 class RenderStats(C.Structure):
@@ -78,7 +78,8 @@ def f3(v):
 
 class BatchOpts(C.Structure):
     _fields_ = [("spp", C.c_uint32), ("spp_offset", C.c_uint32), ("max_depth", C.c_uint32),
-                ("rr_depth", C.c_uint32), ("path_capacity", C.c_uint32), ("grid", C.c_uint32)]
+                ("rr_depth", C.c_uint32), ("path_capacity", C.c_uint32), ("grid", C.c_uint32),
+                ("chunk_paths", C.c_uint32)]
 
 
 class TuneEntry(C.Structure):
@@ -431,18 +432,18 @@ class Scene:
         return (torch.zeros(n * 3, dtype=torch.float64, device=dev), torch.zeros(n, dtype=torch.int32, device=dev))
 
     def render(self, film, spp=None, spp_offset=0, max_depth=None, rr_depth=None, seed=None, path_capacity=0,
-               grid=0, stream=None):
-        o = self._opts(spp, spp_offset, max_depth, rr_depth, seed, path_capacity, grid)
+               grid=0, chunk_paths=0, stream=None):
+        o = self._opts(spp, spp_offset, max_depth, rr_depth, seed, path_capacity, grid, chunk_paths)
         accum, nvalid = film
         self.ctx.launch(lambda s: check(lib().rt_render(self.h, C.byref(o), _ptr(accum), _ptr(nvalid), 0, s)),
                         stream)
         return film
 
-    def _opts(self, spp, spp_offset, max_depth, rr_depth, seed, path_capacity, grid=0):
+    def _opts(self, spp, spp_offset, max_depth, rr_depth, seed, path_capacity, grid=0, chunk_paths=0):
         d = self.desc
         return RenderOpts(spp if spp is not None else d.spp, spp_offset, max_depth or d.max_depth,
                           d.rr_depth if rr_depth is None else rr_depth, d.seed if seed is None else seed,
-                          path_capacity, grid)
+                          path_capacity, grid, chunk_paths)
 
     def render_cached(self, film, cache, cache_depth=2, spp=None, spp_offset=0, max_depth=None, rr_depth=None,
                       seed=None, path_capacity=0, stream=None):
@@ -508,9 +509,9 @@ class Batch:
             pass
 
     def render(self, films, spp, spp_offset=0, max_depth=12, rr_depth=3, seeds=None, path_capacity=0, grid=0,
-               graph=False, stream=None):
+               graph=False, chunk_paths=0, stream=None):
         import numpy as np
-        o = BatchOpts(spp, spp_offset, max_depth, rr_depth, path_capacity, grid)
+        o = BatchOpts(spp, spp_offset, max_depth, rr_depth, path_capacity, grid, chunk_paths)
         n = len(self.scenes)
         acc = (_P * n)(*[_ptr(f[0]) for f in films])
         nv = (_P * n)(*[_ptr(f[1]) for f in films])
@@ -522,7 +523,7 @@ class Batch:
 
     def autotune(self, spp=1, max_depth=12, rr_depth=3):
         """Returns (best TuneEntry, [TuneEntry ...]) and makes best the default."""
-        o = BatchOpts(spp, 0, max_depth, rr_depth, 0, 0)
+        o = BatchOpts(spp, 0, max_depth, rr_depth, 0, 0, 0)
         best = TuneEntry()
         ents = (TuneEntry * 64)()
         cnt = C.c_uint32(0)

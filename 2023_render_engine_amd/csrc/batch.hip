@@ -253,7 +253,8 @@ hipError_t run_frame(rt_batch *b, const rt_batch_opts &o, uint32_t P, uint32_t g
         }
     }
     const uint64_t total = b->bases.back();
-    const uint64_t R = std::min<uint64_t>(total, b->wb.res_cap);
+    const uint64_t chunk = o.chunk_paths ? o.chunk_paths : rt::kChunkPaths;
+    const uint64_t R = std::min<uint64_t>(std::min<uint64_t>(total, chunk), b->wb.res_cap);
     uint32_t *hc = b->wb.host_counters;
     for (uint64_t first = 0; first < total; first += R) {
         wp.first_path = first;
@@ -408,7 +409,8 @@ rt_status rt_batch_render(rt_batch *b, const rt_batch_opts *opts, const uint32_t
     const uint32_t P = wave_size(b, *opts, total);
     const uint32_t grid = opts->grid ? opts->grid : (b->tuned_grid ? b->tuned_grid : 2048);
     st = rt::ensure_wave_buffers(b->wb, P, b->max_lights, "rt_batch_render");
-    if (st == RT_OK) st = rt::ensure_result_buffer(b->wb, std::min<uint64_t>(total, rt::kChunkPaths), "rt_batch_render");
+    const uint64_t chunk = opts->chunk_paths ? opts->chunk_paths : rt::kChunkPaths;
+    if (st == RT_OK) st = rt::ensure_result_buffer(b->wb, std::min<uint64_t>(total, chunk), "rt_batch_render");
     if (st != RT_OK) return st;
     st = upload_table(b, seeds, s);
     if (st != RT_OK) return st;

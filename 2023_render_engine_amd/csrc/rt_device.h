@@ -172,7 +172,17 @@ __device__ __forceinline__ void decode_leaf(int32_t c, uint32_t &first, uint32_t
 // bounds the pending entries, so it cannot overflow.
 // ---------------------------------------------------------------------
 constexpr int kTravBlock = 128;
-constexpr uint32_t kRefillLanes = 16;  // refill a wavefront when at least this many lanes are idle
+// Tuning constants of the persistent traversal (build-time, -D overridable).
+#ifndef RT_REFILL_LANES
+#define RT_REFILL_LANES 16
+#endif
+#ifndef RT_CLAIM_CHUNK
+#define RT_CLAIM_CHUNK 128
+#endif
+#ifndef RT_LEAF_BATCH
+#define RT_LEAF_BATCH 16
+#endif
+constexpr uint32_t kRefillLanes = RT_REFILL_LANES;  // refill a wavefront when at least this many lanes are idle
 
 // No node or leaf: a reference that no tree produces (a leaf reference is
 // ~(first << 3 | count - 1) with first + count <= 2^28).
@@ -283,7 +293,7 @@ __device__ __forceinline__ void cswap_key(uint32_t &ka, int32_t &ra, uint32_t &k
 // hits are unchanged (the nearest accepted triangle does not depend on the
 // order of the tests, only ties would); culling against the running hit
 // distance starts a little later, so a ray may visit a few more nodes.
-constexpr uint32_t kLeafBatch = 16;
+constexpr uint32_t kLeafBatch = RT_LEAF_BATCH;
 
 template <bool ANY, bool STATS>
 __device__ __forceinline__ bool trav_step4(TravState &s, const float4 *__restrict__ nodes,
@@ -416,7 +426,7 @@ static_assert(kStripes == 64, "stripe scans use one wavefront lane per stripe");
 // Claims take kClaimChunk items at once; the wavefront hands them to its
 // idle lanes from the local range [lb, le) of stripe ls, so partial refills
 // cost no atomic.
-constexpr uint32_t kClaimChunk = 128;
+constexpr uint32_t kClaimChunk = RT_CLAIM_CHUNK;
 
 struct StripeCursor {
     uint32_t s;       // stripe being claimed from

@@ -30,6 +30,9 @@ rt_status ensure_result_buffer(WaveBuffers &w, uint64_t R, const char *who);
 void free_wave_buffers(WaveBuffers &w);
 
 // Paths per chunk: the result buffer holds 12 bytes per path (768 MiB).
+// Each chunk ends with a drain, but larger chunks measured slower on the
+// 32-scene batch (a 4.6 GB result buffer: 282 vs 334 Msamples/s) and no
+// faster on a 1080p 64-spp frame.
 constexpr uint64_t kChunkPaths = 1ull << 26;
 
 }  // namespace rt

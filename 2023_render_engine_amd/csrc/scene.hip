@@ -570,7 +570,7 @@ static rt_status render_frames(const char *fn, rt_scene *s, const rt_render_opts
     if (P > (1u << 26)) { set_error(name + ": path_capacity above 2^26"); return RT_ERR_INVALID; }
     if ((uint64_t)P > total) P = (uint32_t)std::max<uint64_t>(1, total);
     P = rt::stripe_capacity(P);
-    const uint64_t R = std::min<uint64_t>(total, rt::kChunkPaths);
+    const uint64_t R = std::min<uint64_t>(total, opts->chunk_paths ? opts->chunk_paths : rt::kChunkPaths);
     rt_status st = rt::ensure_buffers(s, P, fn);
     if (st == RT_OK) st = rt::ensure_result_buffer(s->wb, R, fn);
     if (st != RT_OK) return st;

@@ -295,7 +295,7 @@ def run_c4(args):
     desc = scenes.procedural(width=W, height=H, spp=4)
     label = (f"radiance cache on the procedural scene ({desc.idx.shape[0]} triangles, 8 point lights + 1 sphere "
              f"light, depth 16, RR after 3) at {W}x{H}: 4 spp frames, 65536 training paths/frame, cache after "
-             f"2 bounces; reference {args.c4_ref_spp} spp FP64")
+             f"{args.c4_cache_depth} bounces; reference {args.c4_ref_spp} spp FP64")
     ctx = rt.Context(0)
     sc = rt.Scene(ctx, desc)
     # reference (untimed), in 256-spp chunks at sample offsets far from the runs'
@@ -324,11 +324,11 @@ def run_c4(args):
             if use_cache:
                 # the loss is read back (a host round trip) only on a few frames
                 want = frame % 8 == 0
-                l, _ = sc.train_cache(cache, frame, n_paths=65536, cache_depth=2, want_loss=want)
+                l, _ = sc.train_cache(cache, frame, n_paths=65536, cache_depth=args.c4_cache_depth, want_loss=want)
                 if want:
                     losses.append(l)
                 if frame >= args.c4_warm:
-                    sc.render_cached(film, cache, cache_depth=2, spp=4, spp_offset=4 * frame)
+                    sc.render_cached(film, cache, cache_depth=args.c4_cache_depth, spp=4, spp_offset=4 * frame)
             else:
                 sc.render(film, spp=4, spp_offset=4 * frame)
             torch.cuda.synchronize()
@@ -517,6 +517,8 @@ def main():
     ap.add_argument("--c4-target", type=float, default=1e-2)
     ap.add_argument("--c4-budget", type=float, default=60.0, help="c4: timed seconds per method at most")
     ap.add_argument("--c4-warm", type=int, default=20, help="c4: training-only frames before the cache renders")
+    ap.add_argument("--c4-cache-depth", type=int, default=2,
+                    help="c4: bounces before a path ends in the cache (the workload's definition is 2)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-split", action="store_true",
                     help="c1: skip the separate coherent/incoherent launches (profiling runs)")

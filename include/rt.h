@@ -200,6 +200,7 @@ typedef struct {
     uint32_t seed;        /* PCG32 seed */
     uint32_t path_capacity; /* paths in flight per wave, 0 = default (2^21) */
     uint32_t grid;          /* workgroups of the queue kernels, 0 = default (2048) */
+    uint32_t chunk_paths;   /* paths per streamed chunk (12 B of result buffer each), 0 = default (2^26) */
 } rt_render_opts;
 
 typedef struct rt_scene rt_scene;
@@ -266,6 +267,7 @@ typedef struct {
     uint32_t spp, spp_offset, max_depth, rr_depth;
     uint32_t path_capacity;  /* paths per wave, 0 = the autotuned or default value */
     uint32_t grid;           /* workgroups of the queue kernels, 0 = autotuned or default */
+    uint32_t chunk_paths;    /* paths per streamed chunk, 0 = default (2^26) */
 } rt_batch_opts;
 
 rt_status rt_batch_create(rt_ctx *ctx, rt_scene *const *scenes, uint32_t n, rt_batch **out);

@@ -413,6 +413,7 @@ private:
 // Render settings of the reference's compile-time macros (types.h:41-42,73-74).
 struct RenderSettings {
     uint32_t spp = 16, max_depth = 8, rr_depth = 3, seed = 1, spp_offset = 0, path_capacity = 0, grid = 0;
+    uint32_t chunk_paths = 0;  // paths per streamed chunk, 0 = the library's default
 };
 
 // PathTracerIntegrator(camera).render(scene): renders every pixel at
@@ -438,6 +439,7 @@ public:
         o.seed = s.seed;
         o.path_capacity = s.path_capacity;
         o.grid = s.grid;
+        o.chunk_paths = s.chunk_paths;
         check(rt_render(scene.get(), &o, (double *)accum.get(), (uint32_t *)nvalid.get(), 0, nullptr));
         ctx.synchronize();
         std::vector<double> sums(npix * 3);

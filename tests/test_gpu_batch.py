@@ -47,14 +47,16 @@ def batch3(rt, gpu_ctx):
         s.close()
 
 
-@pytest.mark.parametrize("cap", [0, 5000])
-def test_batch_equals_standalone_renders(rt, batch3, cap):
-    """cap 5000 forces waves that cross scene boundaries."""
+@pytest.mark.parametrize("cap,chunk", [(0, 0), (5000, 0), (3000, 7001)])
+def test_batch_equals_standalone_renders(rt, batch3, cap, chunk):
+    """cap 5000 forces waves that cross scene boundaries; chunk 7001 streams
+    the batch in chunks whose edges fall inside scenes and samples."""
     scs, b = batch3
-    print(f"[batch] 3 scenes, path_capacity {cap}", flush=True)
+    print(f"[batch] 3 scenes, path_capacity {cap}, chunk_paths {chunk}", flush=True)
     seeds = [3, 1, 7]
     films = [s.new_film() for s in scs]
-    b.render(films, spp=8, spp_offset=16, max_depth=8, rr_depth=3, seeds=seeds, path_capacity=cap)
+    b.render(films, spp=8, spp_offset=16, max_depth=8, rr_depth=3, seeds=seeds, path_capacity=cap,
+             chunk_paths=chunk)
     st = b.stats()
     assert st.paths == sum(d.width * d.height for d in descs()) * 8
     for s, f, sd in zip(scs, films, seeds):

@@ -20,10 +20,10 @@ pytestmark = pytest.mark.gpu
 scenes = importlib.import_module("2023_render_engine_amd.scenes")
 
 
-def gpu_render(rt, ctx, desc, spp, spp_offset=0, seed=None, path_capacity=0):
+def gpu_render(rt, ctx, desc, spp, spp_offset=0, seed=None, path_capacity=0, chunk_paths=0):
     sc = rt.Scene(ctx, desc)
     film = sc.new_film()
-    sc.render(film, spp=spp, spp_offset=spp_offset, seed=seed, path_capacity=path_capacity)
+    sc.render(film, spp=spp, spp_offset=spp_offset, seed=seed, path_capacity=path_capacity, chunk_paths=chunk_paths)
     torch.cuda.synchronize()
     acc = film[0].cpu().numpy().copy()
     nv = film[1].cpu().numpy().astype(np.uint32).copy()
@@ -92,6 +92,11 @@ def test_render_deterministic_and_progressive(rt):
     a3, n3, _ = gpu_render(rt, ctx, d, 8, path_capacity=1000)
     assert np.array_equal(n1, n3)
     assert np.allclose(a1, a3, rtol=1e-12, atol=0)
+    # streamed in chunks of 5000 paths (chunk edges inside a sample's pixel
+    # range, several chunks per film pass) == one chunk
+    a4, n4, _ = gpu_render(rt, ctx, d, 8, path_capacity=1024, chunk_paths=5000)
+    assert np.array_equal(n1, n4)
+    assert np.allclose(a1, a4, rtol=1e-12, atol=0)
     # 4 + 4 with spp_offset == 8
     sc = rt.Scene(ctx, d)
     film = sc.new_film()

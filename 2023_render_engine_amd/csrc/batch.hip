@@ -47,11 +47,11 @@ namespace {
 // memory at replay; the wave buffers are fixed while the key holds).
 struct GraphKey {
     uint32_t max_depth = 0, rr_depth = 0, capacity = 0, grid = 0;
-    void *arena = nullptr, *res = nullptr;
+    void *arena = nullptr, *res = nullptr, *hint = nullptr;
     bool operator==(const GraphKey &o) const
     {
         return max_depth == o.max_depth && rr_depth == o.rr_depth && capacity == o.capacity && grid == o.grid &&
-               arena == o.arena && res == o.res;
+               arena == o.arena && res == o.res && hint == o.hint;
     }
 };
 
@@ -70,6 +70,8 @@ struct rt_batch {
     void *d_nodes = nullptr, *d_tris = nullptr;  // the scenes' trees, pooled
     uint32_t *d_root = nullptr;              // root node of scene s in d_nodes
     uint64_t *d_bases = nullptr;             // per call: n + 1 path-index bases
+    uint32_t *d_hint = nullptr;              // scene at each 2^16-path block start
+    size_t hint_cap = 0;
     uint32_t *d_spp_offset = nullptr;
     uint64_t *d_chunk = nullptr;             // {first path, count} of the running chunk
     std::vector<uint64_t> bases;             // host copy for the current spp
@@ -185,7 +187,24 @@ rt_status upload_bases(rt_batch *b, uint32_t spp, hipStream_t s)
     const size_t n = b->scenes.size();
     b->bases.assign(n + 1, 0);
     for (size_t i = 0; i < n; ++i) b->bases[i + 1] = b->bases[i] + (uint64_t)b->npix[i] * spp;
+    // scene of each 2^16-path block start
+    const uint64_t total = b->bases[n];
+    std::vector<uint32_t> hint((size_t)((total >> rt::kHintShift) + 1), 0);
+    uint32_t sid = 0;
+    for (size_t k = 0; k < hint.size(); ++k) {
+        const uint64_t g = (uint64_t)k << rt::kHintShift;
+        while (sid + 1 < n && b->bases[sid + 1] <= g) ++sid;
+        hint[k] = sid;
+    }
+    if (hint.size() > b->hint_cap) {
+        if (b->d_hint) RT_HIP(hipFree(b->d_hint));
+        b->d_hint = nullptr;
+        b->hint_cap = 0;
+        RT_HIP(hipMalloc(&b->d_hint, hint.size() * sizeof(uint32_t)));
+        b->hint_cap = hint.size();
+    }
     RT_HIP(hipMemcpyAsync(b->d_bases, b->bases.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    RT_HIP(hipMemcpyAsync(b->d_hint, hint.data(), hint.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     RT_HIP(hipStreamSynchronize(s));
     b->bases_spp = spp;
     return RT_OK;
@@ -194,7 +213,8 @@ rt_status upload_bases(rt_batch *b, uint32_t spp, hipStream_t s)
 rt::BatchLaunch batch_launch(rt_batch *b, const std::vector<rt::FilmDev> &films)
 {
     rt::BatchLaunch bl{};
-    bl.dev = rt::BatchDev{b->d_table, b->d_bases, (uint32_t)b->scenes.size(), b->d_nodes, b->d_tris, b->d_root};
+    bl.dev = rt::BatchDev{b->d_table, b->d_bases, (uint32_t)b->scenes.size(), b->d_nodes, b->d_tris, b->d_root,
+                          b->d_hint};
     bl.bases = b->bases.data();
     bl.npix = b->npix.data();
     bl.films = films.data();
@@ -234,7 +254,7 @@ hipError_t run_frame(rt_batch *b, const rt_batch_opts &o, uint32_t P, uint32_t g
     if (graph) {
         GraphKey k;
         k.max_depth = o.max_depth; k.rr_depth = o.rr_depth; k.capacity = P; k.grid = grid;
-        k.arena = b->wb.arena; k.res = b->wb.res;
+        k.arena = b->wb.arena; k.res = b->wb.res; k.hint = b->d_hint;
         if (!b->exec || !(k == b->key)) {
             drop_graph(b);
             hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
@@ -383,6 +403,7 @@ rt_status rt_batch_destroy(rt_batch *b)
         if (b->d_nodes) (void)hipFree(b->d_nodes);
         if (b->d_tris) (void)hipFree(b->d_tris);
         if (b->d_root) (void)hipFree(b->d_root);
+        if (b->d_hint) (void)hipFree(b->d_hint);
         delete b;
     }
     rt::ctx_release(ctx);

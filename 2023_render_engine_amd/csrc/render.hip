@@ -143,13 +143,10 @@ __device__ __forceinline__ void init_path(const SceneDev &sc0, const BatchDev &b
 {
     uint32_t sid = 0;
     if constexpr (B) {
-        uint32_t lo = 0, hi = bt.n;
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (bt.bases[mid] <= g) lo = mid;
-            else hi = mid;
-        }
-        sid = lo;
+        // the hint is the scene at the start of g's 2^16-path block; scenes
+        // hold at least one path, so a short forward walk ends at g's scene
+        sid = bt.hint[g >> kHintShift];
+        while (sid + 1 < bt.n && bt.bases[sid + 1] <= g) ++sid;
         g -= bt.bases[sid];
         p.sid[slot] = sid;
     }
@@ -919,7 +916,7 @@ hipError_t chunk_begin(const ChunkLaunch &c, const WaveParams &wp, hipStream_t s
     const dim3 blk(kBlock);
     const dim3 grid = queue_grid(wp);
     const SceneDev &sc = *c.scene;
-    const BatchDev bt = c.batch ? c.batch->dev : BatchDev{nullptr, nullptr, 0, nullptr, nullptr, nullptr};
+    const BatchDev bt = c.batch ? c.batch->dev : BatchDev{nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr};
     hipLaunchKernelGGL(k_reset, dim3(kNumLists + 1), dim3(kStripes), 0, s, q, 0u, 1u);
     if (c.batch) hipLaunchKernelGGL(k_regen<true>, grid, blk, 0, s, sc, bt, c.paths, q, wp, 0u, 1u, c.capacity);
     else hipLaunchKernelGGL(k_regen<false>, grid, blk, 0, s, sc, bt, c.paths, q, wp, 0u, 1u, c.capacity);
@@ -931,7 +928,7 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
 {
     const bool B = c.batch != nullptr;
     const SceneDev &sc = *c.scene;
-    const BatchDev bt = B ? c.batch->dev : BatchDev{nullptr, nullptr, 0, nullptr, nullptr, nullptr};
+    const BatchDev bt = B ? c.batch->dev : BatchDev{nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr};
     const uint32_t stack_cap = B ? c.batch->stack_cap : sc.bvh.stack_cap;
     const bool wide = sc.bvh.wide != 0;  // a batch's scenes share one layout
     const int cap = (int)stack_cap;

@@ -68,7 +68,11 @@ struct BatchDev {
     const void *nodes;
     const void *tris;
     const uint32_t *root;
+    // scene of global path index g << kHintShift: where a search for the
+    // scene of g starts (it moves forward through bases from there)
+    const uint32_t *hint;
 };
+constexpr uint32_t kHintShift = 16;
 
 // Structure-of-arrays path state (capacity P).
 struct PathsDev {

@@ -225,6 +225,16 @@ def run_render(args, which):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     samples = desc.width * desc.height * desc.spp * args.steps
+    # RMSE (the reference's computeRMSE) of the frame against an FP64
+    # high-spp render of the same scene with disjoint samples (untimed)
+    rmse = None
+    if args.ref_spp > 0:
+        ref = sc.new_film()
+        for k in range(0, args.ref_spp, desc.spp):
+            sc.render(ref, spp=desc.spp, spp_offset=(1 << 24) + k)
+        torch.cuda.synchronize()
+        ref_img = ref[0].view(-1, 3) / ref[1].double().clamp(min=1).unsqueeze(1)
+        rmse = {"ref_spp": args.ref_spp, "frame_spp": desc.spp, "rmse": round(_rmse_dev(film[0], film[1], ref_img), 6)}
     out = {
         "metric": METRIC,
         "value": round(samples / wall / 1e6, 3),
@@ -238,6 +248,7 @@ def run_render(args, which):
         "mrays_extension": round(ext / wall / 1e6, 1),
         "mrays_shadow": round(shadow / wall / 1e6, 1),
         "rays_per_sample": round((ext + shadow) / samples, 2),
+        "rmse_vs_ref": rmse,
     }
     if not args.no_cpu_baseline:
         import oracle
@@ -509,6 +520,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c2", "c3", "c4", "copy"])
     ap.add_argument("--c3-ref-spp", type=int, default=128)
+    ap.add_argument("--ref-spp", type=int, default=1024,
+                    help="c0/c2: spp of the FP64 reference the frame's RMSE is taken against (0: skip)")
     ap.add_argument("--width", type=int, default=4, choices=[2, 4],
                     help="c1: tree layout (4 = four-wide quantized nodes, 2 = two-wide FP32 nodes)")
     ap.add_argument("--c4-width", type=int, default=480)

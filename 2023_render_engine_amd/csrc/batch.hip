@@ -89,8 +89,7 @@ namespace {
 
 // Copies one scene's tree into the batch pool: node references move by
 // nbase, leaf references' first triangle by tbase.  Two-wide nodes keep
-// their children in words 12-13; four-wide nodes in words 4-7, used slots
-// marked by bits 24-27 of word 3.
+// their children in words 12-13; four-wide nodes in words 4-7.
 __global__ void k_pool_nodes(const int32_t *__restrict__ src, int32_t *__restrict__ dst, uint32_t n, int32_t nbase,
                              uint32_t tbase, uint32_t wide)
 {
@@ -106,9 +105,8 @@ __global__ void k_pool_nodes(const int32_t *__restrict__ src, int32_t *__restric
                           (enc & (rt::kMaxLeafCount - 1u)));
     };
     if (wide) {
-        const uint32_t valid = ((uint32_t)a[3] >> 24) & 0xFu;
-        for (int k = 0; k < 4; ++k)
-            if ((valid >> k) & 1u) d[4 + k] = rebase(a[4 + k]);
+        // every slot holds a reference (unused ones point at the sentinel leaf)
+        for (int k = 0; k < 4; ++k) d[4 + k] = rebase(a[4 + k]);
     } else {
         d[12] = rebase(a[12]);
         d[13] = rebase(a[13]);

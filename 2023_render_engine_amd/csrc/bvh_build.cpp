@@ -415,6 +415,12 @@ struct Collapser {
             }
         }
         need = (uint32_t)(nk - 1) + sub;
+        // unused slots: an empty box (lo byte 255 > hi byte 0 on every axis,
+        // which no ray enters) around the degenerate sentinel leaf, so even a
+        // box test that rounds an empty box open only tests a triangle that
+        // every test rejects
+        const uint32_t sentinel = (uint32_t)in.tris.size() - 1u;
+        for (int i = nk; i < 4; ++i) refs[i] = encode_leaf(sentinel, 1);
         Node4 &n = out[my];
         std::memset(&n, 0, sizeof n);
         uint32_t meta = 0;
@@ -439,6 +445,7 @@ struct Collapser {
                 qlo |= (uint32_t)l << (8 * i);
                 qhi |= (uint32_t)h << (8 * i);
             }
+            for (int i = nk; i < 4; ++i) qlo |= 255u << (8 * i);
             n.q[2 * a] = qlo;
             n.q[2 * a + 1] = qhi;
         }

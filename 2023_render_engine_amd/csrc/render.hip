@@ -149,6 +149,7 @@ __device__ __forceinline__ void init_path(const SceneDev &sc0, const BatchDev &b
         while (sid + 1 < bt.n && bt.bases[sid + 1] <= g) ++sid;
         g -= bt.bases[sid];
         p.sid[slot] = sid;
+        p.root[slot] = (int32_t)bt.root[sid];
     }
     const SceneDev &sc = B ? bt.scenes[sid] : sc0;
     const uint32_t npix = B ? sc.cam.width * sc.cam.height : wp.npix;
@@ -309,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc0, BatchDev bt, Pa
         const uint32_t r = wave_claim(c, lcount(q, kLClaimExt, 0), count, has, stripe);
         if (r != 0xffffffffu) {
             slot = active[stripe * cs + r];
-            const int32_t root = B ? (int32_t)bt.root[p.sid[slot]] : 0;
+            const int32_t root = B ? p.root[slot] : 0;
             trav_begin(s, p.ox[slot], p.oy[slot], p.oz[slot], p.dx[slot], p.dy[slot], p.dz[slot], INFINITY, false,
                        root);
             has = true;
@@ -536,6 +537,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
         if (kScatter) {
             // next-event estimation over every point light (Scene.cu:468-525)
             uint32_t emit = 0;
+            const int32_t proot = B && live ? p.root[slot] : 0;  // the shadow rays' tree root (batches)
             float *c = p.nee_c + (size_t)slot * p.nee_stride * 3;
             const uint32_t npl = live ? sc.npl : 0u, nal = live ? sc.nlights - sc.npl : 0u;
             const uint32_t npl_w = B ? wave_max(npl) : sc0.npl;
@@ -574,6 +576,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
                     sh.dx[pos] = Lw.x; sh.dy[pos] = Lw.y; sh.dz[pos] = Lw.z;
                     sh.tmax[pos] = dist;
                     sh.slot[pos] = (slot << 5) | l;
+                    if constexpr (B) sh.root[pos] = proot;
                 }
             }
             for (uint32_t a = 0; a < nal_w; ++a) {
@@ -609,6 +612,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
                     sh.dx[pos] = Lw.x; sh.dy[pos] = Lw.y; sh.dz[pos] = Lw.z;
                     sh.tmax[pos] = dist * 0.9999f;
                     sh.slot[pos] = (slot << 5) | l;
+                    if constexpr (B) sh.root[pos] = proot;
                 }
             }
             if (live) {
@@ -731,7 +735,7 @@ __global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc0, BatchDev bt, P
         if (r != 0xffffffffu) {
             const size_t e = stripe * sstride + r;
             sl = sh.slot[e];
-            const int32_t root = B ? (int32_t)bt.root[p.sid[sl >> 5]] : 0;
+            const int32_t root = B ? sh.root[e] : 0;
             trav_begin(s, sh.ox[e], sh.oy[e], sh.oz[e], sh.dx[e], sh.dy[e], sh.dz[e], sh.tmax[e], true, root);
             has = true;
         }

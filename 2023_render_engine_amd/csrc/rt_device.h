@@ -336,7 +336,7 @@ __device__ __forceinline__ bool trav_step4(TravState &s, const float4 *__restric
             const float tn = fmaxf(fmaxf(txn, tyn), fmaxf(tzn, 0.0f));
             float tf = fminf(fminf(txf, tyf), tzf);
             tf = fminf(tf * kTfarScale, tlim);
-            const bool hit = ((meta >> (24 + i)) & 1u) && tn <= tf;
+            const bool hit = tn <= tf;  // unused slots hold empty boxes
             key[i] = hit ? __float_as_uint(tn) : 0xffffffffu;
         }
         cswap_key(key[0], ref[0], key[1], ref[1]);

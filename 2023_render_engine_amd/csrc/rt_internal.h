@@ -53,6 +53,8 @@ inline int32_t encode_leaf(uint32_t first, uint32_t count)
 //   child[4]   node index, or leaf ~(first << 3 | count - 1)
 //   q[0..3]    lo_x, hi_x, lo_y, hi_y: byte i belongs to child i
 //   q[4..5]    lo_z, hi_z; q[6..7] zero
+// An unused slot holds an empty box (lo byte 255, hi byte 0) and a leaf
+// reference to the tree's degenerate sentinel triangle.
 // Child box i along x: [origin.x + lo_x[i] * sx, origin.x + hi_x[i] * sx].
 struct alignas(64) Node4 {
     float origin[3];

@@ -83,6 +83,7 @@ struct PathsDev {
     uint32_t *pix;
     uint32_t *gidx;             // path index within the chunk (result slot)
     uint32_t *sid;              // scene of the path (batched waves)
+    int32_t *root;              // root node of the path's scene in the batch's pooled tree
     uint32_t *meta;             // depth | prev_delta << 16
     float *ppdf, *ppx, *ppy, *ppz;  // previous BSDF pdf and vertex (MIS of sphere lights)
     int32_t *hid;               // >= 0 triangle, <= -2 sphere -(j+2), -1 miss
@@ -106,6 +107,7 @@ struct PathsDev {
 struct ShadowDev {
     float *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
     uint32_t *slot;             // path slot << 5 | light slot
+    int32_t *root;              // batches: root node of the ray's scene in the pooled tree
 };
 
 struct QueuesDev {

@@ -281,6 +281,7 @@ rt_status ensure_wave_buffers(WaveBuffers &w, uint32_t P, uint32_t NL, const cha
     add((void **)&p.pix, (size_t)P * 4);
     add((void **)&p.gidx, (size_t)P * 4);
     add((void **)&p.sid, (size_t)P * 4);
+    add((void **)&p.root, (size_t)P * 4);
     add((void **)&p.meta, (size_t)P * 4);
     add((void **)&p.hid, (size_t)P * 4);
     add((void **)&p.nee_mask, (size_t)P * 4);
@@ -298,6 +299,7 @@ rt_status ensure_wave_buffers(WaveBuffers &w, uint32_t P, uint32_t NL, const cha
     float **sf[] = {&sh.ox, &sh.oy, &sh.oz, &sh.dx, &sh.dy, &sh.dz, &sh.tmax};
     for (float **f : sf) add((void **)f, SQ * 4);
     add((void **)&sh.slot, SQ * 4);
+    add((void **)&sh.root, SQ * 4);
     QueuesDev &q = w.queues;
     add((void **)&q.active[0], (size_t)P * 4);
     add((void **)&q.active[1], (size_t)P * 4);

@@ -56,8 +56,16 @@ def walk4(nodes4, tris):
         depth_max = max(depth_max, dep)
         lo, hi, valid = child_boxes(nodes4[n])
         assert valid in (0b11, 0b111, 0b1111), "slots are filled from 0"
+        u = nodes4[n].copy().view(np.uint32)
         for i in range(4):
             if not (valid >> i) & 1:
+                # unused slot: an empty box (lo byte 255, hi byte 0 on every
+                # axis) around the degenerate sentinel leaf
+                for a in range(3):
+                    assert (int(u[8 + 2 * a]) >> (8 * i)) & 0xFF == 255
+                    assert (int(u[9 + 2 * a]) >> (8 * i)) & 0xFF == 0
+                enc = (~int(refs[n, i])) & 0xFFFFFFFF
+                assert int(refs[n, i]) < 0 and enc >> 3 == tris.shape[0] - 1 and enc & 7 == 0
                 continue
             c = int(refs[n, i])
             boxes = anc + [(lo[i], hi[i])]

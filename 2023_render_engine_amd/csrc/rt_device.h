@@ -182,6 +182,9 @@ constexpr int kTravBlock = 128;
 #ifndef RT_LEAF_BATCH
 #define RT_LEAF_BATCH 16
 #endif
+#ifndef RT_LEAF_BATCH_ANY
+#define RT_LEAF_BATCH_ANY 16
+#endif
 constexpr uint32_t kRefillLanes = RT_REFILL_LANES;  // refill a wavefront when at least this many lanes are idle
 
 // No node or leaf: a reference that no tree produces (a leaf reference is
@@ -293,7 +296,8 @@ __device__ __forceinline__ void cswap_key(uint32_t &ka, int32_t &ra, uint32_t &k
 // hits are unchanged (the nearest accepted triangle does not depend on the
 // order of the tests, only ties would); culling against the running hit
 // distance starts a little later, so a ray may visit a few more nodes.
-constexpr uint32_t kLeafBatch = RT_LEAF_BATCH;
+constexpr uint32_t kLeafBatch = RT_LEAF_BATCH;        // closest hit
+constexpr uint32_t kLeafBatchAny = RT_LEAF_BATCH_ANY;  // any hit (shadow rays end at their first occluder)
 
 template <bool ANY, bool STATS>
 __device__ __forceinline__ bool trav_step4(TravState &s, const float4 *__restrict__ nodes,
@@ -372,7 +376,7 @@ __device__ __forceinline__ bool trav_step4(TravState &s, const float4 *__restric
     // leaf phase of the wavefront (the lanes stepping here)
     const bool want = s.leaf != kNoRef;
     const bool stuck = want && s.cur < 0;  // a second leaf, or nothing left to walk
-    if (__ballot(stuck) != 0 || (uint32_t)__popcll(__ballot(want)) >= kLeafBatch) {
+    if (__ballot(stuck) != 0 || (uint32_t)__popcll(__ballot(want)) >= (ANY ? kLeafBatchAny : kLeafBatch)) {
         if (want) {
             uint32_t first, count;
             decode_leaf(s.leaf, first, count);

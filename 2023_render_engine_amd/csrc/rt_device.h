@@ -182,6 +182,9 @@ constexpr int kTravBlock = 128;
 #ifndef RT_LEAF_BATCH
 #define RT_LEAF_BATCH 16
 #endif
+#ifndef RT_NODE_STEPS
+#define RT_NODE_STEPS 4
+#endif
 #ifndef RT_LEAF_BATCH_ANY
 #define RT_LEAF_BATCH_ANY 16
 #endif
@@ -298,79 +301,86 @@ __device__ __forceinline__ void cswap_key(uint32_t &ka, int32_t &ra, uint32_t &k
 // distance starts a little later, so a ray may visit a few more nodes.
 constexpr uint32_t kLeafBatch = RT_LEAF_BATCH;        // closest hit
 constexpr uint32_t kLeafBatchAny = RT_LEAF_BATCH_ANY;  // any hit (shadow rays end at their first occluder)
+constexpr uint32_t kNodeSteps = RT_NODE_STEPS;         // interior-node visits per traversal step
 
 template <bool ANY, bool STATS>
 __device__ __forceinline__ bool trav_step4(TravState &s, const float4 *__restrict__ nodes,
                                            const float4 *__restrict__ tris, int32_t *stack, int cap,
                                            uint32_t &n_node, uint32_t &n_tri)
 {
-    if (s.cur >= 0) {
-        if (STATS) ++n_node;
-        const float4 *np = nodes + 4 * (size_t)s.cur;
-        const float4 a = np[0], b = np[1], c = np[2], d = np[3];
-        const float tlim = ANY ? s.tmax * kHitScale : fminf(s.h, s.tmax) * kHitScale;
-        const uint32_t meta = __float_as_uint(a.w);
-        const float Ax = __uint_as_float((meta & 0xffu) << 23) * s.br.ix;
-        const float Ay = __uint_as_float(((meta >> 8) & 0xffu) << 23) * s.br.iy;
-        const float Az = __uint_as_float(((meta >> 16) & 0xffu) << 23) * s.br.iz;
-        const float Bx = fmaf(a.x, s.br.ix, -s.br.oix);
-        const float By = fmaf(a.y, s.br.iy, -s.br.oiy);
-        const float Bz = fmaf(a.z, s.br.iz, -s.br.oiz);
-        // Near and far planes per axis by the sign of 1/d: A has that sign
-        // and fma(q, A, B) is monotone in q, so this selects exactly the
-        // min and max of the two plane distances without computing both.
-        const uint32_t qlx = __float_as_uint(c.x), qhx = __float_as_uint(c.y);
-        const uint32_t qly = __float_as_uint(c.z), qhy = __float_as_uint(c.w);
-        const uint32_t qlz = __float_as_uint(d.x), qhz = __float_as_uint(d.y);
-        const bool nx = s.br.ix < 0, ny = s.br.iy < 0, nz = s.br.iz < 0;
-        const uint32_t qnx = nx ? qhx : qlx, qfx = nx ? qlx : qhx;
-        const uint32_t qny = ny ? qhy : qly, qfy = ny ? qly : qhy;
-        const uint32_t qnz = nz ? qhz : qlz, qfz = nz ? qlz : qhz;
-        uint32_t key[4];
-        int32_t ref[4] = {__float_as_int(b.x), __float_as_int(b.y), __float_as_int(b.z), __float_as_int(b.w)};
+    // kNodeSteps interior-node visits (each followed by parking a reached
+    // leaf) per step, so the per-step refill and leaf-phase checks are paid
+    // once for several visits
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int sh = 8 * i;
-            const float txn = fmaf((float)((qnx >> sh) & 0xffu), Ax, Bx);
-            const float txf = fmaf((float)((qfx >> sh) & 0xffu), Ax, Bx);
-            const float tyn = fmaf((float)((qny >> sh) & 0xffu), Ay, By);
-            const float tyf = fmaf((float)((qfy >> sh) & 0xffu), Ay, By);
-            const float tzn = fmaf((float)((qnz >> sh) & 0xffu), Az, Bz);
-            const float tzf = fmaf((float)((qfz >> sh) & 0xffu), Az, Bz);
-            const float tn = fmaxf(fmaxf(txn, tyn), fmaxf(tzn, 0.0f));
-            float tf = fminf(fminf(txf, tyf), tzf);
-            tf = fminf(tf * kTfarScale, tlim);
-            const bool hit = tn <= tf;  // unused slots hold empty boxes
-            key[i] = hit ? __float_as_uint(tn) : 0xffffffffu;
+    for (uint32_t it = 0; it < kNodeSteps; ++it) {
+        if (s.cur >= 0) {
+            if (STATS) ++n_node;
+            const float4 *np = nodes + 4 * (size_t)s.cur;
+            const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+            const float tlim = ANY ? s.tmax * kHitScale : fminf(s.h, s.tmax) * kHitScale;
+            const uint32_t meta = __float_as_uint(a.w);
+            const float Ax = __uint_as_float((meta & 0xffu) << 23) * s.br.ix;
+            const float Ay = __uint_as_float(((meta >> 8) & 0xffu) << 23) * s.br.iy;
+            const float Az = __uint_as_float(((meta >> 16) & 0xffu) << 23) * s.br.iz;
+            const float Bx = fmaf(a.x, s.br.ix, -s.br.oix);
+            const float By = fmaf(a.y, s.br.iy, -s.br.oiy);
+            const float Bz = fmaf(a.z, s.br.iz, -s.br.oiz);
+            // Near and far planes per axis by the sign of 1/d: A has that sign
+            // and fma(q, A, B) is monotone in q, so this selects exactly the
+            // min and max of the two plane distances without computing both.
+            const uint32_t qlx = __float_as_uint(c.x), qhx = __float_as_uint(c.y);
+            const uint32_t qly = __float_as_uint(c.z), qhy = __float_as_uint(c.w);
+            const uint32_t qlz = __float_as_uint(d.x), qhz = __float_as_uint(d.y);
+            const bool nx = s.br.ix < 0, ny = s.br.iy < 0, nz = s.br.iz < 0;
+            const uint32_t qnx = nx ? qhx : qlx, qfx = nx ? qlx : qhx;
+            const uint32_t qny = ny ? qhy : qly, qfy = ny ? qly : qhy;
+            const uint32_t qnz = nz ? qhz : qlz, qfz = nz ? qlz : qhz;
+            uint32_t key[4];
+            int32_t ref[4] = {__float_as_int(b.x), __float_as_int(b.y), __float_as_int(b.z), __float_as_int(b.w)};
+    #pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int sh = 8 * i;
+                const float txn = fmaf((float)((qnx >> sh) & 0xffu), Ax, Bx);
+                const float txf = fmaf((float)((qfx >> sh) & 0xffu), Ax, Bx);
+                const float tyn = fmaf((float)((qny >> sh) & 0xffu), Ay, By);
+                const float tyf = fmaf((float)((qfy >> sh) & 0xffu), Ay, By);
+                const float tzn = fmaf((float)((qnz >> sh) & 0xffu), Az, Bz);
+                const float tzf = fmaf((float)((qfz >> sh) & 0xffu), Az, Bz);
+                const float tn = fmaxf(fmaxf(txn, tyn), fmaxf(tzn, 0.0f));
+                float tf = fminf(fminf(txf, tyf), tzf);
+                tf = fminf(tf * kTfarScale, tlim);
+                const bool hit = tn <= tf;  // unused slots hold empty boxes
+                key[i] = hit ? __float_as_uint(tn) : 0xffffffffu;
+            }
+            cswap_key(key[0], ref[0], key[1], ref[1]);
+            cswap_key(key[2], ref[2], key[3], ref[3]);
+            cswap_key(key[0], ref[0], key[2], ref[2]);
+            cswap_key(key[1], ref[1], key[3], ref[3]);
+            cswap_key(key[1], ref[1], key[2], ref[2]);
+            if (key[0] != 0xffffffffu) {
+    #pragma unroll
+                for (int i = 3; i >= 1; --i)
+                    if (key[i] != 0xffffffffu && s.sp < cap) {
+                        stack[s.sp * kTravBlock] = ref[i];
+                        ++s.sp;
+                    }
+                s.cur = ref[0];
+            } else if (s.sp > 0) {
+                --s.sp;
+                s.cur = stack[s.sp * kTravBlock];
+            } else {
+                s.cur = kNoRef;
+            }
         }
-        cswap_key(key[0], ref[0], key[1], ref[1]);
-        cswap_key(key[2], ref[2], key[3], ref[3]);
-        cswap_key(key[0], ref[0], key[2], ref[2]);
-        cswap_key(key[1], ref[1], key[3], ref[3]);
-        cswap_key(key[1], ref[1], key[2], ref[2]);
-        if (key[0] != 0xffffffffu) {
-#pragma unroll
-            for (int i = 3; i >= 1; --i)
-                if (key[i] != 0xffffffffu && s.sp < cap) {
-                    stack[s.sp * kTravBlock] = ref[i];
-                    ++s.sp;
-                }
-            s.cur = ref[0];
-        } else if (s.sp > 0) {
-            --s.sp;
-            s.cur = stack[s.sp * kTravBlock];
-        } else {
-            s.cur = kNoRef;
-        }
-    }
-    // park a reached leaf while the slot is free and walk on
-    if (s.cur < 0 && s.cur != kNoRef && s.leaf == kNoRef) {
-        s.leaf = s.cur;
-        if (s.sp > 0) {
-            --s.sp;
-            s.cur = stack[s.sp * kTravBlock];
-        } else {
-            s.cur = kNoRef;
+        // park a reached leaf while the slot is free and walk on
+        if (s.cur < 0 && s.cur != kNoRef && s.leaf == kNoRef) {
+            s.leaf = s.cur;
+            if (s.sp > 0) {
+                --s.sp;
+                s.cur = stack[s.sp * kTravBlock];
+            } else {
+                s.cur = kNoRef;
+            }
         }
     }
     // leaf phase of the wavefront (the lanes stepping here)

@@ -217,10 +217,11 @@ rt_status rt_bvh_build(rt_ctx *ctx, const float *xyz, uint32_t nv, const uint32_
         return RT_ERR_INVALID;
     }
     rt::build_bvh4(b->host);
-    // The traversal stack lives in LDS: refuse trees too deep for it.
-    if (rt::trace_lds_bytes(stack_cap(b)) > 64 * 1024) {
+    // The kernels hold a traversal stack of kMaxTraversalStack entries per
+    // lane: refuse trees whose pending-entry bound exceeds it.
+    if (stack_cap(b) > rt::kMaxTraversalStack) {
         delete b;
-        set_error("rt_bvh_build: tree depth exceeds the LDS traversal stack");
+        set_error("rt_bvh_build: tree depth exceeds the traversal stack");
         return RT_ERR_UNSUPPORTED;
     }
     if (!ctx) {  // host-only tree: info and export work, traces are refused

@@ -352,7 +352,7 @@ rt_status rt_batch_create(rt_ctx *ctx, rt_scene *const *scenes, uint32_t n, rt_b
         b->max_lights = std::max(b->max_lights, d.nlights);
         b->paths_per_sample += (uint64_t)d.cam.width * d.cam.height;
     }
-    if (rt::extend_lds_bytes(b->stack_cap) > 64 * 1024) {
+    if (b->stack_cap > rt::kMaxTraversalStack) {
         delete b;
         set_error("rt_batch_create: BVH too deep");
         return RT_ERR_UNSUPPORTED;

@@ -293,7 +293,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc0, BatchDev bt, Pa
                                                    int cap)
 {
     extern __shared__ int32_t s_stack[];
-    int32_t *stack = s_stack + threadIdx.x;
+    StackTail<PathStack> s_tail;
+    const PathStack stack{s_stack + threadIdx.x, s_tail};
     const uint32_t *active = q.active[cur];
     const uint32_t cs = q.cs;
     auto count = [&](uint32_t j) { return *lcount(q, kLActive0 + cur, j); };
@@ -719,7 +720,8 @@ __global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc0, BatchDev bt, P
                                                     int cap)
 {
     extern __shared__ int32_t s_stack[];
-    int32_t *stack = s_stack + threadIdx.x;
+    StackTail<PathStack> s_tail;
+    const PathStack stack{s_stack + threadIdx.x, s_tail};
     const size_t sstride = (size_t)q.cs * p.nee_stride;  // shadow rays per stripe
     auto count = [&](uint32_t j) { return *lcount(q, kLShadow, j); };
     StripeCursor c;
@@ -878,7 +880,10 @@ hipError_t set_u64(uint64_t *p, uint64_t v, hipStream_t s)
     return hipGetLastError();
 }
 
-size_t extend_lds_bytes(uint32_t stack_cap) { return (size_t)stack_cap * kBlock * sizeof(int32_t); }
+size_t extend_lds_bytes(uint32_t stack_cap)
+{
+    return stack_lds_bytes<PathStack>(stack_cap, kBlock);
+}
 
 // Workgroups of a persistent traversal launch: as many as fit on the chip
 // (LDS-limited, at most 16 per CU).

@@ -166,10 +166,10 @@ __device__ __forceinline__ void decode_leaf(int32_t c, uint32_t &first, uint32_t
 // ---------------------------------------------------------------------
 // Persistent traversal state machine, shared by the ray-query kernels and
 // the renderer's extend / connect kernels.  Each lane walks its ray one node
-// visit (or one leaf) per step; the stack lives in LDS laid out
-// [depth][lane] with a stride of kTravBlock entries, so a wavefront's pushes
-// hit distinct banks.  Its depth is the tree's internal depth + 1, which
-// bounds the pending entries, so it cannot overflow.
+// visit (or one leaf) per step; the stack's first entries live in LDS laid
+// out [depth][lane] with a stride of kTravBlock entries, so a wavefront's
+// pushes hit distinct banks.  Its depth is the tree's internal depth + 1,
+// which bounds the pending entries, so it cannot overflow.
 // ---------------------------------------------------------------------
 constexpr int kTravBlock = 128;
 // Tuning constants of the persistent traversal (build-time, -D overridable).
@@ -189,6 +189,66 @@ constexpr int kTravBlock = 128;
 #define RT_LEAF_BATCH_ANY 16
 #endif
 constexpr uint32_t kRefillLanes = RT_REFILL_LANES;  // refill a wavefront when at least this many lanes are idle
+
+// The traversal stack of a lane: its first L entries in LDS ([depth][lane],
+// stride kTravBlock), deeper ones (up to kMaxStack, the most pending entries
+// a tree the host accepts can need) in the lane's private memory.  The ray
+// tracing kernel keeps all of it in LDS (L = kMaxStack: no private part), as
+// its one-block-size launch is not LDS-limited at the depths real trees need;
+// the path tracer's extend and connect kernels keep the first 24 entries in
+// LDS (a 1M-triangle four-wide tree needs 36 but its rays rarely use more than
+// ~20), which leaves room for more workgroups per CU; 24 measured best
+// against 16, 20, 28 and 36 on the 1080p procedural scene.
+#ifndef RT_TRACE_LDS_STACK
+#define RT_TRACE_LDS_STACK 128
+#endif
+#ifndef RT_PATH_LDS_STACK
+#define RT_PATH_LDS_STACK 24
+#endif
+constexpr int kMaxStack = (int)kMaxTraversalStack;  // the host refuses trees that need more
+
+template <int L>
+struct TravStack {
+    static_assert(L > 0 && L <= kMaxStack, "the LDS part holds 1..kMaxStack entries");
+    static constexpr int kLds = L;
+    static constexpr int kTail = kMaxStack - L;  // private entries
+    int32_t *lds;   // this lane's column of the workgroup's LDS stack
+    int32_t *tail;  // this lane's private overflow entries (kTail of them)
+};
+using TraceStack = TravStack<RT_TRACE_LDS_STACK>;
+using PathStack = TravStack<RT_PATH_LDS_STACK>;
+
+// The private part's storage in a kernel (one entry when there is none, as
+// an array cannot be empty; it is then never touched).
+template <class S>
+using StackTail = int32_t[S::kTail > 0 ? S::kTail : 1];
+
+template <class S>
+__device__ __forceinline__ void stack_put(const S &st, int sp, int32_t v)
+{
+    if constexpr (S::kTail == 0) st.lds[sp * kTravBlock] = v;
+    else if (sp < S::kLds) st.lds[sp * kTravBlock] = v;
+    else st.tail[sp - S::kLds] = v;
+}
+
+template <class S>
+__device__ __forceinline__ int32_t stack_get(const S &st, int sp)
+{
+    if constexpr (S::kTail == 0) return st.lds[sp * kTravBlock];
+    else {
+        if (sp < S::kLds) return st.lds[sp * kTravBlock];
+        return st.tail[sp - S::kLds];
+    }
+}
+
+// LDS bytes of a workgroup of `block` lanes whose tree needs `stack_cap`
+// entries per lane.
+template <class S>
+inline size_t stack_lds_bytes(uint32_t stack_cap, uint32_t block)
+{
+    const uint32_t lds = stack_cap < (uint32_t)S::kLds ? stack_cap : (uint32_t)S::kLds;
+    return (size_t)lds * block * sizeof(int32_t);
+}
 
 // No node or leaf: a reference that no tree produces (a leaf reference is
 // ~(first << 3 | count - 1) with first + count <= 2^28).
@@ -224,9 +284,9 @@ __device__ __forceinline__ void trav_begin(TravState &s, float ox, float oy, flo
 }
 
 // One traversal step.  Returns true when the ray is finished.
-template <bool ANY, bool STATS>
+template <bool ANY, bool STATS, class Stack>
 __device__ __forceinline__ bool trav_step(TravState &s, const float4 *__restrict__ nodes,
-                                          const float4 *__restrict__ tris, int32_t *stack, int cap,
+                                          const float4 *__restrict__ tris, const Stack &stack, int cap,
                                           uint32_t &n_node, uint32_t &n_tri)
 {
     if (s.cur >= 0) {
@@ -241,7 +301,7 @@ __device__ __forceinline__ bool trav_step(TravState &s, const float4 *__restrict
         if (h0 && h1) {
             const bool swap = t1 < t0;
             if (s.sp < cap) {
-                stack[s.sp * kTravBlock] = swap ? c0 : c1;
+                stack_put(stack, s.sp, swap ? c0 : c1);
                 ++s.sp;
             }
             s.cur = swap ? c1 : c0;
@@ -271,7 +331,7 @@ __device__ __forceinline__ bool trav_step(TravState &s, const float4 *__restrict
     }
     if (s.sp > 0) {
         --s.sp;
-        s.cur = stack[s.sp * kTravBlock];
+        s.cur = stack_get(stack, s.sp);
         return false;
     }
     return true;
@@ -303,9 +363,9 @@ constexpr uint32_t kLeafBatch = RT_LEAF_BATCH;        // closest hit
 constexpr uint32_t kLeafBatchAny = RT_LEAF_BATCH_ANY;  // any hit (shadow rays end at their first occluder)
 constexpr uint32_t kNodeSteps = RT_NODE_STEPS;         // interior-node visits per traversal step
 
-template <bool ANY, bool STATS>
+template <bool ANY, bool STATS, class Stack>
 __device__ __forceinline__ bool trav_step4(TravState &s, const float4 *__restrict__ nodes,
-                                           const float4 *__restrict__ tris, int32_t *stack, int cap,
+                                           const float4 *__restrict__ tris, const Stack &stack, int cap,
                                            uint32_t &n_node, uint32_t &n_tri)
 {
     // kNodeSteps interior-node visits (each followed by parking a reached
@@ -361,13 +421,13 @@ __device__ __forceinline__ bool trav_step4(TravState &s, const float4 *__restric
     #pragma unroll
                 for (int i = 3; i >= 1; --i)
                     if (key[i] != 0xffffffffu && s.sp < cap) {
-                        stack[s.sp * kTravBlock] = ref[i];
+                        stack_put(stack, s.sp, ref[i]);
                         ++s.sp;
                     }
                 s.cur = ref[0];
             } else if (s.sp > 0) {
                 --s.sp;
-                s.cur = stack[s.sp * kTravBlock];
+                s.cur = stack_get(stack, s.sp);
             } else {
                 s.cur = kNoRef;
             }
@@ -377,7 +437,7 @@ __device__ __forceinline__ bool trav_step4(TravState &s, const float4 *__restric
             s.leaf = s.cur;
             if (s.sp > 0) {
                 --s.sp;
-                s.cur = stack[s.sp * kTravBlock];
+                s.cur = stack_get(stack, s.sp);
             } else {
                 s.cur = kNoRef;
             }
@@ -413,9 +473,9 @@ __device__ __forceinline__ bool trav_step4(TravState &s, const float4 *__restric
 }
 
 // Dispatch on the tree layout.
-template <bool WIDE, bool ANY, bool STATS>
+template <bool WIDE, bool ANY, bool STATS, class Stack>
 __device__ __forceinline__ bool trav_step_any(TravState &s, const float4 *__restrict__ nodes,
-                                              const float4 *__restrict__ tris, int32_t *stack, int cap,
+                                              const float4 *__restrict__ tris, const Stack &stack, int cap,
                                               uint32_t &n_node, uint32_t &n_tri)
 {
     if constexpr (WIDE) return trav_step4<ANY, STATS>(s, nodes, tris, stack, cap, n_node, n_tri);

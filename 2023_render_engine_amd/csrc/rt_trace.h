@@ -12,6 +12,9 @@ namespace rt {
 constexpr uint32_t kStripes = 64;
 constexpr uint32_t kCtrStride = 32;
 constexpr uint32_t kClaimWords = kStripes * kCtrStride;  // one striped counter set
+// Largest traversal-stack bound (pending entries per lane) the kernels hold:
+// the first entries in LDS, the rest in the lane's private memory.
+constexpr uint32_t kMaxTraversalStack = 128;
 
 struct RaysDev {
     const float *ox, *oy, *oz, *dx, *dy, *dz, *tmax;

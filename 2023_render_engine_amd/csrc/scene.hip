@@ -495,7 +495,7 @@ rt_status rt_scene_commit(rt_scene *s)
     d.nlights = d.npl + na;
     d.nmat = (uint32_t)s->mats.size();
     if (d.nlights > 32) { set_error("rt_scene_commit: more than 32 light slots"); return RT_ERR_UNSUPPORTED; }
-    if (rt::extend_lds_bytes(d.bvh.stack_cap) > 64 * 1024) { set_error("rt_scene_commit: BVH too deep"); return RT_ERR_UNSUPPORTED; }
+    if (d.bvh.stack_cap > rt::kMaxTraversalStack) { set_error("rt_scene_commit: BVH too deep"); return RT_ERR_UNSUPPORTED; }
     s->committed = true;
     return RT_OK;
 }

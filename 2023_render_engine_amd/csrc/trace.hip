@@ -44,7 +44,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(const float4 *__restrict_
                                                        unsigned long long *__restrict__ counters)
 {
     extern __shared__ int32_t s_stack[];
-    int32_t *stack = s_stack + threadIdx.x;
+    StackTail<TraceStack> s_tail;
+    const TraceStack stack{s_stack + threadIdx.x, s_tail};
     // the rays are dealt to the stripes in chunks of 2^lc consecutive rays
     // (chunk c to stripe c % kStripes), so every stripe holds a sample of
     // the whole batch and the stripes take about equally long
@@ -109,7 +110,7 @@ hipError_t launch_trace(const DevBvh &bvh, const RaysDev &rays, uint32_t n, int3
     // stripe chunk 2^lc: a power of two in [64, 4096], about n / 2048 rays
     uint32_t lc = 6;
     while (lc < 12 && ((uint64_t)2048 << lc) <= n) ++lc;
-    const size_t lds = (size_t)cap * kTraceBlock * sizeof(int32_t);
+    const size_t lds = trace_lds_bytes(bvh.stack_cap);
     // persistent launch: as many workgroups as fit (LDS-limited, at most 16
     // per CU), never more than the rays need
     size_t per_cu = lds ? (160u * 1024u) / lds : 16;
@@ -148,7 +149,7 @@ hipError_t clear_claims(uint32_t *ctr, hipStream_t stream)
 
 size_t trace_lds_bytes(uint32_t stack_cap)
 {
-    return (size_t)stack_cap * kTraceBlock * sizeof(int32_t);
+    return stack_lds_bytes<TraceStack>(stack_cap, kTraceBlock);
 }
 
 }  // namespace rt

@@ -179,6 +179,9 @@ constexpr int kTravBlock = 128;
 #ifndef RT_CLAIM_CHUNK
 #define RT_CLAIM_CHUNK 128
 #endif
+#ifndef RT_CLAIM_MIN
+#define RT_CLAIM_MIN 8
+#endif
 #ifndef RT_LEAF_BATCH
 #define RT_LEAF_BATCH 16
 #endif
@@ -497,15 +500,24 @@ __device__ __forceinline__ bool trav_step_any(TravState &s, const float4 *__rest
 static_assert(kStripes == 64, "stripe scans use one wavefront lane per stripe");
 
 // A persistent wavefront's position in the stripes (wavefront-uniform).
-// Claims take kClaimChunk items at once; the wavefront hands them to its
-// idle lanes from the local range [lb, le) of stripe ls, so partial refills
-// cost no atomic.
+// A claim takes, by guided self-scheduling, the items left in the stripe
+// (as last seen) over the launch's wavefronts per stripe, rounded down to a
+// power-of-two share and held between kClaimMin and kClaimChunk: a large
+// launch claims kClaimChunk at a time, while a small one, and the last items
+// of every stripe, spread over as many wavefronts as the items allow instead
+// of queueing two or more rays on each lane of a few wavefronts.  The
+// wavefront hands claimed items to its idle lanes from the local range
+// [lb, le) of stripe ls, so partial refills cost no atomic.
 constexpr uint32_t kClaimChunk = RT_CLAIM_CHUNK;
+constexpr uint32_t kClaimMin = RT_CLAIM_MIN;
+static_assert(kClaimMin >= 1 && kClaimMin <= kClaimChunk, "claim sizes run from kClaimMin to kClaimChunk");
 
 struct StripeCursor {
     uint32_t s;       // stripe being claimed from
     uint32_t n;       // items in that stripe
     uint32_t salt;    // per-wavefront pseudo-random state for picking a stripe to help
+    uint32_t left;    // items of stripe s not yet claimed, as last seen (an upper bound)
+    uint32_t wlog;    // log2 of the launch's wavefronts per stripe (rounded down)
     uint32_t ls, lb, le;  // claimed, not yet handed out: items [lb, le) of stripe ls
     bool empty;       // all of stripe s is claimed: scan before the next claim
     bool done;        // every stripe is claimed
@@ -520,6 +532,9 @@ __device__ __forceinline__ void cursor_init(StripeCursor &c, uint32_t home, Coun
 {
     c.s = home;
     c.n = count(home);
+    c.left = c.n;
+    const uint32_t per_stripe = (gridDim.x * (blockDim.x >> 6)) / kStripes;
+    c.wlog = per_stripe ? 31u - (uint32_t)__clz(per_stripe) : 0u;
     c.salt = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 0x9E3779B1u + 0x7F4A7C15u;
     c.ls = home;
     c.lb = 0;
@@ -560,14 +575,22 @@ __device__ __forceinline__ bool cursor_scan(StripeCursor &c, uint32_t *fetch, Co
     const uint32_t t = (uint32_t)(__ffsll((unsigned long long)(sel ? sel : left)) - 1);
     c.s = t;
     c.n = (uint32_t)__shfl((int)n, (int)t);
+    c.left = c.n - (uint32_t)__shfl((int)claimed, (int)t);
     c.empty = false;
     return true;
+}
+
+// Items of the next claim (wavefront-uniform).
+__device__ __forceinline__ uint32_t claim_size(const StripeCursor &c)
+{
+    const uint32_t k = c.left >> c.wlog;
+    return k < kClaimMin ? kClaimMin : (k > kClaimChunk ? kClaimChunk : k);
 }
 
 // Wavefront-uniform refill of idle lanes: when at least kRefillLanes lanes
 // are idle (or all are), they take consecutive items of the local range;
 // an empty local range is first refilled with one atomic claim of
-// kClaimChunk items from the cursor's stripe.  Returns the lane's item
+// claim_size items from the cursor's stripe.  Returns the lane's item
 // offset within `stripe`, or UINT32_MAX.  The claim loop is
 // wavefront-uniform and ends: every turn either claims items or finds a
 // stripe drained, and claims are finite.
@@ -584,8 +607,9 @@ __device__ __forceinline__ uint32_t wave_claim(StripeCursor &c, uint32_t *fetch,
         const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)idle) - 1);
         for (;;) {
             if (c.empty && !cursor_scan(c, fetch, count)) return 0xffffffffu;
+            const uint32_t want = claim_size(c);
             uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(fetch + c.s * kCtrStride, kClaimChunk);
+            if (lane == leader) base = atomicAdd(fetch + c.s * kCtrStride, want);
             base = (uint32_t)__shfl((int)base, (int)leader);
             if (base >= c.n) {
                 c.empty = true;
@@ -593,7 +617,8 @@ __device__ __forceinline__ uint32_t wave_claim(StripeCursor &c, uint32_t *fetch,
             }
             c.ls = c.s;
             c.lb = base;
-            c.le = c.n - base > kClaimChunk ? base + kClaimChunk : c.n;
+            c.le = c.n - base > want ? base + want : c.n;
+            c.left = c.n - c.le;
             if (c.le == c.n) c.empty = true;
             break;
         }

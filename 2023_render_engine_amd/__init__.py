@@ -49,7 +49,7 @@ MAT_LAMBERTIAN, MAT_DIELECTRIC, MAT_TROWBRIDGE, MAT_TROWBRIDGE_BASIC, MAT_LIGHT 
 
 class Material(C.Structure):
     _fields_ = [("type", C.c_uint32), ("kd", C.c_float * 3), ("ior", C.c_float), ("roughness", C.c_float),
-                ("eta", C.c_float * 3), ("k", C.c_float * 3), ("intensity", C.c_float)]
+                ("eta", C.c_float * 3), ("k", C.c_float * 3), ("intensity", C.c_float), ("ke", C.c_float * 3)]
 
 
 class PointLight(C.Structure):

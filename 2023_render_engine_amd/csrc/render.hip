@@ -178,7 +178,9 @@ __device__ __forceinline__ void init_path(const SceneDev &sc0, const BatchDev &b
     p.rng[slot] = r.state;
     p.pix[slot] = pix;
     p.gidx[slot] = k;
-    p.meta[slot] = 1u << 16;  // depth 0, camera ray counts as delta for MIS
+    // depth 0; a camera ray counts as delta for MIS (bit 16) and is a camera
+    // ray (bit 17, Ray::cameraRay)
+    p.meta[slot] = (1u << 16) | (1u << 17);
     p.ppdf[slot] = 0;
     p.ppx[slot] = o.x; p.ppy[slot] = o.y; p.ppz[slot] = o.z;
     p.nee_mask[slot] = 0;
@@ -277,8 +279,10 @@ __device__ __forceinline__ uint32_t classify(const SceneDev &sc, int32_t hid, bo
     if (cache_depth_reached && cacheable(sc.mat[mi])) return kQCache;
     const uint32_t t = sc.mat[mi].type;
     if (t == RT_MAT_LAMBERTIAN) return kQLambert;
-    if (t == RT_MAT_DIELECTRIC) return kQDielectric;
-    if (t == RT_MAT_LIGHT) return kQLight;
+    // the specular non-scattering class: dielectric and mirror
+    if (t == RT_MAT_DIELECTRIC || t == RT_MAT_MIRROR) return kQDielectric;
+    // the emissive terminal class: light and plain (both end the path)
+    if (t == RT_MAT_LIGHT || t == RT_MAT_PLAIN) return kQLight;
     return kQGgx;
 }
 
@@ -622,7 +626,12 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
             }
         }
         if (live && CLS != kQMiss) {
-            if (CLS == kQLight) {
+            if (CLS == kQLight && m.type == RT_MAT_PLAIN) {
+                // PlainMaterial::getEmittedRadiance (PlainMaterial.cu:62-75):
+                // kd towards camera rays, ke towards the others
+                const float *e = ((meta >> 17) & 1u) ? m.kd : m.ke;
+                L = add(L, mk(beta.x * e[0], beta.y * e[1], beta.z * e[2]));
+            } else if (CLS == kQLight) {
                 // getEmittedRadiance (LightMaterial.cu:20-23), MIS-weighted
                 // against the light's own next-event sampling
                 float w = 1;
@@ -661,7 +670,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
                 p.dx[slot] = ww.x; p.dy[slot] = ww.y; p.dz[slot] = ww.z;
                 p.ppdf[slot] = pdf;
                 p.ppx[slot] = s.p.x; p.ppy[slot] = s.p.y; p.ppz[slot] = s.p.z;
-                p.meta[slot] = (depth + 1) | ((delta ? 1u : 0u) << 16);
+                // a camera ray stays one through specular hits (dielectric,
+                // mirror, Trowbridge: SurfaceInteraction.cu:252-254)
+                const uint32_t camera = (CLS != kQLambert) ? (meta >> 17) & 1u : 0u;
+                p.meta[slot] = (depth + 1) | ((delta ? 1u : 0u) << 16) | (camera << 17);
             }
             p.bx[slot] = beta.x; p.by[slot] = beta.y; p.bz[slot] = beta.z;
             p.lx[slot] = L.x; p.ly[slot] = L.y; p.lz[slot] = L.z;

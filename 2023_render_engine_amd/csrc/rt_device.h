@@ -65,8 +65,16 @@ __device__ __forceinline__ TriRay make_tri_ray(float ox, float oy, float oz, flo
 __device__ __forceinline__ float fmax3(float a, float b, float c) { return fmaxf(a, fmaxf(b, c)); }
 
 // Triangle::intersects (shapes/Triangle.cu:433-643), FP32 with the FP64
-// edge-function fallback.  hitOffset is lowered before the deltaT rejection
-// exactly as in the reference (:569-593).  Returns true on acceptance.
+// edge-function fallback.  One step is reordered: the conservative deltaT
+// rejection (:575-593) comes before hitOffset is lowered (:569-573), so a
+// rejected candidate -- typically the triangle a spawned ray just left, met
+// again at t ~ 0 -- leaves hitOffset alone and the result is the closest
+// candidate that passes, in any visit order.  In the reference's order that
+// candidate lowers hitOffset first and every triangle tested after it is
+// rejected, so the outcome depends on the tree's visit order; a nearest-first
+// walk would meet it first almost always and lose the bounce.  Rays with no
+// such candidate get the reference's result unchanged.  Returns true on
+// acceptance.
 __device__ __forceinline__ bool tri_intersects(const TriRay &r, const float4 a, const float4 b,
                                                const float4 c, float &hitOffset)
 {
@@ -109,8 +117,7 @@ __device__ __forceinline__ bool tri_intersects(const TriRay &r, const float4 a, 
     float invDet = 1 / det;
     float t = scaled * invDet;
     if (t > hitOffset) return false;
-    hitOffset = t;
-    // conservative deltaT rejection (:575-593)
+    // conservative deltaT rejection (:575-593), before hitOffset is lowered
     float maxZt = fmax3(fabsf(p0z), fabsf(p1z), fabsf(p2z));
     float deltaZ = error_bound(3) * maxZt;
     float maxXt = fmax3(fabsf(p0x), fabsf(p1x), fabsf(p2x));
@@ -120,7 +127,9 @@ __device__ __forceinline__ bool tri_intersects(const TriRay &r, const float4 a, 
     float deltaE = 2 * (error_bound(2) * maxXt * maxYt + deltaY * maxXt + deltaX * maxYt);
     float maxE = fmax3(fabsf(e0), fabsf(e1), fabsf(e2));
     float deltaT = 3 * (error_bound(3) * maxE * maxZt + deltaE * maxZt + deltaZ * maxE) * fabsf(invDet);
-    return !(hitOffset <= deltaT);
+    if (t <= deltaT) return false;
+    hitOffset = t;
+    return true;
 }
 
 // Slab-test constants.  A zero direction component is replaced by +-1e-30

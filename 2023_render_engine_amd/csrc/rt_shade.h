@@ -314,7 +314,9 @@ __device__ __forceinline__ float bsdf_pdf(const rt_material &m, V3 wo, V3 wi)
 }
 
 // Material::sampleDirection.  Draws: Lambertian 2, dielectric 1, Trowbridge
-// 2 (0 when effectively smooth), light 0.
+// 2 (0 when effectively smooth), light, mirror and plain 0.  The dielectric
+// instantiation also serves mirrors (both are the specular, non-scattering
+// class).
 template <uint32_t TYPE>
 __device__ __forceinline__ V3 bsdf_sample(const rt_material &m, V3 wo, Pcg32 &g, V3 &wi, float &pdf, bool &delta)
 {
@@ -328,6 +330,11 @@ __device__ __forceinline__ V3 bsdf_sample(const rt_material &m, V3 wo, Pcg32 &g,
     }
     if (TYPE == RT_MAT_DIELECTRIC) {
         delta = true;
+        if (m.type == RT_MAT_MIRROR) {
+            // PerfectReflectionBSDF::sampleDirection: weight = colour, no draw
+            wi = mk(-wo.x, -wo.y, wo.z);
+            return mk(m.kd[0], m.kd[1], m.kd[2]);
+        }
         float u = pcg32_float(g);
         if (u < 0.7f) {  // DielectricMaterial.cu:69-76, no 1/0.7 compensation
             float nO = 1, nD = m.ior;

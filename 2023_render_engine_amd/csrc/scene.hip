@@ -376,7 +376,7 @@ rt_status rt_scene_destroy(rt_scene *s)
 rt_status rt_scene_add_material(rt_scene *s, const rt_material *m, uint32_t *id)
 {
     if (!s || !m) { set_error("rt_scene_add_material: null argument"); return RT_ERR_INVALID; }
-    if (m->type > RT_MAT_LIGHT) { set_error("rt_scene_add_material: unknown material type"); return RT_ERR_INVALID; }
+    if (m->type > RT_MAT_PLAIN) { set_error("rt_scene_add_material: unknown material type"); return RT_ERR_INVALID; }
     if (m->type == RT_MAT_DIELECTRIC && !(m->ior > 0)) { set_error("rt_scene_add_material: ior must be > 0"); return RT_ERR_INVALID; }
     if ((m->type == RT_MAT_TROWBRIDGE || m->type == RT_MAT_TROWBRIDGE_BASIC) && !(m->roughness >= 0)) {
         set_error("rt_scene_add_material: roughness must be >= 0");

@@ -13,8 +13,9 @@ Rx(-ax) (0, 0, -1) and rotates the camera's up by the same matrix.  `newmtl`
 does not reset Ke, sigmaS, sigmaA or g; an mtllib's materials never replace
 a name already known; every `o` line advances a mesh counter that starts at
 -1.  Malformed input raises ValueError with the file and line; a material
-that the GPU path does not render raises UnsupportedMaterial when a scene is
-made from it."""
+that the GPU path does not render (Phong, Beckmann, subsurface, textures,
+anisotropic roughness, shadow-transparent Lambertian or Plain) raises
+UnsupportedMaterial when a scene is made from it."""
 from __future__ import annotations
 
 import math
@@ -29,7 +30,7 @@ from .scenes import SceneDesc
 ILLUM_CLASSES = ("PlainMaterial", "LambertianMaterial", "PhongMaterial", "DielectricMaterial", "BeckmannMaterial",
                  "TrowbridgeMaterial", "TrowbridgeBasicMaterial", "SSMaterial")
 # rt_material_type values of include/rt.h
-_LAMBERTIAN, _DIELECTRIC, _TROWBRIDGE, _TROWBRIDGE_BASIC = 0, 1, 2, 3
+_LAMBERTIAN, _DIELECTRIC, _TROWBRIDGE, _TROWBRIDGE_BASIC, _PLAIN = 0, 1, 2, 3, 6
 
 
 class UnsupportedMaterial(ValueError):
@@ -324,13 +325,18 @@ def load_obejota(path: str) -> ObejotaScene:
 
 def material_desc(m: MtlMaterial) -> dict:
     """The SceneDesc material an MTL block stands for, as the GPU path renders
-    it: illum 1 Lambertian(Kd), 3 Dielectric(1, Ni), 5 Trowbridge(Kd, Pr,
-    eta, k), 6 TrowbridgeBasic(Kd, Pr).  Raises UnsupportedMaterial for the
-    others."""
+    it: illum 0 Plain(Kd, Ke), 1 Lambertian(Kd), 3 Dielectric(1, Ni),
+    5 Trowbridge(Kd, Pr, eta, k), 6 TrowbridgeBasic(Kd, Pr).  Raises
+    UnsupportedMaterial for the others."""
     def refuse(why):
         raise UnsupportedMaterial(f"material {m.name} (illum {m.illum}, {ILLUM_CLASSES[m.illum]}): {why}")
     if m.map_kd and m.illum != 3:
         refuse("textures are not rendered on the GPU path")
+    if m.illum == 0:
+        if m.transparent:
+            refuse("a Plain material without shadows is not rendered on the GPU path")
+        return dict(type=_PLAIN, kd=tuple(m.kd), ior=1.5, roughness=0.0, eta=(1, 1, 1), k=(0, 0, 0), intensity=0.0,
+                    ke=tuple(m.ke))
     if m.illum == 1:
         if m.transparent:
             refuse("a Lambertian material without shadows is not rendered on the GPU path")

@@ -46,7 +46,8 @@ class SceneDesc:
         out = []
         for m in self.materials:
             out.append(Material(m["type"], f3(m.get("kd", (0, 0, 0))), m.get("ior", 1.5), m.get("roughness", 0.0),
-                                f3(m.get("eta", (1, 1, 1))), f3(m.get("k", (0, 0, 0))), m.get("intensity", 0.0)))
+                                f3(m.get("eta", (1, 1, 1))), f3(m.get("k", (0, 0, 0))), m.get("intensity", 0.0),
+                                f3(m.get("ke", (0, 0, 0)))))
         return out
 
     def camera_struct(self):
@@ -60,9 +61,9 @@ class SceneDesc:
         return Scene(ctx, self)
 
 
-def _mat(type_, kd=(0, 0, 0), ior=1.5, roughness=0.0, eta=(1, 1, 1), k=(0, 0, 0), intensity=0.0):
+def _mat(type_, kd=(0, 0, 0), ior=1.5, roughness=0.0, eta=(1, 1, 1), k=(0, 0, 0), intensity=0.0, ke=(0, 0, 0)):
     return dict(type=type_, kd=tuple(float(x) for x in kd), ior=float(ior), roughness=float(roughness),
-                eta=tuple(eta), k=tuple(k), intensity=float(intensity))
+                eta=tuple(eta), k=tuple(k), intensity=float(intensity), ke=tuple(float(x) for x in ke))
 
 
 def quad(p0, p1, p2, p3):

@@ -288,9 +288,14 @@ def run_c4(args):
 
     Frames are 4 spp each.  Plain: frames accumulate until the image's RMSE
     against the reference reaches the target.  Cache: every frame first
-    takes one training step on 65,536 training paths (rt_cache_train_scene,
+    takes one training step on 65,536 radiance samples (rt_cache_train_scene,
     cache after 2 bounces), then renders its 4 spp with paths ending in the
-    cache (rt_render_cached); the first `warm` frames only train.  The timed
+    cache (rt_render_cached); the first `warm` frames only train.  Only the
+    training paths that reach a cacheable vertex at the cache depth yield a
+    sample (about one in ten here: most leave the open scene first), so the
+    first step traces 65,536 paths to measure the yield and later steps
+    trace enough paths for 65,536 samples (the excess over the batch is
+    dropped).  The timed
     region holds every render and training call (device-synchronised per
     frame); the RMSE evaluation between frames is not timed.  The reference
     is the plain renderer with FP64 accumulation at ref_spp (its own
@@ -305,7 +310,7 @@ def run_c4(args):
     log(f"generating the procedural scene at {W}x{H}")
     desc = scenes.procedural(width=W, height=H, spp=4)
     label = (f"radiance cache on the procedural scene ({desc.idx.shape[0]} triangles, 8 point lights + 1 sphere "
-             f"light, depth 16, RR after 3) at {W}x{H}: 4 spp frames, 65536 training paths/frame, cache after "
+             f"light, depth 16, RR after 3) at {W}x{H}: 4 spp frames, 65536 training samples/frame, cache after "
              f"{args.c4_cache_depth} bounces; reference {args.c4_ref_spp} spp FP64")
     ctx = rt.Context(0)
     sc = rt.Scene(ctx, desc)
@@ -328,16 +333,22 @@ def run_c4(args):
             hi = np.maximum(desc.xyz.max(0), [0.62, 0.62, 1.22]) + 1e-3
             cache = rt.RadianceCache(ctx, tuple(lo), tuple(hi), levels=16, log2_table=19, batch=65536, seed=1)
         curve, spent, frame, reached = [], 0.0, 0, None
-        losses = []
+        losses, samples = [], []
+        n_paths = 65536
         while spent < budget:
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             if use_cache:
                 # the loss is read back (a host round trip) only on a few frames
                 want = frame % 8 == 0
-                l, _ = sc.train_cache(cache, frame, n_paths=65536, cache_depth=args.c4_cache_depth, want_loss=want)
+                l, ns = sc.train_cache(cache, frame, n_paths=n_paths, cache_depth=args.c4_cache_depth,
+                                       want_loss=want)
                 if want:
                     losses.append(l)
+                    samples.append(ns)
+                if frame == 0:
+                    # paths per step for 65,536 samples at the measured yield (+3 %)
+                    n_paths = min(1 << 22, max(65536, int(65536 * 65536 * 1.03 / max(ns, 1)) // 1024 * 1024))
                 if frame >= args.c4_warm:
                     sc.render_cached(film, cache, cache_depth=args.c4_cache_depth, spp=4, spp_offset=4 * frame)
             else:
@@ -359,6 +370,9 @@ def run_c4(args):
                "curve": curve[:: max(1, len(curve) // 12)] + curve[-1:]}
         if losses:
             res["loss_first_last"] = [round(float(np.mean(losses[:5])), 4), round(float(np.mean(losses[-5:])), 4)]
+            res["training_paths_per_step"] = n_paths
+            res["samples_per_step"] = int(np.mean(samples[1:])) if len(samples) > 1 else int(samples[0])
+            res["samples_first_step"] = int(samples[0])
         return res
 
     log("plain path tracing")

@@ -164,16 +164,19 @@ typedef enum {
     RT_MAT_DIELECTRIC = 1,       /* materials/DielectricMaterial.cu, GlassMaterial.cu (ior 1.5) */
     RT_MAT_TROWBRIDGE = 2,       /* materials/TrowbridgeMaterial.cu (GGX, conductor Fresnel) */
     RT_MAT_TROWBRIDGE_BASIC = 3, /* materials/TrowbridgeBasicMaterial.cu (GGX, F = 1) */
-    RT_MAT_LIGHT = 4             /* materials/LightMaterial.cu (emissive) */
+    RT_MAT_LIGHT = 4,            /* materials/LightMaterial.cu (emissive) */
+    RT_MAT_MIRROR = 5,           /* materials/MirrorMaterial.cu (PerfectReflectionBSDF, weight kd) */
+    RT_MAT_PLAIN = 6             /* materials/PlainMaterial.cu (emits kd to camera rays, ke to others; absorbs) */
 } rt_material_type;
 
 typedef struct {
     uint32_t type;      /* rt_material_type */
-    float kd[3];        /* albedo / colour (kd, ks = kt, light colour) */
+    float kd[3];        /* albedo / colour (kd, ks = kt, light colour, mirror colour) */
     float ior;          /* dielectric index of refraction */
     float roughness;    /* Trowbridge: alpha = sqrt(roughness) (TrowbridgeBSDF.cu:56-59) */
     float eta[3], k[3]; /* Trowbridge conductor Fresnel */
     float intensity;    /* light: emitted radiance = kd * intensity */
+    float ke[3];        /* plain: radiance emitted towards rays that are not camera rays */
 } rt_material;
 
 /* PointLight(translate(pos), color, intensity) (light/PointLight.cu:15-66). */

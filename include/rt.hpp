@@ -1,6 +1,7 @@
 // rt.hpp -- C++17 host API over the C ABI of rt.h, with the reference
 // renderer's nouns: Scene, Shape (TriangleMesh, Sphere), Material
-// (Lambertian, Dielectric/Glass, Trowbridge, TrowbridgeBasic, Light),
+// (Lambertian, Dielectric/Glass, Trowbridge, TrowbridgeBasic, Light, Mirror,
+// Plain),
 // PointLight, ProjectiveCamera, Film, PathTracerIntegrator, BVH.
 //
 // Header-only; link with librt_amd.so.  Every failing C call throws
@@ -194,6 +195,42 @@ public:
 private:
     Spectrum c_;
     float i_;
+};
+
+// MirrorMaterial(color): perfect reflection, weight = color, no next-event
+// estimation (materials/MirrorMaterial.cu, PerfectReflectionBSDF.cu)
+class MirrorMaterial : public Material {
+public:
+    explicit MirrorMaterial(const Spectrum &color) : c_(color) {}
+    rt_material desc() const override
+    {
+        rt_material m{};
+        m.type = RT_MAT_MIRROR;
+        m.kd[0] = c_.r; m.kd[1] = c_.g; m.kd[2] = c_.b;
+        return m;
+    }
+
+private:
+    Spectrum c_;
+};
+
+// PlainMaterial(kd, ke): emits kd towards camera rays (and rays that reached
+// it through specular hits only) and ke towards the others; reflects nothing
+// (materials/PlainMaterial.cu:10-80)
+class PlainMaterial : public Material {
+public:
+    explicit PlainMaterial(const Spectrum &kd, const Spectrum &ke = Spectrum(0.0f)) : kd_(kd), ke_(ke) {}
+    rt_material desc() const override
+    {
+        rt_material m{};
+        m.type = RT_MAT_PLAIN;
+        m.kd[0] = kd_.r; m.kd[1] = kd_.g; m.kd[2] = kd_.b;
+        m.ke[0] = ke_.r; m.ke[1] = ke_.g; m.ke[2] = ke_.b;
+        return m;
+    }
+
+private:
+    Spectrum kd_, ke_;
 };
 
 // ---------------------------------------------------------------------

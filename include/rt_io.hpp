@@ -30,9 +30,9 @@
 // Where the reference reads garbage or crashes (a missing number, an index
 // out of range, an illum it does not know) these loaders throw rt_amd::Error
 // with RT_ERR_INVALID and the file and line.  Materials the GPU path does not
-// render (Plain, Phong, Beckmann, subsurface, textures, anisotropic
-// roughness, shadow-transparent Lambertian) load as data and are refused
-// with RT_ERR_UNSUPPORTED only when a shape is made from them.
+// render (Phong, Beckmann, subsurface, textures, anisotropic roughness,
+// shadow-transparent Lambertian or Plain) load as data and are refused with
+// RT_ERR_UNSUPPORTED only when a shape is made from them.
 #pragma once
 
 #include <cmath>
@@ -456,8 +456,9 @@ inline ProjectiveCamera make_camera(const ObejotaCamera &c, uint32_t w, uint32_t
 }
 
 // The material class an MTL block stands for, as the GPU path renders it:
-// illum 1 Lambertian(Kd), 3 Dielectric(1, Ni), 5 Trowbridge(Kd, Pr, eta, k),
-// 6 TrowbridgeBasic(Kd, Pr).  Throws RT_ERR_UNSUPPORTED for the others.
+// illum 0 Plain(Kd, Ke), 1 Lambertian(Kd), 3 Dielectric(1, Ni),
+// 5 Trowbridge(Kd, Pr, eta, k), 6 TrowbridgeBasic(Kd, Pr).  Throws
+// RT_ERR_UNSUPPORTED for the others.
 inline std::shared_ptr<Material> make_material(const MtlMaterial &m)
 {
     static const char *kNames[8] = {"PlainMaterial", "LambertianMaterial", "PhongMaterial", "DielectricMaterial",
@@ -469,6 +470,9 @@ inline std::shared_ptr<Material> make_material(const MtlMaterial &m)
     };
     if (!m.map_kd.empty() && m.illum != 3) return refuse("textures are not rendered on the GPU path");
     switch (m.illum) {
+    case 0:
+        if (m.transparent) return refuse("a Plain material without shadows is not rendered on the GPU path");
+        return std::make_shared<PlainMaterial>(m.kd, m.ke);
     case 1:
         if (m.transparent) return refuse("a Lambertian material without shadows is not rendered on the GPU path");
         return std::make_shared<LambertianMaterial>(m.kd);

@@ -57,6 +57,7 @@ _SIGS = {
     "orc_camera_ray": (None, [_P, _P, _U32, _U32, _U32, _U32, _F, _F, _P, _P]),
     "orc_fresnel_dielectric": (_F, [_F, _F, _F]),
     "orc_path_initstate": (C.c_uint64, [_U32, _U32]),
+    "orc_path_segments": (C.c_int, [_P, _U32, _U32, _U32, _U32, _U32, _U32, C.c_uint64, _P, _P, _P]),
     "orc_scene_create": (_P, [_P, _P, _U32, _P, _P, _P, _P, _U32, _P, _U32, _P, _U32, _P]),
     "orc_scene_free": (None, [_P]),
     "orc_scene_camera": (None, [_P, _P, _P]),
@@ -113,7 +114,7 @@ def cache_loss_grad(cfg, mlp, grid, inputs, targets, emulate_bf16=True, want_gra
 
 class OrcMaterial(C.Structure):
     _fields_ = [("type", C.c_uint32), ("kd", _F * 3), ("ior", _F), ("roughness", _F), ("eta", _F * 3),
-                ("k", _F * 3), ("intensity", _F)]
+                ("k", _F * 3), ("intensity", _F), ("ke", _F * 3)]
 
 
 class OrcPointLight(C.Structure):
@@ -145,7 +146,8 @@ class OrcScene:
         mats = (OrcMaterial * max(1, len(desc.materials)))()
         for i, m in enumerate(desc.materials):
             mats[i] = OrcMaterial(m["type"], _f3(m.get("kd", (0, 0, 0))), m.get("ior", 1.5), m.get("roughness", 0.0),
-                                  _f3(m.get("eta", (1, 1, 1))), _f3(m.get("k", (0, 0, 0))), m.get("intensity", 0.0))
+                                  _f3(m.get("eta", (1, 1, 1))), _f3(m.get("k", (0, 0, 0))), m.get("intensity", 0.0),
+                                  _f3(m.get("ke", (0, 0, 0))))
         self.mats = mats
         lights = (OrcPointLight * max(1, len(desc.lights)))()
         for i, (p, c, inten) in enumerate(desc.lights):
@@ -183,6 +185,18 @@ class OrcScene:
         ok = lib().orc_path(self.h, px, py, sample, d.seed if seed is None else seed, max_depth or d.max_depth,
                             d.rr_depth if rr_depth is None else rr_depth, stream_base, _p(out))
         return bool(ok), out
+
+    def path_segments(self, px, py, sample, seed=None, max_depth=None, rr_depth=None, stream_base=0):
+        """(ok, L, rays (nseg, 6)): the path and the ray of each of its segments."""
+        d = self.desc
+        md = max_depth or d.max_depth
+        out = np.zeros(3, np.float32)
+        seg = np.zeros((md, 6), np.float32)
+        ns = C.c_uint32()
+        ok = lib().orc_path_segments(self.h, px, py, sample, d.seed if seed is None else seed, md,
+                                     d.rr_depth if rr_depth is None else rr_depth, stream_base, _p(out), _p(seg),
+                                     C.byref(ns))
+        return bool(ok), out, seg[:ns.value]
 
     def render(self, spp=None, spp_offset=0, window=None, seed=None, nthreads=8, stream_base=0,
                max_depth=None, cache_depth=None, cache_rgb=None):

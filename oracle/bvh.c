@@ -162,19 +162,20 @@ void orc_refbvh_sorted_ids(const orc_refbvh *b, uint32_t *out) { memcpy(out, b->
 uint32_t orc_refbvh_morton(const orc_refbvh *b, uint32_t i) { return b->morton[i]; }
 
 static int tri_test(const orc_refbvh *b, uint32_t id, const orc_ray *r, const uint8_t *transparent,
-                    int check_shadow, float *h)
+                    int check_shadow, float *h, int guarded)
 {
     if (check_shadow && transparent && transparent[id]) return 0; /* Triangle.cu:439-440 */
     const float *t = b->tri + 9 * id;
-    return orc_tri_intersects(r, t, t + 3, t + 6, h, NULL);
+    return guarded ? orc_tri_intersects_guarded(r, t, t + 3, t + 6, h, NULL)
+                   : orc_tri_intersects(r, t, t + 3, t + 6, h, NULL);
 }
 
 /* BVH::intersectRay, BVH.cu:355-448: LIFO stack; for each popped node the
  * left child then the right child; leaf children are triangle-tested at
  * once, internal children are box-tested (no hit-distance culling) and
  * pushed, so the right subtree is popped first. */
-int32_t orc_refbvh_intersect(const orc_refbvh *b, const orc_ray *r, const uint8_t *transparent,
-                             int check_shadow, float *t_out, uint64_t *n_box, uint64_t *n_tri)
+static int32_t refbvh_walk(const orc_refbvh *b, const orc_ray *r, const uint8_t *transparent, int check_shadow,
+                           float *t_out, uint64_t *n_box, uint64_t *n_tri, int guarded)
 {
     int32_t hit = -1;
     float h = INFINITY, thit = INFINITY;
@@ -183,7 +184,7 @@ int32_t orc_refbvh_intersect(const orc_refbvh *b, const orc_ray *r, const uint8_
     const ref_node *root = &b->nodes[b->root];
     if (root->object >= 0) {
         ++ntr;
-        if (tri_test(b, (uint32_t)root->object, r, transparent, check_shadow, &h)) { hit = root->object; thit = h; }
+        if (tri_test(b, (uint32_t)root->object, r, transparent, check_shadow, &h, guarded)) { hit = root->object; thit = h; }
         if (t_out) *t_out = thit;
         if (n_tri) *n_tri += ntr;
         return hit;
@@ -198,7 +199,7 @@ int32_t orc_refbvh_intersect(const orc_refbvh *b, const orc_ray *r, const uint8_
             const ref_node *cn = &b->nodes[ch[c]];
             if (cn->object >= 0) {
                 ++ntr;
-                if (tri_test(b, (uint32_t)cn->object, r, transparent, check_shadow, &h)) { hit = cn->object; thit = h; }
+                if (tri_test(b, (uint32_t)cn->object, r, transparent, check_shadow, &h, guarded)) { hit = cn->object; thit = h; }
             } else {
                 float tmp = h;
                 ++nb;
@@ -215,6 +216,18 @@ int32_t orc_refbvh_intersect(const orc_refbvh *b, const orc_ray *r, const uint8_
     if (n_box) *n_box += nb;
     if (n_tri) *n_tri += ntr;
     return hit;
+}
+
+int32_t orc_refbvh_intersect(const orc_refbvh *b, const orc_ray *r, const uint8_t *transparent,
+                             int check_shadow, float *t_out, uint64_t *n_box, uint64_t *n_tri)
+{
+    return refbvh_walk(b, r, transparent, check_shadow, t_out, n_box, n_tri, 0);
+}
+
+int32_t orc_refbvh_intersect_guarded(const orc_refbvh *b, const orc_ray *r, const uint8_t *transparent,
+                                     int check_shadow, float *t_out)
+{
+    return refbvh_walk(b, r, transparent, check_shadow, t_out, NULL, NULL, 1);
 }
 
 /* ------------------------------------------------------------------ */
@@ -424,7 +437,7 @@ static void b2_range(void *p, uint32_t b0, uint32_t e0, int tid)
                 for (uint32_t k = 0; k < cnt; ++k) {
                     const float *tr = c->tris + 12 * (size_t)(first + k);
                     ++ntr;
-                    if (orc_tri_intersects(&r, tr, tr + 4, tr + 8, &h, NULL)) {
+                    if (orc_tri_intersects_guarded(&r, tr, tr + 4, tr + 8, &h, NULL)) {
                         hit = as_i32(tr[3]);
                         thit = h;
                     }
@@ -559,7 +572,7 @@ static void b4_range(void *p, uint32_t b0, uint32_t e0, int tid)
                 for (uint32_t k = 0; k < cnt; ++k) {
                     const float *tr = c->tris + 12 * (size_t)(first + k);
                     ++ntr;
-                    if (orc_tri_intersects(&r, tr, tr + 4, tr + 8, &h, NULL)) {
+                    if (orc_tri_intersects_guarded(&r, tr, tr + 4, tr + 8, &h, NULL)) {
                         hit = as_i32(tr[3]);
                         thit = h;
                     }

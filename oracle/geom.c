@@ -68,9 +68,16 @@ static inline float max3f(float a, float b, float c) { return fmaxf(a, fmaxf(b, 
  *  - conservative deltaT rejection (:575-593).
  * The tail (dpdu/dpdv, alpha stub) never rejects: the alpha stub returns
  * true (:154-158) and the derivative code has no rejecting branch.
+ *
+ * `guarded` applies the deltaT rejection before hitOffset is lowered (the
+ * library's order): a candidate that fails it (typically the triangle a
+ * spawned ray just left) then leaves hitOffset alone, so the result is the
+ * closest candidate that passes, whatever the visit order.  In the
+ * reference's order such a candidate lowers hitOffset first, and every
+ * triangle tested after it is rejected.
  */
-int orc_tri_intersects(const orc_ray *r, const float *wp0, const float *wp1,
-                       const float *wp2, float *hitOffset, orc_tri_info *info)
+static int tri_test_core(const orc_ray *r, const float *wp0, const float *wp1, const float *wp2,
+                         float *hitOffset, orc_tri_info *info, int guarded)
 {
     if (info) { info->reached = 0; info->masked = 0; info->fp64 = 0; info->t = INFINITY; }
 
@@ -127,7 +134,7 @@ int orc_tri_intersects(const orc_ray *r, const float *wp0, const float *wp1,
     float t = scaled * invDet;
     if (info) { info->reached = 1; info->t = t; info->fp64 = used64; }
     if (t > *hitOffset) return 0;
-    *hitOffset = t;
+    if (!guarded) *hitOffset = t;
 
     float maxZt = max3f(fabsf(p0z), fabsf(p1z), fabsf(p2z));
     float deltaZ = orc_error_bound(3) * maxZt;
@@ -138,11 +145,24 @@ int orc_tri_intersects(const orc_ray *r, const float *wp0, const float *wp1,
     float deltaE = 2 * (orc_error_bound(2) * maxXt * maxYt + deltaY * maxXt + deltaX * maxYt);
     float maxE = max3f(fabsf(e0), fabsf(e1), fabsf(e2));
     float deltaT = 3 * (orc_error_bound(3) * maxE * maxZt + deltaE * maxZt + deltaZ * maxE) * fabsf(invDet);
-    if (*hitOffset <= deltaT) {
+    if (t <= deltaT) {
         if (info) info->masked = 1;
         return 0;
     }
+    *hitOffset = t;
     return 1;
+}
+
+int orc_tri_intersects(const orc_ray *r, const float *wp0, const float *wp1,
+                       const float *wp2, float *hitOffset, orc_tri_info *info)
+{
+    return tri_test_core(r, wp0, wp1, wp2, hitOffset, info, 0);
+}
+
+int orc_tri_intersects_guarded(const orc_ray *r, const float *wp0, const float *wp1, const float *wp2,
+                               float *hitOffset, orc_tri_info *info)
+{
+    return tri_test_core(r, wp0, wp1, wp2, hitOffset, info, 1);
 }
 
 /* Bound3f::intersects, geometry/BoundingBox.cu:296-329.  Unqualified

@@ -52,6 +52,12 @@ typedef struct {
  * rejection).  info may be NULL.  Returns 1 on an accepted hit. */
 int orc_tri_intersects(const orc_ray *r, const float *p0, const float *p1,
                        const float *p2, float *hitOffset, orc_tri_info *info);
+/* The same test with the deltaT rejection applied before hitOffset is
+ * lowered (the library's order): a rejected candidate leaves hitOffset
+ * alone, so a traversal returns the closest candidate that passes, in any
+ * visit order. */
+int orc_tri_intersects_guarded(const orc_ray *r, const float *p0, const float *p1,
+                               const float *p2, float *hitOffset, orc_tri_info *info);
 
 /* Bound3f::intersects (geometry/BoundingBox.cu:296-329), CUDA fminf/fmaxf
  * semantics for min/max (NaN operands are ignored). */
@@ -76,6 +82,9 @@ uint32_t orc_refbvh_morton(const orc_refbvh *b, uint32_t i);
 int32_t orc_refbvh_intersect(const orc_refbvh *b, const orc_ray *r,
                              const uint8_t *transparent, int check_shadow,
                              float *t_out, uint64_t *n_box, uint64_t *n_tri);
+/* The same walk with orc_tri_intersects_guarded (the renderer's queries). */
+int32_t orc_refbvh_intersect_guarded(const orc_refbvh *b, const orc_ray *r, const uint8_t *transparent,
+                                     int check_shadow, float *t_out);
 
 /* Batch over SoA rays with nthreads pthreads (BVH.cu:586-635 analogue). */
 void orc_refbvh_trace(const orc_refbvh *b, const float *ox, const float *oy,

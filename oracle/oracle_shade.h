@@ -21,7 +21,9 @@ enum {
     ORC_MAT_DIELECTRIC = 1,       /* materials/DielectricMaterial.cu (GlassMaterial: ior 1.5) */
     ORC_MAT_TROWBRIDGE = 2,       /* materials/TrowbridgeMaterial.cu (conductor Fresnel) */
     ORC_MAT_TROWBRIDGE_BASIC = 3, /* materials/TrowbridgeBasicMaterial.cu (F = 1) */
-    ORC_MAT_LIGHT = 4             /* materials/LightMaterial.cu (emissive, NullBSDF) */
+    ORC_MAT_LIGHT = 4,            /* materials/LightMaterial.cu (emissive, NullBSDF) */
+    ORC_MAT_MIRROR = 5,           /* materials/MirrorMaterial.cu (PerfectReflectionBSDF) */
+    ORC_MAT_PLAIN = 6             /* materials/PlainMaterial.cu (emits kd / ke, absorbs) */
 };
 
 typedef struct {
@@ -31,6 +33,7 @@ typedef struct {
     float roughness;   /* alpha = sqrt(roughness) (TrowbridgeBSDF.cu:56-59) */
     float eta[3], k[3];
     float intensity;   /* light: emitted radiance = kd * intensity */
+    float ke[3];       /* plain: emitted towards rays that are not camera rays */
 } orc_material;
 
 typedef struct {
@@ -57,6 +60,11 @@ void orc_camera_ray(const float *r2c, const float *c2w, uint32_t w, uint32_t h, 
 
 float orc_fresnel_dielectric(float cosO, float nO, float nD);
 uint64_t orc_path_initstate(uint32_t seed, uint32_t sample);
+/* orc_path, also recording each segment's ray (origin, direction: 6 floats
+ * per segment, at most max_depth segments) into seg and their count. */
+int orc_path_segments(const orc_scene *s, uint32_t px, uint32_t py, uint32_t sample, uint32_t seed,
+                      uint32_t max_depth, uint32_t rr_depth, uint64_t stream_base, float out[3], float *seg,
+                      uint32_t *nseg);
 
 /* spheres: 4 floats (cx, cy, cz, r) each; sph_light: 1 = also sampled as a
  * sphere area light.  Triangles are xyz[3*nv] indexed by idx[3*nt]. */

@@ -12,7 +12,7 @@
  *   camera: 2 (film jitter)
  *   per bounce at a scattering material: 2 per sphere area light (NEE)
  *   BSDF: Lambertian 2, Trowbridge 2 (0 if effectively smooth),
- *         dielectric 1, light 0
+ *         dielectric 1, light, mirror and plain 0
  *   Russian roulette: 1, only when it applies.
  */
 #include "oracle.h"
@@ -338,9 +338,20 @@ float orc_fresnel_dielectric(float cosO, float nO, float nD)
     return (r * r + r * r) / 2 * fabsf(cosO);
 }
 
+/* Material::isScattering: Lambertian, Trowbridge and Plain
+ * (PlainMaterial.cu:77-80; its next-event terms are 0: lightReflected = 0, :30-41) */
 static int is_scattering(const orc_material *m)
 {
-    return m->type == ORC_MAT_LAMBERTIAN || m->type == ORC_MAT_TROWBRIDGE || m->type == ORC_MAT_TROWBRIDGE_BASIC;
+    return m->type == ORC_MAT_LAMBERTIAN || m->type == ORC_MAT_TROWBRIDGE || m->type == ORC_MAT_TROWBRIDGE_BASIC ||
+           m->type == ORC_MAT_PLAIN;
+}
+
+/* Material::isSpecular: a camera ray stays one through these hits
+ * (SurfaceInteraction.cu:252-254); Trowbridge counts (TrowbridgeMaterial.cu:55-63) */
+static int is_specular(const orc_material *m)
+{
+    return m->type == ORC_MAT_DIELECTRIC || m->type == ORC_MAT_MIRROR || m->type == ORC_MAT_TROWBRIDGE ||
+           m->type == ORC_MAT_TROWBRIDGE_BASIC;
 }
 
 static float ggx_alpha(const orc_material *m) { return sqrtf(m->roughness); } /* roughnessToAlpha */
@@ -348,6 +359,7 @@ static float ggx_alpha(const orc_material *m) { return sqrtf(m->roughness); } /*
 /* lightReflected for the scattering materials (f * |cos wi|), shading space */
 static v3 bsdf_f(const orc_material *m, v3 wo, v3 wi)
 {
+    if (m->type == ORC_MAT_PLAIN) return mk(0, 0, 0); /* PlainMaterial::lightReflected */
     if (m->type == ORC_MAT_LAMBERTIAN) {
         /* LambertianBRDF: kd/pi * absCosine(wi) (LambertianBSDF.cu:22-38) */
         float c = abscos(wi);
@@ -454,7 +466,13 @@ static v3 bsdf_sample(const orc_material *m, v3 wo, pcg32 *g, v3 *wi, float *pdf
         v3 f = bsdf_f(m, o, *wi);
         return mk(f.x / pdf, f.y / pdf, f.z / pdf);
     }
-    /* light: NullBSDF (NullBSDF.cu) */
+    if (m->type == ORC_MAT_MIRROR) {
+        /* PerfectReflectionBSDF::sampleDirection (PerfectReflectionBSDF.cu): weight = colour */
+        *delta = 1;
+        *wi = mk(-wo.x, -wo.y, wo.z);
+        return mk(m->kd[0], m->kd[1], m->kd[2]);
+    }
+    /* light: NullBSDF (NullBSDF.cu); plain: PlainMaterial::sampleDirection returns 0 */
     *wi = mk(0, 0, 0);
     return mk(0, 0, 0);
 }
@@ -537,7 +555,12 @@ typedef struct {
 
 /* Scene::intersectTriangles + intersectShapes (integration/Scene.cu:165-236):
  * the BVH's triangle, re-tested with Triangle::intersect from hitOffset =
- * +inf, then every sphere in insertion order with a strict < test. */
+ * +inf, then every sphere in insertion order with a strict < test.  The
+ * triangle tests apply the deltaT rejection before lowering the hit distance
+ * (orc_tri_intersects_guarded), as the library does: in the reference's
+ * order a ray that re-meets the triangle it was spawned from (a candidate at
+ * t ~ 0 that fails deltaT) loses every later hit, and whether that happens
+ * depends on the tree's visit order, which the library does not share. */
 static hit_rec scene_intersect(const orc_scene *s, v3 o, v3 d)
 {
     hit_rec h;
@@ -548,12 +571,12 @@ static hit_rec scene_intersect(const orc_scene *s, v3 o, v3 d)
     orc_ray_make(&r, of, df, INFINITY);
     if (s->nt) {
         float tt;
-        int32_t id = orc_refbvh_intersect(s->bvh, &r, NULL, 0, &tt, NULL, NULL);
+        int32_t id = orc_refbvh_intersect_guarded(s->bvh, &r, NULL, 0, &tt);
         if (id >= 0) {
             const float *t = s->xyz + 9 * id;
             float hh = INFINITY;
             orc_tri_info info;
-            if (orc_tri_intersects(&r, t, t + 3, t + 6, &hh, &info)) {
+            if (orc_tri_intersects_guarded(&r, t, t + 3, t + 6, &hh, &info)) {
                 h.kind = 1; h.id = id; h.t = hh;
             }
         }
@@ -596,7 +619,7 @@ static int occluded(const orc_scene *s, v3 o, v3 d, float dist, int skip_sphere)
     int hit = 0;
     if (s->nt) {
         float tt;
-        int32_t id = orc_refbvh_intersect(s->bvh, &r, s->tri_transparent, 1, &tt, NULL, NULL);
+        int32_t id = orc_refbvh_intersect_guarded(s->bvh, &r, s->tri_transparent, 1, &tt);
         if (id >= 0) { hit = 1; t = tt; }
     }
     for (uint32_t j = 0; j < s->nsph; ++j) {
@@ -682,8 +705,9 @@ static int cacheable(const orc_material *m)
  * constant). */
 static int path_impl(const orc_scene *s, uint32_t px, uint32_t py, uint32_t sample, uint32_t seed,
                      uint32_t max_depth, uint32_t rr_depth, uint64_t stream_base, uint32_t cache_depth,
-                     const float *cache_rgb, float out[3])
+                     const float *cache_rgb, float out[3], float *seg, uint32_t *nseg)
 {
+    if (nseg) *nseg = 0;
     const uint32_t W = s->cam.width;
     pcg32 g = pcg_seed(orc_path_initstate(seed, sample), stream_base + (uint64_t)py * W + px);
     float u0 = pcg_float(&g), u1 = pcg_float(&g);
@@ -693,8 +717,14 @@ static int path_impl(const orc_scene *s, uint32_t px, uint32_t py, uint32_t samp
     v3 beta = mk(1, 1, 1), L = mk(0, 0, 0);
     float prev_pdf = 0;   /* BSDF pdf of the direction that produced this ray */
     int prev_delta = 1;   /* camera rays count as delta: no MIS on a light hit */
+    int camera = 1;       /* Ray::cameraRay: primary, then kept through specular hits */
     v3 prev_p = o;
     for (uint32_t depth = 0; depth < max_depth; ++depth) {
+        if (seg) {
+            float *q = seg + 6 * (size_t)depth;
+            q[0] = o.x; q[1] = o.y; q[2] = o.z; q[3] = d.x; q[4] = d.y; q[5] = d.z;
+            *nseg = depth + 1;
+        }
         hit_rec h = scene_intersect(s, o, d);
         if (h.kind == 0) break;
         const orc_material *m = &s->mat[h.kind == 1 ? s->tri_mat[h.id] : s->sph_mat[h.id]];
@@ -759,6 +789,11 @@ static int path_impl(const orc_scene *s, uint32_t px, uint32_t py, uint32_t samp
             float I = m->intensity;
             L = add(L, mk(beta.x * m->kd[0] * I * w, beta.y * m->kd[1] * I * w, beta.z * m->kd[2] * I * w));
         }
+        if (m->type == ORC_MAT_PLAIN) {
+            /* PlainMaterial::getEmittedRadiance (PlainMaterial.cu:62-75): kd to camera rays, ke otherwise */
+            const float *e = camera ? m->kd : m->ke;
+            L = add(L, mk(beta.x * e[0], beta.y * e[1], beta.z * e[2]));
+        }
         v3 wi;
         float pdf;
         int delta;
@@ -777,6 +812,7 @@ static int path_impl(const orc_scene *s, uint32_t px, uint32_t py, uint32_t samp
         prev_pdf = pdf;
         prev_delta = delta;
         prev_p = h.p;
+        camera = camera && is_specular(m);
     }
     out[0] = L.x; out[1] = L.y; out[2] = L.z;
     return isfinite(L.x) && isfinite(L.y) && isfinite(L.z);
@@ -785,7 +821,14 @@ static int path_impl(const orc_scene *s, uint32_t px, uint32_t py, uint32_t samp
 int orc_path(const orc_scene *s, uint32_t px, uint32_t py, uint32_t sample, uint32_t seed, uint32_t max_depth,
              uint32_t rr_depth, uint64_t stream_base, float out[3])
 {
-    return path_impl(s, px, py, sample, seed, max_depth, rr_depth, stream_base, 0, NULL, out);
+    return path_impl(s, px, py, sample, seed, max_depth, rr_depth, stream_base, 0, NULL, out, NULL, NULL);
+}
+
+int orc_path_segments(const orc_scene *s, uint32_t px, uint32_t py, uint32_t sample, uint32_t seed,
+                      uint32_t max_depth, uint32_t rr_depth, uint64_t stream_base, float out[3], float *seg,
+                      uint32_t *nseg)
+{
+    return path_impl(s, px, py, sample, seed, max_depth, rr_depth, stream_base, 0, NULL, out, seg, nseg);
 }
 
 typedef struct {
@@ -810,7 +853,7 @@ static void *render_rows(void *p)
             for (uint32_t k = 0; k < c->spp; ++k) {
                 float L[3];
                 if (path_impl(c->s, x, y, c->spp_offset + k, c->seed, c->max_depth, c->rr_depth, c->stream_base,
-                              c->cache_depth, c->cache_rgb, L)) {
+                              c->cache_depth, c->cache_rgb, L, NULL, NULL)) {
                     acc[0] += L[0]; acc[1] += L[1]; acc[2] += L[2];
                     ++nv;
                 }

@@ -147,3 +147,39 @@ def test_wide_traversal_tmax_and_far_origins(rt, orc):
     assert np.array_equal(p4[clean], bp[clean])
     assert np.array_equal(t4[clean], bt[clean])
     assert (bp[clean] >= 0).sum() > 500
+
+
+def test_self_candidates_do_not_hide_hits(rt, orc):
+    """Rays spawned from a large wall re-meet it at t ~ 0 as a candidate that
+    fails the deltaT rejection.  The library (and its restated traversal)
+    applies deltaT before lowering the hit distance, so such rays still get
+    the closest passing hit, equal to brute force on every ray.  The
+    reference's order (hitOffset lowered first, Triangle.cu:569-593, kept in
+    the oracle's literal test and reference tree) loses the hit whenever the
+    wall is tested first; its Morton tree does that for some of these rays."""
+    import material_scenes as ms
+    W = 64
+    d = ms.probe("lambert", width=W, height=W, spp=1)
+    d.max_depth = 2
+    s = orc.OrcScene(d)
+    segs = []
+    for y in range(W):
+        for x in range(W):
+            ok, L, seg = s.path_segments(x, y, 0)
+            assert ok and len(seg) == 2
+            assert np.array_equal(L, np.float32(0.5) * np.array(ms.KE, np.float32))
+            segs.append(seg[1])
+    segs = np.array(segs, np.float32)
+    rays = [segs[:, k] for k in range(6)]
+    inf, nodes, nodes4, tris = host_bvh4(rt, d.xyz, d.idx)
+    bp, bt, fl = orc.brute_trace(d.xyz, d.idx, rays)
+    masked = (fl & 2) != 0
+    assert masked.sum() > 100, "the scene must produce self candidates"
+    assert (bp >= 0).all()
+    p4, t4, *_ = orc.bvh4_trace(nodes4, tris, rays)
+    p2, t2, *_ = orc.bvh2_trace(nodes, tris, rays)
+    assert np.array_equal(p4, bp) and np.array_equal(t4, bt)
+    assert np.array_equal(p2, bp) and np.array_equal(t2, bt)
+    ref, _ = orc.RefBvh(d.xyz, d.idx).trace(rays)
+    assert (ref[masked] < 0).sum() > 0, "the reference's order loses some of these hits"
+    assert np.array_equal(ref[~masked], bp[~masked])

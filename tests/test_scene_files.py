@@ -121,15 +121,17 @@ def test_material_classes_on_the_gpu_path():
     assert sf.material_desc(m["ggx_basic"])["type"] == 3
     d = sf.material_desc(m["glass2"])  # DielectricMaterial(Spectrum(1), Ni)
     assert (d["type"], d["kd"], d["ior"]) == (1, (1.0, 1.0, 1.0), m["glass2"].ni)
-    for name, why in [("aniso", "anisotropic"), ("plain", "PlainMaterial"), ("skin", "SSMaterial"),
-                      ("brick", "textures"), ("shadowless", "without shadows")]:
+    p = sf.material_desc(m["plain"])  # PlainMaterial(kd, ke); Ke 5 5 5 of its block
+    assert (p["type"], p["kd"], p["ke"]) == (6, (1, 1, 1), (5, 5, 5))
+    for name, why in [("aniso", "anisotropic"), ("skin", "SSMaterial"), ("brick", "textures"),
+                      ("shadowless", "without shadows")]:
         with pytest.raises(sf.UnsupportedMaterial, match=why):
             sf.material_desc(m[name])
     # a scene only converts the materials its shapes use
     s = sf.SceneFiles([fx("features.obj")])
     assert s.tri_mat.tolist() == [0, 1, 1, 2, 3, 4, 5]
     with pytest.raises(sf.UnsupportedMaterial):
-        s.material("plain")
+        s.material("skin")
 
 
 def test_obejota_known_answers():

@@ -127,6 +127,7 @@ _SIGS = {
     "rt_scene_add_material": (C.c_int, [_P, C.POINTER(Material), C.POINTER(_U32)]),
     "rt_scene_add_mesh": (C.c_int, [_P, _P, _U32, _P, _U32, _U32, C.POINTER(_U32)]),
     "rt_scene_add_sphere": (C.c_int, [_P, _P, C.c_float, _U32, C.c_int]),
+    "rt_scene_add_plane": (C.c_int, [_P, _P, _P, _P, _P, _U32]),
     "rt_scene_add_point_light": (C.c_int, [_P, C.POINTER(PointLight)]),
     "rt_scene_set_camera": (C.c_int, [_P, C.POINTER(Camera)]),
     "rt_scene_commit": (C.c_int, [_P]),
@@ -407,6 +408,9 @@ class Scene:
         for c, r, m, is_light in desc.spheres:
             cc = np.asarray(c, np.float32)
             check(L.rt_scene_add_sphere(h, _ptr(cc), float(r), int(m), int(is_light)))
+        for corner, ax, ay, nrm, m in getattr(desc, "planes", []):
+            vs = [np.asarray(v, np.float32) for v in (corner, ax, ay, nrm)]
+            check(L.rt_scene_add_plane(h, *[_ptr(v) for v in vs], int(m)))
         for p, col, inten in desc.lights:
             pl = PointLight(f3(p), f3(col), float(inten))
             check(L.rt_scene_add_point_light(h, C.byref(pl)))

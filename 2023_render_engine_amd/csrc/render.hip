@@ -275,7 +275,8 @@ __device__ __forceinline__ bool cacheable(const rt_material &m)
 __device__ __forceinline__ uint32_t classify(const SceneDev &sc, int32_t hid, bool cache_depth_reached)
 {
     if (hid == -1) return kQMiss;
-    const uint32_t mi = hid >= 0 ? sc.tri_mat[hid] : sc.sph[-hid - 2].material;
+    const uint32_t k = (uint32_t)(-hid - 2);
+    const uint32_t mi = hid >= 0 ? sc.tri_mat[hid] : k < sc.nsph ? sc.sph[k].material : sc.plane[k - sc.nsph].material;
     if (cache_depth_reached && cacheable(sc.mat[mi])) return kQCache;
     const uint32_t t = sc.mat[mi].type;
     if (t == RT_MAT_LAMBERTIAN) return kQLambert;
@@ -353,7 +354,7 @@ __global__ __launch_bounds__(kBlock) void k_classify(SceneDev sc0, BatchDev bt, 
             const SceneDev &sc = scene_of<B>(sc0, bt, p, slot);
             int32_t hid = p.hid[slot];
             float th = p.ht[slot];
-            if (sc.nsph) {
+            if (sc.nsph || sc.nplane) {
                 const V3 o = mk(p.ox[slot], p.oy[slot], p.oz[slot]), d = mk(p.dx[slot], p.dy[slot], p.dz[slot]);
                 bool changed = false;
                 for (uint32_t k = 0; k < sc.nsph; ++k) {
@@ -362,6 +363,15 @@ __global__ __launch_bounds__(kBlock) void k_classify(SceneDev sc0, BatchDev bt, 
                     if (ts >= 0 && ts < th) {
                         th = ts;
                         hid = -(int32_t)k - 2;
+                        changed = true;
+                    }
+                }
+                for (uint32_t k = 0; k < sc.nplane; ++k) {
+                    float u, v, dz;
+                    const float tp = plane_core(sc.plane[k], o, d, INFINITY, u, v, dz);
+                    if (tp > 0 && tp < th) {  // strict (Plane.cu:88-98)
+                        th = tp;
+                        hid = -(int32_t)(sc.nsph + k) - 2;
                         changed = true;
                     }
                 }
@@ -433,14 +443,21 @@ __device__ __forceinline__ void surface_of(const SceneDev &sc, const PathsDev &p
     const V3 o = mk(p.ox[slot], p.oy[slot], p.oz[slot]);
     d = mk(p.dx[slot], p.dy[slot], p.dz[slot]);
     const int32_t hid = p.hid[slot];
+    const uint32_t k = (uint32_t)(-hid - 2);
     if (hid >= 0) {
         triangle_surface(sc, hid, o, d, s);
-    } else {
-        const SphereDev &sp = sc.sph[-hid - 2];
+    } else if (k < sc.nsph) {
+        const SphereDev &sp = sc.sph[k];
         V3 so, sd;
         const float ts = sphere_core(sp, o, d, INFINITY, so, sd);
         sphere_surface(sp, so, sd, ts, s.p, s.n, s.perr);
         s.mat = sp.material;
+    } else {
+        const PlaneDev &pl = sc.plane[k - sc.nsph];
+        float u, v, dz;
+        plane_core(pl, o, d, INFINITY, u, v, dz);
+        plane_surface(pl, u, v, dz, s.p, s.n, s.perr);
+        s.mat = pl.material;
     }
 }
 
@@ -465,7 +482,8 @@ __device__ __forceinline__ void resolve_nee(const PathsDev &p, uint32_t slot)
 }
 
 // Shadow-ray occlusion by the analytic shapes (Scene.cu:108-150): opaque
-// spheres other than `skip` (the sampled light) hit at t <= tmax.
+// spheres other than `skip` (the sampled light) and opaque planes hit at
+// t <= tmax.
 __device__ __forceinline__ bool spheres_occlude(const SceneDev &sc, V3 o, V3 d, float tmax, int32_t skip)
 {
     for (uint32_t j = 0; j < sc.nsph; ++j) {
@@ -473,6 +491,12 @@ __device__ __forceinline__ bool spheres_occlude(const SceneDev &sc, V3 o, V3 d, 
         V3 so, sd;
         const float ts = sphere_core(sc.sph[j], o, d, INFINITY, so, sd);
         if (ts >= 0 && !(ts > tmax)) return true;
+    }
+    for (uint32_t k = 0; k < sc.nplane; ++k) {
+        if (sc.plane[k].transparent) continue;
+        float u, v, dz;
+        const float tp = plane_core(sc.plane[k], o, d, INFINITY, u, v, dz);
+        if (tp > 0 && !(tp > tmax)) return true;
     }
     return false;
 }
@@ -636,7 +660,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
                 // against the light's own next-event sampling
                 float w = 1;
                 const int32_t hid = p.hid[slot];
-                if (hid <= -2 && sc.sph[-hid - 2].is_light && !((meta >> 16) & 1u)) {
+                if (hid <= -2 && (uint32_t)(-hid - 2) < sc.nsph && sc.sph[-hid - 2].is_light && !((meta >> 16) & 1u)) {
                     const float plp = sphere_light_pdf(sc.sph[-hid - 2], mk(p.ppx[slot], p.ppy[slot], p.ppz[slot]));
                     const float pp = p.ppdf[slot];
                     w = pp * pp / (pp * pp + plp * plp);

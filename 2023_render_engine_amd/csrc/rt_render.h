@@ -46,10 +46,12 @@ struct SceneDev {
     const rt_material *mat;
     const rt_point_light *pl;
     SphereDev sph[kMaxSpheres];
+    const PlaneDev *plane;  // nplane planes, tested after the spheres
     CameraDev cam;
     uint32_t nsph, npl, nlights;  // nlights = npl + area lights (NEE slots per path)
     uint32_t area_light[kMaxSpheres];  // sphere index of area light a
     uint32_t nmat;
+    uint32_t nplane;
     // batched rendering: the scene's own PCG seed and stream offset (a
     // single-scene wave takes both from WaveParams)
     uint32_t seed;
@@ -86,7 +88,7 @@ struct PathsDev {
     int32_t *root;              // root node of the path's scene in the batch's pooled tree
     uint32_t *meta;             // depth | prev_delta << 16
     float *ppdf, *ppx, *ppy, *ppz;  // previous BSDF pdf and vertex (MIS of sphere lights)
-    int32_t *hid;               // >= 0 triangle, <= -2 sphere -(j+2), -1 miss
+    int32_t *hid;               // >= 0 triangle, -1 miss, <= -2 analytic shape -(k+2): sphere k < nsph, else plane k - nsph
     float *ht;
     float *nbx, *nby, *nbz;     // throughput at the last next-event estimation
     uint32_t *nee_mask;         // unoccluded light slots of the last NEE

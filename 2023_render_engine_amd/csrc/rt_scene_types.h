@@ -28,7 +28,19 @@ struct SphereDev {
     uint32_t pad;
 };
 
+// Plane(corner, axisX, axisY, normal) (shapes/Plane.cu:27-48):
+// objectToWorld = baseChange(corner, axisX, axisY, normal) with the axis
+// lengths truncated to integers, worldToObject its cofactor inverse; the
+// shape is the unit square of the object's z = 0 plane.
+struct PlaneDev {
+    float o2w[16], w2o[16];
+    uint32_t material;
+    uint32_t transparent;  // dielectric: skipped by shadow rays (Plane.cu:128-129)
+    uint32_t pad[2];
+};
+
 constexpr int kMaxSpheres = 16;
+constexpr int kMaxPlanes = 64;
 constexpr int kMaxPointLights = 32;
 
 }  // namespace rt

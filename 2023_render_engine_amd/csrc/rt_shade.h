@@ -507,6 +507,68 @@ __device__ __forceinline__ void sphere_surface(const SphereDev &s, V3 o, V3 d, f
            mi[2] * on.x + mi[6] * on.y + mi[10] * on.z);
 }
 
+// Plane::intersect / intersects (shapes/Plane.cu:58-164).  The ray goes to
+// object space with Transform::operator()(Ray) (Transform.cu:216-237: the
+// origin error, errorBound(4) of the row sums, moves the origin along the
+// unnormalised direction and shrinks tMax); |d.z| must exceed
+// FLOAT_ERROR_MARGIN (1e-7); the distance is Ray::getOffset,
+// (hit.x - o.x) / d.x (Ray.cu:69-72), so a local d.x of 0 gives NaN and no
+// hit; the hit must lie in [0,1]^2 with 0 < offset <= tMax.  Returns the
+// offset (the caller applies the strict offset < hitOffset), or -1; u, v and
+// the local d.z are kept for the surface.
+__device__ __forceinline__ float plane_core(const PlaneDev &pl, V3 wo_, V3 wd_, float tmax, float &u, float &v,
+                                            float &dz)
+{
+    const float *m = pl.w2o;
+    const float eb4 = error_bound(4);
+    const float xp = m[0] * wo_.x + m[1] * wo_.y + m[2] * wo_.z + m[3];
+    const float yp = m[4] * wo_.x + m[5] * wo_.y + m[6] * wo_.z + m[7];
+    const float zp = m[8] * wo_.x + m[9] * wo_.y + m[10] * wo_.z + m[11];
+    const float wp = m[12] * wo_.x + m[13] * wo_.y + m[14] * wo_.z + m[15];
+    V3 oErr;
+    oErr.x = (fabsf(m[0] * wo_.x) + fabsf(m[1] * wo_.y) + fabsf(m[2] * wo_.z) + fabsf(m[3])) * eb4 * fabsf(wp);
+    oErr.y = (fabsf(m[4] * wo_.x) + fabsf(m[5] * wo_.y) + fabsf(m[6] * wo_.z) + fabsf(m[7])) * eb4 * fabsf(wp);
+    oErr.z = (fabsf(m[8] * wo_.x) + fabsf(m[9] * wo_.y) + fabsf(m[10] * wo_.z) + fabsf(m[11])) * eb4 * fabsf(wp);
+    const float inv = 1.0f / wp;
+    V3 o = mk(inv * xp, inv * yp, inv * zp);
+    const V3 d = xform_vec(m, wd_);
+    float tm = tmax;
+    const float ls = d.x * d.x + d.y * d.y + d.z * d.z;
+    if (ls > 0) {
+        const float dt = (fabsf(d.x) * oErr.x + fabsf(d.y) * oErr.y + fabsf(d.z) * oErr.z) / ls;
+        o.x += d.x * dt; o.y += d.y * dt; o.z += d.z * dt;
+        tm -= dt;
+    }
+    const float invx = 1.0f / d.x;
+    dz = d.z;
+    if (!((double)fabsf(d.z) > 0.0000001)) return -1.0f;  // the margin is a double literal
+    const float sc = o.z / d.z;
+    const float hx = o.x - sc * d.x, hy = o.y - sc * d.y;
+    const float off = (hx - o.x) * invx;
+    u = hx;
+    v = hy;
+    if (off <= 0 || off > tm || u < 0 || u > 1 || v < 0 || v > 1) return -1.0f;
+    return off;  // NaN when the local d.x is 0: never below a hit distance
+}
+
+// The plane's surface: world point with pError = 1e-4 per axis plus the
+// transform's error (errorBound(4)), normal (0,0,1) turned against the
+// local direction and taken to world space through the inverse transpose
+// (worldToObject's third row), normalised.
+__device__ __forceinline__ void plane_surface(const PlaneDev &pl, float u, float v, float dz, V3 &p, V3 &n, V3 &perr)
+{
+    const float *m = pl.o2w;
+    const float eb4 = error_bound(4);
+    const float wp = m[12] * u + m[13] * v + m[14] * 0.0f + m[15];
+    perr.x = 0.0001f + (fabsf(m[0] * u) + fabsf(m[1] * v) + fabsf(m[2] * 0.0f) + fabsf(m[3])) * eb4 * fabsf(wp);
+    perr.y = 0.0001f + (fabsf(m[4] * u) + fabsf(m[5] * v) + fabsf(m[6] * 0.0f) + fabsf(m[7])) * eb4 * fabsf(wp);
+    perr.z = 0.0001f + (fabsf(m[8] * u) + fabsf(m[9] * v) + fabsf(m[10] * 0.0f) + fabsf(m[11])) * eb4 * fabsf(wp);
+    p = xform_point(m, mk(u, v, 0.0f));
+    const float on = dz > 0.0f ? -1.0f : 1.0f;
+    const float *mi = pl.w2o;
+    n = nrm(mk(mi[8] * on, mi[9] * on, mi[10] * on));
+}
+
 // Solid-angle cone sampling of a sphere light (the reference has no area
 // lights; this is the renderer's definition, shared with the oracle).
 __device__ __forceinline__ bool sphere_light_sample(const SphereDev &sp, V3 p, float u0, float u1, V3 &wi,

@@ -155,6 +155,36 @@ void compose_sphere(const float c[3], float r, rt::SphereDev &s)
     s.radius = r;
 }
 
+// Plane(corner, axisX, axisY, normal) (Plane.cu:27-48): the axis lengths
+// truncated to integers, the normal cross(axisX, axisY) turned towards the
+// given one and normalised (normalize = v * (1/length), Vector.cu:454-457),
+// objectToWorld = baseChange(corner, axisX, axisY, normal)
+// (Transform.cu:551-562), worldToObject its cofactor inverse.  Returns
+// false for an axis shorter than 1 (a zero-size plane).
+bool compose_plane(const float c[3], const float ax[3], const float ay[3], const float n[3], rt::PlaneDev &p)
+{
+    const float lx = std::sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+    const float ly = std::sqrt(ay[0] * ay[0] + ay[1] * ay[1] + ay[2] * ay[2]);
+    const int sx = (int)lx, sy = (int)ly;
+    if (sx < 1 || sy < 1) return false;
+    const float cr[3] = {(ax[1] * ay[2]) - (ax[2] * ay[1]), (ax[2] * ay[0]) - (ax[0] * ay[2]),
+                         (ax[0] * ay[1]) - (ax[1] * ay[0])};
+    const float dn = (cr[0] * n[0]) + (cr[1] * n[1]) + (cr[2] * n[2]);
+    float nn[3];
+    for (int k = 0; k < 3; ++k) nn[k] = dn > 0 ? cr[k] : -cr[k];
+    const float ix = 1.0f / lx, iy = 1.0f / ly;
+    const float X[3] = {ax[0] * ix * (float)sx, ax[1] * ix * (float)sx, ax[2] * ix * (float)sx};
+    const float Y[3] = {ay[0] * iy * (float)sy, ay[1] * iy * (float)sy, ay[2] * iy * (float)sy};
+    const float ln = std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+    if (!(ln > 0)) return false;
+    const float in = 1.0f / ln;
+    const float N[3] = {nn[0] * in, nn[1] * in, nn[2] * in};
+    const float m[16] = {X[0], Y[0], N[0], c[0], X[1], Y[1], N[1], c[1], X[2], Y[2], N[2], c[2], 0, 0, 0, 1};
+    for (int k = 0; k < 16; ++k) p.o2w[k] = m[k];
+    mat_inverse(p.o2w, p.w2o);
+    return true;
+}
+
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
@@ -170,6 +200,7 @@ struct rt_scene {
     std::vector<uint32_t> tri_mat;
     std::vector<rt_material> mats;
     std::vector<rt::SphereDev> spheres;
+    std::vector<rt::PlaneDev> planes;
     std::vector<rt_point_light> lights;
     rt_camera cam{};
     bool has_cam = false;
@@ -180,6 +211,7 @@ struct rt_scene {
     uint32_t *d_tri_mat = nullptr;
     rt_material *d_mat = nullptr;
     rt_point_light *d_pl = nullptr;
+    rt::PlaneDev *d_plane = nullptr;
     rt::SceneDev dev{};
     // wavefront buffers (grown on demand)
     rt::WaveBuffers wb;
@@ -353,7 +385,8 @@ static void free_device(rt_scene *s)
     if (s->d_tri_mat) (void)hipFree(s->d_tri_mat);
     if (s->d_mat) (void)hipFree(s->d_mat);
     if (s->d_pl) (void)hipFree(s->d_pl);
-    s->d_tri_xyz = nullptr; s->d_tri_mat = nullptr; s->d_mat = nullptr; s->d_pl = nullptr;
+    if (s->d_plane) (void)hipFree(s->d_plane);
+    s->d_tri_xyz = nullptr; s->d_tri_mat = nullptr; s->d_mat = nullptr; s->d_pl = nullptr; s->d_plane = nullptr;
     rt::free_arena(s);
     s->committed = false;
 }
@@ -427,6 +460,25 @@ rt_status rt_scene_add_sphere(rt_scene *s, const float center[3], float radius, 
     return RT_OK;
 }
 
+rt_status rt_scene_add_plane(rt_scene *s, const float corner[3], const float axis_x[3], const float axis_y[3],
+                             const float normal[3], uint32_t material)
+{
+    if (!s || !corner || !axis_x || !axis_y || !normal) { set_error("rt_scene_add_plane: null argument"); return RT_ERR_INVALID; }
+    if (material >= s->mats.size()) { set_error("rt_scene_add_plane: unknown material id"); return RT_ERR_INVALID; }
+    if (s->planes.size() >= (size_t)rt::kMaxPlanes) { set_error("rt_scene_add_plane: too many planes"); return RT_ERR_UNSUPPORTED; }
+    rt::PlaneDev d{};
+    if (!compose_plane(corner, axis_x, axis_y, normal, d)) {
+        set_error("rt_scene_add_plane: an axis is shorter than 1 or the axes are parallel (Plane truncates axis "
+                  "lengths to integers)");
+        return RT_ERR_INVALID;
+    }
+    d.material = material;
+    d.transparent = s->mats[material].type == RT_MAT_DIELECTRIC ? 1u : 0u;
+    s->planes.push_back(d);
+    s->committed = false;
+    return RT_OK;
+}
+
 rt_status rt_scene_add_point_light(rt_scene *s, const rt_point_light *l)
 {
     if (!s || !l) { set_error("rt_scene_add_point_light: null argument"); return RT_ERR_INVALID; }
@@ -472,6 +524,10 @@ rt_status rt_scene_commit(rt_scene *s)
     RT_HIP(hipMalloc(&s->d_tri_mat, std::max<size_t>(1, nt) * 4));
     RT_HIP(hipMalloc(&s->d_mat, nmat * sizeof(rt_material)));
     RT_HIP(hipMalloc(&s->d_pl, npl * sizeof(rt_point_light)));
+    RT_HIP(hipMalloc(&s->d_plane, std::max<size_t>(1, s->planes.size()) * sizeof(rt::PlaneDev)));
+    if (!s->planes.empty())
+        RT_HIP(hipMemcpyAsync(s->d_plane, s->planes.data(), s->planes.size() * sizeof(rt::PlaneDev),
+                              hipMemcpyHostToDevice, stream));
     if (!s->xyz.empty()) RT_HIP(hipMemcpyAsync(s->d_tri_xyz, s->xyz.data(), s->xyz.size() * 4, hipMemcpyHostToDevice, stream));
     if (nt) RT_HIP(hipMemcpyAsync(s->d_tri_mat, s->tri_mat.data(), nt * 4, hipMemcpyHostToDevice, stream));
     if (!s->mats.empty()) RT_HIP(hipMemcpyAsync(s->d_mat, s->mats.data(), s->mats.size() * sizeof(rt_material), hipMemcpyHostToDevice, stream));
@@ -488,6 +544,8 @@ rt_status rt_scene_commit(rt_scene *s)
     d.pl = s->d_pl;
     d.nsph = (uint32_t)s->spheres.size();
     for (uint32_t j = 0; j < d.nsph; ++j) d.sph[j] = s->spheres[j];
+    d.plane = s->d_plane;
+    d.nplane = (uint32_t)s->planes.size();
     d.npl = (uint32_t)s->lights.size();
     uint32_t na = 0;
     for (uint32_t j = 0; j < d.nsph; ++j)

@@ -32,6 +32,7 @@ class SceneDesc:
     max_depth: int = 8
     rr_depth: int = 3
     seed: int = 1
+    planes: list = field(default_factory=list)    # (corner, axis_x, axis_y, normal, material): Plane shapes
 
     @property
     def width(self):

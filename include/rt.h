@@ -224,6 +224,13 @@ rt_status rt_scene_add_mesh(rt_scene *scene, const float *xyz, uint32_t nv, cons
  * material must be RT_MAT_LIGHT. */
 rt_status rt_scene_add_sphere(rt_scene *scene, const float center[3], float radius, uint32_t material,
                               int is_light);
+/* Plane(corner, axis_x, axis_y, normal) (shapes/Plane.cu:27-48): the
+ * parallelogram corner + [0,1] axis_x + [0,1] axis_y, where the axis lengths
+ * are first truncated to integers as the reference does (an axis shorter
+ * than 1 is refused); normal picks the side of cross(axis_x, axis_y).
+ * Intersected after the spheres with the reference's Plane::intersect. */
+rt_status rt_scene_add_plane(rt_scene *scene, const float corner[3], const float axis_x[3], const float axis_y[3],
+                             const float normal[3], uint32_t material);
 rt_status rt_scene_add_point_light(rt_scene *scene, const rt_point_light *light);
 rt_status rt_scene_set_camera(rt_scene *scene, const rt_camera *cam);
 rt_status rt_scene_commit(rt_scene *scene);

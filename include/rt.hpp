@@ -1,5 +1,5 @@
 // rt.hpp -- C++17 host API over the C ABI of rt.h, with the reference
-// renderer's nouns: Scene, Shape (TriangleMesh, Sphere), Material
+// renderer's nouns: Scene, Shape (TriangleMesh, Sphere, Plane), Material
 // (Lambertian, Dielectric/Glass, Trowbridge, TrowbridgeBasic, Light, Mirror,
 // Plain),
 // PointLight, ProjectiveCamera, Film, PathTracerIntegrator, BVH.
@@ -264,6 +264,21 @@ public:
     bool is_light;
 };
 
+struct Vector3f {
+    float x = 0, y = 0, z = 0;
+};
+
+// Plane(bottomLeftCorner, axisX, axisY, normal) (shapes/Plane.cu:27-48): the
+// parallelogram corner + [0,1] axisX + [0,1] axisY after the reference's
+// truncation of the axis lengths to integers; normal picks the side.
+class Plane : public Shape {
+public:
+    Plane(Point3f corner, Vector3f axis_x, Vector3f axis_y, Vector3f normal, std::shared_ptr<Material> m)
+        : Shape(std::move(m)), corner(corner), axis_x(axis_x), axis_y(axis_y), normal(normal) {}
+    Point3f corner;
+    Vector3f axis_x, axis_y, normal;
+};
+
 // PointLight(position, color, intensity) (light/PointLight.cu:21)
 class PointLight {
 public:
@@ -402,7 +417,8 @@ public:
 
     // Scene::build(shapes, lights, ...): materials are registered once per
     // distinct object; meshes first in the given order (triangle ids follow
-    // it), spheres in the given order (intersection order, Scene.cu:221-227).
+    // it), then spheres, then planes, each in the given order (the analytic
+    // shapes' intersection order, Scene.cu:221-227).
     void build(const std::vector<std::shared_ptr<Shape>> &shapes, const std::vector<PointLight> &lights,
                const ProjectiveCamera &camera)
     {
@@ -427,6 +443,15 @@ public:
                 const uint32_t m = mat_id(s->material);
                 const float c[3] = {sp->center.x, sp->center.y, sp->center.z};
                 check(rt_scene_add_sphere(h_, c, sp->radius, m, sp->is_light ? 1 : 0));
+            }
+        for (const auto &s : shapes)
+            if (auto *pl = dynamic_cast<const Plane *>(s.get())) {
+                const uint32_t m = mat_id(s->material);
+                const float c[3] = {pl->corner.x, pl->corner.y, pl->corner.z};
+                const float ax[3] = {pl->axis_x.x, pl->axis_x.y, pl->axis_x.z};
+                const float ay[3] = {pl->axis_y.x, pl->axis_y.y, pl->axis_y.z};
+                const float n[3] = {pl->normal.x, pl->normal.y, pl->normal.z};
+                check(rt_scene_add_plane(h_, c, ax, ay, n, m));
             }
         for (const auto &l : lights) {
             rt_point_light pl{};

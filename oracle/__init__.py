@@ -60,6 +60,9 @@ _SIGS = {
     "orc_path_segments": (C.c_int, [_P, _U32, _U32, _U32, _U32, _U32, _U32, C.c_uint64, _P, _P, _P]),
     "orc_scene_create": (_P, [_P, _P, _U32, _P, _P, _P, _P, _U32, _P, _U32, _P, _U32, _P]),
     "orc_scene_free": (None, [_P]),
+    "orc_scene_add_planes": (None, [_P, _P, _P, _U32]),
+    "orc_plane_init": (None, [_P, _P, _P, _P, _P]),
+    "orc_plane_intersect": (C.c_int, [_P, C.POINTER(OrcRay), C.POINTER(_F), _P, _P, _P]),
     "orc_scene_camera": (None, [_P, _P, _P]),
     "orc_path": (C.c_int, [_P, _U32, _U32, _U32, _U32, _U32, _U32, C.c_uint64, _P]),
     "orc_bvh4_trace": (None, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _U32, _P, _P, C.POINTER(C.c_uint64),
@@ -160,6 +163,11 @@ class OrcScene:
                                         _p(self.sph), _p(self.sph_mat), _p(self.sph_light), self.sph.shape[0],
                                         C.cast(self.mats, _P), len(desc.materials), C.cast(self.lights, _P),
                                         len(desc.lights), C.byref(self.cam))
+        planes = getattr(desc, "planes", [])
+        if planes:
+            self.planes = np.array([[*c, *ax, *ay, *n] for c, ax, ay, n, m in planes], np.float32)
+            self.plane_mat = np.array([m for c, ax, ay, n, m in planes], np.uint32)
+            lib().orc_scene_add_planes(self.h, _p(self.planes), _p(self.plane_mat), len(planes))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -345,6 +353,31 @@ def bvh4_trace(nodes4, tris, rays, tmax=None, nthreads=8):
     lib().orc_bvh4_trace(_p(nodes4), _p(tris), *[_p(a) for a in rays], _p(tmax), n, _p(prim), _p(t),
                          C.byref(nn), C.byref(nt), C.byref(msp), nthreads)
     return prim, t, nn.value, nt.value, msp.value
+
+
+class Plane:
+    """Plane(corner, axis_x, axis_y, normal) restated (shapes/Plane.cu)."""
+
+    def __init__(self, corner, axis_x, axis_y, normal):
+        self.buf = np.zeros(32, np.float32)
+        args = [f32(v) for v in (corner, axis_x, axis_y, normal)]
+        lib().orc_plane_init(_p(self.buf), *[_p(a) for a in args])
+
+    @property
+    def o2w(self):
+        return self.buf[:16].reshape(4, 4)
+
+    @property
+    def w2o(self):
+        return self.buf[16:].reshape(4, 4)
+
+    def intersect(self, o, d, tmax=np.inf, hit_offset=np.inf):
+        """(hit, offset, point, normal, perr)."""
+        r = make_ray(o, d, tmax)
+        h = _F(hit_offset)
+        p, n, e = np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(3, np.float32)
+        ok = lib().orc_plane_intersect(_p(self.buf), C.byref(r), C.byref(h), _p(p), _p(n), _p(e))
+        return bool(ok), h.value, p, n, e
 
 
 class Sphere:

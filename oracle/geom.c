@@ -403,6 +403,98 @@ int orc_sphere_intersect(const orc_sphere *s, const orc_ray *r, float *hitOffset
     return 1;
 }
 
+/* Plane(bottomLeftCorner, axisX, axisY, normal) (shapes/Plane.cu:27-48):
+ * the axis lengths are truncated to integers (`int size_X = axisX.length()`),
+ * the normal is cross(axisX, axisY) turned towards the given one and
+ * normalised, objectToWorld = baseChange(corner, axisX, axisY, normal)
+ * (Transform.cu:551-562) and worldToObject its cofactor inverse. */
+void orc_plane_init(orc_plane *pl, const float corner[3], const float ax[3], const float ay[3], const float n[3])
+{
+    float lx = sqrtf(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+    float ly = sqrtf(ay[0] * ay[0] + ay[1] * ay[1] + ay[2] * ay[2]);
+    int sx = (int)lx, sy = (int)ly;
+    float c[3] = {(ax[1] * ay[2]) - (ax[2] * ay[1]), (ax[2] * ay[0]) - (ax[0] * ay[2]),
+                  (ax[0] * ay[1]) - (ax[1] * ay[0])};
+    float dn = (c[0] * n[0]) + (c[1] * n[1]) + (c[2] * n[2]);
+    float nn[3];
+    for (int k = 0; k < 3; ++k) nn[k] = dn > 0 ? c[k] : -c[k];
+    /* normalize(v) = v * (1 / length) (Vector.cu:454-457) */
+    float ix = 1.0f / lx, iy = 1.0f / ly;
+    float X[3] = {ax[0] * ix * (float)sx, ax[1] * ix * (float)sx, ax[2] * ix * (float)sx};
+    float Y[3] = {ay[0] * iy * (float)sy, ay[1] * iy * (float)sy, ay[2] * iy * (float)sy};
+    float ln = sqrtf(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+    float in = 1.0f / ln;
+    float N[3] = {nn[0] * in, nn[1] * in, nn[2] * in};
+    float m[16] = {X[0], Y[0], N[0], corner[0], X[1], Y[1], N[1], corner[1], X[2], Y[2], N[2], corner[2], 0, 0, 0, 1};
+    memcpy(pl->o2w, m, sizeof m);
+    orc_mat4_inverse(pl->o2w, pl->w2o);
+}
+
+/* Transform::operator()(Point, pError) (Transform.cu:239-263): the error
+ * ADDS errorBound(4) * |row| . |p| * |w| to pError. */
+static void xform_point_add_err4(const float *m, const float p[3], float out[3], float err[3])
+{
+    float eb4 = orc_error_bound(4);
+    float xp = m[0] * p[0] + m[1] * p[1] + m[2] * p[2] + m[3];
+    float yp = m[4] * p[0] + m[5] * p[1] + m[6] * p[2] + m[7];
+    float zp = m[8] * p[0] + m[9] * p[1] + m[10] * p[2] + m[11];
+    float wp = m[12] * p[0] + m[13] * p[1] + m[14] * p[2] + m[15];
+    for (int i = 0; i < 3; ++i) {
+        float sabs = fabsf(m[i * 4 + 0] * p[0]) + fabsf(m[i * 4 + 1] * p[1]) + fabsf(m[i * 4 + 2] * p[2]) +
+                     fabsf(m[i * 4 + 3]);
+        err[i] += sabs * eb4 * fabsf(wp);
+    }
+    float inv = 1.0f / wp;
+    out[0] = inv * xp; out[1] = inv * yp; out[2] = inv * zp;
+}
+
+/* Plane::intersect / intersects (Plane.cu:58-164).  The ray goes to object
+ * space with Transform::operator()(Ray) (Transform.cu:216-237: the origin
+ * moves to the edge of its error bounds along the direction, which is not
+ * normalised, and tMax shrinks by the same amount); a ray whose |d.z| is not
+ * above FLOAT_ERROR_MARGIN (1e-7, types.h:39) misses; the distance is
+ * Ray::getOffset, (hit.x - origin.x) * invDirection.x (Ray.cu:69-72), so a
+ * ray with local d.x == 0 gets NaN and never hits; the hit must lie in
+ * [0,1]^2 with 0 < offset <= tMax and offset < hitOffset (strict).  The
+ * surface: world point with pError = 1e-4 per axis plus the transform's
+ * error, normal (0,0,1) turned against the local direction, taken to world
+ * space through the inverse transpose (the rows of worldToObject). */
+int orc_plane_intersect(const orc_plane *pl, const orc_ray *r, float *hitOffset, float p[3], float n[3],
+                        float perr[3])
+{
+    float oerr[3] = {0, 0, 0}, o[3], d[3];
+    xform_point_add_err4(pl->w2o, r->o, o, oerr);
+    const float *m = pl->w2o;
+    d[0] = m[0] * r->d[0] + m[1] * r->d[1] + m[2] * r->d[2];
+    d[1] = m[4] * r->d[0] + m[5] * r->d[1] + m[6] * r->d[2];
+    d[2] = m[8] * r->d[0] + m[9] * r->d[1] + m[10] * r->d[2];
+    float tmax = r->tmax;
+    float ls = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (ls > 0) {
+        float dt = (fabsf(d[0]) * oerr[0] + fabsf(d[1]) * oerr[1] + fabsf(d[2]) * oerr[2]) / ls;
+        o[0] += d[0] * dt; o[1] += d[1] * dt; o[2] += d[2] * dt;
+        tmax -= dt;
+    }
+    float invx = 1.0f / d[0];
+    float cosine = d[2];
+    if (!(fabsf(cosine) > 0.0000001)) return 0;
+    float sc = o[2] / d[2];
+    float hp[3] = {o[0] - sc * d[0], o[1] - sc * d[1], o[2] - sc * d[2]};
+    float off = (hp[0] - o[0]) * invx;
+    float u = hp[0], v = hp[1];
+    if (off <= 0 || off > tmax || u < 0 || u > 1 || v < 0 || v > 1) return 0;
+    if (!(off < *hitOffset)) return 0;
+    *hitOffset = off;
+    if (p) {
+        float on = d[2] > 0.0 ? -1.0f : 1.0f; /* sameHemisphere((0,0,1), d): d.z > 0 */
+        float loc[3] = {u, v, 0};
+        perr[0] = perr[1] = perr[2] = 0.0001f;
+        xform_point_add_err4(pl->o2w, loc, p, perr);
+        n[0] = m[8] * on; n[1] = m[9] * on; n[2] = m[10] * on;
+    }
+    return 1;
+}
+
 /* Barycentrics of the watertight test, b_i = e_i * (1/det)
  * (shapes/Triangle.cu:161-290, the same steps as orc_tri_intersects). */
 void orc_tri_barycentrics(const orc_ray *r, const float *wp0, const float *wp1, const float *wp2, float b[3])

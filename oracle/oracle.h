@@ -147,6 +147,19 @@ int orc_sphere_intersects(const orc_sphere *s, const orc_ray *r, float *hitOffse
 int orc_sphere_intersect(const orc_sphere *s, const orc_ray *r, float *hitOffset,
                          float p[3], float n[3], float perr[3]);
 
+typedef struct {
+    float o2w[16], w2o[16];
+} orc_plane;
+
+/* Plane(corner, axisX, axisY, normal) (Plane.cu:27-48). */
+void orc_plane_init(orc_plane *p, const float corner[3], const float axis_x[3], const float axis_y[3],
+                    const float normal[3]);
+/* Plane::intersect (Plane.cu:58-125): 1 and lowers *hitOffset on a hit; p,
+ * n, perr (may be NULL) get the world point, the normal facing the ray and
+ * the point's error bound. */
+int orc_plane_intersect(const orc_plane *pl, const orc_ray *r, float *hitOffset, float p[3], float n[3],
+                        float perr[3]);
+
 /* MESA cofactor inverse used by the reference (Matrix4x4.cu:303-440). */
 void orc_mat4_inverse(const float *m, float *out);
 

@@ -72,6 +72,9 @@ orc_scene *orc_scene_create(const float *xyz, const uint32_t *idx, uint32_t nt, 
                             const float *spheres, const uint32_t *sph_mat, const uint8_t *sph_light,
                             uint32_t nsph, const orc_material *mats, uint32_t nmat,
                             const orc_point_light *lights, uint32_t nlight, const orc_camera *cam);
+/* Plane shapes (12 floats each: corner, axis_x, axis_y, normal), tested
+ * after the spheres. */
+void orc_scene_add_planes(orc_scene *s, const float *planes, const uint32_t *mats, uint32_t n);
 void orc_scene_free(orc_scene *s);
 void orc_scene_camera(const orc_scene *s, float r2c[16], float c2w[16]);
 

@@ -494,6 +494,9 @@ struct orc_scene {
     float r2c[16], c2w[16];
     orc_refbvh *bvh;
     uint32_t *idx_seq;
+    orc_plane *plane;
+    uint32_t *plane_mat;
+    uint32_t nplane;
 };
 
 orc_scene *orc_scene_create(const float *xyz, const uint32_t *idx, uint32_t nt, const uint32_t *tri_mat,
@@ -532,9 +535,25 @@ orc_scene *orc_scene_create(const float *xyz, const uint32_t *idx, uint32_t nt, 
     return s;
 }
 
+/* Plane shapes, tested after the spheres (Scene.cu:221-227 tests the
+ * analytic shapes in insertion order): 12 floats each (corner, axis_x,
+ * axis_y, normal) as Plane(corner, axisX, axisY, normal) takes them. */
+void orc_scene_add_planes(orc_scene *s, const float *planes, const uint32_t *mats, uint32_t n)
+{
+    s->plane = realloc(s->plane, sizeof(orc_plane) * (s->nplane + n));
+    s->plane_mat = realloc(s->plane_mat, sizeof(uint32_t) * (s->nplane + n));
+    for (uint32_t k = 0; k < n; ++k) {
+        const float *q = planes + 12 * k;
+        orc_plane_init(&s->plane[s->nplane + k], q, q + 3, q + 6, q + 9);
+        s->plane_mat[s->nplane + k] = mats[k];
+    }
+    s->nplane += n;
+}
+
 void orc_scene_free(orc_scene *s)
 {
     if (!s) return;
+    free(s->plane); free(s->plane_mat);
     orc_refbvh_free(s->bvh);
     free(s->xyz); free(s->idx_seq); free(s->tri_mat); free(s->mat); free(s->tri_transparent);
     free(s->sph); free(s->sph_mat); free(s->sph_light); free(s->pl); free(s);
@@ -589,6 +608,14 @@ static hit_rec scene_intersect(const orc_scene *s, v3 o, v3 d)
             h.p = mk(p[0], p[1], p[2]); h.n = mk(n[0], n[1], n[2]); h.perr = mk(e[0], e[1], e[2]);
         }
     }
+    for (uint32_t k = 0; k < s->nplane; ++k) {
+        float p[3], n[3], e[3];
+        float hh = h.t;
+        if (orc_plane_intersect(&s->plane[k], &r, &hh, p, n, e)) {
+            h.kind = 3; h.id = (int32_t)k; h.t = hh;
+            h.p = mk(p[0], p[1], p[2]); h.n = nrm(mk(n[0], n[1], n[2])); h.perr = mk(e[0], e[1], e[2]);
+        }
+    }
     if (h.kind == 1) {
         /* surface data of Triangle::intersect (Triangle.cu:161-431) */
         const float *t = s->xyz + 9 * h.id;
@@ -626,6 +653,10 @@ static int occluded(const orc_scene *s, v3 o, v3 d, float dist, int skip_sphere)
         if ((int)j == skip_sphere) continue;
         if (s->mat[s->sph_mat[j]].type == ORC_MAT_DIELECTRIC) continue; /* Sphere.cu:362-363 */
         if (orc_sphere_intersects(&s->sph[j], &r, &t)) hit = 1;
+    }
+    for (uint32_t k = 0; k < s->nplane; ++k) {
+        if (s->mat[s->plane_mat[k]].type == ORC_MAT_DIELECTRIC) continue; /* Plane.cu:128-129 */
+        if (orc_plane_intersect(&s->plane[k], &r, &t, NULL, NULL, NULL)) hit = 1;
     }
     return hit && !(t > dist);
 }
@@ -727,7 +758,8 @@ static int path_impl(const orc_scene *s, uint32_t px, uint32_t py, uint32_t samp
         }
         hit_rec h = scene_intersect(s, o, d);
         if (h.kind == 0) break;
-        const orc_material *m = &s->mat[h.kind == 1 ? s->tri_mat[h.id] : s->sph_mat[h.id]];
+        const orc_material *m =
+            &s->mat[h.kind == 1 ? s->tri_mat[h.id] : h.kind == 2 ? s->sph_mat[h.id] : s->plane_mat[h.id]];
         if (cache_rgb && depth >= cache_depth && cacheable(m)) {
             L = add(L, mk(beta.x * cache_rgb[0], beta.y * cache_rgb[1], beta.z * cache_rgb[2]));
             break;

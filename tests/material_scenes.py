@@ -92,3 +92,36 @@ def cornell_emissive_ceiling(width=64, height=64, spp=16):
     d.tri_mat = tm
     d.lights = []
     return d
+
+
+# the box of cornellBoxGuion.cu:60-92 as Plane(corner, axisX, axisY, normal)
+REF_PLANES = {
+    "plane1": ((0.5, -0.5, -0.5), (0, 1, 0), (0, 0, 1), (-0.5, 0, 0)),   # back wall x = 0.5
+    "plane2": ((0.5, 0.5, -0.5), (-1, 0, 0), (0, 0, 1), (0, -0.5, 0)),   # y = 0.5
+    "plane4": ((0.5, -0.5, -0.5), (-1, 0, 0), (0, 0, 1), (0, 0.5, 0)),   # y = -0.5
+    "plane5": ((-0.5, -0.5, -0.5), (1, 0, 0), (0, 1, 0), (0, 0, 0.5)),   # floor z = -0.5
+    "plane6": ((-0.5, -0.5, 0.5), (1, 0, 0), (0, 1, 0), (0, 0, -0.5)),   # ceiling z = 0.5
+}
+
+
+def cornell_planes(width=64, height=64, spp=16, with_mesh=False):
+    """The reference driver's Cornell box made of Plane shapes (white back
+    wall, floor and ceiling, green y = +0.5, red y = -0.5 walls: the
+    driver's materials), the Cornell workload's two spheres and its point
+    light.  with_mesh adds a small triangle mesh (a box in a corner)."""
+    d = scenes.cornell(width=width, height=height, spp=spp)
+    white, green, red = mat(LAMB, (1, 1, 1)), mat(LAMB, (0, 1, 0)), mat(LAMB, (1, 0, 0))
+    d.materials = list(d.materials) + [white, green, red]
+    iw, ig, ir = len(d.materials) - 3, len(d.materials) - 2, len(d.materials) - 1
+    mats = {"plane1": iw, "plane2": ig, "plane4": ir, "plane5": iw, "plane6": iw}
+    d.planes = [(*REF_PLANES[k], mats[k]) for k in ("plane1", "plane2", "plane4", "plane5", "plane6")]
+    if with_mesh:
+        v = np.array([[0.1, 0.1, -0.5], [0.4, 0.1, -0.5], [0.4, 0.4, -0.5], [0.1, 0.4, -0.5],
+                      [0.1, 0.1, -0.2], [0.4, 0.1, -0.2], [0.4, 0.4, -0.2], [0.1, 0.4, -0.2]], np.float32)
+        f = np.array([[4, 5, 6], [4, 6, 7], [0, 1, 5], [0, 5, 4], [1, 2, 6], [1, 6, 5], [2, 3, 7], [2, 7, 6],
+                      [3, 0, 4], [3, 4, 7]], np.uint32)
+        d.xyz, d.idx, d.tri_mat = v, f, np.zeros(len(f), np.uint32)
+        d.spheres = [d.spheres[1]]
+    else:
+        d.xyz, d.idx, d.tri_mat = np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32), np.zeros(0, np.uint32)
+    return d

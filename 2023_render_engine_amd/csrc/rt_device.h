@@ -189,7 +189,7 @@ constexpr int kTravBlock = 128;
 #define RT_CLAIM_CHUNK 128
 #endif
 #ifndef RT_CLAIM_MIN
-#define RT_CLAIM_MIN 8
+#define RT_CLAIM_MIN 64
 #endif
 #ifndef RT_LEAF_BATCH
 #define RT_LEAF_BATCH 16

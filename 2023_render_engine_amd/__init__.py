@@ -49,7 +49,8 @@ MAT_LAMBERTIAN, MAT_DIELECTRIC, MAT_TROWBRIDGE, MAT_TROWBRIDGE_BASIC, MAT_LIGHT 
 
 class Material(C.Structure):
     _fields_ = [("type", C.c_uint32), ("kd", C.c_float * 3), ("ior", C.c_float), ("roughness", C.c_float),
-                ("eta", C.c_float * 3), ("k", C.c_float * 3), ("intensity", C.c_float), ("ke", C.c_float * 3)]
+                ("eta", C.c_float * 3), ("k", C.c_float * 3), ("intensity", C.c_float), ("ke", C.c_float * 3),
+                ("texture", C.c_uint32)]
 
 
 class PointLight(C.Structure):
@@ -128,6 +129,8 @@ _SIGS = {
     "rt_scene_add_mesh": (C.c_int, [_P, _P, _U32, _P, _U32, _U32, C.POINTER(_U32)]),
     "rt_scene_add_sphere": (C.c_int, [_P, _P, C.c_float, _U32, C.c_int]),
     "rt_scene_add_plane": (C.c_int, [_P, _P, _P, _P, _P, _U32]),
+    "rt_scene_add_mesh_uv": (C.c_int, [_P, _P, _U32, _P, _U32, _P, _U32, _P]),
+    "rt_scene_add_texture": (C.c_int, [_P, _P, _U32, _U32, _P]),
     "rt_scene_add_point_light": (C.c_int, [_P, C.POINTER(PointLight)]),
     "rt_scene_set_camera": (C.c_int, [_P, C.POINTER(Camera)]),
     "rt_scene_commit": (C.c_int, [_P]),
@@ -393,6 +396,11 @@ class Scene:
         for m in desc.material_structs():
             mid = C.c_uint32()
             check(L.rt_scene_add_material(h, C.byref(m), C.byref(mid)))
+        for rgb in getattr(desc, "textures", []):
+            t = np.ascontiguousarray(rgb, np.uint8)
+            check(L.rt_scene_add_texture(h, _ptr(t), t.shape[1], t.shape[0], None))
+        uv = getattr(desc, "tri_uv", None)
+        uv = None if uv is None else np.ascontiguousarray(uv, np.float32).reshape(-1, 6)
         xyz = np.ascontiguousarray(desc.xyz, np.float32)
         tm = np.asarray(desc.tri_mat, np.uint32)
         idx = np.ascontiguousarray(desc.idx, np.uint32)
@@ -403,7 +411,12 @@ class Scene:
             m = tm[start]
             end = start + int(np.argmax(tm[start:] != m)) if np.any(tm[start:] != m) else nt
             sub = np.ascontiguousarray(idx[start:end])
-            check(L.rt_scene_add_mesh(h, _ptr(xyz), xyz.shape[0], _ptr(sub), sub.shape[0], int(m), None))
+            if uv is None:
+                check(L.rt_scene_add_mesh(h, _ptr(xyz), xyz.shape[0], _ptr(sub), sub.shape[0], int(m), None))
+            else:
+                suv = np.ascontiguousarray(uv[start:end])
+                check(L.rt_scene_add_mesh_uv(h, _ptr(xyz), xyz.shape[0], _ptr(sub), sub.shape[0], _ptr(suv), int(m),
+                                             None))
             start = end
         for c, r, m, is_light in desc.spheres:
             cc = np.asarray(c, np.float32)

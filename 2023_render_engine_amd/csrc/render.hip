@@ -399,7 +399,42 @@ __global__ __launch_bounds__(kBlock) void k_classify(SceneDev sc0, BatchDev bt, 
 struct Surface {
     V3 p, n, perr;
     uint32_t mat;
+    float su, sv;  // SurfaceInteraction::surfacePoint, filled when the scene has textures
 };
+
+// Texture::getColor (textures/Texture.cu:172-183): texel
+// ((unsigned)(u * width), (unsigned)(v * height)) clamped to the last
+// row / column, channels / 255.  The float-to-unsigned conversion is taken
+// as the reference's x86-64 host build performs it (truncation to a 64-bit
+// integer, low 32 bits; NaN to 0), so a coordinate in (-1, 0) reads texel 0
+// and one below -1 wraps to the last.
+__device__ __forceinline__ uint32_t texel_index(float u, uint32_t n)
+{
+    float f = u * (float)n;
+    if (f != f) f = 0.0f;
+    f = fminf(fmaxf(f, -4.6e18f), 4.6e18f);
+    const uint32_t x = (uint32_t)(unsigned long long)(long long)f;
+    return x < n - 1 ? x : n - 1;
+}
+
+__device__ __forceinline__ V3 texture_color(const SceneDev &sc, uint32_t t, float u, float v)
+{
+    const TexDev &tx = sc.tex[t];
+    const uint32_t x = texel_index(u, tx.width), y = texel_index(v, tx.height);
+    const uint32_t c = sc.texels[tx.offset + (size_t)y * tx.width + x];
+    return mk((float)(c & 0xffu) / 255.0f, (float)((c >> 8) & 0xffu) / 255.0f, (float)((c >> 16) & 0xffu) / 255.0f);
+}
+
+// A textured Lambertian's kd, or a textured plain material's camera-ray kd,
+// times the texel at the hit (LambertianBSDF.cu:22-74, PlainMaterial.cu:62-75).
+__device__ __forceinline__ void apply_texture(const SceneDev &sc, const Surface &s, rt_material &m)
+{
+    if (m.texture == 0 || !(m.type == RT_MAT_LAMBERTIAN || m.type == RT_MAT_PLAIN)) return;
+    const V3 c = texture_color(sc, m.texture - 1, s.su, s.sv);
+    m.kd[0] = m.kd[0] * c.x;
+    m.kd[1] = m.kd[1] * c.y;
+    m.kd[2] = m.kd[2] * c.z;
+}
 
 __device__ __forceinline__ void triangle_surface(const SceneDev &sc, int32_t id, V3 o, V3 d, Surface &s)
 {
@@ -436,6 +471,18 @@ __device__ __forceinline__ void triangle_surface(const SceneDev &sc, int32_t id,
     if (dot(n, d) > 0) n = neg(n);  // normal faces the ray (Triangle.cu:368-373)
     s.n = n;
     s.mat = sc.tri_mat[id];
+    if (sc.ntex) {
+        // surfacePoint = b0 uv0 + b1 uv1 + b2 uv2 (Triangle.cu:356-358), the
+        // default corners (0,0), (1,0), (1,1) without texture coordinates
+        // (:134-152)
+        float u0 = 0, v0 = 0, u1 = 1, v1 = 0, u2 = 1, v2 = 1;
+        if (sc.tri_uv) {
+            const float *q = sc.tri_uv + 6 * (size_t)id;
+            u0 = q[0]; v0 = q[1]; u1 = q[2]; v1 = q[3]; u2 = q[4]; v2 = q[5];
+        }
+        s.su = b0 * u0 + b1 * u1 + b2 * u2;
+        s.sv = b0 * v0 + b1 * v1 + b2 * v2;
+    }
 }
 
 __device__ __forceinline__ void surface_of(const SceneDev &sc, const PathsDev &p, uint32_t slot, Surface &s, V3 &d)
@@ -452,12 +499,27 @@ __device__ __forceinline__ void surface_of(const SceneDev &sc, const PathsDev &p
         const float ts = sphere_core(sp, o, d, INFINITY, so, sd);
         sphere_surface(sp, so, sd, ts, s.p, s.n, s.perr);
         s.mat = sp.material;
+        if (sc.ntex) {
+            // Sphere::localToSurface (Sphere.cu:269-275): (theta, phi) of the
+            // object-space hit point, in degrees (geometricMath.cu:49-57)
+            V3 hp = add(so, mul(sd, ts));
+            const float dist = sqrtf(hp.x * hp.x + hp.y * hp.y + hp.z * hp.z);
+            const float scl = sp.radius / dist;
+            hp = mk(hp.x * scl, hp.y * scl, hp.z * scl);
+            if (hp.x == 0 && hp.y == 0) hp.x = 1e-5f * sp.radius;
+            const float th = atan2f(sqrtf(hp.x * hp.x + hp.y * hp.y), hp.z);
+            const float ph = atan2f(hp.y, hp.x);
+            s.su = (float)((double)th * 180.0 / 3.14159265358979323846);
+            s.sv = (float)((double)ph * 180.0 / 3.14159265358979323846);
+        }
     } else {
         const PlaneDev &pl = sc.plane[k - sc.nsph];
         float u, v, dz;
         plane_core(pl, o, d, INFINITY, u, v, dz);
         plane_surface(pl, u, v, dz, s.p, s.n, s.perr);
         s.mat = pl.material;
+        s.su = u;
+        s.sv = v;
     }
 }
 
@@ -544,6 +606,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
             if (CLS != kQMiss) {
                 surface_of(sc, p, slot, s, d);
                 m = sc.mat[s.mat];
+                if (sc.ntex) apply_texture(sc, s, m);
                 g = load_rng(p, slot, stream_base_of<B>(sc, wp));
                 meta = p.meta[slot];
                 depth = meta & 0xffffu;

@@ -47,11 +47,15 @@ struct SceneDev {
     const rt_point_light *pl;
     SphereDev sph[kMaxSpheres];
     const PlaneDev *plane;  // nplane planes, tested after the spheres
+    const TexDev *tex;      // ntex textures over `texels`
+    const uint32_t *texels;
+    const float *tri_uv;    // 6 floats per triangle, or null: the default (0,0), (1,0), (1,1)
     CameraDev cam;
     uint32_t nsph, npl, nlights;  // nlights = npl + area lights (NEE slots per path)
     uint32_t area_light[kMaxSpheres];  // sphere index of area light a
     uint32_t nmat;
     uint32_t nplane;
+    uint32_t ntex;
     // batched rendering: the scene's own PCG seed and stream offset (a
     // single-scene wave takes both from WaveParams)
     uint32_t seed;

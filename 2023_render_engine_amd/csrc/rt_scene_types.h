@@ -39,6 +39,12 @@ struct PlaneDev {
     uint32_t pad[2];
 };
 
+// Texture (textures/Texture.cu): width x height texels from offset in the
+// scene's texel array, 0x00BBGGRR, row y = pixelMatrix[y].
+struct TexDev {
+    uint32_t offset, width, height, pad;
+};
+
 constexpr int kMaxSpheres = 16;
 constexpr int kMaxPlanes = 64;
 constexpr int kMaxPointLights = 32;

@@ -201,6 +201,10 @@ struct rt_scene {
     std::vector<rt_material> mats;
     std::vector<rt::SphereDev> spheres;
     std::vector<rt::PlaneDev> planes;
+    std::vector<float> tri_uv;       // 6 floats per triangle once a mesh brings texture coordinates
+    bool has_uv = false;
+    std::vector<rt::TexDev> textures;
+    std::vector<uint32_t> texels;    // 0x00BBGGRR
     std::vector<rt_point_light> lights;
     rt_camera cam{};
     bool has_cam = false;
@@ -212,6 +216,9 @@ struct rt_scene {
     rt_material *d_mat = nullptr;
     rt_point_light *d_pl = nullptr;
     rt::PlaneDev *d_plane = nullptr;
+    float *d_tri_uv = nullptr;
+    rt::TexDev *d_tex = nullptr;
+    uint32_t *d_texels = nullptr;
     rt::SceneDev dev{};
     // wavefront buffers (grown on demand)
     rt::WaveBuffers wb;
@@ -386,6 +393,10 @@ static void free_device(rt_scene *s)
     if (s->d_mat) (void)hipFree(s->d_mat);
     if (s->d_pl) (void)hipFree(s->d_pl);
     if (s->d_plane) (void)hipFree(s->d_plane);
+    if (s->d_tri_uv) (void)hipFree(s->d_tri_uv);
+    if (s->d_tex) (void)hipFree(s->d_tex);
+    if (s->d_texels) (void)hipFree(s->d_texels);
+    s->d_tri_uv = nullptr; s->d_tex = nullptr; s->d_texels = nullptr;
     s->d_tri_xyz = nullptr; s->d_tri_mat = nullptr; s->d_mat = nullptr; s->d_pl = nullptr; s->d_plane = nullptr;
     rt::free_arena(s);
     s->committed = false;
@@ -424,6 +435,12 @@ rt_status rt_scene_add_material(rt_scene *s, const rt_material *m, uint32_t *id)
 rt_status rt_scene_add_mesh(rt_scene *s, const float *xyz, uint32_t nv, const uint32_t *idx, uint32_t nt,
                             uint32_t material, uint32_t *first_id)
 {
+    return rt_scene_add_mesh_uv(s, xyz, nv, idx, nt, nullptr, material, first_id);
+}
+
+rt_status rt_scene_add_mesh_uv(rt_scene *s, const float *xyz, uint32_t nv, const uint32_t *idx, uint32_t nt,
+                               const float *uv, uint32_t material, uint32_t *first_id)
+{
     if (!s || (nt && (!xyz || !idx))) { set_error("rt_scene_add_mesh: null argument"); return RT_ERR_INVALID; }
     if (material >= s->mats.size()) { set_error("rt_scene_add_mesh: unknown material id"); return RT_ERR_INVALID; }
     const size_t base = s->tri_mat.size();
@@ -435,7 +452,46 @@ rt_status rt_scene_add_mesh(rt_scene *s, const float *xyz, uint32_t nv, const ui
         for (int v = 0; v < 3; ++v)
             for (int k = 0; k < 3; ++k) s->xyz[(base + i) * 9 + v * 3 + k] = xyz[3 * (size_t)idx[3 * i + v] + k];
     s->tri_mat.resize(base + nt, material);
+    // texture coordinates are kept once any mesh brings its own; until then
+    // every triangle has the default (0,0), (1,0), (1,1)
+    static const float kDefaultUv[6] = {0, 0, 1, 0, 1, 1};
+    if (uv && !s->has_uv) {
+        s->tri_uv.resize(base * 6);
+        for (size_t i = 0; i < base; ++i) std::copy(kDefaultUv, kDefaultUv + 6, s->tri_uv.begin() + 6 * i);
+        s->has_uv = true;
+    }
+    if (s->has_uv) {
+        s->tri_uv.resize((base + nt) * 6);
+        for (uint32_t i = 0; i < nt; ++i)
+            std::copy(uv ? uv + 6 * (size_t)i : kDefaultUv, (uv ? uv + 6 * (size_t)i : kDefaultUv) + 6,
+                      s->tri_uv.begin() + 6 * (base + i));
+    }
     if (first_id) *first_id = (uint32_t)base;
+    s->committed = false;
+    return RT_OK;
+}
+
+rt_status rt_scene_add_texture(rt_scene *s, const uint8_t *rgb, uint32_t width, uint32_t height, uint32_t *id)
+{
+    if (!s || !rgb) { set_error("rt_scene_add_texture: null argument"); return RT_ERR_INVALID; }
+    if (width == 0 || height == 0 || (uint64_t)width * height > (1ull << 28)) {
+        set_error("rt_scene_add_texture: size must be 1..2^28 texels");
+        return RT_ERR_INVALID;
+    }
+    rt::TexDev t{};
+    t.offset = (uint32_t)s->texels.size();
+    if ((uint64_t)t.offset + (uint64_t)width * height > (1ull << 30)) {
+        set_error("rt_scene_add_texture: more than 2^30 texels in the scene");
+        return RT_ERR_UNSUPPORTED;
+    }
+    t.width = width;
+    t.height = height;
+    const size_t n = (size_t)width * height;
+    s->texels.resize(t.offset + n);
+    for (size_t i = 0; i < n; ++i)
+        s->texels[t.offset + i] = (uint32_t)rgb[3 * i] | ((uint32_t)rgb[3 * i + 1] << 8) | ((uint32_t)rgb[3 * i + 2] << 16);
+    s->textures.push_back(t);
+    if (id) *id = (uint32_t)s->textures.size() - 1;
     s->committed = false;
     return RT_OK;
 }
@@ -528,6 +584,25 @@ rt_status rt_scene_commit(rt_scene *s)
     if (!s->planes.empty())
         RT_HIP(hipMemcpyAsync(s->d_plane, s->planes.data(), s->planes.size() * sizeof(rt::PlaneDev),
                               hipMemcpyHostToDevice, stream));
+    for (const auto &m : s->mats)
+        if (m.texture > s->textures.size()) {
+            set_error("rt_scene_commit: a material names a texture the scene does not have");
+            return RT_ERR_INVALID;
+        }
+    if (!s->textures.empty()) {
+        // texture coordinates are only read when the scene has textures
+        RT_HIP(hipMalloc(&s->d_tex, s->textures.size() * sizeof(rt::TexDev)));
+        RT_HIP(hipMalloc(&s->d_texels, s->texels.size() * sizeof(uint32_t)));
+        RT_HIP(hipMemcpyAsync(s->d_tex, s->textures.data(), s->textures.size() * sizeof(rt::TexDev),
+                              hipMemcpyHostToDevice, stream));
+        RT_HIP(hipMemcpyAsync(s->d_texels, s->texels.data(), s->texels.size() * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, stream));
+        if (s->has_uv && !s->tri_uv.empty()) {
+            RT_HIP(hipMalloc(&s->d_tri_uv, s->tri_uv.size() * sizeof(float)));
+            RT_HIP(hipMemcpyAsync(s->d_tri_uv, s->tri_uv.data(), s->tri_uv.size() * sizeof(float),
+                                  hipMemcpyHostToDevice, stream));
+        }
+    }
     if (!s->xyz.empty()) RT_HIP(hipMemcpyAsync(s->d_tri_xyz, s->xyz.data(), s->xyz.size() * 4, hipMemcpyHostToDevice, stream));
     if (nt) RT_HIP(hipMemcpyAsync(s->d_tri_mat, s->tri_mat.data(), nt * 4, hipMemcpyHostToDevice, stream));
     if (!s->mats.empty()) RT_HIP(hipMemcpyAsync(s->d_mat, s->mats.data(), s->mats.size() * sizeof(rt_material), hipMemcpyHostToDevice, stream));
@@ -546,6 +621,10 @@ rt_status rt_scene_commit(rt_scene *s)
     for (uint32_t j = 0; j < d.nsph; ++j) d.sph[j] = s->spheres[j];
     d.plane = s->d_plane;
     d.nplane = (uint32_t)s->planes.size();
+    d.tex = s->d_tex;
+    d.texels = s->d_texels;
+    d.tri_uv = s->d_tri_uv;
+    d.ntex = (uint32_t)s->textures.size();
     d.npl = (uint32_t)s->lights.size();
     uint32_t na = 0;
     for (uint32_t j = 0; j < d.nsph; ++j)

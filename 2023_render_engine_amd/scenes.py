@@ -33,6 +33,8 @@ class SceneDesc:
     rr_depth: int = 3
     seed: int = 1
     planes: list = field(default_factory=list)    # (corner, axis_x, axis_y, normal, material): Plane shapes
+    textures: list = field(default_factory=list)  # (height, width, 3) uint8 arrays, row y = pixelMatrix[y]
+    tri_uv: object = None                         # (nt, 6) float32 corner texture coordinates, or None
 
     @property
     def width(self):
@@ -48,7 +50,7 @@ class SceneDesc:
         for m in self.materials:
             out.append(Material(m["type"], f3(m.get("kd", (0, 0, 0))), m.get("ior", 1.5), m.get("roughness", 0.0),
                                 f3(m.get("eta", (1, 1, 1))), f3(m.get("k", (0, 0, 0))), m.get("intensity", 0.0),
-                                f3(m.get("ke", (0, 0, 0)))))
+                                f3(m.get("ke", (0, 0, 0))), int(m.get("texture", 0))))
         return out
 
     def camera_struct(self):
@@ -62,9 +64,11 @@ class SceneDesc:
         return Scene(ctx, self)
 
 
-def _mat(type_, kd=(0, 0, 0), ior=1.5, roughness=0.0, eta=(1, 1, 1), k=(0, 0, 0), intensity=0.0, ke=(0, 0, 0)):
+def _mat(type_, kd=(0, 0, 0), ior=1.5, roughness=0.0, eta=(1, 1, 1), k=(0, 0, 0), intensity=0.0, ke=(0, 0, 0),
+         texture=0):
     return dict(type=type_, kd=tuple(float(x) for x in kd), ior=float(ior), roughness=float(roughness),
-                eta=tuple(eta), k=tuple(k), intensity=float(intensity), ke=tuple(float(x) for x in ke))
+                eta=tuple(eta), k=tuple(k), intensity=float(intensity), ke=tuple(float(x) for x in ke),
+                texture=int(texture))
 
 
 def quad(p0, p1, p2, p3):

@@ -177,6 +177,8 @@ typedef struct {
     float eta[3], k[3]; /* Trowbridge conductor Fresnel */
     float intensity;    /* light: emitted radiance = kd * intensity */
     float ke[3];        /* plain: radiance emitted towards rays that are not camera rays */
+    uint32_t texture;   /* 0: none; k + 1: the scene's texture k, multiplying kd of a Lambertian
+                         * (LambertianBSDF.cu:22-74) or the camera-ray kd of a plain material */
 } rt_material;
 
 /* PointLight(translate(pos), color, intensity) (light/PointLight.cu:15-66). */
@@ -219,6 +221,17 @@ rt_status rt_scene_add_material(rt_scene *scene, const rt_material *m, uint32_t 
  * (may be NULL) receives the first. */
 rt_status rt_scene_add_mesh(rt_scene *scene, const float *xyz, uint32_t nv, const uint32_t *idx,
                             uint32_t nt, uint32_t material, uint32_t *first_id);
+/* The same with texture coordinates (TriangleMesh::textureCoords,
+ * shapes/Triangle.cu:134-152): uv holds 6 floats per triangle (u, v of its
+ * three corners); NULL gives every triangle the reference's default
+ * (0,0), (1,0), (1,1). */
+rt_status rt_scene_add_mesh_uv(rt_scene *scene, const float *xyz, uint32_t nv, const uint32_t *idx, uint32_t nt,
+                               const float *uv, uint32_t material, uint32_t *first_id);
+/* A texture (textures/Texture.cu): width x height texels of 3 bytes (R, G,
+ * B), row y = the reference's pixelMatrix[y] (rows in BMP file order), read
+ * nearest-texel at (u * width, v * height) and scaled by 1/255.  id (may be
+ * NULL) receives k for rt_material.texture = k + 1. */
+rt_status rt_scene_add_texture(rt_scene *scene, const uint8_t *rgb, uint32_t width, uint32_t height, uint32_t *id);
 /* Sphere(center, radius) (shapes/Sphere.cu:233-245).  is_light = 1 also
  * samples it as a sphere area light (next-event estimation with MIS); its
  * material must be RT_MAT_LIGHT. */

@@ -61,6 +61,8 @@ _SIGS = {
     "orc_scene_create": (_P, [_P, _P, _U32, _P, _P, _P, _P, _U32, _P, _U32, _P, _U32, _P]),
     "orc_scene_free": (None, [_P]),
     "orc_scene_add_planes": (None, [_P, _P, _P, _U32]),
+    "orc_scene_add_texture": (_U32, [_P, _P, _U32, _U32]),
+    "orc_scene_set_tri_uv": (None, [_P, _P]),
     "orc_plane_init": (None, [_P, _P, _P, _P, _P]),
     "orc_plane_intersect": (C.c_int, [_P, C.POINTER(OrcRay), C.POINTER(_F), _P, _P, _P]),
     "orc_scene_camera": (None, [_P, _P, _P]),
@@ -117,7 +119,7 @@ def cache_loss_grad(cfg, mlp, grid, inputs, targets, emulate_bf16=True, want_gra
 
 class OrcMaterial(C.Structure):
     _fields_ = [("type", C.c_uint32), ("kd", _F * 3), ("ior", _F), ("roughness", _F), ("eta", _F * 3),
-                ("k", _F * 3), ("intensity", _F), ("ke", _F * 3)]
+                ("k", _F * 3), ("intensity", _F), ("ke", _F * 3), ("texture", _U32)]
 
 
 class OrcPointLight(C.Structure):
@@ -150,7 +152,7 @@ class OrcScene:
         for i, m in enumerate(desc.materials):
             mats[i] = OrcMaterial(m["type"], _f3(m.get("kd", (0, 0, 0))), m.get("ior", 1.5), m.get("roughness", 0.0),
                                   _f3(m.get("eta", (1, 1, 1))), _f3(m.get("k", (0, 0, 0))), m.get("intensity", 0.0),
-                                  _f3(m.get("ke", (0, 0, 0))))
+                                  _f3(m.get("ke", (0, 0, 0))), int(m.get("texture", 0)))
         self.mats = mats
         lights = (OrcPointLight * max(1, len(desc.lights)))()
         for i, (p, c, inten) in enumerate(desc.lights):
@@ -163,6 +165,15 @@ class OrcScene:
                                         _p(self.sph), _p(self.sph_mat), _p(self.sph_light), self.sph.shape[0],
                                         C.cast(self.mats, _P), len(desc.materials), C.cast(self.lights, _P),
                                         len(desc.lights), C.byref(self.cam))
+        self.tex = []
+        for rgb in getattr(desc, "textures", []):
+            t = np.ascontiguousarray(rgb, np.uint8)
+            self.tex.append(t)
+            lib().orc_scene_add_texture(self.h, _p(t), t.shape[1], t.shape[0])
+        uv = getattr(desc, "tri_uv", None)
+        if uv is not None and self.idx.shape[0]:
+            self.tri_uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 6)
+            lib().orc_scene_set_tri_uv(self.h, _p(self.tri_uv))
         planes = getattr(desc, "planes", [])
         if planes:
             self.planes = np.array([[*c, *ax, *ay, *n] for c, ax, ay, n, m in planes], np.float32)

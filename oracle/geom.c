@@ -377,8 +377,33 @@ int orc_sphere_intersects(const orc_sphere *s, const orc_ray *r, float *hitOffse
  * with errorBound(3)-scaled absolute sums (Transform.cu:335-350).  The
  * normal is the outward normal (Sphere.cu:160), carried to world space by
  * the inverse transpose (Transform.cu:201-212). */
+/* Sphere::localToSurface (Sphere.cu:269-275) of an object-space point:
+ * (theta, phi) in degrees (to_spherical, geometricMath.cu:49-57). */
+static void sphere_surface_point(const float hp[3], float uv[2])
+{
+    float th = atan2f(sqrtf(hp[0] * hp[0] + hp[1] * hp[1]), hp[2]);
+    float ph = atan2f(hp[1], hp[0]);
+    uv[0] = (float)((double)th * 180.0 / 3.14159265358979323846);
+    uv[1] = (float)((double)ph * 180.0 / 3.14159265358979323846);
+}
+
+static int sphere_intersect_impl(const orc_sphere *s, const orc_ray *r, float *hitOffset, float p[3], float n[3],
+                                 float perr[3], float uv[2]);
+
 int orc_sphere_intersect(const orc_sphere *s, const orc_ray *r, float *hitOffset,
                          float p[3], float n[3], float perr[3])
+{
+    return sphere_intersect_impl(s, r, hitOffset, p, n, perr, NULL);
+}
+
+int orc_sphere_intersect_uv(const orc_sphere *s, const orc_ray *r, float *hitOffset, float p[3], float n[3],
+                            float perr[3], float uv[2])
+{
+    return sphere_intersect_impl(s, r, hitOffset, p, n, perr, uv);
+}
+
+static int sphere_intersect_impl(const orc_sphere *s, const orc_ray *r, float *hitOffset, float p[3], float n[3],
+                                 float perr[3], float uv[2])
 {
     float th, o[3], d[3];
     if (!sphere_core(s, r, &th, o, d)) return 0;
@@ -389,6 +414,7 @@ int orc_sphere_intersect(const orc_sphere *s, const orc_ray *r, float *hitOffset
     float sc = s->radius / dist;
     hp[0] *= sc; hp[1] *= sc; hp[2] *= sc;
     if (hp[0] == 0 && hp[1] == 0) hp[0] = 1e-5f * s->radius;
+    if (uv) sphere_surface_point(hp, uv);
     xform_point_err(s->o2w, hp, p, perr);
     /* outward object normal = hp / r; world normal via inverse transpose of
      * o2w, i.e. w2o^T */
@@ -459,8 +485,23 @@ static void xform_point_add_err4(const float *m, const float p[3], float out[3],
  * surface: world point with pError = 1e-4 per axis plus the transform's
  * error, normal (0,0,1) turned against the local direction, taken to world
  * space through the inverse transpose (the rows of worldToObject). */
+static int plane_intersect_impl(const orc_plane *pl, const orc_ray *r, float *hitOffset, float p[3], float n[3],
+                                float perr[3], float uv[2]);
+
 int orc_plane_intersect(const orc_plane *pl, const orc_ray *r, float *hitOffset, float p[3], float n[3],
                         float perr[3])
+{
+    return plane_intersect_impl(pl, r, hitOffset, p, n, perr, NULL);
+}
+
+int orc_plane_intersect_uv(const orc_plane *pl, const orc_ray *r, float *hitOffset, float p[3], float n[3],
+                           float perr[3], float uv[2])
+{
+    return plane_intersect_impl(pl, r, hitOffset, p, n, perr, uv);
+}
+
+static int plane_intersect_impl(const orc_plane *pl, const orc_ray *r, float *hitOffset, float p[3], float n[3],
+                                float perr[3], float uv[2])
 {
     float oerr[3] = {0, 0, 0}, o[3], d[3];
     xform_point_add_err4(pl->w2o, r->o, o, oerr);
@@ -485,6 +526,7 @@ int orc_plane_intersect(const orc_plane *pl, const orc_ray *r, float *hitOffset,
     if (off <= 0 || off > tmax || u < 0 || u > 1 || v < 0 || v > 1) return 0;
     if (!(off < *hitOffset)) return 0;
     *hitOffset = off;
+    if (uv) { uv[0] = u; uv[1] = v; }
     if (p) {
         float on = d[2] > 0.0 ? -1.0f : 1.0f; /* sameHemisphere((0,0,1), d): d.z > 0 */
         float loc[3] = {u, v, 0};

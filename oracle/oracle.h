@@ -160,6 +160,13 @@ void orc_plane_init(orc_plane *p, const float corner[3], const float axis_x[3], 
 int orc_plane_intersect(const orc_plane *pl, const orc_ray *r, float *hitOffset, float p[3], float n[3],
                         float perr[3]);
 
+/* The same two hits, also giving SurfaceInteraction::surfacePoint: the
+ * plane's (u, v), the sphere's (theta, phi) in degrees. */
+int orc_plane_intersect_uv(const orc_plane *pl, const orc_ray *r, float *hitOffset, float p[3], float n[3],
+                           float perr[3], float uv[2]);
+int orc_sphere_intersect_uv(const orc_sphere *s, const orc_ray *r, float *hitOffset, float p[3], float n[3],
+                            float perr[3], float uv[2]);
+
 /* MESA cofactor inverse used by the reference (Matrix4x4.cu:303-440). */
 void orc_mat4_inverse(const float *m, float *out);
 

@@ -34,6 +34,7 @@ typedef struct {
     float eta[3], k[3];
     float intensity;   /* light: emitted radiance = kd * intensity */
     float ke[3];       /* plain: emitted towards rays that are not camera rays */
+    uint32_t texture;  /* 0 none, k + 1: texture k (Lambertian kd, plain camera-ray kd) */
 } orc_material;
 
 typedef struct {
@@ -75,6 +76,11 @@ orc_scene *orc_scene_create(const float *xyz, const uint32_t *idx, uint32_t nt, 
 /* Plane shapes (12 floats each: corner, axis_x, axis_y, normal), tested
  * after the spheres. */
 void orc_scene_add_planes(orc_scene *s, const float *planes, const uint32_t *mats, uint32_t n);
+/* A texture (3 bytes per texel, row y = pixelMatrix[y]); returns its index. */
+uint32_t orc_scene_add_texture(orc_scene *s, const uint8_t *rgb, uint32_t width, uint32_t height);
+/* Texture coordinates of every triangle (6 floats each, the scene's
+ * triangle order); without them the default (0,0), (1,0), (1,1). */
+void orc_scene_set_tri_uv(orc_scene *s, const float *uv);
 void orc_scene_free(orc_scene *s);
 void orc_scene_camera(const orc_scene *s, float r2c[16], float c2w[16]);
 

@@ -497,6 +497,10 @@ struct orc_scene {
     orc_plane *plane;
     uint32_t *plane_mat;
     uint32_t nplane;
+    float *tri_uv;        /* 6 per triangle, or NULL: (0,0), (1,0), (1,1) */
+    uint8_t **tex;        /* ntex textures, 3 bytes per texel */
+    uint32_t *tex_w, *tex_h;
+    uint32_t ntex;
 };
 
 orc_scene *orc_scene_create(const float *xyz, const uint32_t *idx, uint32_t nt, const uint32_t *tri_mat,
@@ -550,10 +554,52 @@ void orc_scene_add_planes(orc_scene *s, const float *planes, const uint32_t *mat
     s->nplane += n;
 }
 
+uint32_t orc_scene_add_texture(orc_scene *s, const uint8_t *rgb, uint32_t width, uint32_t height)
+{
+    s->tex = realloc(s->tex, sizeof(uint8_t *) * (s->ntex + 1));
+    s->tex_w = realloc(s->tex_w, sizeof(uint32_t) * (s->ntex + 1));
+    s->tex_h = realloc(s->tex_h, sizeof(uint32_t) * (s->ntex + 1));
+    s->tex[s->ntex] = malloc((size_t)width * height * 3);
+    memcpy(s->tex[s->ntex], rgb, (size_t)width * height * 3);
+    s->tex_w[s->ntex] = width;
+    s->tex_h[s->ntex] = height;
+    return s->ntex++;
+}
+
+void orc_scene_set_tri_uv(orc_scene *s, const float *uv)
+{
+    free(s->tri_uv);
+    s->tri_uv = malloc(sizeof(float) * 6 * (s->nt ? s->nt : 1));
+    memcpy(s->tri_uv, uv, sizeof(float) * 6 * s->nt);
+}
+
+/* Texture::getColor (textures/Texture.cu:172-183): texel
+ * ((unsigned)(u * width), (unsigned)(v * height)) clamped to the last
+ * column / row, channels / 255; the float-to-unsigned conversion as the
+ * reference's x86-64 host build performs it (truncation to a 64-bit integer,
+ * low 32 bits; NaN to 0). */
+static uint32_t texel_index(float u, uint32_t n)
+{
+    float f = u * (float)n;
+    if (f != f) f = 0.0f;
+    f = fminf(fmaxf(f, -4.6e18f), 4.6e18f);
+    uint32_t x = (uint32_t)(unsigned long long)(long long)f;
+    return x < n - 1 ? x : n - 1;
+}
+
+static v3 texture_color(const orc_scene *s, uint32_t t, float u, float v)
+{
+    uint32_t x = texel_index(u, s->tex_w[t]), y = texel_index(v, s->tex_h[t]);
+    const uint8_t *c = s->tex[t] + 3 * ((size_t)y * s->tex_w[t] + x);
+    return mk((float)c[0] / 255.0f, (float)c[1] / 255.0f, (float)c[2] / 255.0f);
+}
+
 void orc_scene_free(orc_scene *s)
 {
     if (!s) return;
     free(s->plane); free(s->plane_mat);
+    for (uint32_t t = 0; t < s->ntex; ++t) free(s->tex[t]);
+    free(s->tex); free(s->tex_w); free(s->tex_h); free(s->tri_uv);
     orc_refbvh_free(s->bvh);
     free(s->xyz); free(s->idx_seq); free(s->tri_mat); free(s->mat); free(s->tri_transparent);
     free(s->sph); free(s->sph_mat); free(s->sph_light); free(s->pl); free(s);
@@ -570,6 +616,7 @@ typedef struct {
     int32_t id;
     float t;
     v3 p, n, perr;  /* world hit point, shading normal, error */
+    float su, sv;   /* SurfaceInteraction::surfacePoint */
 } hit_rec;
 
 /* Scene::intersectTriangles + intersectShapes (integration/Scene.cu:165-236):
@@ -603,16 +650,18 @@ static hit_rec scene_intersect(const orc_scene *s, v3 o, v3 d)
     for (uint32_t j = 0; j < s->nsph; ++j) {
         float p[3], n[3], e[3];
         float hh = h.t;
-        if (orc_sphere_intersect(&s->sph[j], &r, &hh, p, n, e)) {
-            h.kind = 2; h.id = (int32_t)j; h.t = hh;
+        float uv[2];
+        if (orc_sphere_intersect_uv(&s->sph[j], &r, &hh, p, n, e, uv)) {
+            h.kind = 2; h.id = (int32_t)j; h.t = hh; h.su = uv[0]; h.sv = uv[1];
             h.p = mk(p[0], p[1], p[2]); h.n = mk(n[0], n[1], n[2]); h.perr = mk(e[0], e[1], e[2]);
         }
     }
     for (uint32_t k = 0; k < s->nplane; ++k) {
         float p[3], n[3], e[3];
         float hh = h.t;
-        if (orc_plane_intersect(&s->plane[k], &r, &hh, p, n, e)) {
-            h.kind = 3; h.id = (int32_t)k; h.t = hh;
+        float uv[2];
+        if (orc_plane_intersect_uv(&s->plane[k], &r, &hh, p, n, e, uv)) {
+            h.kind = 3; h.id = (int32_t)k; h.t = hh; h.su = uv[0]; h.sv = uv[1];
             h.p = mk(p[0], p[1], p[2]); h.n = nrm(mk(n[0], n[1], n[2])); h.perr = mk(e[0], e[1], e[2]);
         }
     }
@@ -630,6 +679,11 @@ static hit_rec scene_intersect(const orc_scene *s, v3 o, v3 d)
         v3 n = nrm(cross(sub(p0, p2), sub(p1, p2)));
         if (dot(n, d) > 0) n = neg(n);
         h.n = n;
+        /* surfacePoint = b0 uv0 + b1 uv1 + b2 uv2 (Triangle.cu:356-358, 134-152) */
+        float uvd[6] = {0, 0, 1, 0, 1, 1};
+        const float *q = s->tri_uv ? s->tri_uv + 6 * (size_t)h.id : uvd;
+        h.su = bc[0] * q[0] + bc[1] * q[2] + bc[2] * q[4];
+        h.sv = bc[0] * q[1] + bc[1] * q[3] + bc[2] * q[5];
     }
     return h;
 }
@@ -760,6 +814,16 @@ static int path_impl(const orc_scene *s, uint32_t px, uint32_t py, uint32_t samp
         if (h.kind == 0) break;
         const orc_material *m =
             &s->mat[h.kind == 1 ? s->tri_mat[h.id] : h.kind == 2 ? s->sph_mat[h.id] : s->plane_mat[h.id]];
+        orc_material textured;
+        if (m->texture && (m->type == ORC_MAT_LAMBERTIAN || m->type == ORC_MAT_PLAIN)) {
+            /* LambertianBSDF.cu:22-74, PlainMaterial.cu:62-75: kd times the texel */
+            textured = *m;
+            v3 c = texture_color(s, m->texture - 1, h.su, h.sv);
+            textured.kd[0] = m->kd[0] * c.x;
+            textured.kd[1] = m->kd[1] * c.y;
+            textured.kd[2] = m->kd[2] * c.z;
+            m = &textured;
+        }
         if (cache_rgb && depth >= cache_depth && cacheable(m)) {
             L = add(L, mk(beta.x * cache_rgb[0], beta.y * cache_rgb[1], beta.z * cache_rgb[2]));
             break;

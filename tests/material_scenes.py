@@ -125,3 +125,68 @@ def cornell_planes(width=64, height=64, spp=16, with_mesh=False):
     else:
         d.xyz, d.idx, d.tri_mat = np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32), np.zeros(0, np.uint32)
     return d
+
+
+# a 2 x 2 texture: pixelMatrix[y][x]
+TEX2 = np.array([[[255, 0, 0], [0, 255, 0]], [[0, 0, 255], [255, 255, 255]]], np.uint8)
+
+
+def texel(t, x, y):
+    return t[y, x].astype(np.float32) / np.float32(255.0)
+
+
+def textured_probe(kind, width=32, height=32, spp=4):
+    """Plain (kd 1) textured with TEX2, seen straight from the camera:
+    'plane': a plane over y, z in [-1, 1] at x = 0.5 (u = (y + 1) / 2,
+    v = (z + 1) / 2); 'quad': the same square as two triangles (0,1,2),
+    (0,2,3) with explicit corner uvs giving the same (u, v); 'default': that
+    quad without uvs (the reference's default corners); 'sphere': a sphere
+    of radius 0.8 at the origin."""
+    kw = dict(width=width, height=height, spp=spp)
+    plain = mat(PLAIN, (1, 1, 1), ke=(0, 0, 0), texture=1)
+    if kind == "plane":
+        d = assemble([(wall_x(0.5, 1), 0)], [plain], **kw)
+        d.xyz, d.idx, d.tri_mat = np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32), np.zeros(0, np.uint32)
+        d.planes = [((0.5, -1.0, -1.0), (0, 2, 0), (0, 0, 2), (-1, 0, 0), 0)]
+    elif kind in ("quad", "default"):
+        # corners (y, z): (-1,-1), (1,-1), (1,1), (-1,1) -> uv (0,0), (1,0), (1,1), (0,1)
+        v = np.array([[0.5, -1, -1], [0.5, 1, -1], [0.5, 1, 1], [0.5, -1, 1]], np.float32)
+        f = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+        d = assemble([((v, f), 0)], [plain], **kw)
+        if kind == "quad":
+            d.tri_uv = np.array([[0, 0, 1, 0, 1, 1], [0, 0, 1, 1, 0, 1]], np.float32)
+    elif kind == "sphere":
+        d = assemble([(wall_x(5.0, 0.01), 0)], [plain], **kw)
+        d.xyz, d.idx, d.tri_mat = np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32), np.zeros(0, np.uint32)
+        d.spheres = [((0.0, 0.0, 0.0), 0.8, 0, 0)]
+    else:
+        raise ValueError(kind)
+    d.textures = [TEX2]
+    return d
+
+
+def checker(n=8, a=(230, 230, 230), b=(40, 90, 160)):
+    t = np.zeros((n, n, 3), np.uint8)
+    for y in range(n):
+        for x in range(n):
+            t[y, x] = a if (x + y) % 2 == 0 else b
+    return t
+
+
+def cornell_textured(width=64, height=64, spp=16):
+    """The Cornell box with a checker-textured Lambertian floor (default
+    triangle uvs) and a Plain back wall textured with TEX2 (kd 1, ke
+    (0.5, 0.5, 0.5)) with explicit uvs."""
+    d = scenes.cornell(width=width, height=height, spp=spp)
+    d.materials = list(d.materials) + [mat(LAMB, (0.9, 0.9, 0.9), texture=1),
+                                       mat(PLAIN, (1, 1, 1), ke=(0.5, 0.5, 0.5), texture=2)]
+    tm = d.tri_mat.copy()
+    tm[0:2] = len(d.materials) - 2   # floor
+    tm[4:6] = len(d.materials) - 1   # back wall
+    d.tri_mat = tm
+    uv = np.tile(np.array([0, 0, 1, 0, 1, 1], np.float32), (len(d.idx), 1))
+    uv[4] = [0, 0, 0, 1, 1, 1]
+    uv[5] = [0, 0, 1, 1, 1, 0]
+    d.tri_uv = uv
+    d.textures = [checker(), TEX2]
+    return d

@@ -4,6 +4,8 @@
 //
 //   scene_files --dump <obejota|-> <obj>...
 //       no GPU: print everything the loaders read, one item per line
+//   scene_files --bmp <file>
+//       no GPU: print the texels loadBMP reads
 //   scene_files --render <out> <width> <height> <spp> <obejota> <obj>...
 //               [--sphere cx cy cz r <material> [<obj index>]]...
 //       render on GPU 0 with PathTracerIntegrator, store <out>.hdrbmp
@@ -83,6 +85,7 @@ static int render(int argc, char **argv)
     if (!sc.has_camera) throw Error(RT_ERR_INVALID, std::string(argv[6]) + ": no camera");
     std::vector<ObjData> objs;
     std::vector<std::map<std::string, std::shared_ptr<Material>>> mats;
+    TextureCache textures;
     std::vector<std::shared_ptr<Shape>> shapes;
     int i = 7;
     int32_t mesh = -1;
@@ -92,7 +95,7 @@ static int render(int argc, char **argv)
     }
     mats.resize(objs.size());  // one material cache per OBJ file, as loadObj keeps one map per call
     for (size_t k = 0; k < objs.size(); ++k)
-        for (auto &s : obj_shapes(objs[k], mats[k])) shapes.push_back(s);
+        for (auto &s : obj_shapes(objs[k], mats[k], &textures)) shapes.push_back(s);
     while (i < argc && std::strcmp(argv[i], "--sphere") == 0) {
         if (i + 5 >= argc) return 2;
         const Point3f c{(float)std::atof(argv[i + 1]), (float)std::atof(argv[i + 2]), (float)std::atof(argv[i + 3])};
@@ -102,7 +105,8 @@ static int render(int argc, char **argv)
         i += 6;
         if (i < argc && std::strcmp(argv[i], "--sphere") != 0) k = (size_t)std::atoi(argv[i++]);
         auto it = mats.at(k).find(name);
-        if (it == mats[k].end()) it = mats[k].emplace(name, make_material(objs.at(k).materials.at(name))).first;
+        if (it == mats[k].end())
+            it = mats[k].emplace(name, make_material(objs.at(k).materials.at(name), &textures)).first;
         shapes.push_back(std::make_shared<Sphere>(c, r, it->second));
     }
     Context ctx(0);
@@ -122,6 +126,14 @@ int main(int argc, char **argv)
     try {
         if (argc >= 3 && std::strcmp(argv[1], "--dump") == 0)
             return dump(argv[2], std::vector<std::string>(argv + 3, argv + argc));
+        if (argc == 3 && std::strcmp(argv[1], "--bmp") == 0) {
+            // the texels as loadBMP reads them: "w h" then one "r g b" line per texel
+            const Texture t = loadBMP(argv[2]);
+            std::printf("%u %u\n", t.width, t.height);
+            for (size_t i = 0; i < (size_t)t.width * t.height; ++i)
+                std::printf("%u %u %u\n", t.rgb[3 * i], t.rgb[3 * i + 1], t.rgb[3 * i + 2]);
+            return 0;
+        }
         if (argc >= 2 && std::strcmp(argv[1], "--render") == 0) {
             const int r = render(argc, argv);
             if (r == 2) std::fprintf(stderr, "usage: scene_files --render <out> <w> <h> <spp> <obejota> <obj>...\n");

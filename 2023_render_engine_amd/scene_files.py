@@ -157,10 +157,12 @@ def load_mtl(path: str) -> dict:
             elif key == "hasIndirect":
                 cur.has_indirect = _flag01(rest, path, ln, key)
             elif key == "map_Kd":
+                # Texture(dir + name), else Texture(dir + "textures/" + name)
+                # (FileImporting.cu:213-235): the file tried is texture_file_name's
                 name = rest[0] if rest else ""
                 d = os.path.dirname(path)
                 p = os.path.join(d, name)
-                cur.map_kd = p if os.path.isfile(p) else os.path.join(d, "textures", name)
+                cur.map_kd = p if os.path.isfile(texture_file_name(p, quiet=True)) else os.path.join(d, "textures", name)
     if opened and cur.name:
         out[cur.name] = cur.copy()
     return out
@@ -323,25 +325,72 @@ def load_obejota(path: str) -> ObejotaScene:
     return sc
 
 
-def material_desc(m: MtlMaterial) -> dict:
+def texture_file_name(path: str, quiet: bool = False) -> str:
+    """Texture::Texture's file name (textures/Texture.cu:127-149): a .jpg or
+    .png name becomes .bmp (with the reference's warning on stdout), then
+    every '.' after the first character except the last one becomes '_'."""
+    dot = path.rfind(".")
+    ext = path[dot + 1:] if dot >= 0 else path
+    if ext in ("jpg", "png"):
+        if not quiet:
+            print(f"Warning: {path} is not a bmp file. Changing extension to bmp.")
+        path = path[:dot] + ".bmp"
+    last = path.rfind(".")
+    return "".join("_" if (c == "." and 0 < i != last) else c for i, c in enumerate(path))
+
+
+def load_bmp(name: str) -> np.ndarray:
+    """Texture::loadBmpImage (textures/Texture.cu:40-119): (height, width, 3)
+    uint8 RGB, row y = the file's row y (pixelMatrix[y]); 24- or 32-bit
+    pixels read as consecutive B, G, R(, A) groups from the header's offset,
+    rows not skipped to 4-byte boundaries, as in the reference."""
+    import struct
+    path = texture_file_name(name)
+    if not os.path.isfile(path):
+        raise ValueError(f"Error loading {path}: Failed to open the texture file")
+    data = open(path, "rb").read()
+    if len(data) < 54:
+        raise ValueError(f"Error loading {path}: truncated header")
+    if data[:2] != b"BM":
+        raise ValueError(f"Error loading {path}: Not a valid BMP file.")
+    offset, = struct.unpack_from("<I", data, 10)
+    width, height = struct.unpack_from("<ii", data, 18)
+    bpp, = struct.unpack_from("<H", data, 28)
+    if bpp not in (24, 32):
+        raise ValueError(f"Error loading {path}: Only 24 or 32 bit BMP files are supported, file is {bpp} bit")
+    if width <= 0 or height <= 0:
+        raise ValueError(f"Error loading {path}: only bottom-up BMP files with a positive size")
+    step = bpp // 8
+    n = width * height * step
+    px = np.frombuffer(data, np.uint8, count=min(n, max(0, len(data) - offset)), offset=min(offset, len(data)))
+    if px.size < n:
+        raise ValueError(f"Error loading {path}: truncated pixel data")
+    px = px.reshape(height, width, step)
+    return np.ascontiguousarray(px[:, :, 2::-1])
+
+
+def material_desc(m: MtlMaterial, texture: int = 0) -> dict:
     """The SceneDesc material an MTL block stands for, as the GPU path renders
-    it: illum 0 Plain(Kd, Ke), 1 Lambertian(Kd), 3 Dielectric(1, Ni),
-    5 Trowbridge(Kd, Pr, eta, k), 6 TrowbridgeBasic(Kd, Pr).  Raises
-    UnsupportedMaterial for the others."""
+    it: illum 0 Plain(Kd, Ke, map_Kd), 1 Lambertian(Kd, map_Kd),
+    3 Dielectric(1, Ni), 5 Trowbridge(Kd, Pr, eta, k), 6 TrowbridgeBasic(Kd,
+    Pr).  `texture` is the scene's texture index + 1 for a map_Kd (SceneFiles
+    loads the file).  Raises UnsupportedMaterial for the others."""
     def refuse(why):
         raise UnsupportedMaterial(f"material {m.name} (illum {m.illum}, {ILLUM_CLASSES[m.illum]}): {why}")
-    if m.map_kd and m.illum != 3:
+    if m.map_kd and m.illum not in (0, 1, 3):
         refuse("textures are not rendered on the GPU path")
+    if m.map_kd and m.illum in (0, 1) and not texture:
+        refuse("its texture must be loaded into the scene (SceneFiles does that)")
     if m.illum == 0:
         if m.transparent:
             refuse("a Plain material without shadows is not rendered on the GPU path")
         return dict(type=_PLAIN, kd=tuple(m.kd), ior=1.5, roughness=0.0, eta=(1, 1, 1), k=(0, 0, 0), intensity=0.0,
-                    ke=tuple(m.ke))
+                    ke=tuple(m.ke), texture=texture if m.map_kd else 0)
     if m.illum == 1:
         if m.transparent:
             refuse("a Lambertian material without shadows is not rendered on the GPU path")
         return dict(type=_LAMBERTIAN, kd=tuple(m.kd), ior=1.5, roughness=0.0, eta=(1, 1, 1), k=(0, 0, 0),
-                    intensity=0.0)
+                    intensity=0.0, texture=texture if m.map_kd else 0)
     if m.illum == 3:
         return dict(type=_DIELECTRIC, kd=(1.0, 1.0, 1.0), ior=m.ni, roughness=0.0, eta=(1, 1, 1), k=(0, 0, 0),
                     intensity=0.0)
@@ -372,24 +421,41 @@ class SceneFiles:
             self.objs.append(o)
         self.obejota = load_obejota(obejota_path) if obejota_path else ObejotaScene()
         self._mats, self._ids = [], {}
-        xyz, idx, tri_mat = [], [], []
+        self.textures, self._tex_ids = [], {}
+        xyz, idx, tri_mat, uv = [], [], [], []
         base = 0
+        default = np.array([0, 0, 1, 0, 1, 1], np.float32)
         for o in self.objs:
             xyz.append(o.vertices)
             idx.append(o.v.astype(np.int64) + base)
             tri_mat += [self._material_id(o, name) for name in o.material]
             base += len(o.vertices)
+            # corner texture coordinates where the file has vt lines (faces
+            # without them, and files without any, keep the default corners)
+            t = np.tile(default, (len(o.v), 1))
+            if len(o.texcoords):
+                has = (o.vt >= 0).all(axis=1)
+                t[has] = o.texcoords[o.vt[has]].reshape(-1, 6)
+            uv.append(t)
         self.xyz = np.concatenate(xyz).astype(np.float32) if xyz else np.zeros((0, 3), np.float32)
         self.idx = np.concatenate(idx).astype(np.uint32) if idx else np.zeros((0, 3), np.uint32)
         self.tri_mat = np.array(tri_mat, np.uint32)
+        self.tri_uv = np.concatenate(uv) if uv and any(len(o.texcoords) for o in self.objs) else None
         self.spheres = []
 
     def _material_id(self, o, name):
         # materials are keyed by (file, name): two OBJ files may define one name differently
         key = (id(o), name)
         if key not in self._ids:
+            m = o.materials[name]
+            tex = 0
+            if m.map_kd and m.illum in (0, 1):
+                if m.map_kd not in self._tex_ids:
+                    self.textures.append(load_bmp(m.map_kd))
+                    self._tex_ids[m.map_kd] = len(self.textures)
+                tex = self._tex_ids[m.map_kd]
             self._ids[key] = len(self._mats)
-            self._mats.append(material_desc(o.materials[name]))
+            self._mats.append(material_desc(m, tex))
         return self._ids[key]
 
     def material(self, name, obj=0):
@@ -411,5 +477,8 @@ class SceneFiles:
             raise ValueError("the scene files have no camera")
         cam = dict(self.obejota.camera, width=int(width), height=int(height))
         lights = [(tuple(p), tuple(c), float(i)) for p, c, i in self.obejota.lights]
-        return SceneDesc(self.xyz, self.idx, self.tri_mat, list(self._mats), list(self.spheres), lights, cam,
-                         spp, max_depth, rr_depth, seed)
+        d = SceneDesc(self.xyz, self.idx, self.tri_mat, list(self._mats), list(self.spheres), lights, cam,
+                      spp, max_depth, rr_depth, seed)
+        d.textures = list(self.textures)
+        d.tri_uv = self.tri_uv
+        return d

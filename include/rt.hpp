@@ -91,16 +91,28 @@ private:
 // ---------------------------------------------------------------------
 // materials (materials/*.cu)
 // ---------------------------------------------------------------------
+// A texture's texels (textures/Texture.cu): width x height, 3 bytes (R, G,
+// B) each, row y = the reference's pixelMatrix[y] (BMP file row order).
+// include/rt_io.hpp's loadBMP reads one from a file.
+struct Texture {
+    uint32_t width = 0, height = 0;
+    std::vector<uint8_t> rgb;
+};
+
 class Material {
 public:
     virtual ~Material() = default;
     virtual rt_material desc() const = 0;
+    // the texture multiplying kd (Lambertian, Plain), or null
+    virtual std::shared_ptr<const Texture> texture() const { return nullptr; }
 };
 
-// LambertianMaterial(kd) (materials/LambertianMaterial.cu:18)
+// LambertianMaterial(kd, texture) (materials/LambertianMaterial.cu:18;
+// kd times the texel at the hit, LambertianBSDF.cu:22-74)
 class LambertianMaterial : public Material {
 public:
-    explicit LambertianMaterial(const Spectrum &kd) : kd_(kd) {}
+    explicit LambertianMaterial(const Spectrum &kd, std::shared_ptr<const Texture> tex = nullptr)
+        : kd_(kd), tex_(std::move(tex)) {}
     rt_material desc() const override
     {
         rt_material m{};
@@ -108,9 +120,11 @@ public:
         m.kd[0] = kd_.r; m.kd[1] = kd_.g; m.kd[2] = kd_.b;
         return m;
     }
+    std::shared_ptr<const Texture> texture() const override { return tex_; }
 
 private:
     Spectrum kd_;
+    std::shared_ptr<const Texture> tex_;
 };
 
 // DielectricMaterial(color, ior) (materials/DielectricMaterial.cu:21)
@@ -219,7 +233,9 @@ private:
 // (materials/PlainMaterial.cu:10-80)
 class PlainMaterial : public Material {
 public:
-    explicit PlainMaterial(const Spectrum &kd, const Spectrum &ke = Spectrum(0.0f)) : kd_(kd), ke_(ke) {}
+    explicit PlainMaterial(const Spectrum &kd, const Spectrum &ke = Spectrum(0.0f),
+                           std::shared_ptr<const Texture> tex = nullptr)
+        : kd_(kd), ke_(ke), tex_(std::move(tex)) {}
     rt_material desc() const override
     {
         rt_material m{};
@@ -228,9 +244,11 @@ public:
         m.ke[0] = ke_.r; m.ke[1] = ke_.g; m.ke[2] = ke_.b;
         return m;
     }
+    std::shared_ptr<const Texture> texture() const override { return tex_; }
 
 private:
     Spectrum kd_, ke_;
+    std::shared_ptr<const Texture> tex_;
 };
 
 // ---------------------------------------------------------------------
@@ -247,10 +265,12 @@ public:
 // Triangle ids are assigned in mesh order, then face order (BVH.cu:288-291).
 class TriangleMesh : public Shape {
 public:
-    TriangleMesh(std::vector<float> xyz, std::vector<uint32_t> idx, std::shared_ptr<Material> m)
-        : Shape(std::move(m)), vertices(std::move(xyz)), indices(std::move(idx)) {}
+    TriangleMesh(std::vector<float> xyz, std::vector<uint32_t> idx, std::shared_ptr<Material> m,
+                 std::vector<float> corner_uv = {})
+        : Shape(std::move(m)), vertices(std::move(xyz)), indices(std::move(idx)), uv(std::move(corner_uv)) {}
     std::vector<float> vertices;
     std::vector<uint32_t> indices;
+    std::vector<float> uv;  // 6 per triangle (corner texture coordinates), or empty: the default corners
 };
 
 // Sphere(center, radius) (shapes/Sphere.cu:233-245).  area_light: also
@@ -423,10 +443,22 @@ public:
                const ProjectiveCamera &camera)
     {
         std::vector<std::pair<const Material *, uint32_t>> ids;
+        std::vector<std::pair<const Texture *, uint32_t>> tex_ids;
         auto mat_id = [&](const std::shared_ptr<Material> &m) {
             for (auto &p : ids)
                 if (p.first == m.get()) return p.second;
             rt_material d = m->desc();
+            if (auto t = m->texture()) {
+                uint32_t tid = 0;
+                bool found = false;
+                for (auto &p : tex_ids)
+                    if (p.first == t.get()) { tid = p.second; found = true; }
+                if (!found) {
+                    check(rt_scene_add_texture(h_, t->rgb.data(), t->width, t->height, &tid));
+                    tex_ids.emplace_back(t.get(), tid);
+                }
+                d.texture = tid + 1;
+            }
             uint32_t id = 0;
             check(rt_scene_add_material(h_, &d, &id));
             ids.emplace_back(m.get(), id);
@@ -435,8 +467,12 @@ public:
         for (const auto &s : shapes)
             if (auto *t = dynamic_cast<const TriangleMesh *>(s.get())) {
                 const uint32_t m = mat_id(s->material);
-                check(rt_scene_add_mesh(h_, t->vertices.data(), (uint32_t)(t->vertices.size() / 3), t->indices.data(),
-                                        (uint32_t)(t->indices.size() / 3), m, nullptr));
+                const uint32_t nt = (uint32_t)(t->indices.size() / 3);
+                if (!t->uv.empty() && t->uv.size() != 6 * (size_t)nt)
+                    throw Error(RT_ERR_INVALID, "TriangleMesh: uv needs 6 floats per triangle");
+                check(rt_scene_add_mesh_uv(h_, t->vertices.data(), (uint32_t)(t->vertices.size() / 3),
+                                           t->indices.data(), nt, t->uv.empty() ? nullptr : t->uv.data(), m,
+                                           nullptr));
             }
         for (const auto &s : shapes)
             if (auto *sp = dynamic_cast<const Sphere *>(s.get())) {

@@ -30,9 +30,10 @@
 // Where the reference reads garbage or crashes (a missing number, an index
 // out of range, an illum it does not know) these loaders throw rt_amd::Error
 // with RT_ERR_INVALID and the file and line.  Materials the GPU path does not
-// render (Phong, Beckmann, subsurface, textures, anisotropic roughness,
-// shadow-transparent Lambertian or Plain) load as data and are refused with
-// RT_ERR_UNSUPPORTED only when a shape is made from them.
+// render (Phong, Beckmann, subsurface, textures on other than Lambertian
+// and Plain, anisotropic roughness, shadow-transparent Lambertian or Plain)
+// load as data and are refused with RT_ERR_UNSUPPORTED only when a shape is
+// made from them.
 #pragma once
 
 #include <cmath>
@@ -187,6 +188,23 @@ inline void warn(ObejotaScene &sc, const std::string &w)
 
 }  // namespace io_detail
 
+// Texture::Texture's file name (textures/Texture.cu:127-149): a .jpg or .png
+// name becomes .bmp (with the reference's warning on stdout), then every '.'
+// after the first character except the last one becomes '_'.
+inline std::string texture_file_name(std::string path, bool quiet = false)
+{
+    const size_t dot = path.find_last_of('.');
+    const std::string ext = path.substr(dot == std::string::npos ? 0 : dot + 1);
+    if (ext == "jpg" || ext == "png") {
+        if (!quiet) std::cout << "Warning: " << path << " is not a bmp file. Changing extension to bmp." << std::endl;
+        path = path.substr(0, dot) + ".bmp";
+    }
+    const size_t last = path.find_last_of('.');
+    for (size_t i = 1; i < path.size(); ++i)
+        if (path[i] == '.' && i != last) path[i] = '_';
+    return path;
+}
+
 // loadMTL (FileImporting.cu:27-246).  A file that cannot be opened gives an
 // empty map and a message on stderr, as in the reference.
 inline std::map<std::string, MtlMaterial> loadMTL(const std::string &path)
@@ -264,10 +282,13 @@ inline std::map<std::string, MtlMaterial> loadMTL(const std::string &path)
         } else if (key == "g") {
             cur.g = read_float(ss, path, ln, key);
         } else if (key == "map_Kd") {
+            // the reference tries Texture(dir + name) and, if that file cannot
+            // be opened, Texture(dir + "textures/" + name) (FileImporting.cu:
+            // 213-235); the file tried is the one texture_file_name gives
             std::string name;
             ss >> name;
             const std::string dir = dir_of(path);
-            cur.map_kd = readable(dir + name) ? dir + name : dir + "textures/" + name;
+            cur.map_kd = readable(texture_file_name(dir + name, true)) ? dir + name : dir + "textures/" + name;
         }
     }
     store();
@@ -448,6 +469,51 @@ inline ObejotaScene loadObejota(const std::string &path)
     return sc;
 }
 
+// Texture::loadBmpImage (textures/Texture.cu:40-119): the 54-byte header,
+// 24- or 32-bit pixels read from the header's offset as width x height
+// consecutive B, G, R(, A) groups -- rows are not skipped to 4-byte
+// boundaries, as in the reference -- row y of the file is pixelMatrix[y].
+// Where the reference would read past the end or size a negative height,
+// this throws.
+inline Texture loadBMP(const std::string &name)
+{
+    const std::string path = texture_file_name(name);
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw Error(RT_ERR_INVALID, "Error loading " + path + ": Failed to open the texture file");
+    unsigned char h[54];
+    if (!f.read(reinterpret_cast<char *>(h), 54)) throw Error(RT_ERR_INVALID, "Error loading " + path + ": truncated header");
+    if (h[0] != 'B' || h[1] != 'M') throw Error(RT_ERR_INVALID, "Error loading " + path + ": Not a valid BMP file.");
+    auto u32 = [&](int o) {
+        return (uint32_t)h[o] | ((uint32_t)h[o + 1] << 8) | ((uint32_t)h[o + 2] << 16) | ((uint32_t)h[o + 3] << 24);
+    };
+    const uint32_t offset = u32(10);
+    const int32_t width = (int32_t)u32(18), height = (int32_t)u32(22);
+    const uint32_t bpp = (uint32_t)h[28] | ((uint32_t)h[29] << 8);
+    if (bpp != 24 && bpp != 32)
+        throw Error(RT_ERR_INVALID, "Error loading " + path + ": Only 24 or 32 bit BMP files are supported, file is " +
+                                        std::to_string(bpp) + " bit");
+    if (width <= 0 || height <= 0)
+        throw Error(RT_ERR_UNSUPPORTED, "Error loading " + path + ": only bottom-up BMP files with a positive size");
+    f.seekg(offset, std::ios::beg);
+    Texture t;
+    t.width = (uint32_t)width;
+    t.height = (uint32_t)height;
+    t.rgb.resize((size_t)width * height * 3);
+    const std::streamsize bytes = bpp / 8;
+    for (size_t i = 0; i < (size_t)width * height; ++i) {
+        unsigned char px[4];
+        if (!f.read(reinterpret_cast<char *>(px), bytes))
+            throw Error(RT_ERR_INVALID, "Error loading " + path + ": truncated pixel data");
+        t.rgb[3 * i] = px[2];
+        t.rgb[3 * i + 1] = px[1];
+        t.rgb[3 * i + 2] = px[0];
+    }
+    return t;
+}
+
+// Textures by file name, so a file used by several materials loads once.
+using TextureCache = std::map<std::string, std::shared_ptr<const Texture>>;
+
 // The .obejota camera with a film of w x h pixels (storeObejotaObject,
 // FileImporting.cu:387-422).
 inline ProjectiveCamera make_camera(const ObejotaCamera &c, uint32_t w, uint32_t h)
@@ -456,10 +522,11 @@ inline ProjectiveCamera make_camera(const ObejotaCamera &c, uint32_t w, uint32_t
 }
 
 // The material class an MTL block stands for, as the GPU path renders it:
-// illum 0 Plain(Kd, Ke), 1 Lambertian(Kd), 3 Dielectric(1, Ni),
-// 5 Trowbridge(Kd, Pr, eta, k), 6 TrowbridgeBasic(Kd, Pr).  Throws
+// illum 0 Plain(Kd, Ke, map_Kd), 1 Lambertian(Kd, map_Kd), 3 Dielectric(1,
+// Ni), 5 Trowbridge(Kd, Pr, eta, k), 6 TrowbridgeBasic(Kd, Pr).  A map_Kd
+// texture is loaded with loadBMP (through `textures` when given).  Throws
 // RT_ERR_UNSUPPORTED for the others.
-inline std::shared_ptr<Material> make_material(const MtlMaterial &m)
+inline std::shared_ptr<Material> make_material(const MtlMaterial &m, TextureCache *textures = nullptr)
 {
     static const char *kNames[8] = {"PlainMaterial", "LambertianMaterial", "PhongMaterial", "DielectricMaterial",
                                     "BeckmannMaterial", "TrowbridgeMaterial", "TrowbridgeBasicMaterial",
@@ -468,14 +535,26 @@ inline std::shared_ptr<Material> make_material(const MtlMaterial &m)
         throw Error(RT_ERR_UNSUPPORTED, "material " + m.name + " (illum " + std::to_string(m.illum) + ", " +
                                             kNames[m.illum & 7] + "): " + why);
     };
-    if (!m.map_kd.empty() && m.illum != 3) return refuse("textures are not rendered on the GPU path");
+    if (!m.map_kd.empty() && m.illum != 0 && m.illum != 1 && m.illum != 3)
+        return refuse("textures are not rendered on the GPU path");
+    std::shared_ptr<const Texture> tex;
+    if (!m.map_kd.empty() && m.illum != 3) {
+        if (textures) {
+            auto it = textures->find(m.map_kd);
+            if (it == textures->end())
+                it = textures->emplace(m.map_kd, std::make_shared<const Texture>(loadBMP(m.map_kd))).first;
+            tex = it->second;
+        } else {
+            tex = std::make_shared<const Texture>(loadBMP(m.map_kd));
+        }
+    }
     switch (m.illum) {
     case 0:
         if (m.transparent) return refuse("a Plain material without shadows is not rendered on the GPU path");
-        return std::make_shared<PlainMaterial>(m.kd, m.ke);
+        return std::make_shared<PlainMaterial>(m.kd, m.ke, tex);
     case 1:
         if (m.transparent) return refuse("a Lambertian material without shadows is not rendered on the GPU path");
-        return std::make_shared<LambertianMaterial>(m.kd);
+        return std::make_shared<LambertianMaterial>(m.kd, tex);
     case 3:
         return std::make_shared<DielectricMaterial>(Spectrum(1.0f), m.ni);
     case 5:
@@ -491,21 +570,36 @@ inline std::shared_ptr<Material> make_material(const MtlMaterial &m)
 // The OBJ's triangles as TriangleMeshes for Scene::build: one mesh per run
 // of consecutive faces with one material, so triangle ids keep face order.
 // `materials` caches the converted materials by name (one object per
-// name, so Scene::build registers each once).  Vertex normals and texture
-// coordinates are not used: the GPU path shades with geometric normals.
+// name, so Scene::build registers each once).  A file with `vt` lines gives
+// every face its corners' texture coordinates (a face without them gets the
+// default corners (0,0), (1,0), (1,1), where the reference would read
+// uninitialised indices); a file without them leaves the default corners to
+// every face.  Vertex normals are not used: the GPU path shades with
+// geometric normals.
 inline std::vector<std::shared_ptr<Shape>> obj_shapes(const ObjData &o,
-                                                      std::map<std::string, std::shared_ptr<Material>> &materials)
+                                                      std::map<std::string, std::shared_ptr<Material>> &materials,
+                                                      TextureCache *textures = nullptr)
 {
     std::vector<std::shared_ptr<Shape>> out;
     size_t i = 0;
     while (i < o.triangles.size()) {
         const std::string &mname = o.triangles[i].material;
         auto it = materials.find(mname);
-        if (it == materials.end()) it = materials.emplace(mname, make_material(o.materials.at(mname))).first;
-        std::vector<float> xyz;
+        if (it == materials.end())
+            it = materials.emplace(mname, make_material(o.materials.at(mname), textures)).first;
+        std::vector<float> xyz, uv;
         std::vector<uint32_t> idx;
         std::map<int32_t, uint32_t> remap;
-        for (; i < o.triangles.size() && o.triangles[i].material == mname; ++i)
+        for (; i < o.triangles.size() && o.triangles[i].material == mname; ++i) {
+            if (!o.texcoords.empty()) {
+                const ObjTriangle &t = o.triangles[i];
+                const bool has = t.vt[0] >= 0 && t.vt[1] >= 0 && t.vt[2] >= 0;
+                static const float kDefault[6] = {0, 0, 1, 0, 1, 1};
+                for (int k = 0; k < 3; ++k) {
+                    uv.push_back(has ? o.texcoords[(size_t)t.vt[k]].first : kDefault[2 * k]);
+                    uv.push_back(has ? o.texcoords[(size_t)t.vt[k]].second : kDefault[2 * k + 1]);
+                }
+            }
             for (int k = 0; k < 3; ++k) {
                 const int32_t v = o.triangles[i].v[k];
                 auto r = remap.find(v);
@@ -516,7 +610,8 @@ inline std::vector<std::shared_ptr<Shape>> obj_shapes(const ObjData &o,
                 }
                 idx.push_back(r->second);
             }
-        out.push_back(std::make_shared<TriangleMesh>(std::move(xyz), std::move(idx), it->second));
+        }
+        out.push_back(std::make_shared<TriangleMesh>(std::move(xyz), std::move(idx), it->second, std::move(uv)));
     }
     return out;
 }

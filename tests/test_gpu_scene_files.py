@@ -58,3 +58,23 @@ def test_python_loaded_cornell_matches_builtin(rt, gpu_ctx):
     a_acc, a_nv, _ = render(rt, gpu_ctx, s.scene(64, 64, spp=16))
     b_acc, b_nv, _ = render(rt, gpu_ctx, scenes.cornell(width=64, height=64, spp=16))
     assert np.array_equal(a_acc, b_acc) and np.array_equal(a_nv, b_nv)
+
+
+def test_cpp_loaded_textured_quad_matches_probe(rt, gpu_ctx, tmp_path):
+    """OBJ with vt corners + MTL map_Kd (a .png name read as .bmp) + BMP,
+    loaded and rendered by the C++ driver, equals the textured quad probe
+    rendered from Python (same seed), to the .hdrbmp's 6 digits."""
+    import material_scenes as ms
+    from test_scene_files import textured_quad_files
+    obj, obejota = textured_quad_files(tmp_path)
+    out = str(tmp_path / "tq")
+    print("[scene files] C++ loaders: textured quad 32x32 4 spp", flush=True)
+    r = subprocess.run([EXE, "--render", out, "32", "32", "4", obejota, obj], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    sums, n, w, h = rt.read_hdrbmp(out, 32, 32)
+    d = ms.textured_probe("quad", 32, 32, 4)
+    d.seed = 1
+    _, _, want = render(rt, gpu_ctx, d)
+    assert n == 4
+    assert np.allclose(sums, want, rtol=1e-5, atol=1e-30)

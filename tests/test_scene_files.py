@@ -123,7 +123,7 @@ def test_material_classes_on_the_gpu_path():
     assert (d["type"], d["kd"], d["ior"]) == (1, (1.0, 1.0, 1.0), m["glass2"].ni)
     p = sf.material_desc(m["plain"])  # PlainMaterial(kd, ke); Ke 5 5 5 of its block
     assert (p["type"], p["kd"], p["ke"]) == (6, (1, 1, 1), (5, 5, 5))
-    for name, why in [("aniso", "anisotropic"), ("skin", "SSMaterial"), ("brick", "textures"),
+    for name, why in [("aniso", "anisotropic"), ("skin", "SSMaterial"), ("brick", "texture must be loaded"),
                       ("shadowless", "without shadows")]:
         with pytest.raises(sf.UnsupportedMaterial, match=why):
             sf.material_desc(m[name])
@@ -318,3 +318,90 @@ def test_reference_mtl_files():
     assert {k: illum.count(k) for k in set(illum)} == {0: 1, 1: 1, 2: 18, 3: 5, 4: 7, 6: 4, 7: 3}
     m2 = sf.load_mtl(os.path.join(REF, "models/bath/bathMaterials.mtl"))
     assert len(m2) == 13
+
+
+# ---------------------------------------------------------------------------
+# textures: BMP files, map_Kd and vt
+# ---------------------------------------------------------------------------
+def write_bmp(path, rgb, bpp=24, pad_rows=False):
+    """A bottom-up BMP whose pixel rows are rgb[0], rgb[1], ... (file order),
+    rows packed without padding (as the reference reads them) unless
+    pad_rows."""
+    import struct
+    h, w, _ = rgb.shape
+    step = bpp // 8
+    rows = []
+    for y in range(h):
+        row = bytearray()
+        for x in range(w):
+            r, g, b = (int(c) for c in rgb[y, x])
+            row += bytes([b, g, r] + ([255] if bpp == 32 else []))
+        if pad_rows:
+            row += bytes((-len(row)) % 4)
+        rows.append(bytes(row))
+    data = b"".join(rows)
+    hdr = b"BM" + struct.pack("<IHHIIiiHHIIiiII", 54 + len(data), 0, 0, 54, 40, w, h, 1, bpp, 0, len(data), 2835,
+                              2835, 0, 0)
+    open(path, "wb").write(hdr + data)
+
+
+def test_bmp_reading_matches_between_cpp_and_python(exe, tmp_path):
+    rng = np.random.default_rng(5)
+    for bpp in (24, 32):
+        rgb = rng.integers(0, 256, (5, 3, 3), dtype=np.uint8)
+        p = str(tmp_path / f"t{bpp}.bmp")
+        write_bmp(p, rgb, bpp)
+        got = sf.load_bmp(p)
+        assert np.array_equal(got, rgb)
+        r = subprocess.run([exe, "--bmp", p], capture_output=True, text=True, timeout=30)
+        assert r.returncode == 0, r.stderr
+        lines = r.stdout.split("\n")
+        assert lines[0] == "3 5"
+        assert np.array_equal(np.array([[int(v) for v in ln.split()] for ln in lines[1:16]], np.uint8).reshape(5, 3, 3),
+                              rgb)
+
+
+def test_bmp_rows_are_read_without_padding(tmp_path):
+    """A 3-pixel-wide 24-bit row is 9 bytes; a padded file has 3 more per row,
+    which the reference reads as pixel data (Texture.cu:84-116)."""
+    rgb = np.arange(2 * 3 * 3, dtype=np.uint8).reshape(2, 3, 3) * 7
+    p = str(tmp_path / "pad.bmp")
+    write_bmp(p, rgb, 24, pad_rows=True)
+    got = sf.load_bmp(p)
+    flat = open(p, "rb").read()[54:54 + 18]
+    want = np.frombuffer(flat, np.uint8).reshape(2, 3, 3)[:, :, ::-1]
+    assert np.array_equal(got, want) and not np.array_equal(got, rgb)
+
+
+def test_texture_file_names():
+    assert sf.texture_file_name("textures/bricks.png") == "textures/bricks.bmp"
+    assert sf.texture_file_name("a.b.c.jpg") == "a_b_c.bmp"
+    assert sf.texture_file_name("./tex/x.y.bmp") == "./tex/x_y.bmp"
+    assert sf.texture_file_name("x.bmp") == "x.bmp"
+
+
+def textured_quad_files(tmp_path):
+    """The textured quad probe as files: an OBJ with vt corners, its MTL
+    (Plain, kd 1, map_Kd tex.png -> tex.bmp) and the 2 x 2 texture."""
+    import material_scenes as ms
+    write_bmp(str(tmp_path / "tex.bmp"), ms.TEX2)
+    (tmp_path / "q.mtl").write_text("newmtl tex\nKd 1 1 1\nKe 0 0 0\nmap_Kd tex.png\nillum 0\n")
+    # the file stores (x, z, y)
+    (tmp_path / "q.obj").write_text("mtllib q.mtl\n"
+                                    "v 0.5 -1 -1\nv 0.5 -1 1\nv 0.5 1 1\nv 0.5 1 -1\n"
+                                    "vt 0 0\nvt 1 0\nvt 1 1\nvt 0 1\n"
+                                    "usemtl tex\nf 1/1 2/2 3/3\nf 1/1 3/3 4/4\n")
+    (tmp_path / "q.obejota").write_text("camera\np -1.75 0 0\nlookat 0 0 0\nup 0 0 1\ncts 0 0 -0.8\nframeY 0.6\n")
+    return str(tmp_path / "q.obj"), str(tmp_path / "q.obejota")
+
+
+def test_map_kd_and_vt_load_to_the_textured_probe(tmp_path):
+    import material_scenes as ms
+    obj, obejota = textured_quad_files(tmp_path)
+    d = sf.SceneFiles([obj], obejota).scene(32, 32, spp=4)
+    want = ms.textured_probe("quad", 32, 32, 4)
+    assert np.array_equal(d.xyz, want.xyz) and np.array_equal(d.idx, want.idx)
+    assert np.array_equal(d.tri_uv, want.tri_uv)
+    assert len(d.textures) == 1 and np.array_equal(d.textures[0], ms.TEX2)
+    m = d.material_structs()[0]
+    assert (m.type, m.texture, tuple(m.kd)) == (6, 1, (1.0, 1.0, 1.0))

@@ -155,6 +155,9 @@ _SIGS = {
     "rt_cache_gradients": (C.c_int, [_P, _P, _P, _U32, _P, _P, C.POINTER(C.c_float)]),
     "rt_cache_get_params": (C.c_int, [_P, _P, _P]),
     "rt_cache_set_params": (C.c_int, [_P, _P, _P]),
+    "rt_image_pixel_bytes": (C.c_size_t, [C.c_int, _U32, _U32]),
+    "rt_image_header": (C.c_size_t, [C.c_int, _U32, _U32, _P, C.c_size_t]),
+    "rt_film_tonemap": (C.c_int, [_P, _P, _P, _U32, _U32, C.c_uint64, C.c_int, C.c_int, _P, C.c_size_t, _P]),
     "rt_gen_soup": (C.c_int, [_U32, C.c_uint64, C.c_float, C.c_float, _P]),
     "rt_gen_rays": (C.c_int, [_U32, _U32, _P, C.c_float, _U32, C.c_uint64, _P, _P, _P, _P, _P, _P]),
 }
@@ -456,6 +459,12 @@ class Scene:
                         stream)
         return film
 
+    def write_image(self, film, name, samples, mode="gammaClamp", fmt="bmp"):
+        """Film::writeToBMP / writeToPPM of a render film (accum, nvalid)
+        holding `samples` samples per pixel."""
+        return write_image(self.ctx, name, film[0], self.desc.width, self.desc.height, samples, mode, fmt,
+                           nvalid=film[1])
+
     def _opts(self, spp, spp_offset, max_depth, rr_depth, seed, path_capacity, grid=0, chunk_paths=0):
         d = self.desc
         return RenderOpts(spp if spp is not None else d.spp, spp_offset, max_depth or d.max_depth,
@@ -596,6 +605,74 @@ def read_hdrbmp(path, width=None, height=None):
     if len(vals) < 3 * w * h:
         raise ValueError(f"{path}.hdrbmp: truncated pixel data")
     return np.array(vals[:3 * w * h], np.float64).reshape(-1, 3), n, w, h
+
+
+# ToneMappingType (camera/Film.cu:13-21), in the reference's order
+TONE_MAPPINGS = ("equalize", "reinhard", "reinhardJodie", "ACESfilmic", "uncharted2", "gammaClamp")
+IMAGE_FORMATS = ("bmp", "ppm")
+
+
+def _tone_id(mode):
+    if isinstance(mode, str):
+        if mode not in TONE_MAPPINGS:
+            raise ValueError(f"unknown tone mapping {mode!r}; one of {TONE_MAPPINGS}")
+        return TONE_MAPPINGS.index(mode)
+    return int(mode)
+
+
+def _format_id(fmt):
+    if fmt not in IMAGE_FORMATS:
+        raise ValueError(f"unknown image format {fmt!r}; one of {IMAGE_FORMATS}")
+    return IMAGE_FORMATS.index(fmt)
+
+
+def image_header(fmt, width, height) -> bytes:
+    """The bytes before the pixel array: BitmapImage's 54-byte header
+    (camera/BMP_Format.cu) or "P6\\n<w> <h>\\n255\\n" (Film.cu:339-350)."""
+    buf = (C.c_uint8 * 64)()
+    n = lib().rt_image_header(_format_id(fmt), width, height, buf, 64)
+    if n == 0:
+        raise ValueError(f"bad image size {width}x{height}")
+    return bytes(buf[:n])
+
+
+def tone_map(ctx, sums, width, height, samples, mode="gammaClamp", fmt="bmp", nvalid=None, out=None, stream=None):
+    """Film::toneMapBMP / toneMapPPM (camera/Film.cu:285-337) on the GPU:
+    returns the image's pixel array as a uint8 device tensor.
+
+    sums: float64 device tensor of width*height*3 pixel sums (Film::pixelLight),
+    or, with nvalid (rt_render's valid counts), the valid-sample sums of a
+    render film; samples: the film's sample count; mode: a ToneMappingType
+    name or id."""
+    import torch
+    nbytes = lib().rt_image_pixel_bytes(_format_id(fmt), width, height)
+    if nbytes == 0:
+        raise ValueError(f"bad image size {width}x{height}")
+    if sums.dtype != torch.float64 or sums.device.type != "cuda" or sums.numel() != width * height * 3:
+        raise ValueError("sums must be a float64 device tensor of width*height*3 values")
+    if nvalid is not None and (nvalid.dtype not in (torch.int32, torch.uint32) or nvalid.numel() != width * height):
+        raise ValueError("nvalid must be an int32 tensor of width*height counts")
+    if out is None:
+        out = torch.empty(nbytes, dtype=torch.uint8, device=sums.device)
+    elif out.dtype != torch.uint8 or out.numel() < nbytes:
+        raise ValueError("out must be a uint8 tensor of at least the pixel array's size")
+    ctx.launch(lambda s: check(lib().rt_film_tonemap(ctx.h, _ptr(sums), _ptr(nvalid), width, height, int(samples),
+                                                     _tone_id(mode), _format_id(fmt), _ptr(out), out.numel(), s)),
+               stream)
+    return out
+
+
+def write_image(ctx, name, sums, width, height, samples, mode="gammaClamp", fmt="bmp", nvalid=None):
+    """Film::writeToBMP / writeToPPM (camera/Film.cu:339-362): tone-maps on
+    the GPU and writes name + ".bmp" (or ".ppm").  Returns the path."""
+    import torch
+    pixels = tone_map(ctx, sums, width, height, samples, mode, fmt, nvalid)
+    torch.cuda.current_stream(pixels.device).synchronize()
+    path = f"{name}.{fmt}"
+    with open(path, "wb") as f:
+        f.write(image_header(fmt, width, height))
+        f.write(pixels.cpu().numpy().tobytes())
+    return path
 
 
 def film_image(film, spp_total):

@@ -21,6 +21,7 @@ struct rt_ctx {
     hipStream_t stream = nullptr;
     std::recursive_mutex lock;
     std::atomic<int> refs{1};
+    void *film_scratch = nullptr;  // partial maxima of the film output kernels, allocated on first use
 };
 
 namespace rt {
@@ -32,6 +33,7 @@ void ctx_release(rt_ctx *c)
     if (c->refs.fetch_sub(1) == 1) {
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
+        if (c->film_scratch) (void)hipFree(c->film_scratch);
         (void)hipStreamDestroy(c->stream);
         delete c;
     }
@@ -67,6 +69,14 @@ int ctx_device(rt_ctx *c) { return c->device; }
 void *ctx_stream_ptr(rt_ctx *c) { return (void *)c->stream; }
 std::recursive_mutex &ctx_lock(rt_ctx *c) { return c->lock; }
 DevBvh bvh_device(const rt_bvh *b) { return b->dev; }
+
+// Device block of `bytes` owned by the context (the caller holds ctx_lock and
+// has made the context's device current); null if the allocation fails.
+void *ctx_film_scratch(rt_ctx *c, size_t bytes)
+{
+    if (!c->film_scratch && hipMalloc(&c->film_scratch, bytes) != hipSuccess) c->film_scratch = nullptr;
+    return c->film_scratch;
+}
 
 }  // namespace rt
 

@@ -1,0 +1,387 @@
+// film.hip -- film output on the GPU: the tone-mapping operators of
+// camera/Film.cu:156-283 applied to every pixel sum, and the pixel arrays
+// that Film::toneMapBMP / toneMapPPM (Film.cu:285-337) hand to the BMP writer
+// (camera/BMP_Format.cu) and the PPM stream.
+//
+// Two launches over the film, which stays in HBM:
+//   k_film_max  grid-stride max of every channel and of the luminance of every
+//               pixel sum (getMaxChannel / getMaxLuminance, Film.cu:120-154);
+//               one partial pair per block, no atomics, no initialisation;
+//   k_film_map  each block folds the partial pairs, then each lane maps four
+//               neighbouring pixels of an output row and stores their twelve
+//               bytes as three dwords (rows are 4-byte aligned in both
+//               formats), writing the BMP row padding itself.
+// Arithmetic is double precision in the reference's operation order (the
+// library is built with -ffp-contract=off), so the bytes match a host
+// evaluation wherever the value before truncation is not within an ulp or
+// so of a multiple of 1/255 (the gamma operator's pow is the only non
+// correctly rounded operation).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <mutex>
+#include <string>
+
+#include "rt_internal.h"
+
+struct rt_ctx;
+namespace rt {
+int ctx_device(rt_ctx *c);
+void *ctx_stream_ptr(rt_ctx *c);
+std::recursive_mutex &ctx_lock(rt_ctx *c);
+void *ctx_film_scratch(rt_ctx *c, size_t bytes);
+}  // namespace rt
+
+using rt::set_error;
+
+#define RT_HIP(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            set_error(std::string(#call) + " failed: " + hipGetErrorString(e_));        \
+            return e_ == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;                 \
+        }                                                                               \
+    } while (0)
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr uint32_t kPartials = 1024;      // blocks of k_film_max
+constexpr uint32_t kMapBlocks = 8192;     // grid cap of k_film_map (grid-stride beyond)
+constexpr uint32_t kMaxSide = 1u << 15;   // width and height limit (BMP stores int32; keeps sizes in 32 bits)
+
+struct FilmIn {
+    const double *sums;      // width * height * 3
+    const uint32_t *nvalid;  // null: sums are the pixel sums
+    double samples;          // Film samplesLoadedFromFile + samplesComputedPixel0
+};
+
+// Pixel sum i: the sums, or the renderer's estimate sums * (samples / nvalid)
+// (integration/pathTracing.cu:474-486; 0 where no sample was valid).
+__device__ inline void pixel_sum(const FilmIn &f, size_t i, double p[3])
+{
+    double s = 1.0;
+    if (f.nvalid) {
+        const uint32_t n = f.nvalid[i];
+        s = n ? f.samples / (double)n : 0.0;
+    }
+    p[0] = f.sums[3 * i];
+    p[1] = f.sums[3 * i + 1];
+    p[2] = f.sums[3 * i + 2];
+    if (f.nvalid) { p[0] = p[0] * s; p[1] = p[1] * s; p[2] = p[2] * s; }
+}
+
+// Spectrum::brightness (radiometry/Spectrum.cu:68-75): float weights held in doubles.
+__device__ inline double brightness(const double p[3])
+{
+    const double wr = 0.212671f, wg = 0.715160f, wb = 0.072169f;
+    return wr * p[0] + wg * p[1] + wb * p[2];
+}
+
+__device__ inline double dmax(double a, double b) { return b > a ? b : a; }
+
+// math.cu:44-47 clamp as max(a, min(b, x)): a NaN yields the upper bound.
+__device__ inline double clamp01(double x)
+{
+    const double m = x < 1.0 ? x : 1.0;
+    return m > 0.0 ? m : 0.0;
+}
+
+// Film.cu:202-211; the float constants meet in float before they widen.
+__device__ inline double uncharted2_partial(double x)
+{
+    const float A = 0.15f, B = 0.50f, C = 0.10f, D = 0.20f, E = 0.02f, F = 0.30f;
+    const float cb = C * B, de = D * E, df = D * F, ef = E / F;
+    return ((x * (x * (double)A + (double)cb) + (double)de) / (x * (x * (double)A + (double)B) + (double)df)) -
+           (double)ef;
+}
+
+// Film.cu:224-229; the Float overload of operator* narrows 0.983729 to float.
+__device__ inline double rtt_and_odt_fit(double v)
+{
+    const float k = 0.983729f;
+    const double a = v * (v + 0.0245786) - 0.000090537;
+    const double b = v * (v * (double)k + 0.4329510) + 0.238081;
+    return a / b;
+}
+
+struct Maxima {
+    double max, max_lum;  // already divided by the sample count
+};
+
+// toneMaxPixel (Film.cu:266-283).
+__device__ inline void tonemap(double p[3], const Maxima &m, double n, int mode)
+{
+    p[0] = p[0] / n; p[1] = p[1] / n; p[2] = p[2] / n;
+    switch (mode) {
+    case RT_TONE_EQUALIZE:
+        p[0] = p[0] / m.max; p[1] = p[1] / m.max; p[2] = p[2] / m.max;
+        break;
+    case RT_TONE_REINHARD: {
+        const double l_old = brightness(p);
+        if (l_old == 0) break;
+        const double numerator = l_old * (1.0 + (l_old / (m.max_lum * m.max_lum)));
+        const double l_new = numerator / (1.0 + l_old);
+        const double s = l_new / l_old;
+        p[0] = clamp01(p[0] * s); p[1] = clamp01(p[1] * s); p[2] = clamp01(p[2] * s);
+        break;
+    }
+    case RT_TONE_REINHARD_JODIE: {
+        // lerp(x, a, b) = (1 - x) * a + x * b with a = b = pixel / (1 + luminance)
+        const double lum = brightness(p);
+        for (int c = 0; c < 3; ++c) {
+            const double rp = p[c] / (1 + lum);
+            const double x = p[c] / (p[c] + 1.0);
+            p[c] = clamp01((1.0 - x) * rp + x * rp);
+        }
+        break;
+    }
+    case RT_TONE_ACES_FILMIC: {
+        const double r = p[0], g = p[1], b = p[2];
+        double c0 = 0.59719 * r + 0.35458 * g + 0.04823 * b;
+        double c1 = 0.07600 * r + 0.90834 * g + 0.01566 * b;
+        double c2 = 0.02840 * r + 0.13383 * g + 0.83777 * b;
+        c0 = rtt_and_odt_fit(c0); c1 = rtt_and_odt_fit(c1); c2 = rtt_and_odt_fit(c2);
+        p[0] = clamp01(1.60475 * c0 + -0.53108 * c1 + -0.07367 * c2);
+        p[1] = clamp01(-0.10208 * c0 + 1.10813 * c1 + -0.00605 * c2);
+        p[2] = clamp01(-0.00327 * c0 + -0.07276 * c1 + 1.07602 * c2);
+        break;
+    }
+    case RT_TONE_UNCHARTED2: {
+        const float exposure_bias = 2.0f;
+        const double white_scale = 1.0 / uncharted2_partial(11.2);
+        for (int c = 0; c < 3; ++c) p[c] = uncharted2_partial(p[c] * (double)exposure_bias) * white_scale;
+        break;
+    }
+    default:  // RT_TONE_GAMMA_CLAMP
+        for (int c = 0; c < 3; ++c) p[c] = pow(p[c] / m.max, 1.0 / 2.2);
+        break;
+    }
+}
+
+// (unsigned char)(v * 255) as the reference's x86-64 host code computes it:
+// a 32-bit truncating conversion (out of range or NaN: 0x80000000), low byte.
+__device__ inline uint32_t to_byte(double v)
+{
+    const double x = v * 255;
+    const int32_t i = (x > -2147483649.0 && x < 2147483648.0) ? (int32_t)x : INT32_MIN;
+    return (uint32_t)i & 0xffu;
+}
+
+__device__ inline void fold(double &a, double &b)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a = dmax(a, __shfl_xor(a, off));
+        b = dmax(b, __shfl_xor(b, off));
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_film_max(FilmIn f, size_t npix, double2 *partial)
+{
+    double m = 0.0, ml = 0.0;
+    const size_t stride = (size_t)gridDim.x * kThreads;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < npix; i += stride) {
+        double p[3];
+        pixel_sum(f, i, p);
+        if (p[0] > m) m = p[0];
+        if (p[1] > m) m = p[1];
+        if (p[2] > m) m = p[2];
+        const double l = brightness(p);
+        if (l > ml) ml = l;
+    }
+    fold(m, ml);
+    __shared__ double2 s[kThreads / 64];
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = make_double2(m, ml);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double2 r = s[0];
+        for (int w = 1; w < kThreads / 64; ++w) { r.x = dmax(r.x, s[w].x); r.y = dmax(r.y, s[w].y); }
+        partial[blockIdx.x] = r;
+    }
+}
+
+struct MapOut {
+    uint8_t *pixels;
+    uint32_t width, height;
+    uint32_t row_pixels;  // pixels of one output row (BMP: width; PPM: all, as one row)
+    uint32_t rows;
+    uint32_t row_bytes;   // BMP: 3 * width rounded up to 4; PPM: 3 * width * height
+    uint32_t groups_per_row;
+    int bmp;
+};
+
+__global__ __launch_bounds__(kThreads) void k_film_map(FilmIn f, const double2 *partial, uint32_t npart, MapOut o,
+                                                       int mode)
+{
+    __shared__ double2 s[kThreads / 64];
+    double m = 0.0, ml = 0.0;
+    for (uint32_t k = threadIdx.x; k < npart; k += kThreads) {
+        const double2 v = partial[k];
+        m = dmax(m, v.x);
+        ml = dmax(ml, v.y);
+    }
+    fold(m, ml);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = make_double2(m, ml);
+    __syncthreads();
+    for (int w = 0; w < kThreads / 64; ++w) { m = dmax(m, s[w].x); ml = dmax(ml, s[w].y); }
+    const Maxima mx{m / f.samples, ml / f.samples};
+
+    const size_t ngroups = (size_t)o.rows * o.groups_per_row;
+    const size_t stride = (size_t)gridDim.x * kThreads;
+    for (size_t g = (size_t)blockIdx.x * kThreads + threadIdx.x; g < ngroups; g += stride) {
+        const uint32_t j = (uint32_t)(g / o.groups_per_row);
+        const uint32_t x0 = (uint32_t)(g % o.groups_per_row) * 4u;
+        const uint32_t cnt = min(4u, o.row_pixels - x0);
+        // source pixel of output position (j, x0): BMP rows run bottom-up
+        const size_t src = o.bmp ? (size_t)(o.height - 1u - j) * o.width + x0 : (size_t)x0;
+        uint32_t b[12];
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+            if (q < cnt) {
+                double p[3];
+                pixel_sum(f, src + q, p);
+                tonemap(p, mx, f.samples, mode);
+                const uint32_t r = to_byte(p[0]), gg = to_byte(p[1]), bb = to_byte(p[2]);
+                b[3 * q] = o.bmp ? bb : r;
+                b[3 * q + 1] = gg;
+                b[3 * q + 2] = o.bmp ? r : bb;
+            } else {
+                b[3 * q] = b[3 * q + 1] = b[3 * q + 2] = 0u;
+            }
+        }
+        uint8_t *dst = o.pixels + (size_t)j * o.row_bytes + 3u * x0;
+        if (cnt == 4) {
+            uint32_t *d = reinterpret_cast<uint32_t *>(dst);
+            d[0] = b[0] | b[1] << 8 | b[2] << 16 | b[3] << 24;
+            d[1] = b[4] | b[5] << 8 | b[6] << 16 | b[7] << 24;
+            d[2] = b[8] | b[9] << 8 | b[10] << 16 | b[11] << 24;
+        } else {
+            // last group of a row: its pixels, then the row padding as zeros
+            const uint32_t end = o.row_bytes - 3u * x0;
+            for (uint32_t k = 0; k < end; ++k) dst[k] = (uint8_t)(k < 3u * cnt ? b[k] : 0u);
+        }
+    }
+}
+
+uint32_t bmp_row_bytes(uint32_t w) { return (3u * w + 3u) & ~3u; }
+
+// The device allocation holding ptr must reach ptr + bytes.
+bool in_allocation(const void *ptr, size_t bytes)
+{
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)ptr) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    const uintptr_t p = (uintptr_t)ptr, b0 = (uintptr_t)base;
+    return p >= b0 && p - b0 <= size && bytes <= size - (p - b0);
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t rt_image_pixel_bytes(rt_image_format format, uint32_t width, uint32_t height)
+{
+    if (width == 0 || height == 0 || width > kMaxSide || height > kMaxSide) return 0;
+    if (format == RT_IMAGE_BMP) return (size_t)bmp_row_bytes(width) * height;
+    if (format == RT_IMAGE_PPM) return (size_t)3 * width * height;
+    return 0;
+}
+
+size_t rt_image_header(rt_image_format format, uint32_t width, uint32_t height, uint8_t *out, size_t cap)
+{
+    if (width == 0 || height == 0 || width > kMaxSide || height > kMaxSide) return 0;
+    uint8_t h[64] = {0};
+    size_t len = 0;
+    auto u16 = [&](size_t at, uint32_t v) { h[at] = (uint8_t)v; h[at + 1] = (uint8_t)(v >> 8); };
+    auto u32 = [&](size_t at, uint32_t v) { for (int k = 0; k < 4; ++k) h[at + k] = (uint8_t)(v >> (8 * k)); };
+    if (format == RT_IMAGE_BMP) {
+        // BitmapImage (BMP_Format.cu): 14-byte file header + 40-byte info
+        // header; Image_Size keeps the reference's (width + 4 - width % 4) * height
+        const uint32_t image_size = (width + 4u - (width % 4u)) * height;
+        h[0] = 'B'; h[1] = 'M';
+        u32(2, 14u + 40u + image_size);
+        u32(10, 54u);
+        u32(14, 40u);
+        u32(18, width);
+        u32(22, height);
+        u16(26, 1u);
+        u16(28, 24u);
+        u32(30, 0u);
+        u32(34, image_size);
+        u32(38, 3780u);
+        u32(42, 3780u);
+        len = 54;
+    } else if (format == RT_IMAGE_PPM) {
+        const std::string s = "P6\n" + std::to_string(width) + " " + std::to_string(height) + "\n255\n";
+        len = s.size();
+        for (size_t k = 0; k < len; ++k) h[k] = (uint8_t)s[k];
+    } else {
+        return 0;
+    }
+    if (out)
+        for (size_t k = 0; k < len && k < cap; ++k) out[k] = h[k];
+    return len;
+}
+
+rt_status rt_film_tonemap(rt_ctx *ctx, const double *sums, const uint32_t *nvalid, uint32_t width, uint32_t height,
+                          uint64_t samples, rt_tone_mapping mode, rt_image_format format, uint8_t *pixels,
+                          size_t pixel_bytes, void *stream_)
+{
+    if (!ctx || !sums || !pixels) { set_error("rt_film_tonemap: null argument"); return RT_ERR_INVALID; }
+    if ((int)mode < RT_TONE_EQUALIZE || (int)mode > RT_TONE_GAMMA_CLAMP) {
+        set_error("rt_film_tonemap: unknown tone mapping");
+        return RT_ERR_INVALID;
+    }
+    const size_t need = rt_image_pixel_bytes(format, width, height);
+    if (need == 0) {
+        set_error("rt_film_tonemap: image format unknown or size outside 1..32768 per side");
+        return RT_ERR_INVALID;
+    }
+    if (pixel_bytes < need) { set_error("rt_film_tonemap: pixel buffer too small"); return RT_ERR_INVALID; }
+    if ((uintptr_t)pixels % 4u || (uintptr_t)sums % 8u || (uintptr_t)nvalid % 4u) {
+        set_error("rt_film_tonemap: pixels must be 4-byte aligned, sums 8-byte, nvalid 4-byte");
+        return RT_ERR_INVALID;
+    }
+    if (samples == 0 || samples > (1ull << 53)) {
+        set_error("rt_film_tonemap: samples must be 1..2^53 (the film divides by it)");
+        return RT_ERR_INVALID;
+    }
+    const size_t npix = (size_t)width * height;
+    std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(ctx));
+    RT_HIP(hipSetDevice(rt::ctx_device(ctx)));
+    if (!in_allocation(sums, npix * 3 * sizeof(double)) || (nvalid && !in_allocation(nvalid, npix * sizeof(uint32_t))) ||
+        !in_allocation(pixels, need)) {
+        set_error("rt_film_tonemap: a buffer is not device memory of the film's size on this context's device");
+        return RT_ERR_INVALID;
+    }
+    double2 *partial = (double2 *)rt::ctx_film_scratch(ctx, kPartials * sizeof(double2));
+    if (!partial) { set_error("rt_film_tonemap: device allocation failed"); return RT_ERR_OOM; }
+    hipStream_t stream = stream_ ? (hipStream_t)stream_ : (hipStream_t)rt::ctx_stream_ptr(ctx);
+
+    const FilmIn f{sums, nvalid, (double)samples};
+    const uint32_t nmax = (uint32_t)std::min<size_t>(kPartials, (npix + kThreads - 1) / kThreads);
+    k_film_max<<<nmax, kThreads, 0, stream>>>(f, npix, partial);
+    RT_HIP(hipGetLastError());
+
+    MapOut o{};
+    o.pixels = pixels;
+    o.width = width;
+    o.height = height;
+    o.bmp = format == RT_IMAGE_BMP;
+    o.row_pixels = o.bmp ? width : (uint32_t)npix;
+    o.rows = o.bmp ? height : 1u;
+    o.row_bytes = o.bmp ? bmp_row_bytes(width) : (uint32_t)(3 * npix);
+    o.groups_per_row = (o.row_pixels + 3u) / 4u;
+    const size_t ngroups = (size_t)o.rows * o.groups_per_row;
+    const uint32_t nmap = (uint32_t)std::min<size_t>(kMapBlocks, (ngroups + kThreads - 1) / kThreads);
+    k_film_map<<<nmap, kThreads, 0, stream>>>(f, partial, nmax, o, (int)mode);
+    RT_HIP(hipGetLastError());
+    return RT_OK;
+}
+
+}  // extern "C"

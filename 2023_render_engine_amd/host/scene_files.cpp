@@ -9,6 +9,9 @@
 //   scene_files --render <out> <width> <height> <spp> <obejota> <obj>...
 //               [--sphere cx cy cz r <material> [<obj index>]]...
 //       render on GPU 0 with PathTracerIntegrator, store <out>.hdrbmp
+//   scene_files --tonemap <film> <width> <height> <mode> <out> [bmp|ppm]
+//       load <film>.hdrbmp and write <out>.bmp (or .ppm) tone-mapped on
+//       GPU 0, as ToneMapper.cu does; <mode> is a ToneMappingType name
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -121,6 +124,27 @@ static int render(int argc, char **argv)
     return 0;
 }
 
+static int tonemap(int argc, char **argv)
+{
+    if (argc < 7 || argc > 8) return 2;
+    static const char *names[] = {"equalize", "reinhard", "reinhardJodie", "ACESfilmic", "uncharted2", "gammaClamp"};
+    int mode = -1;
+    for (int k = 0; k < 6; ++k)
+        if (std::strcmp(argv[5], names[k]) == 0) mode = k;
+    if (mode < 0) throw Error(RT_ERR_INVALID, std::string("unknown tone mapping ") + argv[5]);
+    const std::string fmt = argc == 8 ? argv[7] : "bmp";
+    if (fmt != "bmp" && fmt != "ppm") throw Error(RT_ERR_INVALID, "format must be bmp or ppm");
+    Film film((uint32_t)std::atoi(argv[3]), (uint32_t)std::atoi(argv[4]));
+    if (!film.loadContributionsFromFile(argv[2])) throw Error(RT_ERR_INVALID, std::string(argv[2]) + ".hdrbmp not found");
+    if (fmt == "bmp")
+        film.writeToBMP(argv[6], (ToneMappingType)mode);
+    else
+        film.writeToPPM(argv[6], (ToneMappingType)mode);
+    std::printf("wrote %s.%s (%s, %llu samples)\n", argv[6], fmt.c_str(), names[mode],
+                (unsigned long long)film.samples());
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     try {
@@ -139,7 +163,12 @@ int main(int argc, char **argv)
             if (r == 2) std::fprintf(stderr, "usage: scene_files --render <out> <w> <h> <spp> <obejota> <obj>...\n");
             return r;
         }
-        std::fprintf(stderr, "usage: scene_files --dump <obejota|-> <obj>... | --render ...\n");
+        if (argc >= 2 && std::strcmp(argv[1], "--tonemap") == 0) {
+            const int r = tonemap(argc, argv);
+            if (r == 2) std::fprintf(stderr, "usage: scene_files --tonemap <film> <w> <h> <mode> <out> [bmp|ppm]\n");
+            return r;
+        }
+        std::fprintf(stderr, "usage: scene_files --dump <obejota|-> <obj>... | --render ... | --tonemap ...\n");
         return 2;
     } catch (const Error &e) {
         std::fprintf(stderr, "error (status %d): %s\n", (int)e.status, e.what());

@@ -503,6 +503,66 @@ def run_c3(args):
     return out
 
 
+def run_film(args):
+    """Film output: a 1920x1080 FP64 film (synthetic HDR pixel sums, 64
+    samples) tone-mapped and packed into a BMP pixel array per step
+    (rt_film_tonemap: max pass + map pass).  value = Mpixels/s; the roofline
+    prices one call at 51 algorithmic bytes per pixel (24 B of sums read by
+    each pass, 3 B written)."""
+    import numpy as np
+    import torch
+
+    rt = importlib.import_module("2023_render_engine_amd")
+    w, h, spp = 1920, 1080, 64
+    mode = args.tone
+    rng = np.random.default_rng(7)
+    sums_np = rng.random((w * h, 3)) * 8.0 * spp * rng.random((w * h, 1)) ** 2
+    ctx = rt.Context(0)
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.Stream(device=dev)
+    with torch.cuda.stream(stream):
+        sums = torch.from_numpy(sums_np.reshape(-1)).to(dev)
+        out = torch.empty(rt.lib().rt_image_pixel_bytes(0, w, h), dtype=torch.uint8, device=dev)
+        for _ in range(args.warmup):
+            rt.tone_map(ctx, sums, w, h, spp, mode, "bmp", out=out)
+        stream.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(args.steps):
+            rt.tone_map(ctx, sums, w, h, spp, mode, "bmp", out=out)
+        e1.record(stream)
+        stream.synchronize()
+        wall = time.perf_counter() - t0
+    call_ms = e0.elapsed_time(e1) / args.steps
+    npix = w * h
+    bytes_per_call = 51 * npix
+    gbs = bytes_per_call / (call_ms * 1e-3) / 1e9
+    res = {
+        "metric": "film tone mapping + BMP packing", "value": round(npix * args.steps / wall / 1e6, 1),
+        "unit": "Mpixels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"1920x1080 FP64 film, {spp} samples, {mode}, 24-bit BMP pixel array"},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(gbs / 8000.0, 4), "traffic": None,
+                     "note": "one call = k_film_max + k_film_map, 51 B/pixel, timed with HIP events on its stream"},
+    }
+    if not args.no_cpu_baseline:
+        import oracle
+        spent, done = 0.0, 0
+        while spent < min(args.cpu_budget, 10.0):
+            t1 = time.time()
+            oracle.film_image(sums_np, w, h, spp, mode, "bmp")
+            spent += time.time() - t1
+            done += npix
+        res["cpu_baseline"] = {"value": round(done / spent / 1e6, 2), "unit": "Mpixels/s", "cores": 1,
+                               "kind": "port", "sample": f"{done // npix} whole 1920x1080 images with the oracle's "
+                                                         f"restatement, {spent:.1f} s"}
+    ctx.close()
+    return res
+
+
 def run_copy(args):
     """HBM calibration: a plain device-to-device copy of 1 GiB (torch), so the
     FETCH_SIZE / WRITE_SIZE counters of a kernel of known traffic can scale
@@ -532,7 +592,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c2", "c3", "c4", "copy"])
+    ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c2", "c3", "c4", "copy", "film"])
+    ap.add_argument("--tone", default="gammaClamp", help="film: tone mapping (a ToneMappingType name)")
     ap.add_argument("--c3-ref-spp", type=int, default=128)
     ap.add_argument("--ref-spp", type=int, default=1024,
                     help="c0/c2: spp of the FP64 reference the frame's RMSE is taken against (0: skip)")
@@ -567,6 +628,8 @@ def main():
         out = run_c4(args)
     elif args.workload == "copy":
         out = run_copy(args)
+    elif args.workload == "film":
+        out = run_film(args)
     else:
         out = run_render(args, args.workload)
     print(json.dumps(out), flush=True)

@@ -273,6 +273,53 @@ typedef struct {
 rt_status rt_render_get_stats(const rt_scene *scene, rt_render_stats *stats);
 
 /* ------------------------------------------------------------------ */
+/* Film output: tone mapping and image files                            */
+/* ------------------------------------------------------------------ */
+/* ToneMappingType (camera/Film.cu:13-21), in the reference's order. */
+typedef enum {
+    RT_TONE_EQUALIZE = 0,       /* Film.cu:156-162  pixel / max channel */
+    RT_TONE_REINHARD = 1,       /* Film.cu:164-180  extended Reinhard on luminance, white = max luminance */
+    RT_TONE_REINHARD_JODIE = 2, /* Film.cu:182-193 */
+    RT_TONE_ACES_FILMIC = 3,    /* Film.cu:224-247  fitted ACES RRT+ODT */
+    RT_TONE_UNCHARTED2 = 4,     /* Film.cu:195-222  filmic curve, exposure 2, white 11.2 (no clamp) */
+    RT_TONE_GAMMA_CLAMP = 5     /* Film.cu:250-264  (pixel / max channel)^(1/2.2) */
+} rt_tone_mapping;
+
+/* Film::writeToBMP (camera/BMP_Format.cu, 24-bit, rows bottom-up and padded
+ * to 4 bytes) and Film::writeToPPM (binary P6). */
+typedef enum { RT_IMAGE_BMP = 0, RT_IMAGE_PPM = 1 } rt_image_format;
+
+/* Bytes of the image's pixel array (0 for an unknown format or a side
+ * outside 1..32768). */
+size_t rt_image_pixel_bytes(rt_image_format format, uint32_t width, uint32_t height);
+
+/* Writes the file header that precedes the pixel array into out (at most cap
+ * bytes; out may be null) and returns its length: BMP 54 bytes as
+ * BitmapImage::write_image emits them (BMP_Format.cu), including its
+ * Image_Size field of (width + 4 - width % 4) * height; PPM
+ * "P6\n<w> <h>\n255\n" (Film.cu:339-350).  0 for a bad format or size. */
+size_t rt_image_header(rt_image_format format, uint32_t width, uint32_t height, uint8_t *out, size_t cap);
+
+/* Tone-maps a film and writes its image pixel array into `pixels` (device
+ * memory, rt_image_pixel_bytes bytes, 4-byte aligned), replacing
+ * Film::toneMapBMP / toneMapPPM (camera/Film.cu:285-337) with
+ * toneMaxPixel (Film.cu:266-283) and the maxima of Film.cu:120-154.
+ *   sums     device, width*height*3 doubles: the pixel sums
+ *            (Film::pixelLight), or, when nvalid is not null, rt_render's
+ *            valid-sample sums, whose pixel sum is sums * samples / nvalid
+ *   samples  the film's sample count (samplesLoadedFromFile +
+ *            samplesComputedPixel0), 1..2^53
+ * Bytes are (unsigned char)(value * 255) as the reference computes them on
+ * x86-64: values outside [0, 1] are not clamped by the equalize, uncharted2
+ * and gamma operators, and wrap modulo 256 (NaN gives 0).  Enqueued on
+ * `stream` (null: the context's stream); the call returns without waiting.
+ * Calls on one context share its partial-maxima buffer: issue them on one
+ * stream. */
+rt_status rt_film_tonemap(rt_ctx *ctx, const double *sums, const uint32_t *nvalid, uint32_t width, uint32_t height,
+                          uint64_t samples, rt_tone_mapping mode, rt_image_format format, uint8_t *pixels,
+                          size_t pixel_bytes, void *stream);
+
+/* ------------------------------------------------------------------ */
 /* Batches: many scenes in one launch sequence                          */
 /* ------------------------------------------------------------------ */
 /* No counterpart in the reference, which renders one scene per process

@@ -313,6 +313,16 @@ public:
 // ---------------------------------------------------------------------
 // Per-pixel radiance sums in FP64 and one sample count, as the reference's
 // Film keeps them (the count is tracked for the whole film, Film.cu:112-118).
+// ToneMappingType (camera/Film.cu:13-21)
+enum ToneMappingType {
+    equalize = RT_TONE_EQUALIZE,
+    reinhard = RT_TONE_REINHARD,
+    reinhardJodie = RT_TONE_REINHARD_JODIE,
+    ACESfilmic = RT_TONE_ACES_FILMIC,
+    uncharted2 = RT_TONE_UNCHARTED2,
+    gammaClamp = RT_TONE_GAMMA_CLAMP
+};
+
 class Film {
 public:
     Film(uint32_t w, uint32_t h) : w_(w), h_(h), sums_((size_t)w * h * 3, 0.0) {}
@@ -377,7 +387,46 @@ public:
         return true;
     }
 
+    // Film::writeToBMP / writeToPPM (camera/Film.cu:339-362): tone-mapped on
+    // GPU `device` and written to filename + ".bmp" / ".ppm".
+    void writeToBMP(const std::string &filename, ToneMappingType toneMode, int device = 0) const
+    {
+        writeImage(filename + ".bmp", RT_IMAGE_BMP, toneMode, device);
+    }
+    void writeToPPM(const std::string &filename, ToneMappingType toneMode, int device = 0) const
+    {
+        writeImage(filename + ".ppm", RT_IMAGE_PPM, toneMode, device);
+    }
+
+    // The image file's bytes (header, then the pixel array rt_film_tonemap
+    // writes from these sums on the context's GPU).
+    std::vector<uint8_t> toneMap(Context &ctx, rt_image_format format, ToneMappingType toneMode) const
+    {
+        const size_t hb = rt_image_header(format, w_, h_, nullptr, 0);
+        const size_t pb = rt_image_pixel_bytes(format, w_, h_);
+        if (!hb || !pb) throw Error(RT_ERR_INVALID, "film size outside 1..32768 per side");
+        std::vector<uint8_t> out(hb + pb);
+        rt_image_header(format, w_, h_, out.data(), hb);
+        DeviceBuffer sums(ctx, sums_.size() * sizeof(double)), px(ctx, pb);
+        sums.upload(sums_.data(), sums_.size() * sizeof(double));
+        check(rt_film_tonemap(ctx.get(), static_cast<const double *>(sums.get()), nullptr, w_, h_, samples_,
+                              static_cast<rt_tone_mapping>(toneMode), format, static_cast<uint8_t *>(px.get()), pb,
+                              nullptr));
+        px.download(out.data() + hb, pb);
+        return out;
+    }
+
 private:
+    void writeImage(const std::string &path, rt_image_format format, ToneMappingType toneMode, int device) const
+    {
+        Context ctx(device);
+        const std::vector<uint8_t> bytes = toneMap(ctx, format, toneMode);
+        std::ofstream f(path, std::ios::binary);
+        if (!f) throw Error(RT_ERR_INVALID, "cannot write " + path);
+        f.write(reinterpret_cast<const char *>(bytes.data()), (std::streamsize)bytes.size());
+        if (!f) throw Error(RT_ERR_INVALID, "short write to " + path);
+    }
+
     uint32_t w_, h_;
     std::vector<double> sums_;
     uint64_t samples_ = 0;

@@ -74,7 +74,63 @@ _SIGS = {
                                       C.c_int, _U32, _P]),
     "orc_compute_rmse": (_F, [_P, C.c_double, _P, C.c_double, _U32]),
     "orc_cache_loss_grad": (C.c_double, [_P, _P, _P, _P, _P, _U32, C.c_int, _P, _P, _P, _P]),
+    "orc_film_pixel": (None, [_P, _P, C.c_uint64, C.c_size_t, _P]),
+    "orc_film_maxima": (None, [_P, _P, C.c_size_t, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "orc_brightness": (C.c_double, [_P]),
+    "orc_tonemap_pixel": (None, [_P, C.c_double, C.c_double, C.c_uint64, C.c_int, _P]),
+    "orc_to_byte": (C.c_uint8, [C.c_double]),
+    "orc_image_pixel_bytes": (C.c_size_t, [C.c_int, _U32, _U32]),
+    "orc_image_header": (C.c_size_t, [C.c_int, _U32, _U32, _P, C.c_size_t]),
+    "orc_film_image": (None, [_P, _P, _U32, _U32, C.c_uint64, C.c_int, C.c_int, _P, _P]),
 }
+
+# ToneMappingType (camera/Film.cu:13-21) and the two image formats
+TONE_MAPPINGS = ("equalize", "reinhard", "reinhardJodie", "ACESfilmic", "uncharted2", "gammaClamp")
+IMAGE_FORMATS = ("bmp", "ppm")
+
+
+def _tone(mode):
+    return TONE_MAPPINGS.index(mode) if isinstance(mode, str) else int(mode)
+
+
+def film_maxima(sums, samples, nvalid=None):
+    """(max channel, max luminance) of Film.cu:120-154, each / samples."""
+    s = np.ascontiguousarray(sums, np.float64).reshape(-1)
+    nv = None if nvalid is None else np.ascontiguousarray(nvalid, np.uint32)
+    m, ml = C.c_double(), C.c_double()
+    lib().orc_film_maxima(_p(s), _p(nv), s.size // 3, int(samples), C.byref(m), C.byref(ml))
+    return m.value, ml.value
+
+
+def tonemap_pixel(rgb, max_ch, max_lum, samples, mode):
+    """toneMaxPixel (Film.cu:266-283) on one pixel sum."""
+    i = np.ascontiguousarray(rgb, np.float64)
+    o = np.zeros(3, np.float64)
+    lib().orc_tonemap_pixel(_p(i), max_ch, max_lum, int(samples), _tone(mode), _p(o))
+    return o
+
+
+def to_byte(v):
+    return int(lib().orc_to_byte(float(v)))
+
+
+def image_header(fmt, width, height):
+    buf = np.zeros(64, np.uint8)
+    n = lib().orc_image_header(IMAGE_FORMATS.index(fmt), width, height, _p(buf), 64)
+    return buf[:n].tobytes()
+
+
+def film_image(sums, width, height, samples, mode, fmt="bmp", nvalid=None):
+    """Pixel array of Film::writeToBMP / writeToPPM (Film.cu:285-362) and the
+    tone-mapped values before the byte conversion (H*W*3, film order)."""
+    s = np.ascontiguousarray(sums, np.float64).reshape(-1)
+    assert s.size == width * height * 3
+    nv = None if nvalid is None else np.ascontiguousarray(nvalid, np.uint32).reshape(-1)
+    f = IMAGE_FORMATS.index(fmt)
+    px = np.zeros(lib().orc_image_pixel_bytes(f, width, height), np.uint8)
+    mapped = np.zeros(width * height * 3, np.float64)
+    lib().orc_film_image(_p(s), _p(nv), width, height, int(samples), _tone(mode), f, _p(px), _p(mapped))
+    return px, mapped
 
 
 class OrcCacheCfg(C.Structure):

@@ -3,18 +3,18 @@
 // that Film::toneMapBMP / toneMapPPM (Film.cu:285-337) hand to the BMP writer
 // (camera/BMP_Format.cu) and the PPM stream.
 //
-// Two launches over the film, which stays in HBM:
-//   k_film_max  grid-stride max of every channel and of the luminance of every
-//               pixel sum (getMaxChannel / getMaxLuminance, Film.cu:120-154);
-//               one partial pair per block, no atomics, no initialisation;
-//   k_film_map  each block folds the partial pairs, then each lane maps four
-//               neighbouring pixels of an output row and stores their twelve
-//               bytes as three dwords (rows are 4-byte aligned in both
-//               formats), writing the BMP row padding itself.
+// Two passes over the film, which stays in HBM:
+//   k_film_max   grid-stride max of every channel and of the luminance of
+//                every pixel sum (getMaxChannel / getMaxLuminance,
+//                Film.cu:120-154); one partial pair per block, no atomics,
+//                no initialisation; k_film_fold (one block) folds the pairs;
+//   k_film_map   one lane per pixel of an output row (2-D grid: x across the
+//                row, y = row), three byte stores, the row's BMP padding
+//                written by its last lane.
 // Arithmetic is double precision in the reference's operation order (the
 // library is built with -ffp-contract=off), so the bytes match a host
 // evaluation wherever the value before truncation is not within an ulp or
-// so of a multiple of 1/255 (the gamma operator's pow is the only non
+// so of a multiple of 1/255 (the gamma operator's power is the only non
 // correctly rounded operation).
 #include <hip/hip_runtime.h>
 
@@ -47,8 +47,7 @@ using rt::set_error;
 namespace {
 
 constexpr int kThreads = 256;
-constexpr uint32_t kPartials = 1024;      // blocks of k_film_max
-constexpr uint32_t kMapBlocks = 8192;     // grid cap of k_film_map (grid-stride beyond)
+constexpr uint32_t kPartials = 1024;      // blocks of k_film_max (the scratch holds kPartials + 1 pairs)
 constexpr uint32_t kMaxSide = 1u << 15;   // width and height limit (BMP stores int32; keeps sizes in 32 bits)
 
 struct FilmIn {
@@ -202,18 +201,8 @@ __global__ __launch_bounds__(kThreads) void k_film_max(FilmIn f, size_t npix, do
     }
 }
 
-struct MapOut {
-    uint8_t *pixels;
-    uint32_t width, height;
-    uint32_t row_pixels;  // pixels of one output row (BMP: width; PPM: all, as one row)
-    uint32_t rows;
-    uint32_t row_bytes;   // BMP: 3 * width rounded up to 4; PPM: 3 * width * height
-    uint32_t groups_per_row;
-    int bmp;
-};
-
-__global__ __launch_bounds__(kThreads) void k_film_map(FilmIn f, const double2 *partial, uint32_t npart, MapOut o,
-                                                       int mode)
+// Folds the partial pairs of k_film_max into partial[npart] (one block).
+__global__ __launch_bounds__(kThreads) void k_film_fold(double2 *partial, uint32_t npart)
 {
     __shared__ double2 s[kThreads / 64];
     double m = 0.0, ml = 0.0;
@@ -225,44 +214,75 @@ __global__ __launch_bounds__(kThreads) void k_film_map(FilmIn f, const double2 *
     fold(m, ml);
     if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = make_double2(m, ml);
     __syncthreads();
-    for (int w = 0; w < kThreads / 64; ++w) { m = dmax(m, s[w].x); ml = dmax(ml, s[w].y); }
-    const Maxima mx{m / f.samples, ml / f.samples};
-
-    const size_t ngroups = (size_t)o.rows * o.groups_per_row;
-    const size_t stride = (size_t)gridDim.x * kThreads;
-    for (size_t g = (size_t)blockIdx.x * kThreads + threadIdx.x; g < ngroups; g += stride) {
-        const uint32_t j = (uint32_t)(g / o.groups_per_row);
-        const uint32_t x0 = (uint32_t)(g % o.groups_per_row) * 4u;
-        const uint32_t cnt = min(4u, o.row_pixels - x0);
-        // source pixel of output position (j, x0): BMP rows run bottom-up
-        const size_t src = o.bmp ? (size_t)(o.height - 1u - j) * o.width + x0 : (size_t)x0;
-        uint32_t b[12];
-#pragma unroll
-        for (uint32_t q = 0; q < 4; ++q) {
-            if (q < cnt) {
-                double p[3];
-                pixel_sum(f, src + q, p);
-                tonemap(p, mx, f.samples, mode);
-                const uint32_t r = to_byte(p[0]), gg = to_byte(p[1]), bb = to_byte(p[2]);
-                b[3 * q] = o.bmp ? bb : r;
-                b[3 * q + 1] = gg;
-                b[3 * q + 2] = o.bmp ? r : bb;
-            } else {
-                b[3 * q] = b[3 * q + 1] = b[3 * q + 2] = 0u;
-            }
-        }
-        uint8_t *dst = o.pixels + (size_t)j * o.row_bytes + 3u * x0;
-        if (cnt == 4) {
-            uint32_t *d = reinterpret_cast<uint32_t *>(dst);
-            d[0] = b[0] | b[1] << 8 | b[2] << 16 | b[3] << 24;
-            d[1] = b[4] | b[5] << 8 | b[6] << 16 | b[7] << 24;
-            d[2] = b[8] | b[9] << 8 | b[10] << 16 | b[11] << 24;
-        } else {
-            // last group of a row: its pixels, then the row padding as zeros
-            const uint32_t end = o.row_bytes - 3u * x0;
-            for (uint32_t k = 0; k < end; ++k) dst[k] = (uint8_t)(k < 3u * cnt ? b[k] : 0u);
-        }
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kThreads / 64; ++w) { m = dmax(m, s[w].x); ml = dmax(ml, s[w].y); }
+        partial[npart] = make_double2(m, ml);
     }
+}
+
+// Byte of pow(x, 1 / 2.2) (the gamma operator) without a double pow in the
+// common case; x = c / samples / max, and xa = c * (1 / (samples * max)),
+// which is within a few ulp of it.  A float estimate of xa^(1/2.2) (relative
+// error below 1e-5 for every value the float range holds) refined by one
+// Newton step on y^11 = xa^5 (1 / 2.2 = 5 / 11) is within 1e-9 of the true
+// power; where 255 * y lies more than 1e-4 from an integer, its truncation
+// is the one the reference's pow-multiply-truncate chain gives.  Near an
+// integer, and for xa outside (0, 1] or an estimate that under- or
+// overflows, the double pow of the exact x decides.
+__device__ inline uint32_t gamma_byte(double c, double samples, double max, double inv)
+{
+    const double x = c * inv;
+    if (x > 0.0 && x <= 1.0) {
+        const double y0 = (double)exp2f(log2f((float)x) * (float)(1.0 / 2.2));
+        const double x2 = x * x, x5 = x2 * x2 * x;
+        const double y2 = y0 * y0, y4 = y2 * y2, y10 = y4 * y4 * y2;
+        const double y = y0 - (y10 * y0 - x5) / (11.0 * y10);
+        const double t = y * 255.0;
+        const double k = rint(t);
+        if (fabs(t - k) > 1e-4 || k == 0.0) return (uint32_t)(int32_t)t & 0xffu;
+    }
+    return to_byte(pow(c / samples / max, 1.0 / 2.2));
+}
+
+struct MapOut {
+    uint8_t *pixels;
+    uint32_t width, height;
+    uint32_t row_bytes;  // BMP: 3 * width rounded up to 4; PPM: 3 * width
+    int bmp;
+};
+
+// One lane per pixel of output row blockIdx.y; BMP rows run bottom-up, so
+// output row j holds film row height-1-j, and the row's last lane writes its
+// zero padding.
+__global__ __launch_bounds__(kThreads) void k_film_map(FilmIn f, const double2 *total, MapOut o, int mode)
+{
+    const uint32_t x = blockIdx.x * kThreads + threadIdx.x;
+    const uint32_t j = blockIdx.y;
+    if (x >= o.width) return;
+    const double2 t = *total;
+    const Maxima mx{t.x / f.samples, t.y / f.samples};
+    const uint32_t y = o.bmp ? o.height - 1u - j : j;
+    double p[3];
+    pixel_sum(f, (size_t)y * o.width + x, p);
+    uint32_t r, g, b;
+    if (mode == RT_TONE_GAMMA_CLAMP) {
+        const double inv = 1.0 / (f.samples * mx.max);
+        r = gamma_byte(p[0], f.samples, mx.max, inv);
+        g = gamma_byte(p[1], f.samples, mx.max, inv);
+        b = gamma_byte(p[2], f.samples, mx.max, inv);
+    } else {
+        tonemap(p, mx, f.samples, mode);
+        r = to_byte(p[0]);
+        g = to_byte(p[1]);
+        b = to_byte(p[2]);
+    }
+    uint8_t *row = o.pixels + (size_t)j * o.row_bytes;
+    uint8_t *dst = row + 3u * x;
+    dst[0] = (uint8_t)(o.bmp ? b : r);
+    dst[1] = (uint8_t)g;
+    dst[2] = (uint8_t)(o.bmp ? r : b);
+    if (x == o.width - 1u)
+        for (uint32_t k = 3u * o.width; k < o.row_bytes; ++k) row[k] = 0;
 }
 
 uint32_t bmp_row_bytes(uint32_t w) { return (3u * w + 3u) & ~3u; }
@@ -359,7 +379,7 @@ rt_status rt_film_tonemap(rt_ctx *ctx, const double *sums, const uint32_t *nvali
         set_error("rt_film_tonemap: a buffer is not device memory of the film's size on this context's device");
         return RT_ERR_INVALID;
     }
-    double2 *partial = (double2 *)rt::ctx_film_scratch(ctx, kPartials * sizeof(double2));
+    double2 *partial = (double2 *)rt::ctx_film_scratch(ctx, (kPartials + 1) * sizeof(double2));
     if (!partial) { set_error("rt_film_tonemap: device allocation failed"); return RT_ERR_OOM; }
     hipStream_t stream = stream_ ? (hipStream_t)stream_ : (hipStream_t)rt::ctx_stream_ptr(ctx);
 
@@ -367,19 +387,17 @@ rt_status rt_film_tonemap(rt_ctx *ctx, const double *sums, const uint32_t *nvali
     const uint32_t nmax = (uint32_t)std::min<size_t>(kPartials, (npix + kThreads - 1) / kThreads);
     k_film_max<<<nmax, kThreads, 0, stream>>>(f, npix, partial);
     RT_HIP(hipGetLastError());
+    k_film_fold<<<1, kThreads, 0, stream>>>(partial, nmax);
+    RT_HIP(hipGetLastError());
 
     MapOut o{};
     o.pixels = pixels;
     o.width = width;
     o.height = height;
     o.bmp = format == RT_IMAGE_BMP;
-    o.row_pixels = o.bmp ? width : (uint32_t)npix;
-    o.rows = o.bmp ? height : 1u;
-    o.row_bytes = o.bmp ? bmp_row_bytes(width) : (uint32_t)(3 * npix);
-    o.groups_per_row = (o.row_pixels + 3u) / 4u;
-    const size_t ngroups = (size_t)o.rows * o.groups_per_row;
-    const uint32_t nmap = (uint32_t)std::min<size_t>(kMapBlocks, (ngroups + kThreads - 1) / kThreads);
-    k_film_map<<<nmap, kThreads, 0, stream>>>(f, partial, nmax, o, (int)mode);
+    o.row_bytes = o.bmp ? bmp_row_bytes(width) : 3u * width;
+    const dim3 grid((width + kThreads - 1) / kThreads, height);
+    k_film_map<<<grid, kThreads, 0, stream>>>(f, partial + nmax, o, (int)mode);
     RT_HIP(hipGetLastError());
     return RT_OK;
 }

@@ -158,6 +158,10 @@ _SIGS = {
     "rt_image_pixel_bytes": (C.c_size_t, [C.c_int, _U32, _U32]),
     "rt_image_header": (C.c_size_t, [C.c_int, _U32, _U32, _P, C.c_size_t]),
     "rt_film_tonemap": (C.c_int, [_P, _P, _P, _U32, _U32, C.c_uint64, C.c_int, C.c_int, _P, C.c_size_t, _P]),
+    "rt_photon_map_build": (C.c_int, [_P, _P, _P, _U32, C.POINTER(_P)]),
+    "rt_photon_map_destroy": (C.c_int, [_P]),
+    "rt_photon_map_get_info": (C.c_int, [_P, C.POINTER(_U32), C.POINTER(_U32)]),
+    "rt_photon_map_knn": (C.c_int, [_P, _P, _P, _U32, _U32, C.c_float, _P, _P, _P, _P]),
     "rt_gen_soup": (C.c_int, [_U32, C.c_uint64, C.c_float, C.c_float, _P]),
     "rt_gen_rays": (C.c_int, [_U32, _U32, _P, C.c_float, _U32, C.c_uint64, _P, _P, _P, _P, _P, _P]),
 }
@@ -673,6 +677,57 @@ def write_image(ctx, name, sums, width, height, samples, mode="gammaClamp", fmt=
         f.write(image_header(fmt, width, height))
         f.write(pixels.cpu().numpy().tobytes())
     return path
+
+
+class PhotonMap:
+    """rt_photon_map: the kd-tree photon map (Scene::buildPhotonStructures,
+    integration/Scene.cu:1102-1114) and its k-nearest search
+    (Scene::search_nearest, Scene.cu:704-722)."""
+
+    def __init__(self, ctx, xyz, shape=None):
+        import numpy as np
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        sh = None if shape is None else np.ascontiguousarray(shape, np.int32).reshape(-1)
+        if sh is not None and sh.size != xyz.shape[0]:
+            raise ValueError("one shape id per photon")
+        h = _P()
+        check(lib().rt_photon_map_build(ctx.h, _ptr(xyz), _ptr(sh), xyz.shape[0], C.byref(h)))
+        self.ctx, self.h, self.n = ctx, h, xyz.shape[0]
+
+    def info(self):
+        n, d = _U32(), _U32()
+        check(lib().rt_photon_map_get_info(self.h, C.byref(n), C.byref(d)))
+        return n.value, d.value
+
+    def nearest(self, queries, k, max_distance=float("inf"), query_shape=None, stream=None):
+        """KDTree::nearest_neighbors for each query (a float32 device tensor
+        of shape (m, 3)): returns (count (m,), index (m, k), distance (m, k))
+        device tensors."""
+        import torch
+        q = queries.contiguous()
+        if q.dtype != torch.float32 or q.dim() != 2 or q.shape[1] != 3:
+            raise ValueError("queries must be a float32 (m, 3) device tensor")
+        m = q.shape[0]
+        qs = None if query_shape is None else query_shape.to(device=q.device, dtype=torch.int32).contiguous()
+        if qs is not None and qs.numel() != m:
+            raise ValueError("one query shape id per query")
+        cnt = torch.empty(m, dtype=torch.int32, device=q.device)
+        idx = torch.empty((m, k), dtype=torch.int32, device=q.device)
+        dist = torch.empty((m, k), dtype=torch.float32, device=q.device)
+        self.ctx.launch(lambda s: check(lib().rt_photon_map_knn(self.h, _ptr(q), _ptr(qs), m, k, max_distance,
+                                                                _ptr(cnt), _ptr(idx), _ptr(dist), s)), stream)
+        return cnt, idx, dist
+
+    def close(self):
+        if self.h:
+            lib().rt_photon_map_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def film_image(film, spp_total):

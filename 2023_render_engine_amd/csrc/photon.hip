@@ -1,0 +1,346 @@
+// photon.hip -- the photon map on the GPU: the kd-tree built by
+// photon_build.cpp in HBM, and a batched k-nearest search that keeps the
+// semantics of KDTree::nearest_neighbors (cuda_libraries/kdtree.h, as
+// Scene::search_nearest calls it, integration/Scene.cu:704-722):
+//   * a photon is taken if its distance is strictly below the current search
+//     radius and, for a query with a shape id, if it lies on that shape;
+//   * the first k taken fill the result, which then becomes a max-heap on
+//     distance; each later one replaces the farthest and the radius shrinks
+//     to the heap's new maximum;
+//   * the near child is searched first; the far child only if the distance
+//     to the splitting plane, as the norm of (0, .., p[axis] - split, .., 0),
+//     is still below the radius after the near child is done.
+// Distances are float: the square root of ((0 + dx*dx) + dy*dy) + dz*dz
+// (the library is built with -ffp-contract=off), as the reference's norm.
+//
+// Queries are taken in the Morton order of their position (a radix sort in
+// the map's work space), one lane per query; the recursion becomes an
+// explicit stack of index ranges in private memory (at most the tree's depth + 1 entries); the heap
+// lives in LDS for k <= 128, else in the query's output slice.
+#include <hip/hip_runtime.h>
+#include <hipcub/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+#include <new>
+#include <string>
+
+#include "rt_internal.h"
+#include "rt_photon.h"
+
+struct rt_ctx;
+namespace rt {
+void ctx_retain(rt_ctx *c);
+void ctx_release(rt_ctx *c);
+int ctx_device(rt_ctx *c);
+void *ctx_stream_ptr(rt_ctx *c);
+std::recursive_mutex &ctx_lock(rt_ctx *c);
+}  // namespace rt
+
+using rt::set_error;
+
+#define RT_HIP(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            set_error(std::string(#call) + " failed: " + hipGetErrorString(e_));        \
+            return e_ == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;                 \
+        }                                                                               \
+    } while (0)
+
+struct rt_photon_map {
+    rt_ctx *ctx = nullptr;
+    rt::PhotonNode *nodes = nullptr;  // tree order
+    uint32_t *order = nullptr;        // tree position -> input index
+    uint32_t n = 0;
+    uint32_t depth = 0;
+    float lo[3] = {0, 0, 0}, scale[3] = {0, 0, 0};  // Morton grid of the photons' bounding box
+    // query ordering work space, grown on demand: Morton keys and query ids
+    // (two of each for the radix sort) and the sort's temporary storage
+    void *work = nullptr;
+    size_t work_bytes = 0;
+};
+
+namespace {
+
+constexpr int kThreads = 64;      // one wavefront per block: its LDS heaps take k * 512 bytes
+constexpr uint32_t kStack = 32;    // >= tree depth + 1 (checked on the host)
+constexpr uint32_t kMaxK = 1024;
+constexpr uint32_t kLdsK = 128;    // k up to this keeps its heap in LDS
+
+__device__ inline float norm3(float a, float b, float c)
+{
+    float s = 0.0f;
+    s += a * a;
+    s += b * b;
+    s += c * c;
+    return sqrtf(s);
+}
+
+__device__ inline uint32_t spread10(uint32_t v)
+{
+    v &= 1023u;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// 30-bit Morton code of each query in the photons' bounding box (outside
+// points clamp to its faces), and the identity permutation to sort with it.
+__global__ __launch_bounds__(256) void k_query_morton(const float *q, uint32_t m, float lx, float ly, float lz,
+                                                      float sx, float sy, float sz, uint32_t *key, uint32_t *id)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    const float c[3] = {(q[3 * (size_t)i] - lx) * sx, (q[3 * (size_t)i + 1] - ly) * sy, (q[3 * (size_t)i + 2] - lz) * sz};
+    uint32_t u[3];
+    for (int a = 0; a < 3; ++a) {
+        const float v = c[a] > 0.0f ? (c[a] < 1023.0f ? c[a] : 1023.0f) : 0.0f;  // NaN -> 0
+        u[a] = (uint32_t)v;
+    }
+    key[i] = spread10(u[0]) << 2 | spread10(u[1]) << 1 | spread10(u[2]);
+    id[i] = i;
+}
+
+struct KnnArgs {
+    const uint32_t *perm;     // lane -> query (Morton order), or null
+    const rt::PhotonNode *nodes;
+    const uint32_t *order;
+    uint32_t n;
+    const float *queries;     // m * 3
+    const int32_t *qshape;    // m or null
+    uint32_t m, k;
+    float radius;
+    uint32_t *count;          // m
+    uint32_t *index;          // m * k
+    float *dist;              // m * k
+};
+
+// Strided view of a heap: entry i at base[i * stride].
+struct HeapRef {
+    float *d;
+    uint32_t *v;
+    uint32_t stride;
+    __device__ float &dist(uint32_t i) const { return d[i * stride]; }
+    __device__ uint32_t &id(uint32_t i) const { return v[i * stride]; }
+};
+
+__device__ inline void sift_down(const HeapRef &h, uint32_t n, uint32_t i)
+{
+    const float d = h.dist(i);
+    const uint32_t v = h.id(i);
+    for (;;) {
+        uint32_t c = 2 * i + 1;
+        if (c >= n) break;
+        if (c + 1 < n && h.dist(c + 1) > h.dist(c)) ++c;
+        if (!(h.dist(c) > d)) break;
+        h.dist(i) = h.dist(c);
+        h.id(i) = h.id(c);
+        i = c;
+    }
+    h.dist(i) = d;
+    h.id(i) = v;
+}
+
+// LDS: the heap lives in LDS, slot-major (entry i of lane t at i * 64 + t,
+// so the lanes of the wavefront hit distinct banks), and is copied to the
+// query's output slice at the end; otherwise it lives in the output slice.
+template <bool LDS>
+__global__ __launch_bounds__(kThreads) void k_photon_knn(KnnArgs a)
+{
+    extern __shared__ uint32_t lds_heap[];  // LDS: k * kThreads distances, then k * kThreads ids
+    float *lds_d = reinterpret_cast<float *>(lds_heap);
+    uint32_t *lds_v = lds_heap + (LDS ? a.k * kThreads : 0u);
+    const uint32_t lane = blockIdx.x * kThreads + threadIdx.x;
+    if (lane >= a.m) return;
+    const uint32_t q = a.perm ? a.perm[lane] : lane;
+    const float p[3] = {a.queries[3 * (size_t)q], a.queries[3 * (size_t)q + 1], a.queries[3 * (size_t)q + 2]};
+    const uint32_t want = a.qshape ? (uint32_t)(a.qshape[q] + 1) : 0u;  // 0: any shape
+    const HeapRef h = LDS ? HeapRef{lds_d + threadIdx.x, lds_v + threadIdx.x, (uint32_t)kThreads}
+                          : HeapRef{a.dist + (size_t)q * a.k, a.index + (size_t)q * a.k, 1u};
+    float maxd = a.radius;
+    uint32_t cnt = 0;
+
+    uint32_t sl[kStack], sr[kStack];
+    float spd[kStack];
+    uint32_t sp = 0;
+    sl[0] = 0; sr[0] = a.n; spd[0] = -1.0f; sp = 1;
+    while (sp > 0) {
+        --sp;
+        const uint32_t l = sl[sp], r = sr[sp];
+        if (spd[sp] >= 0.0f && !(spd[sp] < maxd)) continue;
+        if (r <= l) continue;
+        const uint32_t med = l + (r - l) / 2;
+        const rt::PhotonNode nd = a.nodes[med];
+        const float d = norm3(p[0] - nd.x, p[1] - nd.y, p[2] - nd.z);
+        if (d < maxd && (want == 0u || (nd.meta >> 2) == want)) {
+            if (cnt < a.k) {
+                h.dist(cnt) = d;
+                h.id(cnt) = med;
+                ++cnt;
+                if (cnt == a.k)
+                    for (uint32_t i = a.k / 2; i-- > 0;) sift_down(h, a.k, i);
+            } else if (d < h.dist(0)) {
+                h.dist(0) = d;
+                h.id(0) = med;
+                sift_down(h, a.k, 0);
+                maxd = h.dist(0);
+            } else {
+                maxd = h.dist(0);
+            }
+        }
+        if (r - l > 1) {
+            const uint32_t ax = nd.meta & 3u;
+            const float split = ax == 0 ? nd.x : (ax == 1 ? nd.y : nd.z);
+            const float diff = p[ax] - split;
+            const float pd = norm3(ax == 0 ? diff : 0.0f, ax == 1 ? diff : 0.0f, ax == 2 ? diff : 0.0f);
+            uint32_t nl, nr, fl, fr;
+            if (p[ax] < split) { nl = l; nr = med; fl = med + 1; fr = r; }
+            else { nl = med + 1; nr = r; fl = l; fr = med; }
+            if (sp + 2 > kStack) break;  // unreachable: the host checks depth + 1 <= kStack
+            sl[sp] = fl; sr[sp] = fr; spd[sp] = pd; ++sp;
+            sl[sp] = nl; sr[sp] = nr; spd[sp] = -1.0f; ++sp;
+        }
+    }
+    float *od = a.dist + (size_t)q * a.k;
+    uint32_t *oi = a.index + (size_t)q * a.k;
+    for (uint32_t i = 0; i < cnt; ++i) {
+        oi[i] = a.order[h.id(i)];
+        if (LDS) od[i] = h.dist(i);
+    }
+    a.count[q] = cnt;
+}
+
+}  // namespace
+
+extern "C" {
+
+rt_status rt_photon_map_build(rt_ctx *ctx, const float *xyz, const int32_t *shape, uint32_t n, rt_photon_map **out)
+{
+    if (!out) { set_error("rt_photon_map_build: out is null"); return RT_ERR_INVALID; }
+    *out = nullptr;
+    if (!ctx || (n && !xyz)) { set_error("rt_photon_map_build: null argument"); return RT_ERR_INVALID; }
+    if (n > (1u << 30)) { set_error("rt_photon_map_build: at most 2^30 photons"); return RT_ERR_INVALID; }
+    for (size_t i = 0; i < 3 * (size_t)n; ++i)
+        if (!std::isfinite(xyz[i])) { set_error("rt_photon_map_build: non-finite photon location"); return RT_ERR_INVALID; }
+    if (shape)
+        for (uint32_t i = 0; i < n; ++i)
+            if (shape[i] < -1 || shape[i] > (1 << 30) - 2) {
+                set_error("rt_photon_map_build: shape ids must be -1 .. 2^30 - 2");
+                return RT_ERR_INVALID;
+            }
+    rt::HostPhotonTree t;
+    rt::build_photon_tree(xyz, shape, n, t);
+    if (t.depth + 1 > kStack) { set_error("rt_photon_map_build: tree too deep"); return RT_ERR_INVALID; }
+    rt_photon_map *m = new (std::nothrow) rt_photon_map();
+    if (!m) { set_error("rt_photon_map_build: host allocation failed"); return RT_ERR_OOM; }
+    std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(ctx));
+    hipError_t e = hipSetDevice(rt::ctx_device(ctx));
+    if (e == hipSuccess && n) e = hipMalloc(&m->nodes, (size_t)n * sizeof(rt::PhotonNode));
+    if (e == hipSuccess && n) e = hipMalloc(&m->order, (size_t)n * sizeof(uint32_t));
+    if (e == hipSuccess && n) e = hipMemcpy(m->nodes, t.nodes.data(), (size_t)n * sizeof(rt::PhotonNode), hipMemcpyHostToDevice);
+    if (e == hipSuccess && n) e = hipMemcpy(m->order, t.order.data(), (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (m->nodes) (void)hipFree(m->nodes);
+        if (m->order) (void)hipFree(m->order);
+        delete m;
+        set_error(std::string("rt_photon_map_build: ") + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;
+    }
+    m->ctx = ctx;
+    m->n = n;
+    m->depth = t.depth;
+    if (n) {
+        float hi[3];
+        for (int a = 0; a < 3; ++a) m->lo[a] = hi[a] = xyz[a];
+        for (size_t i = 1; i < n; ++i)
+            for (int a = 0; a < 3; ++a) {
+                m->lo[a] = std::min(m->lo[a], xyz[3 * i + a]);
+                hi[a] = std::max(hi[a], xyz[3 * i + a]);
+            }
+        for (int a = 0; a < 3; ++a) m->scale[a] = hi[a] > m->lo[a] ? 1024.0f / (hi[a] - m->lo[a]) : 0.0f;
+    }
+    rt::ctx_retain(ctx);
+    *out = m;
+    return RT_OK;
+}
+
+rt_status rt_photon_map_destroy(rt_photon_map *m)
+{
+    if (!m) return RT_OK;
+    {
+        std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(m->ctx));
+        (void)hipSetDevice(rt::ctx_device(m->ctx));
+        (void)hipStreamSynchronize((hipStream_t)rt::ctx_stream_ptr(m->ctx));
+        if (m->nodes) (void)hipFree(m->nodes);
+        if (m->order) (void)hipFree(m->order);
+        if (m->work) (void)hipFree(m->work);
+    }
+    rt::ctx_release(m->ctx);
+    delete m;
+    return RT_OK;
+}
+
+rt_status rt_photon_map_get_info(const rt_photon_map *m, uint32_t *n, uint32_t *depth)
+{
+    if (!m) { set_error("rt_photon_map_get_info: null map"); return RT_ERR_INVALID; }
+    if (n) *n = m->n;
+    if (depth) *depth = m->depth;
+    return RT_OK;
+}
+
+rt_status rt_photon_map_knn(rt_photon_map *m, const float *queries, const int32_t *query_shape, uint32_t nq,
+                            uint32_t k, float max_distance, uint32_t *count, uint32_t *index, float *distance,
+                            void *stream_)
+{
+    if (!m || (nq && (!queries || !count || !index || !distance))) {
+        set_error("rt_photon_map_knn: null argument");
+        return RT_ERR_INVALID;
+    }
+    if (k == 0 || k > kMaxK) { set_error("rt_photon_map_knn: k must be 1..1024"); return RT_ERR_INVALID; }
+    if (!(max_distance > 0.0f)) { set_error("rt_photon_map_knn: max_distance must be positive"); return RT_ERR_INVALID; }
+    if ((size_t)nq * k > (1ull << 31) || nq > (1u << 30)) {
+        set_error("rt_photon_map_knn: at most 2^30 queries and queries * k <= 2^31");
+        return RT_ERR_INVALID;
+    }
+    if (nq == 0) return RT_OK;
+    std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(m->ctx));
+    RT_HIP(hipSetDevice(rt::ctx_device(m->ctx)));
+    hipStream_t stream = stream_ ? (hipStream_t)stream_ : (hipStream_t)rt::ctx_stream_ptr(m->ctx);
+    // queries in Morton order of their position, so the lanes of a wavefront
+    // walk neighbouring parts of the tree (results go to the callers' slots)
+    size_t sort_bytes = 0;
+    RT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                              (uint32_t *)nullptr, (uint32_t *)nullptr, (int)nq, 0, 30, stream));
+    const size_t ids = ((size_t)nq * sizeof(uint32_t) + 255) & ~(size_t)255;
+    const size_t need = 4 * ids + sort_bytes;
+    if (need > m->work_bytes) {
+        if (m->work) {
+            RT_HIP(hipStreamSynchronize(stream));
+            RT_HIP(hipFree(m->work));
+            m->work = nullptr;
+            m->work_bytes = 0;
+        }
+        RT_HIP(hipMalloc(&m->work, need));
+        m->work_bytes = need;
+    }
+    uint8_t *w = (uint8_t *)m->work;
+    uint32_t *key0 = (uint32_t *)w, *key1 = (uint32_t *)(w + ids), *id0 = (uint32_t *)(w + 2 * ids),
+             *id1 = (uint32_t *)(w + 3 * ids);
+    k_query_morton<<<(nq + 255) / 256, 256, 0, stream>>>(queries, nq, m->lo[0], m->lo[1], m->lo[2], m->scale[0],
+                                                         m->scale[1], m->scale[2], key0, id0);
+    RT_HIP(hipGetLastError());
+    RT_HIP(hipcub::DeviceRadixSort::SortPairs(w + 4 * ids, sort_bytes, key0, key1, id0, id1, (int)nq, 0, 30, stream));
+    KnnArgs a{id1, m->nodes, m->order, m->n, queries, query_shape, nq, k, max_distance, count, index, distance};
+    if (k <= kLdsK)
+        k_photon_knn<true><<<(nq + kThreads - 1) / kThreads, kThreads, (size_t)k * kThreads * 8, stream>>>(a);
+    else
+        k_photon_knn<false><<<(nq + kThreads - 1) / kThreads, kThreads, 0, stream>>>(a);
+    RT_HIP(hipGetLastError());
+    return RT_OK;
+}
+
+}  // extern "C"

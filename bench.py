@@ -563,6 +563,72 @@ def run_film(args):
     return res
 
 
+def run_photon(args):
+    """Photon map gather queries: 2^20 photons (the reference's
+    N_PHOTON_SAMPLES, clustered on 16 blobs as photons gather on surfaces),
+    one k-nearest query per pixel of a 1920x1080 frame, k = 100 within radius
+    100 (MAX_NEIGHBOURS, PHOTON_SEARCH_RADIUS).  One step = one
+    rt_photon_map_knn call over all queries; value = Mqueries/s."""
+    import numpy as np
+    import torch
+
+    rt = importlib.import_module("2023_render_engine_amd")
+    n, m, k, radius = 1 << 20, 1920 * 1080, 100, 100.0
+    rng = np.random.default_rng(3)
+    centres = rng.random((16, 3)) * 10
+    x = (centres[rng.integers(0, 16, n)] + rng.normal(0, 0.2, (n, 3))).astype(np.float32)
+    qn = (centres[rng.integers(0, 16, m)] + rng.normal(0, 0.2, (m, 3))).astype(np.float32)
+    ctx = rt.Context(0)
+    t0 = time.time()
+    pm = rt.PhotonMap(ctx, x)
+    build_s = time.time() - t0
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.Stream(device=dev)
+    with torch.cuda.stream(stream):
+        q = torch.from_numpy(qn).to(dev)
+        for _ in range(args.warmup):
+            pm.nearest(q, k, radius)
+        stream.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(args.steps):
+            cnt, idx, dist = pm.nearest(q, k, radius)
+        e1.record(stream)
+        stream.synchronize()
+        wall = time.perf_counter() - t0
+    call_ms = e0.elapsed_time(e1) / args.steps
+    # compulsory bytes per query: 12 B point, 4 B count, 8 B per neighbour
+    io = m * (12 + 4 + 8 * k)
+    gbs = io / (call_ms * 1e-3) / 1e9
+    res = {
+        "metric": "photon map k-nearest queries", "value": round(m * args.steps / wall / 1e6, 2),
+        "unit": "Mqueries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{n} photons in 16 clusters, {m} queries (one per 1920x1080 pixel), k={k}, "
+                               f"radius {radius}", "build_s": round(build_s, 3)},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(gbs / 8000.0, 4), "traffic": None,
+                     "note": "compulsory I/O only (12 + 4 + 8k B per query); node fetches are cached gathers"},
+    }
+    if not args.no_cpu_baseline:
+        import oracle
+        tree = oracle.kd_build(x)[:2]
+        spent, done, chunk = 0.0, 0, 4096
+        while spent < min(args.cpu_budget, 10.0) and done + chunk <= m:
+            t1 = time.time()
+            oracle.knn(x, qn[done:done + chunk], k, radius, tree=tree, nthreads=cpu_threads())
+            spent += time.time() - t1
+            done += chunk
+        res["cpu_baseline"] = {"value": round(done / spent / 1e6, 4), "unit": "Mqueries/s", "cores": cpu_threads(),
+                               "kind": "port", "sample": f"{done} of the queries with the oracle's restatement of "
+                                                         f"the kd-tree search, {spent:.1f} s"}
+    pm.close()
+    ctx.close()
+    return res
+
+
 def run_copy(args):
     """HBM calibration: a plain device-to-device copy of 1 GiB (torch), so the
     FETCH_SIZE / WRITE_SIZE counters of a kernel of known traffic can scale
@@ -592,7 +658,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c2", "c3", "c4", "copy", "film"])
+    ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c2", "c3", "c4", "copy", "film", "photon"])
     ap.add_argument("--tone", default="gammaClamp", help="film: tone mapping (a ToneMappingType name)")
     ap.add_argument("--c3-ref-spp", type=int, default=128)
     ap.add_argument("--ref-spp", type=int, default=1024,
@@ -630,6 +696,8 @@ def main():
         out = run_copy(args)
     elif args.workload == "film":
         out = run_film(args)
+    elif args.workload == "photon":
+        out = run_photon(args)
     else:
         out = run_render(args, args.workload)
     print(json.dumps(out), flush=True)

@@ -320,6 +320,36 @@ rt_status rt_film_tonemap(rt_ctx *ctx, const double *sums, const uint32_t *nvali
                           size_t pixel_bytes, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* Photon map: kd-tree and k-nearest search                             */
+/* ------------------------------------------------------------------ */
+/* The photon map of Scene::buildPhotonStructures / search_nearest
+ * (integration/Scene.cu:704-722, 1102-1114) over nn::KDTree<Photon, 3,
+ * PhotonPosition> (cuda_libraries/kdtree.h). */
+typedef struct rt_photon_map rt_photon_map;
+
+/* Builds the kd-tree of n photon locations (xyz: host, 3n floats) as
+ * KDTree's constructor does (median split on the axis of largest extent) and
+ * uploads it.  shape: host, n shape ids (Photon::shapeId, -1 .. 2^30-2), or
+ * null for all -1.  Photons are referred to by their index in xyz. */
+rt_status rt_photon_map_build(rt_ctx *ctx, const float *xyz, const int32_t *shape, uint32_t n,
+                              rt_photon_map **out);
+rt_status rt_photon_map_destroy(rt_photon_map *map);
+rt_status rt_photon_map_get_info(const rt_photon_map *map, uint32_t *n, uint32_t *depth);
+
+/* KDTree::nearest_neighbors(p, k, max_distance, shape) for nq query points
+ * (device buffers).  queries: 3*nq floats; query_shape: nq ids (-1: any
+ * shape) or null.  Outputs per query: count[q] <= k photons, their indices
+ * index[q*k ..] and float distances distance[q*k ..] in the reference's
+ * result order (insertion order below k, max-heap order once k were found).
+ * The set is the k nearest photons strictly within max_distance (on the
+ * query's shape); photons at a distance equal to the k-th may be exchanged
+ * for one another.  k: 1..1024, nq*k <= 2^31.  Enqueued on stream (null:
+ * the context's stream). */
+rt_status rt_photon_map_knn(rt_photon_map *map, const float *queries, const int32_t *query_shape, uint32_t nq,
+                            uint32_t k, float max_distance, uint32_t *count, uint32_t *index, float *distance,
+                            void *stream);
+
+/* ------------------------------------------------------------------ */
 /* Batches: many scenes in one launch sequence                          */
 /* ------------------------------------------------------------------ */
 /* No counterpart in the reference, which renders one scene per process

@@ -82,7 +82,37 @@ _SIGS = {
     "orc_image_pixel_bytes": (C.c_size_t, [C.c_int, _U32, _U32]),
     "orc_image_header": (C.c_size_t, [C.c_int, _U32, _U32, _P, C.c_size_t]),
     "orc_film_image": (None, [_P, _P, _U32, _U32, C.c_uint64, C.c_int, C.c_int, _P, _P]),
+    "orc_kd_build": (_U32, [_P, _U32, _P, _P]),
+    "orc_knn_batch": (None, [_P, _P, _P, _P, _U32, _P, _P, _U32, _U32, _F, C.c_int, _P, _P, _P, C.c_int]),
 }
+
+
+def kd_build(xyz):
+    """kdtree.h build_tree: (order, axis, depth)."""
+    x = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    order = np.zeros(x.shape[0], np.uint32)
+    axis = np.zeros(x.shape[0], np.uint8)
+    depth = lib().orc_kd_build(_p(x), x.shape[0], _p(order), _p(axis))
+    return order, axis, depth
+
+
+def knn(xyz, queries, k, radius=np.inf, shape=None, query_shape=None, tree=None, nthreads=8):
+    """k nearest photons per query: brute force (tree None) or the restated
+    kd-tree search over tree = (order, axis).  Returns (count, index, distance)
+    with index / distance of shape (m, k) (entries past count are undefined)."""
+    x = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    q = np.ascontiguousarray(queries, np.float32).reshape(-1, 3)
+    sh = None if shape is None else np.ascontiguousarray(shape, np.int32)
+    qs = None if query_shape is None else np.ascontiguousarray(query_shape, np.int32)
+    m = q.shape[0]
+    cnt = np.zeros(m, np.uint32)
+    idx = np.zeros((m, k), np.uint32)
+    dist = np.zeros((m, k), np.float32)
+    order, axis = (None, None) if tree is None else (np.ascontiguousarray(tree[0], np.uint32),
+                                                     np.ascontiguousarray(tree[1], np.uint8))
+    lib().orc_knn_batch(_p(x), _p(sh), _p(order), _p(axis), x.shape[0], _p(q), _p(qs), m, k, float(radius),
+                        0 if tree is None else 1, _p(cnt), _p(idx), _p(dist), nthreads)
+    return cnt, idx, dist
 
 # ToneMappingType (camera/Film.cu:13-21) and the two image formats
 TONE_MAPPINGS = ("equalize", "reinhard", "reinhardJodie", "ACESfilmic", "uncharted2", "gammaClamp")

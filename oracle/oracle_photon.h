@@ -1,0 +1,52 @@
+/*
+ * oracle_photon.h -- CPU restatement of the reference's photon map search
+ * (TEST INFRASTRUCTURE ONLY; see oracle.h): nn::KDTree's build and
+ * nearest_neighbors (cuda_libraries/kdtree.h) as Scene::search_nearest calls
+ * them (integration/Scene.cu:704-722), plus a brute-force k-nearest search
+ * that states the same result by definition.
+ *
+ * Distances are float, sqrt(((0 + dx*dx) + dy*dy) + dz*dz), as the
+ * reference's default norm.  A photon qualifies if its distance is strictly
+ * below the radius and (for a query shape other than -1) its shape matches.
+ *
+ * Pinning: the brute force is the definition of the result set (the k
+ * qualifying photons of smallest distance); tests/test_oracle_photon.py
+ * checks the restated tree search against it on queries whose k-th and
+ * (k+1)-th distances differ, and the restated build against the median and
+ * widest-axis properties of every node.
+ */
+#ifndef RT_ORACLE_PHOTON_H
+#define RT_ORACLE_PHOTON_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* kdtree.h build_tree: order[i] = input index at tree position i, axis[i] =
+ * split axis of the range whose median is i.  Ties in a coordinate are put
+ * in input order (nth_element leaves them unspecified).  Returns the depth. */
+uint32_t orc_kd_build(const float *xyz, uint32_t n, uint32_t *order, uint8_t *axis);
+
+/* kdtree.h nearest_neighbors_impl over a tree from orc_kd_build; writes up to
+ * k input indices and distances, returns the count. */
+uint32_t orc_kd_knn(const float *xyz, const int32_t *shape, const uint32_t *order, const uint8_t *axis, uint32_t n,
+                    const float q[3], int32_t qshape, uint32_t k, float radius, uint32_t *idx, float *dist);
+
+/* Brute force: the k qualifying photons of smallest (distance, index),
+ * sorted that way. */
+uint32_t orc_brute_knn(const float *xyz, const int32_t *shape, uint32_t n, const float q[3], int32_t qshape,
+                       uint32_t k, float radius, uint32_t *idx, float *dist);
+
+/* Many queries (m * 3 floats, qshape m ids or null), nthreads threads:
+ * brute force (tree == 0) or tree search (tree != 0, over order / axis). */
+void orc_knn_batch(const float *xyz, const int32_t *shape, const uint32_t *order, const uint8_t *axis, uint32_t n,
+                   const float *q, const int32_t *qshape, uint32_t m, uint32_t k, float radius, int tree,
+                   uint32_t *count, uint32_t *idx, float *dist, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,201 @@
+/*
+ * photon.c -- oracle restatement of the photon map search (TEST
+ * INFRASTRUCTURE ONLY; see oracle_photon.h).
+ */
+#include "oracle_photon.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* kdtree.h: norm(difference(p, e)) with the default Euclidean norm in float */
+static float norm3(float a, float b, float c)
+{
+    float s = 0.0f;
+    s += a * a;
+    s += b * b;
+    s += c * c;
+    return sqrtf(s);
+}
+
+static float dist_to(const float *xyz, uint32_t e, const float q[3])
+{
+    return norm3(q[0] - xyz[3 * (size_t)e], q[1] - xyz[3 * (size_t)e + 1], q[2] - xyz[3 * (size_t)e + 2]);
+}
+
+/* ---------------- build (kdtree.h build_tree) ---------------- */
+
+static const float *g_xyz;
+static int g_axis;
+
+static int cmp_axis(const void *a, const void *b)
+{
+    const uint32_t i = *(const uint32_t *)a, j = *(const uint32_t *)b;
+    const float x = g_xyz[3 * (size_t)i + g_axis], y = g_xyz[3 * (size_t)j + g_axis];
+    if (x < y) return -1;
+    if (x > y) return 1;
+    return i < j ? -1 : (i > j);
+}
+
+static uint32_t build(const float *xyz, uint32_t *order, uint8_t *axis, size_t left, size_t right, uint32_t level)
+{
+    if (right - left <= 1) return level + 1;
+    float lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) lo[a] = hi[a] = xyz[3 * (size_t)order[left] + a];
+    for (size_t i = left + 1; i < right; ++i)
+        for (int a = 0; a < 3; ++a) {
+            const float v = xyz[3 * (size_t)order[i] + a];
+            if (lo[a] > v) lo[a] = v;
+            if (hi[a] < v) hi[a] = v;
+        }
+    const size_t median = (right + left) / 2;
+    int ax = 0;
+    float ext = hi[0] - lo[0];
+    for (int a = 1; a < 3; ++a)
+        if (hi[a] - lo[a] > ext) { ax = a; ext = hi[a] - lo[a]; }
+    /* nth_element's partition property, by a full sort of the range */
+    g_xyz = xyz;
+    g_axis = ax;
+    qsort(order + left, right - left, sizeof(uint32_t), cmp_axis);
+    axis[median] = (uint8_t)ax;
+    const uint32_t d0 = build(xyz, order, axis, left, median, level + 1);
+    const uint32_t d1 = build(xyz, order, axis, median + 1, right, level + 1);
+    return d0 > d1 ? d0 : d1;
+}
+
+uint32_t orc_kd_build(const float *xyz, uint32_t n, uint32_t *order, uint8_t *axis)
+{
+    for (uint32_t i = 0; i < n; ++i) { order[i] = i; axis[i] = 0; }
+    return n ? build(xyz, order, axis, 0, n, 0) : 0;
+}
+
+/* ---------------- search (kdtree.h nearest_neighbors_impl) ---------------- */
+
+typedef struct {
+    const float *xyz;
+    const int32_t *shape;
+    const uint32_t *order;
+    const uint8_t *axis;
+    const float *q;
+    int32_t qshape;
+    uint32_t k, cnt;
+    float maxd;
+    uint32_t *idx;
+    float *dist;
+} Search;
+
+static void sift_down(float *d, uint32_t *v, uint32_t n, uint32_t i)
+{
+    for (;;) {
+        uint32_t c = 2 * i + 1;
+        if (c >= n) return;
+        if (c + 1 < n && d[c + 1] > d[c]) ++c;
+        if (!(d[c] > d[i])) return;
+        const float td = d[i]; d[i] = d[c]; d[c] = td;
+        const uint32_t tv = v[i]; v[i] = v[c]; v[c] = tv;
+        i = c;
+    }
+}
+
+static void search(Search *s, size_t left, size_t right)
+{
+    if (right <= left) return;
+    const size_t median = (right + left) / 2;
+    const uint32_t e = s->order[median];
+    const float d = dist_to(s->xyz, e, s->q);
+    if (d < s->maxd && (s->qshape == -1 || (s->shape ? s->shape[e] : -1) == s->qshape)) {
+        /* push_back; make_heap at k; beyond k push_heap + pop_heap + pop_back */
+        if (s->cnt < s->k) {
+            s->idx[s->cnt] = e;
+            s->dist[s->cnt] = d;
+            if (++s->cnt == s->k)
+                for (uint32_t i = s->k / 2; i-- > 0;) sift_down(s->dist, s->idx, s->k, i);
+        } else {
+            if (d < s->dist[0]) {
+                s->dist[0] = d;
+                s->idx[0] = e;
+                sift_down(s->dist, s->idx, s->k, 0);
+            }
+            s->maxd = s->dist[0];
+        }
+    }
+    if (right - left > 1) {
+        const int ax = s->axis[median];
+        const float split = s->xyz[3 * (size_t)e + ax];
+        float pl[3] = {0.0f, 0.0f, 0.0f};
+        pl[ax] = s->q[ax] - split;
+        if (s->q[ax] < split) {
+            search(s, left, median);
+            if (norm3(pl[0], pl[1], pl[2]) < s->maxd) search(s, median + 1, right);
+        } else {
+            search(s, median + 1, right);
+            if (norm3(pl[0], pl[1], pl[2]) < s->maxd) search(s, left, median);
+        }
+    }
+}
+
+uint32_t orc_kd_knn(const float *xyz, const int32_t *shape, const uint32_t *order, const uint8_t *axis, uint32_t n,
+                    const float q[3], int32_t qshape, uint32_t k, float radius, uint32_t *idx, float *dist)
+{
+    Search s = {xyz, shape, order, axis, q, qshape, k, 0, radius, idx, dist};
+    search(&s, 0, n);
+    return s.cnt;
+}
+
+/* ---------------- brute force ---------------- */
+
+uint32_t orc_brute_knn(const float *xyz, const int32_t *shape, uint32_t n, const float q[3], int32_t qshape,
+                       uint32_t k, float radius, uint32_t *idx, float *dist)
+{
+    uint32_t cnt = 0;
+    for (uint32_t e = 0; e < n; ++e) {
+        if (qshape != -1 && (shape ? shape[e] : -1) != qshape) continue;
+        const float d = dist_to(xyz, e, q);
+        if (!(d < radius)) continue;
+        /* insertion into the sorted prefix by (distance, index) */
+        if (cnt == k && !(d < dist[k - 1])) continue;
+        uint32_t i = cnt < k ? cnt++ : k - 1;
+        while (i > 0 && dist[i - 1] > d) { dist[i] = dist[i - 1]; idx[i] = idx[i - 1]; --i; }
+        dist[i] = d;
+        idx[i] = e;
+    }
+    return cnt;
+}
+
+typedef struct {
+    const float *xyz; const int32_t *shape; const uint32_t *order; const uint8_t *axis; uint32_t n;
+    const float *q; const int32_t *qshape; uint32_t m, k; float radius; int tree;
+    uint32_t *count, *idx; float *dist;
+    int t, nt;
+} Batch;
+
+static void *batch_worker(void *p)
+{
+    const Batch *b = (const Batch *)p;
+    for (uint32_t i = (uint32_t)b->t; i < b->m; i += (uint32_t)b->nt) {
+        const int32_t qs = b->qshape ? b->qshape[i] : -1;
+        uint32_t *ix = b->idx + (size_t)i * b->k;
+        float *ds = b->dist + (size_t)i * b->k;
+        b->count[i] = b->tree ? orc_kd_knn(b->xyz, b->shape, b->order, b->axis, b->n, b->q + 3 * (size_t)i, qs, b->k,
+                                           b->radius, ix, ds)
+                              : orc_brute_knn(b->xyz, b->shape, b->n, b->q + 3 * (size_t)i, qs, b->k, b->radius, ix,
+                                              ds);
+    }
+    return NULL;
+}
+
+void orc_knn_batch(const float *xyz, const int32_t *shape, const uint32_t *order, const uint8_t *axis, uint32_t n,
+                   const float *q, const int32_t *qshape, uint32_t m, uint32_t k, float radius, int tree,
+                   uint32_t *count, uint32_t *idx, float *dist, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 64) nthreads = 64;
+    pthread_t th[64];
+    Batch b[64];
+    for (int t = 0; t < nthreads; ++t) {
+        b[t] = (Batch){xyz, shape, order, axis, n, q, qshape, m, k, radius, tree, count, idx, dist, t, nthreads};
+        pthread_create(&th[t], NULL, batch_worker, &b[t]);
+    }
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+}

@@ -162,6 +162,7 @@ _SIGS = {
     "rt_photon_map_destroy": (C.c_int, [_P]),
     "rt_photon_map_get_info": (C.c_int, [_P, C.POINTER(_U32), C.POINTER(_U32)]),
     "rt_photon_map_knn": (C.c_int, [_P, _P, _P, _U32, _U32, C.c_float, _P, _P, _P, _P]),
+    "rt_photon_gather_lambert": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _U32, _U32, C.c_float, _P, _P]),
     "rt_gen_soup": (C.c_int, [_U32, C.c_uint64, C.c_float, C.c_float, _P]),
     "rt_gen_rays": (C.c_int, [_U32, _U32, _P, C.c_float, _U32, C.c_uint64, _P, _P, _P, _P, _P, _P]),
 }
@@ -717,6 +718,29 @@ class PhotonMap:
         self.ctx.launch(lambda s: check(lib().rt_photon_map_knn(self.h, _ptr(q), _ptr(qs), m, k, max_distance,
                                                                 _ptr(cnt), _ptr(idx), _ptr(dist), s)), stream)
         return cnt, idx, dist
+
+    def gather_lambert(self, photon_rgb, photon_dir, points, normals, kd, k=100, max_distance=100.0,
+                       query_shape=None, stream=None):
+        """combinePhotonContributions on Lambertian surfaces (Scene.cu:607-634):
+        device tensors photon_rgb (n, 3) float64 and photon_dir (n, 3)
+        float32 indexed like the map's photons; points, normals, kd (m, 3)
+        float32.  Returns the (m, 3) float64 radiance estimates."""
+        import torch
+        m = points.shape[0]
+        for name, t, dt, rows in (("photon_rgb", photon_rgb, torch.float64, self.n),
+                                  ("photon_dir", photon_dir, torch.float32, self.n),
+                                  ("points", points, torch.float32, m), ("normals", normals, torch.float32, m),
+                                  ("kd", kd, torch.float32, m)):
+            if t.dtype != dt or tuple(t.shape) != (rows, 3) or not t.is_contiguous():
+                raise ValueError(f"{name} must be a contiguous {dt} ({rows}, 3) tensor")
+        qs = None if query_shape is None else query_shape.to(device=points.device, dtype=torch.int32).contiguous()
+        if qs is not None and qs.numel() != m:
+            raise ValueError("one query shape id per query")
+        out = torch.empty((m, 3), dtype=torch.float64, device=points.device)
+        self.ctx.launch(lambda s: check(lib().rt_photon_gather_lambert(
+            self.h, _ptr(photon_rgb), _ptr(photon_dir), _ptr(points), _ptr(normals), _ptr(kd), _ptr(qs), m, k,
+            max_distance, _ptr(out), s)), stream)
+        return out
 
     def close(self):
         if self.h:

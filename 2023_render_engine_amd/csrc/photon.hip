@@ -60,6 +60,8 @@ struct rt_photon_map {
     // (two of each for the radix sort) and the sort's temporary storage
     void *work = nullptr;
     size_t work_bytes = 0;
+    void *gather = nullptr;  // k-nearest results of rt_photon_gather_lambert, grown on demand
+    size_t gather_bytes = 0;
 };
 
 namespace {
@@ -214,6 +216,41 @@ __global__ __launch_bounds__(kThreads) void k_photon_knn(KnnArgs a)
     a.count[q] = cnt;
 }
 
+// combinePhotonContributions (integration/Scene.cu:607-634) on a Lambertian
+// surface: sum over the query's photons of radiance * kd * INV_PI *
+// |cos(n, -photon direction)| (LambertianBSDF::lightReflected), times
+// 1 / (PI * maxDistance^2) in float, maxDistance the farthest photon found.
+__global__ __launch_bounds__(256) void k_photon_gather_lambert(const double *rgb, const float *dir, const float *nrm,
+                                                               const float *kd, uint32_t m, uint32_t k,
+                                                               const uint32_t *count, const uint32_t *index,
+                                                               const float *dist, double *out)
+{
+    const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= m) return;
+    const uint32_t c = count[q];
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (c > 0) {
+        const float inv_pi = 1.0f / 3.14159265358979323846f;
+        const double f0 = (double)kd[3 * (size_t)q] * (double)inv_pi, f1 = (double)kd[3 * (size_t)q + 1] * (double)inv_pi,
+                     f2 = (double)kd[3 * (size_t)q + 2] * (double)inv_pi;
+        const float nx = nrm[3 * (size_t)q], ny = nrm[3 * (size_t)q + 1], nz = nrm[3 * (size_t)q + 2];
+        float maxd = 0.0f;
+        for (uint32_t i = 0; i < c; ++i) {
+            const uint32_t e = index[(size_t)q * k + i];
+            maxd = fmaxf(maxd, dist[(size_t)q * k + i]);
+            const float cosv = fabsf(-(nx * dir[3 * (size_t)e] + ny * dir[3 * (size_t)e + 1] + nz * dir[3 * (size_t)e + 2]));
+            acc[0] += rgb[3 * (size_t)e] * (f0 * (double)cosv);
+            acc[1] += rgb[3 * (size_t)e + 1] * (f1 * (double)cosv);
+            acc[2] += rgb[3 * (size_t)e + 2] * (f2 * (double)cosv);
+        }
+        const float kernel = 1.0f / (3.14159265358979323846f * maxd * maxd);
+        acc[0] *= (double)kernel; acc[1] *= (double)kernel; acc[2] *= (double)kernel;
+    }
+    out[3 * (size_t)q] = acc[0];
+    out[3 * (size_t)q + 1] = acc[1];
+    out[3 * (size_t)q + 2] = acc[2];
+}
+
 }  // namespace
 
 extern "C" {
@@ -278,6 +315,7 @@ rt_status rt_photon_map_destroy(rt_photon_map *m)
         if (m->nodes) (void)hipFree(m->nodes);
         if (m->order) (void)hipFree(m->order);
         if (m->work) (void)hipFree(m->work);
+        if (m->gather) (void)hipFree(m->gather);
     }
     rt::ctx_release(m->ctx);
     delete m;
@@ -339,6 +377,46 @@ rt_status rt_photon_map_knn(rt_photon_map *m, const float *queries, const int32_
         k_photon_knn<true><<<(nq + kThreads - 1) / kThreads, kThreads, (size_t)k * kThreads * 8, stream>>>(a);
     else
         k_photon_knn<false><<<(nq + kThreads - 1) / kThreads, kThreads, 0, stream>>>(a);
+    RT_HIP(hipGetLastError());
+    return RT_OK;
+}
+
+rt_status rt_photon_gather_lambert(rt_photon_map *m, const double *photon_rgb, const float *photon_dir,
+                                   const float *points, const float *normals, const float *kd,
+                                   const int32_t *query_shape, uint32_t nq, uint32_t k, float max_distance,
+                                   double *out, void *stream_)
+{
+    if (!m || (nq && (!points || !normals || !kd || !out)) || (m->n && nq && (!photon_rgb || !photon_dir))) {
+        set_error("rt_photon_gather_lambert: null argument");
+        return RT_ERR_INVALID;
+    }
+    if (nq == 0) return RT_OK;
+    if (k == 0 || k > kMaxK || (size_t)nq * k > (1ull << 31) || nq > (1u << 30)) {
+        set_error("rt_photon_gather_lambert: k must be 1..1024, queries <= 2^30, queries * k <= 2^31");
+        return RT_ERR_INVALID;
+    }
+    std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(m->ctx));
+    RT_HIP(hipSetDevice(rt::ctx_device(m->ctx)));
+    hipStream_t stream = stream_ ? (hipStream_t)stream_ : (hipStream_t)rt::ctx_stream_ptr(m->ctx);
+    const size_t cb = ((size_t)nq * 4 + 255) & ~(size_t)255, ib = ((size_t)nq * k * 4 + 255) & ~(size_t)255;
+    const size_t need = cb + 2 * ib;
+    if (need > m->gather_bytes) {
+        if (m->gather) {
+            RT_HIP(hipStreamSynchronize(stream));
+            RT_HIP(hipFree(m->gather));
+            m->gather = nullptr;
+            m->gather_bytes = 0;
+        }
+        RT_HIP(hipMalloc(&m->gather, need));
+        m->gather_bytes = need;
+    }
+    uint8_t *w = (uint8_t *)m->gather;
+    uint32_t *cnt = (uint32_t *)w, *idx = (uint32_t *)(w + cb);
+    float *dist = (float *)(w + cb + ib);
+    rt_status st = rt_photon_map_knn(m, points, query_shape, nq, k, max_distance, cnt, idx, dist, stream);
+    if (st != RT_OK) return st;
+    k_photon_gather_lambert<<<(nq + 255) / 256, 256, 0, stream>>>(photon_rgb, photon_dir, normals, kd, nq, k, cnt,
+                                                                  idx, dist, out);
     RT_HIP(hipGetLastError());
     return RT_OK;
 }

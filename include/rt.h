@@ -349,6 +349,21 @@ rt_status rt_photon_map_knn(rt_photon_map *map, const float *queries, const int3
                             uint32_t k, float max_distance, uint32_t *count, uint32_t *index, float *distance,
                             void *stream);
 
+/* Photon-mapping density estimate on Lambertian surfaces:
+ * Scene::getPhotonMappingContribution's combinePhotonContributions
+ * (integration/Scene.cu:607-634, 724-746) for nq query points at once.  Per
+ * query: the k nearest photons within max_distance (on query_shape[q] unless
+ * it is -1 or query_shape is null), then
+ *   out = sum(photon_rgb * kd / pi * |cos(normal, -photon_dir)|) / (pi * d_max^2)
+ * with d_max the farthest photon found (0 photons give 0).  photon_rgb
+ * (Photon::radiance, n*3 doubles) and photon_dir (Photon::ray.direction, n*3
+ * floats) are indexed like the map's photons; points, normals (the shading
+ * normals) and kd are nq*3 floats; out is nq*3 doubles; all device memory. */
+rt_status rt_photon_gather_lambert(rt_photon_map *map, const double *photon_rgb, const float *photon_dir,
+                                   const float *points, const float *normals, const float *kd,
+                                   const int32_t *query_shape, uint32_t nq, uint32_t k, float max_distance,
+                                   double *out, void *stream);
+
 /* ------------------------------------------------------------------ */
 /* Batches: many scenes in one launch sequence                          */
 /* ------------------------------------------------------------------ */

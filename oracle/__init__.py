@@ -84,7 +84,24 @@ _SIGS = {
     "orc_film_image": (None, [_P, _P, _U32, _U32, C.c_uint64, C.c_int, C.c_int, _P, _P]),
     "orc_kd_build": (_U32, [_P, _U32, _P, _P]),
     "orc_knn_batch": (None, [_P, _P, _P, _P, _U32, _P, _P, _U32, _U32, _F, C.c_int, _P, _P, _P, C.c_int]),
+    "orc_photon_gather_lambert": (None, [_P, _P, _U32, _P, _P, _P, _P, _P, _P, _U32, _U32, _F, _P]),
 }
+
+
+def photon_gather_lambert(xyz, rgb, dirs, points, normals, kd, k, radius, shape=None, query_shape=None):
+    """combinePhotonContributions on Lambertian surfaces (Scene.cu:607-634)."""
+    x = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    c = np.ascontiguousarray(rgb, np.float64).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    q = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    nr = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+    kd_ = np.ascontiguousarray(kd, np.float32).reshape(-1, 3)
+    sh = None if shape is None else np.ascontiguousarray(shape, np.int32)
+    qs = None if query_shape is None else np.ascontiguousarray(query_shape, np.int32)
+    out = np.zeros((q.shape[0], 3), np.float64)
+    lib().orc_photon_gather_lambert(_p(x), _p(sh), x.shape[0], _p(c), _p(d), _p(q), _p(nr), _p(kd_), _p(qs),
+                                    q.shape[0], k, float(radius), _p(out))
+    return out
 
 
 def kd_build(xyz):

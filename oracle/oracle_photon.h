@@ -45,6 +45,13 @@ void orc_knn_batch(const float *xyz, const int32_t *shape, const uint32_t *order
                    const float *q, const int32_t *qshape, uint32_t m, uint32_t k, float radius, int tree,
                    uint32_t *count, uint32_t *idx, float *dist, int nthreads);
 
+/* combinePhotonContributions (integration/Scene.cu:607-634) on a Lambertian
+ * surface for m queries, over the brute-force k nearest photons:
+ * sum(rgb * (kd * INV_PI) * |cos(normal, -dir)|) * 1 / (PI * d_max^2). */
+void orc_photon_gather_lambert(const float *xyz, const int32_t *shape, uint32_t n, const double *rgb, const float *dir,
+                               const float *q, const float *nrm, const float *kd, const int32_t *qshape, uint32_t m,
+                               uint32_t k, float radius, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -199,3 +199,38 @@ void orc_knn_batch(const float *xyz, const int32_t *shape, const uint32_t *order
     }
     for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
 }
+
+/* ---------------- density estimate (Scene.cu:607-634, Lambertian) ---------------- */
+
+void orc_photon_gather_lambert(const float *xyz, const int32_t *shape, uint32_t n, const double *rgb, const float *dir,
+                               const float *q, const float *nrm, const float *kd, const int32_t *qshape, uint32_t m,
+                               uint32_t k, float radius, double *out)
+{
+    uint32_t *idx = (uint32_t *)malloc(sizeof(uint32_t) * k);
+    float *dist = (float *)malloc(sizeof(float) * k);
+    const float inv_pi = 1.0f / 3.14159265358979323846f; /* INV_PI = 1 / PI in Float */
+    for (uint32_t i = 0; i < m; ++i) {
+        const uint32_t c = orc_brute_knn(xyz, shape, n, q + 3 * (size_t)i, qshape ? qshape[i] : -1, k, radius, idx, dist);
+        double acc[3] = {0.0, 0.0, 0.0};
+        if (c > 0) {
+            float maxd = 0.0f;
+            for (uint32_t j = 0; j < c; ++j) {
+                const uint32_t e = idx[j];
+                if (dist[j] > maxd) maxd = dist[j];
+                /* lightScattered(-photon direction): |cos| to the shading normal, times color * INV_PI */
+                const float w[3] = {-dir[3 * (size_t)e], -dir[3 * (size_t)e + 1], -dir[3 * (size_t)e + 2]};
+                const float cosv = fabsf(nrm[3 * (size_t)i] * w[0] + nrm[3 * (size_t)i + 1] * w[1] +
+                                         nrm[3 * (size_t)i + 2] * w[2]);
+                for (int ch = 0; ch < 3; ++ch) {
+                    const double diffuse = (double)kd[3 * (size_t)i + ch] * (double)inv_pi;
+                    acc[ch] += rgb[3 * (size_t)e + ch] * (diffuse * (double)cosv);
+                }
+            }
+            const float kernel = 1.0f / (3.14159265358979323846f * maxd * maxd);
+            for (int ch = 0; ch < 3; ++ch) acc[ch] = acc[ch] * (double)kernel;
+        }
+        out[3 * (size_t)i] = acc[0]; out[3 * (size_t)i + 1] = acc[1]; out[3 * (size_t)i + 2] = acc[2];
+    }
+    free(idx);
+    free(dist);
+}

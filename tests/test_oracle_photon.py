@@ -110,3 +110,24 @@ def test_brute_force_against_numpy():
         assert np.allclose(dist[i, :cnt[i]], d[i, want], rtol=1e-6)
         # equal as sets up to exchanges among (near-)equal distances
         assert len(set(idx[i, :cnt[i]]) ^ set(want)) <= 2
+
+
+def test_gather_closed_form():
+    """One photon of radiance L arriving along -n at distance d from the
+    query on a surface of albedo kd: L * kd / pi * 1 / (pi d^2) (the density
+    estimate's kernel 1 / (pi d_max^2), Scene.cu:607-634); two photons at
+    distances d and 2d give (L1 + L2 cos) * kd / pi / (pi (2d)^2)."""
+    x = np.array([[0.0, 0.0, 0.5], [1.0, 0.0, 0.0]], np.float32)
+    rgb = np.array([[2.0, 1.0, 0.5], [4.0, 4.0, 4.0]])
+    dirs = np.array([[0.0, 0.0, -1.0], [-0.6, 0.0, -0.8]], np.float32)
+    q = np.zeros((1, 3), np.float32)
+    n = np.array([[0.0, 0.0, 1.0]], np.float32)
+    kd = np.array([[0.5, 0.25, 1.0]], np.float32)
+    one = oracle.photon_gather_lambert(x, rgb, dirs, q, n, kd, 1, 100.0)
+    want = rgb[0] * kd[0] / np.pi / (np.pi * 0.25)
+    assert np.allclose(one[0], want, rtol=1e-6)
+    two = oracle.photon_gather_lambert(x, rgb, dirs, q, n, kd, 2, 100.0)
+    want2 = (rgb[0] * 1.0 + rgb[1] * 0.8) * kd[0] / np.pi / (np.pi * 1.0)
+    assert np.allclose(two[0], want2, rtol=1e-6)
+    none = oracle.photon_gather_lambert(x, rgb, dirs, q, n, kd, 2, 0.1)
+    assert not none.any()

@@ -287,14 +287,23 @@ __global__ __launch_bounds__(kThreads) void k_film_map(FilmIn f, const double2 *
 
 uint32_t bmp_row_bytes(uint32_t w) { return (3u * w + 3u) & ~3u; }
 
-// The device allocation holding ptr must reach ptr + bytes.
-bool in_allocation(const void *ptr, size_t bytes)
+// ptr must be device (or managed) memory of `device`; where the runtime
+// knows the allocation's range (plain hipMalloc blocks), it must also reach
+// ptr + bytes.  Memory whose range the runtime does not report (virtual
+// memory mappings) is accepted on its attributes alone.
+bool device_buffer_ok(const void *ptr, size_t bytes, int device)
 {
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, ptr) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if ((at.type != hipMemoryTypeDevice && at.type != hipMemoryTypeManaged) || at.device != device) return false;
     hipDeviceptr_t base = nullptr;
     size_t size = 0;
     if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)ptr) != hipSuccess) {
         (void)hipGetLastError();
-        return false;
+        return true;
     }
     const uintptr_t p = (uintptr_t)ptr, b0 = (uintptr_t)base;
     return p >= b0 && p - b0 <= size && bytes <= size - (p - b0);
@@ -374,8 +383,9 @@ rt_status rt_film_tonemap(rt_ctx *ctx, const double *sums, const uint32_t *nvali
     const size_t npix = (size_t)width * height;
     std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(ctx));
     RT_HIP(hipSetDevice(rt::ctx_device(ctx)));
-    if (!in_allocation(sums, npix * 3 * sizeof(double)) || (nvalid && !in_allocation(nvalid, npix * sizeof(uint32_t))) ||
-        !in_allocation(pixels, need)) {
+    const int dev = rt::ctx_device(ctx);
+    if (!device_buffer_ok(sums, npix * 3 * sizeof(double), dev) ||
+        (nvalid && !device_buffer_ok(nvalid, npix * sizeof(uint32_t), dev)) || !device_buffer_ok(pixels, need, dev)) {
         set_error("rt_film_tonemap: a buffer is not device memory of the film's size on this context's device");
         return RT_ERR_INVALID;
     }

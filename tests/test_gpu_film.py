@@ -152,6 +152,8 @@ def test_bad_arguments_rejected(rt, gpu_ctx):
     assert L.rt_film_tonemap(gpu_ctx.h, p(sums), None, 4, 4, 1, 0, 0, p(out), 47, None) == rt.RT_ERR_INVALID
     assert L.rt_film_tonemap(gpu_ctx.h, p(sums), None, 4, 4, 1, 0, 0, p(out) + 1, 63, None) == rt.RT_ERR_INVALID
     assert L.rt_film_tonemap(gpu_ctx.h, p(sums), None, 0, 4, 1, 0, 0, p(out), 64, None) == rt.RT_ERR_INVALID
+    host = np.zeros(4 * 4 * 3)
+    assert L.rt_film_tonemap(gpu_ctx.h, p(host), None, 4, 4, 1, 0, 0, p(out), 64, None) == rt.RT_ERR_INVALID
     assert L.rt_film_tonemap(gpu_ctx.h, p(sums), None, 4, 4, 1, 0, 0, p(out), 48, None) == rt.RT_OK
     gpu_ctx.synchronize()
     with pytest.raises(ValueError):

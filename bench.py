@@ -538,6 +538,13 @@ def run_film(args):
     npix = w * h
     bytes_per_call = 51 * npix
     gbs = bytes_per_call / (call_ms * 1e-3) / 1e9
+    traffic = None
+    pmc_path = os.path.join(ROOT, "profiles", "film_tonemap_1080p_hbm_traffic.json")
+    if mode == "gammaClamp" and os.path.exists(pmc_path):
+        try:
+            traffic = json.load(open(pmc_path)).get("hbm_bytes_per_call")
+        except (OSError, ValueError):
+            traffic = None
     res = {
         "metric": "film tone mapping + BMP packing", "value": round(npix * args.steps / wall / 1e6, 1),
         "unit": "Mpixels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -545,8 +552,9 @@ def run_film(args):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"1920x1080 FP64 film, {spp} samples, {mode}, 24-bit BMP pixel array"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
-                     "frac": round(gbs / 8000.0, 4), "traffic": None,
-                     "note": "one call = k_film_max + k_film_map, 51 B/pixel, timed with HIP events on its stream"},
+                     "frac": round(gbs / 8000.0, 4), "traffic": traffic,
+                     "note": "one call = k_film_max + k_film_fold + k_film_map, 51 B/pixel, timed with HIP events on "
+                             "its stream; traffic from the FETCH_SIZE / WRITE_SIZE profile of the same call"},
     }
     if not args.no_cpu_baseline:
         import oracle

@@ -19,6 +19,9 @@ LIB_PATH = os.path.join(_HERE, "librt_amd.so")
 
 RT_OK, RT_ERR_INVALID, RT_ERR_HIP, RT_ERR_OOM, RT_ERR_UNSUPPORTED, RT_ERR_NO_DEVICE = range(6)
 RT_FLAG_HOST_PTRS = 1
+RT_FLAG_SHADOW = 4
+RT_FLAG_PROFILE = 8
+RT_FLAG_COUNT = 16
 
 
 class RtError(RuntimeError):
@@ -70,7 +73,11 @@ class RenderOpts(C.Structure):
 
 class RenderStats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
-                ("ms", C.c_double)]
+                ("ms", C.c_double), ("extend_ms", C.c_double), ("connect_ms", C.c_double),
+                ("extend_launches", C.c_uint32), ("connect_launches", C.c_uint32),
+                ("ext_node_visits", C.c_uint64), ("ext_tri_tests", C.c_uint64),
+                ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64),
+                ("shadow_occluded", C.c_uint64)]
 
 
 def f3(v):
@@ -321,21 +328,25 @@ class Bvh:
     def _rays(o6, tmax):
         return Rays(*[_ptr(x) for x in o6], _ptr(tmax))
 
-    def trace_closest(self, ox, oy, oz, dx, dy, dz, tmax=None, prim=None, t=None, host=False, stream=None):
-        """Device tensors in, device tensors out (or numpy with host=True)."""
+    def trace_closest(self, ox, oy, oz, dx, dy, dz, tmax=None, prim=None, t=None, host=False, stream=None,
+                      shadow=False):
+        """Device tensors in, device tensors out (or numpy with host=True).
+        shadow=True: the closest opaque hit (RT_FLAG_SHADOW, the reference's
+        intersectRays with checkShadows)."""
         n = int(ox.shape[0])
         r = self._rays((ox, oy, oz, dx, dy, dz), tmax)
+        fl = RT_FLAG_SHADOW if shadow else 0
         if host:
             import numpy as np
             prim = np.empty(n, np.int32) if prim is None else prim
             t = np.empty(n, np.float32) if t is None else t
-            check(lib().rt_trace_closest(self.h, C.byref(r), n, _ptr(prim), _ptr(t), RT_FLAG_HOST_PTRS, None))
+            check(lib().rt_trace_closest(self.h, C.byref(r), n, _ptr(prim), _ptr(t), RT_FLAG_HOST_PTRS | fl, None))
             return prim, t
         import torch
         dev = ox.device
         prim = torch.empty(n, dtype=torch.int32, device=dev) if prim is None else prim
         t = torch.empty(n, dtype=torch.float32, device=dev) if t is None else t
-        self.ctx.launch(lambda s: check(lib().rt_trace_closest(self.h, C.byref(r), n, _ptr(prim), _ptr(t), 0, s)),
+        self.ctx.launch(lambda s: check(lib().rt_trace_closest(self.h, C.byref(r), n, _ptr(prim), _ptr(t), fl, s)),
                         stream)
         return prim, t
 
@@ -457,10 +468,14 @@ class Scene:
         return (torch.zeros(n * 3, dtype=torch.float64, device=dev), torch.zeros(n, dtype=torch.int32, device=dev))
 
     def render(self, film, spp=None, spp_offset=0, max_depth=None, rr_depth=None, seed=None, path_capacity=0,
-               grid=0, chunk_paths=0, stream=None):
+               grid=0, chunk_paths=0, stream=None, profile=False, count=False):
+        """rt_render.  profile: time every traversal launch with HIP events
+        (RT_FLAG_PROFILE); count: counting traversal kernels (RT_FLAG_COUNT);
+        both are read back with stats()."""
         o = self._opts(spp, spp_offset, max_depth, rr_depth, seed, path_capacity, grid, chunk_paths)
         accum, nvalid = film
-        self.ctx.launch(lambda s: check(lib().rt_render(self.h, C.byref(o), _ptr(accum), _ptr(nvalid), 0, s)),
+        fl = (RT_FLAG_PROFILE if profile else 0) | (RT_FLAG_COUNT if count else 0)
+        self.ctx.launch(lambda s: check(lib().rt_render(self.h, C.byref(o), _ptr(accum), _ptr(nvalid), fl, s)),
                         stream)
         return film
 
@@ -477,14 +492,14 @@ class Scene:
                           path_capacity, grid, chunk_paths)
 
     def render_cached(self, film, cache, cache_depth=2, spp=None, spp_offset=0, max_depth=None, rr_depth=None,
-                      seed=None, path_capacity=0, stream=None):
+                      seed=None, path_capacity=0, stream=None, profile=False, count=False):
         """rt_render_cached: paths end at their first cacheable vertex at
         depth >= cache_depth and take the radiance cache's answer."""
-        import torch
         o = self._opts(spp, spp_offset, max_depth, rr_depth, seed, path_capacity)
         accum, nvalid = film
+        fl = (RT_FLAG_PROFILE if profile else 0) | (RT_FLAG_COUNT if count else 0)
         self.ctx.launch(lambda s: check(lib().rt_render_cached(self.h, C.byref(o), cache.h, cache_depth, _ptr(accum),
-                                                               _ptr(nvalid), 0, s)), stream)
+                                                               _ptr(nvalid), fl, s)), stream)
         return film
 
     def train_cache(self, cache, frame, n_paths=65536, cache_depth=2, max_depth=None, rr_depth=None, seed=None,
@@ -540,14 +555,15 @@ class Batch:
             pass
 
     def render(self, films, spp, spp_offset=0, max_depth=12, rr_depth=3, seeds=None, path_capacity=0, grid=0,
-               graph=False, chunk_paths=0, stream=None):
+               graph=False, chunk_paths=0, stream=None, profile=False, count=False):
         import numpy as np
         o = BatchOpts(spp, spp_offset, max_depth, rr_depth, path_capacity, grid, chunk_paths)
         n = len(self.scenes)
         acc = (_P * n)(*[_ptr(f[0]) for f in films])
         nv = (_P * n)(*[_ptr(f[1]) for f in films])
         sd = None if seeds is None else np.ascontiguousarray(seeds, np.uint32)
-        flags = RT_FLAG_GRAPH if graph else 0
+        flags = ((RT_FLAG_GRAPH if graph else 0) | (RT_FLAG_PROFILE if profile else 0) |
+                 (RT_FLAG_COUNT if count else 0))
         self.ctx.launch(lambda s: check(lib().rt_batch_render(self.h, C.byref(o), None if sd is None else _ptr(sd),
                                                               acc, nv, flags, s)), stream)
         return films

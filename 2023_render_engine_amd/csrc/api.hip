@@ -352,6 +352,8 @@ static rt_status trace_common(rt_bvh *b, const rt_rays *rays, uint32_t n, int32_
     if (!b->ctx) { set_error("rt_trace: host-only BVH (built without a context)"); return RT_ERR_INVALID; }
     if (n == 0) return RT_OK;
     if (!rays_valid(rays) || (any ? !occ : !prim)) { set_error("rt_trace: null ray or output array"); return RT_ERR_INVALID; }
+    if (flags & ~(uint32_t)(RT_FLAG_HOST_PTRS | RT_FLAG_SHADOW)) { set_error("rt_trace: unknown flag"); return RT_ERR_INVALID; }
+    const bool opaque = (flags & RT_FLAG_SHADOW) != 0;  // any hit skips transparent triangles always
     std::lock_guard<std::recursive_mutex> g(b->ctx->lock);
     RT_HIP(hipSetDevice(b->ctx->device));
     hipStream_t stream = stream_ ? (hipStream_t)stream_ : b->ctx->stream;
@@ -361,7 +363,7 @@ static rt_status trace_common(rt_bvh *b, const rt_rays *rays, uint32_t n, int32_
         uint32_t *rc = b->next_counter();
         RT_HIP(rt::clear_claims(rc, stream));
         RT_HIP(hipEventRecord(b->ev0, stream));
-        RT_HIP(rt::launch_trace(b->dev, rd, n, prim, t, occ, any, stream, nullptr, rc));
+        RT_HIP(rt::launch_trace(b->dev, rd, n, prim, t, occ, any, stream, nullptr, rc, opaque));
         RT_HIP(hipEventRecord(b->ev1, stream));
         b->timed = true;
         return RT_OK;
@@ -389,7 +391,7 @@ static rt_status trace_common(rt_bvh *b, const rt_rays *rays, uint32_t n, int32_
     uint32_t *rc = b->next_counter();
     hipError_t e = rt::clear_claims(rc, stream);
     (void)hipEventRecord(b->ev0, stream);
-    if (e == hipSuccess) e = rt::launch_trace(b->dev, rd, n, dprim, dt, docc, any, stream, nullptr, rc);
+    if (e == hipSuccess) e = rt::launch_trace(b->dev, rd, n, dprim, dt, docc, any, stream, nullptr, rc, opaque);
     (void)hipEventRecord(b->ev1, stream);
     b->timed = true;
     if (e == hipSuccess) {

@@ -83,6 +83,7 @@ struct rt_batch {
     hipGraphExec_t exec = nullptr;
     GraphKey key;
     rt_render_stats stats{};
+    rt::KernelProfile prof;  // event pairs of profiled frames (RT_FLAG_PROFILE)
 };
 
 namespace {
@@ -237,7 +238,8 @@ rt::WaveParams batch_params(rt_batch *b, const rt_batch_opts &o, uint32_t grid)
 // streaming loop.  With `graph`, the iteration block is replayed from the
 // captured graph (captured on first use for the current key).
 hipError_t run_frame(rt_batch *b, const rt_batch_opts &o, uint32_t P, uint32_t grid,
-                     const std::vector<rt::FilmDev> &films, bool graph, hipStream_t s)
+                     const std::vector<rt::FilmDev> &films, bool graph, hipStream_t s,
+                     rt::KernelProfile *prof = nullptr)
 {
     const rt::BatchLaunch bl = batch_launch(b, films);
     rt::ChunkLaunch cl{};
@@ -248,6 +250,7 @@ hipError_t run_frame(rt_batch *b, const rt_batch_opts &o, uint32_t P, uint32_t g
     cl.capacity = P;
     cl.res = b->wb.res;
     cl.batch = &bl;
+    cl.prof = graph ? nullptr : prof;
     rt::WaveParams wp = batch_params(b, o, grid);
     if (graph) {
         GraphKey k;
@@ -415,6 +418,12 @@ rt_status rt_batch_render(rt_batch *b, const rt_batch_opts *opts, const uint32_t
     if (st != RT_OK) return st;
     if (!accum || !nvalid) { set_error("rt_batch_render: null film arrays"); return RT_ERR_INVALID; }
     if (flags & RT_FLAG_HOST_PTRS) { set_error("rt_batch_render: films must be device pointers"); return RT_ERR_UNSUPPORTED; }
+    if (flags & ~(uint32_t)(RT_FLAG_GRAPH | RT_FLAG_PROFILE | RT_FLAG_COUNT)) { set_error("rt_batch_render: unknown flag"); return RT_ERR_INVALID; }
+    const bool measure = (flags & (RT_FLAG_PROFILE | RT_FLAG_COUNT)) != 0;
+    if (measure && (flags & RT_FLAG_GRAPH)) {
+        set_error("rt_batch_render: RT_FLAG_PROFILE / RT_FLAG_COUNT frames run eagerly (not with RT_FLAG_GRAPH)");
+        return RT_ERR_INVALID;
+    }
     const size_t n = b->scenes.size();
     std::vector<rt::FilmDev> films(n);
     for (size_t i = 0; i < n; ++i) {
@@ -436,23 +445,26 @@ rt_status rt_batch_render(rt_batch *b, const rt_batch_opts *opts, const uint32_t
     st = upload_bases(b, opts->spp, s);
     if (st != RT_OK) return st;
     auto t0 = std::chrono::steady_clock::now();
-    RT_HIP(hipMemsetAsync(b->wb.queues.stats, 0, 16, s));
+    RT_HIP(hipMemsetAsync(b->wb.queues.stats, 0, rt::kNumStats * 8, s));
     RT_HIP(rt::set_u32(b->d_spp_offset, opts->spp_offset, s));
+    b->prof.events = (flags & RT_FLAG_PROFILE) != 0;
+    b->prof.count = (flags & RT_FLAG_COUNT) != 0;
+    b->prof.used = 0;
     {
-        hipError_t e = run_frame(b, *opts, P, grid, films, (flags & RT_FLAG_GRAPH) != 0, s);
+        hipError_t e = run_frame(b, *opts, P, grid, films, (flags & RT_FLAG_GRAPH) != 0, s,
+                                 measure ? &b->prof : nullptr);
         if (e != hipSuccess) {
             set_error(std::string("rt_batch_render: ") + hipGetErrorString(e));
             return RT_ERR_HIP;
         }
     }
-    unsigned long long counts[2];
+    unsigned long long counts[rt::kNumStats];
     RT_HIP(hipMemcpyAsync(counts, b->wb.queues.stats, sizeof counts, hipMemcpyDeviceToHost, s));
     RT_HIP(hipStreamSynchronize(s));
+    b->stats = rt_render_stats{};
     b->stats.paths = total;
-    b->stats.extension_rays = counts[0];
-    b->stats.shadow_rays = counts[1];
     b->stats.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    return RT_OK;
+    return rt::fill_stats(b->stats, counts, b->prof, flags, "rt_batch_render");
 }
 
 rt_status rt_batch_autotune(rt_batch *b, const rt_batch_opts *opts, rt_tune_entry *best, rt_tune_entry *entries,

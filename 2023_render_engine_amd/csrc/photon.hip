@@ -130,21 +130,75 @@ struct HeapRef {
     __device__ uint32_t &id(uint32_t i) const { return v[i * stride]; }
 };
 
-__device__ inline void sift_down(const HeapRef &h, uint32_t n, uint32_t i)
+// The max-heap steps of libstdc++'s std::make_heap / push_heap / pop_heap
+// (bits/stl_heap.h) that kdtree.h's search runs, with comp(a, b) =
+// dist(a) < dist(b).  The search returns its vector in this heap's layout, so
+// the steps are kept as they are rather than replaced by an equivalent
+// sift-down (which would order ties and equal-set layouts differently).
+// __push_heap: the hole moves up while its parent compares less.
+__device__ inline void heap_push_up(const HeapRef &h, uint32_t hole, uint32_t top, float d, uint32_t v)
 {
-    const float d = h.dist(i);
-    const uint32_t v = h.id(i);
-    for (;;) {
-        uint32_t c = 2 * i + 1;
-        if (c >= n) break;
-        if (c + 1 < n && h.dist(c + 1) > h.dist(c)) ++c;
-        if (!(h.dist(c) > d)) break;
-        h.dist(i) = h.dist(c);
-        h.id(i) = h.id(c);
-        i = c;
+    while (hole > top) {
+        const uint32_t parent = (hole - 1) / 2;
+        if (!(h.dist(parent) < d)) break;
+        h.dist(hole) = h.dist(parent);
+        h.id(hole) = h.id(parent);
+        hole = parent;
     }
-    h.dist(i) = d;
-    h.id(i) = v;
+    h.dist(hole) = d;
+    h.id(hole) = v;
+}
+
+// __adjust_heap over [0, len): the hole goes down to a leaf along the larger
+// children, then the value is pushed up from there.
+__device__ inline void heap_adjust(const HeapRef &h, uint32_t hole, uint32_t len, float d, uint32_t v)
+{
+    const uint32_t top = hole;
+    uint32_t child = hole;
+    while (len > 0 && child < (len - 1) / 2) {
+        child = 2 * (child + 1);
+        if (h.dist(child) < h.dist(child - 1)) --child;
+        h.dist(hole) = h.dist(child);
+        h.id(hole) = h.id(child);
+        hole = child;
+    }
+    if ((len & 1u) == 0 && len >= 2 && child == (len - 2) / 2) {
+        child = 2 * (child + 1);
+        h.dist(hole) = h.dist(child - 1);
+        h.id(hole) = h.id(child - 1);
+        hole = child - 1;
+    }
+    heap_push_up(h, hole, top, d, v);
+}
+
+// __make_heap over the k entries found so far.
+__device__ inline void heap_make(const HeapRef &h, uint32_t len)
+{
+    if (len < 2) return;
+    for (uint32_t parent = (len - 2) / 2;; --parent) {
+        heap_adjust(h, parent, len, h.dist(parent), h.id(parent));
+        if (parent == 0) return;
+    }
+}
+
+// push_back + push_heap + pop_heap + pop_back on a full heap of len entries.
+// The transient entry len lives in (xd, xv), so the heap needs no slot
+// beyond len.
+__device__ inline void heap_push_pop(const HeapRef &h, uint32_t len, float d, uint32_t v)
+{
+    float xd = d;
+    uint32_t xv = v;
+    if (len > 0) {
+        const uint32_t parent = (len - 1) / 2;
+        if (h.dist(parent) < d) {  // the first push_heap step moves the parent into slot len
+            xd = h.dist(parent);
+            xv = h.id(parent);
+            heap_push_up(h, parent, 0, d, v);
+        }
+    }
+    // pop_heap: the front goes to slot len and is dropped; slot len's entry
+    // is adjusted down from the root
+    heap_adjust(h, 0, len, xd, xv);
 }
 
 // LDS: the heap lives in LDS, slot-major (entry i of lane t at i * 64 + t,
@@ -179,18 +233,15 @@ __global__ __launch_bounds__(kThreads) void k_photon_knn(KnnArgs a)
         const rt::PhotonNode nd = a.nodes[med];
         const float d = norm3(p[0] - nd.x, p[1] - nd.y, p[2] - nd.z);
         if (d < maxd && (want == 0u || (nd.meta >> 2) == want)) {
+            // push_back; make_heap at k; beyond k push_heap + pop_heap +
+            // pop_back and max_distance = the front's (kdtree.h:123-130)
             if (cnt < a.k) {
                 h.dist(cnt) = d;
                 h.id(cnt) = med;
                 ++cnt;
-                if (cnt == a.k)
-                    for (uint32_t i = a.k / 2; i-- > 0;) sift_down(h, a.k, i);
-            } else if (d < h.dist(0)) {
-                h.dist(0) = d;
-                h.id(0) = med;
-                sift_down(h, a.k, 0);
-                maxd = h.dist(0);
+                if (cnt == a.k) heap_make(h, a.k);
             } else {
+                heap_push_pop(h, a.k, d, med);
                 maxd = h.dist(0);
             }
         }

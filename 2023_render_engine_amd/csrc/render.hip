@@ -292,8 +292,9 @@ __device__ __forceinline__ uint32_t classify(const SceneDev &sc, int32_t hid, bo
 // one node visit per step (rt_device.h), so a few long rays do not hold the
 // rest of the wavefront idle.  The analytic shapes and the material
 // classification follow in k_classify, which keeps this loop's register
-// footprint at the traversal's.
-template <bool B, bool WIDE>
+// footprint at the traversal's.  STATS: the counting variant (node visits and
+// triangle tests summed into the stats words), never the timed kernel.
+template <bool B, bool WIDE, bool STATS>
 __global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc0, BatchDev bt, PathsDev p, QueuesDev q, uint32_t cur,
                                                    int cap)
 {
@@ -307,7 +308,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc0, BatchDev bt, Pa
     cursor_init(c, blockIdx.x % kStripes, count);
     TravState s;
     bool has = false;
-    uint32_t slot = 0, dummy0 = 0, dummy1 = 0;
+    uint32_t slot = 0, n_node = 0, n_tri = 0;
     // a batch traverses its pooled trees from each path's scene root
     const float4 *nodes = (const float4 *)(B ? bt.nodes : sc0.bvh.nodes);
     const float4 *tris = (const float4 *)(B ? bt.tris : sc0.bvh.tris);
@@ -325,10 +326,17 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc0, BatchDev bt, Pa
             if (cursor_finished(c)) break;
             continue;
         }
-        if (has && trav_step_any<WIDE, false, false>(s, nodes, tris, stack, cap, dummy0, dummy1)) {
+        if (has && trav_step_any<WIDE, false, STATS>(s, nodes, tris, stack, cap, n_node, n_tri)) {
             p.hid[slot] = s.hit;
             p.ht[slot] = s.thit;
             has = false;
+        }
+    }
+    if (STATS) {  // the loop ends wavefront-uniformly
+        const uint32_t a = wave_sum(n_node), b = wave_sum(n_tri);
+        if (__lane_id() == 0) {
+            atomicAdd(&q.stats[kStatExtNodes], (unsigned long long)a);
+            atomicAdd(&q.stats[kStatExtTris], (unsigned long long)b);
         }
     }
 }
@@ -813,8 +821,9 @@ __global__ __launch_bounds__(kBlock) void k_cache_vertex(SceneDev sc, PathsDev p
 // ---------------------------------------------------------------------
 // Persistent any hit of the shadow queue against the BVH (the analytic
 // shapes were tested when the shadow ray was made, in k_shade); an occluded
-// ray clears its light bit in the path's NEE mask.
-template <bool B, bool WIDE>
+// ray clears its light bit in the path's NEE mask.  STATS: the counting
+// variant (node visits, triangle tests and occluded rays into the stats words).
+template <bool B, bool WIDE, bool STATS>
 __global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc0, BatchDev bt, PathsDev p, ShadowDev sh, QueuesDev q,
                                                     int cap)
 {
@@ -827,7 +836,7 @@ __global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc0, BatchDev bt, P
     cursor_init(c, blockIdx.x % kStripes, count);
     TravState s;
     bool has = false;
-    uint32_t sl = 0, dummy0 = 0, dummy1 = 0;
+    uint32_t sl = 0, n_node = 0, n_tri = 0, n_occ = 0;
     const float4 *nodes = (const float4 *)(B ? bt.nodes : sc0.bvh.nodes);
     const float4 *tris = (const float4 *)(B ? bt.tris : sc0.bvh.tris);
     for (;;) {
@@ -844,9 +853,18 @@ __global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc0, BatchDev bt, P
             if (cursor_finished(c)) break;
             continue;
         }
-        if (has && trav_step_any<WIDE, true, false>(s, nodes, tris, stack, cap, dummy0, dummy1)) {
+        if (has && trav_step_any<WIDE, true, STATS>(s, nodes, tris, stack, cap, n_node, n_tri)) {
             if (s.occ) atomicAnd(&p.nee_mask[sl >> 5], ~(1u << (sl & 31u)));
+            if (STATS) n_occ += s.occ ? 1u : 0u;
             has = false;
+        }
+    }
+    if (STATS) {  // the loop ends wavefront-uniformly
+        const uint32_t a = wave_sum(n_node), b = wave_sum(n_tri), o = wave_sum(n_occ);
+        if (__lane_id() == 0) {
+            atomicAdd(&q.stats[kStatShNodes], (unsigned long long)a);
+            atomicAdd(&q.stats[kStatShTris], (unsigned long long)b);
+            atomicAdd(&q.stats[kStatOccluded], (unsigned long long)o);
         }
     }
 }
@@ -1060,13 +1078,35 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
     if (iterations & 1u) return hipErrorInvalidValue;
     const uint32_t ext_depth = !lookups ? kNoCache : (wp.train ? wp.cache_depth + kTrainSuffix : wp.cache_depth);
     const uint32_t need_record = wp.train ? 1u : 0u;
+    const bool cnt = c.prof && c.prof->count;
     uint32_t cur = 0;
+    // a traversal launch, bracketed by an event pair when profiling
+    auto timed = [&](uint8_t kind, auto launch) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        const bool ev = c.prof && c.prof->events && c.prof->next(kind, e0, e1);
+        if (ev) (void)hipEventRecord(e0, s);
+        launch();
+        if (ev) (void)hipEventRecord(e1, s);
+    };
+    auto extend = [&]() {
+        auto go = [&](auto kern) { hipLaunchKernelGGL(kern, pgrid, blk, lds, s, sc, bt, paths, q, cur, cap); };
+        if (B && wide) cnt ? go(k_extend<true, true, true>) : go(k_extend<true, true, false>);
+        else if (B) cnt ? go(k_extend<true, false, true>) : go(k_extend<true, false, false>);
+        else if (wide) cnt ? go(k_extend<false, true, true>) : go(k_extend<false, true, false>);
+        else cnt ? go(k_extend<false, false, true>) : go(k_extend<false, false, false>);
+    };
+    auto connect = [&]() {
+        auto go = [&](auto kern) { hipLaunchKernelGGL(kern, pgrid, blk, lds, s, sc, bt, paths, shadow, q, cap); };
+        if (B && wide) cnt ? go(k_connect<true, true, true>) : go(k_connect<true, true, false>);
+        else if (B) cnt ? go(k_connect<true, false, true>) : go(k_connect<true, false, false>);
+        else if (wide) cnt ? go(k_connect<false, true, true>) : go(k_connect<false, true, false>);
+        else cnt ? go(k_connect<false, false, true>) : go(k_connect<false, false, false>);
+    };
     for (uint32_t it = 0; it < iterations; ++it) {
         const uint32_t nxt = cur ^ 1u;
         hipLaunchKernelGGL(k_reset, dim3(kNumLists + 1), dim3(kStripes), 0, s, q, cur, 0u);
+        timed(0, extend);
         if (B) {
-            if (wide) hipLaunchKernelGGL((k_extend<true, true>), pgrid, blk, lds, s, sc, bt, paths, q, cur, cap);
-            else hipLaunchKernelGGL((k_extend<true, false>), pgrid, blk, lds, s, sc, bt, paths, q, cur, cap);
             hipLaunchKernelGGL(k_classify<true>, grid, blk, 0, s, sc, bt, paths, q, cur, ext_depth, need_record);
             hipLaunchKernelGGL((k_shade<kQLambert, true>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQDielectric, true>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
@@ -1074,8 +1114,6 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
             hipLaunchKernelGGL((k_shade<kQLight, true>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQMiss, true>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
         } else {
-            if (wide) hipLaunchKernelGGL((k_extend<false, true>), pgrid, blk, lds, s, sc, bt, paths, q, cur, cap);
-            else hipLaunchKernelGGL((k_extend<false, false>), pgrid, blk, lds, s, sc, bt, paths, q, cur, cap);
             hipLaunchKernelGGL(k_classify<false>, grid, blk, 0, s, sc, bt, paths, q, cur, ext_depth, need_record);
             hipLaunchKernelGGL((k_shade<kQLambert, false>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQDielectric, false>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
@@ -1085,10 +1123,7 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
         }
         if (lookups) hipLaunchKernelGGL(k_cache_vertex, grid, blk, 0, s, sc, paths, q);
         hipLaunchKernelGGL(k_count, dim3(1), dim3(kStripes), 0, s, q, cur);
-        if (B && wide) hipLaunchKernelGGL((k_connect<true, true>), pgrid, blk, lds, s, sc, bt, paths, shadow, q, cap);
-        else if (B) hipLaunchKernelGGL((k_connect<true, false>), pgrid, blk, lds, s, sc, bt, paths, shadow, q, cap);
-        else if (wide) hipLaunchKernelGGL((k_connect<false, true>), pgrid, blk, lds, s, sc, bt, paths, shadow, q, cap);
-        else hipLaunchKernelGGL((k_connect<false, false>), pgrid, blk, lds, s, sc, bt, paths, shadow, q, cap);
+        timed(1, connect);
         if (lookups) {
             // cin rows are this iteration's queries (training records are
             // compacted into cin only by k_finalize, after this)

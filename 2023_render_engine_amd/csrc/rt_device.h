@@ -295,8 +295,11 @@ __device__ __forceinline__ void trav_begin(TravState &s, float ox, float oy, flo
     s.occ = false;
 }
 
-// One traversal step.  Returns true when the ray is finished.
-template <bool ANY, bool STATS, class Stack>
+// One traversal step.  Returns true when the ray is finished.  OPAQUE: the
+// closest hit skips transparent triangles, as Triangle::intersects does for
+// shadow checks (shapes/Triangle.cu:439-440) under
+// BVH::intersectRays(..., checkShadows = true) (BVH.cu:541-560).
+template <bool ANY, bool STATS, class Stack, bool OPAQUE = false>
 __device__ __forceinline__ bool trav_step(TravState &s, const float4 *__restrict__ nodes,
                                           const float4 *__restrict__ tris, const Stack &stack, int cap,
                                           uint32_t &n_node, uint32_t &n_tri)
@@ -335,7 +338,7 @@ __device__ __forceinline__ bool trav_step(TravState &s, const float4 *__restrict
                     s.occ = true;
                     return true;
                 }
-            } else if (tri_intersects(s.tr, va, vb, vc, s.h)) {
+            } else if (!(OPAQUE && (__float_as_uint(vb.w) & 1u)) && tri_intersects(s.tr, va, vb, vc, s.h)) {
                 s.hit = __float_as_int(va.w);
                 s.thit = s.h;
             }
@@ -375,7 +378,7 @@ constexpr uint32_t kLeafBatch = RT_LEAF_BATCH;        // closest hit
 constexpr uint32_t kLeafBatchAny = RT_LEAF_BATCH_ANY;  // any hit (shadow rays end at their first occluder)
 constexpr uint32_t kNodeSteps = RT_NODE_STEPS;         // interior-node visits per traversal step
 
-template <bool ANY, bool STATS, class Stack>
+template <bool ANY, bool STATS, class Stack, bool OPAQUE = false>
 __device__ __forceinline__ bool trav_step4(TravState &s, const float4 *__restrict__ nodes,
                                            const float4 *__restrict__ tris, const Stack &stack, int cap,
                                            uint32_t &n_node, uint32_t &n_tri)
@@ -474,7 +477,7 @@ __device__ __forceinline__ bool trav_step4(TravState &s, const float4 *__restric
                         s.occ = true;
                         return true;
                     }
-                } else if (tri_intersects(s.tr, va, vb, vc, s.h)) {
+                } else if (!(OPAQUE && (__float_as_uint(vb.w) & 1u)) && tri_intersects(s.tr, va, vb, vc, s.h)) {
                     s.hit = __float_as_int(va.w);
                     s.thit = s.h;
                 }
@@ -485,13 +488,13 @@ __device__ __forceinline__ bool trav_step4(TravState &s, const float4 *__restric
 }
 
 // Dispatch on the tree layout.
-template <bool WIDE, bool ANY, bool STATS, class Stack>
+template <bool WIDE, bool ANY, bool STATS, class Stack, bool OPAQUE = false>
 __device__ __forceinline__ bool trav_step_any(TravState &s, const float4 *__restrict__ nodes,
                                               const float4 *__restrict__ tris, const Stack &stack, int cap,
                                               uint32_t &n_node, uint32_t &n_tri)
 {
-    if constexpr (WIDE) return trav_step4<ANY, STATS>(s, nodes, tris, stack, cap, n_node, n_tri);
-    else return trav_step<ANY, STATS>(s, nodes, tris, stack, cap, n_node, n_tri);
+    if constexpr (WIDE) return trav_step4<ANY, STATS, Stack, OPAQUE>(s, nodes, tris, stack, cap, n_node, n_tri);
+    else return trav_step<ANY, STATS, Stack, OPAQUE>(s, nodes, tris, stack, cap, n_node, n_tri);
 }
 
 // ---------------------------------------------------------------------

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <vector>
 
 #include "rt_scene_types.h"
 #include "rt_trace.h"
@@ -122,7 +123,7 @@ struct QueuesDev {
     uint32_t *term;
     uint32_t *sctr;             // kStripedWords: striped list counters
     uint32_t *counters;         // kCounterWords: scalar counters
-    unsigned long long *stats;  // [0] extension rays, [1] shadow rays
+    unsigned long long *stats;  // kNumStats words (kStat*)
     uint32_t cs;                // path slots per stripe (capacity / kStripes)
 };
 
@@ -172,6 +173,47 @@ struct BatchLaunch {
 struct CacheDev;
 struct CacheWork;
 
+// Optional measurement of a frame's traversal kernels.  With events, every
+// k_extend / k_connect launch is bracketed by a pair of HIP events on the
+// launch stream (pairs are reused across frames; the host sums their elapsed
+// times after the frame).  With count, the kernels run their counting
+// variants, which add node visits, triangle tests and occluded shadow rays to
+// the queues' stats words 2..6 (never the timed kernels).
+struct KernelProfile {
+    bool events = false;
+    bool count = false;
+    std::vector<hipEvent_t> ev;    // pairs: before, after
+    std::vector<uint8_t> kind;     // per pair: 0 extend, 1 connect
+    size_t used = 0;               // pairs recorded this frame
+    // the next pair for a launch of `k` (created on first use); false if
+    // an event could not be created
+    bool next(uint8_t k, hipEvent_t &a, hipEvent_t &b)
+    {
+        if (used == kind.size()) {
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (hipEventCreate(&e0) != hipSuccess) return false;
+            if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return false; }
+            ev.push_back(e0);
+            ev.push_back(e1);
+            kind.push_back(k);
+        }
+        kind[used] = k;
+        a = ev[2 * used];
+        b = ev[2 * used + 1];
+        ++used;
+        return true;
+    }
+    ~KernelProfile()
+    {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+};
+// Stats words of QueuesDev::stats: rays, then the counting variants' sums.
+enum : uint32_t {
+    kStatExt = 0, kStatShadow = 1, kStatExtNodes = 2, kStatExtTris = 3, kStatShNodes = 4, kStatShTris = 5,
+    kStatOccluded = 6, kNumStats = 8
+};
+
 // Streaming renderer.  A chunk of wp.count paths (global indices
 // wp.first_path ...) runs through `capacity` path slots: every iteration
 // extends, shades and connects the live paths, writes the radiance of the
@@ -193,6 +235,7 @@ struct ChunkLaunch {
     const CacheDev *cache;      // rendering with cache lookups (single scene)
     const CacheWork *cwork;
     const BatchLaunch *batch;
+    KernelProfile *prof;        // optional (never under graph capture)
 };
 // chunk start: clears every counter and fills all slots
 hipError_t chunk_begin(const ChunkLaunch &c, const WaveParams &wp, hipStream_t s);

@@ -33,10 +33,11 @@ struct DevBvh {
 // triangle tests, accumulated).  ray_counter: kClaimWords device words
 // (kStripes claim counters, kCtrStride words apart), cleared by clear_claims
 // before the launch, that the persistent kernel claims rays from; two
-// launches in flight at once need different counters.
+// launches in flight at once need different counters.  opaque (closest hit,
+// no counters): transparent triangles are skipped.
 hipError_t launch_trace(const DevBvh &bvh, const RaysDev &rays, uint32_t n, int32_t *prim, float *t,
                         uint8_t *occluded, bool any, hipStream_t stream, unsigned long long *counters,
-                        uint32_t *ray_counter);
+                        uint32_t *ray_counter, bool opaque = false);
 size_t trace_lds_bytes(uint32_t stack_cap);
 // zeroes the kStripes claim counters of ray_counter (a kernel, so it can be
 // captured into a graph)

@@ -28,6 +28,11 @@ rt_status ensure_wave_buffers(WaveBuffers &w, uint32_t P, uint32_t NL, const cha
 // Grows the chunk result buffer to R paths.
 rt_status ensure_result_buffer(WaveBuffers &w, uint64_t R, const char *who);
 void free_wave_buffers(WaveBuffers &w);
+// rt_render_stats of a finished frame from its stats words (kNumStats) and,
+// under RT_FLAG_PROFILE, the elapsed times of prof's event pairs; resets
+// prof for the next frame.
+rt_status fill_stats(rt_render_stats &st, const unsigned long long *counts, KernelProfile &prof, uint32_t flags,
+                     const char *who);
 
 // Paths per chunk: the result buffer holds 12 bytes per path (768 MiB).
 // Each chunk ends with a drain, but larger chunks measured slower on the

@@ -223,6 +223,7 @@ struct rt_scene {
     // wavefront buffers (grown on demand)
     rt::WaveBuffers wb;
     rt_render_stats stats{};
+    rt::KernelProfile prof;  // event pairs of profiled frames (RT_FLAG_PROFILE)
     uint64_t stream_base = 0;
     // radiance-cache inference scratch (one row per path slot), on demand
     void *cache_arena = nullptr;
@@ -346,7 +347,7 @@ rt_status ensure_wave_buffers(WaveBuffers &w, uint32_t P, uint32_t NL, const cha
     add((void **)&q.term, (size_t)P * 4);
     add((void **)&q.sctr, kStripedWords * 4);
     add((void **)&q.counters, kCounterWords * 4);
-    add((void **)&q.stats, 2 * 8);
+    add((void **)&q.stats, kNumStats * 8);
     char *base = nullptr;
     hipError_t e = hipMalloc(&base, total);
     if (e != hipSuccess) {
@@ -359,6 +360,36 @@ rt_status ensure_wave_buffers(WaveBuffers &w, uint32_t P, uint32_t NL, const cha
     w.arena = base;
     w.capacity = P;
     w.cap_lights = NL;
+    return RT_OK;
+}
+
+// rt_render_stats of a frame from its stats words and, when profiled, the
+// elapsed times of its event pairs (the stream has been synchronised).
+rt_status fill_stats(rt_render_stats &st, const unsigned long long *counts, KernelProfile &prof, uint32_t flags,
+                     const char *who)
+{
+    st.extension_rays = counts[kStatExt];
+    st.shadow_rays = counts[kStatShadow];
+    if (flags & RT_FLAG_COUNT) {
+        st.ext_node_visits = counts[kStatExtNodes];
+        st.ext_tri_tests = counts[kStatExtTris];
+        st.shadow_node_visits = counts[kStatShNodes];
+        st.shadow_tri_tests = counts[kStatShTris];
+        st.shadow_occluded = counts[kStatOccluded];
+    }
+    if (flags & RT_FLAG_PROFILE) {
+        for (size_t i = 0; i < prof.used; ++i) {
+            float ms = 0;
+            hipError_t e = hipEventElapsedTime(&ms, prof.ev[2 * i], prof.ev[2 * i + 1]);
+            if (e != hipSuccess) {
+                set_error(std::string(who) + ": kernel timing: " + hipGetErrorString(e));
+                return RT_ERR_HIP;
+            }
+            if (prof.kind[i] == 0) { st.extend_ms += ms; ++st.extend_launches; }
+            else { st.connect_ms += ms; ++st.connect_launches; }
+        }
+    }
+    prof.used = 0;
     return RT_OK;
 }
 
@@ -697,6 +728,7 @@ static rt_status render_frames(const char *fn, rt_scene *s, const rt_render_opts
     if (!s || !opts || !accum || !nvalid) { set_error(name + ": null argument"); return RT_ERR_INVALID; }
     if (!s->committed) { set_error(name + ": scene not committed"); return RT_ERR_INVALID; }
     if (flags & RT_FLAG_HOST_PTRS) { set_error(name + ": film buffers must be device pointers"); return RT_ERR_UNSUPPORTED; }
+    if (flags & ~(uint32_t)(RT_FLAG_PROFILE | RT_FLAG_COUNT)) { set_error(name + ": unknown flag"); return RT_ERR_INVALID; }
     if (opts->max_depth == 0 || opts->max_depth > 0xffffu) { set_error(name + ": max_depth must be 1..65535"); return RT_ERR_INVALID; }
     if (cache && rt::cache_ctx(cache) != s->ctx) { set_error(name + ": cache and scene belong to different contexts"); return RT_ERR_INVALID; }
     if (cache && cache_depth >= 0xffffu) { set_error(name + ": cache_depth must be below 65535"); return RT_ERR_INVALID; }
@@ -718,7 +750,10 @@ static rt_status render_frames(const char *fn, rt_scene *s, const rt_render_opts
         if (st != RT_OK) return st;
     }
     auto t0 = std::chrono::steady_clock::now();
-    RT_HIP(hipMemsetAsync(s->wb.queues.stats, 0, 16, stream));
+    RT_HIP(hipMemsetAsync(s->wb.queues.stats, 0, rt::kNumStats * 8, stream));
+    s->prof.events = (flags & RT_FLAG_PROFILE) != 0;
+    s->prof.count = (flags & RT_FLAG_COUNT) != 0;
+    s->prof.used = 0;
     rt::FilmDev film{accum, nvalid};
     rt::ChunkLaunch cl{};
     cl.scene = &s->dev;
@@ -730,6 +765,7 @@ static rt_status render_frames(const char *fn, rt_scene *s, const rt_render_opts
     cl.cache = cache ? &rt::cache_device(cache) : nullptr;
     cl.cwork = cache ? &s->cwork : nullptr;
     cl.batch = nullptr;
+    cl.prof = (flags & (RT_FLAG_PROFILE | RT_FLAG_COUNT)) ? &s->prof : nullptr;
     for (uint64_t first = 0; first < total; first += R) {
         rt::WaveParams wp{};
         wp.first_path = first;
@@ -745,14 +781,13 @@ static rt_status render_frames(const char *fn, rt_scene *s, const rt_render_opts
         wp.grid = opts->grid;
         RT_HIP(rt::render_chunk(cl, film, wp, s->wb.host_counters, stream));
     }
-    unsigned long long counts[2];
+    unsigned long long counts[rt::kNumStats];
     RT_HIP(hipMemcpyAsync(counts, s->wb.queues.stats, sizeof counts, hipMemcpyDeviceToHost, stream));
     RT_HIP(hipStreamSynchronize(stream));
+    s->stats = rt_render_stats{};
     s->stats.paths = total;
-    s->stats.extension_rays = counts[0];
-    s->stats.shadow_rays = counts[1];
     s->stats.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    return RT_OK;
+    return rt::fill_stats(s->stats, counts, s->prof, flags, name.c_str());
 }
 
 rt_status rt_render(rt_scene *s, const rt_render_opts *opts, double *accum, uint32_t *nvalid, uint32_t flags,
@@ -785,7 +820,7 @@ rt_status rt_cache_train_scene(rt_cache *c, rt_scene *s, const rt_render_opts *o
     rt_status st = rt::ensure_buffers(s, P, "rt_cache_train_scene");
     if (st == RT_OK) st = rt::ensure_cache_work(s, s->wb.capacity);
     if (st != RT_OK) return st;
-    RT_HIP(hipMemsetAsync(s->wb.queues.stats, 0, 16, stream));
+    RT_HIP(hipMemsetAsync(s->wb.queues.stats, 0, rt::kNumStats * 8, stream));
     rt::WaveParams wp{};
     wp.first_path = (uint64_t)opts->spp_offset * n_paths;
     wp.count = n_paths;

@@ -32,8 +32,10 @@ __device__ __forceinline__ void trav_init(TravState &s, const RaysDev &rays, uin
 }
 
 // STATS = true counts node visits and triangle tests (for pricing the
-// algorithmic bytes of a launch); it is never the timed kernel.
-template <bool WIDE, bool ANY, bool STATS>
+// algorithmic bytes of a launch); it is never the timed kernel.  OPAQUE
+// (closest hit only) skips transparent triangles: the closest opaque hit of
+// BVH::intersectRays(..., checkShadows = true).
+template <bool WIDE, bool ANY, bool STATS, bool OPAQUE = false>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(const float4 *__restrict__ nodes,
                                                        const float4 *__restrict__ tris,
                                                        RaysDev rays, uint32_t n,
@@ -77,7 +79,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(const float4 *__restrict_
             if (cursor_finished(cur)) break;
             continue;
         }
-        if (has && trav_step_any<WIDE, ANY, STATS>(s, nodes, tris, stack, cap, n_node, n_tri)) {
+        if (has && trav_step_any<WIDE, ANY, STATS, TraceStack, OPAQUE>(s, nodes, tris, stack, cap, n_node, n_tri)) {
             if (ANY) {
                 occluded[ray] = s.occ ? 1 : 0;
             } else {
@@ -97,7 +99,7 @@ static int g_cus = 0;
 
 hipError_t launch_trace(const DevBvh &bvh, const RaysDev &rays, uint32_t n, int32_t *prim, float *t,
                         uint8_t *occluded, bool any, hipStream_t stream, unsigned long long *counters,
-                        uint32_t *ray_counter)
+                        uint32_t *ray_counter, bool opaque)
 {
     if (n == 0) return hipSuccess;
     if (g_cus == 0) {
@@ -126,6 +128,10 @@ hipError_t launch_trace(const DevBvh &bvh, const RaysDev &rays, uint32_t n, int3
                            lc, ray_counter, counters);
     };
     const bool wide = bvh.wide != 0, stats = counters != nullptr;
+    if (opaque && !any && !stats) {
+        wide ? go(k_trace<true, false, false, true>) : go(k_trace<false, false, false, true>);
+        return hipGetLastError();
+    }
     if (wide) {
         if (stats) any ? go(k_trace<true, true, true>) : go(k_trace<true, false, true>);
         else any ? go(k_trace<true, true, false>) : go(k_trace<true, false, false>);

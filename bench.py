@@ -190,6 +190,51 @@ def run_c1(args):
     return out
 
 
+# Algorithmic bytes of the renderer's traversal kernels (per ray, before the
+# tree): k_extend reads a queue slot (4 B) and the ray (origin + direction,
+# 24 B) and writes the hit (id + t, 8 B); k_connect reads the shadow ray
+# (origin, direction, tMax: 28 B) and its path slot (4 B), and an occluded
+# ray clears its light bit (a 4 B read-modify-write).  A batch path also
+# reads its scene's root (4 B).  Every node visit reads 64 B and every
+# triangle test 48 B, counted by the kernels' counting variants.
+def traversal_roofline(count_stats, frames, ext_ms, con_ms, batch=False, note=""):
+    """The roofline object of a path-tracing line: the dominant traversal
+    kernel's algorithmic bytes per frame (from a counting frame over the same
+    samples) over its HIP-event time per frame (timed frames)."""
+    c = count_stats
+    root = 4 if batch else 0
+    ext_b = c.extension_rays * (36 + root) + 64 * c.ext_node_visits + 48 * c.ext_tri_tests
+    con_b = (c.shadow_rays * (32 + root) + 8 * c.shadow_occluded + 64 * c.shadow_node_visits
+             + 48 * c.shadow_tri_tests)
+    ext_s, con_s = ext_ms / frames / 1e3, con_ms / frames / 1e3
+    if con_s >= ext_s:
+        name, b, t = "k_connect (any hit, shadow rays)", con_b, con_s
+    else:
+        name, b, t = "k_extend (closest hit, extension rays)", ext_b, ext_s
+    achieved = b / t / 1e9
+    return {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "kernel": name,
+        "bytes_per_frame": int(b), "kernel_ms_per_frame": round(t * 1e3, 3),
+        "per_ray": {
+            "extension": {"node_visits": round(c.ext_node_visits / max(1, c.extension_rays), 3),
+                          "tri_tests": round(c.ext_tri_tests / max(1, c.extension_rays), 3),
+                          "kernel_ms_per_frame": round(ext_s * 1e3, 3),
+                          "achieved_gbs": round(ext_b / ext_s / 1e9, 1) if ext_s > 0 else None},
+            "shadow": {"node_visits": round(c.shadow_node_visits / max(1, c.shadow_rays), 3),
+                       "tri_tests": round(c.shadow_tri_tests / max(1, c.shadow_rays), 3),
+                       "occluded": round(c.shadow_occluded / max(1, c.shadow_rays), 4),
+                       "kernel_ms_per_frame": round(con_s * 1e3, 3),
+                       "achieved_gbs": round(con_b / con_s / 1e9, 1) if con_s > 0 else None},
+        },
+        "note": ("achieved = the kernel's algorithmic bytes per frame (36 B per extension ray or 32 B per shadow "
+                 "ray + 8 B per occluded one, 64 B per node visit, 48 B per triangle test; visits and tests from "
+                 "a counting frame over the same samples) / its summed HIP-event time per frame over the timed "
+                 "frames (events on the launch stream around every launch)" + note),
+    }
+
+
 def run_render(args, which):
     """Path-tracing workloads: c0 = Cornell box 256^2 16 spp depth 8;
     c2 = procedural 3.4M-triangle scene 1920x1080 64 spp depth 16.  One step
@@ -214,14 +259,22 @@ def run_render(args, which):
     film = sc.new_film()
     for _ in range(args.warmup):
         sc.render(film)
+    # counting frame over the same samples as the timed frames (untimed)
+    cfilm = sc.new_film()
+    sc.render(cfilm, count=True)
+    cstats = sc.stats()
+    del cfilm
     torch.cuda.synchronize()
     ext = shadow = 0
+    ext_ms = con_ms = 0.0
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        sc.render(film)
+        sc.render(film, profile=True)
         st = sc.stats()
         ext += st.extension_rays
         shadow += st.shadow_rays
+        ext_ms += st.extend_ms
+        con_ms += st.connect_ms
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     samples = desc.width * desc.height * desc.spp * args.steps
@@ -249,6 +302,7 @@ def run_render(args, which):
         "mrays_shadow": round(shadow / wall / 1e6, 1),
         "rays_per_sample": round((ext + shadow) / samples, 2),
         "rmse_vs_ref": rmse,
+        "roofline": traversal_roofline(cstats, args.steps, ext_ms, con_ms),
     }
     if not args.no_cpu_baseline:
         import oracle
@@ -344,7 +398,7 @@ def run_c4(args):
                 l, ns = sc.train_cache(cache, frame, n_paths=n_paths, cache_depth=args.c4_cache_depth,
                                        want_loss=want)
                 if want:
-                    losses.append(l)
+                    losses.append((frame, l))
                     samples.append(ns)
                 if frame == 0:
                     # paths per step for 65,536 samples at the measured yield (+3 %)
@@ -369,7 +423,9 @@ def run_c4(args):
                "final_rmse": curve[-1][1] if curve else None, "timed_s": round(spent, 3),
                "curve": curve[:: max(1, len(curve) // 12)] + curve[-1:]}
         if losses:
-            res["loss_first_last"] = [round(float(np.mean(losses[:5])), 4), round(float(np.mean(losses[-5:])), 4)]
+            # single readings (frame, relative L2 loss before that frame's step), first and last apart
+            res["loss_readings"] = [(f, round(float(v), 4)) for f, v in losses]
+            res["loss_first_last"] = [round(float(losses[0][1]), 4), round(float(losses[-1][1]), 4)]
             res["training_paths_per_step"] = n_paths
             res["samples_per_step"] = int(np.mean(samples[1:])) if len(samples) > 1 else int(samples[0])
             res["samples_first_step"] = int(samples[0])
@@ -444,6 +500,17 @@ def run_c3(args):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     samples = npix * 32 * args.steps
+    # traversal kernel times: one eager frame with event pairs (graph replays
+    # cannot hold them) and a counting frame, both over the first timed
+    # frame's samples (untimed)
+    pfilms = [s.new_film() for s in scs]
+    batch.render(pfilms, spp=32, spp_offset=0, max_depth=12, seeds=seeds, profile=True)
+    pst = batch.stats()
+    batch.render(pfilms, spp=32, spp_offset=0, max_depth=12, seeds=seeds, count=True)
+    cst = batch.stats()
+    del pfilms
+    torch.cuda.synchronize()
+    eager_ms = pst.ms
     # the same frame rendered scene by scene
     seq = [s.new_film() for s in scs]
     torch.cuda.synchronize()
@@ -477,6 +544,10 @@ def run_c3(args):
         "mrays_extension": round(ext / wall / 1e6, 1),
         "sequential_msamples_s": round(npix * 32 / seq_s / 1e6, 3),
         "batch_speedup_vs_sequential": round(seq_s / (wall / args.steps), 3),
+        "roofline": traversal_roofline(cst, 1, pst.extend_ms, pst.connect_ms, batch=True,
+                                       note=f"; kernel times from one eager frame after the timed graph frames "
+                                            f"(frame {eager_ms:.1f} ms eager vs {wall * 1e3 / args.steps:.1f} ms "
+                                            f"replayed)"),
     }
     if rmse:
         out["rmse_vs_ref"] = {"ref_spp": args.c3_ref_spp, "frames": args.steps, "mean": round(float(np.mean(rmse)), 5),

@@ -36,7 +36,13 @@ typedef enum {
     RT_ERR_NO_DEVICE = 5    /* no HIP device visible */
 } rt_status;
 
-enum { RT_FLAG_HOST_PTRS = 1u, RT_FLAG_GRAPH = 2u };
+enum {
+    RT_FLAG_HOST_PTRS = 1u,  /* ray / hit buffers are host memory; the call blocks */
+    RT_FLAG_GRAPH = 2u,      /* rt_batch_render: capture the frame into a hipGraph and replay it */
+    RT_FLAG_SHADOW = 4u,     /* rt_trace_closest: closest OPAQUE hit (transparent triangles skipped) */
+    RT_FLAG_PROFILE = 8u,    /* rt_render*, rt_batch_render: time every traversal launch with HIP events */
+    RT_FLAG_COUNT = 16u      /* rt_render*, rt_batch_render: counting traversal kernels (node visits, ...) */
+};
 
 /* Thread-local message of the last failing call on this thread. */
 const char *rt_last_error(void);
@@ -132,7 +138,16 @@ typedef struct {
  * and its per-ray BVH::intersectRay (BVH.cu:355-448) with the
  * Triangle::intersects arithmetic of shapes/Triangle.cu:433-643 done
  * bit-for-bit in FP32.  prim[i] = triangle id or -1; t[i] = hit distance
- * (+inf on a miss); t may be NULL. */
+ * (+inf on a miss); t may be NULL.
+ *
+ * flags: RT_FLAG_HOST_PTRS (host buffers, blocking) and RT_FLAG_SHADOW.
+ * RT_FLAG_SHADOW is BVH::intersectRays(rays, ids, checkShadows = true): the
+ * closest hit among the triangles that are not transparent
+ * (rt_bvh_set_transparent), as Triangle::intersects returns false for a
+ * transparent triangle under checkShadow (Triangle.cu:439-440).  Its caller
+ * Scene::intersects re-tests triangles[id] for t (Scene.cu:108-150) and
+ * Scene::isVisible compares that t with the light distance (Scene.cu:238-276);
+ * t[i] here is that same distance.  Any other flag bit is RT_ERR_INVALID. */
 rt_status rt_trace_closest(rt_bvh *bvh, const rt_rays *rays, uint32_t n, int32_t *prim,
                            float *t, uint32_t flags, void *stream);
 
@@ -265,10 +280,21 @@ rt_status rt_render(rt_scene *scene, const rt_render_opts *opts, double *accum, 
 rt_status rt_scene_camera_ray(const rt_scene *scene, uint32_t px, uint32_t py, float u0, float u1,
                               float o[3], float d[3]);
 
-/* Statistics of the last rt_render call. */
+/* Statistics of the last rt_render call.  The rest of the fields are filled
+ * by frames rendered with the measurement flags:
+ *   RT_FLAG_PROFILE: every launch of the closest-hit (extension rays) and
+ *     any-hit (shadow rays) traversal kernels is bracketed by HIP events on
+ *     its stream; *_ms is their summed elapsed time over the frame.
+ *   RT_FLAG_COUNT: those kernels run their counting variants (slower, never
+ *     the timed ones) and sum node visits (64 B each), triangle tests (48 B
+ *     each) and shadow rays found occluded.  Images are unchanged. */
 typedef struct {
     uint64_t paths, extension_rays, shadow_rays;
     double ms;             /* wall time of the call (device-synchronised) */
+    double extend_ms, connect_ms;
+    uint32_t extend_launches, connect_launches;
+    uint64_t ext_node_visits, ext_tri_tests;
+    uint64_t shadow_node_visits, shadow_tri_tests, shadow_occluded;
 } rt_render_stats;
 rt_status rt_render_get_stats(const rt_scene *scene, rt_render_stats *stats);
 
@@ -340,11 +366,13 @@ rt_status rt_photon_map_get_info(const rt_photon_map *map, uint32_t *n, uint32_t
  * (device buffers).  queries: 3*nq floats; query_shape: nq ids (-1: any
  * shape) or null.  Outputs per query: count[q] <= k photons, their indices
  * index[q*k ..] and float distances distance[q*k ..] in the reference's
- * result order (insertion order below k, max-heap order once k were found).
- * The set is the k nearest photons strictly within max_distance (on the
- * query's shape); photons at a distance equal to the k-th may be exchanged
- * for one another.  k: 1..1024, nq*k <= 2^31.  Enqueued on stream (null:
- * the context's stream). */
+ * result order: insertion order below k, then the layout that libstdc++'s
+ * std::make_heap / push_heap / pop_heap leave (kdtree.h:123-130), ties
+ * included.  The set is the k nearest photons strictly within max_distance
+ * (on the query's shape).  k: 1..1024, nq*k <= 2^31.  Enqueued on stream
+ * (null: the context's stream).  The map keeps one work space (query sort
+ * keys, permutation) that every call on it uses: issue the calls on one map
+ * from one stream, or order them with events. */
 rt_status rt_photon_map_knn(rt_photon_map *map, const float *queries, const int32_t *query_shape, uint32_t nq,
                             uint32_t k, float max_distance, uint32_t *count, uint32_t *index, float *distance,
                             void *stream);
@@ -358,7 +386,10 @@ rt_status rt_photon_map_knn(rt_photon_map *map, const float *queries, const int3
  * with d_max the farthest photon found (0 photons give 0).  photon_rgb
  * (Photon::radiance, n*3 doubles) and photon_dir (Photon::ray.direction, n*3
  * floats) are indexed like the map's photons; points, normals (the shading
- * normals) and kd are nq*3 floats; out is nq*3 doubles; all device memory. */
+ * normals) and kd are nq*3 floats; out is nq*3 doubles; all device memory.
+ * Photons are summed in the search's result order, as the reference sums its
+ * vector.  Shares the map's work space and result buffers with
+ * rt_photon_map_knn: one stream per map. */
 rt_status rt_photon_gather_lambert(rt_photon_map *map, const double *photon_rgb, const float *photon_dir,
                                    const float *points, const float *normals, const float *kd,
                                    const int32_t *query_shape, uint32_t nq, uint32_t k, float max_distance,

@@ -85,7 +85,114 @@ _SIGS = {
     "orc_kd_build": (_U32, [_P, _U32, _P, _P]),
     "orc_knn_batch": (None, [_P, _P, _P, _P, _U32, _P, _P, _U32, _U32, _F, C.c_int, _P, _P, _P, C.c_int]),
     "orc_photon_gather_lambert": (None, [_P, _P, _U32, _P, _P, _P, _P, _P, _P, _U32, _U32, _F, _P]),
+    "orc_heap_sequence": (None, [_P, _U32, _U32, _P, _P, _P]),
+    "orc_concentric_disk": (None, [_F, _F, _P]),
+    "orc_ggx_sample_half": (None, [_F, _P, _F, _F, _P]),
+    "orc_ggx_D_n": (None, [_F, _P, _U32, _P]),
+    "orc_ggx_lambda_n": (None, [_F, _P, _U32, _P]),
+    "orc_fresnel_conductor": (_F, [_F, _F, _F]),
+    "orc_bsdf_draws": (C.c_int, [_P]),
+    "orc_bsdf_eval_n": (None, [_P, _P, _P, _U32, _P, _P]),
+    "orc_bsdf_sample_n": (None, [_P, _P, _P, _U32, _P, _P, _P, _P]),
+    "orc_dielectric_refractions": (_U32, [_U32, _U32]),
+    "orc_luminance": (_F, [_P]),
+    "orc_russian_roulette": (C.c_int, [_P, _F, _P]),
+    "orc_sphere_light_sample": (C.c_int, [_P, _F, _P, _F, _F, _P, C.POINTER(_F), C.POINTER(_F)]),
+    "orc_sphere_light_pdf": (_F, [_P, _F, _P]),
 }
+
+
+# ---- the shading functions one at a time (shading frame, z = normal) ----
+
+def material(type, kd=(1, 1, 1), ior=1.5, roughness=0.0, eta=(1, 1, 1), k=(0, 0, 0), intensity=0.0):
+    """An orc_material (same fields as the library's rt_material)."""
+    return OrcMaterial(type, _f3(kd), ior, roughness, _f3(eta), _f3(k), intensity, _f3((0, 0, 0)), 0)
+
+
+def concentric_disk(u0, u1):
+    out = np.zeros(2, np.float32)
+    lib().orc_concentric_disk(u0, u1, _p(out))
+    return out
+
+
+def ggx_sample_half(alpha, wo, u0, u1):
+    w = f32(wo)
+    h = np.zeros(3, np.float32)
+    lib().orc_ggx_sample_half(alpha, _p(w), u0, u1, _p(h))
+    return h
+
+
+def ggx_D(alpha, h):
+    h = f32(h).reshape(-1, 3)
+    out = np.zeros(h.shape[0], np.float32)
+    lib().orc_ggx_D_n(alpha, _p(h), h.shape[0], _p(out))
+    return out
+
+
+def ggx_lambda(alpha, w):
+    w = f32(w).reshape(-1, 3)
+    out = np.zeros(w.shape[0], np.float32)
+    lib().orc_ggx_lambda_n(alpha, _p(w), w.shape[0], _p(out))
+    return out
+
+
+def fresnel_conductor(cosi, eta, k):
+    return lib().orc_fresnel_conductor(cosi, eta, k)
+
+
+def bsdf_draws(m):
+    return lib().orc_bsdf_draws(C.byref(m))
+
+
+def bsdf_eval(m, wo, wi):
+    """(f * |cos wi| (n, 3), pdf (n,)) for direction pairs (n, 3)."""
+    wi = f32(wi).reshape(-1, 3)
+    wo = f32(np.broadcast_to(f32(wo).reshape(-1, 3), wi.shape))
+    n = wi.shape[0]
+    f = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    lib().orc_bsdf_eval_n(C.byref(m), _p(wo), _p(wi), n, _p(f), _p(pdf))
+    return f, pdf
+
+
+def bsdf_sample(m, wo, u):
+    """sampleDirection with explicit draws u (n, 2): (wi, pdf, delta, weight)."""
+    u = f32(u).reshape(-1, 2)
+    n = u.shape[0]
+    wo = f32(np.broadcast_to(f32(wo).reshape(-1, 3), (n, 3)))
+    wi = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    delta = np.zeros(n, np.int32)
+    wgt = np.zeros((n, 3), np.float32)
+    lib().orc_bsdf_sample_n(C.byref(m), _p(wo), _p(u), n, _p(wi), _p(pdf), _p(delta), _p(wgt))
+    return wi, pdf, delta, wgt
+
+
+def luminance(rgb):
+    c = f32(rgb)
+    return lib().orc_luminance(_p(c))
+
+
+def russian_roulette(beta, u):
+    """(survives, beta after the step)."""
+    b = f32(beta)
+    out = np.zeros(3, np.float32)
+    alive = lib().orc_russian_roulette(_p(b), u, _p(out))
+    return bool(alive), out
+
+
+def sphere_light_sample(center, radius, p, u0, u1):
+    """(ok, wi, distance to the near surface point, pdf per solid angle)."""
+    c, q = f32(center), f32(p)
+    wi = np.zeros(3, np.float32)
+    dist, pdf = _F(), _F()
+    ok = lib().orc_sphere_light_sample(_p(c), radius, _p(q), u0, u1, _p(wi), C.byref(dist), C.byref(pdf))
+    return bool(ok), wi, dist.value, pdf.value
+
+
+def sphere_light_pdf(center, radius, p):
+    c, q = f32(center), f32(p)
+    return lib().orc_sphere_light_pdf(_p(c), radius, _p(q))
 
 
 def photon_gather_lambert(xyz, rgb, dirs, points, normals, kd, k, radius, shape=None, query_shape=None):
@@ -102,6 +209,17 @@ def photon_gather_lambert(xyz, rgb, dirs, points, normals, kd, k, radius, shape=
     lib().orc_photon_gather_lambert(_p(x), _p(sh), x.shape[0], _p(c), _p(d), _p(q), _p(nr), _p(kd_), _p(qs),
                                     q.shape[0], k, float(radius), _p(out))
     return out
+
+
+def heap_sequence(dist, k):
+    """nearest_neighbors_impl's vector after accepting candidates 0..n-1 with
+    the given distances: (ids, distances) in vector order."""
+    d = f32(dist)
+    ids = np.zeros(max(1, min(d.size, k)), np.uint32)
+    out = np.zeros(max(1, min(d.size, k)), np.float32)
+    cnt = C.c_uint32()
+    lib().orc_heap_sequence(_p(d), d.size, k, _p(ids), _p(out), C.byref(cnt))
+    return ids[:cnt.value], out[:cnt.value]
 
 
 def kd_build(xyz):

@@ -30,9 +30,17 @@ extern "C" {
 uint32_t orc_kd_build(const float *xyz, uint32_t n, uint32_t *order, uint8_t *axis);
 
 /* kdtree.h nearest_neighbors_impl over a tree from orc_kd_build; writes up to
- * k input indices and distances, returns the count. */
+ * k input indices and distances in the reference's result order (insertion
+ * order below k, then the layout of libstdc++'s make_heap / push_heap /
+ * pop_heap), returns the count. */
 uint32_t orc_kd_knn(const float *xyz, const int32_t *shape, const uint32_t *order, const uint8_t *axis, uint32_t n,
                     const float q[3], int32_t qshape, uint32_t k, float radius, uint32_t *idx, float *dist);
+
+/* The bookkeeping of nearest_neighbors_impl for a sequence of n accepted
+ * candidates (ids 0..n-1, distances dist): push_back below k, make_heap at
+ * k, then push_heap + pop_heap + pop_back.  Writes the final ids and
+ * distances in vector order and their count (min(n, k)). */
+void orc_heap_sequence(const float *dist, uint32_t n, uint32_t k, uint32_t *ids, float *out_dist, uint32_t *count);
 
 /* Brute force: the k qualifying photons of smallest (distance, index),
  * sorted that way. */

@@ -104,6 +104,31 @@ void orc_render_const_cache(const orc_scene *s, uint32_t spp, uint32_t seed, uin
 
 float orc_compute_rmse(const double *sum_a, double n_a, const double *sum_b, double n_b, uint32_t npix);
 
+/* The shading functions one at a time (shading frame, z = normal), for the
+ * tests that pin them:
+ *   concentric disk warp (Sampler.cu:13-36); GGX D, Lambda and the visible-
+ *   normal half-vector sampler with its uniformSampleDisk(2u - 1) mapping
+ *   (TrowbridgeBSDF.cu:71-151); fresnelConductor (geometricMath.cu:242-271);
+ *   every material's f * |cos wi|, pdf and sampleDirection with explicit
+ *   draws; the dielectric branch over a PCG32 stream; Spectrum::brightness;
+ *   Russian roulette (pathTracing.cu:91-102); the sphere light's cone
+ *   sampling and pdf (the library's own light, no reference counterpart). */
+void orc_concentric_disk(float u0, float u1, float out[2]);
+void orc_ggx_sample_half(float alpha, const float wo[3], float u0, float u1, float h[3]);
+void orc_ggx_D_n(float alpha, const float *h, uint32_t n, float *out);
+void orc_ggx_lambda_n(float alpha, const float *w, uint32_t n, float *out);
+float orc_fresnel_conductor(float cosi, float eta, float k);
+int orc_bsdf_draws(const orc_material *m);
+void orc_bsdf_eval_n(const orc_material *m, const float *wo, const float *wi, uint32_t n, float *f, float *pdf);
+void orc_bsdf_sample_n(const orc_material *m, const float *wo, const float *u, uint32_t n, float *wi, float *pdf,
+                       int32_t *delta, float *weight);
+uint32_t orc_dielectric_refractions(uint32_t seed, uint32_t n);
+float orc_luminance(const float rgb[3]);
+int orc_russian_roulette(const float beta[3], float u, float out[3]);
+int orc_sphere_light_sample(const float center[3], float radius, const float p[3], float u0, float u1, float wi[3],
+                            float *dist, float *pdf);
+float orc_sphere_light_pdf(const float center[3], float radius, const float p[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,17 +85,91 @@ typedef struct {
     float *dist;
 } Search;
 
-static void sift_down(float *d, uint32_t *v, uint32_t n, uint32_t i)
+/* The max-heap steps of libstdc++ that kdtree.h's std::make_heap, push_heap
+ * and pop_heap run (bits/stl_heap.h), with comp(a, b) = dist(a) < dist(b):
+ * the result order of a search is this heap's layout, so they are restated
+ * step for step rather than by an equivalent sift-down. */
+static void heap_push_up(float *d, uint32_t *v, size_t hole, size_t top, float dv, uint32_t vv)
 {
-    for (;;) {
-        uint32_t c = 2 * i + 1;
-        if (c >= n) return;
-        if (c + 1 < n && d[c + 1] > d[c]) ++c;
-        if (!(d[c] > d[i])) return;
-        const float td = d[i]; d[i] = d[c]; d[c] = td;
-        const uint32_t tv = v[i]; v[i] = v[c]; v[c] = tv;
-        i = c;
+    /* __push_heap: move the hole up while the parent compares less */
+    while (hole > top) {
+        const size_t parent = (hole - 1) / 2;
+        if (!(d[parent] < dv)) break;
+        d[hole] = d[parent];
+        v[hole] = v[parent];
+        hole = parent;
     }
+    d[hole] = dv;
+    v[hole] = vv;
+}
+
+static void heap_adjust(float *d, uint32_t *v, size_t hole, size_t len, float dv, uint32_t vv)
+{
+    /* __adjust_heap: the hole goes down to a leaf along the larger children
+     * (the right one unless the left compares greater), then the value is
+     * pushed up from there */
+    const size_t top = hole;
+    size_t child = hole;
+    while (len > 0 && child < (len - 1) / 2) {
+        child = 2 * (child + 1);
+        if (d[child] < d[child - 1]) --child;
+        d[hole] = d[child];
+        v[hole] = v[child];
+        hole = child;
+    }
+    if ((len & 1) == 0 && len >= 2 && child == (len - 2) / 2) {
+        child = 2 * (child + 1);
+        d[hole] = d[child - 1];
+        v[hole] = v[child - 1];
+        hole = child - 1;
+    }
+    heap_push_up(d, v, hole, top, dv, vv);
+}
+
+static void heap_make(float *d, uint32_t *v, size_t len)
+{
+    /* __make_heap */
+    if (len < 2) return;
+    for (size_t parent = (len - 2) / 2;; --parent) {
+        heap_adjust(d, v, parent, len, d[parent], v[parent]);
+        if (parent == 0) return;
+    }
+}
+
+/* push_back(x) onto a full heap of len entries, push_heap, pop_heap,
+ * pop_back: d and v hold len + 1 slots. */
+static void heap_push_pop(float *d, uint32_t *v, size_t len, float dv, uint32_t vv)
+{
+    d[len] = dv;
+    v[len] = vv;
+    heap_push_up(d, v, len, 0, dv, vv);        /* push_heap over len + 1 */
+    const float ld = d[len];                    /* pop_heap: value = last, last = front */
+    const uint32_t lv = v[len];
+    d[len] = d[0];
+    v[len] = v[0];
+    heap_adjust(d, v, 0, len, ld, lv);          /* pop_back drops slot len */
+}
+
+void orc_heap_sequence(const float *dist, uint32_t n, uint32_t k, uint32_t *ids, float *out_dist, uint32_t *count)
+{
+    /* nearest_neighbors_impl's bookkeeping for a sequence of accepted
+     * candidates 0..n-1 with distances dist[] (test hook) */
+    float *d = malloc(sizeof(float) * (k + 1));
+    uint32_t *v = malloc(sizeof(uint32_t) * (k + 1));
+    uint32_t c = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (c < k) {
+            d[c] = dist[i];
+            v[c] = i;
+            if (++c == k) heap_make(d, v, k);
+        } else {
+            heap_push_pop(d, v, k, dist[i], i);
+        }
+    }
+    for (uint32_t i = 0; i < c; ++i) { ids[i] = v[i]; out_dist[i] = d[i]; }
+    *count = c;
+    free(d);
+    free(v);
 }
 
 static void search(Search *s, size_t left, size_t right)
@@ -105,18 +179,14 @@ static void search(Search *s, size_t left, size_t right)
     const uint32_t e = s->order[median];
     const float d = dist_to(s->xyz, e, s->q);
     if (d < s->maxd && (s->qshape == -1 || (s->shape ? s->shape[e] : -1) == s->qshape)) {
-        /* push_back; make_heap at k; beyond k push_heap + pop_heap + pop_back */
+        /* push_back; make_heap at k; beyond k push_heap + pop_heap + pop_back,
+         * and max_distance = the front's distance (kdtree.h:123-130) */
         if (s->cnt < s->k) {
             s->idx[s->cnt] = e;
             s->dist[s->cnt] = d;
-            if (++s->cnt == s->k)
-                for (uint32_t i = s->k / 2; i-- > 0;) sift_down(s->dist, s->idx, s->k, i);
+            if (++s->cnt == s->k) heap_make(s->dist, s->idx, s->k);
         } else {
-            if (d < s->dist[0]) {
-                s->dist[0] = d;
-                s->idx[0] = e;
-                sift_down(s->dist, s->idx, s->k, 0);
-            }
+            heap_push_pop(s->dist, s->idx, s->k, d, e);
             s->maxd = s->dist[0];
         }
     }
@@ -138,8 +208,14 @@ static void search(Search *s, size_t left, size_t right)
 uint32_t orc_kd_knn(const float *xyz, const int32_t *shape, const uint32_t *order, const uint8_t *axis, uint32_t n,
                     const float q[3], int32_t qshape, uint32_t k, float radius, uint32_t *idx, float *dist)
 {
-    Search s = {xyz, shape, order, axis, q, qshape, k, 0, radius, idx, dist};
+    if (k == 0) return 0;
+    /* the heap needs one slot beyond k for push_heap + pop_heap */
+    uint32_t vbuf[k + 1];
+    float dbuf[k + 1];
+    Search s = {xyz, shape, order, axis, q, qshape, k, 0, radius, vbuf, dbuf};
     search(&s, 0, n);
+    memcpy(idx, vbuf, sizeof(uint32_t) * s.cnt);
+    memcpy(dist, dbuf, sizeof(float) * s.cnt);
     return s.cnt;
 }
 

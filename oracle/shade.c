@@ -411,22 +411,32 @@ static float bsdf_pdf(const orc_material *m, v3 wo, v3 wi)
     return 0;
 }
 
-/* Material::sampleDirection for every type; returns the weight and sets wi.
- * Consumes the draws listed at the top of this file. */
-static v3 bsdf_sample(const orc_material *m, v3 wo, pcg32 *g, v3 *wi, float *pdf_out, int *delta)
+/* Uniform draws Material::sampleDirection consumes for material m: the
+ * Lambertian and rough Trowbridge warps take 2, the dielectric's branch
+ * choice 1, the rest 0 (the draw order listed at the top of this file). */
+static int bsdf_draws(const orc_material *m)
+{
+    if (m->type == ORC_MAT_LAMBERTIAN) return 2;
+    if (m->type == ORC_MAT_DIELECTRIC) return 1;
+    if (m->type == ORC_MAT_TROWBRIDGE || m->type == ORC_MAT_TROWBRIDGE_BASIC) return ggx_alpha(m) < 1e-3f ? 0 : 2;
+    return 0;
+}
+
+/* Material::sampleDirection for every type with its uniform draws given in
+ * u[0..bsdf_draws(m)-1]; returns the weight and sets wi. */
+static v3 bsdf_sample_u(const orc_material *m, v3 wo, const float *u, v3 *wi, float *pdf_out, int *delta)
 {
     *pdf_out = 0;
     *delta = 0;
     if (m->type == ORC_MAT_LAMBERTIAN) {
-        float u0 = pcg_float(g), u1 = pcg_float(g);
+        float u0 = u[0], u1 = u[1];
         *wi = cosine_hemisphere(u0, u1);
         *pdf_out = (wi->z / len(*wi)) * INV_PIf;
         return mk(m->kd[0], m->kd[1], m->kd[2]); /* weight = kd (LambertianBSDF.cu:52-74) */
     }
     if (m->type == ORC_MAT_DIELECTRIC) {
         *delta = 1;
-        float u = pcg_float(g);
-        if (u < 0.7f) {
+        if (u[0] < 0.7f) {
             /* SpecularRefractionBSDF::sampleDirection (SpecularRefractionBSDF.cu:31-63) */
             float nO = 1, nD = m->ior;
             if (wo.z / len(wo) < 0) { float t = nO; nO = nD; nD = t; }
@@ -458,8 +468,7 @@ static v3 bsdf_sample(const orc_material *m, v3 wo, pcg32 *g, v3 *wi, float *pdf
             return mk(fresnel_conductor(c, m->eta[0], m->k[0]) / c, fresnel_conductor(c, m->eta[1], m->k[1]) / c,
                       fresnel_conductor(c, m->eta[2], m->k[2]) / c);
         }
-        float u0 = pcg_float(g), u1 = pcg_float(g);
-        v3 h = ggx_sample_half(a, a, o, u0, u1);
+        v3 h = ggx_sample_half(a, a, o, u[0], u[1]);
         *wi = add(neg(o), mul(h, 2 * dot(o, h))); /* reflect (geometricMath.cu:196-199) */
         float pdf = ggx_Dv(a, a, o, h) / (4 * abscos2(o, h));
         *pdf_out = pdf;
@@ -475,6 +484,28 @@ static v3 bsdf_sample(const orc_material *m, v3 wo, pcg32 *g, v3 *wi, float *pdf
     /* light: NullBSDF (NullBSDF.cu); plain: PlainMaterial::sampleDirection returns 0 */
     *wi = mk(0, 0, 0);
     return mk(0, 0, 0);
+}
+
+/* bsdf_sample_u with its draws taken from the path's PCG32 stream. */
+static v3 bsdf_sample(const orc_material *m, v3 wo, pcg32 *g, v3 *wi, float *pdf_out, int *delta)
+{
+    float u[2] = {0, 0};
+    const int n = bsdf_draws(m);
+    for (int k = 0; k < n; ++k) u[k] = pcg_float(g);
+    return bsdf_sample_u(m, wo, u, wi, pdf_out, delta);
+}
+
+/* Russian roulette (pathTracing.cu:91-102): q = max(0.05, 1 - brightness(beta))
+ * with beta after this bounce's weight; the path ends if u < q, else beta is
+ * divided by 1 - q.  Returns 1 if the path survives. */
+static inline float lum(v3 c);
+static int russian_roulette(v3 *beta, float u)
+{
+    float q = fmaxf(0.05f, 1 - lum(*beta));
+    if (u < q) return 0;
+    float sc = 1 - q;
+    *beta = mk(beta->x / sc, beta->y / sc, beta->z / sc);
+    return 1;
 }
 
 /* ------------------------------------------------------------------ */
@@ -896,12 +927,7 @@ static int path_impl(const orc_scene *s, uint32_t px, uint32_t py, uint32_t samp
         v3 wgt = bsdf_sample(m, wo, &g, &wi, &pdf, &delta);
         beta = mk(beta.x * wgt.x, beta.y * wgt.y, beta.z * wgt.z);
         if (wgt.x == 0 && wgt.y == 0 && wgt.z == 0) break;
-        if (is_scattering(m) && depth > rr_depth) {
-            float q = fmaxf(0.05f, 1 - lum(beta));
-            if (pcg_float(&g) < q) break;
-            float sc = 1 - q;
-            beta = mk(beta.x / sc, beta.y / sc, beta.z / sc);
-        }
+        if (is_scattering(m) && depth > rr_depth && !russian_roulette(&beta, pcg_float(&g))) break;
         v3 ww = add(add(mul(t, wi.x), mul(b, wi.y)), mul(n, wi.z));
         o = spawn_origin(h.p, h.perr, n, ww);
         d = ww;
@@ -1018,4 +1044,106 @@ float orc_compute_rmse(const double *sum_a, double n_a, const double *sum_b, dou
     }
     rmse /= (float)npix;
     return sqrtf(rmse);
+}
+
+/* ------------------------------------------------------------------ */
+/* The shading functions one at a time, for the tests that pin them     */
+/* against closed forms, quadrature and sampling statistics.  Vectors   */
+/* are in the shading frame (z = normal) unless stated.                 */
+/* ------------------------------------------------------------------ */
+void orc_concentric_disk(float u0, float u1, float out[2]) { concentric_disk(u0, u1, &out[0], &out[1]); }
+
+void orc_ggx_sample_half(float alpha, const float wo[3], float u0, float u1, float h[3])
+{
+    v3 r = ggx_sample_half(alpha, alpha, mk(wo[0], wo[1], wo[2]), u0, u1);
+    h[0] = r.x; h[1] = r.y; h[2] = r.z;
+}
+
+void orc_ggx_D_n(float alpha, const float *h, uint32_t n, float *out)
+{
+    for (uint32_t i = 0; i < n; ++i) out[i] = ggx_D(alpha, alpha, mk(h[3 * i], h[3 * i + 1], h[3 * i + 2]));
+}
+
+void orc_ggx_lambda_n(float alpha, const float *w, uint32_t n, float *out)
+{
+    for (uint32_t i = 0; i < n; ++i) out[i] = ggx_lambda(alpha, alpha, mk(w[3 * i], w[3 * i + 1], w[3 * i + 2]));
+}
+
+float orc_fresnel_conductor(float cosi, float eta, float k) { return fresnel_conductor(cosi, eta, k); }
+
+int orc_bsdf_draws(const orc_material *m) { return bsdf_draws(m); }
+
+/* lightReflected (f * |cos wi|) and pdf of n direction pairs. */
+void orc_bsdf_eval_n(const orc_material *m, const float *wo, const float *wi, uint32_t n, float *f, float *pdf)
+{
+    for (uint32_t i = 0; i < n; ++i) {
+        v3 o = mk(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), w = mk(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]);
+        v3 r = bsdf_f(m, o, w);
+        f[3 * i] = r.x; f[3 * i + 1] = r.y; f[3 * i + 2] = r.z;
+        pdf[i] = bsdf_pdf(m, o, w);
+    }
+}
+
+/* sampleDirection of n outgoing directions with explicit draws (2 per
+ * sample; the first bsdf_draws(m) are used). */
+void orc_bsdf_sample_n(const orc_material *m, const float *wo, const float *u, uint32_t n, float *wi, float *pdf,
+                       int32_t *delta, float *weight)
+{
+    for (uint32_t i = 0; i < n; ++i) {
+        v3 w;
+        int dl;
+        v3 r = bsdf_sample_u(m, mk(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), u + 2 * i, &w, &pdf[i], &dl);
+        wi[3 * i] = w.x; wi[3 * i + 1] = w.y; wi[3 * i + 2] = w.z;
+        weight[3 * i] = r.x; weight[3 * i + 1] = r.y; weight[3 * i + 2] = r.z;
+        delta[i] = dl;
+    }
+}
+
+/* The dielectric's branch over a PCG32 stream: how many of n draws (stream
+ * seeded like a path's) take the refraction branch. */
+uint32_t orc_dielectric_refractions(uint32_t seed, uint32_t n)
+{
+    pcg32 g = pcg_seed(orc_path_initstate(seed, 0), 0);
+    orc_material m;
+    memset(&m, 0, sizeof m);
+    m.type = ORC_MAT_DIELECTRIC;
+    m.kd[0] = m.kd[1] = m.kd[2] = 1;
+    m.ior = 1.5f;
+    uint32_t k = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        v3 w;
+        float pdf;
+        int dl;
+        bsdf_sample(&m, mk(0.3f, 0.2f, 0.9327379f), &g, &w, &pdf, &dl);
+        k += w.z < 0;  /* refracted directions leave through the other side */
+    }
+    return k;
+}
+
+float orc_luminance(const float rgb[3]) { return lum(mk(rgb[0], rgb[1], rgb[2])); }
+
+int orc_russian_roulette(const float beta[3], float u, float out[3])
+{
+    v3 b = mk(beta[0], beta[1], beta[2]);
+    int alive = russian_roulette(&b, u);
+    out[0] = b.x; out[1] = b.y; out[2] = b.z;
+    return alive;
+}
+
+int orc_sphere_light_sample(const float center[3], float radius, const float p[3], float u0, float u1, float wi[3],
+                            float *dist, float *pdf)
+{
+    orc_sphere sp;
+    orc_sphere_init(&sp, center, radius);
+    v3 w;
+    int ok = sphere_light_sample(&sp, mk(p[0], p[1], p[2]), u0, u1, &w, dist, pdf);
+    wi[0] = w.x; wi[1] = w.y; wi[2] = w.z;
+    return ok;
+}
+
+float orc_sphere_light_pdf(const float center[3], float radius, const float p[3])
+{
+    orc_sphere sp;
+    orc_sphere_init(&sp, center, radius);
+    return sphere_light_pdf(&sp, mk(p[0], p[1], p[2]));
 }

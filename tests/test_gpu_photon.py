@@ -133,3 +133,30 @@ def test_gather_lambert_matches_oracle(rt, gpu_ctx, k, radius):
     ok = unambiguous(x, q, k, radius, sh, qs)
     assert ok.mean() > 0.9
     assert np.allclose(got[ok], want[ok], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("k", [1, 2, 5, 16, 100])
+def test_result_order_equals_the_reference_heap_order(rt, gpu_ctx, k):
+    """The search returns its vector as kdtree.h leaves it: insertion order
+    until k photons were accepted, then the layout of libstdc++'s
+    make_heap / push_heap / pop_heap (the oracle's order is checked against
+    the standard library itself in test_oracle_photon_order.py).  With no
+    coordinate ties the GPU's tree is the oracle's, so indices and
+    distances must agree position by position on every query."""
+    print(f"[photon] result order k={k}", flush=True)
+    rng = np.random.default_rng(70 + k)
+    # distinct coordinates on every axis (random float32s collide): one tree
+    x = np.stack([(rng.permutation(1 << 20)[:30000] + 0.5) / (1 << 20) for _ in range(3)], 1).astype(np.float32)
+    for a in range(3):
+        assert np.unique(x[:, a]).size == x.shape[0]
+    q = rng.random((4096, 3)).astype(np.float32)
+    q[:64] = x[:64]  # queries on photons: a distance-0 candidate
+    got = gpu_knn(rt, gpu_ctx, x, q, k, 0.2)
+    tree = oracle.kd_build(x)[:2]
+    want = oracle.knn(x, q, k, 0.2, tree=tree)
+    assert np.array_equal(got[0], want[0])
+    for i in range(q.shape[0]):
+        c = got[0][i]
+        assert np.array_equal(got[1][i, :c], want[1][i, :c]), i
+        assert np.array_equal(got[2][i, :c], want[2][i, :c]), i
+    assert (got[0] == k).mean() > 0.5

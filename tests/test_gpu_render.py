@@ -115,3 +115,61 @@ def test_empty_scene_renders_black(rt):
     d.spheres = []
     a, n, _ = gpu_render(rt, _ctx(rt), d, 2)
     assert np.all(a == 0) and np.all(n == 2)
+
+
+def test_profiled_and_counting_frames_render_the_same_image(rt, orc):
+    """RT_FLAG_PROFILE brackets every traversal launch with HIP events and
+    RT_FLAG_COUNT runs the counting kernel variants: both leave the image
+    bit for bit as rt_render makes it.  The counts describe the frame: every
+    extension and shadow ray visits at least the root, occluded shadow rays
+    are a part of all of them, and a shadow ray's traversal is no longer
+    than the depth allows on average.  Counted on the four-wide tree, the
+    extension rays of a one-bounce frame are the camera rays, whose visits
+    equal the oracle's counting traversal of the same tree within the
+    kernel's parked-leaf slack."""
+    print("[render stats] profiled / counting frames", flush=True)
+    d = scenes.procedural(grid=48, n_spheres=6, subdiv=2, width=96, height=54, spp=4)
+    ctx = _ctx(rt)
+    sc = rt.Scene(ctx, d)
+    films = []
+    stats = []
+    for kw in ({}, {"profile": True}, {"count": True}, {"profile": True, "count": True}):
+        f = sc.new_film()
+        sc.render(f, spp=4, **kw)
+        torch.cuda.synchronize()
+        films.append((f[0].cpu().numpy().copy(), f[1].cpu().numpy().copy()))
+        stats.append(sc.stats())
+    for a, n in films[1:]:
+        assert np.array_equal(a, films[0][0]) and np.array_equal(n, films[0][1])
+    plain, prof, cnt, both = stats
+    assert plain.extend_launches == 0 and plain.ext_node_visits == 0
+    assert prof.extend_launches > 0 and prof.extend_launches == prof.connect_launches
+    assert prof.extend_ms > 0 and prof.connect_ms > 0 and prof.ext_node_visits == 0
+    assert cnt.extend_ms == 0 and cnt.extend_launches == 0
+    for s in (cnt, both):
+        assert s.extension_rays == plain.extension_rays and s.shadow_rays == plain.shadow_rays
+        assert s.ext_node_visits >= s.extension_rays and s.shadow_node_visits >= s.shadow_rays
+        assert s.ext_tri_tests > 0 and 0 < s.shadow_occluded < s.shadow_rays
+    # which rays share a wavefront (dynamic claims) decides when a parked
+    # leaf's hit starts culling, so visit counts vary by a few per frame
+    for a, b in ((cnt.ext_node_visits, both.ext_node_visits), (cnt.shadow_node_visits, both.shadow_node_visits)):
+        assert abs(a - b) <= 1e-3 * a
+    # camera rays only (max depth 1): the kernel's visit count against the
+    # oracle's count of the same four-wide traversal of the scene's tree
+    f = sc.new_film()
+    sc.render(f, spp=1, max_depth=1, count=True)
+    torch.cuda.synchronize()
+    s1 = sc.stats()
+    assert s1.extension_rays == d.width * d.height
+    os_ = orc.OrcScene(d)
+    segs = np.array([os_.path_segments(px, py, 0, max_depth=1)[2][0]
+                     for py in range(d.height) for px in range(d.width)], np.float32)
+    bvh = rt.Bvh(ctx, d.xyz, d.idx)
+    _, tris = bvh.export()
+    _, _, nn, ntr, _ = orc.bvh4_trace(bvh.export4(), tris, [np.ascontiguousarray(segs[:, k]) for k in range(6)])
+    print(f"[render stats] camera rays: kernel {s1.ext_node_visits} visits / {s1.ext_tri_tests} tests, "
+          f"oracle {nn} / {ntr}", flush=True)
+    assert 0.98 * nn <= s1.ext_node_visits <= 1.15 * nn
+    assert 0.98 * ntr <= s1.ext_tri_tests <= 1.3 * ntr
+    sc.close()
+    bvh.close()

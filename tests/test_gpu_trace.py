@@ -170,3 +170,59 @@ def test_full_size_c1_against_reference_bvh(rt, orc, gpu_ctx):
         bp, bt, fl = orc.brute_trace(xyz, idx, sub, nthreads=16)
         explained = (fl != 0) | ((bp == prim[diff]) & (bt == t[diff]))
         assert explained.all(), diff[~explained][:10]
+
+
+def test_shadow_query_replays_scene_is_visible_through_the_c_abi(rt, orc, gpu_ctx):
+    """The reference's shadow test as its callers consume the boundary:
+    Scene::intersects calls BVH::intersectRays(rays, ids, checkShadows=true)
+    on host rays with maximumOffset = +inf (spawnRay), re-tests
+    triangles[id].intersects(ray, hitOffset = inf, checkShadow) for t
+    (Scene.cu:108-150), and Scene::isVisible says visible = !hit || t > dist
+    (Scene.cu:238-276).  Here the boundary call is rt_trace_closest with
+    RT_FLAG_HOST_PTRS | RT_FLAG_SHADOW through ctypes on numpy buffers, the
+    re-test is the oracle's literal Triangle::intersects, and the result must
+    equal the oracle's brute-force closest opaque hit, and rt_trace_any with
+    tmax = the light distance, on every unambiguous ray."""
+    import ctypes as C
+    print("[shadow query] 8192 triangles, 30 % transparent", flush=True)
+    xyz, idx = rt.gen_soup(8192, seed=21, edge_mu=0.03)
+    rng = np.random.default_rng(22)
+    transparent = (rng.uniform(size=idx.shape[0]) < 0.3).astype(np.uint8)
+    n = 8192
+    o = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    light = rng.uniform(-0.2, 1.2, (n, 3)).astype(np.float32)
+    to_l = (light - o).astype(np.float32)
+    dist = np.linalg.norm(to_l, axis=1).astype(np.float32)
+    d = (to_l / dist[:, None]).astype(np.float32)
+    rays = [np.ascontiguousarray(o[:, 0]), np.ascontiguousarray(o[:, 1]), np.ascontiguousarray(o[:, 2]),
+            np.ascontiguousarray(d[:, 0]), np.ascontiguousarray(d[:, 1]), np.ascontiguousarray(d[:, 2])]
+    bvh = rt.Bvh(gpu_ctx, xyz, idx)
+    bvh.set_transparent(transparent)
+    L = rt.lib()
+    r = rt.Rays(*[a.ctypes.data for a in rays], None)  # maximumOffset = +inf
+    ids = np.empty(n, np.int32)
+    tt = np.empty(n, np.float32)
+    rt.check(L.rt_trace_closest(bvh.h, C.byref(r), n, ids.ctypes.data, tt.ctypes.data,
+                                rt.RT_FLAG_HOST_PTRS | rt.RT_FLAG_SHADOW, None))
+    # Scene::intersects: re-test the returned triangle for its distance
+    tri = xyz[idx]
+    visible = np.ones(n, bool)
+    for i in np.nonzero(ids >= 0)[0]:
+        ok, t_re, _ = orc.tri_intersects(o[i], d[i], *tri[ids[i]])
+        assert ok, i
+        assert t_re == tt[i], (i, t_re, tt[i])
+        visible[i] = t_re > dist[i]
+    bp, bt, fl = orc.brute_trace(xyz, idx, rays, transparent=transparent, check_shadow=True)
+    clean = fl == 0
+    assert clean.mean() > 0.99
+    assert np.array_equal(ids[clean], bp[clean]) and np.array_equal(tt[clean], bt[clean])
+    assert not np.any(transparent[ids[ids >= 0]])
+    want_visible = (bp < 0) | (bt > dist)
+    assert np.array_equal(visible[clean], want_visible[clean])
+    occ = bvh.trace_any(*rays, dist, host=True).astype(bool)
+    assert np.array_equal(occ[clean], ~want_visible[clean])
+    assert want_visible.sum() > 200 and (~want_visible).sum() > 200
+    # without the flag, transparent triangles count
+    ids2, _ = bvh.trace_closest(*rays, host=True)
+    assert np.any(transparent[ids2[ids2 >= 0]])
+    bvh.close()

@@ -165,6 +165,8 @@ _SIGS = {
     "rt_image_pixel_bytes": (C.c_size_t, [C.c_int, _U32, _U32]),
     "rt_image_header": (C.c_size_t, [C.c_int, _U32, _U32, _P, C.c_size_t]),
     "rt_film_tonemap": (C.c_int, [_P, _P, _P, _U32, _U32, C.c_uint64, C.c_int, C.c_int, _P, C.c_size_t, _P]),
+    "rt_film_tonemap_noalloc": (C.c_int, [_P, _P, _P, _U32, _U32, C.c_uint64, C.c_int, C.c_int, _P, C.c_size_t,
+                                          _P]),
     "rt_photon_map_build": (C.c_int, [_P, _P, _P, _U32, C.POINTER(_P)]),
     "rt_photon_map_destroy": (C.c_int, [_P]),
     "rt_photon_map_get_info": (C.c_int, [_P, C.POINTER(_U32), C.POINTER(_U32)]),

@@ -357,44 +357,96 @@ size_t rt_image_header(rt_image_format format, uint32_t width, uint32_t height, 
     return len;
 }
 
-rt_status rt_film_tonemap(rt_ctx *ctx, const double *sums, const uint32_t *nvalid, uint32_t width, uint32_t height,
-                          uint64_t samples, rt_tone_mapping mode, rt_image_format format, uint8_t *pixels,
-                          size_t pixel_bytes, void *stream_)
+}  // extern "C"
+
+// The partial maxima of rt_film_tonemap_noalloc: one array per GPU, kept for
+// the life of the process and shared by every such call on that GPU.  Host
+// code never passes the array's name to a kernel (on the host it is a
+// stand-in, not a device address): each device's address is resolved once
+// with hipGetSymbolAddress and used from then on.
+constexpr uint32_t kScratchPairs = kPartials + 1;  // 1025 pairs of doubles, 16.4 KB
+static __device__ __align__(256) double2 scratch[kScratchPairs];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_lock;
+double2 *g_scratch[kMaxScratchDevices] = {};
+
+// The device's copy of `scratch` (that device current), or null with the
+// error text set.
+double2 *scratch_on(int device)
 {
-    if (!ctx || !sums || !pixels) { set_error("rt_film_tonemap: null argument"); return RT_ERR_INVALID; }
+    if (device < 0 || device >= kMaxScratchDevices) {
+        set_error("rt_film_tonemap_noalloc: device index beyond the scratch table");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> g(g_scratch_lock);
+    if (!g_scratch[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error(std::string("rt_film_tonemap_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch[device] = (double2 *)ptr;
+    }
+    return g_scratch[device];
+}
+
+// Shared body of rt_film_tonemap and rt_film_tonemap_noalloc: `partial` is
+// null for the context's scratch (allocated on first use), else a device
+// buffer of kPartials + 1 pairs.
+rt_status tonemap(const char *who, bool noalloc, rt_ctx *ctx, const double *sums, const uint32_t *nvalid,
+                  uint32_t width, uint32_t height, uint64_t samples, rt_tone_mapping mode, rt_image_format format,
+                  uint8_t *pixels, size_t pixel_bytes, void *stream_)
+{
+    const std::string name(who);
+    if (!ctx || !sums || !pixels) { set_error(name + ": null argument"); return RT_ERR_INVALID; }
     if ((int)mode < RT_TONE_EQUALIZE || (int)mode > RT_TONE_GAMMA_CLAMP) {
-        set_error("rt_film_tonemap: unknown tone mapping");
+        set_error(name + ": unknown tone mapping");
         return RT_ERR_INVALID;
     }
     const size_t need = rt_image_pixel_bytes(format, width, height);
     if (need == 0) {
-        set_error("rt_film_tonemap: image format unknown or size outside 1..32768 per side");
+        set_error(name + ": image format unknown or size outside 1..32768 per side");
         return RT_ERR_INVALID;
     }
-    if (pixel_bytes < need) { set_error("rt_film_tonemap: pixel buffer too small"); return RT_ERR_INVALID; }
+    if (pixel_bytes < need) { set_error(name + ": pixel buffer too small"); return RT_ERR_INVALID; }
     if ((uintptr_t)pixels % 4u || (uintptr_t)sums % 8u || (uintptr_t)nvalid % 4u) {
-        set_error("rt_film_tonemap: pixels must be 4-byte aligned, sums 8-byte, nvalid 4-byte");
+        set_error(name + ": pixels must be 4-byte aligned, sums 8-byte, nvalid 4-byte");
         return RT_ERR_INVALID;
     }
     if (samples == 0 || samples > (1ull << 53)) {
-        set_error("rt_film_tonemap: samples must be 1..2^53 (the film divides by it)");
+        set_error(name + ": samples must be 1..2^53 (the film divides by it)");
         return RT_ERR_INVALID;
     }
     const size_t npix = (size_t)width * height;
+    // partial pairs of the max pass (one per block) plus the folded pair
+    const uint32_t nmax = (uint32_t)std::min<size_t>(kPartials, (npix + kThreads - 1) / kThreads);
+    if (noalloc && nmax + 1 > kScratchPairs) {
+        set_error(name + ": the film needs more partial maxima than the scratch array holds");
+        return RT_ERR_INVALID;
+    }
     std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(ctx));
     RT_HIP(hipSetDevice(rt::ctx_device(ctx)));
     const int dev = rt::ctx_device(ctx);
     if (!device_buffer_ok(sums, npix * 3 * sizeof(double), dev) ||
         (nvalid && !device_buffer_ok(nvalid, npix * sizeof(uint32_t), dev)) || !device_buffer_ok(pixels, need, dev)) {
-        set_error("rt_film_tonemap: a buffer is not device memory of the film's size on this context's device");
+        set_error(name + ": a buffer is not device memory of the film's size on this context's device");
         return RT_ERR_INVALID;
     }
-    double2 *partial = (double2 *)rt::ctx_film_scratch(ctx, (kPartials + 1) * sizeof(double2));
-    if (!partial) { set_error("rt_film_tonemap: device allocation failed"); return RT_ERR_OOM; }
+    double2 *partial = nullptr;
+    if (noalloc) {
+        partial = scratch_on(dev);
+        if (!partial) return RT_ERR_HIP;
+    } else {
+        partial = (double2 *)rt::ctx_film_scratch(ctx, (kPartials + 1) * sizeof(double2));
+        if (!partial) { set_error(name + ": device allocation failed"); return RT_ERR_OOM; }
+    }
     hipStream_t stream = stream_ ? (hipStream_t)stream_ : (hipStream_t)rt::ctx_stream_ptr(ctx);
 
     const FilmIn f{sums, nvalid, (double)samples};
-    const uint32_t nmax = (uint32_t)std::min<size_t>(kPartials, (npix + kThreads - 1) / kThreads);
     k_film_max<<<nmax, kThreads, 0, stream>>>(f, npix, partial);
     RT_HIP(hipGetLastError());
     k_film_fold<<<1, kThreads, 0, stream>>>(partial, nmax);
@@ -410,6 +462,26 @@ rt_status rt_film_tonemap(rt_ctx *ctx, const double *sums, const uint32_t *nvali
     k_film_map<<<grid, kThreads, 0, stream>>>(f, partial + nmax, o, (int)mode);
     RT_HIP(hipGetLastError());
     return RT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rt_status rt_film_tonemap(rt_ctx *ctx, const double *sums, const uint32_t *nvalid, uint32_t width, uint32_t height,
+                          uint64_t samples, rt_tone_mapping mode, rt_image_format format, uint8_t *pixels,
+                          size_t pixel_bytes, void *stream)
+{
+    return tonemap("rt_film_tonemap", false, ctx, sums, nvalid, width, height, samples, mode, format, pixels,
+                   pixel_bytes, stream);
+}
+
+rt_status rt_film_tonemap_noalloc(rt_ctx *ctx, const double *sums, const uint32_t *nvalid, uint32_t width,
+                                  uint32_t height, uint64_t samples, rt_tone_mapping mode, rt_image_format format,
+                                  uint8_t *pixels, size_t pixel_bytes, void *stream)
+{
+    return tonemap("rt_film_tonemap_noalloc", true, ctx, sums, nvalid, width, height, samples, mode, format, pixels,
+                   pixel_bytes, stream);
 }
 
 }  // extern "C"

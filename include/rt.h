@@ -345,6 +345,21 @@ rt_status rt_film_tonemap(rt_ctx *ctx, const double *sums, const uint32_t *nvali
                           uint64_t samples, rt_tone_mapping mode, rt_image_format format, uint8_t *pixels,
                           size_t pixel_bytes, void *stream);
 
+/* rt_film_tonemap without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, checks and output as rt_film_tonemap; the partial maxima live in
+ * a file-scope device array of 1025 pairs of doubles (16.4 KB) that each GPU
+ * keeps for the life of the process, instead of the context's scratch that
+ * rt_film_tonemap allocates on its first call.  A film whose max pass would
+ * need more partial pairs than the array holds is refused (RT_ERR_INVALID; no
+ * film within 32768 x 32768 does).  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other contexts, other
+ * threads) race; order them. */
+rt_status rt_film_tonemap_noalloc(rt_ctx *ctx, const double *sums, const uint32_t *nvalid, uint32_t width,
+                                  uint32_t height, uint64_t samples, rt_tone_mapping mode, rt_image_format format,
+                                  uint8_t *pixels, size_t pixel_bytes, void *stream);
+
 /* ------------------------------------------------------------------ */
 /* Photon map: kd-tree and k-nearest search                             */
 /* ------------------------------------------------------------------ */

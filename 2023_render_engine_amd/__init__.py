@@ -137,6 +137,7 @@ _SIGS = {
     "rt_scene_add_sphere": (C.c_int, [_P, _P, C.c_float, _U32, C.c_int]),
     "rt_scene_add_plane": (C.c_int, [_P, _P, _P, _P, _P, _U32]),
     "rt_scene_add_mesh_uv": (C.c_int, [_P, _P, _U32, _P, _U32, _P, _U32, _P]),
+    "rt_scene_add_mesh_normals": (C.c_int, [_P, _P, _U32, _P, _U32, _P, _P, _U32, _P]),
     "rt_scene_add_texture": (C.c_int, [_P, _P, _U32, _U32, _P]),
     "rt_scene_add_point_light": (C.c_int, [_P, C.POINTER(PointLight)]),
     "rt_scene_set_camera": (C.c_int, [_P, C.POINTER(Camera)]),
@@ -422,6 +423,8 @@ class Scene:
             check(L.rt_scene_add_texture(h, _ptr(t), t.shape[1], t.shape[0], None))
         uv = getattr(desc, "tri_uv", None)
         uv = None if uv is None else np.ascontiguousarray(uv, np.float32).reshape(-1, 6)
+        nrm9 = getattr(desc, "tri_nrm", None)
+        nrm9 = None if nrm9 is None else np.ascontiguousarray(nrm9, np.float32).reshape(-1, 9)
         xyz = np.ascontiguousarray(desc.xyz, np.float32)
         tm = np.asarray(desc.tri_mat, np.uint32)
         idx = np.ascontiguousarray(desc.idx, np.uint32)
@@ -432,10 +435,14 @@ class Scene:
             m = tm[start]
             end = start + int(np.argmax(tm[start:] != m)) if np.any(tm[start:] != m) else nt
             sub = np.ascontiguousarray(idx[start:end])
-            if uv is None:
+            suv = None if uv is None else np.ascontiguousarray(uv[start:end])
+            if nrm9 is not None:
+                snr = np.ascontiguousarray(nrm9[start:end])
+                check(L.rt_scene_add_mesh_normals(h, _ptr(xyz), xyz.shape[0], _ptr(sub), sub.shape[0], _ptr(suv),
+                                                  _ptr(snr), int(m), None))
+            elif uv is None:
                 check(L.rt_scene_add_mesh(h, _ptr(xyz), xyz.shape[0], _ptr(sub), sub.shape[0], int(m), None))
             else:
-                suv = np.ascontiguousarray(uv[start:end])
                 check(L.rt_scene_add_mesh_uv(h, _ptr(xyz), xyz.shape[0], _ptr(sub), sub.shape[0], _ptr(suv), int(m),
                                              None))
             start = end

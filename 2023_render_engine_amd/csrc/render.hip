@@ -478,6 +478,18 @@ __device__ __forceinline__ void triangle_surface(const SceneDev &sc, int32_t id,
     V3 n = nrm(cross(sub(P0, P2), sub(P1, P2)));
     if (dot(n, d) > 0) n = neg(n);  // normal faces the ray (Triangle.cu:368-373)
     s.n = n;
+    if (sc.tri_nrm) {
+        // a mesh with vertex normals (Triangle.cu:379-418): the shading normal
+        // is the interpolated one, not turned towards the ray, and the hit
+        // point moves 0.15 along it
+        const float *q = sc.tri_nrm + 9 * (size_t)id;
+        if (q[0] == q[0]) {
+            const V3 ns = nrm(add(add(mul(mk(q[0], q[1], q[2]), b0), mul(mk(q[3], q[4], q[5]), b1)),
+                                  mul(mk(q[6], q[7], q[8]), b2)));
+            s.n = ns;
+            s.p = add(s.p, mul(ns, 0.15f));
+        }
+    }
     s.mat = sc.tri_mat[id];
     if (sc.ntex) {
         // surfacePoint = b0 uv0 + b1 uv1 + b2 uv2 (Triangle.cu:356-358), the

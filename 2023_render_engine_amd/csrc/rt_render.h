@@ -51,6 +51,7 @@ struct SceneDev {
     const TexDev *tex;      // ntex textures over `texels`
     const uint32_t *texels;
     const float *tri_uv;    // 6 floats per triangle, or null: the default (0,0), (1,0), (1,1)
+    const float *tri_nrm;   // 9 floats per triangle (corner normals, NaN: none), or null
     CameraDev cam;
     uint32_t nsph, npl, nlights;  // nlights = npl + area lights (NEE slots per path)
     uint32_t area_light[kMaxSpheres];  // sphere index of area light a

@@ -4,6 +4,8 @@
 // wavefront buffers and launches waves.
 #include <hip/hip_runtime.h>
 
+#include <limits>
+
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
@@ -203,6 +205,8 @@ struct rt_scene {
     std::vector<rt::PlaneDev> planes;
     std::vector<float> tri_uv;       // 6 floats per triangle once a mesh brings texture coordinates
     bool has_uv = false;
+    std::vector<float> tri_nrm;      // 9 floats per triangle (corner normals) once a mesh brings normals;
+    bool has_nrm = false;            // a NaN first component: the triangle has none
     std::vector<rt::TexDev> textures;
     std::vector<uint32_t> texels;    // 0x00BBGGRR
     std::vector<rt_point_light> lights;
@@ -217,6 +221,7 @@ struct rt_scene {
     rt_point_light *d_pl = nullptr;
     rt::PlaneDev *d_plane = nullptr;
     float *d_tri_uv = nullptr;
+    float *d_tri_nrm = nullptr;
     rt::TexDev *d_tex = nullptr;
     uint32_t *d_texels = nullptr;
     rt::SceneDev dev{};
@@ -425,6 +430,8 @@ static void free_device(rt_scene *s)
     if (s->d_pl) (void)hipFree(s->d_pl);
     if (s->d_plane) (void)hipFree(s->d_plane);
     if (s->d_tri_uv) (void)hipFree(s->d_tri_uv);
+    if (s->d_tri_nrm) (void)hipFree(s->d_tri_nrm);
+    s->d_tri_nrm = nullptr;
     if (s->d_tex) (void)hipFree(s->d_tex);
     if (s->d_texels) (void)hipFree(s->d_texels);
     s->d_tri_uv = nullptr; s->d_tex = nullptr; s->d_texels = nullptr;
@@ -472,6 +479,12 @@ rt_status rt_scene_add_mesh(rt_scene *s, const float *xyz, uint32_t nv, const ui
 rt_status rt_scene_add_mesh_uv(rt_scene *s, const float *xyz, uint32_t nv, const uint32_t *idx, uint32_t nt,
                                const float *uv, uint32_t material, uint32_t *first_id)
 {
+    return rt_scene_add_mesh_normals(s, xyz, nv, idx, nt, uv, nullptr, material, first_id);
+}
+
+rt_status rt_scene_add_mesh_normals(rt_scene *s, const float *xyz, uint32_t nv, const uint32_t *idx, uint32_t nt,
+                                    const float *uv, const float *normals, uint32_t material, uint32_t *first_id)
+{
     if (!s || (nt && (!xyz || !idx))) { set_error("rt_scene_add_mesh: null argument"); return RT_ERR_INVALID; }
     if (material >= s->mats.size()) { set_error("rt_scene_add_mesh: unknown material id"); return RT_ERR_INVALID; }
     const size_t base = s->tri_mat.size();
@@ -496,6 +509,17 @@ rt_status rt_scene_add_mesh_uv(rt_scene *s, const float *xyz, uint32_t nv, const
         for (uint32_t i = 0; i < nt; ++i)
             std::copy(uv ? uv + 6 * (size_t)i : kDefaultUv, (uv ? uv + 6 * (size_t)i : kDefaultUv) + 6,
                       s->tri_uv.begin() + 6 * (base + i));
+    }
+    // corner normals likewise, once any mesh brings them (TriangleMeshData::
+    // normals, shapes/Triangle.cu:379-418); NaN marks a triangle without
+    const float nan = std::numeric_limits<float>::quiet_NaN();
+    if (normals && !s->has_nrm) {
+        s->tri_nrm.assign(base * 9, nan);
+        s->has_nrm = true;
+    }
+    if (s->has_nrm) {
+        s->tri_nrm.resize((base + nt) * 9, nan);
+        if (normals) std::copy(normals, normals + 9 * (size_t)nt, s->tri_nrm.begin() + 9 * base);
     }
     if (first_id) *first_id = (uint32_t)base;
     s->committed = false;
@@ -620,6 +644,12 @@ rt_status rt_scene_commit(rt_scene *s)
             set_error("rt_scene_commit: a material names a texture the scene does not have");
             return RT_ERR_INVALID;
         }
+    if (s->has_nrm && !s->tri_nrm.empty()) {
+        // corner normals of every triangle (NaN first component: none)
+        RT_HIP(hipMalloc(&s->d_tri_nrm, s->tri_nrm.size() * sizeof(float)));
+        RT_HIP(hipMemcpyAsync(s->d_tri_nrm, s->tri_nrm.data(), s->tri_nrm.size() * sizeof(float),
+                              hipMemcpyHostToDevice, stream));
+    }
     if (!s->textures.empty()) {
         // texture coordinates are only read when the scene has textures
         RT_HIP(hipMalloc(&s->d_tex, s->textures.size() * sizeof(rt::TexDev)));
@@ -655,6 +685,7 @@ rt_status rt_scene_commit(rt_scene *s)
     d.tex = s->d_tex;
     d.texels = s->d_texels;
     d.tri_uv = s->d_tri_uv;
+    d.tri_nrm = s->d_tri_nrm;
     d.ntex = (uint32_t)s->textures.size();
     d.npl = (uint32_t)s->lights.size();
     uint32_t na = 0;

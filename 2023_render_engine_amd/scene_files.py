@@ -422,7 +422,7 @@ class SceneFiles:
         self.obejota = load_obejota(obejota_path) if obejota_path else ObejotaScene()
         self._mats, self._ids = [], {}
         self.textures, self._tex_ids = [], {}
-        xyz, idx, tri_mat, uv = [], [], [], []
+        xyz, idx, tri_mat, uv, nrm = [], [], [], [], []
         base = 0
         default = np.array([0, 0, 1, 0, 1, 1], np.float32)
         for o in self.objs:
@@ -437,10 +437,19 @@ class SceneFiles:
                 has = (o.vt >= 0).all(axis=1)
                 t[has] = o.texcoords[o.vt[has]].reshape(-1, 6)
             uv.append(t)
+            # corner normals where the file has vn lines (TriangleMesh::normals
+            # is per file, Triangle.cu:379-418); a face that names no normals
+            # gets none (the reference would read its unset normal indices)
+            nr = np.full((len(o.v), 9), np.nan, np.float32)
+            if len(o.normals):
+                has = (o.vn >= 0).all(axis=1)
+                nr[has] = o.normals[o.vn[has]].reshape(-1, 9)
+            nrm.append(nr)
         self.xyz = np.concatenate(xyz).astype(np.float32) if xyz else np.zeros((0, 3), np.float32)
         self.idx = np.concatenate(idx).astype(np.uint32) if idx else np.zeros((0, 3), np.uint32)
         self.tri_mat = np.array(tri_mat, np.uint32)
         self.tri_uv = np.concatenate(uv) if uv and any(len(o.texcoords) for o in self.objs) else None
+        self.tri_nrm = np.concatenate(nrm) if nrm and any(len(o.normals) for o in self.objs) else None
         self.spheres = []
 
     def _material_id(self, o, name):
@@ -481,4 +490,5 @@ class SceneFiles:
                       spp, max_depth, rr_depth, seed)
         d.textures = list(self.textures)
         d.tri_uv = self.tri_uv
+        d.tri_nrm = self.tri_nrm
         return d

@@ -35,6 +35,7 @@ class SceneDesc:
     planes: list = field(default_factory=list)    # (corner, axis_x, axis_y, normal, material): Plane shapes
     textures: list = field(default_factory=list)  # (height, width, 3) uint8 arrays, row y = pixelMatrix[y]
     tri_uv: object = None                         # (nt, 6) float32 corner texture coordinates, or None
+    tri_nrm: object = None                        # (nt, 9) float32 corner normals (NaN row: none), or None
 
     @property
     def width(self):

@@ -242,6 +242,16 @@ rt_status rt_scene_add_mesh(rt_scene *scene, const float *xyz, uint32_t nv, cons
  * (0,0), (1,0), (1,1). */
 rt_status rt_scene_add_mesh_uv(rt_scene *scene, const float *xyz, uint32_t nv, const uint32_t *idx, uint32_t nt,
                                const float *uv, uint32_t material, uint32_t *first_id);
+/* The same with vertex normals as well (TriangleMesh::normals with each
+ * face's normal indices, shapes/Triangle.cu:14-100): normals holds 9 floats
+ * per triangle, the normals of its three corners (a NaN first component: a
+ * triangle whose face names no normals); uv may be NULL.  A triangle with
+ * normals shades with the interpolated normal normalize(b0 n0 + b1 n1 +
+ * b2 n2), not turned towards the ray, and its hit point moves 0.15 along it
+ * (Triangle.cu:379-418). */
+rt_status rt_scene_add_mesh_normals(rt_scene *scene, const float *xyz, uint32_t nv, const uint32_t *idx,
+                                    uint32_t nt, const float *uv, const float *normals, uint32_t material,
+                                    uint32_t *first_id);
 /* A texture (textures/Texture.cu): width x height texels of 3 bytes (R, G,
  * B), row y = the reference's pixelMatrix[y] (rows in BMP file order), read
  * nearest-texel at (u * width, v * height) and scaled by 1/255.  id (may be

@@ -266,11 +266,17 @@ public:
 class TriangleMesh : public Shape {
 public:
     TriangleMesh(std::vector<float> xyz, std::vector<uint32_t> idx, std::shared_ptr<Material> m,
-                 std::vector<float> corner_uv = {})
-        : Shape(std::move(m)), vertices(std::move(xyz)), indices(std::move(idx)), uv(std::move(corner_uv)) {}
+                 std::vector<float> corner_uv = {}, std::vector<float> corner_normals = {})
+        : Shape(std::move(m)), vertices(std::move(xyz)), indices(std::move(idx)), uv(std::move(corner_uv)),
+          normals(std::move(corner_normals)) {}
     std::vector<float> vertices;
     std::vector<uint32_t> indices;
-    std::vector<float> uv;  // 6 per triangle (corner texture coordinates), or empty: the default corners
+    std::vector<float> uv;       // 6 per triangle (corner texture coordinates), or empty: the default corners
+    // 9 per triangle (corner normals; a NaN first component: none), or empty:
+    // the mesh has no vertex normals (TriangleMeshData::normals,
+    // shapes/Triangle.cu:379-418: interpolated shading normal, hit point moved
+    // 0.15 along it)
+    std::vector<float> normals;
 };
 
 // Sphere(center, radius) (shapes/Sphere.cu:233-245).  area_light: also
@@ -519,9 +525,11 @@ public:
                 const uint32_t nt = (uint32_t)(t->indices.size() / 3);
                 if (!t->uv.empty() && t->uv.size() != 6 * (size_t)nt)
                     throw Error(RT_ERR_INVALID, "TriangleMesh: uv needs 6 floats per triangle");
-                check(rt_scene_add_mesh_uv(h_, t->vertices.data(), (uint32_t)(t->vertices.size() / 3),
-                                           t->indices.data(), nt, t->uv.empty() ? nullptr : t->uv.data(), m,
-                                           nullptr));
+                if (!t->normals.empty() && t->normals.size() != 9 * (size_t)nt)
+                    throw Error(RT_ERR_INVALID, "TriangleMesh: normals need 9 floats per triangle");
+                check(rt_scene_add_mesh_normals(h_, t->vertices.data(), (uint32_t)(t->vertices.size() / 3),
+                                                t->indices.data(), nt, t->uv.empty() ? nullptr : t->uv.data(),
+                                                t->normals.empty() ? nullptr : t->normals.data(), m, nullptr));
             }
         for (const auto &s : shapes)
             if (auto *sp = dynamic_cast<const Sphere *>(s.get())) {

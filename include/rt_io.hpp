@@ -41,6 +41,7 @@
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
+#include <limits>
 #include <map>
 #include <memory>
 #include <sstream>
@@ -574,8 +575,10 @@ inline std::shared_ptr<Material> make_material(const MtlMaterial &m, TextureCach
 // every face its corners' texture coordinates (a face without them gets the
 // default corners (0,0), (1,0), (1,1), where the reference would read
 // uninitialised indices); a file without them leaves the default corners to
-// every face.  Vertex normals are not used: the GPU path shades with
-// geometric normals.
+// every face.  A file with `vn` lines gives every face its corners' normals
+// (TriangleMeshData::normals: the shading normal is interpolated and the hit
+// point moved 0.15 along it, Triangle.cu:379-418); a face that names no
+// normals gets none (NaN), where the reference would read unset indices.
 inline std::vector<std::shared_ptr<Shape>> obj_shapes(const ObjData &o,
                                                       std::map<std::string, std::shared_ptr<Material>> &materials,
                                                       TextureCache *textures = nullptr)
@@ -587,7 +590,7 @@ inline std::vector<std::shared_ptr<Shape>> obj_shapes(const ObjData &o,
         auto it = materials.find(mname);
         if (it == materials.end())
             it = materials.emplace(mname, make_material(o.materials.at(mname), textures)).first;
-        std::vector<float> xyz, uv;
+        std::vector<float> xyz, uv, nrm;
         std::vector<uint32_t> idx;
         std::map<int32_t, uint32_t> remap;
         for (; i < o.triangles.size() && o.triangles[i].material == mname; ++i) {
@@ -598,6 +601,19 @@ inline std::vector<std::shared_ptr<Shape>> obj_shapes(const ObjData &o,
                 for (int k = 0; k < 3; ++k) {
                     uv.push_back(has ? o.texcoords[(size_t)t.vt[k]].first : kDefault[2 * k]);
                     uv.push_back(has ? o.texcoords[(size_t)t.vt[k]].second : kDefault[2 * k + 1]);
+                }
+            }
+            if (!o.normals.empty()) {
+                const ObjTriangle &t = o.triangles[i];
+                const bool has = t.vn[0] >= 0 && t.vn[1] >= 0 && t.vn[2] >= 0;
+                for (int k = 0; k < 3; ++k) {
+                    if (has) {
+                        const Point3f &n = o.normals[(size_t)t.vn[k]];
+                        nrm.insert(nrm.end(), {n.x, n.y, n.z});
+                    } else {
+                        const float q = std::numeric_limits<float>::quiet_NaN();
+                        nrm.insert(nrm.end(), {q, q, q});
+                    }
                 }
             }
             for (int k = 0; k < 3; ++k) {
@@ -611,7 +627,8 @@ inline std::vector<std::shared_ptr<Shape>> obj_shapes(const ObjData &o,
                 idx.push_back(r->second);
             }
         }
-        out.push_back(std::make_shared<TriangleMesh>(std::move(xyz), std::move(idx), it->second, std::move(uv)));
+        out.push_back(
+            std::make_shared<TriangleMesh>(std::move(xyz), std::move(idx), it->second, std::move(uv), std::move(nrm)));
     }
     return out;
 }

@@ -63,6 +63,7 @@ _SIGS = {
     "orc_scene_add_planes": (None, [_P, _P, _P, _U32]),
     "orc_scene_add_texture": (_U32, [_P, _P, _U32, _U32]),
     "orc_scene_set_tri_uv": (None, [_P, _P]),
+    "orc_scene_set_tri_normals": (None, [_P, _P]),
     "orc_plane_init": (None, [_P, _P, _P, _P, _P]),
     "orc_plane_intersect": (C.c_int, [_P, C.POINTER(OrcRay), C.POINTER(_F), _P, _P, _P]),
     "orc_scene_camera": (None, [_P, _P, _P]),
@@ -395,6 +396,10 @@ class OrcScene:
         if uv is not None and self.idx.shape[0]:
             self.tri_uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 6)
             lib().orc_scene_set_tri_uv(self.h, _p(self.tri_uv))
+        nrm9 = getattr(desc, "tri_nrm", None)
+        if nrm9 is not None and self.idx.shape[0]:
+            self.tri_nrm = np.ascontiguousarray(nrm9, np.float32).reshape(-1, 9)
+            lib().orc_scene_set_tri_normals(self.h, _p(self.tri_nrm))
         planes = getattr(desc, "planes", [])
         if planes:
             self.planes = np.array([[*c, *ax, *ay, *n] for c, ax, ay, n, m in planes], np.float32)

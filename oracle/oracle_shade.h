@@ -81,6 +81,9 @@ uint32_t orc_scene_add_texture(orc_scene *s, const uint8_t *rgb, uint32_t width,
 /* Texture coordinates of every triangle (6 floats each, the scene's
  * triangle order); without them the default (0,0), (1,0), (1,1). */
 void orc_scene_set_tri_uv(orc_scene *s, const float *uv);
+/* Corner normals of every triangle (9 floats each, the scene's triangle
+ * order; a NaN first component: none) of meshes that have vertex normals. */
+void orc_scene_set_tri_normals(orc_scene *s, const float *nrm9);
 void orc_scene_free(orc_scene *s);
 void orc_scene_camera(const orc_scene *s, float r2c[16], float c2w[16]);
 

@@ -529,6 +529,7 @@ struct orc_scene {
     uint32_t *plane_mat;
     uint32_t nplane;
     float *tri_uv;        /* 6 per triangle, or NULL: (0,0), (1,0), (1,1) */
+    float *tri_nrm;       /* 9 per triangle (corner normals, NaN first: none), or NULL */
     uint8_t **tex;        /* ntex textures, 3 bytes per texel */
     uint32_t *tex_w, *tex_h;
     uint32_t ntex;
@@ -597,6 +598,13 @@ uint32_t orc_scene_add_texture(orc_scene *s, const uint8_t *rgb, uint32_t width,
     return s->ntex++;
 }
 
+void orc_scene_set_tri_normals(orc_scene *s, const float *nrm9)
+{
+    free(s->tri_nrm);
+    s->tri_nrm = malloc(sizeof(float) * 9 * (s->nt ? s->nt : 1));
+    memcpy(s->tri_nrm, nrm9, sizeof(float) * 9 * s->nt);
+}
+
 void orc_scene_set_tri_uv(orc_scene *s, const float *uv)
 {
     free(s->tri_uv);
@@ -630,7 +638,7 @@ void orc_scene_free(orc_scene *s)
     if (!s) return;
     free(s->plane); free(s->plane_mat);
     for (uint32_t t = 0; t < s->ntex; ++t) free(s->tex[t]);
-    free(s->tex); free(s->tex_w); free(s->tex_h); free(s->tri_uv);
+    free(s->tex); free(s->tex_w); free(s->tex_h); free(s->tri_uv); free(s->tri_nrm);
     orc_refbvh_free(s->bvh);
     free(s->xyz); free(s->idx_seq); free(s->tri_mat); free(s->mat); free(s->tri_transparent);
     free(s->sph); free(s->sph_mat); free(s->sph_light); free(s->pl); free(s);
@@ -710,6 +718,16 @@ static hit_rec scene_intersect(const orc_scene *s, v3 o, v3 d)
         v3 n = nrm(cross(sub(p0, p2), sub(p1, p2)));
         if (dot(n, d) > 0) n = neg(n);
         h.n = n;
+        if (s->tri_nrm && s->tri_nrm[9 * (size_t)h.id] == s->tri_nrm[9 * (size_t)h.id]) {
+            /* mesh normals (Triangle.cu:379-418): ns = normalize(b0 n0 + b1 n1 +
+             * b2 n2), not turned towards the ray, shades the hit, and the hit
+             * point moves 0.15 along it */
+            const float *q = s->tri_nrm + 9 * (size_t)h.id;
+            v3 ns = nrm(add(add(mul(mk(q[0], q[1], q[2]), bc[0]), mul(mk(q[3], q[4], q[5]), bc[1])),
+                            mul(mk(q[6], q[7], q[8]), bc[2])));
+            h.n = ns;
+            h.p = add(h.p, mul(ns, 0.15f));
+        }
         /* surfacePoint = b0 uv0 + b1 uv1 + b2 uv2 (Triangle.cu:356-358, 134-152) */
         float uvd[6] = {0, 0, 1, 0, 1, 1};
         const float *q = s->tri_uv ? s->tri_uv + 6 * (size_t)h.id : uvd;

@@ -71,6 +71,16 @@ class RenderOpts(C.Structure):
                 ("grid", C.c_uint32), ("chunk_paths", C.c_uint32)]
 
 
+class CacheFrameOpts(C.Structure):
+    _fields_ = [("cache_depth", C.c_uint32), ("harvest_shift", C.c_uint32), ("train_steps", C.c_uint32),
+                ("plain", C.c_uint32)]
+
+
+class CacheFrameStats(C.Structure):
+    _fields_ = [("samples", C.c_uint32), ("steps", C.c_uint32), ("loss", C.c_float), ("fallback", C.c_uint32),
+                ("reset", C.c_uint32)]
+
+
 class RenderStats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("ms", C.c_double), ("extend_ms", C.c_double), ("connect_ms", C.c_double),
@@ -138,6 +148,8 @@ _SIGS = {
     "rt_scene_add_plane": (C.c_int, [_P, _P, _P, _P, _P, _U32]),
     "rt_scene_add_mesh_uv": (C.c_int, [_P, _P, _U32, _P, _U32, _P, _U32, _P]),
     "rt_scene_add_mesh_normals": (C.c_int, [_P, _P, _U32, _P, _U32, _P, _P, _U32, _P]),
+    "rt_render_cache_frame": (C.c_int, [_P, _P, _P, C.POINTER(CacheFrameOpts), _P, _P, C.POINTER(CacheFrameStats),
+                                        _U32, _P]),
     "rt_scene_add_texture": (C.c_int, [_P, _P, _U32, _U32, _P]),
     "rt_scene_add_point_light": (C.c_int, [_P, C.POINTER(PointLight)]),
     "rt_scene_set_camera": (C.c_int, [_P, C.POINTER(Camera)]),
@@ -510,6 +522,24 @@ class Scene:
         self.ctx.launch(lambda s: check(lib().rt_render_cached(self.h, C.byref(o), cache.h, cache_depth, _ptr(accum),
                                                                _ptr(nvalid), fl, s)), stream)
         return film
+
+    def render_cache_frame(self, film, cache, cache_depth=2, harvest_shift=4, train_steps=1, plain=False, spp=None,
+                           spp_offset=0, max_depth=None, rr_depth=None, seed=None, path_capacity=0, stream=None,
+                           profile=False, count=False):
+        """rt_render_cache_frame: one frame whose paths end in the cache,
+        except one in 2^harvest_shift that goes full length and yields a
+        training sample; then train_steps Adam steps on the frame's samples.
+        plain=True: no path uses the cache (all of them harvest).  Returns
+        CacheFrameStats (the call synchronises)."""
+        o = self._opts(spp, spp_offset, max_depth, rr_depth, seed, path_capacity)
+        co = CacheFrameOpts(cache_depth, harvest_shift, train_steps, 1 if plain else 0)
+        st = CacheFrameStats()
+        accum, nvalid = film
+        fl = (RT_FLAG_PROFILE if profile else 0) | (RT_FLAG_COUNT if count else 0)
+        self.ctx.launch(lambda s: check(lib().rt_render_cache_frame(self.h, C.byref(o), cache.h, C.byref(co),
+                                                                    _ptr(accum), _ptr(nvalid), C.byref(st), fl, s)),
+                        stream)
+        return st
 
     def train_cache(self, cache, frame, n_paths=65536, cache_depth=2, max_depth=None, rr_depth=None, seed=None,
                     want_loss=True, stream=None):

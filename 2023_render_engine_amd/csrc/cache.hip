@@ -307,28 +307,31 @@ __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ Y, const
     if (__lane_id() == 0 && loss) atomicAdd(loss, lsum);
 }
 
-// hash-grid gradient: scatter dX[:, 0..2L) through the trilinear weights
+// hash-grid gradient: scatter dX[:, 0..2L) through the trilinear weights.
+// One lane per (level, sample), level-major, so a wavefront scatters one
+// level (uniform dense / hashed indexing) and the chip has L times more
+// independent atomic streams in flight than with one lane per sample.
 __global__ __launch_bounds__(256) void k_grid_bwd(CacheDev c, const float *__restrict__ in, uint32_t n,
                                                   const uint32_t *__restrict__ n_dev, const float *__restrict__ dX,
                                                   float *__restrict__ dgrid)
 {
     if (n_dev) n = min(n, *n_dev);
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const uint64_t total = (uint64_t)n * c.levels;
+    for (uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (uint64_t)gridDim.x * 256) {
+        const uint32_t l = (uint32_t)(t / n), i = (uint32_t)(t - (uint64_t)l * n);
+        const float g0 = dX[(size_t)i * kCacheWidth + 2 * l], g1 = dX[(size_t)i * kCacheWidth + 2 * l + 1];
+        if (g0 == 0.0f && g1 == 0.0f) continue;
         const float *q = in + (size_t)i * kCacheInStride;
         float px, py, pz;
         normalise_pos(c, q, px, py, pz);
-        for (uint32_t l = 0; l < c.levels; ++l) {
-            const float g0 = dX[(size_t)i * kCacheWidth + 2 * l], g1 = dX[(size_t)i * kCacheWidth + 2 * l + 1];
-            if (g0 == 0.0f && g1 == 0.0f) continue;
-            uint32_t idx[8];
-            float w[8];
-            grid_corners(c, l, px, py, pz, idx, w);
-            float *tab = dgrid + (size_t)l * c.table_size * 2;
+        uint32_t idx[8];
+        float w[8];
+        grid_corners(c, l, px, py, pz, idx, w);
+        float *tab = dgrid + (size_t)l * c.table_size * 2;
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                atomicAdd(&tab[2 * (size_t)idx[k]], w[k] * g0);
-                atomicAdd(&tab[2 * (size_t)idx[k] + 1], w[k] * g1);
-            }
+        for (int k = 0; k < 8; ++k) {
+            atomicAdd(&tab[2 * (size_t)idx[k]], w[k] * g0);
+            atomicAdd(&tab[2 * (size_t)idx[k] + 1], w[k] * g1);
         }
     }
 }
@@ -445,7 +448,8 @@ hipError_t cache_backward(const CacheDev &c, const CacheWork &w, const float *in
     hipLaunchKernelGGL((k_wgrad<64>), gw, blk, 0, s, w.act[0], w.dact[cur], n, n_dev, c.grad + kCacheLayerOffset(0));
     hipLaunchKernelGGL((k_bwd_data<64, false, true>), g, blk, 0, s, w.dact[cur], c.w + kCacheLayerOffset(0),
                        w.act[0], n, n_dev, (void *)w.dx32);
-    hipLaunchKernelGGL(k_grid_bwd, grid_for(n, 256), blk, 0, s, c, in, n, n_dev, w.dx32, c.grid_grad);
+    hipLaunchKernelGGL(k_grid_bwd, grid_for(n * c.levels, 256, 8192), blk, 0, s, c, in, n, n_dev, w.dx32,
+                       c.grid_grad);
     return hipGetLastError();
 }
 

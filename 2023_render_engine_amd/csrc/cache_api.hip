@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <limits>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -40,6 +41,13 @@ struct rt_cache {
     void *arena = nullptr;
     float *d_loss = nullptr;
     uint32_t step = 0;
+    // divergence watch over the steps of rt_render_cache_frame: a moving
+    // average of the step losses; a diverged cache is reinitialised and the
+    // next frame renders plain
+    double loss_ema = 0.0;
+    uint32_t watched = 0;
+    uint32_t resets = 0;
+    bool fallback_pending = false;
 };
 
 // accessors for the renderer (scene.hip)
@@ -86,6 +94,36 @@ rt_status upload_params(rt_cache *c, const float *mlp, const float *grid, hipStr
     RT_HIP(hipMemcpyAsync(d.grid_bf16, gb.data(), gp * 2, hipMemcpyHostToDevice, s));
     RT_HIP(hipStreamSynchronize(s));
     return RT_OK;
+}
+
+// Fresh parameters and optimiser state: Glorot-uniform MLP (zero in the
+// unused output columns), grid features U(-1e-4, 1e-4) (Mueller et al. 2022),
+// zero Adam moments and gradients, step count 0.
+rt_status init_params(rt_cache *c, uint32_t seed, hipStream_t s)
+{
+    const rt::CacheDev &d = c->dev;
+    const size_t gp = grid_params(d);
+    rt_status st = RT_OK;
+    float *zero_ptrs[] = {d.grad, d.m, d.v};
+    for (float *p : zero_ptrs)
+        if (rt::cache_fill(p, rt::kCacheParams, 0.0f, s) != hipSuccess) st = RT_ERR_HIP;
+    float *gzero[] = {d.grid_grad, d.grid_m, d.grid_v};
+    for (float *p : gzero)
+        if (rt::cache_fill(p, (uint32_t)gp, 0.0f, s) != hipSuccess) st = RT_ERR_HIP;
+    if (st != RT_OK) { set_error("cache parameter initialisation: fill kernel failed"); return st; }
+    std::vector<float> mlp(rt::kCacheParams, 0.0f), grid(gp);
+    rt::Pcg32 rng = rt::pcg32_seed(seed, 0xcace);
+    for (uint32_t l = 0; l < rt::kCacheLayers; ++l) {
+        const uint32_t off = rt::kCacheLayerOffset(l);
+        const bool last = l + 1 == rt::kCacheLayers;
+        const uint32_t nout = last ? rt::kCacheOut : rt::kCacheWidth, used = last ? 3u : nout;
+        const float a = std::sqrt(6.0f / (float)(rt::kCacheWidth + used));
+        for (uint32_t k = 0; k < rt::kCacheWidth; ++k)
+            for (uint32_t j = 0; j < used; ++j) mlp[off + k * nout + j] = (2.0f * rt::pcg32_float(rng) - 1.0f) * a;
+    }
+    for (size_t i = 0; i < gp; ++i) grid[i] = (2.0f * rt::pcg32_float(rng) - 1.0f) * 1e-4f;
+    c->step = 0;
+    return upload_params(c, mlp.data(), grid.data(), s);
 }
 
 }  // namespace
@@ -161,28 +199,7 @@ rt_status rt_cache_create(rt_ctx *ctx, const rt_cache_opts *opts, rt_cache **out
     w.dact[1] = (uint16_t *)(base + o_da1);
     w.dx32 = (float *)(base + o_dx);
     c->d_loss = (float *)(base + o_loss);
-    // zero optimiser state and gradients with a fill kernel
-    rt_status st = RT_OK;
-    float *zero_ptrs[] = {d.grad, d.m, d.v};
-    for (float *p : zero_ptrs)
-        if (rt::cache_fill(p, rt::kCacheParams, 0.0f, s) != hipSuccess) st = RT_ERR_HIP;
-    float *gzero[] = {d.grid_grad, d.grid_m, d.grid_v};
-    for (float *p : gzero)
-        if (rt::cache_fill(p, (uint32_t)gp, 0.0f, s) != hipSuccess) st = RT_ERR_HIP;
-    // initial parameters: Glorot-uniform MLP (zero in the unused output
-    // columns), grid features U(-1e-4, 1e-4) (Mueller et al. 2022)
-    std::vector<float> mlp(rt::kCacheParams, 0.0f), grid(gp);
-    rt::Pcg32 rng = rt::pcg32_seed(o.seed, 0xcace);
-    for (uint32_t l = 0; l < rt::kCacheLayers; ++l) {
-        const uint32_t off = rt::kCacheLayerOffset(l);
-        const bool last = l + 1 == rt::kCacheLayers;
-        const uint32_t nout = last ? rt::kCacheOut : rt::kCacheWidth, used = last ? 3u : nout;
-        const float a = std::sqrt(6.0f / (float)(rt::kCacheWidth + used));
-        for (uint32_t k = 0; k < rt::kCacheWidth; ++k)
-            for (uint32_t j = 0; j < used; ++j) mlp[off + k * nout + j] = (2.0f * rt::pcg32_float(rng) - 1.0f) * a;
-    }
-    for (size_t i = 0; i < gp; ++i) grid[i] = (2.0f * rt::pcg32_float(rng) - 1.0f) * 1e-4f;
-    if (st == RT_OK) st = upload_params(c, mlp.data(), grid.data(), s);
+    rt_status st = init_params(c, o.seed, s);
     if (st != RT_OK) {
         (void)hipFree(c->arena);
         delete c;
@@ -262,6 +279,64 @@ rt_status cache_train_rows_dev(rt_cache *c, const float *in, const float *target
     RT_HIP(rt::cache_adam(c->dev, od, c->step, s));
     return RT_OK;
 }
+
+// `steps` Adam steps on `total` harvested (record, target) rows: each step
+// takes the next min(batch, total) rows, wrapping around, so a small harvest
+// is revisited rather than padded with empty steps.  Synchronises to read the
+// last step's loss, which feeds the divergence watch: a non-finite loss, or
+// one above 100 times the moving average of the watched steps (after 16 of
+// them), reinitialises the cache with fresh parameters and asks for the next
+// frame to be rendered without it.
+rt_status cache_train_harvest(rt_cache *c, const float *in, const float *target, uint32_t total, uint32_t steps,
+                              hipStream_t s, uint32_t *steps_done, float *mean_loss, bool *reset)
+{
+    *steps_done = 0;
+    *mean_loss = std::numeric_limits<float>::quiet_NaN();
+    *reset = false;
+    if (total == 0 || steps == 0) return RT_OK;
+    const uint32_t n = std::min(c->work.rows, total);
+    uint32_t last = n;
+    for (uint32_t k = 0; k < steps; ++k) {
+        const uint32_t off = (uint32_t)(((uint64_t)k * n) % total);
+        last = std::min(n, total - off);
+        rt_status st = cache_train_rows_dev(c, in + (size_t)off * kCacheInStride, target + (size_t)off * 3, last,
+                                            nullptr, s, nullptr);
+        if (st != RT_OK) return st;
+    }
+    float l = 0;
+    RT_HIP(hipMemcpyAsync(&l, c->d_loss, 4, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    *steps_done = steps;
+    const float mean = l / (3.0f * (float)last);
+    *mean_loss = mean;
+    bool diverged = !std::isfinite(mean);
+    if (!diverged && c->watched >= 16 && (double)mean > 100.0 * c->loss_ema) diverged = true;
+    if (!diverged) {
+        c->loss_ema = c->watched ? 0.9 * c->loss_ema + 0.1 * (double)mean : (double)mean;
+        ++c->watched;
+        return RT_OK;
+    }
+    ++c->resets;
+    rt_status st = init_params(c, c->opts.seed + 0x9E3779B1u * c->resets, s);
+    if (st != RT_OK) return st;
+    c->loss_ema = 0.0;
+    c->watched = 0;
+    c->fallback_pending = true;
+    *reset = true;
+    return RT_OK;
+}
+
+// The frame about to render falls back to plain path tracing when the cache
+// was reset after diverging (the flag is consumed).
+bool cache_take_fallback(rt_cache *c)
+{
+    const bool f = c->fallback_pending;
+    c->fallback_pending = false;
+    return f;
+}
+
+uint32_t cache_resets(const rt_cache *c) { return c->resets; }
+
 }  // namespace rt
 
 extern "C" {

@@ -179,8 +179,15 @@ __device__ __forceinline__ void init_path(const SceneDev &sc0, const BatchDev &b
     p.pix[slot] = pix;
     p.gidx[slot] = k;
     // depth 0; a camera ray counts as delta for MIS (bit 16) and is a camera
-    // ray (bit 17, Ray::cameraRay)
-    p.meta[slot] = (1u << 16) | (1u << 17);
+    // ray (bit 17, Ray::cameraRay); harvest paths carry bit 18
+    uint32_t harvest = 0;
+    if (!B && wp.harvest_shift != kNoHarvest) {
+        // Fibonacci hash of the global path index: its top harvest_shift bits
+        // are zero for one path in 2^harvest_shift
+        const uint32_t h = (uint32_t)((g * 0x9E3779B97F4A7C15ull) >> 32);
+        harvest = (wp.harvest_shift == 0 || (h >> (32u - wp.harvest_shift)) == 0) ? kMetaHarvest : 0u;
+    }
+    p.meta[slot] = (1u << 16) | (1u << 17) | harvest;
     p.ppdf[slot] = 0;
     p.ppx[slot] = o.x; p.ppy[slot] = o.y; p.ppz[slot] = o.z;
     p.nee_mask[slot] = 0;
@@ -389,8 +396,9 @@ __global__ __launch_bounds__(kBlock) void k_classify(SceneDev sc0, BatchDev bt, 
                 }
             }
             // training paths end in the cache only after their training record
-            const bool deep = cache_depth != kNoCache && (p.meta[slot] & 0xffffu) >= cache_depth &&
-                              (!need_record || p.tvalid[slot] != 0);
+            const uint32_t meta = p.meta[slot];
+            const bool deep = cache_depth != kNoCache && (meta & 0xffffu) >= cache_depth &&
+                              (!need_record || p.tvalid[slot] != 0) && !(meta & kMetaHarvest);
             cls = classify(sc, hid, deep);
         }
 #pragma unroll
@@ -638,7 +646,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
                 // training paths remember their first cacheable vertex at or
                 // beyond the cache depth: the radiance gathered from here on,
                 // divided by the throughput on arrival, is the cache's target
-                if (!B && kScatter && wp.train && depth >= wp.cache_depth && p.tvalid[slot] == 0 && cacheable(m)) {
+                if (!B && kScatter && (wp.train || (meta & kMetaHarvest)) && depth >= wp.cache_depth &&
+                    p.tvalid[slot] == 0 && cacheable(m)) {
                     write_record(p.tin + (size_t)slot * 12, s, d, m);
                     p.tLb[3 * (size_t)slot] = L.x; p.tLb[3 * (size_t)slot + 1] = L.y; p.tLb[3 * (size_t)slot + 2] = L.z;
                     p.tB[3 * (size_t)slot] = beta.x; p.tB[3 * (size_t)slot + 1] = beta.y; p.tB[3 * (size_t)slot + 2] = beta.z;
@@ -780,7 +789,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
                 // a camera ray stays one through specular hits (dielectric,
                 // mirror, Trowbridge: SurfaceInteraction.cu:252-254)
                 const uint32_t camera = (CLS != kQLambert) ? (meta >> 17) & 1u : 0u;
-                p.meta[slot] = (depth + 1) | ((delta ? 1u : 0u) << 16) | (camera << 17);
+                p.meta[slot] = (depth + 1) | ((delta ? 1u : 0u) << 16) | (camera << 17) | (meta & kMetaHarvest);
             }
             p.bx[slot] = beta.x; p.by[slot] = beta.y; p.bz[slot] = beta.z;
             p.lx[slot] = L.x; p.ly[slot] = L.y; p.lz[slot] = L.z;
@@ -889,7 +898,8 @@ __global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc0, BatchDev bt, P
 // 0), compacted with its record into cin / ttgt; non-finite targets are
 // dropped.
 // ---------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_finalize(PathsDev p, QueuesDev q, float *res, uint32_t train)
+__global__ __launch_bounds__(kBlock) void k_finalize(PathsDev p, QueuesDev q, float *res, uint32_t train,
+                                                    uint32_t harvest, uint32_t cap)
 {
     const StripeSpan sp = stripe_span();
     const uint32_t n = *lcount(q, kLTerm, sp.s);
@@ -899,13 +909,11 @@ __global__ __launch_bounds__(kBlock) void k_finalize(PathsDev p, QueuesDev q, fl
         const bool live = i < n;
         const uint32_t slot = live ? term[i] : 0;
         if (live) resolve_nee(p, slot);
-        if (!train) {
-            if (live) {
-                float *r = res + 3 * (size_t)p.gidx[slot];
-                r[0] = p.lx[slot]; r[1] = p.ly[slot]; r[2] = p.lz[slot];
-            }
-            continue;  // wavefront-uniform (train is a kernel argument)
+        if (!train && live) {
+            float *r = res + 3 * (size_t)p.gidx[slot];
+            r[0] = p.lx[slot]; r[1] = p.ly[slot]; r[2] = p.lz[slot];
         }
+        if (!train && !harvest) continue;  // wavefront-uniform (kernel arguments)
         bool ok = false;
         float tg[3] = {0, 0, 0};
         if (live && p.tvalid[slot]) {
@@ -918,7 +926,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(PathsDev p, QueuesDev q, fl
             }
         }
         const uint32_t k = wave_append(scount(q, kCTrain), ok);
-        if (ok) {
+        if (ok && k < cap) {
             const float4 *src = (const float4 *)(p.tin + (size_t)slot * 12);
             float4 *dst = (float4 *)(p.cin + (size_t)k * 12);
             dst[0] = src[0]; dst[1] = src[1]; dst[2] = src[2];
@@ -1084,7 +1092,9 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
     // at the next cacheable vertex, whose cache answer completes their
     // target (self-training on the cache's own prediction).
     const bool lookups = !B && wp.cache_depth != kNoCache && c.cache && c.cwork;
-    if (!wp.train && wp.cache_depth != kNoCache && !lookups) return hipErrorInvalidValue;
+    // a cache depth without lookups: only a plain wave that harvests records
+    if (!wp.train && wp.cache_depth != kNoCache && !lookups && wp.harvest_shift == kNoHarvest)
+        return hipErrorInvalidValue;
     if (B && wp.train) return hipErrorInvalidValue;
     if (lookups && c.cwork->rows < c.capacity) return hipErrorInvalidValue;
     if (iterations & 1u) return hipErrorInvalidValue;
@@ -1144,7 +1154,9 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
             if (e != hipSuccess) return e;
             hipLaunchKernelGGL(k_cache_apply, grid, blk, 0, s, paths, q);
         }
-        hipLaunchKernelGGL(k_finalize, grid, blk, 0, s, paths, q, c.res, wp.train);
+        const uint32_t harvest = wp.harvest_shift != kNoHarvest ? 1u : 0u;
+        const uint32_t cap = wp.train ? c.capacity : (harvest ? wp.harvest_cap : 0u);
+        hipLaunchKernelGGL(k_finalize, grid, blk, 0, s, paths, q, c.res, wp.train, harvest, cap);
         if (B) hipLaunchKernelGGL(k_regen<true>, grid, blk, 0, s, sc, bt, paths, q, wp, nxt, 0u, c.capacity);
         else hipLaunchKernelGGL(k_regen<false>, grid, blk, 0, s, sc, bt, paths, q, wp, nxt, 0u, c.capacity);
         hipLaunchKernelGGL(k_regen_commit, dim3(1), dim3(kStripes), 0, s, q, wp, nxt, 0u, c.capacity);

@@ -159,7 +159,17 @@ struct WaveParams {
     uint32_t grid;              // workgroups of the queue kernels (0: default)
     const uint32_t *spp_offset_dev;  // if set, the first sample index is read here (graph replays)
     const uint64_t *chunk_dev;  // if set, {first_path, count} are read here (graph replays)
+    // harvesting (rendering waves): one path in 2^harvest_shift (by a hash of
+    // its global index; kNoHarvest: none) ignores the cache, and its first
+    // cacheable vertex at cache_depth or deeper becomes a training record;
+    // k_finalize compacts up to harvest_cap records into cin / ttgt
+    uint32_t harvest_shift = 0xffffffffu;  // kNoHarvest
+    uint32_t harvest_cap = 0;
 };
+constexpr uint32_t kNoHarvest = 0xffffffffu;
+// path meta word: depth (bits 0-15), previous bounce delta (16), camera ray
+// (17), harvest path (18)
+constexpr uint32_t kMetaHarvest = 1u << 18;
 
 // Host view of a batch for one wave launch: the device tables plus, per
 // scene, the path-index base (n + 1 entries), pixel count and film.

@@ -32,6 +32,9 @@ DevBvh bvh_device(const rt_bvh *b);
 const CacheDev &cache_device(const rt_cache *c);
 rt_ctx *cache_ctx(const rt_cache *c);
 uint32_t cache_batch(const rt_cache *c);
+rt_status cache_train_harvest(rt_cache *c, const float *in, const float *target, uint32_t total, uint32_t steps,
+                              hipStream_t s, uint32_t *steps_done, float *mean_loss, bool *reset);
+bool cache_take_fallback(rt_cache *c);
 rt_status cache_train_rows_dev(rt_cache *c, const float *in, const float *target, uint32_t n, const uint32_t *n_dev,
                                hipStream_t s, float **loss_dev);
 }  // namespace rt
@@ -751,9 +754,14 @@ rt_status rt_scene_camera_ray(const rt_scene *s, uint32_t px, uint32_t py, float
     return RT_OK;
 }
 
-// Shared body of rt_render and rt_render_cached.
+// Shared body of rt_render, rt_render_cached and rt_render_cache_frame.
+// harvest_shift != kNoHarvest: one path in 2^harvest_shift ignores the cache
+// and records its first cacheable vertex at cache_depth or deeper (with no
+// cache, every path renders plain and those paths still record); *harvested
+// receives the records of the (last) chunk, at most the wave capacity.
 static rt_status render_frames(const char *fn, rt_scene *s, const rt_render_opts *opts, rt_cache *cache,
-                               uint32_t cache_depth, double *accum, uint32_t *nvalid, uint32_t flags, void *stream_)
+                               uint32_t cache_depth, double *accum, uint32_t *nvalid, uint32_t flags, void *stream_,
+                               uint32_t harvest_shift = rt::kNoHarvest, uint32_t *harvested = nullptr)
 {
     const std::string name(fn);
     if (!s || !opts || !accum || !nvalid) { set_error(name + ": null argument"); return RT_ERR_INVALID; }
@@ -807,10 +815,17 @@ static rt_status render_frames(const char *fn, rt_scene *s, const rt_render_opts
         wp.max_depth = opts->max_depth;
         wp.rr_depth = opts->rr_depth;
         wp.stream_base = s->stream_base;
-        wp.cache_depth = cache ? cache_depth : rt::kNoCache;
+        wp.cache_depth = (cache || harvest_shift != rt::kNoHarvest) ? cache_depth : rt::kNoCache;
         wp.train = 0;
         wp.grid = opts->grid;
+        wp.harvest_shift = harvest_shift;
+        wp.harvest_cap = cl.capacity;
         RT_HIP(rt::render_chunk(cl, film, wp, s->wb.host_counters, stream));
+        if (harvested) {
+            // the chunk's last counter read-back came after its final finalize
+            const uint32_t k = s->wb.host_counters[rt::kCTrain * rt::kCtrStride];
+            *harvested = std::min(k, cl.capacity);
+        }
     }
     unsigned long long counts[rt::kNumStats];
     RT_HIP(hipMemcpyAsync(counts, s->wb.queues.stats, sizeof counts, hipMemcpyDeviceToHost, stream));
@@ -832,6 +847,45 @@ rt_status rt_render_cached(rt_scene *s, const rt_render_opts *opts, rt_cache *ca
 {
     if (!cache) { set_error("rt_render_cached: null cache"); return RT_ERR_INVALID; }
     return render_frames("rt_render_cached", s, opts, cache, cache_depth, accum, nvalid, flags, stream);
+}
+
+rt_status rt_render_cache_frame(rt_scene *s, const rt_render_opts *opts, rt_cache *cache,
+                                const rt_cache_frame_opts *copts, double *accum, uint32_t *nvalid,
+                                rt_cache_frame_stats *stats, uint32_t flags, void *stream_)
+{
+    if (stats) *stats = rt_cache_frame_stats{};
+    if (!cache || !copts) { set_error("rt_render_cache_frame: null cache or options"); return RT_ERR_INVALID; }
+    if (copts->cache_depth >= 0xffffu) { set_error("rt_render_cache_frame: cache_depth must be below 65535"); return RT_ERR_INVALID; }
+    if (s && rt::cache_ctx(cache) != s->ctx) { set_error("rt_render_cache_frame: cache and scene belong to different contexts"); return RT_ERR_INVALID; }
+    if (!s) { set_error("rt_render_cache_frame: null scene"); return RT_ERR_INVALID; }
+    std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(s->ctx));
+    RT_HIP(hipSetDevice(rt::ctx_device(s->ctx)));
+    hipStream_t stream = stream_ ? (hipStream_t)stream_ : (hipStream_t)rt::ctx_stream_ptr(s->ctx);
+    // a cache reset after diverging sits this frame out: plain path tracing,
+    // every path harvested for its fresh parameters
+    const bool fallback = rt::cache_take_fallback(cache);
+    const bool plain = copts->plain != 0 || fallback;
+    const uint32_t shift = plain ? 0u : (copts->harvest_shift >= 32 ? rt::kNoHarvest : copts->harvest_shift);
+    uint32_t harvested = 0;
+    rt_status st = render_frames("rt_render_cache_frame", s, opts, plain ? nullptr : cache, copts->cache_depth, accum,
+                                 nvalid, flags, stream_, shift, &harvested);
+    if (st != RT_OK) return st;
+    uint32_t done = 0;
+    float loss = std::numeric_limits<float>::quiet_NaN();
+    bool reset = false;
+    if (shift != rt::kNoHarvest) {
+        st = rt::cache_train_harvest(cache, s->wb.paths.cin, s->wb.paths.ttgt, harvested, copts->train_steps, stream,
+                                     &done, &loss, &reset);
+        if (st != RT_OK) return st;
+    }
+    if (stats) {
+        stats->samples = harvested;
+        stats->steps = done;
+        stats->loss = loss;
+        stats->fallback = fallback ? 1u : 0u;
+        stats->reset = reset ? 1u : 0u;
+    }
+    return RT_OK;
 }
 
 rt_status rt_cache_train_scene(rt_cache *c, rt_scene *s, const rt_render_opts *opts, uint32_t cache_depth,

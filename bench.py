@@ -338,23 +338,23 @@ def _rmse_dev(acc, nv, ref_img):
 
 def run_c4(args):
     """Time to RMSE <= target against a high-spp FP64 reference, plain path
-    tracing vs the online-trained radiance cache, on the procedural scene.
+    tracing vs path tracing with the online-trained radiance cache, on the
+    procedural scene at 1920x1080.
 
-    Frames are 4 spp each.  Plain: frames accumulate until the image's RMSE
-    against the reference reaches the target.  Cache: every frame first
-    takes one training step on 65,536 radiance samples (rt_cache_train_scene,
-    cache after 2 bounces), then renders its 4 spp with paths ending in the
-    cache (rt_render_cached); the first `warm` frames only train.  Only the
-    training paths that reach a cacheable vertex at the cache depth yield a
-    sample (about one in ten here: most leave the open scene first), so the
-    first step traces 65,536 paths to measure the yield and later steps
-    trace enough paths for 65,536 samples (the excess over the batch is
-    dropped).  The timed
-    region holds every render and training call (device-synchronised per
-    frame); the RMSE evaluation between frames is not timed.  The reference
-    is the plain renderer with FP64 accumulation at ref_spp (its own
-    parity against the CPU restatement of the reference estimator is tested
-    separately), with sample indices disjoint from both runs."""
+    Frames are 4 spp each and accumulate into one film.  Plain: rt_render
+    frames until the image's RMSE against the reference reaches the target.
+    Cache (rt_render_cache_frame): the cache trains inside the frames it
+    renders, with no separate training paths.  The first --c4-warm frames
+    are plain path tracing (their samples count towards the image) whose
+    paths all yield training samples, followed by --c4-warm-steps Adam steps
+    each; after them, paths end in the cache at their first cache vertex two
+    bounces in, except one in 2^--c4-harvest-shift, which goes full length
+    and yields a training sample (a full-path target), and --c4-steps Adam
+    steps follow each frame.  The timed region holds every frame and every
+    training step (device-synchronised per frame); the RMSE evaluation
+    between frames is not timed.  The reference is the plain renderer with
+    FP64 accumulation at --c4-ref-spp, with sample indices disjoint from both
+    runs."""
     import numpy as np
     import torch
 
@@ -364,11 +364,12 @@ def run_c4(args):
     log(f"generating the procedural scene at {W}x{H}")
     desc = scenes.procedural(width=W, height=H, spp=4)
     label = (f"radiance cache on the procedural scene ({desc.idx.shape[0]} triangles, 8 point lights + 1 sphere "
-             f"light, depth 16, RR after 3) at {W}x{H}: 4 spp frames, 65536 training samples/frame, cache after "
-             f"{args.c4_cache_depth} bounces; reference {args.c4_ref_spp} spp FP64")
+             f"light, depth 16, RR after 3) at {W}x{H}: 4 spp frames, cache after {args.c4_cache_depth} bounces, "
+             f"trained inside the frames ({args.c4_warm} plain warm-up frames x {args.c4_warm_steps} steps, then 1 "
+             f"path in 2^{args.c4_harvest_shift} harvested and {args.c4_steps} steps per frame, batch 65536); "
+             f"reference {args.c4_ref_spp} spp FP64")
     ctx = rt.Context(0)
     sc = rt.Scene(ctx, desc)
-    # reference (untimed), in 256-spp chunks at sample offsets far from the runs'
     ref = sc.new_film()
     t0 = time.time()
     for k in range(0, args.c4_ref_spp, 256):
@@ -377,66 +378,68 @@ def run_c4(args):
     log(f"reference {args.c4_ref_spp} spp in {time.time() - t0:.1f} s")
     ref_img = ref[0].view(-1, 3) / ref[1].double().clamp(min=1).unsqueeze(1)
     target = args.c4_target
-    budget = args.c4_budget
+    lo = np.minimum(desc.xyz.min(0), [0.38, 0.38, 0.98]) - 1e-3
+    hi = np.maximum(desc.xyz.max(0), [0.62, 0.62, 1.22]) + 1e-3
+
+    def new_cache():
+        return rt.RadianceCache(ctx, tuple(lo), tuple(hi), levels=16, log2_table=19, batch=65536, seed=1)
+
+    def frame_call(cache, film, frame, **kw):
+        warm = frame < args.c4_warm
+        return sc.render_cache_frame(film, cache, cache_depth=args.c4_cache_depth,
+                                     harvest_shift=args.c4_harvest_shift,
+                                     train_steps=args.c4_warm_steps if warm else args.c4_steps, plain=warm, spp=4,
+                                     spp_offset=4 * frame, **kw)
 
     def run(use_cache):
         film = sc.new_film()
-        cache = None
-        if use_cache:
-            lo = np.minimum(desc.xyz.min(0), [0.38, 0.38, 0.98]) - 1e-3
-            hi = np.maximum(desc.xyz.max(0), [0.62, 0.62, 1.22]) + 1e-3
-            cache = rt.RadianceCache(ctx, tuple(lo), tuple(hi), levels=16, log2_table=19, batch=65536, seed=1)
+        cache = new_cache() if use_cache else None
         curve, spent, frame, reached = [], 0.0, 0, None
-        losses, samples = [], []
-        n_paths = 65536
-        while spent < budget:
+        losses, samples, ext_ms, con_ms, cache_frames = [], [], 0.0, 0.0, 0
+        while spent < args.c4_budget:
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             if use_cache:
-                # the loss is read back (a host round trip) only on a few frames
-                want = frame % 8 == 0
-                l, ns = sc.train_cache(cache, frame, n_paths=n_paths, cache_depth=args.c4_cache_depth,
-                                       want_loss=want)
-                if want:
-                    losses.append((frame, l))
-                    samples.append(ns)
-                if frame == 0:
-                    # paths per step for 65,536 samples at the measured yield (+3 %)
-                    n_paths = min(1 << 22, max(65536, int(65536 * 65536 * 1.03 / max(ns, 1)) // 1024 * 1024))
-                if frame >= args.c4_warm:
-                    sc.render_cached(film, cache, cache_depth=args.c4_cache_depth, spp=4, spp_offset=4 * frame)
+                st = frame_call(cache, film, frame, profile=True)
+                losses.append((frame, float(st.loss)))
+                samples.append(int(st.samples))
             else:
-                sc.render(film, spp=4, spp_offset=4 * frame)
+                sc.render(film, spp=4, spp_offset=4 * frame, profile=True)
             torch.cuda.synchronize()
             spent += time.perf_counter() - t1
+            rs = sc.stats()
+            if not use_cache or frame >= args.c4_warm:
+                ext_ms += rs.extend_ms
+                con_ms += rs.connect_ms
+                cache_frames += 1
             frame += 1
-            if use_cache and frame <= args.c4_warm:
-                continue
             r = _rmse_dev(film[0], film[1], ref_img)
             curve.append((round(spent, 3), round(r, 5)))
             if r <= target and reached is None:
                 reached = spent
                 break
-        if cache is not None:
-            cache.close()
         res = {"time_s": None if reached is None else round(reached, 3), "frames": frame,
                "final_rmse": curve[-1][1] if curve else None, "timed_s": round(spent, 3),
+               "ms_per_frame": round(spent * 1e3 / max(1, frame), 2),
                "curve": curve[:: max(1, len(curve) // 12)] + curve[-1:]}
         if losses:
-            # single readings (frame, relative L2 loss before that frame's step), first and last apart
-            res["loss_readings"] = [(f, round(float(v), 4)) for f, v in losses]
-            res["loss_first_last"] = [round(float(losses[0][1]), 4), round(float(losses[-1][1]), 4)]
-            res["training_paths_per_step"] = n_paths
-            res["samples_per_step"] = int(np.mean(samples[1:])) if len(samples) > 1 else int(samples[0])
-            res["samples_first_step"] = int(samples[0])
-        return res
+            res["loss_readings"] = [(f, round(v, 4)) for f, v in losses]
+            res["loss_first_last"] = [round(losses[0][1], 4), round(losses[-1][1], 4)]
+            res["samples_per_frame"] = {"warm_up": samples[0], "cache": samples[-1]}
+        return res, cache, film, (ext_ms, con_ms, cache_frames)
 
     log("plain path tracing")
-    plain = run(False)
+    plain, _, _, _ = run(False)
     log(f"plain: {plain}")
-    log("radiance cache")
-    cached = run(True)
+    log("radiance cache trained inside the frames")
+    cached, cache, film, (ext_ms, con_ms, nprof) = run(True)
     log(f"cache: {cached}")
+    # counting frame of the cache method, over the last timed frame's samples (untimed, no training)
+    cf = sc.new_film()
+    sc.render_cache_frame(cf, cache, cache_depth=args.c4_cache_depth, harvest_shift=args.c4_harvest_shift,
+                          train_steps=0, spp=4, spp_offset=4 * (cached["frames"] - 1), count=True)
+    cst = sc.stats()
+    cache.close()
     tp, tc = plain["time_s"], cached["time_s"]
     ratio = round(tp / tc, 3) if (tp and tc) else None
     out = {
@@ -444,15 +447,47 @@ def run_c4(args):
         "value": ratio,
         "unit": "x (time-to-RMSE, plain / cache)",
         "n_gpus": 1, "steps": cached["frames"], "warmup": args.c4_warm,
-        "ms_per_step": round(cached["timed_s"] * 1e3 / max(1, cached["frames"]), 2),
+        "ms_per_step": cached["ms_per_frame"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 shading, bf16 MFMA cache, FP64 film", "data": "synthetic",
         "config": {"workload": label, "triangles": int(desc.idx.shape[0]), "target_rmse": target},
         "plain": plain, "cache": cached,
+        "roofline": traversal_roofline(cst, max(1, nprof), ext_ms, con_ms,
+                                       note="; the cache frames after the warm-up (their training steps are outside "
+                                            "these kernels)"),
     }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_cache(args, lo, hi)
     sc.close()
     ctx.close()
     return out
+
+
+def cpu_baseline_cache(args, lo, hi):
+    """The oracle's FP64 forward + backward of the same cache (16 levels,
+    2^19 entries, 64-wide MLP) on random records inside the scene's box, one
+    core: training samples per second (the GPU trains 65536 per step)."""
+    import numpy as np
+    import oracle
+    cfg = oracle.cache_cfg(16, 19, 16, 1.38, lo, hi)
+    rng = np.random.default_rng(4)
+    mlp = (rng.random(oracle.CACHE_PARAMS, dtype=np.float32) - 0.5) * 0.2
+    grid = (rng.random(16 * (1 << 19) * 2, dtype=np.float32) - 0.5) * 2e-4
+    spent, done, n = 0.0, 0, 512
+    while spent < min(args.cpu_budget, 10.0):
+        rec = np.zeros((n, 12), np.float32)
+        rec[:, :3] = lo + rng.random((n, 3)) * (np.asarray(hi) - lo)
+        rec[:, 3:6] = [0, 0, 1]
+        rec[:, 6:9] = [0, 0, 1]
+        rec[:, 9] = 1
+        tgt = rng.random((n, 3)).astype(np.float32)
+        t1 = time.time()
+        oracle.cache_loss_grad(cfg, mlp, grid, rec, tgt)
+        spent += time.time() - t1
+        done += n
+    return {"value": round(done / spent, 1), "unit": "training samples/s", "cores": 1, "kind": "port",
+            "sample": f"{done} random records in the scene's box through the oracle's FP64 cache forward and "
+                      f"backward pass (no optimiser step), {spent:.1f} s"}
 
 
 def run_c3(args):
@@ -744,12 +779,16 @@ def main():
                     help="c0/c2: spp of the FP64 reference the frame's RMSE is taken against (0: skip)")
     ap.add_argument("--width", type=int, default=4, choices=[2, 4],
                     help="c1: tree layout (4 = four-wide quantized nodes, 2 = two-wide FP32 nodes)")
-    ap.add_argument("--c4-width", type=int, default=480)
-    ap.add_argument("--c4-height", type=int, default=270)
-    ap.add_argument("--c4-ref-spp", type=int, default=16384)
+    ap.add_argument("--c4-width", type=int, default=1920)
+    ap.add_argument("--c4-height", type=int, default=1080)
+    ap.add_argument("--c4-ref-spp", type=int, default=2048)
     ap.add_argument("--c4-target", type=float, default=1e-2)
-    ap.add_argument("--c4-budget", type=float, default=60.0, help="c4: timed seconds per method at most")
-    ap.add_argument("--c4-warm", type=int, default=20, help="c4: training-only frames before the cache renders")
+    ap.add_argument("--c4-budget", type=float, default=20.0, help="c4: timed seconds per method at most")
+    ap.add_argument("--c4-warm", type=int, default=4, help="c4: plain frames whose paths all train the cache")
+    ap.add_argument("--c4-warm-steps", type=int, default=8, help="c4: Adam steps after each warm-up frame")
+    ap.add_argument("--c4-steps", type=int, default=2, help="c4: Adam steps after each cache frame")
+    ap.add_argument("--c4-harvest-shift", type=int, default=4,
+                    help="c4: one path in 2^shift of a cache frame goes full length and trains the cache")
     ap.add_argument("--c4-cache-depth", type=int, default=2,
                     help="c4: bounces before a path ends in the cache (the workload's definition is 2)")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -547,6 +547,43 @@ rt_status rt_cache_set_params(rt_cache *cache, const float *mlp, const float *gr
  * requested.  Both may be NULL. */
 rt_status rt_render_cached(rt_scene *scene, const rt_render_opts *opts, rt_cache *cache, uint32_t cache_depth,
                            double *accum, uint32_t *nvalid, uint32_t flags, void *stream);
+
+/* One frame of online radiance caching with its training inside the frame
+ * (no separate training paths): rt_render_cached, except that one path in
+ * 2^harvest_shift (chosen by a hash of its index) ignores the cache, goes
+ * its full length like rt_render's paths (so it stays unbiased in the image)
+ * and, at its first cache vertex, records that vertex and the radiance
+ * gathered from there on, divided by the throughput on arrival: a training
+ * sample with a full-path target.  After the frame, train_steps Adam steps
+ * run, each on the next min(batch, samples) samples of the frame (wrapping
+ * around).  plain = 1 renders every path without the cache (a warm-up
+ * frame: plain path tracing whose paths all harvest).  Samples come from
+ * the frame's last chunk, at most the wave capacity of them.
+ *
+ * Divergence: the call reads the last step's loss back (it synchronises the
+ * stream).  A non-finite loss, or one above 100 times the moving average of
+ * the steps before (after 16 of them), reinitialises the cache's parameters
+ * and Adam state, and the next rt_render_cache_frame call renders plain
+ * (stats->fallback = 1), so the plain path tracer stands in while the cache
+ * starts over.  stats may be NULL. */
+typedef struct {
+    uint32_t cache_depth;    /* paths end at their first cache vertex at depth >= cache_depth */
+    uint32_t harvest_shift;  /* one path in 2^harvest_shift trains the cache (>= 32: none) */
+    uint32_t train_steps;    /* Adam steps after the frame (0: none) */
+    uint32_t plain;          /* 1: no path ends in the cache */
+} rt_cache_frame_opts;
+
+typedef struct {
+    uint32_t samples;  /* training samples harvested by the frame */
+    uint32_t steps;    /* Adam steps taken */
+    float loss;        /* mean relative L2 loss of the last step before its update (NaN: no step) */
+    uint32_t fallback; /* 1: rendered plain because the cache was reset after diverging */
+    uint32_t reset;    /* 1: this frame's training diverged and the cache was reinitialised */
+} rt_cache_frame_stats;
+
+rt_status rt_render_cache_frame(rt_scene *scene, const rt_render_opts *opts, rt_cache *cache,
+                                const rt_cache_frame_opts *copts, double *accum, uint32_t *nvalid,
+                                rt_cache_frame_stats *stats, uint32_t flags, void *stream);
 rt_status rt_cache_train_scene(rt_cache *cache, rt_scene *scene, const rt_render_opts *opts,
                                uint32_t cache_depth, uint32_t n_paths, float *loss, uint32_t *n_samples,
                                void *stream);

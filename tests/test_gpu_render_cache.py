@@ -158,3 +158,113 @@ def test_online_training_beats_plain_and_untrained(rt, gpu_ctx):
     assert trained8 ** 2 * t_c < plain8 ** 2 * t_p
     sc.close()
     cache.close()
+
+
+def _eq(a, b):
+    return np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_cache_frame_limit_cases_equal_the_plain_and_cached_renders(rt, gpu_ctx):
+    """rt_render_cache_frame at its limits: a plain frame, and a frame in which
+    every path harvests (shift 0), render rt_render's image bit for bit (the
+    harvest only records); with no path harvesting (shift >= 32) it is
+    rt_render_cached's image; with shift 3 about an eighth of shift 0's paths
+    yield samples.  No training step runs (train_steps = 0), so the constant
+    cache is the same in every frame."""
+    d = scenes.procedural(grid=48, n_spheres=6, subdiv=2, width=96, height=54, spp=4)
+    print("[cache frame] limit cases", flush=True)
+    sc = rt.Scene(gpu_ctx, d)
+    lo, hi = aabb(d)
+    cache = constant_cache(rt, gpu_ctx, lo, hi, C)
+
+    def film_of(fn):
+        f = sc.new_film()
+        out = fn(f)
+        return films(sc, f), out
+    plain, _ = film_of(lambda f: sc.render(f, spp=16, spp_offset=8))
+    cached, _ = film_of(lambda f: sc.render_cached(f, cache, cache_depth=2, spp=16, spp_offset=8))
+    frame = lambda **kw: film_of(lambda f: sc.render_cache_frame(f, cache, cache_depth=2, train_steps=0, spp=16,
+                                                                  spp_offset=8, **kw))
+    a, st_a = frame(harvest_shift=4, plain=True)
+    b, st_b = frame(harvest_shift=0)
+    c, st_c = frame(harvest_shift=40)
+    e, st_e = frame(harvest_shift=3)
+    assert _eq(a, plain) and _eq(b, plain) and _eq(c, cached)
+    assert st_a.samples == st_b.samples > 1000 and st_c.samples == 0
+    assert st_a.steps == 0 and np.isnan(st_a.loss) and st_a.fallback == 0 and st_a.reset == 0
+    assert 0.6 / 8 < st_e.samples / st_b.samples < 1.4 / 8, (st_e.samples, st_b.samples)
+    # a shift-3 frame mixes: its pixels differ from both pure renders
+    assert not _eq(e, plain) and not _eq(e, cached)
+    sc.close()
+    cache.close()
+
+
+def test_cache_frames_train_inside_the_frame(rt, gpu_ctx):
+    """Frames that train the cache on their own paths (plain warm-up frames
+    harvesting every path, then cache frames harvesting one in 16): the loss
+    falls, and against a 1024-spp reference the trained cache's frames end
+    closer than an untrained cache's at the same frame count."""
+    import importlib as il
+    d = scenes.procedural(grid=64, n_spheres=8, subdiv=2, width=128, height=72, spp=4)
+    print("[cache frame] training inside the frames", flush=True)
+    sc = rt.Scene(gpu_ctx, d)
+    lo = np.minimum(d.xyz.min(0), [0.38, 0.38, 0.98]) - 1e-3
+    hi = np.maximum(d.xyz.max(0), [0.62, 0.62, 1.22]) + 1e-3
+    ref = sc.new_film()
+    sc.render(ref, spp=1024, spp_offset=1 << 24)
+    ri = mean_img(*films(sc, ref))
+
+    def run(train):
+        cache = rt.RadianceCache(gpu_ctx, tuple(lo), tuple(hi), levels=16, log2_table=16, batch=16384, seed=1)
+        f = sc.new_film()
+        losses = []
+        for k in range(24):
+            st = sc.render_cache_frame(f, cache, cache_depth=2, harvest_shift=4 if train else 40,
+                                       train_steps=(8 if k < 4 else 2) if train else 0, plain=train and k < 4,
+                                       spp=4, spp_offset=4 * k)
+            if train:
+                assert st.steps > 0 and st.samples > 0 and np.isfinite(st.loss) and st.reset == 0
+                losses.append(st.loss)
+        cache.close()
+        return rt.compute_rmse(mean_img(*films(sc, f)), ri), losses
+    r_trained, losses = run(True)
+    r_untrained, _ = run(False)
+    print(f"[cache frame] rmse after 24 frames: trained {r_trained:.4f}, untrained {r_untrained:.4f}; "
+          f"loss {np.mean(losses[:4]):.3f} -> {np.mean(losses[-4:]):.3f}", flush=True)
+    assert r_trained < r_untrained
+    assert np.mean(losses[-6:]) < np.mean(losses[:4])
+    sc.close()
+
+
+def test_diverged_cache_falls_back_to_plain_path_tracing(rt, gpu_ctx):
+    """A cache whose parameters went to NaN answers NaN, so its frame's
+    training loss is NaN: the cache is reinitialised (finite parameters, the
+    step count restarts) and the next frame is plain path tracing, bit for
+    bit rt_render's image for its samples; the frame after that uses the
+    fresh cache again."""
+    d = scenes.procedural(grid=48, n_spheres=6, subdiv=2, width=96, height=54, spp=4)
+    print("[cache frame] divergence fallback", flush=True)
+    sc = rt.Scene(gpu_ctx, d)
+    lo, hi = aabb(d)
+    cache = rt.RadianceCache(gpu_ctx, lo, hi, levels=8, log2_table=14, batch=4096, seed=3)
+    mlp, grid = cache.get_params()
+    mlp[:] = np.nan
+    cache.set_params(mlp, grid)
+    f1 = sc.new_film()
+    st1 = sc.render_cache_frame(f1, cache, cache_depth=2, harvest_shift=2, train_steps=1, spp=4, spp_offset=0)
+    assert st1.reset == 1 and st1.fallback == 0 and not np.isfinite(st1.loss)
+    mlp2, grid2 = cache.get_params()
+    assert np.isfinite(mlp2).all() and np.isfinite(grid2).all()
+    f2 = sc.new_film()
+    st2 = sc.render_cache_frame(f2, cache, cache_depth=2, harvest_shift=2, train_steps=1, spp=4, spp_offset=4)
+    assert st2.fallback == 1 and st2.reset == 0 and np.isfinite(st2.loss)
+    p2 = sc.new_film()
+    sc.render(p2, spp=4, spp_offset=4)
+    assert _eq(films(sc, f2), films(sc, p2))
+    f3 = sc.new_film()
+    st3 = sc.render_cache_frame(f3, cache, cache_depth=2, harvest_shift=2, train_steps=1, spp=4, spp_offset=8)
+    p3 = sc.new_film()
+    sc.render(p3, spp=4, spp_offset=8)
+    assert st3.fallback == 0 and not _eq(films(sc, f3), films(sc, p3))
+    sc.close()
+    cache.close()

@@ -93,11 +93,24 @@ __device__ __forceinline__ StripeSpan stripe_span()
 }
 
 // The scene of path `slot`: the kernel's scene, or the batch table entry.
+// The record of a path slot and its words.
+__device__ __forceinline__ float4 *path_rec(const PathsDev &p, uint32_t slot)
+{
+    return p.rec + (size_t)slot * kPathQuads;
+}
+__device__ __forceinline__ uint32_t *path_word(const PathsDev &p, uint32_t slot, uint32_t quad, uint32_t w)
+{
+    return (uint32_t *)(path_rec(p, slot) + quad) + w;
+}
+__device__ __forceinline__ uint32_t bits(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ float as_f(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ float as_f(int32_t i) { return __int_as_float(i); }
+
 template <bool B>
 __device__ __forceinline__ const SceneDev &scene_of(const SceneDev &sc0, const BatchDev &bt, const PathsDev &p,
                                                     uint32_t slot)
 {
-    if constexpr (B) return bt.scenes[p.sid[slot]];
+    if constexpr (B) return bt.scenes[*path_word(p, slot, kPqScene, 0)];
     else return sc0;
 }
 
@@ -108,11 +121,12 @@ __device__ __forceinline__ uint64_t stream_base_of(const SceneDev &sc, const Wav
     else return wp.stream_base;
 }
 
-__device__ __forceinline__ Pcg32 load_rng(const PathsDev &p, uint32_t slot, uint64_t stream_base)
+// PCG32 generator of a path from its kPqRng quad (state, pixel)
+__device__ __forceinline__ Pcg32 path_rng(float4 q, uint64_t stream_base)
 {
     Pcg32 g;
-    g.state = p.rng[slot];
-    g.inc = ((stream_base + p.pix[slot]) << 1u) | 1u;
+    g.state = (uint64_t)bits(q.x) | ((uint64_t)bits(q.y) << 32);
+    g.inc = ((stream_base + bits(q.z)) << 1u) | 1u;
     return g;
 }
 
@@ -141,6 +155,7 @@ template <bool B>
 __device__ __forceinline__ void init_path(const SceneDev &sc0, const BatchDev &bt, const PathsDev &p,
                                           const WaveParams &wp, uint32_t slot, uint64_t g, uint32_t k)
 {
+    float4 *rec = path_rec(p, slot);
     uint32_t sid = 0;
     if constexpr (B) {
         // the hint is the scene at the start of g's 2^16-path block; scenes
@@ -148,8 +163,7 @@ __device__ __forceinline__ void init_path(const SceneDev &sc0, const BatchDev &b
         sid = bt.hint[g >> kHintShift];
         while (sid + 1 < bt.n && bt.bases[sid + 1] <= g) ++sid;
         g -= bt.bases[sid];
-        p.sid[slot] = sid;
-        p.root[slot] = (int32_t)bt.root[sid];
+        rec[kPqScene] = make_float4(as_f(sid), as_f(bt.root[sid]), 0, 0);
     }
     const SceneDev &sc = B ? bt.scenes[sid] : sc0;
     const uint32_t npix = B ? sc.cam.width * sc.cam.height : wp.npix;
@@ -171,13 +185,6 @@ __device__ __forceinline__ void init_path(const SceneDev &sc0, const BatchDev &b
     const float u0 = pcg32_float(r), u1 = pcg32_float(r);
     V3 o, d;
     camera_ray(sc.cam, px, py, u0, u1, o, d);
-    p.ox[slot] = o.x; p.oy[slot] = o.y; p.oz[slot] = o.z;
-    p.dx[slot] = d.x; p.dy[slot] = d.y; p.dz[slot] = d.z;
-    p.bx[slot] = 1; p.by[slot] = 1; p.bz[slot] = 1;
-    p.lx[slot] = 0; p.ly[slot] = 0; p.lz[slot] = 0;
-    p.rng[slot] = r.state;
-    p.pix[slot] = pix;
-    p.gidx[slot] = k;
     // depth 0; a camera ray counts as delta for MIS (bit 16) and is a camera
     // ray (bit 17, Ray::cameraRay); harvest paths carry bit 18
     uint32_t harvest = 0;
@@ -187,11 +194,15 @@ __device__ __forceinline__ void init_path(const SceneDev &sc0, const BatchDev &b
         const uint32_t h = (uint32_t)((g * 0x9E3779B97F4A7C15ull) >> 32);
         harvest = (wp.harvest_shift == 0 || (h >> (32u - wp.harvest_shift)) == 0) ? kMetaHarvest : 0u;
     }
-    p.meta[slot] = (1u << 16) | (1u << 17) | harvest;
-    p.ppdf[slot] = 0;
-    p.ppx[slot] = o.x; p.ppy[slot] = o.y; p.ppz[slot] = o.z;
-    p.nee_mask[slot] = 0;
-    p.tvalid[slot] = 0;
+    const uint32_t meta = (1u << 16) | (1u << 17) | harvest;
+    // the whole record but the scene quad, so the line is written in full
+    rec[kPqRay] = make_float4(o.x, o.y, o.z, d.x);
+    rec[kPqHit] = make_float4(d.y, d.z, INFINITY, as_f(-1));
+    rec[kPqBeta] = make_float4(1, 1, 1, as_f(meta));
+    rec[kPqL] = make_float4(0, 0, 0, as_f(k));
+    rec[kPqRng] = make_float4(as_f((uint32_t)r.state), as_f((uint32_t)(r.state >> 32)), as_f(pix), as_f(0u));
+    rec[kPqPrev] = make_float4(0, o.x, o.y, o.z);
+    rec[kPqNee] = make_float4(0, 0, 0, as_f(0u));
 }
 
 // Path regeneration, per stripe.  The free slots of stripe s (all of them at
@@ -324,9 +335,11 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc0, BatchDev bt, Pa
         const uint32_t r = wave_claim(c, lcount(q, kLClaimExt, 0), count, has, stripe);
         if (r != 0xffffffffu) {
             slot = active[stripe * cs + r];
-            const int32_t root = B ? p.root[slot] : 0;
-            trav_begin(s, p.ox[slot], p.oy[slot], p.oz[slot], p.dx[slot], p.dy[slot], p.dz[slot], INFINITY, false,
-                       root);
+            const float4 *rec = path_rec(p, slot);
+            const int32_t root = B ? (int32_t)*path_word(p, slot, kPqScene, 1) : 0;
+            const float4 a = rec[kPqRay];
+            const float2 b = *(const float2 *)(rec + kPqHit);
+            trav_begin(s, a.x, a.y, a.z, a.w, b.x, b.y, INFINITY, false, root);
             has = true;
         }
         if (__ballot(has) == 0) {
@@ -334,8 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc0, BatchDev bt, Pa
             continue;
         }
         if (has && trav_step_any<WIDE, false, STATS>(s, nodes, tris, stack, cap, n_node, n_tri)) {
-            p.hid[slot] = s.hit;
-            p.ht[slot] = s.thit;
+            ((float2 *)(path_rec(p, slot) + kPqHit))[1] = make_float2(s.thit, as_f(s.hit));
             has = false;
         }
     }
@@ -367,10 +379,13 @@ __global__ __launch_bounds__(kBlock) void k_classify(SceneDev sc0, BatchDev bt, 
         if (j < n) {
             slot = q.active[cur][base + j];
             const SceneDev &sc = scene_of<B>(sc0, bt, p, slot);
-            int32_t hid = p.hid[slot];
-            float th = p.ht[slot];
+            float4 *rec = path_rec(p, slot);
+            const float4 qh = rec[kPqHit];
+            int32_t hid = (int32_t)bits(qh.w);
+            float th = qh.z;
             if (sc.nsph || sc.nplane) {
-                const V3 o = mk(p.ox[slot], p.oy[slot], p.oz[slot]), d = mk(p.dx[slot], p.dy[slot], p.dz[slot]);
+                const float4 qr = rec[kPqRay];
+                const V3 o = mk(qr.x, qr.y, qr.z), d = mk(qr.w, qh.x, qh.y);
                 bool changed = false;
                 for (uint32_t k = 0; k < sc.nsph; ++k) {
                     V3 so, sd;
@@ -390,15 +405,12 @@ __global__ __launch_bounds__(kBlock) void k_classify(SceneDev sc0, BatchDev bt, 
                         changed = true;
                     }
                 }
-                if (changed) {
-                    p.hid[slot] = hid;
-                    p.ht[slot] = th;
-                }
+                if (changed) ((float2 *)(rec + kPqHit))[1] = make_float2(th, as_f(hid));
             }
             // training paths end in the cache only after their training record
-            const uint32_t meta = p.meta[slot];
+            const uint32_t meta = *path_word(p, slot, kPqBeta, 3);
             const bool deep = cache_depth != kNoCache && (meta & 0xffffu) >= cache_depth &&
-                              (!need_record || p.tvalid[slot] != 0) && !(meta & kMetaHarvest);
+                              (!need_record || *path_word(p, slot, kPqNee, 3) != 0) && !(meta & kMetaHarvest);
             cls = classify(sc, hid, deep);
         }
 #pragma unroll
@@ -513,11 +525,13 @@ __device__ __forceinline__ void triangle_surface(const SceneDev &sc, int32_t id,
     }
 }
 
-__device__ __forceinline__ void surface_of(const SceneDev &sc, const PathsDev &p, uint32_t slot, Surface &s, V3 &d)
+// The hit surface of a path from its kPqRay and kPqHit quads; d = the ray
+// direction.
+__device__ __forceinline__ void surface_of(const SceneDev &sc, float4 qr, float4 qh, Surface &s, V3 &d)
 {
-    const V3 o = mk(p.ox[slot], p.oy[slot], p.oz[slot]);
-    d = mk(p.dx[slot], p.dy[slot], p.dz[slot]);
-    const int32_t hid = p.hid[slot];
+    const V3 o = mk(qr.x, qr.y, qr.z);
+    d = mk(qr.w, qh.x, qh.y);
+    const int32_t hid = (int32_t)bits(qh.w);
     const uint32_t k = (uint32_t)(-hid - 2);
     if (hid >= 0) {
         triangle_surface(sc, hid, o, d, s);
@@ -551,12 +565,12 @@ __device__ __forceinline__ void surface_of(const SceneDev &sc, const PathsDev &p
     }
 }
 
-// Adds the previous bounce's unoccluded next-event contributions, in light
-// order, times the throughput they were computed with.
-__device__ __forceinline__ void resolve_nee(const PathsDev &p, uint32_t slot)
+// The radiance L after adding the previous bounce's unoccluded next-event
+// contributions (mask: their light slots), in light order, times the
+// throughput nb they were computed with.
+__device__ __forceinline__ V3 resolved_L(const PathsDev &p, uint32_t slot, uint32_t mask, V3 L, V3 nb)
 {
-    uint32_t mask = p.nee_mask[slot];
-    if (mask == 0) return;
+    if (mask == 0) return L;
     V3 Ld = mk(0, 0, 0);
     const float *c = p.nee_c + (size_t)slot * p.nee_stride * 3;
     while (mask) {
@@ -564,11 +578,20 @@ __device__ __forceinline__ void resolve_nee(const PathsDev &p, uint32_t slot)
         mask &= mask - 1;
         Ld = add(Ld, mk(c[3 * l], c[3 * l + 1], c[3 * l + 2]));
     }
-    const V3 nb = mk(p.nbx[slot], p.nby[slot], p.nbz[slot]);
-    p.lx[slot] = p.lx[slot] + nb.x * Ld.x;
-    p.ly[slot] = p.ly[slot] + nb.y * Ld.y;
-    p.lz[slot] = p.lz[slot] + nb.z * Ld.z;
-    p.nee_mask[slot] = 0;
+    return mk(L.x + nb.x * Ld.x, L.y + nb.y * Ld.y, L.z + nb.z * Ld.z);
+}
+
+// resolved_L in place: the record's radiance takes the contributions and its
+// mask is cleared.
+__device__ __forceinline__ void resolve_nee(const PathsDev &p, uint32_t slot)
+{
+    float4 *rec = path_rec(p, slot);
+    const uint32_t mask = bits(rec[kPqRng].w);
+    if (mask == 0) return;
+    const float4 qL = rec[kPqL], qn = rec[kPqNee];
+    const V3 L = resolved_L(p, slot, mask, mk(qL.x, qL.y, qL.z), mk(qn.x, qn.y, qn.z));
+    rec[kPqL] = make_float4(L.x, L.y, L.z, qL.w);
+    *path_word(p, slot, kPqRng, 3) = 0;
 }
 
 // Shadow-ray occlusion by the analytic shapes (Scene.cu:108-150): opaque
@@ -622,24 +645,32 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
         const bool live = i < n;
         const uint32_t slot = live ? queue[i] : 0;
         const SceneDev &sc = scene_of<B>(sc0, bt, p, slot);
+        float4 *rec = path_rec(p, slot);
         bool alive = false;
         Surface s{};
         V3 d = mk(0, 0, 1), t = mk(1, 0, 0), b = mk(0, 1, 0), wo = mk(0, 0, 1);
         rt_material m{};
         Pcg32 g{0, 1};
-        uint32_t meta = 0, depth = 0;
+        uint32_t meta = 0, depth = 0, tvalid = 0, emit = 0;
         V3 beta = mk(0, 0, 0), L = mk(0, 0, 0);
+        float4 qh{}, qL{}, qg{};
         if (live) {
-            resolve_nee(p, slot);
+            // the previous bounce's next-event contributions first
+            qL = rec[kPqL];
+            qg = rec[kPqRng];
+            const float4 qn = rec[kPqNee];
+            L = resolved_L(p, slot, bits(qg.w), mk(qL.x, qL.y, qL.z), mk(qn.x, qn.y, qn.z));
+            tvalid = bits(qn.w);
             if (CLS != kQMiss) {
-                surface_of(sc, p, slot, s, d);
+                const float4 qr = rec[kPqRay], qb = rec[kPqBeta];
+                qh = rec[kPqHit];
+                surface_of(sc, qr, qh, s, d);
                 m = sc.mat[s.mat];
                 if (sc.ntex) apply_texture(sc, s, m);
-                g = load_rng(p, slot, stream_base_of<B>(sc, wp));
-                meta = p.meta[slot];
+                g = path_rng(qg, stream_base_of<B>(sc, wp));
+                meta = bits(qb.w);
                 depth = meta & 0xffffu;
-                beta = mk(p.bx[slot], p.by[slot], p.bz[slot]);
-                L = mk(p.lx[slot], p.ly[slot], p.lz[slot]);
+                beta = mk(qb.x, qb.y, qb.z);
                 onb(s.n, t, b);
                 const V3 wow = neg(d);
                 wo = mk(dot(wow, t), dot(wow, b), dot(wow, s.n));
@@ -647,18 +678,18 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
                 // beyond the cache depth: the radiance gathered from here on,
                 // divided by the throughput on arrival, is the cache's target
                 if (!B && kScatter && (wp.train || (meta & kMetaHarvest)) && depth >= wp.cache_depth &&
-                    p.tvalid[slot] == 0 && cacheable(m)) {
+                    tvalid == 0 && cacheable(m)) {
                     write_record(p.tin + (size_t)slot * 12, s, d, m);
                     p.tLb[3 * (size_t)slot] = L.x; p.tLb[3 * (size_t)slot + 1] = L.y; p.tLb[3 * (size_t)slot + 2] = L.z;
                     p.tB[3 * (size_t)slot] = beta.x; p.tB[3 * (size_t)slot + 1] = beta.y; p.tB[3 * (size_t)slot + 2] = beta.z;
-                    p.tvalid[slot] = 1;
+                    tvalid = 1;
                 }
             }
         }
         if (kScatter) {
             // next-event estimation over every point light (Scene.cu:468-525)
-            uint32_t emit = 0;
-            const int32_t proot = B && live ? p.root[slot] : 0;  // the shadow rays' tree root (batches)
+            // the shadow rays' tree root (batches)
+            const int32_t proot = B && live ? (int32_t)*path_word(p, slot, kPqScene, 1) : 0;
             float *c = p.nee_c + (size_t)slot * p.nee_stride * 3;
             const uint32_t npl = live ? sc.npl : 0u, nal = live ? sc.nlights - sc.npl : 0u;
             const uint32_t npl_w = B ? wave_max(npl) : sc0.npl;
@@ -736,10 +767,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
                     if constexpr (B) sh.root[pos] = proot;
                 }
             }
-            if (live) {
-                p.nbx[slot] = beta.x; p.nby[slot] = beta.y; p.nbz[slot] = beta.z;
-                p.nee_mask[slot] = emit;
-            }
+            // the throughput these estimates were made with
+            if (live) rec[kPqNee] = make_float4(beta.x, beta.y, beta.z, as_f(tvalid));
         }
         if (live && CLS != kQMiss) {
             if (CLS == kQLight && m.type == RT_MAT_PLAIN) {
@@ -751,10 +780,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
                 // getEmittedRadiance (LightMaterial.cu:20-23), MIS-weighted
                 // against the light's own next-event sampling
                 float w = 1;
-                const int32_t hid = p.hid[slot];
+                const int32_t hid = (int32_t)bits(qh.w);
                 if (hid <= -2 && (uint32_t)(-hid - 2) < sc.nsph && sc.sph[-hid - 2].is_light && !((meta >> 16) & 1u)) {
-                    const float plp = sphere_light_pdf(sc.sph[-hid - 2], mk(p.ppx[slot], p.ppy[slot], p.ppz[slot]));
-                    const float pp = p.ppdf[slot];
+                    const float4 qp = rec[kPqPrev];
+                    const float plp = sphere_light_pdf(sc.sph[-hid - 2], mk(qp.y, qp.z, qp.w));
+                    const float pp = qp.x;
                     w = pp * pp / (pp * pp + plp * plp);
                 }
                 const float I = m.intensity;
@@ -779,21 +809,27 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
                 }
             }
             if (alive && depth + 1 >= wp.max_depth) alive = false;
+            uint32_t meta_out = meta;
             if (alive) {
                 const V3 ww = add(add(mul(t, wi.x), mul(b, wi.y)), mul(s.n, wi.z));
                 const V3 no = spawn_origin(s.p, s.perr, s.n, ww);
-                p.ox[slot] = no.x; p.oy[slot] = no.y; p.oz[slot] = no.z;
-                p.dx[slot] = ww.x; p.dy[slot] = ww.y; p.dz[slot] = ww.z;
-                p.ppdf[slot] = pdf;
-                p.ppx[slot] = s.p.x; p.ppy[slot] = s.p.y; p.ppz[slot] = s.p.z;
+                rec[kPqRay] = make_float4(no.x, no.y, no.z, ww.x);
+                rec[kPqHit] = make_float4(ww.y, ww.z, qh.z, qh.w);
+                rec[kPqPrev] = make_float4(pdf, s.p.x, s.p.y, s.p.z);
                 // a camera ray stays one through specular hits (dielectric,
                 // mirror, Trowbridge: SurfaceInteraction.cu:252-254)
                 const uint32_t camera = (CLS != kQLambert) ? (meta >> 17) & 1u : 0u;
-                p.meta[slot] = (depth + 1) | ((delta ? 1u : 0u) << 16) | (camera << 17) | (meta & kMetaHarvest);
+                meta_out = (depth + 1) | ((delta ? 1u : 0u) << 16) | (camera << 17) | (meta & kMetaHarvest);
             }
-            p.bx[slot] = beta.x; p.by[slot] = beta.y; p.bz[slot] = beta.z;
-            p.lx[slot] = L.x; p.ly[slot] = L.y; p.lz[slot] = L.z;
-            p.rng[slot] = g.state;
+            rec[kPqBeta] = make_float4(beta.x, beta.y, beta.z, as_f(meta_out));
+            qg.x = as_f((uint32_t)g.state);
+            qg.y = as_f((uint32_t)(g.state >> 32));
+        }
+        if (live) {
+            // radiance, generator state and the new next-event mask (none
+            // outside the scattering classes: the old one is resolved)
+            rec[kPqL] = make_float4(L.x, L.y, L.z, qL.w);
+            rec[kPqRng] = make_float4(qg.x, qg.y, qg.z, as_f(emit));
         }
         const uint32_t ppos = wave_append(lcount(q, kLActive0 + nxt, sp.s), live && alive);
         if (live && alive) q.active[nxt][base + ppos] = slot;
@@ -825,7 +861,8 @@ __global__ __launch_bounds__(kBlock) void k_cache_vertex(SceneDev sc, PathsDev p
         V3 d = mk(0, 0, 1);
         if (live) {
             resolve_nee(p, slot);
-            surface_of(sc, p, slot, s, d);
+            const float4 *rec = path_rec(p, slot);
+            surface_of(sc, rec[kPqRay], rec[kPqHit], s, d);
         }
         const uint32_t k = row0 + i;
         if (live) {
@@ -875,7 +912,7 @@ __global__ __launch_bounds__(kBlock) void k_connect(SceneDev sc0, BatchDev bt, P
             continue;
         }
         if (has && trav_step_any<WIDE, true, STATS>(s, nodes, tris, stack, cap, n_node, n_tri)) {
-            if (s.occ) atomicAnd(&p.nee_mask[sl >> 5], ~(1u << (sl & 31u)));
+            if (s.occ) atomicAnd(path_word(p, sl >> 5, kPqRng, 3), ~(1u << (sl & 31u)));
             if (STATS) n_occ += s.occ ? 1u : 0u;
             has = false;
         }
@@ -908,16 +945,26 @@ __global__ __launch_bounds__(kBlock) void k_finalize(PathsDev p, QueuesDev q, fl
         const uint32_t i = j0 + threadIdx.x;
         const bool live = i < n;
         const uint32_t slot = live ? term[i] : 0;
-        if (live) resolve_nee(p, slot);
+        // the slot is refilled next, so the resolved radiance stays in registers
+        V3 L = mk(0, 0, 0);
+        uint32_t tvalid = 0;
+        float4 qL{};
+        if (live) {
+            const float4 *rec = path_rec(p, slot);
+            qL = rec[kPqL];
+            const float4 qn = rec[kPqNee];
+            L = resolved_L(p, slot, bits(rec[kPqRng].w), mk(qL.x, qL.y, qL.z), mk(qn.x, qn.y, qn.z));
+            tvalid = bits(qn.w);
+        }
         if (!train && live) {
-            float *r = res + 3 * (size_t)p.gidx[slot];
-            r[0] = p.lx[slot]; r[1] = p.ly[slot]; r[2] = p.lz[slot];
+            float *r = res + 3 * (size_t)bits(qL.w);
+            r[0] = L.x; r[1] = L.y; r[2] = L.z;
         }
         if (!train && !harvest) continue;  // wavefront-uniform (kernel arguments)
         bool ok = false;
         float tg[3] = {0, 0, 0};
-        if (live && p.tvalid[slot]) {
-            const float Le[3] = {p.lx[slot], p.ly[slot], p.lz[slot]};
+        if (live && tvalid) {
+            const float Le[3] = {L.x, L.y, L.z};
             ok = true;
             for (int c = 0; c < 3; ++c) {
                 const float b = p.tB[3 * (size_t)slot + c];
@@ -945,9 +992,9 @@ __global__ __launch_bounds__(kBlock) void k_cache_apply(PathsDev p, QueuesDev q)
     for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
         const uint32_t slot = p.cslot[k];
         const float *c = p.crgb + 3 * (size_t)k;
-        p.lx[slot] = p.lx[slot] + p.bx[slot] * c[0];
-        p.ly[slot] = p.ly[slot] + p.by[slot] * c[1];
-        p.lz[slot] = p.lz[slot] + p.bz[slot] * c[2];
+        float4 *rec = path_rec(p, slot);
+        const float4 qb = rec[kPqBeta], qL = rec[kPqL];
+        rec[kPqL] = make_float4(qL.x + qb.x * c[0], qL.y + qb.y * c[1], qL.z + qb.z * c[2], qL.w);
     }
 }
 

@@ -83,21 +83,30 @@ struct BatchDev {
 constexpr uint32_t kHintShift = 16;
 
 // Structure-of-arrays path state (capacity P).
+// Path state: one 128-byte record (kPathQuads float4, one cache line) per
+// slot.  The kernels of a bounce reach paths through queues whose slots are
+// scattered, so a path's fields share a line: a kernel that touches a path
+// moves whole lines instead of a 32- or 64-byte sector per 4-byte field.
+// Quads of a record (words 4q .. 4q+3):
+//   kPqRay   origin xyz, direction x
+//   kPqHit   direction yz, hit t, hit id (int bits: >= 0 triangle, -1 miss,
+//            <= -2 analytic shape -(k+2): sphere k < nsph, else plane k - nsph)
+//   kPqBeta  throughput xyz, meta (depth | prev_delta << 16 | camera << 17 | harvest << 18)
+//   kPqL     radiance xyz, path index within the chunk (result slot)
+//   kPqRng   PCG32 state (low, high word), pixel, NEE mask (unoccluded light
+//            slots of the last next-event estimation)
+//   kPqPrev  previous BSDF pdf, previous vertex xyz (MIS of sphere lights)
+//   kPqNee   throughput at the last next-event estimation xyz, training
+//            record valid
+//   kPqScene scene of the path, root node of its scene in the batch's pooled
+//            tree (batched waves), two unused words
+enum : uint32_t {
+    kPqRay = 0, kPqHit = 1, kPqBeta = 2, kPqL = 3, kPqRng = 4, kPqPrev = 5, kPqNee = 6, kPqScene = 7,
+    kPathQuads = 8
+};
+
 struct PathsDev {
-    float *ox, *oy, *oz, *dx, *dy, *dz;
-    float *bx, *by, *bz;        // throughput
-    float *lx, *ly, *lz;        // radiance
-    uint64_t *rng;              // PCG32 state (the increment is derived from the pixel)
-    uint32_t *pix;
-    uint32_t *gidx;             // path index within the chunk (result slot)
-    uint32_t *sid;              // scene of the path (batched waves)
-    int32_t *root;              // root node of the path's scene in the batch's pooled tree
-    uint32_t *meta;             // depth | prev_delta << 16
-    float *ppdf, *ppx, *ppy, *ppz;  // previous BSDF pdf and vertex (MIS of sphere lights)
-    int32_t *hid;               // >= 0 triangle, -1 miss, <= -2 analytic shape -(k+2): sphere k < nsph, else plane k - nsph
-    float *ht;
-    float *nbx, *nby, *nbz;     // throughput at the last next-event estimation
-    uint32_t *nee_mask;         // unoccluded light slots of the last NEE
+    float4 *rec;                // kPathQuads per slot
     float *nee_c;               // P * nee_stride * 3: per-light f * Li * w
     uint32_t nee_stride;        // light slots per path (max over a batch's scenes)
     // radiance cache: query records (P x 12) and answers (P x 3) of one
@@ -108,8 +117,7 @@ struct PathsDev {
     // training paths: the record at the cache depth, L and beta on arrival
     float *tin;                 // P x 12
     float *tLb, *tB;            // P x 3 each
-    uint32_t *tvalid;
-    float *ttgt;                // compacted targets (P x 3); records compacted into cin
+    float *ttgt;               // compacted targets (P x 3); records compacted into cin
 };
 
 struct ShadowDev {

@@ -322,17 +322,7 @@ rt_status ensure_wave_buffers(WaveBuffers &w, uint32_t P, uint32_t NL, const cha
     std::vector<Piece> pieces;
     auto add = [&](void **dst, size_t bytes) { pieces.push_back({dst, take(bytes)}); };
     PathsDev &p = w.paths;
-    float **pf[] = {&p.ox, &p.oy, &p.oz, &p.dx, &p.dy, &p.dz, &p.bx, &p.by, &p.bz, &p.lx, &p.ly, &p.lz,
-                    &p.ppdf, &p.ppx, &p.ppy, &p.ppz, &p.ht, &p.nbx, &p.nby, &p.nbz};
-    for (float **f : pf) add((void **)f, (size_t)P * 4);
-    add((void **)&p.rng, (size_t)P * 8);
-    add((void **)&p.pix, (size_t)P * 4);
-    add((void **)&p.gidx, (size_t)P * 4);
-    add((void **)&p.sid, (size_t)P * 4);
-    add((void **)&p.root, (size_t)P * 4);
-    add((void **)&p.meta, (size_t)P * 4);
-    add((void **)&p.hid, (size_t)P * 4);
-    add((void **)&p.nee_mask, (size_t)P * 4);
+    add((void **)&p.rec, (size_t)P * kPathQuads * sizeof(float4));
     add((void **)&p.nee_c, (size_t)P * NL * 12);
     add((void **)&p.cin, (size_t)P * 48);
     add((void **)&p.crgb, (size_t)P * 12);
@@ -341,7 +331,6 @@ rt_status ensure_wave_buffers(WaveBuffers &w, uint32_t P, uint32_t NL, const cha
     add((void **)&p.tin, (size_t)P * 48);
     add((void **)&p.tLb, (size_t)P * 12);
     add((void **)&p.tB, (size_t)P * 12);
-    add((void **)&p.tvalid, (size_t)P * 4);
     add((void **)&p.ttgt, (size_t)P * 12);
     ShadowDev &sh = w.shadow;
     float **sf[] = {&sh.ox, &sh.oy, &sh.oz, &sh.dx, &sh.dy, &sh.dz, &sh.tmax};

@@ -197,10 +197,30 @@ def run_c1(args):
 # ray clears its light bit (a 4 B read-modify-write).  A batch path also
 # reads its scene's root (4 B).  Every node visit reads 64 B and every
 # triangle test 48 B, counted by the kernels' counting variants.
-def traversal_roofline(count_stats, frames, ext_ms, con_ms, batch=False, note=""):
+def measured_traffic(profile, kernel, alg_bytes):
+    """HBM bytes per frame of `kernel` from a committed FETCH_SIZE /
+    WRITE_SIZE profile (copy-scaled), or None when there is none or it was
+    taken on another workload (its algorithmic bytes per frame differ from
+    this run's by more than 1 %)."""
+    if not profile:
+        return None
+    path = os.path.join(ROOT, "profiles", profile)
+    try:
+        k = json.load(open(path))["kernels"][kernel]
+    except (OSError, ValueError, KeyError):
+        return None
+    ref = k.get("algorithmic_bytes_per_frame") or 0
+    if ref <= 0 or abs(alg_bytes - ref) > 0.01 * ref:
+        return None
+    return int(k["hbm_bytes_per_frame"])
+
+
+def traversal_roofline(count_stats, frames, ext_ms, con_ms, batch=False, note="", traffic_profile=None):
     """The roofline object of a path-tracing line: the dominant traversal
     kernel's algorithmic bytes per frame (from a counting frame over the same
-    samples) over its HIP-event time per frame (timed frames)."""
+    samples) over its HIP-event time per frame (timed frames).  traffic: that
+    kernel's HBM bytes per frame from traffic_profile (profiles/*.json) when
+    it covers this workload, else None."""
     c = count_stats
     root = 4 if batch else 0
     ext_b = c.extension_rays * (36 + root) + 64 * c.ext_node_visits + 48 * c.ext_tri_tests
@@ -214,7 +234,7 @@ def traversal_roofline(count_stats, frames, ext_ms, con_ms, batch=False, note=""
     achieved = b / t / 1e9
     return {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(traffic_profile, name, b),
         "kernel": name,
         "bytes_per_frame": int(b), "kernel_ms_per_frame": round(t * 1e3, 3),
         "per_ray": {
@@ -231,7 +251,8 @@ def traversal_roofline(count_stats, frames, ext_ms, con_ms, batch=False, note=""
         "note": ("achieved = the kernel's algorithmic bytes per frame (36 B per extension ray or 32 B per shadow "
                  "ray + 8 B per occluded one, 64 B per node visit, 48 B per triangle test; visits and tests from "
                  "a counting frame over the same samples) / its summed HIP-event time per frame over the timed "
-                 "frames (events on the launch stream around every launch)" + note),
+                 "frames (events on the launch stream around every launch); traffic = its HBM bytes per frame "
+                 "from the FETCH_SIZE / WRITE_SIZE profile of the same workload (null without one)" + note),
     }
 
 
@@ -302,7 +323,9 @@ def run_render(args, which):
         "mrays_shadow": round(shadow / wall / 1e6, 1),
         "rays_per_sample": round((ext + shadow) / samples, 2),
         "rmse_vs_ref": rmse,
-        "roofline": traversal_roofline(cstats, args.steps, ext_ms, con_ms),
+        "roofline": traversal_roofline(cstats, args.steps, ext_ms, con_ms,
+                                       traffic_profile="c2_path_trace_procedural_hbm_traffic.json"
+                                       if which == "c2" else None),
     }
     if not args.no_cpu_baseline:
         import oracle

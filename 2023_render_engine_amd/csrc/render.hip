@@ -92,7 +92,6 @@ __device__ __forceinline__ StripeSpan stripe_span()
     return {s, (blockIdx.x / kStripes) * kBlock, nb * kBlock};
 }
 
-// The scene of path `slot`: the kernel's scene, or the batch table entry.
 // The record of a path slot and its words.
 __device__ __forceinline__ float4 *path_rec(const PathsDev &p, uint32_t slot)
 {
@@ -106,12 +105,34 @@ __device__ __forceinline__ uint32_t bits(float f) { return __float_as_uint(f); }
 __device__ __forceinline__ float as_f(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ float as_f(int32_t i) { return __int_as_float(i); }
 
+// The scene of path `slot`: the kernel's scene, or the batch table entry.
 template <bool B>
 __device__ __forceinline__ const SceneDev &scene_of(const SceneDev &sc0, const BatchDev &bt, const PathsDev &p,
                                                     uint32_t slot)
 {
     if constexpr (B) return bt.scenes[*path_word(p, slot, kPqScene, 0)];
     else return sc0;
+}
+
+// Read-only data at a wavefront-uniform address, read through the constant
+// address space so the compiler issues scalar loads: the scene record of a
+// wavefront whose paths all belong to one scene of a batch, and what that
+// record points to at uniform indices (point lights, light materials).
+// The empty asm statement keeps the constant-space pointer (in scalar
+// registers) opaque, so the casts are not folded back into a global pointer;
+// ptr must be wavefront-uniform.
+template <typename T>
+__device__ __forceinline__ const T &const_ref(const T *ptr)
+{
+    const __attribute__((address_space(4))) T *cp = (const __attribute__((address_space(4))) T *)ptr;
+    asm("" : "+s"(cp));
+    return *(const T *)cp;
+}
+template <bool UNI, typename T>
+__device__ __forceinline__ const T &uniform_ref(const T *ptr)
+{
+    if constexpr (UNI) return const_ref(ptr);
+    else return *ptr;
 }
 
 template <bool B>
@@ -375,43 +396,58 @@ __global__ __launch_bounds__(kBlock) void k_classify(SceneDev sc0, BatchDev bt, 
     // ballots below see the whole wavefront
     for (uint32_t j0 = sp.j0; j0 < n; j0 += sp.step) {
         const uint32_t j = j0 + threadIdx.x;
-        uint32_t slot = 0, cls = 0xffu;
-        if (j < n) {
-            slot = q.active[cur][base + j];
-            const SceneDev &sc = scene_of<B>(sc0, bt, p, slot);
-            float4 *rec = path_rec(p, slot);
-            const float4 qh = rec[kPqHit];
-            int32_t hid = (int32_t)bits(qh.w);
-            float th = qh.z;
-            if (sc.nsph || sc.nplane) {
-                const float4 qr = rec[kPqRay];
-                const V3 o = mk(qr.x, qr.y, qr.z), d = mk(qr.w, qh.x, qh.y);
-                bool changed = false;
-                for (uint32_t k = 0; k < sc.nsph; ++k) {
-                    V3 so, sd;
-                    const float ts = sphere_core(sc.sph[k], o, d, INFINITY, so, sd);
-                    if (ts >= 0 && ts < th) {
-                        th = ts;
-                        hid = -(int32_t)k - 2;
-                        changed = true;
+        const bool live = j < n;
+        const uint32_t slot = live ? q.active[cur][base + j] : 0;
+        // a batch wavefront whose paths share one scene reads its record with
+        // scalar loads (the ray-shape tests loop over that scene's shapes)
+        auto body = [&](const SceneDev &sc, auto uni) -> uint32_t {
+            constexpr bool UNI = decltype(uni)::value;
+                float4 *rec = path_rec(p, slot);
+                const float4 qh = rec[kPqHit];
+                int32_t hid = (int32_t)bits(qh.w);
+                float th = qh.z;
+                if (sc.nsph || sc.nplane) {
+                    const float4 qr = rec[kPqRay];
+                    const V3 o = mk(qr.x, qr.y, qr.z), d = mk(qr.w, qh.x, qh.y);
+                    bool changed = false;
+                    for (uint32_t k = 0; k < sc.nsph; ++k) {
+                        V3 so, sd;
+                        const float ts = sphere_core(sc.sph[k], o, d, INFINITY, so, sd);
+                        if (ts >= 0 && ts < th) {
+                            th = ts;
+                            hid = -(int32_t)k - 2;
+                            changed = true;
+                        }
                     }
-                }
-                for (uint32_t k = 0; k < sc.nplane; ++k) {
-                    float u, v, dz;
-                    const float tp = plane_core(sc.plane[k], o, d, INFINITY, u, v, dz);
-                    if (tp > 0 && tp < th) {  // strict (Plane.cu:88-98)
-                        th = tp;
-                        hid = -(int32_t)(sc.nsph + k) - 2;
-                        changed = true;
+                    for (uint32_t k = 0; k < sc.nplane; ++k) {
+                        float u, v, dz;
+                        const float tp = plane_core(uniform_ref<UNI>(sc.plane + k), o, d, INFINITY, u, v, dz);
+                        if (tp > 0 && tp < th) {  // strict (Plane.cu:88-98)
+                            th = tp;
+                            hid = -(int32_t)(sc.nsph + k) - 2;
+                            changed = true;
+                        }
                     }
+                    if (changed) ((float2 *)(rec + kPqHit))[1] = make_float2(th, as_f(hid));
                 }
-                if (changed) ((float2 *)(rec + kPqHit))[1] = make_float2(th, as_f(hid));
+                // training paths end in the cache only after their training record
+                const uint32_t meta = *path_word(p, slot, kPqBeta, 3);
+                const bool deep = cache_depth != kNoCache && (meta & 0xffffu) >= cache_depth &&
+                                  (!need_record || *path_word(p, slot, kPqNee, 3) != 0) && !(meta & kMetaHarvest);
+                return classify(sc, hid, deep);
+        };
+        uint32_t cls = 0xffu;
+        if constexpr (B) {
+            const uint32_t sid = *path_word(p, slot, kPqScene, 0);
+            const uint32_t s0 = __builtin_amdgcn_readfirstlane(sid);
+            if (__ballot(live && sid != s0) == 0) {
+                const SceneDev &su = const_ref(bt.scenes + (s0 < bt.n ? s0 : 0u));
+                if (live) cls = body(su, std::true_type{});
+            } else if (live) {
+                cls = body(bt.scenes[sid], std::false_type{});
             }
-            // training paths end in the cache only after their training record
-            const uint32_t meta = *path_word(p, slot, kPqBeta, 3);
-            const bool deep = cache_depth != kNoCache && (meta & 0xffffu) >= cache_depth &&
-                              (!need_record || *path_word(p, slot, kPqNee, 3) != 0) && !(meta & kMetaHarvest);
-            cls = classify(sc, hid, deep);
+        } else if (live) {
+            cls = body(sc0, std::true_type{});
         }
 #pragma unroll
         for (uint32_t k = 0; k < kNumQ; ++k) {
@@ -644,197 +680,209 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
         const uint32_t i = j0 + threadIdx.x;
         const bool live = i < n;
         const uint32_t slot = live ? queue[i] : 0;
-        const SceneDev &sc = scene_of<B>(sc0, bt, p, slot);
-        float4 *rec = path_rec(p, slot);
-        bool alive = false;
-        Surface s{};
-        V3 d = mk(0, 0, 1), t = mk(1, 0, 0), b = mk(0, 1, 0), wo = mk(0, 0, 1);
-        rt_material m{};
-        Pcg32 g{0, 1};
-        uint32_t meta = 0, depth = 0, tvalid = 0, emit = 0;
-        V3 beta = mk(0, 0, 0), L = mk(0, 0, 0);
-        float4 qh{}, qL{}, qg{};
-        if (live) {
-            // the previous bounce's next-event contributions first
-            qL = rec[kPqL];
-            qg = rec[kPqRng];
-            const float4 qn = rec[kPqNee];
-            L = resolved_L(p, slot, bits(qg.w), mk(qL.x, qL.y, qL.z), mk(qn.x, qn.y, qn.z));
-            tvalid = bits(qn.w);
-            if (CLS != kQMiss) {
-                const float4 qr = rec[kPqRay], qb = rec[kPqBeta];
-                qh = rec[kPqHit];
-                surface_of(sc, qr, qh, s, d);
-                m = sc.mat[s.mat];
-                if (sc.ntex) apply_texture(sc, s, m);
-                g = path_rng(qg, stream_base_of<B>(sc, wp));
-                meta = bits(qb.w);
-                depth = meta & 0xffffu;
-                beta = mk(qb.x, qb.y, qb.z);
-                onb(s.n, t, b);
-                const V3 wow = neg(d);
-                wo = mk(dot(wow, t), dot(wow, b), dot(wow, s.n));
-                // training paths remember their first cacheable vertex at or
-                // beyond the cache depth: the radiance gathered from here on,
-                // divided by the throughput on arrival, is the cache's target
-                if (!B && kScatter && (wp.train || (meta & kMetaHarvest)) && depth >= wp.cache_depth &&
-                    tvalid == 0 && cacheable(m)) {
-                    write_record(p.tin + (size_t)slot * 12, s, d, m);
-                    p.tLb[3 * (size_t)slot] = L.x; p.tLb[3 * (size_t)slot + 1] = L.y; p.tLb[3 * (size_t)slot + 2] = L.z;
-                    p.tB[3 * (size_t)slot] = beta.x; p.tB[3 * (size_t)slot + 1] = beta.y; p.tB[3 * (size_t)slot + 2] = beta.z;
-                    tvalid = 1;
-                }
-            }
-        }
-        if (kScatter) {
-            // next-event estimation over every point light (Scene.cu:468-525)
-            // the shadow rays' tree root (batches)
-            const int32_t proot = B && live ? (int32_t)*path_word(p, slot, kPqScene, 1) : 0;
-            float *c = p.nee_c + (size_t)slot * p.nee_stride * 3;
-            const uint32_t npl = live ? sc.npl : 0u, nal = live ? sc.nlights - sc.npl : 0u;
-            const uint32_t npl_w = B ? wave_max(npl) : sc0.npl;
-            const uint32_t nal_w = B ? wave_max(nal) : sc0.nlights - sc0.npl;
-            for (uint32_t l = 0; l < npl_w; ++l) {
-                bool e = false;
-                V3 Lw = mk(0, 0, 1), o2 = mk(0, 0, 0);
-                float dist = 0;
-                if (live && l < npl) {
-                    const rt_point_light &pl = sc.pl[l];
-                    const V3 lp = mk(pl.pos[0], pl.pos[1], pl.pos[2]);
-                    const V3 tol = sub(lp, s.p);
-                    Lw = nrm(tol);
-                    const float d2 = dot(tol, tol);
-                    const float I = pl.intensity;
-                    const V3 Li = mk(pl.color[0] * I * (1 / d2), pl.color[1] * I * (1 / d2), pl.color[2] * I * (1 / d2));
-                    if (dot(Lw, s.n) > 0) {  // isOrientedTo (SurfaceInteraction.cu:283)
-                        const V3 wi = mk(dot(Lw, t), dot(Lw, b), dot(Lw, s.n));
-                        const V3 f = bsdf_f_of<CLS == kQLambert>(m, wo, wi);
-                        const float pf = bsdf_pdf_of<CLS == kQLambert>(m, wo, wi);
-                        if (!black(f) && !black(Li)) {
-                            const float w = 1.0f / (1.0f + pf * pf);  // powerHeuristic(1, 1, 1, pf)
-                            c[3 * l] = f.x * Li.x * w;
-                            c[3 * l + 1] = f.y * Li.y * w;
-                            c[3 * l + 2] = f.z * Li.z * w;
-                            dist = len(tol);
-                            o2 = spawn_origin(s.p, s.perr, s.n, Lw);
-                            e = !spheres_occlude(sc, o2, Lw, dist, -1);
-                        }
+        // a batch wavefront whose paths share one scene reads that scene's
+        // record with scalar loads; a mixed one per lane
+        auto body = [&](const SceneDev &sc, auto uni) {
+            constexpr bool UNI = decltype(uni)::value;
+            float4 *rec = path_rec(p, slot);
+            bool alive = false;
+            Surface s{};
+            V3 d = mk(0, 0, 1), t = mk(1, 0, 0), b = mk(0, 1, 0), wo = mk(0, 0, 1);
+            rt_material m{};
+            Pcg32 g{0, 1};
+            uint32_t meta = 0, depth = 0, tvalid = 0, emit = 0;
+            V3 beta = mk(0, 0, 0), L = mk(0, 0, 0);
+            float4 qh{}, qL{}, qg{};
+            if (live) {
+                // the previous bounce's next-event contributions first
+                qL = rec[kPqL];
+                qg = rec[kPqRng];
+                const float4 qn = rec[kPqNee];
+                L = resolved_L(p, slot, bits(qg.w), mk(qL.x, qL.y, qL.z), mk(qn.x, qn.y, qn.z));
+                tvalid = bits(qn.w);
+                if (CLS != kQMiss) {
+                    const float4 qr = rec[kPqRay], qb = rec[kPqBeta];
+                    qh = rec[kPqHit];
+                    surface_of(sc, qr, qh, s, d);
+                    m = sc.mat[s.mat];
+                    if (sc.ntex) apply_texture(sc, s, m);
+                    g = path_rng(qg, stream_base_of<B>(sc, wp));
+                    meta = bits(qb.w);
+                    depth = meta & 0xffffu;
+                    beta = mk(qb.x, qb.y, qb.z);
+                    onb(s.n, t, b);
+                    const V3 wow = neg(d);
+                    wo = mk(dot(wow, t), dot(wow, b), dot(wow, s.n));
+                    // training paths remember their first cacheable vertex at or
+                    // beyond the cache depth: the radiance gathered from here on,
+                    // divided by the throughput on arrival, is the cache's target
+                    if (!B && kScatter && (wp.train || (meta & kMetaHarvest)) && depth >= wp.cache_depth &&
+                        tvalid == 0 && cacheable(m)) {
+                        write_record(p.tin + (size_t)slot * 12, s, d, m);
+                        p.tLb[3 * (size_t)slot] = L.x; p.tLb[3 * (size_t)slot + 1] = L.y; p.tLb[3 * (size_t)slot + 2] = L.z;
+                        p.tB[3 * (size_t)slot] = beta.x; p.tB[3 * (size_t)slot + 1] = beta.y; p.tB[3 * (size_t)slot + 2] = beta.z;
+                        tvalid = 1;
                     }
                 }
-                const size_t pos = sbase + wave_append(lcount(q, kLShadow, sp.s), e);
-                if (e) {
-                    emit |= 1u << l;
-                    sh.ox[pos] = o2.x; sh.oy[pos] = o2.y; sh.oz[pos] = o2.z;
-                    sh.dx[pos] = Lw.x; sh.dy[pos] = Lw.y; sh.dz[pos] = Lw.z;
-                    sh.tmax[pos] = dist;
-                    sh.slot[pos] = (slot << 5) | l;
-                    if constexpr (B) sh.root[pos] = proot;
-                }
             }
-            for (uint32_t a = 0; a < nal_w; ++a) {
-                uint32_t j = 0, l = 0;
-                bool e = false;
-                V3 Lw = mk(0, 0, 1), o2 = mk(0, 0, 0);
-                float dist = 0;
-                if (live && a < nal) {
-                    j = sc.area_light[a];
-                    l = sc.npl + a;
-                    const float a0 = pcg32_float(g), a1 = pcg32_float(g);
-                    float plp;
-                    if (sphere_light_sample(sc.sph[j], s.p, a0, a1, Lw, dist, plp) && dot(Lw, s.n) > 0) {
-                        const V3 wi = mk(dot(Lw, t), dot(Lw, b), dot(Lw, s.n));
-                        const V3 f = bsdf_f_of<CLS == kQLambert>(m, wo, wi);
-                        if (!black(f)) {
+            if (kScatter) {
+                // next-event estimation over every point light (Scene.cu:468-525)
+                // the shadow rays' tree root (batches)
+                const int32_t proot = B && live ? (int32_t)*path_word(p, slot, kPqScene, 1) : 0;
+                float *c = p.nee_c + (size_t)slot * p.nee_stride * 3;
+                const uint32_t npl = live ? sc.npl : 0u, nal = live ? sc.nlights - sc.npl : 0u;
+                const uint32_t npl_w = UNI ? sc.npl : wave_max(npl);
+                const uint32_t nal_w = UNI ? sc.nlights - sc.npl : wave_max(nal);
+                for (uint32_t l = 0; l < npl_w; ++l) {
+                    bool e = false;
+                    V3 Lw = mk(0, 0, 1), o2 = mk(0, 0, 0);
+                    float dist = 0;
+                    if (live && l < npl) {
+                        const rt_point_light &pl = uniform_ref<UNI>(sc.pl + l);
+                        const V3 lp = mk(pl.pos[0], pl.pos[1], pl.pos[2]);
+                        const V3 tol = sub(lp, s.p);
+                        Lw = nrm(tol);
+                        const float d2 = dot(tol, tol);
+                        const float I = pl.intensity;
+                        const V3 Li = mk(pl.color[0] * I * (1 / d2), pl.color[1] * I * (1 / d2), pl.color[2] * I * (1 / d2));
+                        if (dot(Lw, s.n) > 0) {  // isOrientedTo (SurfaceInteraction.cu:283)
+                            const V3 wi = mk(dot(Lw, t), dot(Lw, b), dot(Lw, s.n));
+                            const V3 f = bsdf_f_of<CLS == kQLambert>(m, wo, wi);
                             const float pf = bsdf_pdf_of<CLS == kQLambert>(m, wo, wi);
-                            const rt_material &lm = sc.mat[sc.sph[j].material];
-                            const float w = plp * plp / (plp * plp + pf * pf);
-                            const float k = w / plp;
-                            c[3 * l] = f.x * lm.kd[0] * lm.intensity * k;
-                            c[3 * l + 1] = f.y * lm.kd[1] * lm.intensity * k;
-                            c[3 * l + 2] = f.z * lm.kd[2] * lm.intensity * k;
-                            o2 = spawn_origin(s.p, s.perr, s.n, Lw);
-                            e = !spheres_occlude(sc, o2, Lw, dist * 0.9999f, (int32_t)j);
+                            if (!black(f) && !black(Li)) {
+                                const float w = 1.0f / (1.0f + pf * pf);  // powerHeuristic(1, 1, 1, pf)
+                                c[3 * l] = f.x * Li.x * w;
+                                c[3 * l + 1] = f.y * Li.y * w;
+                                c[3 * l + 2] = f.z * Li.z * w;
+                                dist = len(tol);
+                                o2 = spawn_origin(s.p, s.perr, s.n, Lw);
+                                e = !spheres_occlude(sc, o2, Lw, dist, -1);
+                            }
                         }
                     }
+                    const size_t pos = sbase + wave_append(lcount(q, kLShadow, sp.s), e);
+                    if (e) {
+                        emit |= 1u << l;
+                        sh.ox[pos] = o2.x; sh.oy[pos] = o2.y; sh.oz[pos] = o2.z;
+                        sh.dx[pos] = Lw.x; sh.dy[pos] = Lw.y; sh.dz[pos] = Lw.z;
+                        sh.tmax[pos] = dist;
+                        sh.slot[pos] = (slot << 5) | l;
+                        if constexpr (B) sh.root[pos] = proot;
+                    }
                 }
-                const size_t pos = sbase + wave_append(lcount(q, kLShadow, sp.s), e);
-                if (e) {
-                    emit |= 1u << l;
-                    sh.ox[pos] = o2.x; sh.oy[pos] = o2.y; sh.oz[pos] = o2.z;
-                    sh.dx[pos] = Lw.x; sh.dy[pos] = Lw.y; sh.dz[pos] = Lw.z;
-                    sh.tmax[pos] = dist * 0.9999f;
-                    sh.slot[pos] = (slot << 5) | l;
-                    if constexpr (B) sh.root[pos] = proot;
+                for (uint32_t a = 0; a < nal_w; ++a) {
+                    uint32_t j = 0, l = 0;
+                    bool e = false;
+                    V3 Lw = mk(0, 0, 1), o2 = mk(0, 0, 0);
+                    float dist = 0;
+                    if (live && a < nal) {
+                        j = sc.area_light[a];
+                        l = sc.npl + a;
+                        const float a0 = pcg32_float(g), a1 = pcg32_float(g);
+                        float plp;
+                        if (sphere_light_sample(sc.sph[j], s.p, a0, a1, Lw, dist, plp) && dot(Lw, s.n) > 0) {
+                            const V3 wi = mk(dot(Lw, t), dot(Lw, b), dot(Lw, s.n));
+                            const V3 f = bsdf_f_of<CLS == kQLambert>(m, wo, wi);
+                            if (!black(f)) {
+                                const float pf = bsdf_pdf_of<CLS == kQLambert>(m, wo, wi);
+                                const rt_material &lm = uniform_ref<UNI>(sc.mat + sc.sph[j].material);
+                                const float w = plp * plp / (plp * plp + pf * pf);
+                                const float k = w / plp;
+                                c[3 * l] = f.x * lm.kd[0] * lm.intensity * k;
+                                c[3 * l + 1] = f.y * lm.kd[1] * lm.intensity * k;
+                                c[3 * l + 2] = f.z * lm.kd[2] * lm.intensity * k;
+                                o2 = spawn_origin(s.p, s.perr, s.n, Lw);
+                                e = !spheres_occlude(sc, o2, Lw, dist * 0.9999f, (int32_t)j);
+                            }
+                        }
+                    }
+                    const size_t pos = sbase + wave_append(lcount(q, kLShadow, sp.s), e);
+                    if (e) {
+                        emit |= 1u << l;
+                        sh.ox[pos] = o2.x; sh.oy[pos] = o2.y; sh.oz[pos] = o2.z;
+                        sh.dx[pos] = Lw.x; sh.dy[pos] = Lw.y; sh.dz[pos] = Lw.z;
+                        sh.tmax[pos] = dist * 0.9999f;
+                        sh.slot[pos] = (slot << 5) | l;
+                        if constexpr (B) sh.root[pos] = proot;
+                    }
                 }
+                // the throughput these estimates were made with
+                if (live) rec[kPqNee] = make_float4(beta.x, beta.y, beta.z, as_f(tvalid));
             }
-            // the throughput these estimates were made with
-            if (live) rec[kPqNee] = make_float4(beta.x, beta.y, beta.z, as_f(tvalid));
+            if (live && CLS != kQMiss) {
+                if (CLS == kQLight && m.type == RT_MAT_PLAIN) {
+                    // PlainMaterial::getEmittedRadiance (PlainMaterial.cu:62-75):
+                    // kd towards camera rays, ke towards the others
+                    const float *e = ((meta >> 17) & 1u) ? m.kd : m.ke;
+                    L = add(L, mk(beta.x * e[0], beta.y * e[1], beta.z * e[2]));
+                } else if (CLS == kQLight) {
+                    // getEmittedRadiance (LightMaterial.cu:20-23), MIS-weighted
+                    // against the light's own next-event sampling
+                    float w = 1;
+                    const int32_t hid = (int32_t)bits(qh.w);
+                    if (hid <= -2 && (uint32_t)(-hid - 2) < sc.nsph && sc.sph[-hid - 2].is_light && !((meta >> 16) & 1u)) {
+                        const float4 qp = rec[kPqPrev];
+                        const float plp = sphere_light_pdf(sc.sph[-hid - 2], mk(qp.y, qp.z, qp.w));
+                        const float pp = qp.x;
+                        w = pp * pp / (pp * pp + plp * plp);
+                    }
+                    const float I = m.intensity;
+                    L = add(L, mk(beta.x * m.kd[0] * I * w, beta.y * m.kd[1] * I * w, beta.z * m.kd[2] * I * w));
+                }
+                V3 wi;
+                float pdf;
+                bool delta;
+                constexpr uint32_t MT = CLS == kQLambert ? RT_MAT_LAMBERTIAN
+                                        : CLS == kQDielectric ? RT_MAT_DIELECTRIC
+                                        : CLS == kQGgx ? RT_MAT_TROWBRIDGE : RT_MAT_LIGHT;
+                const V3 wgt = bsdf_sample<MT>(m, wo, g, wi, pdf, delta);
+                beta = hmul(beta, wgt);
+                alive = !black(wgt);
+                if (alive && kScatter && depth > wp.rr_depth) {
+                    const float qq = fmaxf(0.05f, 1 - luminance(beta));
+                    if (pcg32_float(g) < qq) {
+                        alive = false;
+                    } else {
+                        const float sc1 = 1 - qq;
+                        beta = mk(beta.x / sc1, beta.y / sc1, beta.z / sc1);
+                    }
+                }
+                if (alive && depth + 1 >= wp.max_depth) alive = false;
+                uint32_t meta_out = meta;
+                if (alive) {
+                    const V3 ww = add(add(mul(t, wi.x), mul(b, wi.y)), mul(s.n, wi.z));
+                    const V3 no = spawn_origin(s.p, s.perr, s.n, ww);
+                    rec[kPqRay] = make_float4(no.x, no.y, no.z, ww.x);
+                    rec[kPqHit] = make_float4(ww.y, ww.z, qh.z, qh.w);
+                    rec[kPqPrev] = make_float4(pdf, s.p.x, s.p.y, s.p.z);
+                    // a camera ray stays one through specular hits (dielectric,
+                    // mirror, Trowbridge: SurfaceInteraction.cu:252-254)
+                    const uint32_t camera = (CLS != kQLambert) ? (meta >> 17) & 1u : 0u;
+                    meta_out = (depth + 1) | ((delta ? 1u : 0u) << 16) | (camera << 17) | (meta & kMetaHarvest);
+                }
+                rec[kPqBeta] = make_float4(beta.x, beta.y, beta.z, as_f(meta_out));
+                qg.x = as_f((uint32_t)g.state);
+                qg.y = as_f((uint32_t)(g.state >> 32));
+            }
+            if (live) {
+                // radiance, generator state and the new next-event mask (none
+                // outside the scattering classes: the old one is resolved)
+                rec[kPqL] = make_float4(L.x, L.y, L.z, qL.w);
+                rec[kPqRng] = make_float4(qg.x, qg.y, qg.z, as_f(emit));
+            }
+            const uint32_t ppos = wave_append(lcount(q, kLActive0 + nxt, sp.s), live && alive);
+            if (live && alive) q.active[nxt][base + ppos] = slot;
+            const uint32_t tpos = wave_append(lcount(q, kLTerm, sp.s), live && !alive);
+            if (live && !alive) q.term[base + tpos] = slot;
+        };
+        if constexpr (B) {
+            const uint32_t sid = *path_word(p, slot, kPqScene, 0);
+            const uint32_t s0 = __builtin_amdgcn_readfirstlane(sid);
+            if (__ballot(live && sid != s0) == 0) body(const_ref(bt.scenes + (s0 < bt.n ? s0 : 0u)), std::true_type{});
+            else body(bt.scenes[sid], std::false_type{});
+        } else {
+            body(sc0, std::true_type{});
         }
-        if (live && CLS != kQMiss) {
-            if (CLS == kQLight && m.type == RT_MAT_PLAIN) {
-                // PlainMaterial::getEmittedRadiance (PlainMaterial.cu:62-75):
-                // kd towards camera rays, ke towards the others
-                const float *e = ((meta >> 17) & 1u) ? m.kd : m.ke;
-                L = add(L, mk(beta.x * e[0], beta.y * e[1], beta.z * e[2]));
-            } else if (CLS == kQLight) {
-                // getEmittedRadiance (LightMaterial.cu:20-23), MIS-weighted
-                // against the light's own next-event sampling
-                float w = 1;
-                const int32_t hid = (int32_t)bits(qh.w);
-                if (hid <= -2 && (uint32_t)(-hid - 2) < sc.nsph && sc.sph[-hid - 2].is_light && !((meta >> 16) & 1u)) {
-                    const float4 qp = rec[kPqPrev];
-                    const float plp = sphere_light_pdf(sc.sph[-hid - 2], mk(qp.y, qp.z, qp.w));
-                    const float pp = qp.x;
-                    w = pp * pp / (pp * pp + plp * plp);
-                }
-                const float I = m.intensity;
-                L = add(L, mk(beta.x * m.kd[0] * I * w, beta.y * m.kd[1] * I * w, beta.z * m.kd[2] * I * w));
-            }
-            V3 wi;
-            float pdf;
-            bool delta;
-            constexpr uint32_t MT = CLS == kQLambert ? RT_MAT_LAMBERTIAN
-                                    : CLS == kQDielectric ? RT_MAT_DIELECTRIC
-                                    : CLS == kQGgx ? RT_MAT_TROWBRIDGE : RT_MAT_LIGHT;
-            const V3 wgt = bsdf_sample<MT>(m, wo, g, wi, pdf, delta);
-            beta = hmul(beta, wgt);
-            alive = !black(wgt);
-            if (alive && kScatter && depth > wp.rr_depth) {
-                const float qq = fmaxf(0.05f, 1 - luminance(beta));
-                if (pcg32_float(g) < qq) {
-                    alive = false;
-                } else {
-                    const float sc1 = 1 - qq;
-                    beta = mk(beta.x / sc1, beta.y / sc1, beta.z / sc1);
-                }
-            }
-            if (alive && depth + 1 >= wp.max_depth) alive = false;
-            uint32_t meta_out = meta;
-            if (alive) {
-                const V3 ww = add(add(mul(t, wi.x), mul(b, wi.y)), mul(s.n, wi.z));
-                const V3 no = spawn_origin(s.p, s.perr, s.n, ww);
-                rec[kPqRay] = make_float4(no.x, no.y, no.z, ww.x);
-                rec[kPqHit] = make_float4(ww.y, ww.z, qh.z, qh.w);
-                rec[kPqPrev] = make_float4(pdf, s.p.x, s.p.y, s.p.z);
-                // a camera ray stays one through specular hits (dielectric,
-                // mirror, Trowbridge: SurfaceInteraction.cu:252-254)
-                const uint32_t camera = (CLS != kQLambert) ? (meta >> 17) & 1u : 0u;
-                meta_out = (depth + 1) | ((delta ? 1u : 0u) << 16) | (camera << 17) | (meta & kMetaHarvest);
-            }
-            rec[kPqBeta] = make_float4(beta.x, beta.y, beta.z, as_f(meta_out));
-            qg.x = as_f((uint32_t)g.state);
-            qg.y = as_f((uint32_t)(g.state >> 32));
-        }
-        if (live) {
-            // radiance, generator state and the new next-event mask (none
-            // outside the scattering classes: the old one is resolved)
-            rec[kPqL] = make_float4(L.x, L.y, L.z, qL.w);
-            rec[kPqRng] = make_float4(qg.x, qg.y, qg.z, as_f(emit));
-        }
-        const uint32_t ppos = wave_append(lcount(q, kLActive0 + nxt, sp.s), live && alive);
-        if (live && alive) q.active[nxt][base + ppos] = slot;
-        const uint32_t tpos = wave_append(lcount(q, kLTerm, sp.s), live && !alive);
-        if (live && !alive) q.term[base + tpos] = slot;
     }
 }
 

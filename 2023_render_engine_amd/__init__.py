@@ -56,6 +56,15 @@ class Material(C.Structure):
                 ("texture", C.c_uint32)]
 
 
+class PhotonOpts(C.Structure):
+    _fields_ = [("photons", C.c_uint32), ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32), ("seed", C.c_uint32)]
+
+
+class SssParams(C.Structure):
+    _fields_ = [("kd", C.c_float * 3), ("sigma_s", C.c_float * 3), ("sigma_a", C.c_float * 3), ("g", C.c_float),
+                ("eta", C.c_float)]
+
+
 class PointLight(C.Structure):
     _fields_ = [("pos", C.c_float * 3), ("color", C.c_float * 3), ("intensity", C.c_float)]
 
@@ -185,6 +194,9 @@ _SIGS = {
     "rt_photon_map_get_info": (C.c_int, [_P, C.POINTER(_U32), C.POINTER(_U32)]),
     "rt_photon_map_knn": (C.c_int, [_P, _P, _P, _U32, _U32, C.c_float, _P, _P, _P, _P]),
     "rt_photon_gather_lambert": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _U32, _U32, C.c_float, _P, _P]),
+    "rt_photon_gather": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _U32, _U32, C.c_float, _P, _P]),
+    "rt_photon_subsurface": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _U32, C.c_float, _P, _P, _P]),
+    "rt_scene_trace_photons": (C.c_int, [_P, _P, _U32, _P, _P, _P, _P, _P, _P, C.POINTER(_U32), _P]),
     "rt_gen_soup": (C.c_int, [_U32, C.c_uint64, C.c_float, C.c_float, _P]),
     "rt_gen_rays": (C.c_int, [_U32, _U32, _P, C.c_float, _U32, C.c_uint64, _P, _P, _P, _P, _P, _P]),
 }
@@ -481,6 +493,35 @@ class Scene:
             self.close()
         except Exception:
             pass
+
+    def trace_photons(self, photons=1000000, max_depth=30, rr_depth=3, seed=1, capacity=None, stream=None):
+        """rt_scene_trace_photons: the photon pre-pass (generatePhotons +
+        computePhotons).  Returns a dict of device tensors location,
+        direction, normal (n, 3 float32), radiance (n, 3 float64), shape
+        (n, int32) and key (n, int32: path << 5 | bounce), ordered by photon
+        path then bounce.  capacity (default 4 per path) grows once if the
+        pass stores more."""
+        import torch
+        cap = capacity if capacity is not None else 4 * photons
+        for _ in range(2):
+            dev = self.ctx.device
+            out = dict(location=torch.empty((cap, 3), dtype=torch.float32, device=dev),
+                       direction=torch.empty((cap, 3), dtype=torch.float32, device=dev),
+                       normal=torch.empty((cap, 3), dtype=torch.float32, device=dev),
+                       radiance=torch.empty((cap, 3), dtype=torch.float64, device=dev),
+                       shape=torch.empty(cap, dtype=torch.int32, device=dev),
+                       key=torch.empty(cap, dtype=torch.int32, device=dev))
+            o = PhotonOpts(photons, max_depth, rr_depth, seed)
+            n = _U32()
+            st = self.ctx.launch(lambda s: lib().rt_scene_trace_photons(
+                self.h, C.byref(o), cap, _ptr(out["location"]), _ptr(out["direction"]), _ptr(out["normal"]),
+                _ptr(out["radiance"]), _ptr(out["shape"]), _ptr(out["key"]), C.byref(n), s), stream)
+            if st == 0:
+                return {k: v[:n.value] for k, v in out.items()}
+            if n.value <= cap:
+                check(st)
+            cap = n.value
+        check(st)
 
     def new_film(self):
         import torch
@@ -795,6 +836,42 @@ class PhotonMap:
         self.ctx.launch(lambda s: check(lib().rt_photon_gather_lambert(
             self.h, _ptr(photon_rgb), _ptr(photon_dir), _ptr(points), _ptr(normals), _ptr(kd), _ptr(qs), m, k,
             max_distance, _ptr(out), s)), stream)
+        return out
+
+    def gather(self, photon_rgb, photon_dir, points, normals, views, materials, k=100, max_distance=100.0,
+               query_shape=None, stream=None):
+        """combinePhotonContributions for any material (rt_photon_gather):
+        views (m, 3) float32 towards the viewer, materials a sequence of m
+        Material structs (or one for all).  Returns (m, 3) float64."""
+        import torch
+        m = points.shape[0]
+        if isinstance(materials, Material):
+            materials = [materials] * m
+        if len(materials) != m:
+            raise ValueError("one material per query")
+        mats = (Material * max(1, m))(*materials)
+        mt = torch.frombuffer(bytearray(bytes(mats)), dtype=torch.uint8).to(points.device)
+        qs = None if query_shape is None else query_shape.to(device=points.device, dtype=torch.int32).contiguous()
+        out = torch.empty((m, 3), dtype=torch.float64, device=points.device)
+        self.ctx.launch(lambda s: check(lib().rt_photon_gather(
+            self.h, _ptr(photon_rgb), _ptr(photon_dir), _ptr(points), _ptr(normals), _ptr(views), _ptr(mt), _ptr(qs),
+            m, k, max_distance, _ptr(out), s)), stream)
+        return out
+
+    def subsurface(self, photon_rgb, photon_dir, photon_nrm, points, normals, views, params, radius=0.3,
+                   query_shape=None, stream=None):
+        """Scene::combinePhotonsSubsurface (rt_photon_subsurface); params =
+        (kd, sigma_s, sigma_a, g, eta).  Returns (m, 3) float64 (NaN where
+        no photon lies within the radius, as in the reference)."""
+        import torch
+        m = points.shape[0]
+        kd, ss, sa, g, eta = params
+        prm = SssParams((C.c_float * 3)(*kd), (C.c_float * 3)(*ss), (C.c_float * 3)(*sa), g, eta)
+        qs = None if query_shape is None else query_shape.to(device=points.device, dtype=torch.int32).contiguous()
+        out = torch.empty((m, 3), dtype=torch.float64, device=points.device)
+        self.ctx.launch(lambda s: check(lib().rt_photon_subsurface(
+            self.h, _ptr(photon_rgb), _ptr(photon_dir), _ptr(photon_nrm), _ptr(points), _ptr(normals), _ptr(views),
+            _ptr(qs), m, radius, C.byref(prm), _ptr(out), s)), stream)
         return out
 
     def close(self):

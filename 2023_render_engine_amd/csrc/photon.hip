@@ -28,6 +28,7 @@
 
 #include "rt_internal.h"
 #include "rt_photon.h"
+#include "rt_shade.h"
 
 struct rt_ctx;
 namespace rt {
@@ -302,7 +303,193 @@ __global__ __launch_bounds__(256) void k_photon_gather_lambert(const double *rgb
     out[3 * (size_t)q + 2] = acc[2];
 }
 
+// combinePhotonContributions for any material: the photon's radiance times
+// SurfaceInteraction::lightScattered(-photon direction), i.e. the material's
+// lightReflected (f * |cos wi|, the renderer's bsdf_f) with wo the view
+// direction, both in the shading frame of the query's normal; Lambertian and
+// Trowbridge materials only (plain: lightReflected = 0; the rest are not
+// scattering).  Sum in double in the search's order, times 1 / (PI * maxd^2)
+// in float.
+__global__ __launch_bounds__(256) void k_photon_gather_mat(const double *rgb, const float *dir, const float *nrm,
+                                                           const float *view, const rt_material *mats, uint32_t m,
+                                                           uint32_t k, const uint32_t *count, const uint32_t *index,
+                                                           const float *dist, double *out)
+{
+    const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= m) return;
+    const uint32_t c = count[q];
+    const rt_material mt = mats[q];
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (c > 0 && rt::is_scattering(mt.type)) {
+        const rt::V3 n = rt::mk(nrm[3 * (size_t)q], nrm[3 * (size_t)q + 1], nrm[3 * (size_t)q + 2]);
+        rt::V3 t, b;
+        rt::onb(n, t, b);
+        const rt::V3 vw = rt::mk(view[3 * (size_t)q], view[3 * (size_t)q + 1], view[3 * (size_t)q + 2]);
+        const rt::V3 wo = rt::mk(rt::dot(vw, t), rt::dot(vw, b), rt::dot(vw, n));
+        float maxd = 0.0f;
+        for (uint32_t i = 0; i < c; ++i) {
+            const uint32_t e = index[(size_t)q * k + i];
+            maxd = fmaxf(maxd, dist[(size_t)q * k + i]);
+            const rt::V3 w = rt::mk(-dir[3 * (size_t)e], -dir[3 * (size_t)e + 1], -dir[3 * (size_t)e + 2]);
+            const rt::V3 wi = rt::mk(rt::dot(w, t), rt::dot(w, b), rt::dot(w, n));
+            const rt::V3 f = rt::bsdf_f(mt, wo, wi);
+            acc[0] += rgb[3 * (size_t)e] * (double)f.x;
+            acc[1] += rgb[3 * (size_t)e + 1] * (double)f.y;
+            acc[2] += rgb[3 * (size_t)e + 2] * (double)f.z;
+        }
+        const float kernel = 1.0f / (3.14159265358979323846f * maxd * maxd);
+        acc[0] *= (double)kernel; acc[1] *= (double)kernel; acc[2] *= (double)kernel;
+    }
+    out[3 * (size_t)q] = acc[0];
+    out[3 * (size_t)q + 1] = acc[1];
+    out[3 * (size_t)q + 2] = acc[2];
+}
+
+// impPaper_dipoleDiffusionAproximation (geometricMath.cu:451-483) for one
+// channel: float scalars widened where the reference's Spectrum (double)
+// meets them.
+__device__ inline double dipole_rd(double ss, double sa, float g, float fdr, float A, float r)
+{
+    const double ssp = (double)(1.0f - g) * ss;
+    const double stp = ssp + sa;
+    const double alpha = ssp / stp;
+    const double str = sqrt(3.0 * sa * stp);
+    const double zr = sqrt(3.0 * (1.0 - alpha)) / str;
+    const double zv = zr * (double)A;
+    const double r2 = (double)(r * r);
+    const double dr = sqrt(r2 + zr * zr), dv = sqrt(r2 + zv * zv);
+    const double sdr = str * dr, sdv = str * dv;
+    const double rd = (sdr + 1.0) * exp(-sdr) * zr / (dr * dr * dr) + (sdv + 1.0) * exp(-sdv) * zv / (dv * dv * dv);
+    const double c1 = zr * (str + 1.0 / dr);
+    return c1 * rd * (double)(1.0f - fdr) * alpha;
+}
+
+// cosine(v1, v2) (geometricMath.cu:139-142)
+__device__ inline float cosine3(float ax, float ay, float az, float bx, float by, float bz)
+{
+    const float d = ax * bx + ay * by + az * bz;
+    return d / (sqrtf(ax * ax + ay * ay + az * az) * sqrtf(bx * bx + by * by + bz * bz));
+}
+
+struct SssArgs {
+    const uint32_t *perm;
+    const rt::PhotonNode *nodes;
+    const uint32_t *order;
+    uint32_t n;
+    const double *rgb;
+    const float *pdir, *pnrm;
+    const float *queries, *qnrm, *qview;
+    const int32_t *qshape;
+    uint32_t m;
+    float radius;
+    rt_sss_params prm;
+    double *out;
+};
+
+// Scene::combinePhotonsSubsurface over every photon strictly within the
+// radius on the query's shape (the unlimited search never forms a heap, so
+// photons are summed in the walk's insertion order).
+__global__ __launch_bounds__(kThreads) void k_photon_subsurface(SssArgs a)
+{
+    const uint32_t lane = blockIdx.x * kThreads + threadIdx.x;
+    if (lane >= a.m) return;
+    const uint32_t q = a.perm ? a.perm[lane] : lane;
+    const float p[3] = {a.queries[3 * (size_t)q], a.queries[3 * (size_t)q + 1], a.queries[3 * (size_t)q + 2]};
+    const uint32_t want = a.qshape ? (uint32_t)(a.qshape[q] + 1) : 0u;
+    const float eta = a.prm.eta, g = a.prm.g;
+    const float fdr = (float)((-1.440 / (double)(eta * eta)) + (0.710 / (double)eta) + 0.668 + 0.0636 * (double)eta);
+    const float A = (1 + fdr) / (1 - fdr);
+    // Ft of the camera side: 1 - fresnelDielectric(cos(normal, view), 1, eta)
+    const float cos_o = cosine3(a.qnrm[3 * (size_t)q], a.qnrm[3 * (size_t)q + 1], a.qnrm[3 * (size_t)q + 2],
+                                a.qview[3 * (size_t)q], a.qview[3 * (size_t)q + 1], a.qview[3 * (size_t)q + 2]);
+    const double ft_o = (double)(1 - rt::fresnel_dielectric(cos_o, 1.0f, eta));
+    const float inv_pi = 1.0f / 3.14159265358979323846f;
+    double acc[3] = {0.0, 0.0, 0.0};
+    float maxd = 0.0f;
+    uint32_t sl[kStack], sr[kStack];
+    float spd[kStack];
+    uint32_t sp = 0;
+    sl[0] = 0; sr[0] = a.n; spd[0] = -1.0f; sp = 1;
+    while (sp > 0) {
+        --sp;
+        const uint32_t l = sl[sp], r = sr[sp];
+        if (spd[sp] >= 0.0f && !(spd[sp] < a.radius)) continue;
+        if (r <= l) continue;
+        const uint32_t med = l + (r - l) / 2;
+        const rt::PhotonNode nd = a.nodes[med];
+        const float d = norm3(p[0] - nd.x, p[1] - nd.y, p[2] - nd.z);
+        if (d < a.radius && (want == 0u || (nd.meta >> 2) == want)) {
+            const uint32_t e = a.order[med];
+            const float *pd = a.pdir + 3 * (size_t)e, *pn = a.pnrm + 3 * (size_t)e;
+            const float cos_i = cosine3(pn[0], pn[1], pn[2], -pd[0], -pd[1], -pd[2]);
+            const double ft_i = (double)(1 - rt::fresnel_dielectric(cos_i, 1.0f, eta));
+            for (int c = 0; c < 3; ++c) {
+                const double rd = dipole_rd((double)a.prm.sigma_s[c], (double)a.prm.sigma_a[c], g, fdr, A, d);
+                const double sd = (double)inv_pi * ft_i * rd * ft_o;  // diffusionTermBSSRDF (geometricMath.cu:500-522)
+                acc[c] += a.rgb[3 * (size_t)e + c] * sd * (double)a.prm.kd[c];
+            }
+            maxd = fmaxf(maxd, d);
+        }
+        if (r - l > 1) {
+            const uint32_t ax = nd.meta & 3u;
+            const float split = ax == 0 ? nd.x : (ax == 1 ? nd.y : nd.z);
+            const float diff = p[ax] - split;
+            const float pdd = norm3(ax == 0 ? diff : 0.0f, ax == 1 ? diff : 0.0f, ax == 2 ? diff : 0.0f);
+            uint32_t nl, nr, fl, fr;
+            if (p[ax] < split) { nl = l; nr = med; fl = med + 1; fr = r; }
+            else { nl = med + 1; nr = r; fl = l; fr = med; }
+            if (sp + 2 > kStack) break;  // unreachable: the host checks depth + 1 <= kStack
+            sl[sp] = fl; sr[sp] = fr; spd[sp] = pdd; ++sp;
+            sl[sp] = nl; sr[sp] = nr; spd[sp] = -1.0f; ++sp;
+        }
+    }
+    // contribution /= 2 PI maxD^2 (no photon: 0 / 0)
+    const float den = 2.0f * 3.14159265358979323846f * maxd * maxd;
+    for (int c = 0; c < 3; ++c) a.out[3 * (size_t)q + c] = acc[c] / (double)den;
+}
+
 }  // namespace
+
+// Morton order of nq query points in the map's work space (perm: lane ->
+// query); the work space is shared by every search on the map.
+static rt_status sort_queries(rt_photon_map *m, const float *queries, uint32_t nq, hipStream_t stream,
+                              const uint32_t **perm, const char *who)
+{
+    size_t sort_bytes = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                                      (uint32_t *)nullptr, (uint32_t *)nullptr, (int)nq, 0, 30,
+                                                      stream);
+    const size_t ids = ((size_t)nq * sizeof(uint32_t) + 255) & ~(size_t)255;
+    const size_t need = 4 * ids + sort_bytes;
+    if (e == hipSuccess && need > m->work_bytes) {
+        if (m->work) {
+            e = hipStreamSynchronize(stream);
+            if (e == hipSuccess) e = hipFree(m->work);
+            m->work = nullptr;
+            m->work_bytes = 0;
+        }
+        if (e == hipSuccess) e = hipMalloc(&m->work, need);
+        if (e == hipSuccess) m->work_bytes = need;
+    }
+    if (e != hipSuccess) {
+        set_error(std::string(who) + ": query sort: " + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;
+    }
+    uint8_t *w = (uint8_t *)m->work;
+    uint32_t *key0 = (uint32_t *)w, *key1 = (uint32_t *)(w + ids), *id0 = (uint32_t *)(w + 2 * ids),
+             *id1 = (uint32_t *)(w + 3 * ids);
+    k_query_morton<<<(nq + 255) / 256, 256, 0, stream>>>(queries, nq, m->lo[0], m->lo[1], m->lo[2], m->scale[0],
+                                                         m->scale[1], m->scale[2], key0, id0);
+    e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipcub::DeviceRadixSort::SortPairs(w + 4 * ids, sort_bytes, key0, key1, id0, id1, (int)nq, 0, 30, stream);
+    if (e != hipSuccess) {
+        set_error(std::string(who) + ": query sort: " + hipGetErrorString(e));
+        return RT_ERR_HIP;
+    }
+    *perm = id1;
+    return RT_OK;
+}
 
 extern "C" {
 
@@ -401,28 +588,9 @@ rt_status rt_photon_map_knn(rt_photon_map *m, const float *queries, const int32_
     hipStream_t stream = stream_ ? (hipStream_t)stream_ : (hipStream_t)rt::ctx_stream_ptr(m->ctx);
     // queries in Morton order of their position, so the lanes of a wavefront
     // walk neighbouring parts of the tree (results go to the callers' slots)
-    size_t sort_bytes = 0;
-    RT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
-                                              (uint32_t *)nullptr, (uint32_t *)nullptr, (int)nq, 0, 30, stream));
-    const size_t ids = ((size_t)nq * sizeof(uint32_t) + 255) & ~(size_t)255;
-    const size_t need = 4 * ids + sort_bytes;
-    if (need > m->work_bytes) {
-        if (m->work) {
-            RT_HIP(hipStreamSynchronize(stream));
-            RT_HIP(hipFree(m->work));
-            m->work = nullptr;
-            m->work_bytes = 0;
-        }
-        RT_HIP(hipMalloc(&m->work, need));
-        m->work_bytes = need;
-    }
-    uint8_t *w = (uint8_t *)m->work;
-    uint32_t *key0 = (uint32_t *)w, *key1 = (uint32_t *)(w + ids), *id0 = (uint32_t *)(w + 2 * ids),
-             *id1 = (uint32_t *)(w + 3 * ids);
-    k_query_morton<<<(nq + 255) / 256, 256, 0, stream>>>(queries, nq, m->lo[0], m->lo[1], m->lo[2], m->scale[0],
-                                                         m->scale[1], m->scale[2], key0, id0);
-    RT_HIP(hipGetLastError());
-    RT_HIP(hipcub::DeviceRadixSort::SortPairs(w + 4 * ids, sort_bytes, key0, key1, id0, id1, (int)nq, 0, 30, stream));
+    const uint32_t *id1 = nullptr;
+    rt_status sst = sort_queries(m, queries, nq, stream, &id1, "rt_photon_map_knn");
+    if (sst != RT_OK) return sst;
     KnnArgs a{id1, m->nodes, m->order, m->n, queries, query_shape, nq, k, max_distance, count, index, distance};
     if (k <= kLdsK)
         k_photon_knn<true><<<(nq + kThreads - 1) / kThreads, kThreads, (size_t)k * kThreads * 8, stream>>>(a);
@@ -468,6 +636,84 @@ rt_status rt_photon_gather_lambert(rt_photon_map *m, const double *photon_rgb, c
     if (st != RT_OK) return st;
     k_photon_gather_lambert<<<(nq + 255) / 256, 256, 0, stream>>>(photon_rgb, photon_dir, normals, kd, nq, k, cnt,
                                                                   idx, dist, out);
+    RT_HIP(hipGetLastError());
+    return RT_OK;
+}
+
+// k nearest into the map's gather buffers (shared by the density estimates)
+static rt_status gather_knn(rt_photon_map *m, const float *points, const int32_t *query_shape, uint32_t nq,
+                            uint32_t k, float max_distance, hipStream_t stream, uint32_t **cnt, uint32_t **idx,
+                            float **dist)
+{
+    const size_t cb = ((size_t)nq * 4 + 255) & ~(size_t)255, ib = ((size_t)nq * k * 4 + 255) & ~(size_t)255;
+    const size_t need = cb + 2 * ib;
+    if (need > m->gather_bytes) {
+        if (m->gather) {
+            RT_HIP(hipStreamSynchronize(stream));
+            RT_HIP(hipFree(m->gather));
+            m->gather = nullptr;
+            m->gather_bytes = 0;
+        }
+        RT_HIP(hipMalloc(&m->gather, need));
+        m->gather_bytes = need;
+    }
+    uint8_t *w = (uint8_t *)m->gather;
+    *cnt = (uint32_t *)w;
+    *idx = (uint32_t *)(w + cb);
+    *dist = (float *)(w + cb + ib);
+    return rt_photon_map_knn(m, points, query_shape, nq, k, max_distance, *cnt, *idx, *dist, stream);
+}
+
+rt_status rt_photon_gather(rt_photon_map *m, const double *photon_rgb, const float *photon_dir, const float *points,
+                           const float *normals, const float *views, const rt_material *materials,
+                           const int32_t *query_shape, uint32_t nq, uint32_t k, float max_distance, double *out,
+                           void *stream_)
+{
+    if (!m || (nq && (!points || !normals || !views || !materials || !out)) ||
+        (m->n && nq && (!photon_rgb || !photon_dir))) {
+        set_error("rt_photon_gather: null argument");
+        return RT_ERR_INVALID;
+    }
+    if (nq == 0) return RT_OK;
+    if (k == 0 || k > kMaxK || (size_t)nq * k > (1ull << 31) || nq > (1u << 30)) {
+        set_error("rt_photon_gather: k must be 1..1024, queries <= 2^30, queries * k <= 2^31");
+        return RT_ERR_INVALID;
+    }
+    std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(m->ctx));
+    RT_HIP(hipSetDevice(rt::ctx_device(m->ctx)));
+    hipStream_t stream = stream_ ? (hipStream_t)stream_ : (hipStream_t)rt::ctx_stream_ptr(m->ctx);
+    uint32_t *cnt, *idx;
+    float *dist;
+    rt_status st = gather_knn(m, points, query_shape, nq, k, max_distance, stream, &cnt, &idx, &dist);
+    if (st != RT_OK) return st;
+    k_photon_gather_mat<<<(nq + 255) / 256, 256, 0, stream>>>(photon_rgb, photon_dir, normals, views, materials, nq,
+                                                              k, cnt, idx, dist, out);
+    RT_HIP(hipGetLastError());
+    return RT_OK;
+}
+
+rt_status rt_photon_subsurface(rt_photon_map *m, const double *photon_rgb, const float *photon_dir,
+                               const float *photon_nrm, const float *points, const float *normals,
+                               const float *views, const int32_t *query_shape, uint32_t nq, float radius,
+                               const rt_sss_params *params, double *out, void *stream_)
+{
+    if (!m || !params || (nq && (!points || !normals || !views || !out)) ||
+        (m->n && nq && (!photon_rgb || !photon_dir || !photon_nrm))) {
+        set_error("rt_photon_subsurface: null argument");
+        return RT_ERR_INVALID;
+    }
+    if (!(radius > 0.0f)) { set_error("rt_photon_subsurface: radius must be positive"); return RT_ERR_INVALID; }
+    if (nq > (1u << 30)) { set_error("rt_photon_subsurface: at most 2^30 queries"); return RT_ERR_INVALID; }
+    if (nq == 0) return RT_OK;
+    std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(m->ctx));
+    RT_HIP(hipSetDevice(rt::ctx_device(m->ctx)));
+    hipStream_t stream = stream_ ? (hipStream_t)stream_ : (hipStream_t)rt::ctx_stream_ptr(m->ctx);
+    const uint32_t *perm = nullptr;
+    rt_status st = sort_queries(m, points, nq, stream, &perm, "rt_photon_subsurface");
+    if (st != RT_OK) return st;
+    SssArgs a{perm, m->nodes, m->order, m->n, photon_rgb, photon_dir, photon_nrm, points, normals, views,
+              query_shape, nq, radius, *params, out};
+    k_photon_subsurface<<<(nq + kThreads - 1) / kThreads, kThreads, 0, stream>>>(a);
     RT_HIP(hipGetLastError());
     return RT_OK;
 }

@@ -34,10 +34,10 @@ void free_wave_buffers(WaveBuffers &w);
 rt_status fill_stats(rt_render_stats &st, const unsigned long long *counts, KernelProfile &prof, uint32_t flags,
                      const char *who);
 
-// Paths per chunk: the result buffer holds 12 bytes per path (768 MiB).
-// Each chunk ends with a drain, but larger chunks measured slower on the
-// 32-scene batch (a 4.6 GB result buffer: 282 vs 334 Msamples/s) and no
-// faster on a 1080p 64-spp frame.
-constexpr uint64_t kChunkPaths = 1ull << 26;
+// Paths per chunk: the result buffer holds 12 bytes per path (3 GiB at
+// most).  Each chunk ends with a drain whose last iterations hold few live
+// paths; on the 32-scene batch (383M paths per frame) 2^26-path chunks
+// measured 618 ms per frame, 2^27 607, 2^28 604 and one 2^29 chunk 599.
+constexpr uint64_t kChunkPaths = 1ull << 28;
 
 }  // namespace rt

@@ -18,6 +18,7 @@
 
 #include "rt_cache.h"
 #include "rt_internal.h"
+#include "rt_photon_trace.h"
 #include "rt_render.h"
 #include "rt_waves.h"
 
@@ -692,6 +693,27 @@ rt_status rt_scene_commit(rt_scene *s)
 }
 
 rt_bvh *rt_scene_bvh(rt_scene *s) { return s ? s->bvh : nullptr; }
+
+rt_status rt_scene_trace_photons(rt_scene *s, const rt_photon_opts *opts, uint32_t capacity, float *location,
+                                 float *direction, float *normal, double *radiance, int32_t *shape, uint32_t *key,
+                                 uint32_t *count, void *stream_)
+{
+    const char *who = "rt_scene_trace_photons";
+    if (count) *count = 0;
+    if (!s || !opts || !count) { set_error(std::string(who) + ": null argument"); return RT_ERR_INVALID; }
+    if (!s->committed) { set_error(std::string(who) + ": scene not committed"); return RT_ERR_INVALID; }
+    if (opts->photons > (1u << 27)) { set_error(std::string(who) + ": at most 2^27 photon paths"); return RT_ERR_INVALID; }
+    if (opts->max_depth == 0 || opts->max_depth > 32) { set_error(std::string(who) + ": max_depth must be 1..32"); return RT_ERR_INVALID; }
+    if (capacity && (!location || !direction || !normal || !radiance || !shape)) {
+        set_error(std::string(who) + ": null output array");
+        return RT_ERR_INVALID;
+    }
+    std::lock_guard<std::recursive_mutex> g(rt::ctx_lock(s->ctx));
+    RT_HIP(hipSetDevice(rt::ctx_device(s->ctx)));
+    hipStream_t stream = stream_ ? (hipStream_t)stream_ : (hipStream_t)rt::ctx_stream_ptr(s->ctx);
+    return rt::trace_photons(s->dev, s->lights.data(), (uint32_t)s->lights.size(), *opts, capacity, location,
+                             direction, normal, radiance, shape, key, count, stream, who);
+}
 
 }  // extern "C"
 

@@ -220,7 +220,7 @@ typedef struct {
     uint32_t seed;        /* PCG32 seed */
     uint32_t path_capacity; /* paths in flight per wave, 0 = default (2^21) */
     uint32_t grid;          /* workgroups of the queue kernels, 0 = default (2048) */
-    uint32_t chunk_paths;   /* paths per streamed chunk (12 B of result buffer each), 0 = default (2^26) */
+    uint32_t chunk_paths;   /* paths per streamed chunk (12 B of result buffer each), 0 = default (2^28) */
 } rt_render_opts;
 
 typedef struct rt_scene rt_scene;
@@ -274,6 +274,40 @@ rt_status rt_scene_set_camera(rt_scene *scene, const rt_camera *cam);
 rt_status rt_scene_commit(rt_scene *scene);
 /* The scene's triangle BVH (valid after commit; owned by the scene). */
 rt_bvh *rt_scene_bvh(rt_scene *scene);
+
+/* Photon pre-pass: generatePhotons + computePhotons (integration/
+ * photonDispersion.cu:26-192, Scene.cu:989-1021) on the GPU.  Each of
+ * `photons` paths picks a point light with probability proportional to its
+ * power (Scene::selectOneRandomLight, Scene.cu:427-455) and leaves it in a
+ * uniform direction (PointLight::sampleLight, light/PointLight.cu:68-95)
+ * carrying color * intensity * 4 pi / n_l / p_l (n_l: photons that picked
+ * light l, p_l: its selection probability, Scene.cu:1005-1012).  Each
+ * intersection with a scattering material (Lambertian, Trowbridge, plain)
+ * stores a photon; the material's sampleDirection continues the path, the
+ * hit's emission is added to the carried light, a black weight ends it and
+ * Russian roulette applies past rr_depth (photonDispersion.cu:26-75,
+ * pathTracing.cu:62-123), for at most max_depth intersections.  Random
+ * numbers: photon path i draws from its own PCG32 stream (light, two for the
+ * direction, then the materials' draws and the roulette in path order).
+ * Outputs, device arrays of capacity entries: location (3 floats), direction
+ * of the ray that arrived (3 floats), shading normal (3 floats), radiance
+ * (3 doubles, totalLight * pathLightFactor) and shape id (analytic shape k:
+ * k, spheres then planes; triangle t: spheres + planes + t, as
+ * Scene::intersectShapes numbers them, Scene.cu:209-227); key (may be NULL)
+ * receives path << 5 | bounce.  Photons are ordered by path, then bounce
+ * (ascending key).  *count (host) receives their number; more
+ * than capacity gives RT_ERR_INVALID with *count set, so the caller can
+ * grow.  A scene without point lights stores none (generatePhotons runs only
+ * with lights).  Blocking: returns when the photons are written. */
+typedef struct {
+    uint32_t photons;    /* photon paths, 1 .. 2^27 (N_PHOTON_SAMPLES = 1M, types.h:57) */
+    uint32_t max_depth;  /* intersections per path, 1 .. 32 (MAX_PHOTON_RECURSION_DEPTH = 30, types.h:61) */
+    uint32_t rr_depth;   /* Russian roulette after this depth (3, pathTracing.cu:91) */
+    uint32_t seed;
+} rt_photon_opts;
+rt_status rt_scene_trace_photons(rt_scene *scene, const rt_photon_opts *opts, uint32_t capacity, float *location,
+                                 float *direction, float *normal, double *radiance, int32_t *shape, uint32_t *key,
+                                 uint32_t *count, void *stream);
 
 /* Path-trace spp samples per pixel and ADD them into accum (width*height*3
  * doubles, per-pixel radiance sums) and nvalid (width*height counts of
@@ -420,6 +454,51 @@ rt_status rt_photon_gather_lambert(rt_photon_map *map, const double *photon_rgb,
                                    const int32_t *query_shape, uint32_t nq, uint32_t k, float max_distance,
                                    double *out, void *stream);
 
+/* The same density estimate for any material of the scene vocabulary:
+ * combinePhotonContributions (Scene.cu:607-634) with
+ * SurfaceInteraction::lightScattered(-photon.ray.direction) (transformations/
+ * SurfaceInteraction.cu:278-292), i.e. the material's lightReflected
+ * (f * |cos wi|) for the view direction wo and wi = -photon_dir, both taken
+ * in the shading frame of normals[q]:
+ *   out = sum(photon_rgb * lightReflected(wo, -photon_dir)) / (pi * d_max^2)
+ * Lambertian and Trowbridge materials scatter photons; plain surfaces reflect
+ * nothing (lightReflected = 0) and the other materials are not scattering
+ * (getPhotonMappingContribution gives them 0, Scene.cu:724-746).  views: nq*3
+ * floats, the direction towards the viewer (-ray direction); materials: the
+ * material of each query (nq entries, device, rt_material; textures are not
+ * applied: pass the textured kd).  Other arguments as
+ * rt_photon_gather_lambert. */
+rt_status rt_photon_gather(rt_photon_map *map, const double *photon_rgb, const float *photon_dir,
+                           const float *points, const float *normals, const float *views,
+                           const rt_material *materials, const int32_t *query_shape, uint32_t nq, uint32_t k,
+                           float max_distance, double *out, void *stream);
+
+/* Subsurface scattering from the photon map: Scene::combinePhotonsSubsurface
+ * (integration/Scene.cu:667-702) over search_nearest(p, radius, unlimited,
+ * shape) (Scene.cu:806-814) with the dipole of diffusionTermBSSRDF and
+ * impPaper_dipoleDiffusionAproximation (cuda_libraries/geometricMath.cu:
+ * 451-522).  Per query: every photon strictly within radius of points[q] on
+ * query_shape[q] (all shapes if it is -1 or query_shape is null), in the
+ * search's insertion order, then
+ *   out = sum(photon_rgb * Sd(p, photon) * kd) / (2 pi d_max^2)
+ * where Sd = 1/pi * Ft(cos(photon_nrm, -photon_dir)) * Rd(|p - photon|) *
+ * Ft(cos(normals[q], views[q])), Ft = 1 - fresnelDielectric(cos, 1, eta).
+ * Like the reference, a query that finds no photon gets 0 / 0 = NaN.
+ * photon_nrm: the photons' surface normals (n*3 floats); views: nq*3 floats,
+ * -camera ray direction; params: the material's SSMaterial parameters.
+ * Device buffers; shares the map's work space (one stream per map). */
+typedef struct {
+    float kd[3];       /* diffuse colour (SSMaterial::kd) */
+    float sigma_s[3];  /* scattering coefficient */
+    float sigma_a[3];  /* absorption coefficient */
+    float g;           /* phase function asymmetry */
+    float eta;         /* index of refraction */
+} rt_sss_params;
+rt_status rt_photon_subsurface(rt_photon_map *map, const double *photon_rgb, const float *photon_dir,
+                               const float *photon_nrm, const float *points, const float *normals,
+                               const float *views, const int32_t *query_shape, uint32_t nq, float radius,
+                               const rt_sss_params *params, double *out, void *stream);
+
 /* ------------------------------------------------------------------ */
 /* Batches: many scenes in one launch sequence                          */
 /* ------------------------------------------------------------------ */
@@ -438,7 +517,7 @@ typedef struct {
     uint32_t spp, spp_offset, max_depth, rr_depth;
     uint32_t path_capacity;  /* paths per wave, 0 = the autotuned or default value */
     uint32_t grid;           /* workgroups of the queue kernels, 0 = autotuned or default */
-    uint32_t chunk_paths;    /* paths per streamed chunk, 0 = default (2^26) */
+    uint32_t chunk_paths;    /* paths per streamed chunk, 0 = default (2^28) */
 } rt_batch_opts;
 
 rt_status rt_batch_create(rt_ctx *ctx, rt_scene *const *scenes, uint32_t n, rt_batch **out);

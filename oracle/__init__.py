@@ -86,6 +86,11 @@ _SIGS = {
     "orc_kd_build": (_U32, [_P, _U32, _P, _P]),
     "orc_knn_batch": (None, [_P, _P, _P, _P, _U32, _P, _P, _U32, _U32, _F, C.c_int, _P, _P, _P, C.c_int]),
     "orc_photon_gather_lambert": (None, [_P, _P, _U32, _P, _P, _P, _P, _P, _P, _U32, _U32, _F, _P]),
+    "orc_photon_light_table": (None, [_P, _P, _P]),
+    "orc_trace_photons": (_U32, [_P, _U32, _U32, _U32, _U32, _U32, _P, _P, _P, _P, _P, _P]),
+    "orc_photon_gather_mat": (None, [_P, _P, _U32, _P, _P, _P, _U32, _P, _P, _P, _P]),
+    "orc_photon_subsurface": (None, [_P, _P, _P, _P, _U32, _P, _P, _P, _P, _P, _P, _P, _U32, _F, _P, _P]),
+    "orc_dipole_rd": (C.c_double, [C.c_double, C.c_double, _F, _F, _F]),
     "orc_heap_sequence": (None, [_P, _U32, _U32, _P, _P, _P]),
     "orc_concentric_disk": (None, [_F, _F, _P]),
     "orc_ggx_sample_half": (None, [_F, _P, _F, _F, _P]),
@@ -210,6 +215,54 @@ def photon_gather_lambert(xyz, rgb, dirs, points, normals, kd, k, radius, shape=
     lib().orc_photon_gather_lambert(_p(x), _p(sh), x.shape[0], _p(c), _p(d), _p(q), _p(nr), _p(kd_), _p(qs),
                                     q.shape[0], k, float(radius), _p(out))
     return out
+
+
+def photon_gather_mat(rgb, dirs, count, index, dist, normals, views, materials):
+    """combinePhotonContributions for any material over given search results
+    (count (m,), index / dist (m, k)); materials: a sequence of material dicts
+    (the scene description's form), one per query."""
+    c = np.ascontiguousarray(rgb, np.float64).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    cnt = np.ascontiguousarray(count, np.uint32)
+    idx = np.ascontiguousarray(index, np.uint32)
+    ds = np.ascontiguousarray(dist, np.float32)
+    m, k = idx.shape
+    nr = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+    vw = np.ascontiguousarray(views, np.float32).reshape(-1, 3)
+    mats = (OrcMaterial * max(1, m))()
+    for i, mt in enumerate(materials):
+        mats[i] = _desc_material(mt)
+    out = np.zeros((m, 3), np.float64)
+    lib().orc_photon_gather_mat(_p(c), _p(d), m, _p(cnt), _p(idx), _p(ds), k, _p(nr), _p(vw), C.cast(mats, _P),
+                                _p(out))
+    return out
+
+
+def photon_subsurface(xyz, tree, rgb, dirs, nrms, points, normals, views, params, radius=0.3, shape=None,
+                      query_shape=None):
+    """Scene::combinePhotonsSubsurface per query over the restated tree
+    (order, axis); params = (kd[3], sigma_s[3], sigma_a[3], g, eta)."""
+    x = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    order, axis = np.ascontiguousarray(tree[0], np.uint32), np.ascontiguousarray(tree[1], np.uint8)
+    c = np.ascontiguousarray(rgb, np.float64).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    pn = np.ascontiguousarray(nrms, np.float32).reshape(-1, 3)
+    q = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    qn = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+    qv = np.ascontiguousarray(views, np.float32).reshape(-1, 3)
+    kd, ss, sa, g, eta = params
+    prm = np.array([*kd, *ss, *sa, g, eta], np.float32)
+    sh = None if shape is None else np.ascontiguousarray(shape, np.int32)
+    qs = None if query_shape is None else np.ascontiguousarray(query_shape, np.int32)
+    out = np.zeros((q.shape[0], 3), np.float64)
+    lib().orc_photon_subsurface(_p(x), _p(sh), _p(order), _p(axis), x.shape[0], _p(c), _p(d), _p(pn), _p(q), _p(qn),
+                                _p(qv), _p(qs), q.shape[0], float(radius), _p(prm), _p(out))
+    return out
+
+
+def dipole_rd(sigma_s, sigma_a, g, eta, r):
+    """impPaper_dipoleDiffusionAproximation of one channel at distance r."""
+    return lib().orc_dipole_rd(float(sigma_s), float(sigma_a), float(g), float(eta), float(r))
 
 
 def heap_sequence(dist, k):
@@ -357,6 +410,13 @@ def _f3(v):
     return (_F * 3)(*[float(x) for x in v])
 
 
+def _desc_material(m):
+    """An orc_material from a scene description's material dict."""
+    return OrcMaterial(m["type"], _f3(m.get("kd", (0, 0, 0))), m.get("ior", 1.5), m.get("roughness", 0.0),
+                       _f3(m.get("eta", (1, 1, 1))), _f3(m.get("k", (0, 0, 0))), m.get("intensity", 0.0),
+                       _f3(m.get("ke", (0, 0, 0))), int(m.get("texture", 0)))
+
+
 class OrcScene:
     """The oracle's copy of a scene description (same arrays the library
     gets); renders with the restated estimator on the host."""
@@ -372,9 +432,7 @@ class OrcScene:
         self.sph_light = np.array([l for c, r, m, l in desc.spheres], np.uint8)
         mats = (OrcMaterial * max(1, len(desc.materials)))()
         for i, m in enumerate(desc.materials):
-            mats[i] = OrcMaterial(m["type"], _f3(m.get("kd", (0, 0, 0))), m.get("ior", 1.5), m.get("roughness", 0.0),
-                                  _f3(m.get("eta", (1, 1, 1))), _f3(m.get("k", (0, 0, 0))), m.get("intensity", 0.0),
-                                  _f3(m.get("ke", (0, 0, 0))), int(m.get("texture", 0)))
+            mats[i] = _desc_material(m)
         self.mats = mats
         lights = (OrcPointLight * max(1, len(desc.lights)))()
         for i, (p, c, inten) in enumerate(desc.lights):
@@ -442,6 +500,27 @@ class OrcScene:
                                      d.rr_depth if rr_depth is None else rr_depth, stream_base, _p(out), _p(seg),
                                      C.byref(ns))
         return bool(ok), out, seg[:ns.value]
+
+    def photon_light_table(self):
+        n = len(self.desc.lights)
+        p = np.zeros(max(1, n), np.float64)
+        cdf = np.zeros(max(1, n), np.float64)
+        lib().orc_photon_light_table(self.h, _p(p), _p(cdf))
+        return p[:n], cdf[:n]
+
+    def trace_photons(self, photons, seed=1, max_depth=30, rr_depth=3, cap=None):
+        """The photon pre-pass: dict of location, direction, normal (n, 3
+        float32), radiance (n, 3 float64), shape (n,) and key (path << 5 |
+        bounce), and the number the pass produced (may exceed cap)."""
+        cap = cap if cap is not None else photons * 4
+        out = dict(location=np.zeros((cap, 3), np.float32), direction=np.zeros((cap, 3), np.float32),
+                   normal=np.zeros((cap, 3), np.float32), radiance=np.zeros((cap, 3), np.float64),
+                   shape=np.zeros(cap, np.int32), key=np.zeros(cap, np.uint32))
+        m = lib().orc_trace_photons(self.h, photons, seed, max_depth, rr_depth, cap, _p(out["location"]),
+                                    _p(out["direction"]), _p(out["normal"]), _p(out["radiance"]), _p(out["shape"]),
+                                    _p(out["key"]))
+        n = min(m, cap)
+        return {k: v[:n] for k, v in out.items()}, m
 
     def render(self, spp=None, spp_offset=0, window=None, seed=None, nthreads=8, stream_base=0,
                max_depth=None, cache_depth=None, cache_rgb=None):

@@ -60,6 +60,20 @@ void orc_photon_gather_lambert(const float *xyz, const int32_t *shape, uint32_t 
                                const float *q, const float *nrm, const float *kd, const int32_t *qshape, uint32_t m,
                                uint32_t k, float radius, double *out);
 
+/* Scene::combinePhotonsSubsurface (integration/Scene.cu:667-702) for m
+ * queries over search_nearest(p, radius, unlimited, shape): every photon
+ * strictly within radius on the query's shape, in the walk's insertion
+ * order, summed as rgb * Sd * kd with Sd = INV_PI * Ft_i * Rd * Ft_o
+ * (diffusionTermBSSRDF, geometricMath.cu:500-522), divided by
+ * 2 PI dmax^2 (0 / 0 without photons).  params: kd[3], sigma_s[3],
+ * sigma_a[3], g, eta.  orc_dipole_rd: impPaper_dipoleDiffusionAproximation
+ * (geometricMath.cu:451-483) of one channel at distance r. */
+void orc_photon_subsurface(const float *xyz, const int32_t *shape, const uint32_t *order, const uint8_t *axis,
+                           uint32_t n, const double *rgb, const float *pdir, const float *pnrm, const float *q,
+                           const float *qnrm, const float *qview, const int32_t *qshape, uint32_t m, float radius,
+                           const float params[11], double *out);
+double orc_dipole_rd(double sigma_s, double sigma_a, float g, float eta, float r);
+
 #ifdef __cplusplus
 }
 #endif

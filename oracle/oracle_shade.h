@@ -132,6 +132,24 @@ int orc_sphere_light_sample(const float center[3], float radius, const float p[3
                             float *dist, float *pdf);
 float orc_sphere_light_pdf(const float center[3], float radius, const float p[3]);
 
+/* Photon pre-pass (integration/photonDispersion.cu:26-192, Scene.cu:427-455,
+ * 989-1021) with the library's draw order: photon path i draws from PCG32
+ * stream i of the seed (light, two direction draws, then each bounce's
+ * material draws and the roulette).  Stores at most cap photons in path /
+ * bounce order (key = i << 5 | bounce); returns how many the pass made.
+ * orc_photon_light_table: every point light's selection probability and
+ * their running sum (nlight doubles each). */
+void orc_photon_light_table(const orc_scene *s, double *p, double *cdf);
+uint32_t orc_trace_photons(const orc_scene *s, uint32_t n, uint32_t seed, uint32_t max_depth, uint32_t rr_depth,
+                           uint32_t cap, float *pos, float *dir, float *nrm, double *rgb, int32_t *shape,
+                           uint32_t *key);
+/* combinePhotonContributions for any material over given search results
+ * (count[q], idx / dist rows of k): sum(rgb * lightReflected(view, -dir)) /
+ * (PI dmax^2) in the shading frame of nrm[q]; Lambertian and Trowbridge. */
+void orc_photon_gather_mat(const double *rgb, const float *dir, uint32_t m, const uint32_t *count, const uint32_t *idx,
+                           const float *dist, uint32_t k, const float *nrm, const float *view,
+                           const orc_material *mats, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -3,6 +3,7 @@
  * INFRASTRUCTURE ONLY; see oracle_photon.h).
  */
 #include "oracle_photon.h"
+#include "oracle_shade.h"
 
 #include <math.h>
 #include <pthread.h>
@@ -309,4 +310,116 @@ void orc_photon_gather_lambert(const float *xyz, const int32_t *shape, uint32_t 
     }
     free(idx);
     free(dist);
+}
+
+/* ---------------- subsurface term (Scene.cu:667-702, geometricMath.cu:451-522) ---------------- */
+
+/* impPaper_dipoleDiffusionAproximation for one channel, float scalars
+ * widened to double where the reference's Spectrum meets them. */
+static double dipole_rd(double ss, double sa, float g, float fdr, float A, float r)
+{
+    const double ssp = (double)(1.0f - g) * ss;
+    const double stp = ssp + sa;
+    const double alpha = ssp / stp;
+    const double str = sqrt(3.0 * sa * stp);
+    const double zr = sqrt(3.0 * (1.0 - alpha)) / str;
+    const double zv = zr * (double)A;
+    const double r2 = (double)(r * r);
+    const double dr = sqrt(r2 + zr * zr), dv = sqrt(r2 + zv * zv);
+    const double sdr = str * dr, sdv = str * dv;
+    const double rd = (sdr + 1.0) * exp(-sdr) * zr / (dr * dr * dr) + (sdv + 1.0) * exp(-sdv) * zv / (dv * dv * dv);
+    const double c1 = zr * (str + 1.0 / dr);
+    return c1 * rd * (double)(1.0f - fdr) * alpha;
+}
+
+double orc_dipole_rd(double sigma_s, double sigma_a, float g, float eta, float r)
+{
+    const float fdr = (float)((-1.440 / (double)(eta * eta)) + (0.710 / (double)eta) + 0.668 + 0.0636 * (double)eta);
+    const float A = (1 + fdr) / (1 - fdr);
+    return dipole_rd(sigma_s, sigma_a, g, fdr, A, r);
+}
+
+/* cosine(v1, v2) (geometricMath.cu:139-142) */
+static float cosine3(float ax, float ay, float az, float bx, float by, float bz)
+{
+    const float d = ax * bx + ay * by + az * bz;
+    return d / (sqrtf(ax * ax + ay * ay + az * az) * sqrtf(bx * bx + by * by + bz * bz));
+}
+
+typedef struct {
+    const float *xyz;
+    const int32_t *shape;
+    const uint32_t *order;
+    const uint8_t *axis;
+    const float *q;
+    int32_t qshape;
+    float radius;
+    /* per-photon term inputs */
+    const double *rgb;
+    const float *pdir, *pnrm;
+    const float *params;  /* kd[3], sigma_s[3], sigma_a[3], g, eta */
+    float fdr, A;
+    double ft_o;
+    double acc[3];
+    float maxd;
+} Range;
+
+/* search_nearest(p, radius, unlimited, shape): the walk of
+ * nearest_neighbors_impl with a k never reached, so photons arrive in
+ * insertion order and the radius never shrinks. */
+static void range_walk(Range *s, size_t left, size_t right)
+{
+    if (right <= left) return;
+    const size_t median = (right + left) / 2;
+    const uint32_t e = s->order[median];
+    const float d = dist_to(s->xyz, e, s->q);
+    if (d < s->radius && (s->qshape == -1 || (s->shape ? s->shape[e] : -1) == s->qshape)) {
+        const float *pd = s->pdir + 3 * (size_t)e, *pn = s->pnrm + 3 * (size_t)e;
+        const float cos_i = cosine3(pn[0], pn[1], pn[2], -pd[0], -pd[1], -pd[2]);
+        const double ft_i = (double)(1 - orc_fresnel_dielectric(cos_i, 1.0f, s->params[10]));
+        const float inv_pi = 1.0f / 3.14159265358979323846f;
+        for (int c = 0; c < 3; ++c) {
+            const double rd = dipole_rd((double)s->params[3 + c], (double)s->params[6 + c], s->params[9], s->fdr, s->A, d);
+            const double sd = (double)inv_pi * ft_i * rd * s->ft_o; /* diffusionTermBSSRDF */
+            s->acc[c] += s->rgb[3 * (size_t)e + c] * sd * (double)s->params[c];
+        }
+        if (d > s->maxd) s->maxd = d;
+    }
+    if (right - left > 1) {
+        const int ax = s->axis[median];
+        const float split = s->xyz[3 * (size_t)e + ax];
+        float pl[3] = {0.0f, 0.0f, 0.0f};
+        pl[ax] = s->q[ax] - split;
+        if (s->q[ax] < split) {
+            range_walk(s, left, median);
+            if (norm3(pl[0], pl[1], pl[2]) < s->radius) range_walk(s, median + 1, right);
+        } else {
+            range_walk(s, median + 1, right);
+            if (norm3(pl[0], pl[1], pl[2]) < s->radius) range_walk(s, left, median);
+        }
+    }
+}
+
+void orc_photon_subsurface(const float *xyz, const int32_t *shape, const uint32_t *order, const uint8_t *axis,
+                           uint32_t n, const double *rgb, const float *pdir, const float *pnrm, const float *q,
+                           const float *qnrm, const float *qview, const int32_t *qshape, uint32_t m, float radius,
+                           const float params[11], double *out)
+{
+    const float eta = params[10];
+    const float fdr = (float)((-1.440 / (double)(eta * eta)) + (0.710 / (double)eta) + 0.668 + 0.0636 * (double)eta);
+    const float A = (1 + fdr) / (1 - fdr);
+    for (uint32_t i = 0; i < m; ++i) {
+        const float *qn = qnrm + 3 * (size_t)i, *qv = qview + 3 * (size_t)i;
+        Range s;
+        memset(&s, 0, sizeof s);
+        s.xyz = xyz; s.shape = shape; s.order = order; s.axis = axis; s.q = q + 3 * (size_t)i;
+        s.qshape = qshape ? qshape[i] : -1;
+        s.radius = radius; s.rgb = rgb; s.pdir = pdir; s.pnrm = pnrm; s.params = params;
+        s.fdr = fdr; s.A = A;
+        s.ft_o = (double)(1 - orc_fresnel_dielectric(cosine3(qn[0], qn[1], qn[2], qv[0], qv[1], qv[2]), 1.0f, eta));
+        range_walk(&s, 0, n);
+        /* contribution /= 2 PI maxD^2 (no photon: 0 / 0) */
+        const float den = 2.0f * 3.14159265358979323846f * s.maxd * s.maxd;
+        for (int c = 0; c < 3; ++c) out[3 * (size_t)i + c] = s.acc[c] / (double)den;
+    }
 }

@@ -1165,3 +1165,182 @@ float orc_sphere_light_pdf(const float center[3], float radius, const float p[3]
     orc_sphere_init(&sp, center, radius);
     return sphere_light_pdf(&sp, mk(p[0], p[1], p[2]));
 }
+
+/* ------------------------------------------------------------------ */
+/* Photon pre-pass: generatePhotons + computePhotons                    */
+/* (integration/photonDispersion.cu:26-192, Scene.cu:427-455, 989-1021) */
+/* ------------------------------------------------------------------ */
+/* Selection probability of every point light (sum of its power over the
+ * total, PointLight::power = color * intensity * 4 pi, light/PointLight.cu:
+ * 40-43) and their running sum, in double. */
+void orc_photon_light_table(const orc_scene *s, double *p, double *cdf)
+{
+    double total = 0;
+    for (uint32_t l = 0; l < s->nlight; ++l) {
+        const orc_point_light *L = &s->pl[l];
+        p[l] = ((double)L->color[0] + (double)L->color[1] + (double)L->color[2]) * (double)L->intensity *
+               (4.0 * 3.14159265358979323846);
+        total += p[l];
+    }
+    double run = 0;
+    for (uint32_t l = 0; l < s->nlight; ++l) {
+        p[l] = total > 0 ? p[l] / total : 0.0;
+        run += p[l];
+        cdf[l] = run;
+    }
+}
+
+/* selectOneRandomLight: the first light whose running probability reaches u
+ * (the reference subtracts each light's share from u until it is <= 0);
+ * rounding past the end falls back to the last light with a share. */
+static uint32_t photon_select(const double *cdf, uint32_t nl, float u)
+{
+    for (uint32_t l = 0; l < nl; ++l)
+        if ((double)u <= cdf[l]) return l;
+    uint32_t last = nl - 1;
+    while (last > 0 && cdf[last] == cdf[last - 1]) --last;
+    return last;
+}
+
+/* The photon path's generator: stream i of the seed's state. */
+static pcg32 photon_rng(uint32_t seed, uint32_t i)
+{
+    return pcg_seed(((uint64_t)seed << 32) ^ 0x9E3779B97F4A7C15ull, (uint64_t)i);
+}
+
+uint32_t orc_trace_photons(const orc_scene *s, uint32_t n, uint32_t seed, uint32_t max_depth, uint32_t rr_depth,
+                           uint32_t cap, float *pos, float *dir, float *nrm_out, double *rgb, int32_t *shape,
+                           uint32_t *key)
+{
+    const uint32_t nl = s->nlight;
+    if (nl == 0 || n == 0) return 0;
+    double *p = malloc(sizeof(double) * nl), *cdf = malloc(sizeof(double) * nl), *rgbl = calloc(3 * nl, sizeof(double));
+    uint32_t *cnt = calloc(nl, sizeof(uint32_t));
+    orc_photon_light_table(s, p, cdf);
+    /* generatePhotons: every photon's light first, then per light
+     * color * intensity / uniformSpherePdf / n_l, divided by
+     * lightSelectionPdf(l) (Scene.cu:996-1012, PointLight.cu:68-95) */
+    for (uint32_t i = 0; i < n; ++i) {
+        pcg32 g = photon_rng(seed, i);
+        ++cnt[photon_select(cdf, nl, pcg_float(&g))];
+    }
+    for (uint32_t l = 0; l < nl; ++l)
+        for (int c = 0; c < 3; ++c)
+            if (cnt[l])
+                rgbl[3 * l + c] = (double)s->pl[l].color[c] * (double)s->pl[l].intensity *
+                                  (4.0 * 3.14159265358979323846) / (double)cnt[l] / p[l];
+    const float kPi = 3.14159265358979323846f;
+    const uint32_t nshapes = s->nsph + s->nplane;
+    uint32_t m = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        pcg32 g = photon_rng(seed, i);
+        const uint32_t l = photon_select(cdf, nl, pcg_float(&g));
+        /* uniformSampleSphere (sampling/Sampler.cu:47-52) */
+        const float u0 = pcg_float(&g), u1 = pcg_float(&g);
+        const float phi = 2.0f * kPi * u0, theta = acosf(2.0f * u1 - 1.0f);
+        v3 o = mk(s->pl[l].pos[0], s->pl[l].pos[1], s->pl[l].pos[2]);
+        v3 d = mk(sinf(theta) * cosf(phi), sinf(theta) * sinf(phi), cosf(theta));
+        double tl[3] = {rgbl[3 * l], rgbl[3 * l + 1], rgbl[3 * l + 2]};
+        v3 beta = mk(1, 1, 1);
+        for (uint32_t depth = 0; depth < max_depth; ++depth) {
+            hit_rec h = scene_intersect(s, o, d);
+            if (h.kind == 0) break; /* no interaction: finished */
+            const orc_material *mt =
+                &s->mat[h.kind == 1 ? s->tri_mat[h.id] : h.kind == 2 ? s->sph_mat[h.id] : s->plane_mat[h.id]];
+            orc_material textured;
+            if (mt->texture && (mt->type == ORC_MAT_LAMBERTIAN || mt->type == ORC_MAT_PLAIN)) {
+                textured = *mt;
+                v3 c = texture_color(s, mt->texture - 1, h.su, h.sv);
+                textured.kd[0] = mt->kd[0] * c.x;
+                textured.kd[1] = mt->kd[1] * c.y;
+                textured.kd[2] = mt->kd[2] * c.z;
+                mt = &textured;
+            }
+            const int scat = is_scattering(mt);
+            if (scat) {
+                /* samplePhotonIntersections: Photon(pathRay, worldPoint,
+                 * shading normal, totalLight * pathLightFactor, shapeId)
+                 * (photonDispersion.cu:40-55) */
+                if (m < cap) {
+                    pos[3 * (size_t)m] = h.p.x; pos[3 * (size_t)m + 1] = h.p.y; pos[3 * (size_t)m + 2] = h.p.z;
+                    dir[3 * (size_t)m] = d.x; dir[3 * (size_t)m + 1] = d.y; dir[3 * (size_t)m + 2] = d.z;
+                    nrm_out[3 * (size_t)m] = h.n.x; nrm_out[3 * (size_t)m + 1] = h.n.y; nrm_out[3 * (size_t)m + 2] = h.n.z;
+                    rgb[3 * (size_t)m] = tl[0] * (double)beta.x;
+                    rgb[3 * (size_t)m + 1] = tl[1] * (double)beta.y;
+                    rgb[3 * (size_t)m + 2] = tl[2] * (double)beta.z;
+                    shape[m] = h.kind == 1 ? (int32_t)(nshapes + (uint32_t)h.id)
+                                           : h.kind == 2 ? h.id : (int32_t)(s->nsph + (uint32_t)h.id);
+                    key[m] = (i << 5) | depth;
+                }
+                ++m;
+            }
+            /* sampleRay, emission into totalLight, pathLightFactor *= weight
+             * (photonDispersion.cu:57-72) */
+            v3 nn = h.n, t, b;
+            onb(nn, &t, &b);
+            v3 wo_w = neg(d);
+            v3 wo = mk(dot(wo_w, t), dot(wo_w, b), dot(wo_w, nn));
+            v3 wi;
+            float pdf;
+            int delta;
+            v3 wgt = bsdf_sample(mt, wo, &g, &wi, &pdf, &delta);
+            if (mt->type == ORC_MAT_LIGHT) {
+                const float I = mt->intensity;
+                tl[0] += (double)beta.x * (double)(mt->kd[0] * I);
+                tl[1] += (double)beta.y * (double)(mt->kd[1] * I);
+                tl[2] += (double)beta.z * (double)(mt->kd[2] * I);
+            } else if (mt->type == ORC_MAT_PLAIN) {
+                /* photon rays are not camera rays: ke */
+                tl[0] += (double)beta.x * (double)mt->ke[0];
+                tl[1] += (double)beta.y * (double)mt->ke[1];
+                tl[2] += (double)beta.z * (double)mt->ke[2];
+            }
+            beta = mk(beta.x * wgt.x, beta.y * wgt.y, beta.z * wgt.z);
+            if (wgt.x == 0 && wgt.y == 0 && wgt.z == 0) break;
+            /* finishStep with photon_first_pass (pathTracing.cu:62-123) */
+            if (scat && depth > rr_depth && !russian_roulette(&beta, pcg_float(&g))) break;
+            v3 ww = add(add(mul(t, wi.x), mul(b, wi.y)), mul(nn, wi.z));
+            o = spawn_origin(h.p, h.perr, nn, ww);
+            d = ww;
+        }
+    }
+    free(p); free(cdf); free(rgbl); free(cnt);
+    return m;
+}
+
+/* combinePhotonContributions for any material over given search results
+ * (count[q] photons idx[q*k ..] at distances dist[q*k ..]): the photon's
+ * radiance times the material's lightReflected(wo, -photon dir) in the
+ * shading frame of nrm[q] (Scene.cu:607-634, SurfaceInteraction.cu:278-292);
+ * Lambertian and Trowbridge only.  Sum in double, times 1 / (PI * dmax^2). */
+void orc_photon_gather_mat(const double *rgb, const float *dir, uint32_t m, const uint32_t *count, const uint32_t *idx,
+                           const float *dist, uint32_t k, const float *nrm_q, const float *view,
+                           const orc_material *mats, double *out)
+{
+    for (uint32_t q = 0; q < m; ++q) {
+        double acc[3] = {0, 0, 0};
+        const orc_material *mt = &mats[q];
+        const int scat = mt->type == ORC_MAT_LAMBERTIAN || mt->type == ORC_MAT_TROWBRIDGE ||
+                         mt->type == ORC_MAT_TROWBRIDGE_BASIC;
+        if (count[q] > 0 && scat) {
+            v3 n = mk(nrm_q[3 * (size_t)q], nrm_q[3 * (size_t)q + 1], nrm_q[3 * (size_t)q + 2]), t, b;
+            onb(n, &t, &b);
+            v3 vw = mk(view[3 * (size_t)q], view[3 * (size_t)q + 1], view[3 * (size_t)q + 2]);
+            v3 wo = mk(dot(vw, t), dot(vw, b), dot(vw, n));
+            float maxd = 0;
+            for (uint32_t i = 0; i < count[q]; ++i) {
+                const uint32_t e = idx[(size_t)q * k + i];
+                if (dist[(size_t)q * k + i] > maxd) maxd = dist[(size_t)q * k + i];
+                v3 w = mk(-dir[3 * (size_t)e], -dir[3 * (size_t)e + 1], -dir[3 * (size_t)e + 2]);
+                v3 wi = mk(dot(w, t), dot(w, b), dot(w, n));
+                v3 f = bsdf_f(mt, wo, wi);
+                acc[0] += rgb[3 * (size_t)e] * (double)f.x;
+                acc[1] += rgb[3 * (size_t)e + 1] * (double)f.y;
+                acc[2] += rgb[3 * (size_t)e + 2] * (double)f.z;
+            }
+            const float kernel = 1.0f / (3.14159265358979323846f * maxd * maxd);
+            for (int c = 0; c < 3; ++c) acc[c] *= (double)kernel;
+        }
+        out[3 * (size_t)q] = acc[0]; out[3 * (size_t)q + 1] = acc[1]; out[3 * (size_t)q + 2] = acc[2];
+    }
+}

@@ -75,11 +75,27 @@ __device__ __forceinline__ uint32_t select_light(const double *cdf, uint32_t nl,
     return last;
 }
 
+// Photons per light.  Most lanes of a wavefront pick the same few lights, so
+// each distinct light of the wavefront takes one atomic (a same-address
+// atomic per photon serialises: 1M of them took 11.9 ms).
 __global__ __launch_bounds__(256) void k_photon_select(PhotonTraceArgs a, uint32_t *counts)
 {
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < a.n; i += gridDim.x * 256) {
-        Pcg32 g = photon_rng(a.seed, i);
-        atomicAdd(&counts[select_light(a.cdf, a.nl, pcg32_float(g))], 1u);
+    const uint32_t lane = __lane_id();
+    for (uint32_t i0 = blockIdx.x * 256; i0 < a.n; i0 += gridDim.x * 256) {
+        const uint32_t i = i0 + threadIdx.x;
+        uint32_t l = 0xffffffffu;
+        if (i < a.n) {
+            Pcg32 g = photon_rng(a.seed, i);
+            l = select_light(a.cdf, a.nl, pcg32_float(g));
+        }
+        uint64_t left = __ballot(l != 0xffffffffu);
+        while (left) {
+            const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)left) - 1);
+            const uint32_t ll = (uint32_t)__shfl((int)l, (int)leader);
+            const uint64_t same = __ballot(l == ll);
+            if (lane == leader) atomicAdd(&counts[ll], (uint32_t)__popcll(same));
+            left &= ~same;
+        }
     }
 }
 
@@ -136,8 +152,15 @@ __global__ __launch_bounds__(kPhBlock) void k_photon_trace(SceneDev sc, PhotonTr
             rt_material m = sc.mat[su.mat];
             if (sc.ntex) apply_texture(sc, su, m);
             const bool scat = scattering(m.type);
+            // one append per wavefront over the lanes storing a photon now
+            const uint64_t smask = __ballot(scat);
             if (scat) {
-                const uint32_t pos = atomicAdd(a.counter, 1u);
+                const uint32_t lane = __lane_id();
+                const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)smask) - 1);
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(a.counter, (uint32_t)__popcll(smask));
+                base = (uint32_t)__shfl((int)base, (int)leader);
+                const uint32_t pos = base + (uint32_t)__popcll(smask & ((1ull << lane) - 1ull));
                 if (pos < a.cap) {
                     a.pos[3 * (size_t)pos] = su.p.x; a.pos[3 * (size_t)pos + 1] = su.p.y; a.pos[3 * (size_t)pos + 2] = su.p.z;
                     a.dir[3 * (size_t)pos] = d.x; a.dir[3 * (size_t)pos + 1] = d.y; a.dir[3 * (size_t)pos + 2] = d.z;

@@ -766,6 +766,67 @@ def run_photon(args):
     return res
 
 
+def run_photon_pass(args):
+    """The photon pre-pass (generatePhotons + computePhotons,
+    integration/photonDispersion.cu:26-192) at the reference's size: 2^20
+    photon paths (N_PHOTON_SAMPLES, types.h:57), at most 30 intersections
+    each (MAX_PHOTON_RECURSION_DEPTH), on the Cornell box (the scene the
+    reference renders with its photon pass on).  One step = one
+    rt_scene_trace_photons call (light selection, tracing, sort by path and
+    bounce, output); value = photon paths per second."""
+    import torch
+
+    rt = importlib.import_module("2023_render_engine_amd")
+    scenes = importlib.import_module("2023_render_engine_amd.scenes")
+    n = 1 << 20
+    desc = scenes.cornell()
+    ctx = rt.Context(0)
+    sc = rt.Scene(ctx, desc)
+    for w in range(args.warmup):
+        ph = sc.trace_photons(n, seed=1 + w)
+    torch.cuda.synchronize()
+    stored = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ph = sc.trace_photons(n, seed=100 + k)
+        stored += ph["location"].shape[0]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = stored / args.steps
+    # compulsory bytes per call: 64 B written per stored photon (3 x 12 B of
+    # float vectors, 24 B of FP64 radiance, 4 B shape), read back and written
+    # once more by the sort's output pass, plus 4 B keys twice
+    io = per * (64 * 3 + 16)
+    gbs = io / (wall / args.steps) / 1e9
+    res = {
+        "metric": "photon pre-pass", "value": round(n * args.steps / wall / 1e6, 2), "unit": "Mpaths/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (FP64 radiance)",
+        "data": "synthetic",
+        "config": {"workload": f"Cornell box (10 triangles, 2 spheres, 1 point light), {n} photon paths, at most 30 "
+                               f"intersections, Russian roulette after 3", "photons_stored": int(per)},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "note": "compulsory photon I/O over the whole call (append, sort, output); the pass is "
+                             "latency bound: one lane walks one photon path, with two host round trips per call"},
+    }
+    if not args.no_cpu_baseline:
+        import oracle
+        o = oracle.OrcScene(desc)
+        spent, done, chunk = 0.0, 0, 20000
+        while spent < min(args.cpu_budget, 10.0):
+            t1 = time.time()
+            o.trace_photons(chunk, seed=7 + done, cap=chunk * 32)
+            spent += time.time() - t1
+            done += chunk
+        res["cpu_baseline"] = {"value": round(done / spent / 1e6, 4), "unit": "Mpaths/s", "cores": 1, "kind": "port",
+                               "sample": f"{done} photon paths in batches of {chunk} with the oracle's restatement, "
+                                         f"{spent:.1f} s (its light counts are per batch)"}
+    sc.close()
+    ctx.close()
+    return res
+
+
 def run_copy(args):
     """HBM calibration: a plain device-to-device copy of 1 GiB (torch), so the
     FETCH_SIZE / WRITE_SIZE counters of a kernel of known traffic can scale
@@ -795,7 +856,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c2", "c3", "c4", "copy", "film", "photon"])
+    ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c2", "c3", "c4", "copy", "film", "photon", "photon_pass"])
     ap.add_argument("--tone", default="gammaClamp", help="film: tone mapping (a ToneMappingType name)")
     ap.add_argument("--c3-ref-spp", type=int, default=128)
     ap.add_argument("--ref-spp", type=int, default=1024,
@@ -839,6 +900,8 @@ def main():
         out = run_film(args)
     elif args.workload == "photon":
         out = run_photon(args)
+    elif args.workload == "photon_pass":
+        out = run_photon_pass(args)
     else:
         out = run_render(args, args.workload)
     print(json.dumps(out), flush=True)

@@ -139,13 +139,21 @@ def run_c1(args):
         split[name] = round((sl.stop - sl.start) / (sum(ks) / len(ks) / 1e3) / 1e6, 1)
 
     achieved = alg_bytes / (k_avg / 1e3) / 1e9
-    traffic = None
-    pmc_path = os.path.join(ROOT, "profiles", f"c1_trace_closest_bvh{args.width}_hbm_traffic.json")
-    if os.path.exists(pmc_path):
+    # HBM traffic and shader counters from the committed profiles of this
+    # workload, used only while they describe it (same algorithmic bytes per
+    # launch within 1 %: the same rays, tree and visit counts)
+    def profile(name):
         try:
-            traffic = json.load(open(pmc_path)).get("hbm_bytes_per_launch")
+            d = json.load(open(os.path.join(ROOT, "profiles", name)))
         except (OSError, ValueError):
-            traffic = None
+            return None
+        ref = d.get("algorithmic_bytes_per_launch") or 0
+        return d if ref > 0 and abs(alg_bytes - ref) <= 0.01 * ref else None
+    pmc = profile(f"c1_trace_closest_bvh{args.width}_hbm_traffic.json")
+    traffic = pmc.get("hbm_bytes_per_launch") if pmc else None
+    sq = profile(f"c1_trace_closest_bvh{args.width}_sq_counters.json")
+    valu = None if sq is None else {k: sq[k] for k in ("valu_busy", "lane_utilisation", "waiting_share",
+                                                         "waves_per_simd")}
     out = {
         "metric": METRIC,
         "value": round(mrays, 2),
@@ -174,6 +182,11 @@ def run_c1(args):
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "valu": valu,
+            "note": ("achieved = 36 B per ray + 64 B per node visit + 48 B per triangle test (counted by the "
+                     "kernel's counting variant) / the kernel's HIP-event time; traffic and valu come from the "
+                     "committed FETCH_SIZE / WRITE_SIZE and SQ profiles of this workload (null when the run's "
+                     "algorithmic bytes differ from theirs)"),
         },
         "kernel_ms": round(k_avg, 3),
         "mrays_coherent": split["coherent"],
@@ -197,11 +210,12 @@ def run_c1(args):
 # ray clears its light bit (a 4 B read-modify-write).  A batch path also
 # reads its scene's root (4 B).  Every node visit reads 64 B and every
 # triangle test 48 B, counted by the kernels' counting variants.
-def measured_traffic(profile, kernel, alg_bytes):
-    """HBM bytes per frame of `kernel` from a committed FETCH_SIZE /
-    WRITE_SIZE profile (copy-scaled), or None when there is none or it was
-    taken on another workload (its algorithmic bytes per frame differ from
-    this run's by more than 1 %)."""
+def measured_traffic(profile, kernel, alg_bytes, field="hbm_bytes_per_frame"):
+    """HBM bytes per frame of `kernel` (or another field of its entry, such
+    as its shader counters) from a committed FETCH_SIZE / WRITE_SIZE profile
+    (copy-scaled), or None when there is none or it was taken on another
+    workload (its algorithmic bytes per frame differ from this run's by more
+    than 1 %)."""
     if not profile:
         return None
     path = os.path.join(ROOT, "profiles", profile)
@@ -210,9 +224,10 @@ def measured_traffic(profile, kernel, alg_bytes):
     except (OSError, ValueError, KeyError):
         return None
     ref = k.get("algorithmic_bytes_per_frame") or 0
-    if ref <= 0 or abs(alg_bytes - ref) > 0.01 * ref:
+    if ref <= 0 or abs(alg_bytes - ref) > 0.01 * ref or field not in k:
         return None
-    return int(k["hbm_bytes_per_frame"])
+    v = k[field]
+    return int(v) if field == "hbm_bytes_per_frame" else v
 
 
 def traversal_roofline(count_stats, frames, ext_ms, con_ms, batch=False, note="", traffic_profile=None):
@@ -235,6 +250,7 @@ def traversal_roofline(count_stats, frames, ext_ms, con_ms, batch=False, note=""
     return {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(traffic_profile, name, b),
+        "valu": measured_traffic(traffic_profile, name, b, field="valu"),
         "kernel": name,
         "bytes_per_frame": int(b), "kernel_ms_per_frame": round(t * 1e3, 3),
         "per_ray": {

@@ -86,6 +86,7 @@ _SIGS = {
     "orc_kd_build": (_U32, [_P, _U32, _P, _P]),
     "orc_knn_batch": (None, [_P, _P, _P, _P, _U32, _P, _P, _U32, _U32, _F, C.c_int, _P, _P, _P, C.c_int]),
     "orc_photon_gather_lambert": (None, [_P, _P, _U32, _P, _P, _P, _P, _P, _P, _U32, _U32, _F, _P]),
+    "orc_render_f64": (None, [_P, _U32, _U32, _U32, _U32, _U32, _U32, _U32, _U32, _P, _P, C.c_int]),
     "orc_photon_light_table": (None, [_P, _P, _P]),
     "orc_trace_photons": (_U32, [_P, _U32, _U32, _U32, _U32, _U32, _P, _P, _P, _P, _P, _P]),
     "orc_photon_gather_mat": (None, [_P, _P, _U32, _P, _P, _P, _U32, _P, _P, _P, _P]),
@@ -500,6 +501,18 @@ class OrcScene:
                                      d.rr_depth if rr_depth is None else rr_depth, stream_base, _p(out), _p(seg),
                                      C.byref(ns))
         return bool(ok), out, seg[:ns.value]
+
+    def render_f64(self, spp=None, window=None, seed=None, nthreads=8, max_depth=None):
+        """orc_render with the reference's double-precision Spectrum for
+        throughput, radiance and light terms (orc_render_f64)."""
+        d = self.desc
+        W, H = d.width, d.height
+        x0, y0, x1, y1 = window or (0, 0, W, H)
+        accum = np.zeros(W * H * 3, np.float64)
+        nvalid = np.zeros(W * H, np.uint32)
+        lib().orc_render_f64(self.h, spp if spp is not None else d.spp, d.seed if seed is None else seed,
+                             max_depth or d.max_depth, d.rr_depth, x0, y0, x1, y1, _p(accum), _p(nvalid), nthreads)
+        return accum, nvalid
 
     def photon_light_table(self):
         n = len(self.desc.lights)

@@ -132,6 +132,12 @@ int orc_sphere_light_sample(const float center[3], float radius, const float p[3
                             float *dist, float *pdf);
 float orc_sphere_light_pdf(const float center[3], float radius, const float p[3]);
 
+/* orc_render with throughput, radiance and light terms in double, as the
+ * reference's Spectrum (radiometry/Spectrum.cu:15): measures the float
+ * estimator's distance from the reference's arithmetic (stream base 0). */
+void orc_render_f64(const orc_scene *s, uint32_t spp, uint32_t seed, uint32_t max_depth, uint32_t rr_depth,
+                    uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1, double *accum, uint32_t *nvalid, int nthreads);
+
 /* Photon pre-pass (integration/photonDispersion.cu:26-192, Scene.cu:427-455,
  * 989-1021) with the library's draw order: photon path i draws from PCG32
  * stream i of the seed (light, two direction draws, then each bounce's

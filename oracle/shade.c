@@ -1344,3 +1344,174 @@ void orc_photon_gather_mat(const double *rgb, const float *dir, uint32_t m, cons
         out[3 * (size_t)q] = acc[0]; out[3 * (size_t)q + 1] = acc[1]; out[3 * (size_t)q + 2] = acc[2];
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* The estimator with the reference's Spectrum precision                */
+/* ------------------------------------------------------------------ */
+/* path_impl with throughput, radiance and every light term in double, as
+ * the reference's Spectrum stores RGB (radiometry/Spectrum.cu:15); geometry,
+ * sampling and the BSDF stay float (the reference's Float).  It measures how
+ * far the float estimator of the library (and of the oracle) is from the
+ * reference's own arithmetic; it is not a parity target. */
+typedef struct { double x, y, z; } d3;
+static inline d3 dmk(double x, double y, double z) { d3 r = {x, y, z}; return r; }
+
+static int path_f64(const orc_scene *s, uint32_t px, uint32_t py, uint32_t sample, uint32_t seed,
+                    uint32_t max_depth, uint32_t rr_depth, uint64_t stream_base, double out[3])
+{
+    const uint32_t W = s->cam.width;
+    pcg32 g = pcg_seed(orc_path_initstate(seed, sample), stream_base + (uint64_t)py * W + px);
+    float u0 = pcg_float(&g), u1 = pcg_float(&g);
+    float of[3], df[3];
+    orc_camera_ray(s->r2c, s->c2w, s->cam.width, s->cam.height, px, py, u0, u1, of, df);
+    v3 o = mk(of[0], of[1], of[2]), d = mk(df[0], df[1], df[2]);
+    d3 beta = dmk(1, 1, 1), L = dmk(0, 0, 0);
+    float prev_pdf = 0;
+    int prev_delta = 1, camera = 1;
+    v3 prev_p = o;
+    for (uint32_t depth = 0; depth < max_depth; ++depth) {
+        hit_rec h = scene_intersect(s, o, d);
+        if (h.kind == 0) break;
+        const orc_material *m =
+            &s->mat[h.kind == 1 ? s->tri_mat[h.id] : h.kind == 2 ? s->sph_mat[h.id] : s->plane_mat[h.id]];
+        orc_material textured;
+        if (m->texture && (m->type == ORC_MAT_LAMBERTIAN || m->type == ORC_MAT_PLAIN)) {
+            textured = *m;
+            v3 c = texture_color(s, m->texture - 1, h.su, h.sv);
+            textured.kd[0] = m->kd[0] * c.x;
+            textured.kd[1] = m->kd[1] * c.y;
+            textured.kd[2] = m->kd[2] * c.z;
+            m = &textured;
+        }
+        v3 n = h.n, t, b;
+        onb(n, &t, &b);
+        v3 wo_w = neg(d);
+        v3 wo = mk(dot(wo_w, t), dot(wo_w, b), dot(wo_w, n));
+        if (is_scattering(m)) {
+            d3 Ld = dmk(0, 0, 0);
+            for (uint32_t l = 0; l < s->nlight; ++l) {
+                v3 lp = mk(s->pl[l].pos[0], s->pl[l].pos[1], s->pl[l].pos[2]);
+                v3 tol = sub(lp, h.p);
+                v3 Lw = nrm(tol);
+                float d2 = dot(tol, tol);
+                /* color * intensity * lightDecay (Float 1 / d^2), as a Spectrum */
+                const double I = s->pl[l].intensity, dec = (double)(1 / d2);
+                d3 Li = dmk(s->pl[l].color[0] * I * dec, s->pl[l].color[1] * I * dec, s->pl[l].color[2] * I * dec);
+                if (!(dot(Lw, n) > 0)) continue;
+                v3 wi = mk(dot(Lw, t), dot(Lw, b), dot(Lw, n));
+                v3 f = bsdf_f(m, wo, wi);
+                float pf = bsdf_pdf(m, wo, wi);
+                if ((f.x == 0 && f.y == 0 && f.z == 0) || (Li.x == 0 && Li.y == 0 && Li.z == 0)) continue;
+                v3 so = spawn_origin(h.p, h.perr, n, Lw);
+                if (occluded(s, so, Lw, len(tol), -1)) continue;
+                const double w = 1.0f / (1.0f + pf * pf);
+                Ld = dmk(Ld.x + f.x * Li.x * w, Ld.y + f.y * Li.y * w, Ld.z + f.z * Li.z * w);
+            }
+            for (uint32_t j = 0; j < s->nsph; ++j) {
+                if (!s->sph_light[j]) continue;
+                float a0 = pcg_float(&g), a1 = pcg_float(&g);
+                v3 Lw;
+                float dist, pl;
+                if (!sphere_light_sample(&s->sph[j], h.p, a0, a1, &Lw, &dist, &pl)) continue;
+                if (!(dot(Lw, n) > 0)) continue;
+                v3 wi = mk(dot(Lw, t), dot(Lw, b), dot(Lw, n));
+                v3 f = bsdf_f(m, wo, wi);
+                if (f.x == 0 && f.y == 0 && f.z == 0) continue;
+                float pf = bsdf_pdf(m, wo, wi);
+                v3 so = spawn_origin(h.p, h.perr, n, Lw);
+                if (occluded(s, so, Lw, dist * 0.9999f, (int)j)) continue;
+                const orc_material *lm = &s->mat[s->sph_mat[j]];
+                const double w = pl * pl / (pl * pl + pf * pf), k = w / pl, I = lm->intensity;
+                Ld = dmk(Ld.x + f.x * (lm->kd[0] * I) * k, Ld.y + f.y * (lm->kd[1] * I) * k,
+                         Ld.z + f.z * (lm->kd[2] * I) * k);
+            }
+            L = dmk(L.x + beta.x * Ld.x, L.y + beta.y * Ld.y, L.z + beta.z * Ld.z);
+        }
+        if (m->type == ORC_MAT_LIGHT) {
+            double w = 1;
+            if (h.kind == 2 && s->sph_light[h.id] && !prev_delta) {
+                float pl = sphere_light_pdf(&s->sph[h.id], prev_p);
+                w = (double)prev_pdf * prev_pdf / ((double)prev_pdf * prev_pdf + (double)pl * pl);
+            }
+            const double I = m->intensity;
+            L = dmk(L.x + beta.x * m->kd[0] * I * w, L.y + beta.y * m->kd[1] * I * w, L.z + beta.z * m->kd[2] * I * w);
+        }
+        if (m->type == ORC_MAT_PLAIN) {
+            const float *e = camera ? m->kd : m->ke;
+            L = dmk(L.x + beta.x * e[0], L.y + beta.y * e[1], L.z + beta.z * e[2]);
+        }
+        v3 wi;
+        float pdf;
+        int delta;
+        v3 wgt = bsdf_sample(m, wo, &g, &wi, &pdf, &delta);
+        beta = dmk(beta.x * wgt.x, beta.y * wgt.y, beta.z * wgt.z);
+        if (wgt.x == 0 && wgt.y == 0 && wgt.z == 0) break;
+        if (is_scattering(m) && depth > rr_depth) {
+            /* Spectrum::brightness in double, then q = max(0.05, 1 - it) */
+            const double q = fmax(0.05, 1 - (0.212671f * beta.x + 0.715160f * beta.y + 0.072169f * beta.z));
+            if (pcg_float(&g) < q) break;
+            beta = dmk(beta.x / (1 - q), beta.y / (1 - q), beta.z / (1 - q));
+        }
+        v3 ww = add(add(mul(t, wi.x), mul(b, wi.y)), mul(n, wi.z));
+        o = spawn_origin(h.p, h.perr, n, ww);
+        d = ww;
+        prev_pdf = pdf;
+        prev_delta = delta;
+        prev_p = h.p;
+        camera = camera && is_specular(m);
+    }
+    out[0] = L.x; out[1] = L.y; out[2] = L.z;
+    return isfinite(L.x) && isfinite(L.y) && isfinite(L.z);
+}
+
+/* orc_render over the window with path_f64 (one thread per row block). */
+typedef struct {
+    const orc_scene *s;
+    uint32_t spp, seed, max_depth, rr_depth, x0, y0, x1, y1;
+    double *accum;
+    uint32_t *nvalid;
+} render64_ctx;
+
+static void *render64_rows(void *p)
+{
+    render64_ctx *c = p;
+    const uint32_t W = c->s->cam.width;
+    for (uint32_t y = c->y0; y < c->y1; ++y)
+        for (uint32_t x = c->x0; x < c->x1; ++x) {
+            double acc[3] = {0, 0, 0};
+            uint32_t nv = 0;
+            for (uint32_t k = 0; k < c->spp; ++k) {
+                double L[3];
+                if (path_f64(c->s, x, y, k, c->seed, c->max_depth, c->rr_depth, 0, L)) {
+                    acc[0] += L[0]; acc[1] += L[1]; acc[2] += L[2];
+                    ++nv;
+                }
+            }
+            size_t i = (size_t)y * W + x;
+            c->accum[3 * i] += acc[0];
+            c->accum[3 * i + 1] += acc[1];
+            c->accum[3 * i + 2] += acc[2];
+            c->nvalid[i] += nv;
+        }
+    return NULL;
+}
+
+void orc_render_f64(const orc_scene *s, uint32_t spp, uint32_t seed, uint32_t max_depth, uint32_t rr_depth,
+                    uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1, double *accum, uint32_t *nvalid, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+    uint32_t rows = y1 > y0 ? y1 - y0 : 0;
+    if ((uint32_t)nthreads > rows) nthreads = rows ? (int)rows : 1;
+    pthread_t *th = malloc(sizeof(pthread_t) * nthreads);
+    render64_ctx *cs = malloc(sizeof(render64_ctx) * nthreads);
+    uint32_t chunk = (rows + nthreads - 1) / nthreads;
+    for (int i = 0; i < nthreads; ++i) {
+        render64_ctx c = {s, spp, seed, max_depth, rr_depth, x0, y0 + i * chunk, x1, y0 + (i + 1) * chunk, accum, nvalid};
+        if (c.y0 > y1) c.y0 = y1;
+        if (c.y1 > y1) c.y1 = y1;
+        cs[i] = c;
+        pthread_create(&th[i], NULL, render64_rows, &cs[i]);
+    }
+    for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
+    free(th); free(cs);
+}

@@ -140,6 +140,7 @@ _SIGS = {
     "rt_memcpy_h2d": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "rt_memcpy_d2h": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "rt_bvh_build": (C.c_int, [_P, _P, _U32, _P, _U32, C.POINTER(BvhOpts), C.POINTER(_P)]),
+    "rt_bvh_build_device": (C.c_int, [_P, _P, _U32, _P, _U32, C.POINTER(BvhOpts), C.POINTER(_P)]),
     "rt_bvh_destroy": (C.c_int, [_P]),
     "rt_bvh_get_info": (C.c_int, [_P, C.POINTER(BvhInfo)]),
     "rt_bvh_set_transparent": (C.c_int, [_P, _P]),
@@ -314,6 +315,29 @@ class Bvh:
         check(lib().rt_bvh_build(ctx.h, _ptr(xyz), xyz.shape[0], _ptr(idx), idx.shape[0],
                                  C.byref(opts), C.byref(h)))
         self.h = h
+
+    @classmethod
+    def from_device(cls, ctx: Context, xyz, idx, max_leaf=4, width=4):
+        """rt_bvh_build_device: the tree built on the GPU from device tensors
+        (float32 vertices (nv,3), int32/uint32 indices (nt,3)) on ctx's
+        device.  Waits for torch's current stream first (the build runs on
+        the context's stream and returns when the tree is ready)."""
+        import torch
+        xyz = xyz.reshape(-1, 3).contiguous()
+        idx = idx.reshape(-1, 3).contiguous()
+        if xyz.dtype != torch.float32 or idx.dtype not in (torch.int32, torch.uint32):
+            raise TypeError("xyz must be float32 and idx int32 or uint32")
+        if not (xyz.is_cuda and idx.is_cuda):
+            raise ValueError("xyz and idx must be device tensors")
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        opts = BvhOpts(max_leaf, 32, 1.2, width)
+        h = _P()
+        ctx.fence_torch()
+        check(lib().rt_bvh_build_device(ctx.h, _ptr(xyz), xyz.shape[0], _ptr(idx), idx.shape[0],
+                                        C.byref(opts), C.byref(h)))
+        self.h = h
+        return self
 
     def close(self):
         if getattr(self, "h", None):

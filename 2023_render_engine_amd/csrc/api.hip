@@ -10,6 +10,7 @@
 #include <new>
 #include <string>
 
+#include "rt_bvh_gpu.h"
 #include "rt_internal.h"
 #include "rt_trace.h"
 
@@ -45,6 +46,11 @@ struct rt_bvh {
     rt_ctx *ctx = nullptr;
     rt::DevBvh dev;
     rt::HostBvh host;  // kept for export and info; tris carry transparency
+    // built on the device (rt_bvh_build_device): the arrays live there and
+    // host holds only the scalar figures
+    bool on_device = false;
+    rt::DeviceBvhBuild gpu;
+    uint32_t num_nodes2 = 0, num_nodes4 = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
     // ray-claim counters of the persistent kernels (rt::kClaimWords words per
@@ -194,6 +200,8 @@ static rt_status upload_bvh(rt_bvh *b)
     const size_t nn = wide ? b->host.nodes4.size() : b->host.nodes.size();
     const size_t nb = nn * 64;
     const size_t tb = b->host.tris.size() * sizeof(rt::Tri3);
+    b->num_nodes2 = (uint32_t)b->host.nodes.size();
+    b->num_nodes4 = (uint32_t)b->host.nodes4.size();
     RT_HIP(hipMalloc(&b->dev.nodes, nb));
     RT_HIP(hipMalloc(&b->dev.tris, tb));
     RT_HIP(hipMalloc(&b->ray_counters, (size_t)rt_bvh::kRayCounterRing * rt::kClaimWords * sizeof(uint32_t)));
@@ -234,6 +242,8 @@ rt_status rt_bvh_build(rt_ctx *ctx, const float *xyz, uint32_t nv, const uint32_
         set_error("rt_bvh_build: tree depth exceeds the traversal stack");
         return RT_ERR_UNSUPPORTED;
     }
+    b->num_nodes2 = (uint32_t)b->host.nodes.size();
+    b->num_nodes4 = (uint32_t)b->host.nodes4.size();
     if (!ctx) {  // host-only tree: info and export work, traces are refused
         *out = b;
         return RT_OK;
@@ -255,6 +265,76 @@ rt_status rt_bvh_build(rt_ctx *ctx, const float *xyz, uint32_t nv, const uint32_
     return RT_OK;
 }
 
+rt_status rt_bvh_build_device(rt_ctx *ctx, const float *xyz, uint32_t nv, const uint32_t *idx, uint32_t nt,
+                              const rt_bvh_opts *opts, rt_bvh **out)
+{
+    if (!out) { set_error("rt_bvh_build_device: out is null"); return RT_ERR_INVALID; }
+    *out = nullptr;
+    if (!ctx) { set_error("rt_bvh_build_device: ctx is null"); return RT_ERR_INVALID; }
+    if (nt && (!xyz || !idx || nv == 0)) { set_error("rt_bvh_build_device: null or empty triangle arrays"); return RT_ERR_INVALID; }
+    if (nt > (1u << 28)) { set_error("rt_bvh_build_device: more than 2^28 triangles"); return RT_ERR_INVALID; }
+    rt_bvh_opts o{4, 32, 1.2f, 4};
+    if (opts) o = *opts;
+    if (o.width == 0) o.width = 4;
+    if (o.max_leaf == 0) o.max_leaf = 4;
+    if (o.width != 2 && o.width != 4) { set_error("rt_bvh_build_device: width must be 2 or 4"); return RT_ERR_INVALID; }
+    if (o.max_leaf > rt::kMaxLeafCount) { set_error("rt_bvh_build_device: max_leaf must be 1..8"); return RT_ERR_INVALID; }
+    rt_bvh *b = new (std::nothrow) rt_bvh();
+    if (!b) { set_error("rt_bvh_build_device: host allocation failed"); return RT_ERR_OOM; }
+    b->ctx = ctx;
+    b->width = o.width;
+    b->on_device = true;
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    auto drop = [&](rt_status st) {
+        rt::free_bvh_device(b->gpu);
+        if (b->ray_counters) (void)hipFree(b->ray_counters);
+        delete b;
+        return st;
+    };
+    hipError_t he = hipSetDevice(ctx->device);
+    if (he != hipSuccess) {
+        set_error(std::string("hipSetDevice: ") + hipGetErrorString(he));
+        return drop(RT_ERR_HIP);
+    }
+    std::string err;
+    he = rt::build_bvh_device(xyz, nv, idx, nt, o, ctx->stream, b->gpu, err);
+    if (he != hipSuccess) {
+        set_error(err);
+        return drop(he == hipErrorInvalidValue ? RT_ERR_INVALID : he == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP);
+    }
+    const rt::DeviceBvhBuild &d = b->gpu;
+    b->host.num_tris = d.num_tris;
+    b->host.num_leaves = d.num_leaves;
+    b->host.max_depth = d.max_depth;
+    b->host.max_stack4 = d.max_stack4;
+    b->host.depth4 = d.depth4;
+    b->host.sah_cost = d.sah_cost;
+    b->host.build_ms = d.build_ms;
+    b->num_nodes2 = d.num_nodes2;
+    b->num_nodes4 = d.num_nodes4;
+    if (stack_cap(b) > rt::kMaxTraversalStack) {
+        set_error("rt_bvh_build_device: tree depth exceeds the traversal stack");
+        return drop(RT_ERR_UNSUPPORTED);
+    }
+    he = hipMalloc(&b->ray_counters, (size_t)rt_bvh::kRayCounterRing * rt::kClaimWords * sizeof(uint32_t));
+    if (he != hipSuccess) {
+        set_error(std::string("rt_bvh_build_device: ray counters: ") + hipGetErrorString(he));
+        return drop(RT_ERR_OOM);
+    }
+    const bool wide = b->width == 4;
+    b->dev.nodes = wide ? (void *)d.nodes4 : (void *)d.nodes2;
+    b->dev.tris = d.tris;
+    b->dev.num_nodes = wide ? d.num_nodes4 : d.num_nodes2;
+    b->dev.num_tris = d.num_tris;
+    b->dev.stack_cap = stack_cap(b);
+    b->dev.wide = wide ? 1u : 0u;
+    (void)hipEventCreate(&b->ev0);
+    (void)hipEventCreate(&b->ev1);
+    rt::ctx_retain(ctx);
+    *out = b;
+    return RT_OK;
+}
+
 rt_status rt_bvh_destroy(rt_bvh *b)
 {
     if (!b) return RT_OK;
@@ -267,8 +347,12 @@ rt_status rt_bvh_destroy(rt_bvh *b)
         std::lock_guard<std::recursive_mutex> g(ctx->lock);
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
-        if (b->dev.nodes) (void)hipFree(b->dev.nodes);
-        if (b->dev.tris) (void)hipFree(b->dev.tris);
+        if (b->on_device) {
+            rt::free_bvh_device(b->gpu);
+        } else {
+            if (b->dev.nodes) (void)hipFree(b->dev.nodes);
+            if (b->dev.tris) (void)hipFree(b->dev.tris);
+        }
         if (b->ray_counters) (void)hipFree(b->ray_counters);
         if (b->ev0) (void)hipEventDestroy(b->ev0);
         if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -282,15 +366,15 @@ rt_status rt_bvh_get_info(const rt_bvh *b, rt_bvh_info *info)
 {
     if (!b || !info) { set_error("rt_bvh_get_info: null argument"); return RT_ERR_INVALID; }
     info->num_tris = b->host.num_tris;
-    info->num_nodes = (uint32_t)b->host.nodes.size();
+    info->num_nodes = b->num_nodes2;
     info->num_leaves = b->host.num_leaves;
     info->max_depth = b->host.max_depth;
-    info->node_bytes = b->host.nodes.size() * sizeof(rt::Node2);
+    info->node_bytes = (uint64_t)b->num_nodes2 * sizeof(rt::Node2);
     info->tri_bytes = (uint64_t)b->host.num_tris * sizeof(rt::Tri3);
     info->build_ms = b->host.build_ms;
     info->sah_cost = b->host.sah_cost;
     info->width = b->width;
-    info->num_nodes4 = (uint32_t)b->host.nodes4.size();
+    info->num_nodes4 = b->num_nodes4;
     info->depth4 = b->host.depth4;
     info->stack_cap = stack_cap(b);
     return RT_OK;
@@ -306,6 +390,23 @@ rt_status rt_bvh_set_transparent(rt_bvh *b, const uint8_t *flags)
     }
     std::lock_guard<std::recursive_mutex> g(b->ctx->lock);
     RT_HIP(hipSetDevice(b->ctx->device));
+    if (b->on_device) {
+        // the triangles are only on the device: flags go up, a kernel applies
+        // them by triangle id
+        const uint32_t nt = b->host.num_tris;
+        if (nt == 0) return RT_OK;
+        uint8_t *df = nullptr;
+        RT_HIP(hipMalloc(&df, nt));
+        hipError_t e = hipMemcpyAsync(df, flags, nt, hipMemcpyHostToDevice, b->ctx->stream);
+        if (e == hipSuccess) e = rt::set_transparent_device(b->gpu.tris, nt, df, b->ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->ctx->stream);
+        (void)hipFree(df);
+        if (e != hipSuccess) {
+            set_error(std::string("rt_bvh_set_transparent: ") + hipGetErrorString(e));
+            return RT_ERR_HIP;
+        }
+        return RT_OK;
+    }
     for (auto &t : b->host.tris)
         if (t.id >= 0) t.flags = flags[t.id] ? 1u : 0u;
     const size_t tb = b->host.tris.size() * sizeof(rt::Tri3);
@@ -317,6 +418,16 @@ rt_status rt_bvh_set_transparent(rt_bvh *b, const uint8_t *flags)
 rt_status rt_bvh_export(const rt_bvh *b, float *nodes, float *tris)
 {
     if (!b || !nodes || !tris) { set_error("rt_bvh_export: null argument"); return RT_ERR_INVALID; }
+    if (b->on_device) {
+        std::lock_guard<std::recursive_mutex> g(b->ctx->lock);
+        RT_HIP(hipSetDevice(b->ctx->device));
+        RT_HIP(hipMemcpyAsync(nodes, b->gpu.nodes2, (size_t)b->num_nodes2 * sizeof(rt::Node2), hipMemcpyDeviceToHost,
+                              b->ctx->stream));
+        RT_HIP(hipMemcpyAsync(tris, b->gpu.tris, ((size_t)b->host.num_tris + 1) * sizeof(rt::Tri3),
+                              hipMemcpyDeviceToHost, b->ctx->stream));
+        RT_HIP(hipStreamSynchronize(b->ctx->stream));
+        return RT_OK;
+    }
     std::memcpy(nodes, b->host.nodes.data(), b->host.nodes.size() * sizeof(rt::Node2));
     // num_tris records plus the sentinel
     std::memcpy(tris, b->host.tris.data(), b->host.tris.size() * sizeof(rt::Tri3));
@@ -326,6 +437,15 @@ rt_status rt_bvh_export(const rt_bvh *b, float *nodes, float *tris)
 rt_status rt_bvh_export4(const rt_bvh *b, float *nodes4)
 {
     if (!b || !nodes4) { set_error("rt_bvh_export4: null argument"); return RT_ERR_INVALID; }
+    if (b->on_device) {
+        if (!b->num_nodes4) return RT_OK;  // a two-wide device build has no wide nodes
+        std::lock_guard<std::recursive_mutex> g(b->ctx->lock);
+        RT_HIP(hipSetDevice(b->ctx->device));
+        RT_HIP(hipMemcpyAsync(nodes4, b->gpu.nodes4, (size_t)b->num_nodes4 * sizeof(rt::Node4), hipMemcpyDeviceToHost,
+                              b->ctx->stream));
+        RT_HIP(hipStreamSynchronize(b->ctx->stream));
+        return RT_OK;
+    }
     std::memcpy(nodes4, b->host.nodes4.data(), b->host.nodes4.size() * sizeof(rt::Node4));
     return RT_OK;
 }

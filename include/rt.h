@@ -105,6 +105,24 @@ typedef struct rt_bvh rt_bvh;
 
 rt_status rt_bvh_build(rt_ctx *ctx, const float *xyz, uint32_t nv, const uint32_t *idx,
                        uint32_t nt, const rt_bvh_opts *opts, rt_bvh **out);
+/* The same tree object built on the GPU, from DEVICE arrays on ctx's device
+ * (xyz: nv x 3 floats, idx: nt x 3 vertex indices), ordered on ctx's stream;
+ * returns when the tree is ready.  This is the reference's own arrangement:
+ * its BVH constructor (cuda_libraries/BVH.cu:314-341) takes the triangles
+ * already on the device and builds there (Morton codes, sort, pairing,
+ * BVH.cu:136-151).  Here: 30-bit Morton codes of the triangle centres, a
+ * radix sort, a binary radix tree over the sorted codes (one internal node
+ * per adjacent pair of leaves, ties split by index), boxes bottom-up,
+ * subtrees of at most max_leaf triangles collapsed into leaves, then the
+ * same 64-byte node records, padding and four-wide collapse as
+ * rt_bvh_build.  opts->bins and opts->trav_cost are ignored.  The tree's SAH
+ * cost is higher than the binned-SAH build's (rt_bvh_info.sah_cost), so
+ * traces are slower on it; the build takes milliseconds instead of
+ * hundreds.  Everything rt_bvh_build's tree supports works on it; export4
+ * of a two-wide build copies nothing (num_nodes4 = 0).  Invalid indices or
+ * non-finite coordinates return RT_ERR_INVALID. */
+rt_status rt_bvh_build_device(rt_ctx *ctx, const float *xyz, uint32_t nv, const uint32_t *idx,
+                              uint32_t nt, const rt_bvh_opts *opts, rt_bvh **out);
 rt_status rt_bvh_destroy(rt_bvh *bvh);
 rt_status rt_bvh_get_info(const rt_bvh *bvh, rt_bvh_info *info);
 /* Per-triangle transparency (host array of nt bytes, indexed by triangle

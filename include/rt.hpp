@@ -621,6 +621,15 @@ public:
         check(rt_bvh_build(ctx ? ctx->get() : nullptr, xyz.data(), (uint32_t)(xyz.size() / 3), idx.data(),
                            (uint32_t)(idx.size() / 3), &o, &h_));
     }
+    // built on the GPU from device arrays (rt_bvh_build_device): xyz holds
+    // nv x 3 floats and idx nt x 3 indices on ctx's device
+    struct OnDevice {};
+    BVH(OnDevice, Context &ctx, const float *xyz, uint32_t nv, const uint32_t *idx, uint32_t nt,
+        uint32_t max_leaf = 4, uint32_t width = 4)
+    {
+        rt_bvh_opts o{max_leaf, 32, 1.2f, width};
+        check(rt_bvh_build_device(ctx.get(), xyz, nv, idx, nt, &o, &h_));
+    }
     ~BVH() { if (h_) rt_bvh_destroy(h_); }
     BVH(const BVH &) = delete;
     BVH &operator=(const BVH &) = delete;

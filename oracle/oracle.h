@@ -122,6 +122,13 @@ void orc_bvh2_trace(const float *nodes, const float *tris, const float *ox,
 /* Traversal of the library's four-wide quantized layout (include/rt.h:
  * rt_bvh_export4), operation for operation as the kernel walks it: counts
  * node visits and triangle tests and reports the deepest stack used. */
+/* The GPU-built BVH restated on the CPU (lbvh.c): nodes2 holds
+ * max(1, nt - 1) x 16 floats, tris (nt + 1) x 12, nodes4 max(1, nt - 1) x 16
+ * (written when wide); info = num_nodes2, num_leaves, max_depth,
+ * num_nodes4, depth4, max_stack4.  Returns -1 on bad input. */
+int orc_lbvh_build(const float *xyz, uint32_t nv, const uint32_t *idx, uint32_t nt, uint32_t max_leaf, int wide,
+                   float *nodes2, float *tris, float *nodes4, uint32_t *info);
+
 void orc_bvh4_trace(const float *nodes4, const float *tris, const float *ox,
                     const float *oy, const float *oz, const float *dx,
                     const float *dy, const float *dz, const float *tmax,

@@ -69,6 +69,15 @@ def test_bad_options_rejected(rt):
     assert L.rt_bvh_build(None, xyz.ctypes.data, 30, idx.ctypes.data, 10, C.byref(o), C.byref(h)) == rt.RT_ERR_INVALID
 
 
+def test_device_build_needs_a_context(rt):
+    """rt_bvh_build_device checks its arguments before any device call."""
+    L = rt.lib()
+    h = C.c_void_p()
+    assert L.rt_bvh_build_device(None, None, 0, None, 0, None, C.byref(h)) == rt.RT_ERR_INVALID
+    assert b"ctx" in L.rt_last_error()
+    assert not h.value
+
+
 def test_generators_deterministic(rt):
     a, _ = rt.gen_soup(1000, seed=7)
     b, _ = rt.gen_soup(1000, seed=7)

@@ -5,19 +5,23 @@
 // triangles (cuda_libraries/BVH.cu:136-151, 314-341: codes, a sort, leaves
 // initialised in parallel, then pairing).  This builder follows the same
 // idea with a linear-time radix tree:
-//   1. triangle boxes, and the bounds of their centres (k_tri_boxes);
+//   1. triangle boxes, and the bounds of their centres (k_tri_boxes per
+//      block, k_bounds over the blocks: no contended atomics);
 //   2. 30-bit Morton codes of the centres in those bounds, sorted with the
 //      triangle index as the value (rocPRIM radix sort);
 //   3. Karras' binary radix tree over the sorted codes (k_radix_tree: each
 //      internal node finds its range and split from the common prefixes of
 //      its neighbours; equal codes are split by index);
-//   4. boxes bottom-up (k_node_boxes: the second child to finish computes
-//      its parent, with device-scope fences between the two);
-//   5. subtrees of at most max_leaf triangles become leaves (their
-//      triangles are consecutive in code order), the other internal nodes
-//      are numbered by a prefix sum and written as Node2 records with both
-//      child boxes (k_emit_node2), and the triangles as Tri3 records in code
-//      order plus the degenerate sentinel;
+//   4. boxes and SAH costs bottom-up in passes (k_node_pass: a node is
+//      computed once both children were computed by an earlier launch, so
+//      nothing is read across work-groups within a launch and no
+//      device-scope fence is needed -- on this GPU each such fence writes
+//      back the XCD's L2); a subtree of at most max_leaf triangles becomes
+//      a leaf when that is no dearer by the SAH;
+//   5. the kept internal nodes are numbered by a prefix sum and written as
+//      Node2 records with both child boxes (k_emit_node2), and the
+//      triangles as Tri3 records in code order plus the degenerate
+//      sentinel (k_emit_tris);
 //   6. for the four-wide layout, the Node2 tree is collapsed level by level
 //      (k_collapse4_count, a prefix sum, k_collapse4): each wide node opens
 //      its largest-area internal child until it has four, as the host
@@ -62,12 +66,13 @@ __device__ inline uint32_t spread10(uint32_t v)
     return v;
 }
 
-// Per triangle: its box (6 floats: lo xyz, hi xyz).  Reduced into cb as
-// ordered ints: cb[0..2] / cb[3..5] the min / max of the centres, cb[6..8] /
-// cb[9..11] the min / max of the boxes.  Indices out of range or non-finite
-// coordinates set *bad.
+// Per triangle: its box (6 floats: lo xyz, hi xyz).  Each block reduces
+// its triangles' bounds into part[block * 12 ..] as ordered ints: [0..2] /
+// [3..5] the min / max of the centres, [6..8] / [9..11] the min / max of the
+// boxes (k_bounds finishes the reduction).  Indices out of range or
+// non-finite coordinates set *bad.
 __global__ __launch_bounds__(kBB) void k_tri_boxes(const float *xyz, uint32_t nv, const uint32_t *idx, uint32_t nt,
-                                                   float *tbox, uint32_t *cb, uint32_t *bad)
+                                                   float *tbox, uint32_t *part, uint32_t *bad)
 {
     __shared__ uint32_t smin[6][kBB], smax[6][kBB];
     const uint32_t t = blockIdx.x * kBB + threadIdx.x;
@@ -107,13 +112,35 @@ __global__ __launch_bounds__(kBB) void k_tri_boxes(const float *xyz, uint32_t nv
             }
         __syncthreads();
     }
-    if (threadIdx.x == 0)
-        for (int a = 0; a < 3; ++a) {
-            atomicMin(&cb[a], smin[a][0]);
-            atomicMax(&cb[3 + a], smax[a][0]);
-            atomicMin(&cb[6 + a], smin[3 + a][0]);
-            atomicMax(&cb[9 + a], smax[3 + a][0]);
+    if (threadIdx.x < 12) {
+        const uint32_t j = threadIdx.x, a = j % 3, q = j / 3;  // q: centre min, centre max, box min, box max
+        const uint32_t v = (q & 1u) ? smax[(q >> 1) * 3 + a][0] : smin[(q >> 1) * 3 + a][0];
+        part[(size_t)blockIdx.x * 12 + j] = v;
+    }
+}
+
+// One block: the per-block bounds of k_tri_boxes into cb[0..11].
+__global__ __launch_bounds__(kBB) void k_bounds(const uint32_t *part, uint32_t nblocks, uint32_t *cb)
+{
+    __shared__ uint32_t red[12][kBB];
+    uint32_t v[12];
+    for (int j = 0; j < 12; ++j) v[j] = ((j / 3) & 1) ? 0u : 0xffffffffu;
+    for (uint32_t b = threadIdx.x; b < nblocks; b += kBB)
+        for (int j = 0; j < 12; ++j) {
+            const uint32_t x = part[(size_t)b * 12 + j];
+            v[j] = ((j / 3) & 1) ? max(v[j], x) : min(v[j], x);
         }
+    for (int j = 0; j < 12; ++j) red[j][threadIdx.x] = v[j];
+    __syncthreads();
+    for (int st = kBB / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int j = 0; j < 12; ++j) {
+                const uint32_t x = red[j][threadIdx.x + st];
+                red[j][threadIdx.x] = ((j / 3) & 1) ? max(red[j][threadIdx.x], x) : min(red[j][threadIdx.x], x);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 12) cb[threadIdx.x] = red[threadIdx.x][0];
 }
 
 // Outward padding of every child box, the host build's rule: 2^-19 of
@@ -173,6 +200,7 @@ __device__ inline int32_t leaf_ref(uint32_t first, uint32_t count)
 
 // Child reference: internal node c, or leaf (sorted position) c | kLeafBit.
 constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kNotReady = 0xffffffffu;  // pass index of a node not computed yet
 
 __global__ __launch_bounds__(kBB) void k_radix_tree(const uint32_t *key, uint32_t n, uint32_t *child, uint32_t *lo,
                                                     uint32_t *hi, uint32_t *parent)
@@ -208,57 +236,83 @@ __global__ __launch_bounds__(kBB) void k_radix_tree(const uint32_t *key, uint32_
     parent[(c1 & kLeafBit) ? (N - 1 + (c1 & ~kLeafBit)) : c1] = (uint32_t)i;
 }
 
-__device__ inline void child_box(uint32_t c, const float *tbox, const uint32_t *val, const float *nbox, float b[6])
-{
-    const float *src = (c & kLeafBit) ? tbox + 6 * (size_t)val[c & ~kLeafBit] : nbox + 6 * (size_t)c;
-    for (int k = 0; k < 6; ++k) b[k] = __hip_atomic_load(src + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Bottom-up boxes: every leaf walks up; at each internal node the first of
-// its two children to arrive stops, the second computes the node's box and
-// goes on.  Each box store is released (device-scope fence) before the
-// arrival counter's increment and acquired after it, so the second arrival
-// sees its sibling's box wherever it was computed; the loads above bypass
-// the non-coherent caches.
-__global__ __launch_bounds__(kBB) void k_node_boxes(const uint32_t *child, const uint32_t *parent, const float *tbox,
-                                                    const uint32_t *val, uint32_t n, float *nbox, uint32_t *arrive)
-{
-    const uint32_t k = blockIdx.x * kBB + threadIdx.x;
-    if (k >= n) return;
-    uint32_t p = parent[n - 1 + k];
-    for (;;) {
-        __threadfence();
-        if (atomicAdd(&arrive[p], 1u) == 0) return;
-        __threadfence();
-        float b0[6], b1[6];
-        child_box(child[2 * p], tbox, val, nbox, b0);
-        child_box(child[2 * p + 1], tbox, val, nbox, b1);
-        for (int a = 0; a < 3; ++a) {
-            __hip_atomic_store(nbox + 6 * (size_t)p + a, fminf(b0[a], b1[a]), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(nbox + 6 * (size_t)p + 3 + a, fmaxf(b0[3 + a], b1[3 + a]), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (p == 0) return;
-        p = parent[p];
-    }
-}
-
-// Kept internal nodes: the root and every node with more than max_leaf
-// triangles (a smaller subtree is a leaf of its parent).
-__global__ __launch_bounds__(kBB) void k_keep(const uint32_t *lo, const uint32_t *hi, uint32_t n, uint32_t max_leaf,
-                                              uint32_t *keep)
-{
-    const uint32_t i = blockIdx.x * kBB + threadIdx.x;
-    if (i >= n - 1) return;
-    keep[i] = (i == 0 || hi[i] - lo[i] + 1 > max_leaf) ? 1u : 0u;
-}
-
 __device__ inline double area6(const float b[6])
 {
     double d[3];
     for (int a = 0; a < 3; ++a) d[a] = fmax(0.0, (double)b[3 + a] - (double)b[a]);
     return 2.0 * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]);
+}
+
+// Bottom-up boxes and SAH costs in passes: pass p computes every internal
+// node whose children are leaves or were computed in an earlier pass
+// (ready[c] < p).  All it reads was written by earlier launches, so no
+// fence or coherent load is needed; the root is done after as many passes
+// as the tree is high.  Per node: its box (the union of its children's),
+// its SAH cost (cost of a triangle test 1, of a node visit trav_cost, per
+// unit of area) and its choice: a node over at most max_leaf triangles
+// becomes a leaf when (triangles x area) does not exceed (trav_cost x area
+// + its children's costs); and the height of the kept tree below it (0
+// for a node that became a leaf).  The root never becomes a leaf.
+__global__ __launch_bounds__(kBB) void k_node_pass(const uint32_t *child, const uint32_t *lo, const uint32_t *hi,
+                                                   const float *tbox, const uint32_t *val, uint32_t n,
+                                                   uint32_t pass, uint32_t max_leaf, double trav_cost, float *nbox,
+                                                   double *cost, uint32_t *height, uint32_t *leafc, uint32_t *ready)
+{
+    const uint32_t i = blockIdx.x * kBB + threadIdx.x;
+    if (i >= n - 1 || ready[i] != kNotReady) return;
+    const uint32_t c[2] = {child[2 * i], child[2 * i + 1]};
+    for (int k = 0; k < 2; ++k)
+        if (!(c[k] & kLeafBit) && !(ready[c[k]] < pass)) return;
+    float b[2][6];
+    double cc[2];
+    uint32_t hh[2];
+    for (int k = 0; k < 2; ++k) {
+        if (c[k] & kLeafBit) {
+            const float *src = tbox + 6 * (size_t)val[c[k] & ~kLeafBit];
+            for (int j = 0; j < 6; ++j) b[k][j] = src[j];
+            cc[k] = area6(b[k]);
+            hh[k] = 0;
+        } else {
+            for (int j = 0; j < 6; ++j) b[k][j] = nbox[6 * (size_t)c[k] + j];
+            cc[k] = cost[c[k]];
+            hh[k] = height[c[k]];
+        }
+    }
+    float box[6];
+    for (int a = 0; a < 3; ++a) {
+        box[a] = fminf(b[0][a], b[1][a]);
+        box[3 + a] = fmaxf(b[0][3 + a], b[1][3 + a]);
+    }
+    const double area = area6(box);
+    const double split = trav_cost * area + cc[0] + cc[1];
+    const uint32_t size = hi[i] - lo[i] + 1;
+    const double as_leaf = (double)size * area;
+    const bool leaf = i != 0 && size <= max_leaf && as_leaf <= split;
+    for (int j = 0; j < 6; ++j) nbox[6 * (size_t)i + j] = box[j];
+    cost[i] = leaf ? as_leaf : split;
+    height[i] = leaf ? 0u : 1u + (hh[0] > hh[1] ? hh[0] : hh[1]);
+    leafc[i] = leaf ? 1u : 0u;
+    ready[i] = pass;
+}
+
+// Kept internal nodes: the root, and every node neither of whose own
+// choice nor any ancestor's choice was a leaf (only nodes over at most
+// max_leaf triangles choose; the walk up stops at the first larger one).
+__global__ __launch_bounds__(kBB) void k_keep(const uint32_t *lo, const uint32_t *hi, const uint32_t *parent,
+                                              const uint32_t *leafc, uint32_t n, uint32_t max_leaf, uint32_t *keep)
+{
+    const uint32_t i = blockIdx.x * kBB + threadIdx.x;
+    if (i >= n - 1) return;
+    uint32_t k = 1;
+    if (i != 0) {
+        if (leafc[i]) {
+            k = 0;
+        } else {
+            for (uint32_t p = parent[i]; p != 0 && hi[p] - lo[p] + 1 <= max_leaf; p = parent[p])
+                if (leafc[p]) { k = 0; break; }
+        }
+    }
+    keep[i] = k;
 }
 
 // Node2 record of each kept node, its children as node indices or leaf
@@ -336,22 +390,6 @@ __global__ __launch_bounds__(kBB) void k_emit_tris(const float *xyz, uint32_t nv
         t.id = (int32_t)tri;
     }
     tris[k] = t;
-}
-
-// Levels of the kept tree above each leaf (the two-wide traversal's stack
-// bound is the deepest).
-__global__ __launch_bounds__(kBB) void k_depth(const uint32_t *parent, const uint32_t *keep, uint32_t n,
-                                               uint32_t *maxdepth)
-{
-    const uint32_t k = blockIdx.x * kBB + threadIdx.x;
-    if (k >= n) return;
-    uint32_t p = parent[n - 1 + k], d = 0;
-    for (;;) {
-        d += keep[p];
-        if (p == 0) break;
-        p = parent[p];
-    }
-    atomicMax(maxdepth, d);
 }
 
 // ---- four-wide collapse (bvh_build.cpp's Collapser, level by level) ----
@@ -562,11 +600,13 @@ hipError_t build_bvh_device(const float *xyz, uint32_t nv, const uint32_t *idx, 
         out.sah_cost = area > 0.0 ? 1.0 : 0.0;
     } else {
         const size_t N = nt;
+        const size_t nb = blocks(N);
         size_t off = 0;
         auto take = [&](size_t bytes) { size_t r = off; off += (bytes + 255) & ~(size_t)255; return r; };
-        const size_t o_tbox = take(N * 24), o_cb = take(16 * 4), o_key = take(N * 4), o_key2 = take(N * 4),
-                     o_val = take(N * 4), o_val2 = take(N * 4), o_child = take(2 * N * 4), o_lo = take(N * 4),
-                     o_hi = take(N * 4), o_parent = take(2 * N * 4), o_nbox = take(N * 24), o_arrive = take(N * 4),
+        const size_t o_tbox = take(N * 24), o_cb = take(16 * 4), o_part = take(nb * 48), o_key = take(N * 4),
+                     o_key2 = take(N * 4), o_val = take(N * 4), o_val2 = take(N * 4), o_child = take(2 * N * 4),
+                     o_lo = take(N * 4), o_hi = take(N * 4), o_parent = take(2 * N * 4), o_nbox = take(N * 24),
+                     o_cost = take(N * 8), o_height = take(N * 4), o_leafc = take(N * 4), o_ready = take(N * 4),
                      o_keep = take(N * 4), o_new = take(N * 4);
         size_t sort_bytes = 0, scan_bytes = 0;
         if ((e = hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
@@ -580,50 +620,68 @@ hipError_t build_bvh_device(const float *xyz, uint32_t nv, const uint32_t *idx, 
         if ((e = hipMalloc(&ar.base, off)) != hipSuccess) return fail(e, "work space");
         char *b = ar.base;
         float *tbox = (float *)(b + o_tbox), *nbox = (float *)(b + o_nbox);
-        // cb: centre and box bounds (12 words), bad-input flag, deepest kept
-        // level, then the SAH sum (a double at word 14)
-        uint32_t *cb = (uint32_t *)(b + o_cb), *bad = cb + 12, *maxd = cb + 13;
-        double *sah = (double *)(cb + 14);
-        uint32_t *key = (uint32_t *)(b + o_key), *key2 = (uint32_t *)(b + o_key2), *val = (uint32_t *)(b + o_val),
-                 *val2 = (uint32_t *)(b + o_val2), *child = (uint32_t *)(b + o_child), *lo = (uint32_t *)(b + o_lo),
-                 *hi = (uint32_t *)(b + o_hi), *parent = (uint32_t *)(b + o_parent),
-                 *arrive = (uint32_t *)(b + o_arrive), *keep = (uint32_t *)(b + o_keep),
-                 *newidx = (uint32_t *)(b + o_new);
-        const uint32_t init[14] = {~0u, ~0u, ~0u, 0, 0, 0, ~0u, ~0u, ~0u, 0, 0, 0, 0, 0};
-        if ((e = hipMemcpyAsync(cb, init, sizeof init, hipMemcpyHostToDevice, s)) != hipSuccess ||
-            (e = hipMemsetAsync(sah, 0, 8, s)) != hipSuccess || (e = hipMemsetAsync(arrive, 0, N * 4, s)) != hipSuccess)
+        // cb: centre and box bounds (12 words), the bad-input flag, then the
+        // SAH sum (a double at word 14)
+        uint32_t *cb = (uint32_t *)(b + o_cb), *bad = cb + 12;
+        double *sah = (double *)(cb + 14), *cost = (double *)(b + o_cost);
+        uint32_t *part = (uint32_t *)(b + o_part), *key = (uint32_t *)(b + o_key), *key2 = (uint32_t *)(b + o_key2),
+                 *val = (uint32_t *)(b + o_val), *val2 = (uint32_t *)(b + o_val2), *child = (uint32_t *)(b + o_child),
+                 *lo = (uint32_t *)(b + o_lo), *hi = (uint32_t *)(b + o_hi), *parent = (uint32_t *)(b + o_parent),
+                 *height = (uint32_t *)(b + o_height), *leafc = (uint32_t *)(b + o_leafc),
+                 *ready = (uint32_t *)(b + o_ready), *keep = (uint32_t *)(b + o_keep), *newidx = (uint32_t *)(b + o_new);
+        if ((e = hipMemsetAsync(cb, 0, 16 * 4, s)) != hipSuccess ||
+            (e = hipMemsetAsync(ready, 0xff, N * 4, s)) != hipSuccess)
             return fail(e, "clear");
-        hipLaunchKernelGGL(k_tri_boxes, dim3(blocks(N)), dim3(kBB), 0, s, xyz, nv, idx, nt, tbox, cb, bad);
-        hipLaunchKernelGGL(k_morton, dim3(blocks(N)), dim3(kBB), 0, s, tbox, nt, cb, key, val);
+        const double trav_cost = o.trav_cost > 0 ? o.trav_cost : 1.2f;
+        hipLaunchKernelGGL(k_tri_boxes, dim3(nb), dim3(kBB), 0, s, xyz, nv, idx, nt, tbox, part, bad);
+        hipLaunchKernelGGL(k_bounds, dim3(1), dim3(kBB), 0, s, part, (uint32_t)nb, cb);
+        hipLaunchKernelGGL(k_morton, dim3(nb), dim3(kBB), 0, s, tbox, nt, cb, key, val);
         if ((e = hipGetLastError()) != hipSuccess) return fail(e, "boxes");
         if ((e = hipcub::DeviceRadixSort::SortPairs(b + o_tmp, sort_bytes, key, key2, val, val2, (int)N, 0, 30, s)) !=
             hipSuccess)
             return fail(e, "sort");
         hipLaunchKernelGGL(k_radix_tree, dim3(blocks(N - 1)), dim3(kBB), 0, s, key2, nt, child, lo, hi, parent);
-        hipLaunchKernelGGL(k_node_boxes, dim3(blocks(N)), dim3(kBB), 0, s, child, parent, tbox, val2, nt, nbox, arrive);
-        hipLaunchKernelGGL(k_keep, dim3(blocks(N - 1)), dim3(kBB), 0, s, lo, hi, nt, max_leaf, keep);
         if ((e = hipGetLastError()) != hipSuccess) return fail(e, "radix tree");
+        // passes in batches of eight until the root is done; a radix tree of
+        // 30-bit codes split further by 32-bit positions is at most 62 high
+        uint32_t pass = 0, root_ready = kNotReady;
+        while (root_ready == kNotReady) {
+            if (pass >= 64) {
+                err = "rt_bvh_build_device: radix tree higher than 64 levels";
+                return hipErrorUnknown;
+            }
+            for (int k = 0; k < 8; ++k) {
+                ++pass;
+                hipLaunchKernelGGL(k_node_pass, dim3(blocks(N - 1)), dim3(kBB), 0, s, child, lo, hi, tbox, val2, nt,
+                                   pass, max_leaf, trav_cost, nbox, cost, height, leafc, ready);
+            }
+            if ((e = hipGetLastError()) != hipSuccess ||
+                (e = hipMemcpyAsync(&root_ready, ready, 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+                (e = hipStreamSynchronize(s)) != hipSuccess)
+                return fail(e, "boxes");
+        }
+        hipLaunchKernelGGL(k_keep, dim3(blocks(N - 1)), dim3(kBB), 0, s, lo, hi, parent, leafc, nt, max_leaf, keep);
+        if ((e = hipGetLastError()) != hipSuccess) return fail(e, "collapse");
         if ((e = hipcub::DeviceScan::ExclusiveSum(b + o_tmp, scan_bytes, keep, newidx, (int)(N - 1), s)) != hipSuccess)
             return fail(e, "scan");
         hipLaunchKernelGGL(k_emit_node2, dim3(blocks(N - 1)), dim3(kBB), 0, s, child, lo, hi, keep, newidx, tbox, val2,
                            nbox, nt, cb, out.nodes2, sah);
         hipLaunchKernelGGL(k_emit_tris, dim3(blocks(N + 1)), dim3(kBB), 0, s, xyz, nv, idx, val2, nt, out.tris);
-        hipLaunchKernelGGL(k_depth, dim3(blocks(N)), dim3(kBB), 0, s, parent, keep, nt, maxd);
         if ((e = hipGetLastError()) != hipSuccess) return fail(e, "emit");
-        uint32_t last[2] = {0, 0}, flags[2] = {0, 0};
+        uint32_t last[2] = {0, 0}, bad_h = 0, height_h = 0;
         double sah_h = 0.0;
         float root_box[6];
         if ((e = hipMemcpyAsync(last, newidx + (N - 2), 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
             (e = hipMemcpyAsync(last + 1, keep + (N - 2), 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
-            (e = hipMemcpyAsync(flags, bad, 8, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+            (e = hipMemcpyAsync(&bad_h, bad, 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+            (e = hipMemcpyAsync(&height_h, height, 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
             (e = hipMemcpyAsync(&sah_h, sah, 8, hipMemcpyDeviceToHost, s)) != hipSuccess ||
             (e = hipMemcpyAsync(root_box, nbox, 24, hipMemcpyDeviceToHost, s)) != hipSuccess ||
             (e = hipStreamSynchronize(s)) != hipSuccess)
             return fail(e, "read-back");
-        if (flags[0]) { err = "rt_bvh_build_device: triangle index out of range or non-finite coordinate"; return hipErrorInvalidValue; }
+        if (bad_h) { err = "rt_bvh_build_device: triangle index out of range or non-finite coordinate"; return hipErrorInvalidValue; }
         out.num_nodes2 = last[0] + last[1];
-        out.max_depth = flags[1];
-        out.num_leaves = 0;
+        out.max_depth = height_h;
         double ra = 0.0;
         {
             double d[3];
@@ -631,7 +689,7 @@ hipError_t build_bvh_device(const float *xyz, uint32_t nv, const uint32_t *idx, 
             ra = 2.0 * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]);
         }
         out.sah_cost = ra > 0 ? sah_h / ra : 0.0;
-        // leaves = kept nodes' leaf children: a kept tree of m nodes has m + 1
+        // a binary tree of m kept nodes has m + 1 leaves
         out.num_leaves = out.num_nodes2 + 1;
     }
     if (wide) {

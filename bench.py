@@ -782,6 +782,108 @@ def run_photon(args):
     return res
 
 
+def run_bvh_build(args):
+    """The BVH built on the GPU (rt_bvh_build_device, the arrangement of the
+    reference's BVH constructor, cuda_libraries/BVH.cu:314-341) over the 1M
+    triangle soup of the closest-hit workload (seed 7), vertices and indices
+    resident in HBM.  One step = one build (work space allocation, Morton
+    sort, radix tree, boxes, two-wide and four-wide records); value =
+    Mtriangles/s.  Beside it: closest-hit Mrays/s over the same 16M rays on
+    the tree built here and on the host's binned-SAH tree, the quality the
+    build trades for its speed."""
+    import numpy as np
+    import torch
+
+    rt = importlib.import_module("2023_render_engine_amd")
+    dev = torch.device("cuda", 0)
+    xyz, idx = rt.gen_soup(1 << 20, seed=7)
+    nt = idx.shape[0]
+    ctx = rt.Context(0)
+    dx = torch.from_numpy(xyz).to(dev)
+    di = torch.from_numpy(idx.view(np.int32)).to(dev)
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        rt.Bvh.from_device(ctx, dx, di).close()
+    times = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        b = rt.Bvh.from_device(ctx, dx, di)
+        times.append(time.perf_counter() - t1)
+        b.close()
+    wall = time.perf_counter() - t0
+    ms = sum(times) / len(times) * 1e3
+    lb = rt.Bvh.from_device(ctx, dx, di)
+    inf = lb.info()
+    # compulsory bytes of one build: each triangle's three indices and nine
+    # coordinates in (48 B), its 48 B record out, 64 B per two-wide and
+    # four-wide node out
+    alg = nt * (48 + 48) + 64 * (inf.num_nodes + inf.num_nodes4)
+    gbs = alg / (ms * 1e-3) / 1e9
+
+    rays_np = rt.gen_rays(4096, 2048, eye=(0.5, 0.5, -1.0), vfov=60.0, n_incoherent=8 << 20, seed=7)
+    n = rays_np[0].shape[0]
+    rays = [torch.from_numpy(a).to(dev) for a in rays_np]
+    prim = torch.empty(n, dtype=torch.int32, device=dev)
+    t = torch.empty(n, dtype=torch.float32, device=dev)
+    s = torch.cuda.current_stream(dev).cuda_stream
+
+    def trace_rate(bvh):
+        for _ in range(2):
+            bvh.trace_closest(*rays, prim=prim, t=t, stream=s)
+        torch.cuda.synchronize()
+        ks = []
+        for _ in range(5):
+            bvh.trace_closest(*rays, prim=prim, t=t, stream=s)
+            ks.append(bvh.last_kernel_ms())
+        nn, ntr = bvh.trace_stats(*rays)
+        return round(n / (sum(ks) / len(ks) / 1e3) / 1e6, 1), round(nn / n, 2), round(ntr / n, 3)
+
+    lb_rate, lb_nodes, lb_tris = trace_rate(lb)
+    t1 = time.time()
+    sah = rt.Bvh(ctx, xyz, idx, max_leaf=4, bins=32, width=4)
+    sah_wall = time.time() - t1
+    sinf = sah.info()
+    sah_rate, sah_nodes, sah_tris = trace_rate(sah)
+    res = {
+        "metric": "BVH build on the GPU", "value": round(nt * args.steps / wall / 1e6, 2),
+        "unit": "Mtriangles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "rt_bvh_build_device over the 1M-triangle soup (seed 7), leaves <= 4, four-wide",
+                   "triangles": nt, "nodes2": int(inf.num_nodes), "nodes4": int(inf.num_nodes4),
+                   "depth4": int(inf.depth4), "stack_cap": int(inf.stack_cap)},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None,
+                     "note": ("whole build, not one kernel: compulsory bytes (96 B per triangle + 64 B per node "
+                              "record) / the build's wall time; the build is a chain of short launches and "
+                              "per-level read-backs, so it is latency bound")},
+        "build_ms": round(ms, 3),
+        "sah_cost": round(inf.sah_cost, 2),
+        "trace_mrays_device_tree": lb_rate,
+        "node_visits_per_ray_device_tree": lb_nodes,
+        "tri_tests_per_ray_device_tree": lb_tris,
+        "host_sah_build_ms": round(sinf.build_ms, 1),
+        "host_sah_build_wall_s": round(sah_wall, 2),
+        "host_sah_cost": round(sinf.sah_cost, 2),
+        "trace_mrays_host_sah_tree": sah_rate,
+        "node_visits_per_ray_host_sah_tree": sah_nodes,
+        "tri_tests_per_ray_host_sah_tree": sah_tris,
+    }
+    if not args.no_cpu_baseline:
+        import oracle
+        t1 = time.time()
+        oracle.lbvh_build(xyz, idx)
+        cs = time.time() - t1
+        res["cpu_baseline"] = {"value": round(nt / cs / 1e6, 3), "unit": "Mtriangles/s", "cores": 1, "kind": "port",
+                               "sample": f"one build of the same 1M triangles by the oracle's restatement "
+                                         f"(oracle/lbvh.c), {cs:.1f} s"}
+    lb.close()
+    sah.close()
+    ctx.close()
+    return res
+
+
 def run_photon_pass(args):
     """The photon pre-pass (generatePhotons + computePhotons,
     integration/photonDispersion.cu:26-192) at the reference's size: 2^20
@@ -872,7 +974,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c2", "c3", "c4", "copy", "film", "photon", "photon_pass"])
+    ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c2", "c3", "c4", "copy", "film", "photon", "photon_pass",
+                                                         "bvh_build"])
     ap.add_argument("--tone", default="gammaClamp", help="film: tone mapping (a ToneMappingType name)")
     ap.add_argument("--c3-ref-spp", type=int, default=128)
     ap.add_argument("--ref-spp", type=int, default=1024,
@@ -918,6 +1021,8 @@ def main():
         out = run_photon(args)
     elif args.workload == "photon_pass":
         out = run_photon_pass(args)
+    elif args.workload == "bvh_build":
+        out = run_bvh_build(args)
     else:
         out = run_render(args, args.workload)
     print(json.dumps(out), flush=True)

@@ -112,15 +112,17 @@ rt_status rt_bvh_build(rt_ctx *ctx, const float *xyz, uint32_t nv, const uint32_
  * already on the device and builds there (Morton codes, sort, pairing,
  * BVH.cu:136-151).  Here: 30-bit Morton codes of the triangle centres, a
  * radix sort, a binary radix tree over the sorted codes (one internal node
- * per adjacent pair of leaves, ties split by index), boxes bottom-up,
- * subtrees of at most max_leaf triangles collapsed into leaves, then the
- * same 64-byte node records, padding and four-wide collapse as
- * rt_bvh_build.  opts->bins and opts->trav_cost are ignored.  The tree's SAH
- * cost is higher than the binned-SAH build's (rt_bvh_info.sah_cost), so
- * traces are slower on it; the build takes milliseconds instead of
- * hundreds.  Everything rt_bvh_build's tree supports works on it; export4
- * of a two-wide build copies nothing (num_nodes4 = 0).  Invalid indices or
- * non-finite coordinates return RT_ERR_INVALID. */
+ * per adjacent pair of leaves, ties split by index), boxes and SAH costs
+ * bottom-up; a subtree of at most max_leaf triangles becomes a leaf when
+ * (triangles x area) <= (trav_cost x area + its children's costs); then
+ * the same 64-byte node records, padding and four-wide collapse as
+ * rt_bvh_build.  opts->bins is ignored.  The same input always gives the
+ * same records.  The tree's SAH cost is higher than the binned-SAH build's
+ * (rt_bvh_info.sah_cost), so traces are somewhat slower on it; the build
+ * takes milliseconds instead of hundreds.  Everything rt_bvh_build's tree
+ * supports works on it; export4 of a two-wide build copies nothing
+ * (num_nodes4 = 0).  Invalid indices or non-finite coordinates return
+ * RT_ERR_INVALID. */
 rt_status rt_bvh_build_device(rt_ctx *ctx, const float *xyz, uint32_t nv, const uint32_t *idx,
                               uint32_t nt, const rt_bvh_opts *opts, rt_bvh **out);
 rt_status rt_bvh_destroy(rt_bvh *bvh);

@@ -38,7 +38,7 @@ _SIGS = {
     "orc_next_float_up": (_F, [_F]),
     "orc_next_float_down": (_F, [_F]),
     "orc_ray_make": (None, [C.POINTER(OrcRay), _P, _P, _F]),
-    "orc_lbvh_build": (C.c_int, [_P, _U32, _P, _U32, _U32, C.c_int, _P, _P, _P, _P]),
+    "orc_lbvh_build": (C.c_int, [_P, _U32, _P, _U32, _U32, _F, C.c_int, _P, _P, _P, _P]),
     "orc_tri_intersects": (C.c_int, [C.POINTER(OrcRay), _P, _P, _P, C.POINTER(_F), C.POINTER(OrcTriInfo)]),
     "orc_box_intersects": (C.c_int, [_P, _P, C.POINTER(OrcRay), C.POINTER(_F)]),
     "orc_refbvh_build": (_P, [_P, _P, _U32]),
@@ -640,7 +640,7 @@ class RefBvh:
         return prim, t
 
 
-def lbvh_build(xyz, idx, max_leaf=4, wide=True):
+def lbvh_build(xyz, idx, max_leaf=4, wide=True, trav_cost=1.2):
     """The GPU-built BVH restated (lbvh.c): returns (info dict, nodes2
     (m, 16) float32, tris (nt + 1, 12) float32, nodes4 (m4, 16) float32 or
     None), in the library's export layouts."""
@@ -652,8 +652,8 @@ def lbvh_build(xyz, idx, max_leaf=4, wide=True):
     tris = np.zeros((nt + 1, 12), np.float32)
     nodes4 = np.zeros((cap, 16), np.float32)
     info = np.zeros(6, np.uint32)
-    if lib().orc_lbvh_build(_p(x), x.shape[0], _p(ix), nt, max_leaf, int(bool(wide)), _p(nodes2), _p(tris),
-                            _p(nodes4), _p(info)) != 0:
+    if lib().orc_lbvh_build(_p(x), x.shape[0], _p(ix), nt, max_leaf, trav_cost, int(bool(wide)), _p(nodes2),
+                            _p(tris), _p(nodes4), _p(info)) != 0:
         raise ValueError("orc_lbvh_build: bad input")
     keys = ("num_nodes", "num_leaves", "max_depth", "num_nodes4", "depth4", "max_stack4")
     inf = {k: int(v) for k, v in zip(keys, info)}

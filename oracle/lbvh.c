@@ -12,7 +12,8 @@
  * range's last key does; an internal node is numbered by the end of its
  * range that touches its sibling (left child: its last position, right
  * child: its first), the root is 0.  Subtrees of at most max_leaf triangles
- * become leaves; kept nodes are numbered in ascending radix-tree index.
+ * become leaves where the SAH says a leaf is no dearer; kept nodes are
+ * numbered in ascending radix-tree index.
  * The four-wide collapse is a FIFO breadth-first walk that opens the
  * largest-area internal kid until four, with the quantization searched
  * upward from the smallest exponent (not from a logarithm).
@@ -56,13 +57,19 @@ typedef struct {
     uint32_t lo, hi; /* sorted positions, inclusive */
     int32_t child[2]; /* radix node index, or ~position for a single triangle */
     float box[6];
+    double cost;      /* SAH cost of the subtree as built */
+    int leaf;         /* the subtree became one leaf */
 } RNode;
 
 typedef struct {
     const LKey *key;
     const float *tbox; /* per input triangle: lo xyz, hi xyz */
     RNode *nodes;
+    uint32_t max_leaf;
+    double trav_cost;
 } RTree;
+
+static double area_of(const float *b);
 
 static void grow_box(float *b, const float *c)
 {
@@ -72,8 +79,11 @@ static void grow_box(float *b, const float *c)
     }
 }
 
-/* builds node `self` over [lo, hi] (hi > lo), returns nothing; boxes are the
- * exact unions of the triangle boxes */
+/* builds node `self` over [lo, hi] (hi > lo); boxes are the exact unions of
+ * the triangle boxes.  Then the SAH choice: a node over at most max_leaf
+ * triangles (not the root) becomes a leaf when triangles x area <=
+ * trav_cost x area + the children's costs (a single triangle costs its
+ * box's area). */
 static void build_range(RTree *t, uint32_t self, uint32_t lo, uint32_t hi)
 {
     RNode *n = &t->nodes[self];
@@ -101,6 +111,17 @@ static void build_range(RTree *t, uint32_t self, uint32_t lo, uint32_t hi)
         build_range(t, split + 1, split + 1, hi);
         grow_box(n->box, t->nodes[split + 1].box);
     }
+    double cc[2];
+    for (int c = 0; c < 2; ++c) {
+        const int32_t ch = n->child[c];
+        cc[c] = ch < 0 ? area_of(t->tbox + 6 * (size_t)t->key[~ch].index) : t->nodes[ch].cost;
+    }
+    const double area = area_of(n->box);
+    const double split_cost = t->trav_cost * area + cc[0] + cc[1];
+    const uint32_t size = hi - lo + 1;
+    const double leaf_cost = (double)size * area;
+    n->leaf = self != 0 && size <= t->max_leaf && leaf_cost <= split_cost;
+    n->cost = n->leaf ? leaf_cost : split_cost;
 }
 
 static double area_of(const float *b)
@@ -165,8 +186,8 @@ static int smallest_exponent(double extent)
     return e;
 }
 
-int orc_lbvh_build(const float *xyz, uint32_t nv, const uint32_t *idx, uint32_t nt, uint32_t max_leaf, int wide,
-                   float *nodes2, float *tris, float *nodes4, uint32_t *info)
+int orc_lbvh_build(const float *xyz, uint32_t nv, const uint32_t *idx, uint32_t nt, uint32_t max_leaf,
+                   float trav_cost, int wide, float *nodes2, float *tris, float *nodes4, uint32_t *info)
 {
     /* info: num_nodes2, num_leaves, max_depth, num_nodes4, depth4, max_stack4 */
     memset(info, 0, 6 * sizeof(uint32_t));
@@ -237,21 +258,32 @@ int orc_lbvh_build(const float *xyz, uint32_t nv, const uint32_t *idx, uint32_t 
         depth = 1;
         info[1] = nt;
     } else {
-        RTree tr = {key, tbox, calloc(nt - 1, sizeof(RNode))};
+        RTree tr = {key, tbox, calloc(nt - 1, sizeof(RNode)), max_leaf, (double)(trav_cost > 0 ? trav_cost : 1.2f)};
         build_range(&tr, 0, 0, nt - 1);
-        /* kept: the root and nodes over more than max_leaf triangles */
+        /* kept: the root, and below a kept node every internal child that did
+           not become a leaf; numbered in ascending radix-tree index */
         uint32_t *newidx = malloc(sizeof(uint32_t) * (nt - 1));
         uint32_t *lvl = calloc(nt - 1, sizeof(uint32_t));
-        for (uint32_t i = 0; i < nt - 1; ++i) {
-            const int keep = i == 0 || tr.nodes[i].hi - tr.nodes[i].lo + 1 > max_leaf;
-            newidx[i] = keep ? n2++ : UINT32_MAX;
+        uint32_t *stack = malloc(sizeof(uint32_t) * (nt + 1));
+        uint32_t sp = 0;
+        for (uint32_t i = 0; i < nt - 1; ++i) newidx[i] = UINT32_MAX;
+        newidx[0] = 0;
+        stack[sp++] = 0;
+        while (sp) {
+            const RNode *n = &tr.nodes[stack[--sp]];
+            for (int c = 0; c < 2; ++c)
+                if (n->child[c] >= 0 && !tr.nodes[n->child[c]].leaf) {
+                    newidx[n->child[c]] = 0;
+                    stack[sp++] = (uint32_t)n->child[c];
+                }
         }
+        for (uint32_t i = 0; i < nt - 1; ++i)
+            if (newidx[i] != UINT32_MAX) newidx[i] = n2++;
         const float pad = pad_of(smin, smax);
         lvl[0] = 1;
         /* parents have smaller depth than children but not smaller indices:
            walk the kept tree from the root with an explicit stack */
-        uint32_t *stack = malloc(sizeof(uint32_t) * (nt + 1));
-        uint32_t sp = 0;
+        sp = 0;
         stack[sp++] = 0;
         while (sp) {
             const uint32_t i = stack[--sp];

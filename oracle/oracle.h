@@ -126,8 +126,8 @@ void orc_bvh2_trace(const float *nodes, const float *tris, const float *ox,
  * max(1, nt - 1) x 16 floats, tris (nt + 1) x 12, nodes4 max(1, nt - 1) x 16
  * (written when wide); info = num_nodes2, num_leaves, max_depth,
  * num_nodes4, depth4, max_stack4.  Returns -1 on bad input. */
-int orc_lbvh_build(const float *xyz, uint32_t nv, const uint32_t *idx, uint32_t nt, uint32_t max_leaf, int wide,
-                   float *nodes2, float *tris, float *nodes4, uint32_t *info);
+int orc_lbvh_build(const float *xyz, uint32_t nv, const uint32_t *idx, uint32_t nt, uint32_t max_leaf,
+                   float trav_cost, int wide, float *nodes2, float *tris, float *nodes4, uint32_t *info);
 
 void orc_bvh4_trace(const float *nodes4, const float *tris, const float *ox,
                     const float *oy, const float *oz, const float *dx,

@@ -67,6 +67,42 @@ def subtree_range(nodes, link):
     return subtree_range(nodes, ci[0])[0], subtree_range(nodes, ci[1])[1]
 
 
+def area(lo, hi):
+    d = np.maximum(0.0, hi.astype(np.float64) - lo.astype(np.float64))
+    return 2.0 * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0])
+
+
+class SahRule:
+    """The SAH leaf rule recomputed from the sorted keys: the radix subtree
+    over positions [lo, hi] splits at the highest bit in which its first and
+    last keys differ; a subtree over at most max_leaf triangles (not the
+    root) is a leaf when triangles x area <= trav x area + its children's
+    costs, a single triangle costing its box's area."""
+
+    def __init__(self, key, tlo, thi, max_leaf, trav=float(np.float32(1.2))):
+        self.key, self.tlo, self.thi, self.max_leaf, self.trav = key, tlo, thi, max_leaf, trav
+        self.memo = {}
+
+    def cost(self, lo, hi, root=False):
+        if lo == hi:
+            return area(self.tlo[lo], self.thi[lo]), False
+        if (lo, hi, root) in self.memo:
+            return self.memo[(lo, hi, root)]
+        bit = (int(self.key[lo]) ^ int(self.key[hi])).bit_length() - 1
+        s = lo
+        while (int(self.key[s + 1]) >> bit) & 1 == (int(self.key[lo]) >> bit) & 1:
+            s += 1
+        c0, _ = self.cost(lo, s)
+        c1, _ = self.cost(s + 1, hi)
+        a = area(self.tlo[lo:hi + 1].min(axis=0), self.thi[lo:hi + 1].max(axis=0))
+        split = self.trav * a + c0 + c1
+        size = hi - lo + 1
+        leaf = not root and size <= self.max_leaf and size * a <= split
+        r = (size * a if leaf else split), leaf
+        self.memo[(lo, hi, root)] = r
+        return r
+
+
 def stack_need(nodes4, n=0):
     refs = nodes4[n, 4:8].copy().view(np.int32)
     valid = (int(nodes4[n, 3:4].copy().view(np.uint32)[0]) >> 24) & 0xF
@@ -96,14 +132,22 @@ def test_structure_and_order(rt, orc, n, max_leaf, dup):
     if n:
         order, _ = morton_order(xyz, idx)
         assert np.array_equal(ids[:n], order)
-    # leaves hold at most max_leaf triangles, kept nodes more
+    # leaves hold at most max_leaf triangles; every kept node and every leaf
+    # is what the SAH rule chooses
     ci = nodes[:, 12:14].copy().view(np.int32)
-    for i in range(nn):
-        lo, hi = subtree_range(nodes, i) if n > 1 else (0, max(n - 1, 0))
-        assert i == 0 or hi - lo + 1 > max_leaf
-        for c in ci[i]:
-            if c < 0 and n:
-                assert leaf_range(c)[1] <= max_leaf or n <= 1
+    if n > 1:
+        _, code = morton_order(xyz, idx)
+        key = (code << np.uint64(32)) | np.arange(n, dtype=np.uint64)
+        v = xyz[idx[ids[:n]].astype(np.int64)]
+        rule = SahRule(key, v.min(axis=1), v.max(axis=1), max_leaf)
+        for i in range(nn):
+            lo, hi = subtree_range(nodes, i)
+            assert not rule.cost(lo, hi, root=i == 0)[1], "a kept node the rule makes a leaf"
+            for c in ci[i]:
+                if c < 0:
+                    f, cnt = leaf_range(c)
+                    assert cnt <= max_leaf
+                    assert cnt == 1 or rule.cost(f, f + cnt - 1)[1], "a leaf the rule splits"
     reached4, nn4, depth4 = walk4(nodes4, tris)
     assert nn4 == inf["num_nodes4"] and depth4 == inf["depth4"]
     assert sorted(int(ids[k]) for k in reached4 if ids[k] >= 0) == list(range(n))

@@ -177,6 +177,9 @@ rt_status rt_cache_create(rt_ctx *ctx, const rt_cache_opts *opts, rt_cache **out
     const size_t o_y = take((size_t)rows * 16 * 4), o_dy = take((size_t)rows * 16 * 2);
     const size_t o_da0 = take((size_t)rows * 64 * 2), o_da1 = take((size_t)rows * 64 * 2);
     const size_t o_dx = take((size_t)rows * 64 * 4), o_loss = take(4);
+    const size_t sort_bytes = std::max<size_t>(rt::cache_sort_bytes(rows), 1);
+    const size_t o_sk0 = take((size_t)rows * 4), o_sk1 = take((size_t)rows * 4), o_si0 = take((size_t)rows * 4),
+                 o_si1 = take((size_t)rows * 4), o_stmp = take(sort_bytes);
     char *base = nullptr;
     hipError_t e = hipMalloc(&base, total);
     if (e != hipSuccess) {
@@ -198,6 +201,12 @@ rt_status rt_cache_create(rt_ctx *ctx, const rt_cache_opts *opts, rt_cache **out
     w.dact[0] = (uint16_t *)(base + o_da0);
     w.dact[1] = (uint16_t *)(base + o_da1);
     w.dx32 = (float *)(base + o_dx);
+    w.skey[0] = (uint32_t *)(base + o_sk0);
+    w.skey[1] = (uint32_t *)(base + o_sk1);
+    w.sidx[0] = (uint32_t *)(base + o_si0);
+    w.sidx[1] = (uint32_t *)(base + o_si1);
+    w.sort_tmp = base + o_stmp;
+    w.sort_bytes = sort_bytes;
     c->d_loss = (float *)(base + o_loss);
     rt_status st = init_params(c, o.seed, s);
     if (st != RT_OK) {

@@ -43,7 +43,16 @@ struct CacheWork {
     uint16_t *dy16;               // [rows][16] loss gradient
     uint16_t *dact[2];            // [rows][64] ping-pong data gradients
     float *dx32;                  // [rows][64] encoding gradient
+    // the grid gradient visits the batch in Morton order of position:
+    // keys and sample indices (two buffers each for the sort) and the sort's
+    // temporary storage (cache_sort_bytes(rows) bytes)
+    uint32_t *skey[2], *sidx[2];
+    void *sort_tmp;
+    size_t sort_bytes;
 };
+
+// Temporary bytes the batch's Morton sort needs for `rows` samples.
+size_t cache_sort_bytes(uint32_t rows);
 
 // n is the row count (and grid size); with n_dev the kernels read the
 // actual count from device memory (capped at n), so a launch sequence can be

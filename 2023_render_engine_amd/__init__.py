@@ -60,6 +60,13 @@ class PhotonOpts(C.Structure):
     _fields_ = [("photons", C.c_uint32), ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32), ("seed", C.c_uint32)]
 
 
+class PhotonRenderOpts(C.Structure):
+    _fields_ = [("mode", C.c_uint32), ("k", C.c_uint32), ("radius", C.c_float)]
+
+
+RT_RENDER_PHOTON_MAPPING = 1
+
+
 class SssParams(C.Structure):
     _fields_ = [("kd", C.c_float * 3), ("sigma_s", C.c_float * 3), ("sigma_a", C.c_float * 3), ("g", C.c_float),
                 ("eta", C.c_float)]
@@ -167,6 +174,8 @@ _SIGS = {
     "rt_scene_bvh": (_P, [_P]),
     "rt_scene_camera_ray": (C.c_int, [_P, _U32, _U32, C.c_float, C.c_float, _P, _P]),
     "rt_render": (C.c_int, [_P, C.POINTER(RenderOpts), _P, _P, _U32, _P]),
+    "rt_render_photon": (C.c_int, [_P, C.POINTER(RenderOpts), _P, _P, _P, C.POINTER(PhotonRenderOpts), _P, _P, _U32,
+                                   _P]),
     "rt_render_get_stats": (C.c_int, [_P, C.POINTER(RenderStats)]),
     "rt_render_cached": (C.c_int, [_P, C.POINTER(RenderOpts), _P, _U32, _P, _P, _U32, _P]),
     "rt_batch_create": (C.c_int, [_P, C.POINTER(_P), _U32, C.POINTER(_P)]),
@@ -563,6 +572,22 @@ class Scene:
         fl = (RT_FLAG_PROFILE if profile else 0) | (RT_FLAG_COUNT if count else 0)
         self.ctx.launch(lambda s: check(lib().rt_render(self.h, C.byref(o), _ptr(accum), _ptr(nvalid), fl, s)),
                         stream)
+        return film
+
+    def render_photon(self, film, pmap, radiance, direction, k=100, radius=100.0, spp=None, spp_offset=0,
+                      max_depth=None, rr_depth=None, seed=None, path_capacity=0, stream=None):
+        """rt_render_photon (RENDER_MODE PHOTON_MAPPING): paths end at their
+        first scattering hit with the photon map's estimate there instead of
+        next-event estimation.  pmap: a PhotonMap built on the photons'
+        locations and shapes; radiance (n,3) float64 and direction (n,3)
+        float32 device tensors in the map's input order."""
+        o = self._opts(spp, spp_offset, max_depth, rr_depth, seed, path_capacity)
+        po = PhotonRenderOpts(RT_RENDER_PHOTON_MAPPING, k, radius)
+        accum, nvalid = film
+        rgb = radiance.contiguous()
+        dr = direction.contiguous()
+        self.ctx.launch(lambda s: check(lib().rt_render_photon(self.h, C.byref(o), pmap.h, _ptr(rgb), _ptr(dr),
+                                                               C.byref(po), _ptr(accum), _ptr(nvalid), 0, s)), stream)
         return film
 
     def write_image(self, film, name, samples, mode="gammaClamp", fmt="bmp"):

@@ -29,6 +29,7 @@
 
 #include "rt_internal.h"
 #include "rt_photon.h"
+#include "rt_photon_render.h"
 #include "rt_shade.h"
 
 struct rt_ctx;
@@ -94,11 +95,19 @@ __device__ inline uint32_t spread10(uint32_t v)
 
 // 30-bit Morton code of each query in the photons' bounding box (outside
 // points clamp to its faces), and the identity permutation to sort with it.
+// With m_dev, only the first *m_dev of the m rows are queries: the rest get
+// the largest key and sort last.
 __global__ __launch_bounds__(256) void k_query_morton(const float *q, uint32_t m, float lx, float ly, float lz,
-                                                      float sx, float sy, float sz, uint32_t *key, uint32_t *id)
+                                                      float sx, float sy, float sz, uint32_t *key, uint32_t *id,
+                                                      const uint32_t *m_dev)
 {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= m) return;
+    if (m_dev && i >= *m_dev) {
+        key[i] = 0xffffffffu;
+        id[i] = i;
+        return;
+    }
     const float c[3] = {(q[3 * (size_t)i] - lx) * sx, (q[3 * (size_t)i + 1] - ly) * sy, (q[3 * (size_t)i + 2] - lz) * sz};
     uint32_t u[3];
     for (int a = 0; a < 3; ++a) {
@@ -121,6 +130,7 @@ struct KnnArgs {
     uint32_t *count;          // m
     uint32_t *index;          // m * k
     float *dist;              // m * k
+    const uint32_t *m_dev;    // if set: only the first *m_dev of the m lanes are queries
 };
 
 // Strided view of a heap: distance i at d[i * sd], id i at v[i * sv].
@@ -218,7 +228,7 @@ __global__ __launch_bounds__(kThreads) void k_photon_knn(KnnArgs a)
     float *lds_d = reinterpret_cast<float *>(lds_heap);
     uint32_t *lds_v = lds_heap + (MODE == kHeapLds ? a.k * kThreads : 0u);
     const uint32_t lane = blockIdx.x * kThreads + threadIdx.x;
-    if (lane >= a.m) return;
+    if (lane >= (a.m_dev ? min(a.m, *a.m_dev) : a.m)) return;
     const uint32_t q = a.perm ? a.perm[lane] : lane;
     const float p[3] = {a.queries[3 * (size_t)q], a.queries[3 * (size_t)q + 1], a.queries[3 * (size_t)q + 2]};
     const uint32_t want = a.qshape ? (uint32_t)(a.qshape[q] + 1) : 0u;  // 0: any shape
@@ -322,10 +332,10 @@ __global__ __launch_bounds__(256) void k_photon_gather_lambert(const double *rgb
 __global__ __launch_bounds__(256) void k_photon_gather_mat(const double *rgb, const float *dir, const float *nrm,
                                                            const float *view, const rt_material *mats, uint32_t m,
                                                            uint32_t k, const uint32_t *count, const uint32_t *index,
-                                                           const float *dist, double *out)
+                                                           const float *dist, double *out, const uint32_t *m_dev)
 {
     const uint32_t q = blockIdx.x * 256 + threadIdx.x;
-    if (q >= m) return;
+    if (q >= (m_dev ? min(m, *m_dev) : m)) return;
     const uint32_t c = count[q];
     const rt_material mt = mats[q];
     double acc[3] = {0.0, 0.0, 0.0};
@@ -488,7 +498,7 @@ static rt_status sort_queries(rt_photon_map *m, const float *queries, uint32_t n
     uint32_t *key0 = (uint32_t *)w, *key1 = (uint32_t *)(w + ids), *id0 = (uint32_t *)(w + 2 * ids),
              *id1 = (uint32_t *)(w + 3 * ids);
     k_query_morton<<<(nq + 255) / 256, 256, 0, stream>>>(queries, nq, m->lo[0], m->lo[1], m->lo[2], m->scale[0],
-                                                         m->scale[1], m->scale[2], key0, id0);
+                                                         m->scale[1], m->scale[2], key0, id0, nullptr);
     e = hipGetLastError();
     if (e == hipSuccess)
         e = hipcub::DeviceRadixSort::SortPairs(w + 4 * ids, sort_bytes, key0, key1, id0, id1, (int)nq, 0, 30, stream);
@@ -499,6 +509,87 @@ static rt_status sort_queries(rt_photon_map *m, const float *queries, uint32_t n
     *perm = id1;
     return RT_OK;
 }
+
+namespace rt {
+
+rt_ctx *photon_map_ctx(const rt_photon_map *m) { return m->ctx; }
+
+// Work space of a render's searches: the query sort (keys and ids twice,
+// the sort's temporary storage) and the k-nearest results, for cap queries.
+static void render_layout(uint32_t cap, uint32_t k, size_t sort_bytes, size_t &ids, size_t &work, size_t &cb,
+                          size_t &ib, size_t &gather)
+{
+    ids = ((size_t)cap * sizeof(uint32_t) + 255) & ~(size_t)255;
+    work = 4 * ids + sort_bytes;
+    cb = ((size_t)cap * 4 + 255) & ~(size_t)255;
+    ib = ((size_t)cap * k * 4 + 255) & ~(size_t)255;
+    gather = cb + 2 * ib;
+}
+
+hipError_t photon_render_reserve(rt_photon_map *m, uint32_t cap, uint32_t k, hipStream_t s, std::string &err)
+{
+    size_t sort_bytes = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                                      (uint32_t *)nullptr, (uint32_t *)nullptr, (int)cap, 0, 32, s);
+    if (e != hipSuccess) { err = std::string("photon search sizing: ") + hipGetErrorString(e); return e; }
+    size_t ids, work, cb, ib, gather;
+    render_layout(cap, k, sort_bytes, ids, work, cb, ib, gather);
+    if (work > m->work_bytes || gather > m->gather_bytes) {
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) { err = hipGetErrorString(e); return e; }
+    }
+    if (work > m->work_bytes) {
+        if (m->work) (void)hipFree(m->work);
+        m->work = nullptr;
+        m->work_bytes = 0;
+        if ((e = hipMalloc(&m->work, work)) != hipSuccess) { err = "photon search work space"; return e; }
+        m->work_bytes = work;
+    }
+    if (gather > m->gather_bytes) {
+        if (m->gather) (void)hipFree(m->gather);
+        m->gather = nullptr;
+        m->gather_bytes = 0;
+        if ((e = hipMalloc(&m->gather, gather)) != hipSuccess) { err = "photon search results"; return e; }
+        m->gather_bytes = gather;
+    }
+    return hipSuccess;
+}
+
+hipError_t photon_gather_render(rt_photon_map *m, const double *photon_rgb, const float *photon_dir,
+                                const float *points, const float *normals, const float *views,
+                                const rt_material *mats, const int32_t *qshape, uint32_t cap,
+                                const uint32_t *n_dev, uint32_t k, float radius, double *out, hipStream_t s)
+{
+    if (cap == 0) return hipSuccess;
+    size_t sort_bytes = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                                      (uint32_t *)nullptr, (uint32_t *)nullptr, (int)cap, 0, 32, s);
+    if (e != hipSuccess) return e;
+    size_t ids, work, cb, ib, gather;
+    render_layout(cap, k, sort_bytes, ids, work, cb, ib, gather);
+    if (work > m->work_bytes || gather > m->gather_bytes) return hipErrorInvalidValue;  // reserve first
+    uint8_t *w = (uint8_t *)m->work;
+    uint32_t *key0 = (uint32_t *)w, *key1 = (uint32_t *)(w + ids), *id0 = (uint32_t *)(w + 2 * ids),
+             *id1 = (uint32_t *)(w + 3 * ids);
+    k_query_morton<<<(cap + 255) / 256, 256, 0, s>>>(points, cap, m->lo[0], m->lo[1], m->lo[2], m->scale[0],
+                                                      m->scale[1], m->scale[2], key0, id0, n_dev);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipcub::DeviceRadixSort::SortPairs(w + 4 * ids, sort_bytes, key0, key1, id0, id1, (int)cap, 0, 32, s)) !=
+        hipSuccess)
+        return e;
+    uint8_t *gw = (uint8_t *)m->gather;
+    uint32_t *cnt = (uint32_t *)gw, *idx = (uint32_t *)(gw + cb);
+    float *dist = (float *)(gw + cb + ib);
+    KnnArgs a{id1, m->nodes, m->order, m->n, points, qshape, cap, k, radius, cnt, idx, dist, n_dev};
+    const unsigned blocks = (cap + kThreads - 1) / kThreads;
+    if (k <= kLdsK) k_photon_knn<kHeapLdsDist><<<blocks, kThreads, (size_t)k * kThreads * 4, s>>>(a);
+    else k_photon_knn<kHeapGlobal><<<blocks, kThreads, 0, s>>>(a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_photon_gather_mat<<<(cap + 255) / 256, 256, 0, s>>>(photon_rgb, photon_dir, normals, views, mats, cap, k, cnt,
+                                                          idx, dist, out, n_dev);
+    return hipGetLastError();
+}
+
+}  // namespace rt
 
 extern "C" {
 
@@ -600,7 +691,8 @@ rt_status rt_photon_map_knn(rt_photon_map *m, const float *queries, const int32_
     const uint32_t *id1 = nullptr;
     rt_status sst = sort_queries(m, queries, nq, stream, &id1, "rt_photon_map_knn");
     if (sst != RT_OK) return sst;
-    KnnArgs a{id1, m->nodes, m->order, m->n, queries, query_shape, nq, k, max_distance, count, index, distance};
+    KnnArgs a{id1, m->nodes, m->order, m->n, queries, query_shape, nq, k, max_distance, count, index, distance,
+              nullptr};
     const unsigned blocks = (nq + kThreads - 1) / kThreads;
     if (k <= kLdsK) k_photon_knn<kHeapLdsDist><<<blocks, kThreads, (size_t)k * kThreads * 4, stream>>>(a);
     else k_photon_knn<kHeapGlobal><<<blocks, kThreads, 0, stream>>>(a);
@@ -695,7 +787,7 @@ rt_status rt_photon_gather(rt_photon_map *m, const double *photon_rgb, const flo
     rt_status st = gather_knn(m, points, query_shape, nq, k, max_distance, stream, &cnt, &idx, &dist);
     if (st != RT_OK) return st;
     k_photon_gather_mat<<<(nq + 255) / 256, 256, 0, stream>>>(photon_rgb, photon_dir, normals, views, materials, nq,
-                                                              k, cnt, idx, dist, out);
+                                                              k, cnt, idx, dist, out, nullptr);
     RT_HIP(hipGetLastError());
     return RT_OK;
 }

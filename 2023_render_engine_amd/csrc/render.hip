@@ -31,6 +31,7 @@
 #include "rt_device.h"
 #include "rt_render.h"
 #include "rt_shade.h"
+#include "rt_photon_render.h"
 #include "rt_surface.h"
 #include "rt_cache.h"
 
@@ -312,12 +313,22 @@ __device__ __forceinline__ bool cacheable(const rt_material &m)
            ((m.type == RT_MAT_TROWBRIDGE || m.type == RT_MAT_TROWBRIDGE_BASIC) && m.roughness >= 0.04f);
 }
 
-__device__ __forceinline__ uint32_t classify(const SceneDev &sc, int32_t hid, bool cache_depth_reached)
+// Photon mapping: the scattering hits whose photon estimate can be nonzero
+// (Lambertian and Trowbridge; a plain surface reflects no photons, so it
+// keeps its emissive class and ends there as well).
+__device__ __forceinline__ bool photon_vertex(const rt_material &m)
+{
+    return m.type == RT_MAT_LAMBERTIAN || m.type == RT_MAT_TROWBRIDGE || m.type == RT_MAT_TROWBRIDGE_BASIC;
+}
+
+__device__ __forceinline__ uint32_t classify(const SceneDev &sc, int32_t hid, bool cache_depth_reached,
+                                             bool photon = false)
 {
     if (hid == -1) return kQMiss;
     const uint32_t k = (uint32_t)(-hid - 2);
     const uint32_t mi = hid >= 0 ? sc.tri_mat[hid] : k < sc.nsph ? sc.sph[k].material : sc.plane[k - sc.nsph].material;
     if (cache_depth_reached && cacheable(sc.mat[mi])) return kQCache;
+    if (photon && photon_vertex(sc.mat[mi])) return kQCache;
     const uint32_t t = sc.mat[mi].type;
     if (t == RT_MAT_LAMBERTIAN) return kQLambert;
     // the specular non-scattering class: dielectric and mirror
@@ -388,7 +399,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc0, BatchDev bt, Pa
 // queue, on the stripe's counter).
 template <bool B>
 __global__ __launch_bounds__(kBlock) void k_classify(SceneDev sc0, BatchDev bt, PathsDev p, QueuesDev q, uint32_t cur,
-                                                     uint32_t cache_depth, uint32_t need_record)
+                                                     uint32_t cache_depth, uint32_t need_record, uint32_t photon)
 {
     const StripeSpan sp = stripe_span();
     const uint32_t cs = q.cs, base = sp.s * cs;
@@ -435,7 +446,7 @@ __global__ __launch_bounds__(kBlock) void k_classify(SceneDev sc0, BatchDev bt, 
                 const uint32_t meta = *path_word(p, slot, kPqBeta, 3);
                 const bool deep = cache_depth != kNoCache && (meta & 0xffffu) >= cache_depth &&
                                   (!need_record || *path_word(p, slot, kPqNee, 3) != 0) && !(meta & kMetaHarvest);
-                return classify(sc, hid, deep);
+                return classify(sc, hid, deep, photon != 0);
         };
         uint32_t cls = 0xffu;
         if constexpr (B) {
@@ -782,6 +793,64 @@ __global__ __launch_bounds__(kBlock) void k_cache_vertex(SceneDev sc, PathsDev p
     }
 }
 
+// Photon vertices (photon mapping): the path ends at its first scattering
+// hit.  Its query -- the hit point, the shading normal, the direction back
+// along the ray, the material with the textured kd and the shape id in the
+// photons' numbering (spheres, planes, then triangles) -- goes to the
+// iteration's query rows (dense in stripe order, like the cache's), and
+// k_photon_apply adds the throughput times the map's estimate
+// (getPhotonMappingContribution, Scene.cu:724-746, added in
+// computeDirectLightning, pathTracing.cu:186-204).  Lambertian and
+// Trowbridge surfaces emit nothing, so nothing else is added here.
+__global__ __launch_bounds__(kBlock) void k_photon_vertex(SceneDev sc, PathsDev p, QueuesDev q, PhotonLaunch ph)
+{
+    const StripeSpan sp = stripe_span();
+    const uint32_t base = sp.s * q.cs;
+    const uint32_t n = *lcount(q, kLQueue + kQCache, sp.s);
+    uint32_t total = 0;
+    const uint32_t pre = wave_excl_scan(*lcount(q, kLQueue + kQCache, __lane_id()), total);
+    const uint32_t row0 = (uint32_t)__shfl((int)pre, (int)sp.s);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *scount(q, kCCacheQ) = total;
+    for (uint32_t j0 = sp.j0; j0 < n; j0 += sp.step) {
+        const uint32_t i = j0 + threadIdx.x;
+        const bool live = i < n;
+        const uint32_t slot = live ? q.q[kQCache][base + i] : 0;
+        if (live) {
+            resolve_nee(p, slot);
+            const float4 *rec = path_rec(p, slot);
+            Surface s{};
+            V3 d = mk(0, 0, 1);
+            surface_of(sc, rec[kPqRay], rec[kPqHit], s, d);
+            const int32_t hid = (int32_t)bits(rec[kPqHit].w);
+            const uint32_t k = row0 + i;
+            ph.qp[3 * (size_t)k] = s.p.x; ph.qp[3 * (size_t)k + 1] = s.p.y; ph.qp[3 * (size_t)k + 2] = s.p.z;
+            ph.qn[3 * (size_t)k] = s.n.x; ph.qn[3 * (size_t)k + 1] = s.n.y; ph.qn[3 * (size_t)k + 2] = s.n.z;
+            ph.qv[3 * (size_t)k] = -d.x; ph.qv[3 * (size_t)k + 1] = -d.y; ph.qv[3 * (size_t)k + 2] = -d.z;
+            rt_material m = sc.mat[s.mat];
+            apply_texture(sc, s, m);
+            ph.qm[k] = m;
+            ph.qs[k] = hid >= 0 ? (int32_t)(sc.nsph + sc.nplane) + hid : -hid - 2;
+            ph.qslot[k] = slot;
+        }
+        const uint32_t tpos = wave_append(lcount(q, kLTerm, sp.s), live);
+        if (live) q.term[base + tpos] = slot;
+    }
+}
+
+// L += beta * estimate for this iteration's photon queries; the paths ended
+// at their photon vertex and are finalized right after.
+__global__ __launch_bounds__(kBlock) void k_photon_apply(PathsDev p, QueuesDev q, PhotonLaunch ph)
+{
+    const uint32_t n = *scount(q, kCCacheQ);
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+        float4 *rec = path_rec(p, ph.qslot[k]);
+        const double *c = ph.qrgb + 3 * (size_t)k;
+        const float4 qb = rec[kPqBeta], qL = rec[kPqL];
+        rec[kPqL] = make_float4(qL.x + qb.x * (float)c[0], qL.y + qb.y * (float)c[1], qL.z + qb.z * (float)c[2],
+                                qL.w);
+    }
+}
+
 // ---------------------------------------------------------------------
 // connect: any hit for shadow rays (Scene.cu:108-150, 238-276)
 // ---------------------------------------------------------------------
@@ -1047,6 +1116,8 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
     // at the next cacheable vertex, whose cache answer completes their
     // target (self-training on the cache's own prediction).
     const bool lookups = !B && wp.cache_depth != kNoCache && c.cache && c.cwork;
+    const bool photon = !B && c.photon != nullptr;
+    if (photon && (lookups || wp.train || wp.harvest_shift != kNoHarvest)) return hipErrorInvalidValue;
     // a cache depth without lookups: only a plain wave that harvests records
     if (!wp.train && wp.cache_depth != kNoCache && !lookups && wp.harvest_shift == kNoHarvest)
         return hipErrorInvalidValue;
@@ -1084,14 +1155,15 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
         hipLaunchKernelGGL(k_reset, dim3(kNumLists + 1), dim3(kStripes), 0, s, q, cur, 0u);
         timed(0, extend);
         if (B) {
-            hipLaunchKernelGGL(k_classify<true>, grid, blk, 0, s, sc, bt, paths, q, cur, ext_depth, need_record);
+            hipLaunchKernelGGL(k_classify<true>, grid, blk, 0, s, sc, bt, paths, q, cur, ext_depth, need_record, 0u);
             hipLaunchKernelGGL((k_shade<kQLambert, true>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQDielectric, true>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQGgx, true>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQLight, true>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQMiss, true>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
         } else {
-            hipLaunchKernelGGL(k_classify<false>, grid, blk, 0, s, sc, bt, paths, q, cur, ext_depth, need_record);
+            hipLaunchKernelGGL(k_classify<false>, grid, blk, 0, s, sc, bt, paths, q, cur, ext_depth, need_record,
+                               photon ? 1u : 0u);
             hipLaunchKernelGGL((k_shade<kQLambert, false>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQDielectric, false>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQGgx, false>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
@@ -1099,6 +1171,7 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
             hipLaunchKernelGGL((k_shade<kQMiss, false>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
         }
         if (lookups) hipLaunchKernelGGL(k_cache_vertex, grid, blk, 0, s, sc, paths, q);
+        if (photon) hipLaunchKernelGGL(k_photon_vertex, grid, blk, 0, s, sc, paths, q, *c.photon);
         hipLaunchKernelGGL(k_count, dim3(1), dim3(kStripes), 0, s, q, cur);
         timed(1, connect);
         if (lookups) {
@@ -1108,6 +1181,13 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
                                        scount(q, kCCacheQ));
             if (e != hipSuccess) return e;
             hipLaunchKernelGGL(k_cache_apply, grid, blk, 0, s, paths, q);
+        }
+        if (photon) {
+            const PhotonLaunch &ph = *c.photon;
+            hipError_t e = photon_gather_render(ph.map, ph.rgb, ph.dir, ph.qp, ph.qn, ph.qv, ph.qm, ph.qs, c.capacity,
+                                                scount(q, kCCacheQ), ph.k, ph.radius, ph.qrgb, s);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(k_photon_apply, grid, blk, 0, s, paths, q, ph);
         }
         const uint32_t harvest = wp.harvest_shift != kNoHarvest ? 1u : 0u;
         const uint32_t cap = wp.train ? c.capacity : (harvest ? wp.harvest_cap : 0u);

@@ -192,6 +192,25 @@ struct BatchLaunch {
 struct CacheDev;
 struct CacheWork;
 
+// Photon mapping in the bounce loop (integration/Scene.cu:912-967 with
+// RENDER_MODE PHOTON_MAPPING): a path ends at its first scattering hit
+// (pathTracing.cu:84-88) and its estimate there is the photon map's
+// density estimate instead of next-event estimation.  The map (kd-tree of
+// photon locations and shapes), each photon's radiance and direction, the
+// search's k and radius, and per wave slot the query buffers.
+struct PhotonLaunch {
+    rt_photon_map *map;
+    const double *rgb;          // 3 per photon
+    const float *dir;           // 3 per photon
+    uint32_t k;
+    float radius;
+    float *qp, *qn, *qv;        // query point, shading normal, direction to the viewer (3 each per slot)
+    rt_material *qm;            // query material (textured kd)
+    int32_t *qs;                // query shape id (the photons' numbering)
+    uint32_t *qslot;            // the query's path slot
+    double *qrgb;               // the estimate (3 per query)
+};
+
 // Optional measurement of a frame's traversal kernels.  With events, every
 // k_extend / k_connect launch is bracketed by a pair of HIP events on the
 // launch stream (pairs are reused across frames; the host sums their elapsed
@@ -255,6 +274,7 @@ struct ChunkLaunch {
     const CacheWork *cwork;
     const BatchLaunch *batch;
     KernelProfile *prof;        // optional (never under graph capture)
+    const PhotonLaunch *photon; // photon mapping (single scene, no cache)
 };
 // chunk start: clears every counter and fills all slots
 hipError_t chunk_begin(const ChunkLaunch &c, const WaveParams &wp, hipStream_t s);

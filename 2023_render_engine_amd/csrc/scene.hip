@@ -18,6 +18,7 @@
 
 #include "rt_cache.h"
 #include "rt_internal.h"
+#include "rt_photon_render.h"
 #include "rt_photon_trace.h"
 #include "rt_render.h"
 #include "rt_waves.h"
@@ -237,6 +238,9 @@ struct rt_scene {
     // radiance-cache inference scratch (one row per path slot), on demand
     void *cache_arena = nullptr;
     rt::CacheWork cwork{};
+    // photon-mapping query rows (one per path slot), on demand
+    void *photon_arena = nullptr;
+    uint32_t photon_rows = 0;
 };
 
 namespace rt {
@@ -279,6 +283,38 @@ static void free_arena(rt_scene *s)
     if (s->cache_arena) (void)hipFree(s->cache_arena);
     s->cache_arena = nullptr;
     s->cwork = rt::CacheWork{};
+    if (s->photon_arena) (void)hipFree(s->photon_arena);
+    s->photon_arena = nullptr;
+    s->photon_rows = 0;
+}
+
+// Photon query rows for `rows` path slots: point, normal, view (3 floats
+// each), material, shape, slot and the estimate (3 doubles).
+static rt_status ensure_photon_rows(rt_scene *s, uint32_t rows, PhotonLaunch &ph)
+{
+    auto bytes = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t v3 = bytes((size_t)rows * 12), mat = bytes((size_t)rows * sizeof(rt_material)),
+                 u = bytes((size_t)rows * 4), rgb = bytes((size_t)rows * 24);
+    if (!s->photon_arena || s->photon_rows < rows) {
+        if (s->photon_arena) (void)hipFree(s->photon_arena);
+        s->photon_arena = nullptr;
+        s->photon_rows = 0;
+        hipError_t e = hipMalloc(&s->photon_arena, 3 * v3 + mat + 2 * u + rgb);
+        if (e != hipSuccess) {
+            set_error(std::string("rt_render_photon: photon query rows: ") + hipGetErrorString(e));
+            return RT_ERR_OOM;
+        }
+        s->photon_rows = rows;
+    }
+    char *b = (char *)s->photon_arena;
+    ph.qp = (float *)b;
+    ph.qn = (float *)(b + v3);
+    ph.qv = (float *)(b + 2 * v3);
+    ph.qm = (rt_material *)(b + 3 * v3);
+    ph.qs = (int32_t *)(b + 3 * v3 + mat);
+    ph.qslot = (uint32_t *)(b + 3 * v3 + mat + u);
+    ph.qrgb = (double *)(b + 3 * v3 + mat + 2 * u);
+    return RT_OK;
 }
 
 // Inference activations for `rows` cache queries (forward only).
@@ -772,7 +808,8 @@ rt_status rt_scene_camera_ray(const rt_scene *s, uint32_t px, uint32_t py, float
 // receives the records of the (last) chunk, at most the wave capacity.
 static rt_status render_frames(const char *fn, rt_scene *s, const rt_render_opts *opts, rt_cache *cache,
                                uint32_t cache_depth, double *accum, uint32_t *nvalid, uint32_t flags, void *stream_,
-                               uint32_t harvest_shift = rt::kNoHarvest, uint32_t *harvested = nullptr)
+                               uint32_t harvest_shift = rt::kNoHarvest, uint32_t *harvested = nullptr,
+                               rt::PhotonLaunch *photon = nullptr)
 {
     const std::string name(fn);
     if (!s || !opts || !accum || !nvalid) { set_error(name + ": null argument"); return RT_ERR_INVALID; }
@@ -799,6 +836,16 @@ static rt_status render_frames(const char *fn, rt_scene *s, const rt_render_opts
         st = rt::ensure_cache_work(s, s->wb.capacity);
         if (st != RT_OK) return st;
     }
+    if (photon) {
+        st = rt::ensure_photon_rows(s, s->wb.capacity, *photon);
+        if (st != RT_OK) return st;
+        std::string err;
+        hipError_t e = rt::photon_render_reserve(photon->map, s->wb.capacity, photon->k, stream, err);
+        if (e != hipSuccess) {
+            set_error(name + ": " + err + ": " + hipGetErrorString(e));
+            return e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;
+        }
+    }
     auto t0 = std::chrono::steady_clock::now();
     RT_HIP(hipMemsetAsync(s->wb.queues.stats, 0, rt::kNumStats * 8, stream));
     s->prof.events = (flags & RT_FLAG_PROFILE) != 0;
@@ -816,6 +863,7 @@ static rt_status render_frames(const char *fn, rt_scene *s, const rt_render_opts
     cl.cwork = cache ? &s->cwork : nullptr;
     cl.batch = nullptr;
     cl.prof = (flags & (RT_FLAG_PROFILE | RT_FLAG_COUNT)) ? &s->prof : nullptr;
+    cl.photon = photon;
     for (uint64_t first = 0; first < total; first += R) {
         rt::WaveParams wp{};
         wp.first_path = first;
@@ -851,6 +899,29 @@ rt_status rt_render(rt_scene *s, const rt_render_opts *opts, double *accum, uint
                     void *stream)
 {
     return render_frames("rt_render", s, opts, nullptr, 0, accum, nvalid, flags, stream);
+}
+
+rt_status rt_render_photon(rt_scene *s, const rt_render_opts *opts, rt_photon_map *map, const double *photon_rgb,
+                           const float *photon_dir, const rt_photon_render_opts *popts, double *accum,
+                           uint32_t *nvalid, uint32_t flags, void *stream)
+{
+    if (!s || !map || !popts) { set_error("rt_render_photon: null argument"); return RT_ERR_INVALID; }
+    if (popts->mode != RT_RENDER_PHOTON_MAPPING) { set_error("rt_render_photon: mode must be RT_RENDER_PHOTON_MAPPING"); return RT_ERR_INVALID; }
+    if (popts->k == 0 || popts->k > 1024) { set_error("rt_render_photon: k must be 1..1024"); return RT_ERR_INVALID; }
+    if (!(popts->radius > 0.0f)) { set_error("rt_render_photon: radius must be positive"); return RT_ERR_INVALID; }
+    if (rt::photon_map_ctx(map) != s->ctx) { set_error("rt_render_photon: map and scene belong to different contexts"); return RT_ERR_INVALID; }
+    uint32_t n = 0, depth = 0;
+    rt_status st = rt_photon_map_get_info(map, &n, &depth);
+    if (st != RT_OK) return st;
+    if (n && (!photon_rgb || !photon_dir)) { set_error("rt_render_photon: null photon arrays"); return RT_ERR_INVALID; }
+    rt::PhotonLaunch ph{};
+    ph.map = map;
+    ph.rgb = photon_rgb;
+    ph.dir = photon_dir;
+    ph.k = popts->k;
+    ph.radius = popts->radius;
+    return render_frames("rt_render_photon", s, opts, nullptr, 0, accum, nvalid, flags, stream, rt::kNoHarvest,
+                         nullptr, &ph);
 }
 
 rt_status rt_render_cached(rt_scene *s, const rt_render_opts *opts, rt_cache *cache, uint32_t cache_depth,

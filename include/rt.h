@@ -519,6 +519,35 @@ rt_status rt_photon_subsurface(rt_photon_map *map, const double *photon_rgb, con
                                const float *views, const int32_t *query_shape, uint32_t nq, float radius,
                                const rt_sss_params *params, double *out, void *stream);
 
+/* Photon mapping in the bounce loop: the reference's RENDER_MODE
+ * PHOTON_MAPPING (cuda_libraries/types.h:51; Scene::sampleDirectLight,
+ * integration/Scene.cu:912-967; finishStep, pathTracing.cu:84-88).  Paths
+ * start and bounce as in rt_render (same seeds, camera, specular bounces,
+ * emission), but there is no next-event estimation: at its first
+ * scattering hit (Lambertian, Trowbridge, plain) a path adds its throughput
+ * times getPhotonMappingContribution -- the map's k nearest photons within
+ * radius on the hit's shape, through combinePhotonContributions, as
+ * rt_photon_gather computes it -- and ends.  The map holds the photons of
+ * rt_scene_trace_photons (or any others: locations and shape ids, with
+ * shape k = sphere k, then planes, then triangles); photon_rgb and
+ * photon_dir are their radiance (3 doubles) and direction (3 floats) in the
+ * map's input order, device memory.  The film, flags and stream are as for
+ * rt_render; the map must be on the scene's context, and its work space
+ * serves this call (one render or search per map at a time).  Each
+ * iteration searches every path that ended there (k results per path slot
+ * of work space: 8 * k bytes per slot). */
+enum { RT_RENDER_PATH_TRACING = 0, RT_RENDER_PHOTON_MAPPING = 1 };
+typedef struct {
+    uint32_t mode;   /* RT_RENDER_PHOTON_MAPPING */
+    uint32_t k;      /* photons per estimate, 1..1024 (MAX_NEIGHBOURS, types.h:60: 100) */
+    float radius;    /* search radius (PHOTON_SEARCH_RADIUS, types.h:58: 100) */
+} rt_photon_render_opts;
+
+rt_status rt_render_photon(rt_scene *scene, const rt_render_opts *opts, rt_photon_map *map,
+                           const double *photon_rgb, const float *photon_dir,
+                           const rt_photon_render_opts *popts, double *accum, uint32_t *nvalid,
+                           uint32_t flags, void *stream);
+
 /* ------------------------------------------------------------------ */
 /* Batches: many scenes in one launch sequence                          */
 /* ------------------------------------------------------------------ */

@@ -21,6 +21,11 @@ _U32 = C.c_uint32
 _F = C.c_float
 
 
+class OrcPhotonRender(C.Structure):
+    _fields_ = [("xyz", _P), ("shape", _P), ("order", _P), ("axis", _P), ("n", _U32), ("rgb", _P), ("dir", _P),
+                ("k", _U32), ("radius", _F)]
+
+
 class OrcRay(C.Structure):
     _fields_ = [("o", _F * 3), ("d", _F * 3), ("inv", _F * 3), ("tmax", _F)]
 
@@ -38,6 +43,8 @@ _SIGS = {
     "orc_next_float_up": (_F, [_F]),
     "orc_next_float_down": (_F, [_F]),
     "orc_ray_make": (None, [C.POINTER(OrcRay), _P, _P, _F]),
+    "orc_render_photon": (None, [_P, _U32, _U32, _U32, _U32, _U32, C.c_uint64, _U32, _U32, _U32, _U32, _P, _P, C.c_int,
+                                 C.POINTER(OrcPhotonRender)]),
     "orc_lbvh_build": (C.c_int, [_P, _U32, _P, _U32, _U32, _F, C.c_int, _P, _P, _P, _P]),
     "orc_tri_intersects": (C.c_int, [C.POINTER(OrcRay), _P, _P, _P, C.POINTER(_F), C.POINTER(OrcTriInfo)]),
     "orc_box_intersects": (C.c_int, [_P, _P, C.POINTER(OrcRay), C.POINTER(_F)]),
@@ -555,6 +562,28 @@ class OrcScene:
         else:
             c = np.ascontiguousarray(cache_rgb, np.float32)
             lib().orc_render_const_cache(*args, cache_depth, _p(c))
+        return accum, nvalid
+
+    def render_photon(self, photons, k=100, radius=100.0, spp=None, spp_offset=0, window=None, seed=None,
+                      nthreads=8, stream_base=0, max_depth=None):
+        """orc_render_photon (RENDER_MODE PHOTON_MAPPING) over photons =
+        {location, shape, radiance, direction} (numpy, as trace_photons
+        returns them); the kd-tree is built here.  Returns (accum, nvalid)
+        like render."""
+        d = self.desc
+        W, H = d.width, d.height
+        x0, y0, x1, y1 = window or (0, 0, W, H)
+        loc = np.ascontiguousarray(photons["location"], np.float32).reshape(-1, 3)
+        shp = np.ascontiguousarray(photons["shape"], np.int32).reshape(-1)
+        rgb = np.ascontiguousarray(photons["radiance"], np.float64).reshape(-1, 3)
+        dr = np.ascontiguousarray(photons["direction"], np.float32).reshape(-1, 3)
+        order, axis, _ = kd_build(loc)
+        ph = OrcPhotonRender(_p(loc), _p(shp), _p(order), _p(axis), loc.shape[0], _p(rgb), _p(dr), k, radius)
+        accum = np.zeros(W * H * 3, np.float64)
+        nvalid = np.zeros(W * H, np.uint32)
+        lib().orc_render_photon(self.h, spp if spp is not None else d.spp, d.seed if seed is None else seed,
+                                max_depth or d.max_depth, d.rr_depth, spp_offset, stream_base, x0, y0, x1, y1,
+                                _p(accum), _p(nvalid), nthreads, C.byref(ph))
         return accum, nvalid
 
 _lib = None

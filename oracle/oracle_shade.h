@@ -96,6 +96,29 @@ void orc_render(const orc_scene *s, uint32_t spp, uint32_t seed, uint32_t max_de
                 uint32_t spp_offset, uint64_t stream_base, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1,
                 double *accum, uint32_t *nvalid, int nthreads);
 
+/* RENDER_MODE PHOTON_MAPPING (Scene.cu:912-967, pathTracing.cu:84-88): the
+ * photon map (n photons: locations xyz, shape ids, the kd-tree of
+ * orc_kd_build as order / axis, radiance rgb and directions dir) and the
+ * search's k (<= 1024) and radius. */
+typedef struct {
+    const float *xyz;
+    const int32_t *shape;
+    const uint32_t *order;
+    const uint8_t *axis;
+    uint32_t n;
+    const double *rgb;
+    const float *dir;
+    uint32_t k;
+    float radius;
+} orc_photon_render;
+
+/* orc_render without next-event estimation: a path's first scattering hit
+ * adds beta times the photon map's estimate there (then its emission) and
+ * the path ends. */
+void orc_render_photon(const orc_scene *s, uint32_t spp, uint32_t seed, uint32_t max_depth, uint32_t rr_depth,
+                       uint32_t spp_offset, uint64_t stream_base, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1,
+                       double *accum, uint32_t *nvalid, int nthreads, const orc_photon_render *ph);
+
 /* orc_render with paths that end at their first cacheable vertex (diffuse, or
  * GGX roughness >= 0.04) at depth >= cache_depth and add beta * cache_rgb
  * there: the library's cache termination with a cache that answers one

@@ -17,6 +17,7 @@
  */
 #include "oracle.h"
 #include "oracle_shade.h"
+#include "oracle_photon.h"
 
 #include <float.h>
 #include <math.h>
@@ -839,7 +840,8 @@ static int cacheable(const orc_material *m)
  * constant). */
 static int path_impl(const orc_scene *s, uint32_t px, uint32_t py, uint32_t sample, uint32_t seed,
                      uint32_t max_depth, uint32_t rr_depth, uint64_t stream_base, uint32_t cache_depth,
-                     const float *cache_rgb, float out[3], float *seg, uint32_t *nseg)
+                     const float *cache_rgb, float out[3], float *seg, uint32_t *nseg,
+                     const orc_photon_render *ph)
 {
     if (nseg) *nseg = 0;
     const uint32_t W = s->cam.width;
@@ -875,6 +877,31 @@ static int path_impl(const orc_scene *s, uint32_t px, uint32_t py, uint32_t samp
         }
         if (cache_rgb && depth >= cache_depth && cacheable(m)) {
             L = add(L, mk(beta.x * cache_rgb[0], beta.y * cache_rgb[1], beta.z * cache_rgb[2]));
+            break;
+        }
+        if (ph && is_scattering(m)) {
+            /* RENDER_MODE PHOTON_MAPPING: no direct-light sampling; the
+             * scattering hit takes getPhotonMappingContribution (Scene.cu:
+             * 724-746: the k nearest photons within the radius on the hit's
+             * shape, combinePhotonContributions), added times the throughput
+             * (pathTracing.cu:186-204), then its emission, and the path ends
+             * (finishStep, pathTracing.cu:84-88).  Photon shape ids: spheres,
+             * planes, then triangles. */
+            uint32_t idx[1024];
+            float dist[1024];
+            const int32_t qshape = h.kind == 1 ? (int32_t)(s->nsph + s->nplane + h.id)
+                                   : h.kind == 2 ? (int32_t)h.id : (int32_t)(s->nsph + h.id);
+            const float q[3] = {h.p.x, h.p.y, h.p.z};
+            uint32_t cnt = orc_kd_knn(ph->xyz, ph->shape, ph->order, ph->axis, ph->n, q, qshape, ph->k, ph->radius,
+                                      idx, dist);
+            const float nq[3] = {h.n.x, h.n.y, h.n.z}, view[3] = {-d.x, -d.y, -d.z};
+            double est[3];
+            orc_photon_gather_mat(ph->rgb, ph->dir, 1, &cnt, idx, dist, ph->k, nq, view, m, est);
+            L = add(L, mk(beta.x * (float)est[0], beta.y * (float)est[1], beta.z * (float)est[2]));
+            if (m->type == ORC_MAT_PLAIN) {
+                const float *e = camera ? m->kd : m->ke;
+                L = add(L, mk(beta.x * e[0], beta.y * e[1], beta.z * e[2]));
+            }
             break;
         }
         v3 n = h.n, t, b;
@@ -961,14 +988,14 @@ static int path_impl(const orc_scene *s, uint32_t px, uint32_t py, uint32_t samp
 int orc_path(const orc_scene *s, uint32_t px, uint32_t py, uint32_t sample, uint32_t seed, uint32_t max_depth,
              uint32_t rr_depth, uint64_t stream_base, float out[3])
 {
-    return path_impl(s, px, py, sample, seed, max_depth, rr_depth, stream_base, 0, NULL, out, NULL, NULL);
+    return path_impl(s, px, py, sample, seed, max_depth, rr_depth, stream_base, 0, NULL, out, NULL, NULL, NULL);
 }
 
 int orc_path_segments(const orc_scene *s, uint32_t px, uint32_t py, uint32_t sample, uint32_t seed,
                       uint32_t max_depth, uint32_t rr_depth, uint64_t stream_base, float out[3], float *seg,
                       uint32_t *nseg)
 {
-    return path_impl(s, px, py, sample, seed, max_depth, rr_depth, stream_base, 0, NULL, out, seg, nseg);
+    return path_impl(s, px, py, sample, seed, max_depth, rr_depth, stream_base, 0, NULL, out, seg, nseg, NULL);
 }
 
 typedef struct {
@@ -980,6 +1007,7 @@ typedef struct {
     uint64_t stream_base;
     uint32_t cache_depth;
     const float *cache_rgb;
+    const orc_photon_render *ph;
 } render_ctx;
 
 static void *render_rows(void *p)
@@ -993,7 +1021,7 @@ static void *render_rows(void *p)
             for (uint32_t k = 0; k < c->spp; ++k) {
                 float L[3];
                 if (path_impl(c->s, x, y, c->spp_offset + k, c->seed, c->max_depth, c->rr_depth, c->stream_base,
-                              c->cache_depth, c->cache_rgb, L, NULL, NULL)) {
+                              c->cache_depth, c->cache_rgb, L, NULL, NULL, c->ph)) {
                     acc[0] += L[0]; acc[1] += L[1]; acc[2] += L[2];
                     ++nv;
                 }
@@ -1010,7 +1038,7 @@ static void *render_rows(void *p)
 static void render_impl(const orc_scene *s, uint32_t spp, uint32_t seed, uint32_t max_depth, uint32_t rr_depth,
                         uint32_t spp_offset, uint64_t stream_base, uint32_t x0, uint32_t y0, uint32_t x1,
                         uint32_t y1, double *accum, uint32_t *nvalid, int nthreads, uint32_t cache_depth,
-                        const float *cache_rgb)
+                        const float *cache_rgb, const orc_photon_render *ph)
 {
     if (nthreads < 1) nthreads = 1;
     uint32_t rows = y1 > y0 ? y1 - y0 : 0;
@@ -1021,7 +1049,7 @@ static void render_impl(const orc_scene *s, uint32_t spp, uint32_t seed, uint32_
     uint32_t chunk = (rows + nthreads - 1) / nthreads;
     for (int i = 0; i < nthreads; ++i) {
         render_ctx c = {s, spp, seed, max_depth, rr_depth, spp_offset, x0, y0 + i * chunk, x1, y0 + (i + 1) * chunk,
-                        accum, nvalid, stream_base, cache_depth, cache_rgb};
+                        accum, nvalid, stream_base, cache_depth, cache_rgb, ph};
         if (c.y0 > y1) c.y0 = y1;
         if (c.y1 > y1) c.y1 = y1;
         cs[i] = c;
@@ -1036,7 +1064,15 @@ void orc_render(const orc_scene *s, uint32_t spp, uint32_t seed, uint32_t max_de
                 double *accum, uint32_t *nvalid, int nthreads)
 {
     render_impl(s, spp, seed, max_depth, rr_depth, spp_offset, stream_base, x0, y0, x1, y1, accum, nvalid,
-                nthreads, 0, NULL);
+                nthreads, 0, NULL, NULL);
+}
+
+void orc_render_photon(const orc_scene *s, uint32_t spp, uint32_t seed, uint32_t max_depth, uint32_t rr_depth,
+                       uint32_t spp_offset, uint64_t stream_base, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1,
+                       double *accum, uint32_t *nvalid, int nthreads, const orc_photon_render *ph)
+{
+    render_impl(s, spp, seed, max_depth, rr_depth, spp_offset, stream_base, x0, y0, x1, y1, accum, nvalid,
+                nthreads, 0, NULL, ph);
 }
 
 void orc_render_const_cache(const orc_scene *s, uint32_t spp, uint32_t seed, uint32_t max_depth,
@@ -1045,7 +1081,7 @@ void orc_render_const_cache(const orc_scene *s, uint32_t spp, uint32_t seed, uin
                             uint32_t cache_depth, const float cache_rgb[3])
 {
     render_impl(s, spp, seed, max_depth, rr_depth, spp_offset, stream_base, x0, y0, x1, y1, accum, nvalid,
-                nthreads, cache_depth, cache_rgb);
+                nthreads, cache_depth, cache_rgb, NULL);
 }
 
 /* computeRMSE (camera/Film.cu:611-632): per pixel the squared norm of the

@@ -65,6 +65,7 @@ class PhotonRenderOpts(C.Structure):
 
 
 RT_RENDER_PHOTON_MAPPING = 1
+RT_RENDER_HYBRID = 2
 
 
 class SssParams(C.Structure):
@@ -575,14 +576,16 @@ class Scene:
         return film
 
     def render_photon(self, film, pmap, radiance, direction, k=100, radius=100.0, spp=None, spp_offset=0,
-                      max_depth=None, rr_depth=None, seed=None, path_capacity=0, stream=None):
-        """rt_render_photon (RENDER_MODE PHOTON_MAPPING): paths end at their
+                      max_depth=None, rr_depth=None, seed=None, path_capacity=0, stream=None, hybrid=False):
+        """rt_render_photon.  RENDER_MODE PHOTON_MAPPING: paths end at their
         first scattering hit with the photon map's estimate there instead of
-        next-event estimation.  pmap: a PhotonMap built on the photons'
-        locations and shapes; radiance (n,3) float64 and direction (n,3)
-        float32 device tensors in the map's input order."""
+        next-event estimation; hybrid=True (RENDER_MODE HYBRID): every
+        scattering hit takes half of each and the path goes on.  pmap: a
+        PhotonMap built on the photons' locations and shapes; radiance (n,3)
+        float64 and direction (n,3) float32 device tensors in the map's input
+        order."""
         o = self._opts(spp, spp_offset, max_depth, rr_depth, seed, path_capacity)
-        po = PhotonRenderOpts(RT_RENDER_PHOTON_MAPPING, k, radius)
+        po = PhotonRenderOpts(RT_RENDER_HYBRID if hybrid else RT_RENDER_PHOTON_MAPPING, k, radius)
         accum, nvalid = film
         rgb = radiance.contiguous()
         dr = direction.contiguous()

@@ -446,7 +446,7 @@ __global__ __launch_bounds__(kBlock) void k_classify(SceneDev sc0, BatchDev bt, 
                 const uint32_t meta = *path_word(p, slot, kPqBeta, 3);
                 const bool deep = cache_depth != kNoCache && (meta & 0xffffu) >= cache_depth &&
                                   (!need_record || *path_word(p, slot, kPqNee, 3) != 0) && !(meta & kMetaHarvest);
-                return classify(sc, hid, deep, photon != 0);
+                return classify(sc, hid, deep, photon == 1u);
         };
         uint32_t cls = 0xffu;
         if constexpr (B) {
@@ -465,6 +465,12 @@ __global__ __launch_bounds__(kBlock) void k_classify(SceneDev sc0, BatchDev bt, 
         for (uint32_t k = 0; k < kNumQ; ++k) {
             const uint32_t pos = wave_append(lcount(q, kLQueue + k, sp.s), cls == k);
             if (cls == k) q.q[k][base + pos] = slot;
+        }
+        if (photon == 2u) {
+            // hybrid: the scattering hits are shaded as usual and also queried
+            const bool pq = cls == kQLambert || cls == kQGgx;
+            const uint32_t pos = wave_append(lcount(q, kLQueue + kQCache, sp.s), pq);
+            if (pq) q.q[kQCache][base + pos] = slot;
         }
     }
 }
@@ -622,9 +628,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
                             const float pf = bsdf_pdf_of<CLS == kQLambert>(m, wo, wi);
                             if (!black(f) && !black(Li)) {
                                 const float w = 1.0f / (1.0f + pf * pf);  // powerHeuristic(1, 1, 1, pf)
-                                c[3 * l] = f.x * Li.x * w;
-                                c[3 * l + 1] = f.y * Li.y * w;
-                                c[3 * l + 2] = f.z * Li.z * w;
+                                c[3 * l] = f.x * Li.x * w * wp.nee_scale;
+                                c[3 * l + 1] = f.y * Li.y * w * wp.nee_scale;
+                                c[3 * l + 2] = f.z * Li.z * w * wp.nee_scale;
                                 dist = len(tol);
                                 o2 = spawn_origin(s.p, s.perr, s.n, Lw);
                                 e = !spheres_occlude(sc, o2, Lw, dist, -1);
@@ -659,9 +665,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc0, BatchDev bt, Pat
                                 const rt_material &lm = uniform_ref<UNI>(sc.mat + sc.sph[j].material);
                                 const float w = plp * plp / (plp * plp + pf * pf);
                                 const float k = w / plp;
-                                c[3 * l] = f.x * lm.kd[0] * lm.intensity * k;
-                                c[3 * l + 1] = f.y * lm.kd[1] * lm.intensity * k;
-                                c[3 * l + 2] = f.z * lm.kd[2] * lm.intensity * k;
+                                c[3 * l] = f.x * lm.kd[0] * lm.intensity * k * wp.nee_scale;
+                                c[3 * l + 1] = f.y * lm.kd[1] * lm.intensity * k * wp.nee_scale;
+                                c[3 * l + 2] = f.z * lm.kd[2] * lm.intensity * k * wp.nee_scale;
                                 o2 = spawn_origin(s.p, s.perr, s.n, Lw);
                                 e = !spheres_occlude(sc, o2, Lw, dist * 0.9999f, (int32_t)j);
                             }
@@ -802,6 +808,7 @@ __global__ __launch_bounds__(kBlock) void k_cache_vertex(SceneDev sc, PathsDev p
 // (getPhotonMappingContribution, Scene.cu:724-746, added in
 // computeDirectLightning, pathTracing.cu:186-204).  Lambertian and
 // Trowbridge surfaces emit nothing, so nothing else is added here.
+template <bool HYBRID>
 __global__ __launch_bounds__(kBlock) void k_photon_vertex(SceneDev sc, PathsDev p, QueuesDev q, PhotonLaunch ph)
 {
     const StripeSpan sp = stripe_span();
@@ -816,13 +823,17 @@ __global__ __launch_bounds__(kBlock) void k_photon_vertex(SceneDev sc, PathsDev 
         const bool live = i < n;
         const uint32_t slot = live ? q.q[kQCache][base + i] : 0;
         if (live) {
-            resolve_nee(p, slot);
+            if (!HYBRID) resolve_nee(p, slot);
             const float4 *rec = path_rec(p, slot);
             Surface s{};
             V3 d = mk(0, 0, 1);
             surface_of(sc, rec[kPqRay], rec[kPqHit], s, d);
             const int32_t hid = (int32_t)bits(rec[kPqHit].w);
             const uint32_t k = row0 + i;
+            if (HYBRID) {
+                const float4 qb = rec[kPqBeta];
+                ph.qbeta[3 * (size_t)k] = qb.x; ph.qbeta[3 * (size_t)k + 1] = qb.y; ph.qbeta[3 * (size_t)k + 2] = qb.z;
+            }
             ph.qp[3 * (size_t)k] = s.p.x; ph.qp[3 * (size_t)k + 1] = s.p.y; ph.qp[3 * (size_t)k + 2] = s.p.z;
             ph.qn[3 * (size_t)k] = s.n.x; ph.qn[3 * (size_t)k + 1] = s.n.y; ph.qn[3 * (size_t)k + 2] = s.n.z;
             ph.qv[3 * (size_t)k] = -d.x; ph.qv[3 * (size_t)k + 1] = -d.y; ph.qv[3 * (size_t)k + 2] = -d.z;
@@ -832,22 +843,34 @@ __global__ __launch_bounds__(kBlock) void k_photon_vertex(SceneDev sc, PathsDev 
             ph.qs[k] = hid >= 0 ? (int32_t)(sc.nsph + sc.nplane) + hid : -hid - 2;
             ph.qslot[k] = slot;
         }
-        const uint32_t tpos = wave_append(lcount(q, kLTerm, sp.s), live);
-        if (live) q.term[base + tpos] = slot;
+        if (!HYBRID) {
+            const uint32_t tpos = wave_append(lcount(q, kLTerm, sp.s), live);
+            if (live) q.term[base + tpos] = slot;
+        }
     }
 }
 
-// L += beta * estimate for this iteration's photon queries; the paths ended
-// at their photon vertex and are finalized right after.
+// L += beta * estimate for this iteration's photon queries: photon mapping,
+// the paths ended at their photon vertex (beta is the record's) and are
+// finalized right after; hybrid, L += beta on arrival * estimate / 2 after
+// the vertex was shaded (its next-event terms are halved in k_shade).
+template <bool HYBRID>
 __global__ __launch_bounds__(kBlock) void k_photon_apply(PathsDev p, QueuesDev q, PhotonLaunch ph)
 {
     const uint32_t n = *scount(q, kCCacheQ);
     for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
         float4 *rec = path_rec(p, ph.qslot[k]);
         const double *c = ph.qrgb + 3 * (size_t)k;
-        const float4 qb = rec[kPqBeta], qL = rec[kPqL];
-        rec[kPqL] = make_float4(qL.x + qb.x * (float)c[0], qL.y + qb.y * (float)c[1], qL.z + qb.z * (float)c[2],
-                                qL.w);
+        const float4 qL = rec[kPqL];
+        if (HYBRID) {
+            const float *b = ph.qbeta + 3 * (size_t)k;
+            rec[kPqL] = make_float4(qL.x + b[0] * ((float)c[0] * 0.5f), qL.y + b[1] * ((float)c[1] * 0.5f),
+                                    qL.z + b[2] * ((float)c[2] * 0.5f), qL.w);
+        } else {
+            const float4 qb = rec[kPqBeta];
+            rec[kPqL] = make_float4(qL.x + qb.x * (float)c[0], qL.y + qb.y * (float)c[1], qL.z + qb.z * (float)c[2],
+                                    qL.w);
+        }
     }
 }
 
@@ -1163,7 +1186,10 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
             hipLaunchKernelGGL((k_shade<kQMiss, true>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
         } else {
             hipLaunchKernelGGL(k_classify<false>, grid, blk, 0, s, sc, bt, paths, q, cur, ext_depth, need_record,
-                               photon ? 1u : 0u);
+                               photon ? (c.photon->hybrid ? 2u : 1u) : 0u);
+            // hybrid queries record the throughput on arrival, before shading
+            if (photon && c.photon->hybrid)
+                hipLaunchKernelGGL(k_photon_vertex<true>, grid, blk, 0, s, sc, paths, q, *c.photon);
             hipLaunchKernelGGL((k_shade<kQLambert, false>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQDielectric, false>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
             hipLaunchKernelGGL((k_shade<kQGgx, false>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
@@ -1171,7 +1197,8 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
             hipLaunchKernelGGL((k_shade<kQMiss, false>), grid, blk, 0, s, sc, bt, paths, shadow, q, wp, cur);
         }
         if (lookups) hipLaunchKernelGGL(k_cache_vertex, grid, blk, 0, s, sc, paths, q);
-        if (photon) hipLaunchKernelGGL(k_photon_vertex, grid, blk, 0, s, sc, paths, q, *c.photon);
+        if (photon && !c.photon->hybrid)
+            hipLaunchKernelGGL(k_photon_vertex<false>, grid, blk, 0, s, sc, paths, q, *c.photon);
         hipLaunchKernelGGL(k_count, dim3(1), dim3(kStripes), 0, s, q, cur);
         timed(1, connect);
         if (lookups) {
@@ -1187,7 +1214,8 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
             hipError_t e = photon_gather_render(ph.map, ph.rgb, ph.dir, ph.qp, ph.qn, ph.qv, ph.qm, ph.qs, c.capacity,
                                                 scount(q, kCCacheQ), ph.k, ph.radius, ph.qrgb, s);
             if (e != hipSuccess) return e;
-            hipLaunchKernelGGL(k_photon_apply, grid, blk, 0, s, paths, q, ph);
+            if (ph.hybrid) hipLaunchKernelGGL(k_photon_apply<true>, grid, blk, 0, s, paths, q, ph);
+            else hipLaunchKernelGGL(k_photon_apply<false>, grid, blk, 0, s, paths, q, ph);
         }
         const uint32_t harvest = wp.harvest_shift != kNoHarvest ? 1u : 0u;
         const uint32_t cap = wp.train ? c.capacity : (harvest ? wp.harvest_cap : 0u);

@@ -173,6 +173,7 @@ struct WaveParams {
     // k_finalize compacts up to harvest_cap records into cin / ttgt
     uint32_t harvest_shift = 0xffffffffu;  // kNoHarvest
     uint32_t harvest_cap = 0;
+    float nee_scale = 1.0f;     // factor of every next-event term (RENDER_MODE HYBRID: 0.5)
 };
 constexpr uint32_t kNoHarvest = 0xffffffffu;
 // path meta word: depth (bits 0-15), previous bounce delta (16), camera ray
@@ -192,10 +193,11 @@ struct BatchLaunch {
 struct CacheDev;
 struct CacheWork;
 
-// Photon mapping in the bounce loop (integration/Scene.cu:912-967 with
-// RENDER_MODE PHOTON_MAPPING): a path ends at its first scattering hit
+// Photon mapping in the bounce loop (integration/Scene.cu:912-967): with
+// RENDER_MODE PHOTON_MAPPING a path ends at its first scattering hit
 // (pathTracing.cu:84-88) and its estimate there is the photon map's
-// density estimate instead of next-event estimation.  The map (kd-tree of
+// density estimate instead of next-event estimation; with HYBRID every
+// scattering hit takes half of each and the path goes on.  The map (kd-tree of
 // photon locations and shapes), each photon's radiance and direction, the
 // search's k and radius, and per wave slot the query buffers.
 struct PhotonLaunch {
@@ -209,6 +211,8 @@ struct PhotonLaunch {
     int32_t *qs;                // query shape id (the photons' numbering)
     uint32_t *qslot;            // the query's path slot
     double *qrgb;               // the estimate (3 per query)
+    float *qbeta;               // hybrid: the throughput on arrival (3 per query)
+    uint32_t hybrid;            // RENDER_MODE HYBRID: (estimate + next-event) / 2, the path goes on
 };
 
 // Optional measurement of a frame's traversal kernels.  With events, every

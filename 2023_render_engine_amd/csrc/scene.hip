@@ -289,17 +289,19 @@ static void free_arena(rt_scene *s)
 }
 
 // Photon query rows for `rows` path slots: point, normal, view (3 floats
-// each), material, shape, slot and the estimate (3 doubles).
+// each), material, shape, slot, the estimate (3 doubles) and the
+// throughput on arrival (3 floats, hybrid).
 static rt_status ensure_photon_rows(rt_scene *s, uint32_t rows, PhotonLaunch &ph)
 {
     auto bytes = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t v3 = bytes((size_t)rows * 12), mat = bytes((size_t)rows * sizeof(rt_material)),
                  u = bytes((size_t)rows * 4), rgb = bytes((size_t)rows * 24);
+    // point, normal, view, throughput on arrival: four rows of 3 floats
     if (!s->photon_arena || s->photon_rows < rows) {
         if (s->photon_arena) (void)hipFree(s->photon_arena);
         s->photon_arena = nullptr;
         s->photon_rows = 0;
-        hipError_t e = hipMalloc(&s->photon_arena, 3 * v3 + mat + 2 * u + rgb);
+        hipError_t e = hipMalloc(&s->photon_arena, 4 * v3 + mat + 2 * u + rgb);
         if (e != hipSuccess) {
             set_error(std::string("rt_render_photon: photon query rows: ") + hipGetErrorString(e));
             return RT_ERR_OOM;
@@ -314,6 +316,7 @@ static rt_status ensure_photon_rows(rt_scene *s, uint32_t rows, PhotonLaunch &ph
     ph.qs = (int32_t *)(b + 3 * v3 + mat);
     ph.qslot = (uint32_t *)(b + 3 * v3 + mat + u);
     ph.qrgb = (double *)(b + 3 * v3 + mat + 2 * u);
+    ph.qbeta = (float *)(b + 3 * v3 + mat + 2 * u + rgb);
     return RT_OK;
 }
 
@@ -879,6 +882,7 @@ static rt_status render_frames(const char *fn, rt_scene *s, const rt_render_opts
         wp.grid = opts->grid;
         wp.harvest_shift = harvest_shift;
         wp.harvest_cap = cl.capacity;
+        wp.nee_scale = photon && photon->hybrid ? 0.5f : 1.0f;
         RT_HIP(rt::render_chunk(cl, film, wp, s->wb.host_counters, stream));
         if (harvested) {
             // the chunk's last counter read-back came after its final finalize
@@ -906,7 +910,10 @@ rt_status rt_render_photon(rt_scene *s, const rt_render_opts *opts, rt_photon_ma
                            uint32_t *nvalid, uint32_t flags, void *stream)
 {
     if (!s || !map || !popts) { set_error("rt_render_photon: null argument"); return RT_ERR_INVALID; }
-    if (popts->mode != RT_RENDER_PHOTON_MAPPING) { set_error("rt_render_photon: mode must be RT_RENDER_PHOTON_MAPPING"); return RT_ERR_INVALID; }
+    if (popts->mode != RT_RENDER_PHOTON_MAPPING && popts->mode != RT_RENDER_HYBRID) {
+        set_error("rt_render_photon: mode must be RT_RENDER_PHOTON_MAPPING or RT_RENDER_HYBRID");
+        return RT_ERR_INVALID;
+    }
     if (popts->k == 0 || popts->k > 1024) { set_error("rt_render_photon: k must be 1..1024"); return RT_ERR_INVALID; }
     if (!(popts->radius > 0.0f)) { set_error("rt_render_photon: radius must be positive"); return RT_ERR_INVALID; }
     if (rt::photon_map_ctx(map) != s->ctx) { set_error("rt_render_photon: map and scene belong to different contexts"); return RT_ERR_INVALID; }
@@ -920,6 +927,7 @@ rt_status rt_render_photon(rt_scene *s, const rt_render_opts *opts, rt_photon_ma
     ph.dir = photon_dir;
     ph.k = popts->k;
     ph.radius = popts->radius;
+    ph.hybrid = popts->mode == RT_RENDER_HYBRID ? 1u : 0u;
     return render_frames("rt_render_photon", s, opts, nullptr, 0, accum, nvalid, flags, stream, rt::kNoHarvest,
                          nullptr, &ph);
 }

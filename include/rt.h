@@ -535,10 +535,13 @@ rt_status rt_photon_subsurface(rt_photon_map *map, const double *photon_rgb, con
  * rt_render; the map must be on the scene's context, and its work space
  * serves this call (one render or search per map at a time).  Each
  * iteration searches every path that ended there (k results per path slot
- * of work space: 8 * k bytes per slot). */
-enum { RT_RENDER_PATH_TRACING = 0, RT_RENDER_PHOTON_MAPPING = 1 };
+ * of work space: 8 * k bytes per slot).  RT_RENDER_HYBRID (RENDER_MODE
+ * HYBRID): every scattering hit adds its throughput times (photon estimate
+ * + next-event estimate) / 2 (Scene.cu:963-967) and the path goes on as in
+ * rt_render. */
+enum { RT_RENDER_PATH_TRACING = 0, RT_RENDER_PHOTON_MAPPING = 1, RT_RENDER_HYBRID = 2 };
 typedef struct {
-    uint32_t mode;   /* RT_RENDER_PHOTON_MAPPING */
+    uint32_t mode;   /* RT_RENDER_PHOTON_MAPPING or RT_RENDER_HYBRID */
     uint32_t k;      /* photons per estimate, 1..1024 (MAX_NEIGHBOURS, types.h:60: 100) */
     float radius;    /* search radius (PHOTON_SEARCH_RADIUS, types.h:58: 100) */
 } rt_photon_render_opts;

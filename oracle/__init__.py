@@ -23,7 +23,7 @@ _F = C.c_float
 
 class OrcPhotonRender(C.Structure):
     _fields_ = [("xyz", _P), ("shape", _P), ("order", _P), ("axis", _P), ("n", _U32), ("rgb", _P), ("dir", _P),
-                ("k", _U32), ("radius", _F)]
+                ("k", _U32), ("radius", _F), ("mode", _U32)]
 
 
 class OrcRay(C.Structure):
@@ -565,8 +565,8 @@ class OrcScene:
         return accum, nvalid
 
     def render_photon(self, photons, k=100, radius=100.0, spp=None, spp_offset=0, window=None, seed=None,
-                      nthreads=8, stream_base=0, max_depth=None):
-        """orc_render_photon (RENDER_MODE PHOTON_MAPPING) over photons =
+                      nthreads=8, stream_base=0, max_depth=None, hybrid=False):
+        """orc_render_photon (RENDER_MODE PHOTON_MAPPING, or HYBRID) over photons =
         {location, shape, radiance, direction} (numpy, as trace_photons
         returns them); the kd-tree is built here.  Returns (accum, nvalid)
         like render."""
@@ -578,7 +578,8 @@ class OrcScene:
         rgb = np.ascontiguousarray(photons["radiance"], np.float64).reshape(-1, 3)
         dr = np.ascontiguousarray(photons["direction"], np.float32).reshape(-1, 3)
         order, axis, _ = kd_build(loc)
-        ph = OrcPhotonRender(_p(loc), _p(shp), _p(order), _p(axis), loc.shape[0], _p(rgb), _p(dr), k, radius)
+        ph = OrcPhotonRender(_p(loc), _p(shp), _p(order), _p(axis), loc.shape[0], _p(rgb), _p(dr), k, radius,
+                             2 if hybrid else 1)
         accum = np.zeros(W * H * 3, np.float64)
         nvalid = np.zeros(W * H, np.uint32)
         lib().orc_render_photon(self.h, spp if spp is not None else d.spp, d.seed if seed is None else seed,

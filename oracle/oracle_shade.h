@@ -110,11 +110,14 @@ typedef struct {
     const float *dir;
     uint32_t k;
     float radius;
+    uint32_t mode; /* 1: PHOTON_MAPPING, 2: HYBRID (Scene.cu:963-967) */
 } orc_photon_render;
 
-/* orc_render without next-event estimation: a path's first scattering hit
- * adds beta times the photon map's estimate there (then its emission) and
- * the path ends. */
+/* mode 1: orc_render without next-event estimation: a path's first
+ * scattering hit adds beta times the photon map's estimate there (then its
+ * emission) and the path ends.  mode 2: every scattering hit adds beta times
+ * half the estimate, then half of its next-event terms, and the path goes
+ * on. */
 void orc_render_photon(const orc_scene *s, uint32_t spp, uint32_t seed, uint32_t max_depth, uint32_t rr_depth,
                        uint32_t spp_offset, uint64_t stream_base, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1,
                        double *accum, uint32_t *nvalid, int nthreads, const orc_photon_render *ph);

@@ -879,6 +879,7 @@ static int path_impl(const orc_scene *s, uint32_t px, uint32_t py, uint32_t samp
             L = add(L, mk(beta.x * cache_rgb[0], beta.y * cache_rgb[1], beta.z * cache_rgb[2]));
             break;
         }
+        float nee_scale = 1.0f;
         if (ph && is_scattering(m)) {
             /* RENDER_MODE PHOTON_MAPPING: no direct-light sampling; the
              * scattering hit takes getPhotonMappingContribution (Scene.cu:
@@ -897,12 +898,21 @@ static int path_impl(const orc_scene *s, uint32_t px, uint32_t py, uint32_t samp
             const float nq[3] = {h.n.x, h.n.y, h.n.z}, view[3] = {-d.x, -d.y, -d.z};
             double est[3];
             orc_photon_gather_mat(ph->rgb, ph->dir, 1, &cnt, idx, dist, ph->k, nq, view, m, est);
-            L = add(L, mk(beta.x * (float)est[0], beta.y * (float)est[1], beta.z * (float)est[2]));
-            if (m->type == ORC_MAT_PLAIN) {
-                const float *e = camera ? m->kd : m->ke;
-                L = add(L, mk(beta.x * e[0], beta.y * e[1], beta.z * e[2]));
+            if (ph->mode == 2) {
+                /* HYBRID (Scene.cu:963-967): (photons + direct) / 2, added as
+                 * the library adds them (the photon half, then the halved
+                 * next-event terms); the path goes on */
+                L = add(L, mk(beta.x * ((float)est[0] * 0.5f), beta.y * ((float)est[1] * 0.5f),
+                              beta.z * ((float)est[2] * 0.5f)));
+                nee_scale = 0.5f;
+            } else {
+                L = add(L, mk(beta.x * (float)est[0], beta.y * (float)est[1], beta.z * (float)est[2]));
+                if (m->type == ORC_MAT_PLAIN) {
+                    const float *e = camera ? m->kd : m->ke;
+                    L = add(L, mk(beta.x * e[0], beta.y * e[1], beta.z * e[2]));
+                }
+                break;
             }
-            break;
         }
         v3 n = h.n, t, b;
         onb(n, &t, &b);
@@ -927,7 +937,7 @@ static int path_impl(const orc_scene *s, uint32_t px, uint32_t py, uint32_t samp
                 v3 so = spawn_origin(h.p, h.perr, n, Lw);
                 if (occluded(s, so, Lw, len(tol), -1)) continue;
                 float w = 1.0f / (1.0f + pf * pf); /* powerHeuristic(1, 1, 1, pf), Scene.cu:331 */
-                Ld = add(Ld, mk(f.x * Li.x * w, f.y * Li.y * w, f.z * Li.z * w));
+                Ld = add(Ld, mk(f.x * Li.x * w * nee_scale, f.y * Li.y * w * nee_scale, f.z * Li.z * w * nee_scale));
             }
             for (uint32_t j = 0; j < s->nsph; ++j) {
                 if (!s->sph_light[j]) continue;
@@ -945,8 +955,9 @@ static int path_impl(const orc_scene *s, uint32_t px, uint32_t py, uint32_t samp
                 const orc_material *lm = &s->mat[s->sph_mat[j]];
                 float w = pl * pl / (pl * pl + pf * pf);
                 float k = w / pl;
-                Ld = add(Ld, mk(f.x * lm->kd[0] * lm->intensity * k, f.y * lm->kd[1] * lm->intensity * k,
-                                f.z * lm->kd[2] * lm->intensity * k));
+                Ld = add(Ld, mk(f.x * lm->kd[0] * lm->intensity * k * nee_scale,
+                                f.y * lm->kd[1] * lm->intensity * k * nee_scale,
+                                f.z * lm->kd[2] * lm->intensity * k * nee_scale));
             }
             L = add(L, mk(beta.x * Ld.x, beta.y * Ld.y, beta.z * Ld.z));
         }

@@ -119,3 +119,37 @@ def test_glass_is_seen_through(orc):
     acc2, _ = orc.OrcScene(d).render_photon(ph, k=50, radius=100.0, spp=32)
     img2 = acc2.reshape(d.height, d.width, 3)[..., 0]
     assert np.all(img2[W // 2 - 2:W // 2 + 2, W // 2 - 2:W // 2 + 2] == 0)
+
+
+def test_hybrid_is_the_mean_of_both_estimates_on_a_floor(orc):
+    """RENDER_MODE HYBRID (Scene.cu:963-967): (photon + direct) / 2 at each
+    scattering hit, and the path goes on.  On a floor under a point light a
+    continued path never hits again, and the hybrid draws no extra samples,
+    so every hybrid pixel is half the photon-mapping pixel plus half the
+    path-tracing pixel (up to the FP64 film's summation order)."""
+    d = floor_scene()
+    sc = orc.OrcScene(d)
+    ph, m = sc.trace_photons(50_000, seed=13, max_depth=1)
+    pm, _ = sc.render_photon(ph, k=50, radius=100.0, spp=4)
+    hy, nv = sc.render_photon(ph, k=50, radius=100.0, spp=4, hybrid=True)
+    pt, _ = sc.render(spp=4)
+    assert np.all(nv == 4)
+    assert pm.mean() > 0 and pt.mean() > 0 and not np.allclose(pm, pt, rtol=1e-2)  # distinct estimates
+    assert np.allclose(hy, 0.5 * (pm + pt), rtol=1e-6, atol=0)
+
+
+def test_hybrid_continues_past_scattering_hits(orc):
+    """In a closed Lambertian box the hybrid keeps bouncing (unlike the
+    photon-mapping mode, which stops at the first hit): with photons
+    removed it is exactly half of path tracing's next-event terms plus
+    nothing, so it equals half the path-traced image."""
+    from test_oracle_photon_pass import box_desc as box
+    d = box(kd=0.6, light=((0.1, 0.2, 0.3), (1, 1, 1), 2.0))
+    d.camera = dict(d.camera, width=16, height=16, pos=(0, 0, 0.5), lookat=(0, 0, -1), frame_y=1.0)
+    sc = orc.OrcScene(d)
+    none = dict(location=np.zeros((1, 3), np.float32) + 1e3, shape=np.array([-5], np.int32),
+                radiance=np.ones((1, 3)), direction=np.array([[0, 0, -1]], np.float32))
+    hy, nv = sc.render_photon(none, k=10, radius=1.0, spp=4, max_depth=6, hybrid=True)
+    pt, _ = sc.render(spp=4, max_depth=6)
+    assert pt.mean() > 0
+    assert np.allclose(hy, 0.5 * pt, rtol=1e-6, atol=0)

@@ -884,6 +884,86 @@ def run_bvh_build(args):
     return res
 
 
+def run_photon_render(args):
+    """RENDER_MODE PHOTON_MAPPING on the reference's own photon-mapping setup
+    (integration/PathTracerIntegrator.cu:157-161, Scene.cu:912-967): the
+    Cornell box at 256x256, 16 spp, depth 8; 2^20 photon paths
+    (N_PHOTON_SAMPLES) traced once by rt_scene_trace_photons and put in a
+    photon map; every path ends at its first scattering hit with the k = 100
+    nearest photons within radius 100 (MAX_NEIGHBOURS, PHOTON_SEARCH_RADIUS).
+    One step = one rt_render_photon frame; value = Msamples/s.  The photon
+    pass and the map build are outside the timed region (the reference also
+    builds them once per render)."""
+    import numpy as np
+    import torch
+
+    rt = importlib.import_module("2023_render_engine_amd")
+    scenes = importlib.import_module("2023_render_engine_amd.scenes")
+    desc = scenes.cornell(width=256, height=256, spp=16, max_depth=8, seed=1)
+    ctx = rt.Context(0)
+    sc = rt.Scene(ctx, desc)
+    t0 = time.time()
+    ph = sc.trace_photons(1 << 20, seed=1)
+    torch.cuda.synchronize()
+    pm = rt.PhotonMap(ctx, ph["location"].cpu().numpy(), ph["shape"].cpu().numpy())
+    setup_s = time.time() - t0
+    nphot = int(ph["location"].shape[0])
+    film = sc.new_film()
+    k, radius = 100, 100.0
+    for _ in range(args.warmup):
+        sc.render_photon(film, pm, ph["radiance"], ph["direction"], k=k, radius=radius, spp=16)
+    torch.cuda.synchronize()
+    dev = torch.device("cuda", 0)
+    s = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(s)
+    for i in range(args.steps):
+        sc.render_photon(film, pm, ph["radiance"], ph["direction"], k=k, radius=radius, spp=16,
+                         spp_offset=16 * (i + 1), stream=s.cuda_stream)
+    e1.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    frame_ms = e0.elapsed_time(e1) / args.steps
+    samples = 256 * 256 * 16
+    # compulsory bytes of the frame's photon searches: per query 12 B point,
+    # 4 B count and 8 B per neighbour (index, distance), one query per sample
+    # that reaches a scattering surface (at most one per sample)
+    io = samples * (12 + 4 + 8 * k)
+    gbs = io / (frame_ms * 1e-3) / 1e9
+    res = {
+        "metric": "photon-mapping render (RENDER_MODE PHOTON_MAPPING)", "value": round(samples * args.steps / wall / 1e6, 3),
+        "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (FP64 film, FP64 photon sums)", "data": "synthetic",
+        "config": {"workload": "Cornell box 256x256, 16 spp, depth 8; 2^20 photon paths, k=100 within radius 100",
+                   "photons": nphot, "photon_pass_and_map_build_s": round(setup_s, 3)},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None,
+                     "note": ("whole frame (HIP events on the launch stream): the photon searches' compulsory I/O "
+                              "(12 + 4 + 8k B per sample) / the frame time; the k-nearest search is a dependent "
+                              "walk per query, bound by latency")},
+        "frame_ms": round(frame_ms, 3),
+    }
+    if not args.no_cpu_baseline:
+        import oracle
+        host = {kk: ph[kk].cpu().numpy() for kk in ("location", "shape", "radiance", "direction")}
+        osc = oracle.OrcScene(desc)
+        t1 = time.time()
+        win = (112, 112, 144, 144)
+        osc.render_photon(host, k=k, radius=radius, spp=16, window=win, nthreads=cpu_threads())
+        cs = time.time() - t1
+        n = 32 * 32 * 16
+        res["cpu_baseline"] = {"value": round(n / cs / 1e6, 4), "unit": "Msamples/s", "cores": cpu_threads(),
+                               "kind": "port", "sample": f"the centre 32x32 window at 16 spp ({n} samples) with the "
+                                                         f"oracle's restatement and the same photons, {cs:.1f} s "
+                                                         f"(kd-tree build included)"}
+    pm.close()
+    sc.close()
+    ctx.close()
+    return res
+
+
 def run_photon_pass(args):
     """The photon pre-pass (generatePhotons + computePhotons,
     integration/photonDispersion.cu:26-192) at the reference's size: 2^20
@@ -975,7 +1055,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c2", "c3", "c4", "copy", "film", "photon", "photon_pass",
-                                                         "bvh_build"])
+                                                         "bvh_build", "photon_render"])
     ap.add_argument("--tone", default="gammaClamp", help="film: tone mapping (a ToneMappingType name)")
     ap.add_argument("--c3-ref-spp", type=int, default=128)
     ap.add_argument("--ref-spp", type=int, default=1024,
@@ -1023,6 +1103,8 @@ def main():
         out = run_photon_pass(args)
     elif args.workload == "bvh_build":
         out = run_bvh_build(args)
+    elif args.workload == "photon_render":
+        out = run_photon_render(args)
     else:
         out = run_render(args, args.workload)
     print(json.dumps(out), flush=True)

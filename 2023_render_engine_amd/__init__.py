@@ -149,6 +149,7 @@ _SIGS = {
     "rt_memcpy_d2h": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "rt_bvh_build": (C.c_int, [_P, _P, _U32, _P, _U32, C.POINTER(BvhOpts), C.POINTER(_P)]),
     "rt_bvh_build_device": (C.c_int, [_P, _P, _U32, _P, _U32, C.POINTER(BvhOpts), C.POINTER(_P)]),
+    "rt_scene_set_bvh": (C.c_int, [_P, C.POINTER(BvhOpts), _U32]),
     "rt_bvh_destroy": (C.c_int, [_P]),
     "rt_bvh_get_info": (C.c_int, [_P, C.POINTER(BvhInfo)]),
     "rt_bvh_set_transparent": (C.c_int, [_P, _P]),
@@ -465,7 +466,10 @@ class Scene:
     """rt_scene built from a scenes.SceneDesc; film buffers are torch
     tensors on the context's device (FP64 sums + per-pixel valid counts)."""
 
-    def __init__(self, ctx: Context, desc):
+    def __init__(self, ctx: Context, desc, bvh_on_device=False, bvh_width=4, bvh_max_leaf=4):
+        """bvh_on_device: the scene's BVH is built on the GPU
+        (rt_scene_set_bvh, rt_bvh_build_device) instead of the host's binned
+        SAH."""
         import numpy as np
         self.ctx = ctx
         self.desc = desc
@@ -473,6 +477,9 @@ class Scene:
         h = _P()
         check(L.rt_scene_create(ctx.h, C.byref(h)))
         self.h = h
+        if bvh_on_device or bvh_width != 4 or bvh_max_leaf != 4:
+            bo = BvhOpts(bvh_max_leaf, 32, 1.2, bvh_width)
+            check(L.rt_scene_set_bvh(h, C.byref(bo), int(bool(bvh_on_device))))
         for m in desc.material_structs():
             mid = C.c_uint32()
             check(L.rt_scene_add_material(h, C.byref(m), C.byref(mid)))

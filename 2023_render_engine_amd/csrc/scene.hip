@@ -241,6 +241,9 @@ struct rt_scene {
     // photon-mapping query rows (one per path slot), on demand
     void *photon_arena = nullptr;
     uint32_t photon_rows = 0;
+    // the BVH of the next commit
+    rt_bvh_opts bvh_opts{4, 32, 1.2f, 4};
+    bool bvh_on_device = false;
 };
 
 namespace rt {
@@ -642,6 +645,23 @@ rt_status rt_scene_set_camera(rt_scene *s, const rt_camera *c)
     return RT_OK;
 }
 
+rt_status rt_scene_set_bvh(rt_scene *s, const rt_bvh_opts *opts, uint32_t on_device)
+{
+    if (!s) { set_error("rt_scene_set_bvh: null scene"); return RT_ERR_INVALID; }
+    rt_bvh_opts o{4, 32, 1.2f, 4};
+    if (opts) o = *opts;
+    if (o.width == 0) o.width = 4;
+    if (o.max_leaf == 0) o.max_leaf = 4;
+    if ((o.width != 2 && o.width != 4) || o.max_leaf > 8) {
+        set_error("rt_scene_set_bvh: width must be 2 or 4 and max_leaf 1..8");
+        return RT_ERR_INVALID;
+    }
+    s->bvh_opts = o;
+    s->bvh_on_device = on_device != 0;
+    s->committed = false;
+    return RT_OK;
+}
+
 rt_status rt_scene_commit(rt_scene *s)
 {
     if (!s) { set_error("rt_scene_commit: null scene"); return RT_ERR_INVALID; }
@@ -653,8 +673,30 @@ rt_status rt_scene_commit(rt_scene *s)
     const uint32_t nt = (uint32_t)s->tri_mat.size();
     std::vector<uint32_t> idx(3 * (size_t)nt);
     for (uint32_t i = 0; i < 3 * nt; ++i) idx[i] = i;
-    rt_bvh_opts o{4, 32, 1.2f, 4};
-    rt_status st = rt_bvh_build(s->ctx, s->xyz.data(), 3 * nt, idx.data(), nt, &o, &s->bvh);
+    rt_status st = RT_OK;
+    if (s->bvh_on_device) {
+        // the triangles go up once for the build; the scene's own copies
+        // follow below
+        float *dx = nullptr;
+        uint32_t *di = nullptr;
+        if (nt) {
+            RT_HIP(hipMalloc(&dx, s->xyz.size() * 4));
+            hipError_t e = hipMalloc(&di, idx.size() * 4);
+            if (e == hipSuccess) e = hipMemcpyAsync(dx, s->xyz.data(), s->xyz.size() * 4, hipMemcpyHostToDevice, stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(di, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, stream);
+            if (e != hipSuccess) {
+                (void)hipFree(dx);
+                if (di) (void)hipFree(di);
+                set_error(std::string("rt_scene_commit: triangle upload: ") + hipGetErrorString(e));
+                return RT_ERR_HIP;
+            }
+        }
+        st = rt_bvh_build_device(s->ctx, dx, 3 * nt, di, nt, &s->bvh_opts, &s->bvh);
+        if (dx) (void)hipFree(dx);
+        if (di) (void)hipFree(di);
+    } else {
+        st = rt_bvh_build(s->ctx, s->xyz.data(), 3 * nt, idx.data(), nt, &s->bvh_opts, &s->bvh);
+    }
     if (st != RT_OK) return st;
     std::vector<uint8_t> transparent(nt);
     for (uint32_t i = 0; i < nt; ++i) transparent[i] = s->mats[s->tri_mat[i]].type == RT_MAT_DIELECTRIC;

@@ -291,8 +291,9 @@ def run_render(args, which):
                  f"{desc.idx.shape[0]} triangles, 8 point lights + 1 sphere light, 1920x1080, 64 spp, depth 16")
     ctx = rt.Context(0)
     t0 = time.time()
-    sc = rt.Scene(ctx, desc)
-    log(f"scene committed in {time.time() - t0:.1f} s")
+    sc = rt.Scene(ctx, desc, bvh_on_device=args.bvh_device)
+    commit_s = time.time() - t0
+    log(f"scene committed in {commit_s:.1f} s ({'GPU' if args.bvh_device else 'host binned-SAH'} BVH build)")
     film = sc.new_film()
     for _ in range(args.warmup):
         sc.render(film)
@@ -333,7 +334,9 @@ def run_render(args, which):
         "ms_per_step": round(wall * 1e3 / args.steps, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 (FP64 film)", "data": "synthetic",
-        "config": {"workload": label, "triangles": int(desc.idx.shape[0])},
+        "config": {"workload": label, "triangles": int(desc.idx.shape[0]),
+                   "bvh_build": "GPU (rt_bvh_build_device)" if args.bvh_device else "host binned SAH",
+                   "scene_commit_s": round(commit_s, 3)},
         "mrays_total": round((ext + shadow) / wall / 1e6, 1),
         "mrays_extension": round(ext / wall / 1e6, 1),
         "mrays_shadow": round(shadow / wall / 1e6, 1),
@@ -551,8 +554,9 @@ def run_c3(args):
     npix = sum(d.width * d.height for d in descs)
     ctx = rt.Context(0)
     t0 = time.time()
-    scs = [rt.Scene(ctx, d) for d in descs]
-    log(f"32 scenes committed in {time.time() - t0:.1f} s ({ntris} triangles, {npix} pixels)")
+    scs = [rt.Scene(ctx, d, bvh_on_device=args.bvh_device) for d in descs]
+    commit_s = time.time() - t0
+    log(f"32 scenes committed in {commit_s:.1f} s ({ntris} triangles, {npix} pixels)")
     batch = rt.Batch(ctx, scs)
     seeds = [1000 + i for i in range(len(descs))]
     best, entries = batch.autotune(spp=1, max_depth=12)
@@ -613,7 +617,9 @@ def run_c3(args):
                                 "{256,512,1024}^2), 32 spp, depth 12, one rt_batch launch sequence, graph-replayed "
                                 "frames, autotuned wave size and grid"),
                    "triangles": ntris, "pixels": npix,
-                   "autotuned": {"path_capacity": best.path_capacity, "grid": best.grid}},
+                   "autotuned": {"path_capacity": best.path_capacity, "grid": best.grid},
+                   "bvh_build": "GPU (rt_bvh_build_device)" if args.bvh_device else "host binned SAH",
+                   "scene_commit_s": round(commit_s, 3)},
         "mrays_total": round((ext + shadow) / wall / 1e6, 1),
         "mrays_extension": round(ext / wall / 1e6, 1),
         "sequential_msamples_s": round(npix * 32 / seq_s / 1e6, 3),
@@ -1060,6 +1066,8 @@ def main():
     ap.add_argument("--c3-ref-spp", type=int, default=128)
     ap.add_argument("--ref-spp", type=int, default=1024,
                     help="c0/c2: spp of the FP64 reference the frame's RMSE is taken against (0: skip)")
+    ap.add_argument("--bvh-device", action="store_true",
+                    help="c0/c2/c3: build the scenes' BVHs on the GPU (rt_scene_set_bvh) instead of the host's binned SAH")
     ap.add_argument("--width", type=int, default=4, choices=[2, 4],
                     help="c1: tree layout (4 = four-wide quantized nodes, 2 = two-wide FP32 nodes)")
     ap.add_argument("--c4-width", type=int, default=1920)

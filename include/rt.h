@@ -291,6 +291,13 @@ rt_status rt_scene_add_plane(rt_scene *scene, const float corner[3], const float
                              const float normal[3], uint32_t material);
 rt_status rt_scene_add_point_light(rt_scene *scene, const rt_point_light *light);
 rt_status rt_scene_set_camera(rt_scene *scene, const rt_camera *cam);
+/* The BVH rt_scene_commit builds (no counterpart in the reference, whose
+ * Scene::build always builds its GPU Morton tree, Scene.cu:56-95): opts as
+ * for rt_bvh_build (NULL: the defaults), and on_device != 0 builds it on
+ * the GPU with rt_bvh_build_device instead of the host's binned SAH
+ * (milliseconds instead of seconds for millions of triangles; traces a few
+ * per cent slower).  Takes effect at the next commit. */
+rt_status rt_scene_set_bvh(rt_scene *scene, const rt_bvh_opts *opts, uint32_t on_device);
 rt_status rt_scene_commit(rt_scene *scene);
 /* The scene's triangle BVH (valid after commit; owned by the scene). */
 rt_bvh *rt_scene_bvh(rt_scene *scene);

@@ -490,6 +490,15 @@ public:
     Scene(const Scene &) = delete;
     Scene &operator=(const Scene &) = delete;
 
+    // The BVH the next build() makes: on the GPU from the scene's triangles
+    // (rt_bvh_build_device, as the reference builds its tree) or, by
+    // default, the host's binned SAH.
+    void setBvh(bool on_device, uint32_t max_leaf = 4, uint32_t width = 4)
+    {
+        rt_bvh_opts o{max_leaf, 32, 1.2f, width};
+        check(rt_scene_set_bvh(h_, &o, on_device ? 1u : 0u));
+    }
+
     // Scene::build(shapes, lights, ...): materials are registered once per
     // distinct object; meshes first in the given order (triangle ids follow
     // it), then spheres, then planes, each in the given order (the analytic

@@ -356,7 +356,10 @@ __device__ __forceinline__ bool trav_step(TravState &s, const float4 *__restrict
 // four child boxes are dequantized with one FMA per bound against the
 // node's origin and power-of-two scale, hit children are sorted by entry
 // distance (a 5-comparator network on the distance bits; misses sort last),
-// the nearest is followed and the others are pushed farthest first.
+// the nearest is followed and the others are pushed farthest first.  Any
+// hit (shadow rays, most of which are not occluded and so visit every node
+// they enter whatever the order) only finds the nearest (3 comparators) and
+// pushes the others as they come.
 __device__ __forceinline__ void cswap_key(uint32_t &ka, int32_t &ra, uint32_t &kb, int32_t &rb)
 {
     const bool sw = kb < ka;
@@ -430,8 +433,10 @@ __device__ __forceinline__ bool trav_step4(TravState &s, const float4 *__restric
             cswap_key(key[0], ref[0], key[1], ref[1]);
             cswap_key(key[2], ref[2], key[3], ref[3]);
             cswap_key(key[0], ref[0], key[2], ref[2]);
-            cswap_key(key[1], ref[1], key[3], ref[3]);
-            cswap_key(key[1], ref[1], key[2], ref[2]);
+            if constexpr (!ANY) {
+                cswap_key(key[1], ref[1], key[3], ref[3]);
+                cswap_key(key[1], ref[1], key[2], ref[2]);
+            }
             if (key[0] != 0xffffffffu) {
     #pragma unroll
                 for (int i = 3; i >= 1; --i)

@@ -3,7 +3,9 @@
 // materials, a point light and the .obejota camera, renders it with
 // PathTracerIntegrator and stores the film as <out>.hdrbmp.
 //
-//   cornell_box <out> [width height spp]     render on GPU 0
+//   cornell_box <out> [width height spp [mode [photons]]]   render on GPU 0
+//       mode: pt (path tracing, the default), photon (PHOTON_MAPPING) or
+//       hybrid (HYBRID); photons: photon paths of the pre-pass (2^20)
 //   cornell_box --host-check <dir>           no GPU: BVH build + film I/O
 //
 // Geometry, materials and camera are those of the Cornell-box workload: a
@@ -88,23 +90,34 @@ int main(int argc, char **argv)
 {
     if (argc >= 3 && std::string(argv[1]) == "--host-check") return host_check(argv[2]);
     if (argc < 2) {
-        std::fprintf(stderr, "usage: %s <out> [width height spp] | --host-check <dir>\n", argv[0]);
+        std::fprintf(stderr, "usage: %s <out> [width height spp [pt|photon|hybrid [photons]]] | --host-check <dir>\n",
+                     argv[0]);
         return 2;
     }
     const uint32_t w = argc > 3 ? (uint32_t)std::atoi(argv[2]) : 256;
     const uint32_t h = argc > 3 ? (uint32_t)std::atoi(argv[3]) : 256;
     const uint32_t spp = argc > 4 ? (uint32_t)std::atoi(argv[4]) : 16;
+    const std::string mode = argc > 5 ? argv[5] : "pt";
+    RenderSettings s;
+    s.spp = spp;
+    if (mode == "photon") {
+        s.mode = RenderMode::PHOTON_MAPPING;
+    } else if (mode == "hybrid") {
+        s.mode = RenderMode::HYBRID;
+    } else if (mode != "pt") {
+        std::fprintf(stderr, "unknown mode %s (pt, photon or hybrid)\n", mode.c_str());
+        return 2;
+    }
+    if (argc > 6) s.photons = (uint32_t)std::atoi(argv[6]);
     try {
         Context ctx(0);
         ProjectiveCamera cam({-1.75f, 0, 0}, {0, 0, 0}, {0, 0, 1}, {0, 0, -0.8f}, 0.6f, w, h);
         Scene scene(ctx);
         scene.build(cornell_shapes(), {PointLight({0, 0, 0.5f - 1e-3f}, Spectrum(1.0f), 10.0f)}, cam);
         PathTracerIntegrator integrator(&cam);
-        RenderSettings s;
-        s.spp = spp;
         integrator.render(scene, s);
         cam.film.storeContributionsToFile(argv[1]);
-        std::printf("rendered %ux%u at %u spp into %s.hdrbmp\n", w, h, spp, argv[1]);
+        std::printf("rendered %ux%u at %u spp (%s) into %s.hdrbmp\n", w, h, spp, mode.c_str(), argv[1]);
     } catch (const Error &e) {
         std::fprintf(stderr, "error (%d): %s\n", (int)e.status, e.what());
         return 1;

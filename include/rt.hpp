@@ -575,9 +575,19 @@ private:
 };
 
 // Render settings of the reference's compile-time macros (types.h:41-42,73-74).
+// RENDER_MODE (cuda_libraries/types.h:51) as a run-time setting; the
+// reference's MACHINE_LEARNING mode is not offered.
+enum class RenderMode { PATH_TRACING, PHOTON_MAPPING, HYBRID };
+
 struct RenderSettings {
     uint32_t spp = 16, max_depth = 8, rr_depth = 3, seed = 1, spp_offset = 0, path_capacity = 0, grid = 0;
     uint32_t chunk_paths = 0;  // paths per streamed chunk, 0 = the library's default
+    RenderMode mode = RenderMode::PATH_TRACING;
+    // photon-mapping modes (types.h:57-60): photon paths traced before the
+    // render, their depth and roulette, and the search's k and radius
+    uint32_t photons = 1u << 20, photon_max_depth = 30, photon_rr_depth = 3, photon_seed = 1;
+    uint32_t neighbours = 100;
+    float search_radius = 100.0f;
 };
 
 // PathTracerIntegrator(camera).render(scene): renders every pixel at
@@ -604,7 +614,11 @@ public:
         o.path_capacity = s.path_capacity;
         o.grid = s.grid;
         o.chunk_paths = s.chunk_paths;
-        check(rt_render(scene.get(), &o, (double *)accum.get(), (uint32_t *)nvalid.get(), 0, nullptr));
+        if (s.mode == RenderMode::PATH_TRACING) {
+            check(rt_render(scene.get(), &o, (double *)accum.get(), (uint32_t *)nvalid.get(), 0, nullptr));
+        } else {
+            renderPhotonMapping(scene, o, s, accum, nvalid);
+        }
         ctx.synchronize();
         std::vector<double> sums(npix * 3);
         std::vector<uint32_t> nv(npix);
@@ -612,6 +626,52 @@ public:
         nvalid.download(nv.data(), nv.size() * sizeof(uint32_t));
         cam_->film.addRender(sums, nv, s.spp);
     }
+
+private:
+    // PathTracerIntegrator::render with RENDER_MODE PHOTON_MAPPING or HYBRID
+    // (PathTracerIntegrator.cu:157-161): generatePhotons on the GPU
+    // (rt_scene_trace_photons, growing the arrays once if they were too
+    // small), buildPhotonStructures (rt_photon_map_build over the photons'
+    // locations and shapes), then the bounce loop with the map.
+    void renderPhotonMapping(Scene &scene, const rt_render_opts &o, const RenderSettings &s, DeviceBuffer &accum,
+                             DeviceBuffer &nvalid)
+    {
+        Context &ctx = scene.context();
+        const rt_photon_opts po{s.photons, s.photon_max_depth, s.photon_rr_depth, s.photon_seed};
+        uint32_t cap = s.photons * 4 + 1024, n = 0;
+        for (int attempt = 0;; ++attempt) {
+            DeviceBuffer loc(ctx, (size_t)cap * 12), dir(ctx, (size_t)cap * 12), nrm(ctx, (size_t)cap * 12),
+                rgb(ctx, (size_t)cap * 24), shp(ctx, (size_t)cap * 4);
+            const rt_status st = rt_scene_trace_photons(scene.get(), &po, cap, (float *)loc.get(), (float *)dir.get(),
+                                                        (float *)nrm.get(), (double *)rgb.get(), (int32_t *)shp.get(),
+                                                        nullptr, &n, nullptr);
+            if (st == RT_ERR_INVALID && n > cap && attempt == 0) {
+                cap = n;
+                continue;
+            }
+            check(st);
+            std::vector<float> hloc((size_t)n * 3);
+            std::vector<int32_t> hshp(n);
+            if (n) {
+                loc.download(hloc.data(), hloc.size() * sizeof(float));
+                shp.download(hshp.data(), hshp.size() * sizeof(int32_t));
+            }
+            rt_photon_map *map = nullptr;
+            check(rt_photon_map_build(ctx.get(), hloc.data(), hshp.data(), n, &map));
+            const rt_photon_render_opts ro{s.mode == RenderMode::HYBRID ? (uint32_t)RT_RENDER_HYBRID
+                                                                        : (uint32_t)RT_RENDER_PHOTON_MAPPING,
+                                           s.neighbours, s.search_radius};
+            const rt_status rs = rt_render_photon(scene.get(), &o, map, (const double *)rgb.get(),
+                                                  (const float *)dir.get(), &ro, (double *)accum.get(),
+                                                  (uint32_t *)nvalid.get(), 0, nullptr);
+            if (rs == RT_OK) ctx.synchronize();
+            rt_photon_map_destroy(map);
+            check(rs);
+            return;
+        }
+    }
+
+public:
 
 private:
     ProjectiveCamera *cam_;

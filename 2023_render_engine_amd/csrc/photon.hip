@@ -56,6 +56,7 @@ struct rt_photon_map {
     rt_ctx *ctx = nullptr;
     rt::PhotonNode *nodes = nullptr;  // tree order
     uint32_t *order = nullptr;        // tree position -> input index
+    uint32_t *sig = nullptr;          // tree position -> shape signature of its subtree
     uint32_t n = 0;
     uint32_t depth = 0;
     float lo[3] = {0, 0, 0}, scale[3] = {0, 0, 0};  // Morton grid of the photons' bounding box
@@ -122,6 +123,7 @@ struct KnnArgs {
     const uint32_t *perm;     // lane -> query (Morton order), or null
     const rt::PhotonNode *nodes;
     const uint32_t *order;
+    const uint32_t *sig;      // subtree shape signatures
     uint32_t n;
     const float *queries;     // m * 3
     const int32_t *qshape;    // m or null
@@ -232,6 +234,7 @@ __global__ __launch_bounds__(kThreads) void k_photon_knn(KnnArgs a)
     const uint32_t q = a.perm ? a.perm[lane] : lane;
     const float p[3] = {a.queries[3 * (size_t)q], a.queries[3 * (size_t)q + 1], a.queries[3 * (size_t)q + 2]};
     const uint32_t want = a.qshape ? (uint32_t)(a.qshape[q] + 1) : 0u;  // 0: any shape
+    const uint32_t want_bit = rt::photon_shape_bit(want);
     uint32_t *out_v = a.index + (size_t)q * a.k;
     const HeapRef h = MODE == kHeapLds ? HeapRef{lds_d + threadIdx.x, lds_v + threadIdx.x, (uint32_t)kThreads,
                                                  (uint32_t)kThreads}
@@ -250,6 +253,10 @@ __global__ __launch_bounds__(kThreads) void k_photon_knn(KnnArgs a)
         if (spd[sp] >= 0.0f && !(spd[sp] < maxd)) continue;
         if (r <= l) continue;
         const uint32_t med = l + (r - l) / 2;
+        // a subtree with no photon of the wanted shape takes none, so the
+        // heap and radius leave it as they entered: passing over it keeps
+        // every later step of the walk, and so the result, the same
+        if (want && !(a.sig[med] & want_bit)) continue;
         const rt::PhotonNode nd = a.nodes[med];
         const float d = norm3(p[0] - nd.x, p[1] - nd.y, p[2] - nd.z);
         if (d < maxd && (want == 0u || (nd.meta >> 2) == want)) {
@@ -394,6 +401,7 @@ struct SssArgs {
     const uint32_t *perm;
     const rt::PhotonNode *nodes;
     const uint32_t *order;
+    const uint32_t *sig;
     uint32_t n;
     const double *rgb;
     const float *pdir, *pnrm;
@@ -415,6 +423,7 @@ __global__ __launch_bounds__(kThreads) void k_photon_subsurface(SssArgs a)
     const uint32_t q = a.perm ? a.perm[lane] : lane;
     const float p[3] = {a.queries[3 * (size_t)q], a.queries[3 * (size_t)q + 1], a.queries[3 * (size_t)q + 2]};
     const uint32_t want = a.qshape ? (uint32_t)(a.qshape[q] + 1) : 0u;
+    const uint32_t want_bit = rt::photon_shape_bit(want);
     const float eta = a.prm.eta, g = a.prm.g;
     const float fdr = (float)((-1.440 / (double)(eta * eta)) + (0.710 / (double)eta) + 0.668 + 0.0636 * (double)eta);
     const float A = (1 + fdr) / (1 - fdr);
@@ -435,6 +444,7 @@ __global__ __launch_bounds__(kThreads) void k_photon_subsurface(SssArgs a)
         if (spd[sp] >= 0.0f && !(spd[sp] < a.radius)) continue;
         if (r <= l) continue;
         const uint32_t med = l + (r - l) / 2;
+        if (want && !(a.sig[med] & want_bit)) continue;  // no photon of the shape below: nothing to sum
         const rt::PhotonNode nd = a.nodes[med];
         const float d = norm3(p[0] - nd.x, p[1] - nd.y, p[2] - nd.z);
         if (d < a.radius && (want == 0u || (nd.meta >> 2) == want)) {
@@ -579,7 +589,7 @@ hipError_t photon_gather_render(rt_photon_map *m, const double *photon_rgb, cons
     uint8_t *gw = (uint8_t *)m->gather;
     uint32_t *cnt = (uint32_t *)gw, *idx = (uint32_t *)(gw + cb);
     float *dist = (float *)(gw + cb + ib);
-    KnnArgs a{id1, m->nodes, m->order, m->n, points, qshape, cap, k, radius, cnt, idx, dist, n_dev};
+    KnnArgs a{id1, m->nodes, m->order, m->sig, m->n, points, qshape, cap, k, radius, cnt, idx, dist, n_dev};
     const unsigned blocks = (cap + kThreads - 1) / kThreads;
     if (k <= kLdsK) k_photon_knn<kHeapLdsDist><<<blocks, kThreads, (size_t)k * kThreads * 4, s>>>(a);
     else k_photon_knn<kHeapGlobal><<<blocks, kThreads, 0, s>>>(a);
@@ -616,11 +626,14 @@ rt_status rt_photon_map_build(rt_ctx *ctx, const float *xyz, const int32_t *shap
     hipError_t e = hipSetDevice(rt::ctx_device(ctx));
     if (e == hipSuccess && n) e = hipMalloc(&m->nodes, (size_t)n * sizeof(rt::PhotonNode));
     if (e == hipSuccess && n) e = hipMalloc(&m->order, (size_t)n * sizeof(uint32_t));
+    if (e == hipSuccess && n) e = hipMalloc(&m->sig, (size_t)n * sizeof(uint32_t));
     if (e == hipSuccess && n) e = hipMemcpy(m->nodes, t.nodes.data(), (size_t)n * sizeof(rt::PhotonNode), hipMemcpyHostToDevice);
     if (e == hipSuccess && n) e = hipMemcpy(m->order, t.order.data(), (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && n) e = hipMemcpy(m->sig, t.sig.data(), (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         if (m->nodes) (void)hipFree(m->nodes);
         if (m->order) (void)hipFree(m->order);
+        if (m->sig) (void)hipFree(m->sig);
         delete m;
         set_error(std::string("rt_photon_map_build: ") + hipGetErrorString(e));
         return e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;
@@ -652,6 +665,7 @@ rt_status rt_photon_map_destroy(rt_photon_map *m)
         (void)hipStreamSynchronize((hipStream_t)rt::ctx_stream_ptr(m->ctx));
         if (m->nodes) (void)hipFree(m->nodes);
         if (m->order) (void)hipFree(m->order);
+        if (m->sig) (void)hipFree(m->sig);
         if (m->work) (void)hipFree(m->work);
         if (m->gather) (void)hipFree(m->gather);
     }
@@ -691,7 +705,7 @@ rt_status rt_photon_map_knn(rt_photon_map *m, const float *queries, const int32_
     const uint32_t *id1 = nullptr;
     rt_status sst = sort_queries(m, queries, nq, stream, &id1, "rt_photon_map_knn");
     if (sst != RT_OK) return sst;
-    KnnArgs a{id1, m->nodes, m->order, m->n, queries, query_shape, nq, k, max_distance, count, index, distance,
+    KnnArgs a{id1, m->nodes, m->order, m->sig, m->n, queries, query_shape, nq, k, max_distance, count, index, distance,
               nullptr};
     const unsigned blocks = (nq + kThreads - 1) / kThreads;
     if (k <= kLdsK) k_photon_knn<kHeapLdsDist><<<blocks, kThreads, (size_t)k * kThreads * 4, stream>>>(a);
@@ -811,7 +825,7 @@ rt_status rt_photon_subsurface(rt_photon_map *m, const double *photon_rgb, const
     const uint32_t *perm = nullptr;
     rt_status st = sort_queries(m, points, nq, stream, &perm, "rt_photon_subsurface");
     if (st != RT_OK) return st;
-    SssArgs a{perm, m->nodes, m->order, m->n, photon_rgb, photon_dir, photon_nrm, points, normals, views,
+    SssArgs a{perm, m->nodes, m->order, m->sig, m->n, photon_rgb, photon_dir, photon_nrm, points, normals, views,
               query_shape, nq, radius, *params, out};
     k_photon_subsurface<<<(nq + kThreads - 1) / kThreads, kThreads, 0, stream>>>(a);
     RT_HIP(hipGetLastError());

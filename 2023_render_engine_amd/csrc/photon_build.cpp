@@ -55,6 +55,17 @@ struct Builder {
     }
 };
 
+// Shape signatures of the subtree of every range [l, r), bottom-up.
+uint32_t fill_signatures(const std::vector<PhotonNode> &nodes, std::vector<uint32_t> &sig, size_t l, size_t r)
+{
+    if (r <= l) return 0u;
+    const size_t med = (l + r) / 2;
+    const uint32_t s = photon_shape_bit(nodes[med].meta >> 2) | fill_signatures(nodes, sig, l, med) |
+                       fill_signatures(nodes, sig, med + 1, r);
+    sig[med] = s;
+    return s;
+}
+
 }  // namespace
 
 void build_photon_tree(const float *xyz, const int32_t *shape, uint32_t n, HostPhotonTree &out)
@@ -74,6 +85,8 @@ void build_photon_tree(const float *xyz, const int32_t *shape, uint32_t n, HostP
         p.z = xyz[3 * (size_t)e + 2];
         p.meta = photon_meta(shape ? shape[e] : -1, axis[i]);
     }
+    out.sig.assign(n, 0u);
+    fill_signatures(out.nodes, out.sig, 0, n);
 }
 
 }  // namespace rt

@@ -160,3 +160,34 @@ def test_result_order_equals_the_reference_heap_order(rt, gpu_ctx, k):
         assert np.array_equal(got[1][i, :c], want[1][i, :c]), i
         assert np.array_equal(got[2][i, :c], want[2][i, :c]), i
     assert (got[0] == k).mean() > 0.5
+
+
+@pytest.mark.parametrize("k,radius", [(16, 0.2), (100, 100.0)])
+def test_shape_filtered_order_equals_the_reference_heap_order(rt, gpu_ctx, k, radius):
+    """Shape-restricted searches, as the photon-mapping render makes them:
+    the GPU passes over subtrees whose shape signature (a 32-bit OR of the
+    shapes' bits) lacks the query's shape, which must leave the walk's
+    insertions, and so the result order, exactly the oracle's.  Shapes are
+    spatial cells (as surfaces are), 40 of them so that shapes 32 apart share
+    a signature bit, plus queries with no shape (-1) and on shapes with no
+    photon."""
+    print(f"[photon] shape-filtered result order k={k} r={radius}", flush=True)
+    rng = np.random.default_rng(90 + k)
+    x = np.stack([(rng.permutation(1 << 20)[:30000] + 0.5) / (1 << 20) for _ in range(3)], 1).astype(np.float32)
+    sh = (np.floor(x[:, 0] * 8) + 8 * np.floor(x[:, 1] * 5)).astype(np.int32)
+    sh[rng.random(len(x)) < 0.02] = -1  # photons with no shape
+    q = rng.random((4096, 3)).astype(np.float32)
+    q[:64] = x[:64]
+    qs = (np.floor(q[:, 0] * 8) + 8 * np.floor(q[:, 1] * 5)).astype(np.int32)
+    qs[rng.random(len(q)) < 0.1] = -1
+    qs[rng.random(len(q)) < 0.05] = 45  # no photon of this shape
+    got = gpu_knn(rt, gpu_ctx, x, q, k, radius, sh, qs)
+    tree = oracle.kd_build(x)[:2]
+    want = oracle.knn(x, q, k, radius, sh, qs, tree=tree)
+    assert np.array_equal(got[0], want[0])
+    for i in range(q.shape[0]):
+        c = got[0][i]
+        assert np.array_equal(got[1][i, :c], want[1][i, :c]), i
+        assert np.array_equal(got[2][i, :c], want[2][i, :c]), i
+    assert np.all(got[0][qs == 45] == 0)
+    assert (got[0] == k).mean() > 0.5

@@ -16,8 +16,9 @@
 // Queries are taken in the Morton order of their position (a radix sort in
 // the map's work space), one lane per query; the recursion becomes an
 // explicit stack of index ranges in private memory (at most the tree's depth
-// + 1 entries).  For k <= 128 the heap's distances live in LDS and its ids in
-// the query's output slice; beyond, both live in the output slice.
+// + 1 entries).  For k <= 128 the heap's distances and one-byte slot numbers
+// live in LDS and the photons of the slots in the query's output slice;
+// beyond, the heap lives in the output slice.
 #include <hip/hip_runtime.h>
 #include <hipcub/device/device_radix_sort.hpp>
 
@@ -26,6 +27,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <type_traits>
 
 #include "rt_internal.h"
 #include "rt_photon.h"
@@ -135,13 +137,15 @@ struct KnnArgs {
     const uint32_t *m_dev;    // if set: only the first *m_dev of the m lanes are queries
 };
 
-// Strided view of a heap: distance i at d[i * sd], id i at v[i * sv].
+// Strided view of a heap: distance i at d[i * sd], id i at v[i * sv] (a
+// photon's tree position, or a slot number that names where it is kept).
+template <typename V>
 struct HeapRef {
     float *d;
-    uint32_t *v;
+    V *v;
     uint32_t sd, sv;
     __device__ float &dist(uint32_t i) const { return d[i * sd]; }
-    __device__ uint32_t &id(uint32_t i) const { return v[i * sv]; }
+    __device__ V &id(uint32_t i) const { return v[i * sv]; }
 };
 
 // The max-heap steps of libstdc++'s std::make_heap / push_heap / pop_heap
@@ -150,7 +154,8 @@ struct HeapRef {
 // the steps are kept as they are rather than replaced by an equivalent
 // sift-down (which would order ties and equal-set layouts differently).
 // __push_heap: the hole moves up while its parent compares less.
-__device__ inline void heap_push_up(const HeapRef &h, uint32_t hole, uint32_t top, float d, uint32_t v)
+template <typename H>
+__device__ inline void heap_push_up(const H &h, uint32_t hole, uint32_t top, float d, uint32_t v)
 {
     while (hole > top) {
         const uint32_t parent = (hole - 1) / 2;
@@ -165,7 +170,8 @@ __device__ inline void heap_push_up(const HeapRef &h, uint32_t hole, uint32_t to
 
 // __adjust_heap over [0, len): the hole goes down to a leaf along the larger
 // children, then the value is pushed up from there.
-__device__ inline void heap_adjust(const HeapRef &h, uint32_t hole, uint32_t len, float d, uint32_t v)
+template <typename H>
+__device__ inline void heap_adjust(const H &h, uint32_t hole, uint32_t len, float d, uint32_t v)
 {
     const uint32_t top = hole;
     uint32_t child = hole;
@@ -186,7 +192,8 @@ __device__ inline void heap_adjust(const HeapRef &h, uint32_t hole, uint32_t len
 }
 
 // __make_heap over the k entries found so far.
-__device__ inline void heap_make(const HeapRef &h, uint32_t len)
+template <typename H>
+__device__ inline void heap_make(const H &h, uint32_t len)
 {
     if (len < 2) return;
     for (uint32_t parent = (len - 2) / 2;; --parent) {
@@ -198,7 +205,8 @@ __device__ inline void heap_make(const HeapRef &h, uint32_t len)
 // push_back + push_heap + pop_heap + pop_back on a full heap of len entries.
 // The transient entry len lives in (xd, xv), so the heap needs no slot
 // beyond len.
-__device__ inline void heap_push_pop(const HeapRef &h, uint32_t len, float d, uint32_t v)
+template <typename H>
+__device__ inline void heap_push_pop(const H &h, uint32_t len, float d, uint32_t v)
 {
     float xd = d;
     uint32_t xv = v;
@@ -215,20 +223,25 @@ __device__ inline void heap_push_pop(const HeapRef &h, uint32_t len, float d, ui
     heap_adjust(h, 0, len, xd, xv);
 }
 
-// Where the heap lives.  kHeapLds: distances and ids in LDS, slot-major
-// (entry i of lane t at i * 64 + t, so the lanes of the wavefront hit
-// distinct banks), copied to the query's output slice at the end.
-// kHeapLdsDist: distances in LDS, ids in the output slice (half the LDS per
-// query, so twice the wavefronts per CU; measured 24.6 against 19.0
-// Mqueries/s at k = 100).  kHeapGlobal: both in the output slice.
-enum : int { kHeapGlobal = 0, kHeapLds = 1, kHeapLdsDist = 2 };
-template <int MODE>
+// Where the heap lives.  kHeapLdsSlot: distances in LDS, slot-major (entry
+// i of lane t at i * 64 + t, so the lanes of the wavefront hit distinct
+// banks), each with a one-byte slot number beside it; the photon of slot s
+// is kept in word s of the query's distance slice, written once when it
+// enters.  The first k photons take slots 0 .. k-1; later each takes the
+// slot of the entry dropped (the heap's root) when it is not dropped itself.
+// So the heap's sift
+// steps move distances and slot numbers in LDS only, and the photons' ids
+// are looked up once at the end.  (Ids moved along with the distances in
+// the output slice cost a dependent global load and store per sift level:
+// 24.6 M queries/s at k = 100.)  kHeapGlobal: both in the output slice.
+enum : int { kHeapGlobal = 0, kHeapLdsSlot = 1 };
+template <int MODE, uint32_t STACK>
 __global__ __launch_bounds__(kThreads) void k_photon_knn(KnnArgs a)
 {
-    constexpr bool LDS = MODE != kHeapGlobal;
-    extern __shared__ uint32_t lds_heap[];  // k * kThreads distances, then (kHeapLds) k * kThreads ids
+    constexpr bool SLOT = MODE == kHeapLdsSlot;
+    extern __shared__ uint32_t lds_heap[];  // k * kThreads distances, then k * kThreads slot bytes
     float *lds_d = reinterpret_cast<float *>(lds_heap);
-    uint32_t *lds_v = lds_heap + (MODE == kHeapLds ? a.k * kThreads : 0u);
+    uint8_t *lds_s = reinterpret_cast<uint8_t *>(lds_heap + a.k * kThreads);
     const uint32_t lane = blockIdx.x * kThreads + threadIdx.x;
     if (lane >= (a.m_dev ? min(a.m, *a.m_dev) : a.m)) return;
     const uint32_t q = a.perm ? a.perm[lane] : lane;
@@ -236,60 +249,96 @@ __global__ __launch_bounds__(kThreads) void k_photon_knn(KnnArgs a)
     const uint32_t want = a.qshape ? (uint32_t)(a.qshape[q] + 1) : 0u;  // 0: any shape
     const uint32_t want_bit = rt::photon_shape_bit(want);
     uint32_t *out_v = a.index + (size_t)q * a.k;
-    const HeapRef h = MODE == kHeapLds ? HeapRef{lds_d + threadIdx.x, lds_v + threadIdx.x, (uint32_t)kThreads,
-                                                 (uint32_t)kThreads}
-                      : MODE == kHeapLdsDist ? HeapRef{lds_d + threadIdx.x, out_v, (uint32_t)kThreads, 1u}
-                                             : HeapRef{a.dist + (size_t)q * a.k, out_v, 1u, 1u};
+    float *out_d = a.dist + (size_t)q * a.k;
+    // kHeapLdsSlot: out_d[s] holds the tree position of slot s (as float bits) until the end
+    using H = HeapRef<typename std::conditional<SLOT, uint8_t, uint32_t>::type>;
+    H h;
+    if constexpr (SLOT) h = H{lds_d + threadIdx.x, lds_s + threadIdx.x, (uint32_t)kThreads, (uint32_t)kThreads};
+    else h = H{out_d, out_v, 1u, 1u};
     float maxd = a.radius;
     uint32_t cnt = 0;
 
-    uint32_t sl[kStack], sr[kStack];
-    float spd[kStack];
+    // kdtree.h's recursion as a loop: the near child is searched next; the
+    // far child is stacked with the distance to its splitting plane and
+    // searched once the near side is done, if that distance is still below
+    // the radius then.  Only far children are stacked (at most depth - 1
+    // entries).  The stack lives in registers, indexed through selects, so
+    // each entry costs a compare and select per word and per slot.
+    uint32_t sl[STACK], sr[STACK];
+    float spd[STACK];
     uint32_t sp = 0;
-    sl[0] = 0; sr[0] = a.n; spd[0] = -1.0f; sp = 1;
-    while (sp > 0) {
-        --sp;
-        const uint32_t l = sl[sp], r = sr[sp];
-        if (spd[sp] >= 0.0f && !(spd[sp] < maxd)) continue;
-        if (r <= l) continue;
+    uint32_t l = 0, r = a.n;
+    for (;;) {
+        if (r <= l) {
+            // this subtree is done: the last stacked far child whose plane
+            // is still within the radius
+            while (sp > 0 && !(spd[sp - 1] < maxd)) --sp;
+            if (sp == 0) break;
+            --sp;
+            l = sl[sp];
+            r = sr[sp];
+        }
         const uint32_t med = l + (r - l) / 2;
         // a subtree with no photon of the wanted shape takes none, so the
         // heap and radius leave it as they entered: passing over it keeps
         // every later step of the walk, and so the result, the same
-        if (want && !(a.sig[med] & want_bit)) continue;
+        if (want && !(a.sig[med] & want_bit)) { r = l; continue; }
         const rt::PhotonNode nd = a.nodes[med];
         const float d = norm3(p[0] - nd.x, p[1] - nd.y, p[2] - nd.z);
-        if (d < maxd && (want == 0u || (nd.meta >> 2) == want)) {
-            // push_back; make_heap at k; beyond k push_heap + pop_heap +
-            // pop_back and max_distance = the front's (kdtree.h:123-130)
-            if (cnt < a.k) {
-                h.dist(cnt) = d;
-                h.id(cnt) = med;
-                ++cnt;
-                if (cnt == a.k) heap_make(h, a.k);
-            } else {
-                heap_push_pop(h, a.k, d, med);
-                maxd = h.dist(0);
-            }
-        }
+        const bool take = d < maxd && (want == 0u || (nd.meta >> 2) == want);
         if (r - l > 1) {
             const uint32_t ax = nd.meta & 3u;
             const float split = ax == 0 ? nd.x : (ax == 1 ? nd.y : nd.z);
             const float diff = p[ax] - split;
             const float pd = norm3(ax == 0 ? diff : 0.0f, ax == 1 ? diff : 0.0f, ax == 2 ? diff : 0.0f);
-            uint32_t nl, nr, fl, fr;
-            if (p[ax] < split) { nl = l; nr = med; fl = med + 1; fr = r; }
-            else { nl = med + 1; nr = r; fl = l; fr = med; }
-            if (sp + 2 > kStack) break;  // unreachable: the host checks depth + 1 <= kStack
-            sl[sp] = fl; sr[sp] = fr; spd[sp] = pd; ++sp;
-            sl[sp] = nl; sr[sp] = nr; spd[sp] = -1.0f; ++sp;
+            uint32_t fl, fr;
+            if (p[ax] < split) { fl = med + 1; fr = r; r = med; }
+            else { fl = l; fr = med; l = med + 1; }
+            if (fr > fl) {
+                if (sp >= STACK) break;  // unreachable: the host checks depth - 1 <= STACK
+                sl[sp] = fl; sr[sp] = fr; spd[sp] = pd; ++sp;
+            }
+        } else {
+            r = l;
+        }
+        // the stacked plane distance is compared with the radius only when
+        // popped, so the heap step may follow the push
+        if (take) {
+            // push_back; make_heap at k; beyond k push_heap + pop_heap +
+            // pop_back and max_distance = the front's (kdtree.h:123-130)
+            if (cnt < a.k) {
+                h.dist(cnt) = d;
+                if constexpr (SLOT) {
+                    out_d[cnt] = __uint_as_float(med);
+                    h.id(cnt) = (uint8_t)cnt;
+                } else {
+                    h.id(cnt) = med;
+                }
+                ++cnt;
+                if (cnt == a.k) heap_make(h, a.k);
+            } else {
+                if constexpr (SLOT) {
+                    // The farthest of the k + 1 is dropped.  That is the root,
+                    // unless the new photon is farther: the radius becomes
+                    // the root's distance only at the first replacement, so
+                    // the first photon after make_heap can be, and then it
+                    // rises to the root and is dropped itself.  It moves
+                    // with the spare slot number k, which names no photon.
+                    const uint32_t slot = h.dist(0) < d ? a.k : h.id(0);
+                    if (slot < a.k) out_d[slot] = __uint_as_float(med);
+                    heap_push_pop(h, a.k, d, slot);
+                } else {
+                    heap_push_pop(h, a.k, d, med);
+                }
+                maxd = h.dist(0);
+            }
         }
     }
-    float *od = a.dist + (size_t)q * a.k;
-    uint32_t *oi = a.index + (size_t)q * a.k;
-    for (uint32_t i = 0; i < cnt; ++i) {
-        oi[i] = a.order[h.id(i)];
-        if (LDS) od[i] = h.dist(i);
+    if constexpr (SLOT) {
+        for (uint32_t i = 0; i < cnt; ++i) out_v[i] = a.order[__float_as_uint(out_d[h.id(i)])];
+        for (uint32_t i = 0; i < cnt; ++i) out_d[i] = h.dist(i);  // over the slots, all read above
+    } else {
+        for (uint32_t i = 0; i < cnt; ++i) out_v[i] = a.order[out_v[i]];
     }
     a.count[q] = cnt;
 }
@@ -477,6 +526,23 @@ __global__ __launch_bounds__(kThreads) void k_photon_subsurface(SssArgs a)
     for (int c = 0; c < 3; ++c) a.out[3 * (size_t)q + c] = acc[c] / (double)den;
 }
 
+// The search kernel for a k and a tree depth: the LDS heap up to kLdsK,
+// and the smaller register stack when the tree is shallow enough for it
+// (every map of up to 2^25 photons).
+constexpr uint32_t kStackSmall = 24;
+void launch_knn(const KnnArgs &a, uint32_t depth, uint32_t blocks, hipStream_t s)
+{
+    const bool small = depth <= kStackSmall + 1;
+    if (a.k <= kLdsK) {
+        const size_t lds = (size_t)a.k * kThreads * 5;  // float distance + slot byte per entry
+        if (small) k_photon_knn<kHeapLdsSlot, kStackSmall><<<blocks, kThreads, lds, s>>>(a);
+        else k_photon_knn<kHeapLdsSlot, kStack><<<blocks, kThreads, lds, s>>>(a);
+    } else {
+        if (small) k_photon_knn<kHeapGlobal, kStackSmall><<<blocks, kThreads, 0, s>>>(a);
+        else k_photon_knn<kHeapGlobal, kStack><<<blocks, kThreads, 0, s>>>(a);
+    }
+}
+
 }  // namespace
 
 // Morton order of nq query points in the map's work space (perm: lane ->
@@ -591,8 +657,7 @@ hipError_t photon_gather_render(rt_photon_map *m, const double *photon_rgb, cons
     float *dist = (float *)(gw + cb + ib);
     KnnArgs a{id1, m->nodes, m->order, m->sig, m->n, points, qshape, cap, k, radius, cnt, idx, dist, n_dev};
     const unsigned blocks = (cap + kThreads - 1) / kThreads;
-    if (k <= kLdsK) k_photon_knn<kHeapLdsDist><<<blocks, kThreads, (size_t)k * kThreads * 4, s>>>(a);
-    else k_photon_knn<kHeapGlobal><<<blocks, kThreads, 0, s>>>(a);
+    launch_knn(a, m->depth, blocks, s);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_photon_gather_mat<<<(cap + 255) / 256, 256, 0, s>>>(photon_rgb, photon_dir, normals, views, mats, cap, k, cnt,
                                                           idx, dist, out, n_dev);
@@ -708,8 +773,7 @@ rt_status rt_photon_map_knn(rt_photon_map *m, const float *queries, const int32_
     KnnArgs a{id1, m->nodes, m->order, m->sig, m->n, queries, query_shape, nq, k, max_distance, count, index, distance,
               nullptr};
     const unsigned blocks = (nq + kThreads - 1) / kThreads;
-    if (k <= kLdsK) k_photon_knn<kHeapLdsDist><<<blocks, kThreads, (size_t)k * kThreads * 4, stream>>>(a);
-    else k_photon_knn<kHeapGlobal><<<blocks, kThreads, 0, stream>>>(a);
+    launch_knn(a, m->depth, blocks, stream);
     RT_HIP(hipGetLastError());
     return RT_OK;
 }

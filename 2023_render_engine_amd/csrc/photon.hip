@@ -68,6 +68,11 @@ struct rt_photon_map {
     size_t work_bytes = 0;
     void *gather = nullptr;  // k-nearest results of rt_photon_gather_lambert, grown on demand
     size_t gather_bytes = 0;
+    // the photons' radiance (3 doubles) and direction (3 floats) in tree
+    // order, copied at the start of each render that uses the map, so a
+    // query's photons (a few subtrees) are read from a few cache lines
+    double *trgb = nullptr;
+    float *tdir = nullptr;
 };
 
 namespace {
@@ -124,7 +129,7 @@ __global__ __launch_bounds__(256) void k_query_morton(const float *q, uint32_t m
 struct KnnArgs {
     const uint32_t *perm;     // lane -> query (Morton order), or null
     const rt::PhotonNode *nodes;
-    const uint32_t *order;
+    const uint32_t *order;    // tree position -> input index, or null: results are tree positions
     const uint32_t *sig;      // subtree shape signatures
     uint32_t n;
     const float *queries;     // m * 3
@@ -335,10 +340,14 @@ __global__ __launch_bounds__(kThreads) void k_photon_knn(KnnArgs a)
         }
     }
     if constexpr (SLOT) {
-        for (uint32_t i = 0; i < cnt; ++i) out_v[i] = a.order[__float_as_uint(out_d[h.id(i)])];
+        for (uint32_t i = 0; i < cnt; ++i) {
+            const uint32_t pos = __float_as_uint(out_d[h.id(i)]);
+            out_v[i] = a.order ? a.order[pos] : pos;
+        }
         for (uint32_t i = 0; i < cnt; ++i) out_d[i] = h.dist(i);  // over the slots, all read above
     } else {
-        for (uint32_t i = 0; i < cnt; ++i) out_v[i] = a.order[out_v[i]];
+        if (a.order)
+            for (uint32_t i = 0; i < cnt; ++i) out_v[i] = a.order[out_v[i]];
     }
     a.count[q] = cnt;
 }
@@ -526,6 +535,19 @@ __global__ __launch_bounds__(kThreads) void k_photon_subsurface(SssArgs a)
     for (int c = 0; c < 3; ++c) a.out[3 * (size_t)q + c] = acc[c] / (double)den;
 }
 
+// Radiance and direction of tree position i = those of photon order[i].
+__global__ __launch_bounds__(256) void k_photon_tree_copy(const uint32_t *order, uint32_t n, const double *rgb,
+                                                          const float *dir, double *trgb, float *tdir)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const size_t e = order[i];
+    for (int c = 0; c < 3; ++c) {
+        trgb[3 * (size_t)i + c] = rgb[3 * e + c];
+        tdir[3 * (size_t)i + c] = dir[3 * e + c];
+    }
+}
+
 // The search kernel for a k and a tree depth: the LDS heap up to kLdsK,
 // and the smaller register stack when the tree is shallow enough for it
 // (every map of up to 2^25 photons).
@@ -602,7 +624,8 @@ static void render_layout(uint32_t cap, uint32_t k, size_t sort_bytes, size_t &i
     gather = cb + 2 * ib;
 }
 
-hipError_t photon_render_reserve(rt_photon_map *m, uint32_t cap, uint32_t k, hipStream_t s, std::string &err)
+hipError_t photon_render_reserve(rt_photon_map *m, uint32_t cap, uint32_t k, const double *photon_rgb,
+                                 const float *photon_dir, hipStream_t s, std::string &err)
 {
     size_t sort_bytes = 0;
     hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
@@ -627,11 +650,19 @@ hipError_t photon_render_reserve(rt_photon_map *m, uint32_t cap, uint32_t k, hip
         if ((e = hipMalloc(&m->gather, gather)) != hipSuccess) { err = "photon search results"; return e; }
         m->gather_bytes = gather;
     }
+    if (m->n && !m->trgb) {
+        if ((e = hipMalloc(&m->trgb, (size_t)m->n * 24)) != hipSuccess) { err = "photon radiance in tree order"; return e; }
+        if ((e = hipMalloc(&m->tdir, (size_t)m->n * 12)) != hipSuccess) { err = "photon directions in tree order"; return e; }
+    }
+    if (m->n) {
+        k_photon_tree_copy<<<(m->n + 255) / 256, 256, 0, s>>>(m->order, m->n, photon_rgb, photon_dir, m->trgb,
+                                                              m->tdir);
+        if ((e = hipGetLastError()) != hipSuccess) { err = "photon tree-order copy"; return e; }
+    }
     return hipSuccess;
 }
 
-hipError_t photon_gather_render(rt_photon_map *m, const double *photon_rgb, const float *photon_dir,
-                                const float *points, const float *normals, const float *views,
+hipError_t photon_gather_render(rt_photon_map *m, const float *points, const float *normals, const float *views,
                                 const rt_material *mats, const int32_t *qshape, uint32_t cap,
                                 const uint32_t *n_dev, uint32_t k, float radius, double *out, hipStream_t s)
 {
@@ -655,12 +686,13 @@ hipError_t photon_gather_render(rt_photon_map *m, const double *photon_rgb, cons
     uint8_t *gw = (uint8_t *)m->gather;
     uint32_t *cnt = (uint32_t *)gw, *idx = (uint32_t *)(gw + cb);
     float *dist = (float *)(gw + cb + ib);
-    KnnArgs a{id1, m->nodes, m->order, m->sig, m->n, points, qshape, cap, k, radius, cnt, idx, dist, n_dev};
+    // results as tree positions, read against the tree-ordered copies
+    KnnArgs a{id1, m->nodes, nullptr, m->sig, m->n, points, qshape, cap, k, radius, cnt, idx, dist, n_dev};
     const unsigned blocks = (cap + kThreads - 1) / kThreads;
     launch_knn(a, m->depth, blocks, s);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_photon_gather_mat<<<(cap + 255) / 256, 256, 0, s>>>(photon_rgb, photon_dir, normals, views, mats, cap, k, cnt,
-                                                          idx, dist, out, n_dev);
+    k_photon_gather_mat<<<(cap + 255) / 256, 256, 0, s>>>(m->trgb, m->tdir, normals, views, mats, cap, k, cnt, idx,
+                                                          dist, out, n_dev);
     return hipGetLastError();
 }
 
@@ -731,6 +763,8 @@ rt_status rt_photon_map_destroy(rt_photon_map *m)
         if (m->nodes) (void)hipFree(m->nodes);
         if (m->order) (void)hipFree(m->order);
         if (m->sig) (void)hipFree(m->sig);
+        if (m->trgb) (void)hipFree(m->trgb);
+        if (m->tdir) (void)hipFree(m->tdir);
         if (m->work) (void)hipFree(m->work);
         if (m->gather) (void)hipFree(m->gather);
     }

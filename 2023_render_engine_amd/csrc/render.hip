@@ -1211,7 +1211,7 @@ hipError_t chunk_iterate(const ChunkLaunch &c, const WaveParams &wp, uint32_t it
         }
         if (photon) {
             const PhotonLaunch &ph = *c.photon;
-            hipError_t e = photon_gather_render(ph.map, ph.rgb, ph.dir, ph.qp, ph.qn, ph.qv, ph.qm, ph.qs, c.capacity,
+            hipError_t e = photon_gather_render(ph.map, ph.qp, ph.qn, ph.qv, ph.qm, ph.qs, c.capacity,
                                                 scount(q, kCCacheQ), ph.k, ph.radius, ph.qrgb, s);
             if (e != hipSuccess) return e;
             if (ph.hybrid) hipLaunchKernelGGL(k_photon_apply<true>, grid, blk, 0, s, paths, q, ph);

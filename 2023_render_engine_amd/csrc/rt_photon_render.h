@@ -11,17 +11,20 @@
 
 namespace rt {
 
-// Sizes the map's work space for up to `cap` queries of k photons (may
-// allocate and synchronize the stream; call before the loop).
-hipError_t photon_render_reserve(rt_photon_map *m, uint32_t cap, uint32_t k, hipStream_t s, std::string &err);
+// Sizes the map's work space for up to `cap` queries of k photons and
+// copies the photons' radiance (3 doubles) and direction (3 floats, both in
+// the map's input order) into tree order on stream s (may allocate and
+// synchronize the stream; call before the loop).
+hipError_t photon_render_reserve(rt_photon_map *m, uint32_t cap, uint32_t k, const double *photon_rgb,
+                                 const float *photon_dir, hipStream_t s, std::string &err);
 
 // The k nearest photons within radius (on the query's shape) and
 // combinePhotonContributions for the first *n_dev of `cap` queries:
 // points, normals (shading) and views (towards the viewer) 3 floats each,
 // the query's material (textured kd) and shape id; out: 3 doubles per query.
 // Launches only (no allocation, no synchronisation).
-hipError_t photon_gather_render(rt_photon_map *m, const double *photon_rgb, const float *photon_dir,
-                                const float *points, const float *normals, const float *views,
+// The photons' radiance and direction are the copies the reserve made.
+hipError_t photon_gather_render(rt_photon_map *m, const float *points, const float *normals, const float *views,
                                 const rt_material *mats, const int32_t *qshape, uint32_t cap,
                                 const uint32_t *n_dev, uint32_t k, float radius, double *out, hipStream_t s);
 

@@ -885,7 +885,8 @@ static rt_status render_frames(const char *fn, rt_scene *s, const rt_render_opts
         st = rt::ensure_photon_rows(s, s->wb.capacity, *photon);
         if (st != RT_OK) return st;
         std::string err;
-        hipError_t e = rt::photon_render_reserve(photon->map, s->wb.capacity, photon->k, stream, err);
+        hipError_t e = rt::photon_render_reserve(photon->map, s->wb.capacity, photon->k, photon->rgb, photon->dir, stream,
+                                                 err);
         if (e != hipSuccess) {
             set_error(name + ": " + err + ": " + hipGetErrorString(e));
             return e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;

@@ -269,19 +269,21 @@ __global__ __launch_bounds__(kThreads) void k_photon_knn(KnnArgs a)
     // the radius then.  Only far children are stacked (at most depth - 1
     // entries).  The stack lives in registers, indexed through selects, so
     // each entry costs a compare and select per word and per slot.
-    uint32_t sl[STACK], sr[STACK];
-    float spd[STACK];
+    struct Far {
+        uint32_t l, r;
+        float pd;
+    } st[STACK];
     uint32_t sp = 0;
     uint32_t l = 0, r = a.n;
     for (;;) {
         if (r <= l) {
             // this subtree is done: the last stacked far child whose plane
             // is still within the radius
-            while (sp > 0 && !(spd[sp - 1] < maxd)) --sp;
+            while (sp > 0 && !(st[sp - 1].pd < maxd)) --sp;
             if (sp == 0) break;
             --sp;
-            l = sl[sp];
-            r = sr[sp];
+            l = st[sp].l;
+            r = st[sp].r;
         }
         const uint32_t med = l + (r - l) / 2;
         // a subtree with no photon of the wanted shape takes none, so the
@@ -301,7 +303,8 @@ __global__ __launch_bounds__(kThreads) void k_photon_knn(KnnArgs a)
             else { fl = l; fr = med; l = med + 1; }
             if (fr > fl) {
                 if (sp >= STACK) break;  // unreachable: the host checks depth - 1 <= STACK
-                sl[sp] = fl; sr[sp] = fr; spd[sp] = pd; ++sp;
+                st[sp] = Far{fl, fr, pd};
+                ++sp;
             }
         } else {
             r = l;

@@ -495,17 +495,24 @@ __global__ __launch_bounds__(kThreads) void k_photon_subsurface(SssArgs a)
     const float inv_pi = 1.0f / 3.14159265358979323846f;
     double acc[3] = {0.0, 0.0, 0.0};
     float maxd = 0.0f;
-    uint32_t sl[kStack], sr[kStack];
-    float spd[kStack];
+    // the k-nearest search's walk (near child next, far children stacked in
+    // scratch) with a fixed radius
+    struct Far {
+        uint32_t l, r;
+        float pd;
+    } st[kStack];
     uint32_t sp = 0;
-    sl[0] = 0; sr[0] = a.n; spd[0] = -1.0f; sp = 1;
-    while (sp > 0) {
-        --sp;
-        const uint32_t l = sl[sp], r = sr[sp];
-        if (spd[sp] >= 0.0f && !(spd[sp] < a.radius)) continue;
-        if (r <= l) continue;
+    uint32_t l = 0, r = a.n;
+    for (;;) {
+        if (r <= l) {
+            while (sp > 0 && !(st[sp - 1].pd < a.radius)) --sp;
+            if (sp == 0) break;
+            --sp;
+            l = st[sp].l;
+            r = st[sp].r;
+        }
         const uint32_t med = l + (r - l) / 2;
-        if (want && !(a.sig[med] & want_bit)) continue;  // no photon of the shape below: nothing to sum
+        if (want && !(a.sig[med] & want_bit)) { r = l; continue; }  // no photon of the shape below: nothing to sum
         const rt::PhotonNode nd = a.nodes[med];
         const float d = norm3(p[0] - nd.x, p[1] - nd.y, p[2] - nd.z);
         if (d < a.radius && (want == 0u || (nd.meta >> 2) == want)) {
@@ -525,12 +532,16 @@ __global__ __launch_bounds__(kThreads) void k_photon_subsurface(SssArgs a)
             const float split = ax == 0 ? nd.x : (ax == 1 ? nd.y : nd.z);
             const float diff = p[ax] - split;
             const float pdd = norm3(ax == 0 ? diff : 0.0f, ax == 1 ? diff : 0.0f, ax == 2 ? diff : 0.0f);
-            uint32_t nl, nr, fl, fr;
-            if (p[ax] < split) { nl = l; nr = med; fl = med + 1; fr = r; }
-            else { nl = med + 1; nr = r; fl = l; fr = med; }
-            if (sp + 2 > kStack) break;  // unreachable: the host checks depth + 1 <= kStack
-            sl[sp] = fl; sr[sp] = fr; spd[sp] = pdd; ++sp;
-            sl[sp] = nl; sr[sp] = nr; spd[sp] = -1.0f; ++sp;
+            uint32_t fl, fr;
+            if (p[ax] < split) { fl = med + 1; fr = r; r = med; }
+            else { fl = l; fr = med; l = med + 1; }
+            if (fr > fl) {
+                if (sp >= kStack) break;  // unreachable: the host checks depth + 1 <= kStack
+                st[sp] = Far{fl, fr, pdd};
+                ++sp;
+            }
+        } else {
+            r = l;
         }
     }
     // contribution /= 2 PI maxD^2 (no photon: 0 / 0)

@@ -137,9 +137,11 @@ struct KnnArgs {
     uint32_t m, k;
     float radius;
     uint32_t *count;          // m
-    uint32_t *index;          // m * k
-    float *dist;              // m * k
+    uint32_t *index;          // m * k: entry i of query q at index[q * qstride + i * istride]
+    float *dist;              // m * k, row q at q * k
     const uint32_t *m_dev;    // if set: only the first *m_dev of the m lanes are queries
+    uint32_t qstride = 0, istride = 1;  // qstride 0: k (rows of k entries)
+    float *qmax = nullptr;    // if set: each query's farthest distance here instead of its row of distances
 };
 
 // Strided view of a heap: distance i at d[i * sd], id i at v[i * sv] (a
@@ -253,7 +255,7 @@ __global__ __launch_bounds__(kThreads) void k_photon_knn(KnnArgs a)
     const float p[3] = {a.queries[3 * (size_t)q], a.queries[3 * (size_t)q + 1], a.queries[3 * (size_t)q + 2]};
     const uint32_t want = a.qshape ? (uint32_t)(a.qshape[q] + 1) : 0u;  // 0: any shape
     const uint32_t want_bit = rt::photon_shape_bit(want);
-    uint32_t *out_v = a.index + (size_t)q * a.k;
+    uint32_t *out_v = a.index + (size_t)q * (a.qstride ? a.qstride : a.k);
     float *out_d = a.dist + (size_t)q * a.k;
     // kHeapLdsSlot: out_d[s] holds the tree position of slot s (as float bits) until the end
     using H = HeapRef<typename std::conditional<SLOT, uint8_t, uint32_t>::type>;
@@ -345,9 +347,15 @@ __global__ __launch_bounds__(kThreads) void k_photon_knn(KnnArgs a)
     if constexpr (SLOT) {
         for (uint32_t i = 0; i < cnt; ++i) {
             const uint32_t pos = __float_as_uint(out_d[h.id(i)]);
-            out_v[i] = a.order ? a.order[pos] : pos;
+            out_v[i * a.istride] = a.order ? a.order[pos] : pos;
         }
-        for (uint32_t i = 0; i < cnt; ++i) out_d[i] = h.dist(i);  // over the slots, all read above
+        if (a.qmax) {
+            float mx = 0.0f;
+            for (uint32_t i = 0; i < cnt; ++i) mx = fmaxf(mx, h.dist(i));
+            a.qmax[q] = mx;
+        } else {
+            for (uint32_t i = 0; i < cnt; ++i) out_d[i] = h.dist(i);  // over the slots, all read above
+        }
     } else {
         if (a.order)
             for (uint32_t i = 0; i < cnt; ++i) out_v[i] = a.order[out_v[i]];
@@ -397,10 +405,13 @@ __global__ __launch_bounds__(256) void k_photon_gather_lambert(const double *rgb
 // Trowbridge materials only (plain: lightReflected = 0; the rest are not
 // scattering).  Sum in double in the search's order, times 1 / (PI * maxd^2)
 // in float.
+// index: entry i of query q at index[q * qstride + i * istride]; qmax (if
+// set): each query's farthest distance, read instead of its row of dist.
 __global__ __launch_bounds__(256) void k_photon_gather_mat(const double *rgb, const float *dir, const float *nrm,
                                                            const float *view, const rt_material *mats, uint32_t m,
                                                            uint32_t k, const uint32_t *count, const uint32_t *index,
-                                                           const float *dist, double *out, const uint32_t *m_dev)
+                                                           uint32_t qstride, uint32_t istride, const float *dist,
+                                                           const float *qmax, double *out, const uint32_t *m_dev)
 {
     const uint32_t q = blockIdx.x * 256 + threadIdx.x;
     if (q >= (m_dev ? min(m, *m_dev) : m)) return;
@@ -413,10 +424,11 @@ __global__ __launch_bounds__(256) void k_photon_gather_mat(const double *rgb, co
         rt::onb(n, t, b);
         const rt::V3 vw = rt::mk(view[3 * (size_t)q], view[3 * (size_t)q + 1], view[3 * (size_t)q + 2]);
         const rt::V3 wo = rt::mk(rt::dot(vw, t), rt::dot(vw, b), rt::dot(vw, n));
-        float maxd = 0.0f;
+        float maxd = qmax ? qmax[q] : 0.0f;
+        const uint32_t *ix = index + (size_t)q * qstride;
         for (uint32_t i = 0; i < c; ++i) {
-            const uint32_t e = index[(size_t)q * k + i];
-            maxd = fmaxf(maxd, dist[(size_t)q * k + i]);
+            const uint32_t e = ix[(size_t)i * istride];
+            if (!qmax) maxd = fmaxf(maxd, dist[(size_t)q * k + i]);
             const rt::V3 w = rt::mk(-dir[3 * (size_t)e], -dir[3 * (size_t)e + 1], -dir[3 * (size_t)e + 2]);
             const rt::V3 wi = rt::mk(rt::dot(w, t), rt::dot(w, b), rt::dot(w, n));
             const rt::V3 f = rt::bsdf_f(mt, wo, wi);
@@ -635,7 +647,7 @@ static void render_layout(uint32_t cap, uint32_t k, size_t sort_bytes, size_t &i
     work = 4 * ids + sort_bytes;
     cb = ((size_t)cap * 4 + 255) & ~(size_t)255;
     ib = ((size_t)cap * k * 4 + 255) & ~(size_t)255;
-    gather = cb + 2 * ib;
+    gather = 2 * cb + 2 * ib;  // counts, indices, distances (the search's slot pool), farthest distances
 }
 
 hipError_t photon_render_reserve(rt_photon_map *m, uint32_t cap, uint32_t k, const double *photon_rgb,
@@ -699,14 +711,24 @@ hipError_t photon_gather_render(rt_photon_map *m, const float *points, const flo
         return e;
     uint8_t *gw = (uint8_t *)m->gather;
     uint32_t *cnt = (uint32_t *)gw, *idx = (uint32_t *)(gw + cb);
-    float *dist = (float *)(gw + cb + ib);
-    // results as tree positions, read against the tree-ordered copies
+    float *dist = (float *)(gw + cb + ib), *qmax = (float *)(gw + cb + 2 * ib);
+    // Results as tree positions, read against the tree-ordered copies.  With
+    // the heap in LDS the indices are stored entry-major (entry i of every
+    // query together), so the density estimate's lanes read them coalesced,
+    // and only each query's farthest distance is written.
     KnnArgs a{id1, m->nodes, nullptr, m->sig, m->n, points, qshape, cap, k, radius, cnt, idx, dist, n_dev};
+    const bool lds_heap = k <= kLdsK;
+    if (lds_heap) {
+        a.qstride = 1;
+        a.istride = cap;
+        a.qmax = qmax;
+    }
     const unsigned blocks = (cap + kThreads - 1) / kThreads;
     launch_knn(a, m->depth, blocks, s);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_photon_gather_mat<<<(cap + 255) / 256, 256, 0, s>>>(m->trgb, m->tdir, normals, views, mats, cap, k, cnt, idx,
-                                                          dist, out, n_dev);
+                                                          lds_heap ? 1u : k, lds_heap ? cap : 1u, dist,
+                                                          lds_heap ? qmax : nullptr, out, n_dev);
     return hipGetLastError();
 }
 
@@ -913,7 +935,7 @@ rt_status rt_photon_gather(rt_photon_map *m, const double *photon_rgb, const flo
     rt_status st = gather_knn(m, points, query_shape, nq, k, max_distance, stream, &cnt, &idx, &dist);
     if (st != RT_OK) return st;
     k_photon_gather_mat<<<(nq + 255) / 256, 256, 0, stream>>>(photon_rgb, photon_dir, normals, views, materials, nq,
-                                                              k, cnt, idx, dist, out, nullptr);
+                                                              k, cnt, idx, k, 1u, dist, nullptr, out, nullptr);
     RT_HIP(hipGetLastError());
     return RT_OK;
 }

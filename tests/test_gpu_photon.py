@@ -135,7 +135,7 @@ def test_gather_lambert_matches_oracle(rt, gpu_ctx, k, radius):
     assert np.allclose(got[ok], want[ok], rtol=1e-12, atol=0)
 
 
-@pytest.mark.parametrize("k", [1, 2, 5, 16, 100])
+@pytest.mark.parametrize("k", [1, 2, 5, 16, 100, 129, 1024])
 def test_result_order_equals_the_reference_heap_order(rt, gpu_ctx, k):
     """The search returns its vector as kdtree.h leaves it: insertion order
     until k photons were accepted, then the layout of libstdc++'s
@@ -151,9 +151,10 @@ def test_result_order_equals_the_reference_heap_order(rt, gpu_ctx, k):
         assert np.unique(x[:, a]).size == x.shape[0]
     q = rng.random((4096, 3)).astype(np.float32)
     q[:64] = x[:64]  # queries on photons: a distance-0 candidate
-    got = gpu_knn(rt, gpu_ctx, x, q, k, 0.2)
+    radius = 0.2 if k <= 100 else 0.5  # k > 128: the heap in the output slice
+    got = gpu_knn(rt, gpu_ctx, x, q, k, radius)
     tree = oracle.kd_build(x)[:2]
-    want = oracle.knn(x, q, k, 0.2, tree=tree)
+    want = oracle.knn(x, q, k, radius, tree=tree)
     assert np.array_equal(got[0], want[0])
     for i in range(q.shape[0]):
         c = got[0][i]

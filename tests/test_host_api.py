@@ -26,6 +26,14 @@ def test_cpp_host_check(exe, tmp_path):
     assert "host check ok" in r.stdout
 
 
+def test_cpp_driver_rejects_an_unknown_render_mode(exe, tmp_path):
+    """RenderSettings::mode from the driver's fifth argument: pt, photon or
+    hybrid; anything else is refused before the GPU is touched."""
+    r = subprocess.run([exe, str(tmp_path / "x"), "4", "4", "1", "bogus"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2
+    assert "unknown mode bogus" in r.stderr
+
+
 def test_cpp_film_file_reads_in_python(rt, exe, tmp_path):
     subprocess.run([exe, "--host-check", str(tmp_path)], check=True, capture_output=True, timeout=60)
     sums, n, w, h = rt.read_hdrbmp(str(tmp_path / "film"))

@@ -271,21 +271,28 @@ __global__ __launch_bounds__(kThreads) void k_photon_knn(KnnArgs a)
     // the radius then.  Only far children are stacked (at most depth - 1
     // entries).  The stack lives in registers, indexed through selects, so
     // each entry costs a compare and select per word and per slot.
+    // The top entry is kept in registers and the ones below it in st: a pop
+    // loads the next top while the popped subtree is walked, so the walk does
+    // not wait on scratch memory before its next node.
     struct Far {
         uint32_t l, r;
         float pd;
-    } st[STACK];
-    uint32_t sp = 0;
+    } st[STACK], top{};
+    uint32_t sp = 0;  // entries, the top included
     uint32_t l = 0, r = a.n;
     for (;;) {
         if (r <= l) {
             // this subtree is done: the last stacked far child whose plane
             // is still within the radius
-            while (sp > 0 && !(st[sp - 1].pd < maxd)) --sp;
+            while (sp > 0 && !(top.pd < maxd)) {
+                --sp;
+                if (sp > 0) top = st[sp - 1];
+            }
             if (sp == 0) break;
+            l = top.l;
+            r = top.r;
             --sp;
-            l = st[sp].l;
-            r = st[sp].r;
+            if (sp > 0) top = st[sp - 1];
         }
         const uint32_t med = l + (r - l) / 2;
         // a subtree with no photon of the wanted shape takes none, so the
@@ -305,7 +312,8 @@ __global__ __launch_bounds__(kThreads) void k_photon_knn(KnnArgs a)
             else { fl = l; fr = med; l = med + 1; }
             if (fr > fl) {
                 if (sp >= STACK) break;  // unreachable: the host checks depth - 1 <= STACK
-                st[sp] = Far{fl, fr, pd};
+                if (sp > 0) st[sp - 1] = top;
+                top = Far{fl, fr, pd};
                 ++sp;
             }
         } else {
@@ -512,16 +520,20 @@ __global__ __launch_bounds__(kThreads) void k_photon_subsurface(SssArgs a)
     struct Far {
         uint32_t l, r;
         float pd;
-    } st[kStack];
-    uint32_t sp = 0;
+    } st[kStack], top{};
+    uint32_t sp = 0;  // entries, the top (in registers) included
     uint32_t l = 0, r = a.n;
     for (;;) {
         if (r <= l) {
-            while (sp > 0 && !(st[sp - 1].pd < a.radius)) --sp;
+            while (sp > 0 && !(top.pd < a.radius)) {
+                --sp;
+                if (sp > 0) top = st[sp - 1];
+            }
             if (sp == 0) break;
+            l = top.l;
+            r = top.r;
             --sp;
-            l = st[sp].l;
-            r = st[sp].r;
+            if (sp > 0) top = st[sp - 1];
         }
         const uint32_t med = l + (r - l) / 2;
         if (want && !(a.sig[med] & want_bit)) { r = l; continue; }  // no photon of the shape below: nothing to sum
@@ -549,7 +561,8 @@ __global__ __launch_bounds__(kThreads) void k_photon_subsurface(SssArgs a)
             else { fl = l; fr = med; l = med + 1; }
             if (fr > fl) {
                 if (sp >= kStack) break;  // unreachable: the host checks depth + 1 <= kStack
-                st[sp] = Far{fl, fr, pdd};
+                if (sp > 0) st[sp - 1] = top;
+                top = Far{fl, fr, pdd};
                 ++sp;
             }
         } else {

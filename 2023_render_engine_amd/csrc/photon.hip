@@ -269,11 +269,13 @@ __global__ __launch_bounds__(kThreads) void k_photon_knn(KnnArgs a)
     // far child is stacked with the distance to its splitting plane and
     // searched once the near side is done, if that distance is still below
     // the radius then.  Only far children are stacked (at most depth - 1
-    // entries).  The stack lives in registers, indexed through selects, so
-    // each entry costs a compare and select per word and per slot.
-    // The top entry is kept in registers and the ones below it in st: a pop
-    // loads the next top while the popped subtree is walked, so the walk does
-    // not wait on scratch memory before its next node.
+    // entries).  The stack lives in scratch (an array of structs, which the
+    // compiler does not turn into registers indexed through selects), with
+    // its top entry in registers: a pop loads the next top while the popped
+    // subtree is walked, so the walk does not wait on scratch memory before
+    // its next node.  (k_photon_subsurface walks the same way; the two loops
+    // are kept apart because a shared walker taking lambdas compiled to
+    // slower code.)
     struct Far {
         uint32_t l, r;
         float pd;

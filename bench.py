@@ -1075,10 +1075,10 @@ def main():
     ap.add_argument("--c4-ref-spp", type=int, default=2048)
     ap.add_argument("--c4-target", type=float, default=1e-2)
     ap.add_argument("--c4-budget", type=float, default=20.0, help="c4: timed seconds per method at most")
-    ap.add_argument("--c4-warm", type=int, default=4, help="c4: plain frames whose paths all train the cache")
-    ap.add_argument("--c4-warm-steps", type=int, default=8, help="c4: Adam steps after each warm-up frame")
+    ap.add_argument("--c4-warm", type=int, default=2, help="c4: plain frames whose paths all train the cache")
+    ap.add_argument("--c4-warm-steps", type=int, default=12, help="c4: Adam steps after each warm-up frame")
     ap.add_argument("--c4-steps", type=int, default=2, help="c4: Adam steps after each cache frame")
-    ap.add_argument("--c4-harvest-shift", type=int, default=4,
+    ap.add_argument("--c4-harvest-shift", type=int, default=7,
                     help="c4: one path in 2^shift of a cache frame goes full length and trains the cache")
     ap.add_argument("--c4-cache-depth", type=int, default=2,
                     help="c4: bounces before a path ends in the cache (the workload's definition is 2)")

@@ -42,10 +42,12 @@ def gpu_photon_render(rt, ctx, desc, photons, k, radius, spp, hybrid=False):
 
 @pytest.mark.parametrize("name,k,radius,hybrid", [("cornell", 50, 0.3, False), ("cornell", 100, 100.0, False),
                                                  ("procedural", 40, 0.5, False), ("cornell", 50, 0.3, True),
-                                                 ("procedural", 40, 0.5, True)])
+                                                 ("procedural", 40, 0.5, True), ("cornell", 150, 0.3, False)])
 def test_photon_render_matches_oracle(rt, orc, gpu_ctx, name, k, radius, hybrid):
     """hybrid: RENDER_MODE HYBRID, every scattering hit takes half the photon
-    estimate and half its next-event terms and the path goes on."""
+    estimate and half its next-event terms and the path goes on.  k = 150:
+    beyond the LDS heap (k <= 128), the search keeps its heap in the result
+    rows."""
     print(f"[photon render] {name} k={k} r={radius} hybrid={hybrid}", flush=True)
     if name == "cornell":
         desc = scenes.cornell(width=32, height=32, spp=4)

@@ -491,11 +491,13 @@ rt_status rt_batch_autotune(rt_batch *b, const rt_batch_opts *opts, rt_tune_entr
         off += ((size_t)b->npix[i] * 4 + 255) & ~(size_t)255;
     }
     rt_batch_opts o = *opts;
-    o.spp = 1;
+    // trial frames of opts->spp samples per pixel (1..4): with more than one,
+    // waves larger than one sample's paths are among the candidates
+    o.spp = std::max(1u, std::min(opts->spp, 4u));
     o.spp_offset = 0x40000000u;  // far from rendering samples
-    const uint64_t total = b->paths_per_sample;
+    const uint64_t total = b->paths_per_sample * o.spp;
     std::vector<uint32_t> caps;
-    for (uint32_t P = 1u << 19; P <= (1u << 23); P <<= 1) {
+    for (uint32_t P = 1u << 19; P <= (1u << 24); P <<= 1) {
         const uint32_t c = rt::stripe_capacity((uint32_t)std::min<uint64_t>(P, total));
         if (caps.empty() || caps.back() != c) caps.push_back(c);
     }
@@ -506,7 +508,7 @@ rt_status rt_batch_autotune(rt_batch *b, const rt_batch_opts *opts, rt_tune_entr
     rt_status result = RT_OK;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) result = RT_ERR_HIP;
     if (result == RT_OK) result = upload_table(b, nullptr, s);
-    if (result == RT_OK) result = upload_bases(b, 1, s);
+    if (result == RT_OK) result = upload_bases(b, o.spp, s);
     for (uint32_t P : caps) {
         if (result != RT_OK) break;
         result = rt::ensure_wave_buffers(b->wb, P, b->max_lights, "rt_batch_autotune");
@@ -537,7 +539,7 @@ rt_status rt_batch_autotune(rt_batch *b, const rt_batch_opts *opts, rt_tune_entr
     if (e1) (void)hipEventDestroy(e1);
     (void)hipStreamSynchronize(s);
     (void)hipFree(scratch);
-    b->bases_spp = 0;  // bases were set up for spp = 1
+    b->bases_spp = 0;  // bases were set up for the trial frames
     if (result != RT_OK) return result;
     b->tuned_capacity = top.path_capacity;
     b->tuned_grid = top.grid;

@@ -559,7 +559,8 @@ def run_c3(args):
     log(f"32 scenes committed in {commit_s:.1f} s ({ntris} triangles, {npix} pixels)")
     batch = rt.Batch(ctx, scs)
     seeds = [1000 + i for i in range(len(descs))]
-    best, entries = batch.autotune(spp=1, max_depth=12)
+    # 2 spp per trial, so that waves up to 2^24 paths (above the 11.9M pixels) are among the candidates
+    best, entries = batch.autotune(spp=2, max_depth=12)
     log("autotune: " + ", ".join(f"P={e.path_capacity}/grid={e.grid}: {e.ms:.1f} ms" for e in entries))
     films = [s.new_film() for s in scs]
     for w in range(args.warmup):

@@ -593,14 +593,14 @@ rt_status rt_batch_destroy(rt_batch *batch);
 rt_status rt_batch_render(rt_batch *batch, const rt_batch_opts *opts, const uint32_t *seeds, double *const *accum,
                           uint32_t *const *nvalid, uint32_t flags, void *stream);
 
-/* Autotuner: renders one sample per pixel of every scene (into scratch
- * films) for each candidate wave size {2^19 .. 2^23, capped at the batch's
- * paths per sample} x grid {1024, 2048, 4096, 8192}, twice each, and keeps
- * the fastest pair as the batch's default.  entries (may be NULL) receives up to
+/* Autotuner: renders opts->spp samples per pixel (clamped to 1..4) of every
+ * scene (into scratch films) for each candidate wave size {2^19 .. 2^24,
+ * capped at the trial frame's paths} x grid {1024, 2048, 4096, 8192}, twice
+ * each, and keeps the fastest pair as the batch's default.  entries (may be NULL) receives up to
  * max_entries (path_capacity, grid, ms) triples in the order tried. */
 typedef struct {
     uint32_t path_capacity, grid;
-    float ms;  /* best of the two trials of one 1-spp frame */
+    float ms;  /* best of the two runs of one trial frame */
 } rt_tune_entry;
 rt_status rt_batch_autotune(rt_batch *batch, const rt_batch_opts *opts, rt_tune_entry *best, rt_tune_entry *entries,
                             uint32_t max_entries, uint32_t *n_entries);

@@ -316,7 +316,7 @@ rt_status check_opts(const char *fn, const rt_batch *b, const rt_batch_opts *o)
 
 uint32_t wave_size(const rt_batch *b, const rt_batch_opts &o, uint64_t total)
 {
-    uint32_t P = o.path_capacity ? o.path_capacity : (b->tuned_capacity ? b->tuned_capacity : (1u << 21));
+    uint32_t P = o.path_capacity ? o.path_capacity : (b->tuned_capacity ? b->tuned_capacity : (1u << 24));
     if ((uint64_t)P > total) P = (uint32_t)std::max<uint64_t>(1, total);
     return rt::stripe_capacity(P);
 }

@@ -210,18 +210,25 @@ def run_c1(args):
 # ray clears its light bit (a 4 B read-modify-write).  A batch path also
 # reads its scene's root (4 B).  Every node visit reads 64 B and every
 # triangle test 48 B, counted by the kernels' counting variants.
-def measured_traffic(profile, kernel, alg_bytes, field="hbm_bytes_per_frame"):
+# rt_render's default wave size (paths in flight), capped at a render's paths
+DEFAULT_WAVE = 1 << 24
+
+
+def measured_traffic(profile, kernel, alg_bytes, field="hbm_bytes_per_frame", wave=None):
     """HBM bytes per frame of `kernel` (or another field of its entry, such
     as its shader counters) from a committed FETCH_SIZE / WRITE_SIZE profile
     (copy-scaled), or None when there is none or it was taken on another
     workload (its algorithmic bytes per frame differ from this run's by more
-    than 1 %)."""
+    than 1 %, or its wave size, when given, is not this run's)."""
     if not profile:
         return None
     path = os.path.join(ROOT, "profiles", profile)
     try:
-        k = json.load(open(path))["kernels"][kernel]
+        d = json.load(open(path))
+        k = d["kernels"][kernel]
     except (OSError, ValueError, KeyError):
+        return None
+    if wave is not None and d.get("path_capacity") != wave:
         return None
     ref = k.get("algorithmic_bytes_per_frame") or 0
     if ref <= 0 or abs(alg_bytes - ref) > 0.01 * ref or field not in k:
@@ -230,7 +237,7 @@ def measured_traffic(profile, kernel, alg_bytes, field="hbm_bytes_per_frame"):
     return int(v) if field == "hbm_bytes_per_frame" else v
 
 
-def traversal_roofline(count_stats, frames, ext_ms, con_ms, batch=False, note="", traffic_profile=None):
+def traversal_roofline(count_stats, frames, ext_ms, con_ms, batch=False, note="", traffic_profile=None, wave=None):
     """The roofline object of a path-tracing line: the dominant traversal
     kernel's algorithmic bytes per frame (from a counting frame over the same
     samples) over its HIP-event time per frame (timed frames).  traffic: that
@@ -249,8 +256,8 @@ def traversal_roofline(count_stats, frames, ext_ms, con_ms, batch=False, note=""
     achieved = b / t / 1e9
     return {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(traffic_profile, name, b),
-        "valu": measured_traffic(traffic_profile, name, b, field="valu"),
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(traffic_profile, name, b, wave=wave),
+        "valu": measured_traffic(traffic_profile, name, b, field="valu", wave=wave),
         "kernel": name,
         "bytes_per_frame": int(b), "kernel_ms_per_frame": round(t * 1e3, 3),
         "per_ray": {
@@ -343,7 +350,8 @@ def run_render(args, which):
         "rays_per_sample": round((ext + shadow) / samples, 2),
         "rmse_vs_ref": rmse,
         "roofline": traversal_roofline(cstats, args.steps, ext_ms, con_ms,
-                                       traffic_profile="c2_path_trace_procedural_hbm_traffic.json"
+                                       traffic_profile="c2_path_trace_procedural_hbm_traffic.json",
+                                       wave=min(DEFAULT_WAVE, desc.width * desc.height * desc.spp)
                                        if which == "c2" else None),
     }
     if not args.no_cpu_baseline:
@@ -1076,8 +1084,8 @@ def main():
     ap.add_argument("--c4-ref-spp", type=int, default=2048)
     ap.add_argument("--c4-target", type=float, default=1e-2)
     ap.add_argument("--c4-budget", type=float, default=20.0, help="c4: timed seconds per method at most")
-    ap.add_argument("--c4-warm", type=int, default=2, help="c4: plain frames whose paths all train the cache")
-    ap.add_argument("--c4-warm-steps", type=int, default=12, help="c4: Adam steps after each warm-up frame")
+    ap.add_argument("--c4-warm", type=int, default=1, help="c4: plain frames whose paths all train the cache")
+    ap.add_argument("--c4-warm-steps", type=int, default=32, help="c4: Adam steps after each warm-up frame")
     ap.add_argument("--c4-steps", type=int, default=2, help="c4: Adam steps after each cache frame")
     ap.add_argument("--c4-harvest-shift", type=int, default=7,
                     help="c4: one path in 2^shift of a cache frame goes full length and trains the cache")

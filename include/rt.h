@@ -238,7 +238,8 @@ typedef struct {
     uint32_t max_depth;   /* MAX_RAY_RECURSION_DEPTH (types.h:41) */
     uint32_t rr_depth;    /* Russian roulette when depth > rr_depth (pathTracing.cu:91), 3 */
     uint32_t seed;        /* PCG32 seed */
-    uint32_t path_capacity; /* paths in flight per wave, 0 = default (2^21) */
+    uint32_t path_capacity; /* paths in flight per wave, 0 = default (2^24, capped at the render's paths;
+                               ~600 B of device memory per path) */
     uint32_t grid;          /* workgroups of the queue kernels, 0 = default (2048) */
     uint32_t chunk_paths;   /* paths per streamed chunk (12 B of result buffer each), 0 = default (2^28) */
 } rt_render_opts;
